@@ -1,0 +1,780 @@
+// The EMC C-ABI (emc.h, emc_hip.h) on top of the HIP kernels.
+//
+// Each entry point keeps the reference's signature and observable behaviour
+// (emc_cuda.cu:154-505): the same device buffers are read and written, the
+// same host arrays are filled, and every call returns after its work is
+// complete.  Inside, the engine keeps derived operands of caller buffers
+// (masked-pixel packing of the patterns, packed slices, the per-pattern sums
+// that let the scaling update reuse the likelihood's contraction) and
+// validates them against the buffers they came from.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace emc {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static char g_err[1024];
+static int g_err_code = 0;
+static bool g_err_printed = false;
+
+void set_error(const char* fmt, ...) {
+  if (g_err_code) return;  // keep the first error
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  g_err_code = 1;
+  fprintf(stderr, "emc_hip error: %s\n", g_err);
+  g_err_printed = true;
+  const char* ab = getenv("EMC_HIP_ABORT_ON_ERROR");
+  if (ab && ab[0] == '1') abort();
+}
+
+bool failed() { return g_err_code != 0; }
+
+bool DBuf::ensure(size_t b) {
+  if (b == 0) b = 1;
+  if (p && bytes >= b) return true;
+  release();
+  if (hipMalloc(&p, b) != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    (void)hipGetLastError();
+    set_error("out of device memory allocating %zu bytes", b);
+    return false;
+  }
+  bytes = b;
+  return true;
+}
+
+void DBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+static Engine* g_engine = nullptr;
+
+bool Engine::init() {
+  if (hipGetDevice(&device) != hipSuccess) {
+    set_error("no HIP device available");
+    return false;
+  }
+  if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return false;
+  }
+  return true;
+}
+
+void Engine::release_all() {
+  DBuf* bufs[] = {&d_pix, &d_B, &d_K32, &d_Kt32, &d_K16, &d_K16lo, &d_Kt16, &d_Kt16lo, &d_flags,
+                  &d_S32, &d_S16, &d_S16lo, &d_Sscale, &d_A, &d_phi_used, &d_smax, &d_ssum, &d_sacc,
+                  &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Pscale, &d_Sc, &d_wts, &d_wT,
+                  &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
+  for (DBuf* b : bufs) b->release();
+  pack_key = {};
+  map_key = {};
+  like = {};
+}
+
+Engine& engine() {
+  if (!g_engine) {
+    g_engine = new Engine();
+    if (!g_engine->init()) g_engine->stream = nullptr;
+    const char* v = getenv("EMC_HIP_VERBOSE");
+    if (v && v[0] == '1') g_engine->verbose = true;
+  }
+  return *g_engine;
+}
+
+bool ready() {
+  Engine& e = engine();
+  if (failed()) return false;
+  return e.stream != nullptr;
+}
+
+PhaseTimer::PhaseTimer(int ph) : phase(ph), on(false) {
+  Engine& e = engine();
+  if (!e.timing || failed()) return;
+  on = true;
+  if (!e.ev0) {
+    EMC_CHECK(hipEventCreate(&e.ev0));
+    EMC_CHECK(hipEventCreate(&e.ev1));
+  }
+  EMC_CHECK(hipEventRecord(e.ev0, e.stream));
+}
+
+PhaseTimer::~PhaseTimer() {
+  if (!on) return;
+  Engine& e = engine();
+  EMC_CHECK(hipEventRecord(e.ev1, e.stream));
+  EMC_CHECK(hipEventSynchronize(e.ev1));
+  float ms = 0.0f;
+  EMC_CHECK(hipEventElapsedTime(&ms, e.ev0, e.ev1));
+  e.phase_ms[phase] += ms;
+  e.phase_launches[phase] += 1;
+}
+
+static void sync() { EMC_CHECK(hipStreamSynchronize(engine().stream)); }
+
+// ---------------------------------------------------------------------------
+// packing of the patterns and the mask
+// ---------------------------------------------------------------------------
+static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
+  Engine& e = engine();
+  auto& k = e.pack_key;
+  if (k.images == images && k.mask == mask && k.N == N && k.N2d == N2d && k.precision == e.precision &&
+      k.stamp == e.invalidate_stamp)
+    return true;
+  k = {};
+  if (N <= 0 || N2d <= 0) {
+    set_error("invalid sizes N_images=%d N_2d=%d", N, N2d);
+    return false;
+  }
+  const int S = (int)std::lround(std::sqrt((double)N2d));
+  if (S * S != N2d) {
+    set_error("N_2d=%d is not a square detector", N2d);
+    return false;
+  }
+  std::vector<int> hmask(N2d);
+  EMC_CHECK(hipMemcpyAsync(hmask.data(), mask, sizeof(int) * N2d, hipMemcpyDeviceToHost, e.stream));
+  sync();
+  if (failed()) return false;
+  std::vector<int> pix, col_of(N2d, -1), ins;
+  for (int p = 0; p < N2d; p++)
+    if (hmask[p] != 0) {
+      col_of[p] = (int)pix.size();
+      pix.push_back(p);
+    }
+  // inserted pixels (mask == 1) in the reference's x-outer, y-inner order
+  for (int x = 0; x < S; x++)
+    for (int y = 0; y < S; y++)
+      if (hmask[y * S + x] == 1) ins.push_back(y * S + x);
+  e.n_m = (int)pix.size();
+  e.n_mp = (int)round_up(std::max(e.n_m, 1), kPadK);
+  e.n_ins = (int)ins.size();
+  e.N_pad = (int)round_up(N, kTileM);
+  std::vector<int> pixp(e.n_mp, -1), inscol(std::max(e.n_ins, 1));
+  std::copy(pix.begin(), pix.end(), pixp.begin());
+  for (int t = 0; t < e.n_ins; t++) inscol[t] = col_of[ins[t]];
+  if (!e.d_pix.ensure(sizeof(int) * e.n_mp) || !e.d_ins_pix.ensure(sizeof(int) * std::max(e.n_ins, 1)) ||
+      !e.d_ins_col.ensure(sizeof(int) * std::max(e.n_ins, 1)) || !e.d_B.ensure(sizeof(double) * N))
+    return false;
+  EMC_CHECK(hipMemcpyAsync(e.d_pix.p, pixp.data(), sizeof(int) * e.n_mp, hipMemcpyHostToDevice, e.stream));
+  if (e.n_ins) {
+    EMC_CHECK(hipMemcpyAsync(e.d_ins_pix.p, ins.data(), sizeof(int) * e.n_ins, hipMemcpyHostToDevice, e.stream));
+    EMC_CHECK(hipMemcpyAsync(e.d_ins_col.p, inscol.data(), sizeof(int) * e.n_ins, hipMemcpyHostToDevice, e.stream));
+  }
+  const size_t nk = (size_t)e.N_pad * e.n_mp;
+  PhaseTimer pt(PH_PACK);
+  if (e.precision == PREC_FP32) {
+    e.d_K16.release(); e.d_K16lo.release(); e.d_Kt16.release(); e.d_Kt16lo.release();
+    if (!e.d_K32.ensure(nk * 4) || !e.d_Kt32.ensure(nk * 4)) return false;
+    launch_pack_images_f32(images, e.d_pix.as<int>(), N, N2d, e.n_mp, e.N_pad, e.d_K32.as<float>(),
+                           e.d_Kt32.as<float>(), e.d_B.as<double>(), e.stream);
+    e.k_exact = true;
+    e.k_scale = 1.0f;
+  } else {
+    e.d_K32.release(); e.d_Kt32.release();
+    if (!e.d_flags.ensure(16)) return false;
+    unsigned int zero[4] = {0, 0, 0, 0}, flags[4];
+    EMC_CHECK(hipMemcpyAsync(e.d_flags.p, zero, 16, hipMemcpyHostToDevice, e.stream));
+    launch_images_stats(images, e.d_pix.as<int>(), N, N2d, e.n_mp, e.d_flags.as<unsigned int>(), e.stream);
+    EMC_CHECK(hipMemcpyAsync(flags, e.d_flags.p, 16, hipMemcpyDeviceToHost, e.stream));
+    sync();
+    if (failed()) return false;
+    float kmax;
+    memcpy(&kmax, &flags[1], 4);
+    e.k_exact = flags[0] == 0;
+    e.k_scale = 1.0f;
+    if (!e.k_exact && kmax > 0.0f && std::isfinite(kmax)) {
+      int ex;
+      std::frexp(kmax, &ex);
+      e.k_scale = std::ldexp(1.0f, std::max(-120, std::min(120, 14 - ex)));
+    }
+    if (!e.d_K16.ensure(nk * 2) || !e.d_Kt16.ensure(nk * 2)) return false;
+    if (!e.k_exact) {
+      if (!e.d_K16lo.ensure(nk * 2) || !e.d_Kt16lo.ensure(nk * 2)) return false;
+    } else {
+      e.d_K16lo.release();
+      e.d_Kt16lo.release();
+    }
+    launch_pack_images_f16(images, e.d_pix.as<int>(), N, N2d, e.n_mp, e.N_pad, e.k_scale, !e.k_exact,
+                           e.d_K16.as<_Float16>(), e.d_K16lo.as<_Float16>(), e.d_Kt16.as<_Float16>(),
+                           e.d_Kt16lo.as<_Float16>(), e.d_B.as<double>(), e.stream);
+  }
+  if (failed()) return false;
+  k.images = images;
+  k.mask = mask;
+  k.N = N;
+  k.N2d = N2d;
+  k.precision = e.precision;
+  k.stamp = e.invalidate_stamp;
+  e.like.valid = false;
+  return true;
+}
+
+// Packed slice operand for the current precision from an [R][N2d] buffer.
+static bool pack_slices(const float* slices, int R, int N2d) {
+  Engine& e = engine();
+  e.R_pad = (int)round_up(R, kTileN);
+  const size_t ns = (size_t)e.R_pad * e.n_mp;
+  if (!e.d_A.ensure(sizeof(double) * R)) return false;
+  PhaseTimer pt(PH_PACK);
+  if (e.precision == PREC_FP32) {
+    if (!e.d_S32.ensure(ns * 4)) return false;
+    launch_pack_slices_f32(slices, e.d_pix.as<int>(), R, N2d, e.n_mp, e.R_pad, e.d_S32.as<float>(),
+                           e.d_A.as<double>(), e.stream);
+  } else {
+    if (!e.d_S16.ensure(ns * 2) || !e.d_S16lo.ensure(ns * 2) || !e.d_Sscale.ensure(sizeof(float) * R)) return false;
+    launch_pack_slices_f16(slices, e.d_pix.as<int>(), R, N2d, e.n_mp, e.R_pad, e.d_S16.as<_Float16>(),
+                           e.d_S16lo.as<_Float16>(), e.d_Sscale.as<float>(), e.d_A.as<double>(), e.stream);
+  }
+  return !failed();
+}
+
+static uint64_t g_slices_epoch = 1;   // bumped on every library write to a slices buffer
+static uint64_t g_respons_epoch = 1;  // bumped on every library write of L into a respons buffer
+
+// The E-step contraction with either the likelihood epilogue (P == nullptr)
+// or the sum_r P_ri C_ir epilogue into D.
+static void run_estep(int N, int R, float sigma, const float* phi, float* L, const float* P, double* D) {
+  Engine& e = engine();
+  LikeEpilogue ep{};
+  ep.L = L;
+  ep.P = P;
+  ep.D = D;
+  ep.A = e.d_A.as<double>();
+  ep.B = e.d_B.as<double>();
+  ep.phi = phi;
+  ep.cscale = e.precision == PREC_FP16 ? e.d_Sscale.as<float>() : nullptr;
+  ep.kscale_inv = 1.0f / e.k_scale;
+  ep.inv_2nms2 = 1.0 / (2.0 * (double)(float)e.n_m * ((double)sigma * (double)sigma));
+  ep.N = N;
+  ep.R = R;
+  ep.vecL = (N % 4 == 0) && L && ((uintptr_t)L % 16 == 0);
+  PhaseTimer pt(PH_LIKE);
+  if (e.precision == PREC_FP32)
+    launch_estep(PREC_FP32, false, e.d_K32.p, nullptr, e.d_S32.p, nullptr, e.N_pad, e.R_pad, e.n_mp, ep, e.stream);
+  else
+    launch_estep(PREC_FP16, !e.k_exact, e.d_K16.p, e.d_K16lo.p, e.d_S16.p, e.d_S16lo.p, e.N_pad, e.R_pad, e.n_mp, ep,
+                 e.stream);
+}
+
+}  // namespace emc
+
+using namespace emc;
+
+// ---------------------------------------------------------------------------
+// emc.h
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void cuda_get_slices(sp_3matrix* model, real* d_model, real* d_slices, real* d_rot, real* d_x_coordinates,
+                     real* d_y_coordinates, real* d_z_coordinates, int N_slices) {
+  if (!ready()) return;
+  Engine& e = engine();
+  if (!model || !d_model || !d_slices || !d_rot || N_slices < 0) {
+    set_error("cuda_get_slices: invalid argument");
+    return;
+  }
+  if (model->x != model->y || model->x != model->z) {
+    set_error("cuda_get_slices: model must be a cube (got %u x %u x %u)", model->x, model->y, model->z);
+    return;
+  }
+  {
+    PhaseTimer pt(PH_EXPAND);
+    launch_expand(d_model, (int)model->x, d_rot, N_slices, d_x_coordinates, d_y_coordinates, d_z_coordinates,
+                  d_slices, e.interp, e.stream);
+  }
+  g_slices_epoch++;
+  sync();
+}
+
+void cuda_calculate_responsabilities(real* slices, real* images, int* d_mask, real sigma, real* d_scaling,
+                                     real* d_respons, int N_2d, int N_images, int N_slices, real* respons) {
+  if (!ready()) return;
+  Engine& e = engine();
+  if (!slices || !images || !d_mask || !d_scaling || !d_respons || N_slices <= 0) {
+    set_error("cuda_calculate_responsabilities: invalid argument");
+    return;
+  }
+  if (!ensure_pack(images, d_mask, N_images, N_2d)) return;
+  if (!pack_slices(slices, N_slices, N_2d)) return;
+  if (!e.d_phi_used.ensure(sizeof(float) * N_images)) return;
+  EMC_CHECK(hipMemcpyAsync(e.d_phi_used.p, d_scaling, sizeof(float) * N_images, hipMemcpyDeviceToDevice, e.stream));
+  run_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), d_respons, nullptr, nullptr);
+  g_respons_epoch++;
+  auto& lk = e.like;
+  lk.valid = !failed();
+  lk.respons = d_respons;
+  lk.slices = slices;
+  lk.images = images;
+  lk.slices_gen = g_slices_epoch;
+  lk.N = N_images;
+  lk.R = N_slices;
+  lk.N2d = N_2d;
+  lk.sigma2x = 2.0 * (double)(float)e.n_m * ((double)sigma * (double)sigma);
+  lk.normalized = false;
+  lk.have_total = false;
+  if (respons) {
+    EMC_CHECK(hipMemcpyAsync(respons, d_respons, sizeof(float) * (size_t)N_images * N_slices,
+                             hipMemcpyDeviceToHost, e.stream));
+    sync();
+    if (e.verbose) {
+      double s = 0;
+      for (size_t k = 0; k < (size_t)N_images * N_slices; k++) s += respons[k];
+      printf("respons_sum = %f\n", s);
+    }
+  }
+  sync();
+}
+
+void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images) {
+  if (!ready()) return;
+  Engine& e = engine();
+  if (!d_respons || N_slices <= 0 || N_images <= 0) {
+    set_error("cuda_normalize_responsabilities: invalid argument");
+    return;
+  }
+  auto& lk = e.like;
+  const bool fuse = lk.valid && !lk.normalized && lk.respons == d_respons && lk.N == N_images && lk.R == N_slices &&
+                    e.d_A.p && e.d_B.p;
+  const int chunks = softmax_chunks(N_images, N_slices);
+  const size_t cn = (size_t)chunks * N_images;
+  const int nb = (N_images + 255) / 256;
+  if (!e.d_smax.ensure(cn * 4) || !e.d_ssum.ensure(cn * 8) || !e.d_sacc.ensure(4 * cn * 8) ||
+      !e.d_phiden.ensure(sizeof(double) * N_images) || !e.d_red.ensure(sizeof(double) * (nb + 1 + 1200)))
+    return;
+  {
+    PhaseTimer pt(PH_SOFTMAX);
+    launch_softmax(d_respons, N_slices, N_images, chunks, e.d_smax.as<float>(), e.d_ssum.as<double>(),
+                   e.d_sacc.as<double>(), fuse ? e.d_A.as<double>() : nullptr, fuse, e.stream);
+    launch_softmax_reduce(N_images, chunks, e.d_sacc.as<double>(), fuse, fuse ? e.d_B.as<double>() : nullptr,
+                          fuse ? e.d_phi_used.as<float>() : nullptr, lk.sigma2x, e.d_phiden.as<double>(),
+                          e.d_red.as<double>(), e.stream);
+  }
+  double total = 0.0;
+  EMC_CHECK(hipMemcpyAsync(&total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  sync();
+  if (failed()) return;
+  if (lk.respons == d_respons && lk.N == N_images && lk.R == N_slices) {
+    lk.normalized = fuse;
+    lk.have_total = true;
+    lk.total = total;
+  } else {
+    lk.valid = false;
+  }
+}
+
+real cuda_total_respons(real* d_respons, real* respons, int n) {
+  (void)respons;
+  if (!ready()) return 0.0f;
+  Engine& e = engine();
+  auto& lk = e.like;
+  if (lk.have_total && lk.respons == d_respons && (size_t)n == (size_t)lk.N * lk.R) return (real)lk.total;
+  if (!d_respons || n < 0) {
+    set_error("cuda_total_respons: invalid argument");
+    return 0.0f;
+  }
+  if (!e.d_red.ensure(sizeof(double) * 1100)) return 0.0f;
+  double total = 0.0;
+  {
+    PhaseTimer pt(PH_OTHER);
+    launch_total_respons(d_respons, (size_t)n, e.d_red.as<double>(), e.stream);
+  }
+  EMC_CHECK(hipMemcpyAsync(&total, e.d_red.as<double>() + 1024, sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  sync();
+  return (real)total;
+}
+
+void cuda_update_scaling(real* d_images, real* slices, int* d_mask, real* d_respons, real* d_scaling, int N_images,
+                         int N_slices, int N_2d, real* scaling) {
+  if (!ready()) return;
+  Engine& e = engine();
+  if (!d_images || !slices || !d_mask || !d_respons || !d_scaling || N_images <= 0 || N_slices <= 0) {
+    set_error("cuda_update_scaling: invalid argument");
+    return;
+  }
+  if (!ensure_pack(d_images, d_mask, N_images, N_2d)) return;
+  auto& lk = e.like;  // ensure_pack clears lk.valid if it had to repack
+  const bool fused = lk.valid && lk.normalized && lk.respons == d_respons && lk.slices == slices &&
+                     lk.images == d_images && lk.slices_gen == g_slices_epoch && lk.N == N_images &&
+                     lk.R == N_slices && lk.N2d == N_2d;
+  if (!e.d_phiden.ensure(sizeof(double) * N_images) || !e.d_invphi.ensure(sizeof(float) * N_images)) return;
+  if (!fused) {
+    // the reference's method: recompute sum_p m K S for every (pattern, rotation)
+    if (!pack_slices(slices, N_slices, N_2d)) return;
+    launch_fill(reinterpret_cast<float*>(e.d_phiden.p), (size_t)N_images * 2, 0.0f, e.stream);
+    run_estep(N_images, N_slices, 1.0f, d_scaling, nullptr, d_respons, e.d_phiden.as<double>());
+  }
+  {
+    PhaseTimer pt(PH_SCALING);
+    launch_scaling_finish(N_images, e.d_B.as<double>(), e.d_phiden.as<double>(), d_scaling, e.d_invphi.as<float>(),
+                          e.stream);
+  }
+  if (scaling)
+    EMC_CHECK(hipMemcpyAsync(scaling, d_scaling, sizeof(float) * N_images, hipMemcpyDeviceToHost, e.stream));
+  sync();
+}
+
+real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respons, real* d_scaling, int N_images,
+                        int N_slices, int N_2d, sp_3matrix* model, real* d_model, real* x_coordinates,
+                        real* y_coordinates, real* z_coordinates, real* d_rotations, real* weights, real* d_weight,
+                        Setup setup, sp_matrix** images) {
+  (void)setup;
+  (void)images;
+  if (!ready()) return 0.0f;
+  Engine& e = engine();
+  if (!d_images || !slices || !d_mask || !d_respons || !d_scaling || !model || !d_model || !d_rotations ||
+      !weights || !d_weight || N_images <= 0 || N_slices <= 0) {
+    set_error("cuda_update_slices: invalid argument");
+    return 0.0f;
+  }
+  const int S = (int)model->x;
+  if (model->y != model->x || model->z != model->x || (size_t)S * S != (size_t)N_2d) {
+    set_error("cuda_update_slices: model %u^3 does not match N_2d=%d", model->x, N_2d);
+    return 0.0f;
+  }
+  if (!ensure_pack(d_images, d_mask, N_images, N_2d)) return 0.0f;
+  const int R = N_slices, N = N_images;
+  e.R_pad = (int)round_up(R, kTileN);
+  if (!e.d_invphi.ensure(sizeof(float) * N) || !e.d_T.ensure(sizeof(double) * R) || !e.d_Tf.ensure(sizeof(float) * R) ||
+      !e.d_Pscale.ensure(sizeof(float) * R) || !e.d_Sc.ensure(sizeof(float) * (size_t)e.R_pad * e.n_mp) ||
+      !e.d_wts.ensure(sizeof(float) * R) || !e.d_wT.ensure(sizeof(float) * R))
+    return 0.0f;
+  EMC_CHECK(hipMemcpyAsync(e.d_wts.p, weights, sizeof(float) * R, hipMemcpyHostToDevice, e.stream));
+  // 1/phi from the current scaling: the M-step divides by the updated
+  // factors (emc_atomic.cu:94)
+  launch_reciprocal(d_scaling, N, e.d_invphi.as<float>(), e.stream);
+  const bool fp16 = e.precision == PREC_FP16;
+  {
+    PhaseTimer pt(PH_ROWSUM);
+    launch_rowsum(d_respons, R, N, e.d_invphi.as<float>(), e.d_T.as<double>(), e.d_Tf.as<float>(),
+                  e.d_Pscale.as<float>(), fp16, e.stream);
+  }
+  {
+    MstepArgs a{};
+    a.P = d_respons;
+    a.invphi = e.d_invphi.as<float>();
+    a.rowscale = fp16 ? e.d_Pscale.as<float>() : nullptr;
+    a.Tf = e.d_Tf.as<float>();
+    a.pix = e.d_pix.as<int>();
+    a.slices = slices;
+    a.Sc = e.d_Sc.as<float>();
+    a.kscale_inv = 1.0f / e.k_scale;
+    a.N = N;
+    a.R = R;
+    a.N_pad = e.N_pad;
+    a.R_pad = e.R_pad;
+    a.n_mp = e.n_mp;
+    a.n_m = e.n_m;
+    a.N2d = N_2d;
+    a.vecP = (N % 4 == 0) && ((uintptr_t)d_respons % 16 == 0);
+    PhaseTimer pt(PH_MSTEP);
+    if (fp16)
+      launch_mstep(PREC_FP16, !e.k_exact, e.d_Kt16.p, e.d_Kt16lo.p, a, e.stream);
+    else
+      launch_mstep(PREC_FP32, false, e.d_Kt32.p, nullptr, a, e.stream);
+  }
+  g_slices_epoch++;
+  launch_set_wT(e.d_wts.as<float>(), e.d_Tf.as<float>(), R, e.d_wT.as<float>(), e.stream);
+  if (e.insert_mode == INSERT_DETERMINISTIC) {
+    auto& mk = e.map_key;
+    const bool have = mk.valid && mk.rot == d_rotations && mk.x == x_coordinates && mk.y == y_coordinates &&
+                      mk.z == z_coordinates && mk.mask == d_mask && mk.R == R && mk.S == S && mk.interp == e.interp &&
+                      mk.stamp == e.invalidate_stamp && e.map_entries > 0;
+    if (!have) {
+      mk = {};
+      PhaseTimer pt(PH_OTHER);
+      launch_build_map(e.d_ins_pix.as<int>(), e.n_ins, d_rotations, R, S, x_coordinates, y_coordinates, z_coordinates,
+                       e.interp, e, e.stream);
+      if (failed()) return 0.0f;
+      mk.valid = true;
+      mk.rot = d_rotations;
+      mk.x = x_coordinates;
+      mk.y = y_coordinates;
+      mk.z = z_coordinates;
+      mk.mask = d_mask;
+      mk.R = R;
+      mk.S = S;
+      mk.interp = e.interp;
+      mk.stamp = e.invalidate_stamp;
+    }
+    PhaseTimer pt(PH_INSERT);
+    launch_insert_gather(e.d_Sc.as<float>(), e.n_mp, e.d_ins_col.as<int>(), e.n_ins, e.d_wT.as<float>(),
+                         e.d_map_off.as<uint32_t>(), e.d_map_ent.as<uint32_t>(),
+                         e.interp == INTERP_NEAREST ? nullptr : e.d_map_wt.as<float>(), S, e.interp, d_model, d_weight,
+                         e.stream);
+  } else {
+    PhaseTimer pt(PH_INSERT);
+    launch_insert_atomic(e.d_Sc.as<float>(), e.n_mp, e.d_pix.as<int>(), d_mask, e.d_wT.as<float>(), d_rotations, R, S,
+                         x_coordinates, y_coordinates, z_coordinates, d_model, d_weight, e.interp, e.stream);
+  }
+  std::vector<float> T(R);
+  EMC_CHECK(hipMemcpyAsync(T.data(), e.d_Tf.p, sizeof(float) * R, hipMemcpyDeviceToHost, e.stream));
+  EMC_CHECK(hipMemcpyAsync(model->data, d_model, sizeof(float) * (size_t)S * S * S, hipMemcpyDeviceToHost, e.stream));
+  sync();
+  e.like.valid = false;
+  float overall = 0.0f;  // emc_cuda.cu:348-351
+  for (int r = 0; r < R; r++) overall += T[r];
+  return overall;
+}
+
+real cuda_model_max(real* model, int model_size) {
+  if (!ready()) return 0.0f;
+  Engine& e = engine();
+  if (!model || model_size < 0) {
+    set_error("cuda_model_max: invalid argument");
+    return 0.0f;
+  }
+  if (!e.d_red.ensure(sizeof(double) * 1100)) return 0.0f;
+  float* out = e.d_red.as<float>();
+  launch_fill(out, 1, 0.0f, e.stream);
+  launch_model_max(model, model_size, out, e.stream);
+  float m = 0.0f;
+  EMC_CHECK(hipMemcpyAsync(&m, out, sizeof(float), hipMemcpyDeviceToHost, e.stream));
+  sync();
+  return m;
+}
+
+static void* alloc_dev(size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("out of device memory allocating %zu bytes", bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+void cuda_allocate_slices(real** slices, Setup setup, int N_slices) {
+  if (!ready() || !slices) return;
+  *slices = (real*)alloc_dev(sizeof(real) * (size_t)setup.side * setup.side * (size_t)N_slices);
+}
+
+void cuda_allocate_model(real** d_model, sp_3matrix* model) {
+  if (!ready() || !d_model || !model) return;
+  const size_t n = sp_3matrix_size(model);
+  *d_model = (real*)alloc_dev(sizeof(real) * n);
+  if (*d_model) EMC_CHECK(hipMemcpy(*d_model, model->data, sizeof(real) * n, hipMemcpyHostToDevice));
+}
+
+void cuda_allocate_mask(int** d_mask, sp_imatrix* mask) {
+  if (!ready() || !d_mask || !mask) return;
+  const size_t n = sp_imatrix_size(mask);
+  *d_mask = (int*)alloc_dev(sizeof(int) * n);
+  if (*d_mask) EMC_CHECK(hipMemcpy(*d_mask, mask->data, sizeof(int) * n, hipMemcpyHostToDevice));
+  engine().invalidate_stamp++;
+}
+
+void cuda_reset_model(sp_3matrix* model, real* d_model) {
+  if (!ready() || !model || !d_model) return;
+  launch_fill(d_model, sp_3matrix_size(model), 0.0f, engine().stream);
+  sync();
+}
+
+void cuda_normalize_model(sp_3matrix* model, real* d_model, real* d_weight) {
+  if (!ready()) return;
+  Engine& e = engine();
+  if (!model || !d_model || !d_weight) {
+    set_error("cuda_normalize_model: invalid argument");
+    return;
+  }
+  if (!e.d_red.ensure(sizeof(double) * 1100)) return;
+  {
+    PhaseTimer pt(PH_NORMALIZE);
+    launch_normalize_model(d_model, d_weight, (int)sp_3matrix_size(model), e.d_red.as<double>(), e.stream);
+  }
+  sync();
+}
+
+void cuda_allocate_rotations(real** d_rotations, Quaternion** rotations, int N_slices) {
+  if (!ready() || !d_rotations || !rotations || N_slices < 0) return;
+  std::vector<float> q((size_t)4 * N_slices);
+  for (int i = 0; i < N_slices; i++) memcpy(&q[4 * (size_t)i], rotations[i]->q, sizeof(float) * 4);
+  *d_rotations = (real*)alloc_dev(sizeof(real) * q.size());
+  if (*d_rotations) EMC_CHECK(hipMemcpy(*d_rotations, q.data(), sizeof(real) * q.size(), hipMemcpyHostToDevice));
+  engine().invalidate_stamp++;
+}
+
+void cuda_allocate_images(real** d_images, sp_matrix** images, int N_images) {
+  if (!ready() || !d_images || !images || N_images <= 0) return;
+  const size_t n2 = sp_matrix_size(images[0]);
+  *d_images = (real*)alloc_dev(sizeof(real) * n2 * N_images);
+  if (!*d_images) return;
+  for (int i = 0; i < N_images; i++)
+    EMC_CHECK(hipMemcpy(*d_images + n2 * i, images[i]->data, sizeof(real) * n2, hipMemcpyHostToDevice));
+  engine().invalidate_stamp++;
+}
+
+void cuda_allocate_coords(real** d_x, real** d_y, real** d_z, sp_matrix* x, sp_matrix* y, sp_matrix* z) {
+  if (!ready() || !d_x || !d_y || !d_z || !x || !y || !z) return;
+  const size_t n = sp_matrix_size(x);
+  real** dst[3] = {d_x, d_y, d_z};
+  sp_matrix* src[3] = {x, y, z};
+  for (int k = 0; k < 3; k++) {
+    *dst[k] = (real*)alloc_dev(sizeof(real) * n);
+    if (*dst[k]) EMC_CHECK(hipMemcpy(*dst[k], src[k]->data, sizeof(real) * n, hipMemcpyHostToDevice));
+  }
+  engine().invalidate_stamp++;
+}
+
+void cuda_allocate_real(real** x, int n) {
+  if (!ready() || !x) return;
+  if (n < 0) {
+    set_error("cuda_allocate_real: negative size %d", n);
+    return;
+  }
+  *x = (real*)alloc_dev((size_t)n);
+}
+
+void cuda_allocate_scaling(real** d_scaling, int N_images) {
+  if (!ready() || !d_scaling || N_images < 0) return;
+  *d_scaling = (real*)alloc_dev(sizeof(real) * (size_t)N_images);
+  if (*d_scaling) launch_fill(*d_scaling, (size_t)N_images, 1.0f, engine().stream);
+  sync();
+}
+
+// ---------------------------------------------------------------------------
+// emc_hip.h
+// ---------------------------------------------------------------------------
+int emc_hip_last_error(char* buf, size_t n) {
+  if (buf && n) {
+    strncpy(buf, g_err_code ? g_err : "", n - 1);
+    buf[n - 1] = 0;
+  }
+  return g_err_code;
+}
+
+void emc_hip_clear_error(void) {
+  g_err_code = 0;
+  g_err[0] = 0;
+  (void)hipGetLastError();
+}
+
+void emc_hip_free(void* d_ptr) {
+  if (!d_ptr) return;
+  Engine& e = engine();
+  if (e.stream) (void)hipStreamSynchronize(e.stream);
+  (void)hipFree(d_ptr);
+  e.invalidate_stamp++;
+  e.like.valid = false;
+}
+
+void emc_hip_synchronize(void) {
+  if (engine().stream) sync();
+}
+
+void emc_hip_shutdown(void) {
+  if (!g_engine) return;
+  if (g_engine->stream) (void)hipStreamSynchronize(g_engine->stream);
+  g_engine->release_all();
+  if (g_engine->ev0) (void)hipEventDestroy(g_engine->ev0);
+  if (g_engine->ev1) (void)hipEventDestroy(g_engine->ev1);
+  if (g_engine->stream) (void)hipStreamDestroy(g_engine->stream);
+  delete g_engine;
+  g_engine = nullptr;
+}
+
+void emc_hip_allocate_bytes(void** d_ptr, size_t bytes) {
+  if (!ready() || !d_ptr) return;
+  *d_ptr = alloc_dev(bytes);
+}
+
+int emc_hip_memcpy_to_host(void* dst, const void* d_src, size_t bytes) {
+  if (!ready()) return 1;
+  EMC_CHECK(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, engine().stream));
+  sync();
+  return failed() ? 1 : 0;
+}
+
+int emc_hip_memcpy_to_device(void* d_dst, const void* src, size_t bytes) {
+  if (!ready()) return 1;
+  EMC_CHECK(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, engine().stream));
+  sync();
+  Engine& e = engine();
+  e.invalidate_stamp++;
+  e.like.valid = false;
+  return failed() ? 1 : 0;
+}
+
+int emc_hip_set_option(const char* key, const char* value) {
+  Engine& e = engine();
+  if (!key || !value) return 1;
+  auto is = [&](const char* a, const char* b) { return strcmp(a, b) == 0; };
+  if (is(key, "precision")) {
+    if (is(value, "fp32")) e.precision = PREC_FP32;
+    else if (is(value, "fp16")) e.precision = PREC_FP16;
+    else return 1;
+  } else if (is(key, "insert")) {
+    if (is(value, "atomic")) e.insert_mode = INSERT_ATOMIC;
+    else if (is(value, "deterministic")) e.insert_mode = INSERT_DETERMINISTIC;
+    else return 1;
+  } else if (is(key, "interp")) {
+    if (is(value, "nearest")) e.interp = INTERP_NEAREST;
+    else if (is(value, "trilinear")) e.interp = INTERP_TRILINEAR;
+    else return 1;
+  } else if (is(key, "timing")) {
+    e.timing = value[0] == '1';
+  } else if (is(key, "verbose")) {
+    e.verbose = value[0] == '1';
+  } else {
+    return 1;
+  }
+  e.like.valid = false;
+  return 0;
+}
+
+void emc_hip_invalidate(void) {
+  Engine& e = engine();
+  e.invalidate_stamp++;
+  e.like.valid = false;
+}
+
+static const char* kPhaseNames[PH_COUNT] = {"expand", "pack", "likelihood", "softmax", "scaling",
+                                            "rowsum", "mstep", "insert", "normalize", "other"};
+
+int emc_hip_phase_times(double* ms, long long* launches, int n) {
+  Engine& e = engine();
+  int k = 0;
+  for (; k < n && k < PH_COUNT; k++) {
+    if (ms) ms[k] = e.phase_ms[k];
+    if (launches) launches[k] = e.phase_launches[k];
+  }
+  return k;
+}
+
+void emc_hip_reset_phase_times(void) {
+  Engine& e = engine();
+  for (int k = 0; k < PH_COUNT; k++) {
+    e.phase_ms[k] = 0;
+    e.phase_launches[k] = 0;
+  }
+}
+
+const char* emc_hip_phase_name(int k) { return k >= 0 && k < PH_COUNT ? kPhaseNames[k] : nullptr; }
+
+int emc_hip_pack_info(int* out, int n) {
+  Engine& e = engine();
+  const int v[4] = {e.n_m, e.n_mp, e.k_exact ? 1 : 0, e.precision == PREC_FP16 ? (e.k_exact ? 2 : 3) : 1};
+  int k = 0;
+  for (; k < n && k < 4; k++) out[k] = v[k];
+  return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,247 @@
+// Internal state and kernel launchers of libemc_hip (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "emc.h"
+#include "emc_hip.h"
+
+namespace emc {
+
+// ---- errors -------------------------------------------------------------
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+bool failed();  // sticky error present
+
+#define EMC_CHECK(call)                                                                       \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) ::emc::set_error("%s:%d: %s: %s", __FILE__, __LINE__, #call,        \
+                                           hipGetErrorString(e_));                            \
+  } while (0)
+#define EMC_LAUNCHED() EMC_CHECK(hipGetLastError())
+
+// ---- device buffers owned by the engine ----------------------------------
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  // Grow to at least `b` bytes (contents not preserved).  False on failure.
+  bool ensure(size_t b);
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// ---- phases for timing -----------------------------------------------------
+enum Phase {
+  PH_EXPAND = 0, PH_PACK, PH_LIKE, PH_SOFTMAX, PH_SCALING, PH_ROWSUM, PH_MSTEP,
+  PH_INSERT, PH_NORMALIZE, PH_OTHER, PH_COUNT
+};
+
+enum Precision { PREC_FP32 = 0, PREC_FP16 = 1 };
+enum InsertMode { INSERT_ATOMIC = 0, INSERT_DETERMINISTIC = 1 };
+enum Interp { INTERP_NEAREST = 0, INTERP_TRILINEAR = 1 };
+
+// Tile sizes shared by the GEMM kernels and the padding of packed operands.
+constexpr int kTileM = 128;
+constexpr int kTileN = 128;
+constexpr int kPadK = 128;  // pixel-dimension padding: multiple of every BK and of kTileN
+
+inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+struct Engine {
+  int device = -1;
+  hipStream_t stream = nullptr;
+
+  // options
+  int precision = PREC_FP32;
+  int insert_mode = INSERT_ATOMIC;
+  int interp = INTERP_NEAREST;
+  bool timing = false;
+  bool verbose = false;
+
+  // generation counters: bumped whenever the library writes a buffer kind
+  uint64_t gen = 1;
+  uint64_t next_gen() { return ++gen; }
+
+  // ---- packed patterns (depends on images, mask, N, N2d, precision) ----
+  struct {
+    const float* images = nullptr;
+    const int* mask = nullptr;
+    int N = 0, N2d = 0, precision = -1;
+    uint64_t stamp = 0;  // invalidation stamp at build time
+  } pack_key;
+  int n_m = 0;      // masked pixels (mask != 0)
+  int n_mp = 0;     // padded to kPadK
+  int n_ins = 0;    // pixels with mask == 1 (inserted)
+  int N_pad = 0;    // patterns padded to the GEMM tile
+  bool k_exact = true;   // FP16 operand holds the counts exactly
+  float k_scale = 1.0f;  // FP16 operand = K * k_scale (power of two)
+  DBuf d_pix;       // int [n_mp]: pixel index of compact column, -1 padding
+  DBuf d_B;         // double [N]: sum_p m K^2
+  DBuf d_K32;       // float [N_pad][n_mp]
+  DBuf d_Kt32;      // float [n_mp][N_pad]
+  DBuf d_K16, d_K16lo;    // half [N_pad][n_mp]
+  DBuf d_Kt16, d_Kt16lo;  // half [n_mp][N_pad]
+  DBuf d_flags;     // scratch flags for the pack kernels
+
+  // ---- packed slices (per likelihood call) ----
+  int R_pad = 0;
+  DBuf d_S32;           // float [R_pad][n_mp]
+  DBuf d_S16, d_S16lo;  // half [R_pad][n_mp]
+  DBuf d_Sscale;        // float [R]: 1/scale_r, where the FP16 operand is S * scale_r (power of two)
+  DBuf d_A;             // double [R]: sum_p m S^2
+
+  // ---- likelihood -> softmax -> scaling fusion state ----
+  struct {
+    bool valid = false;
+    const float* respons = nullptr;  // buffer holding L (then P)
+    const float* slices = nullptr;
+    const float* images = nullptr;
+    uint64_t slices_gen = 0;
+    int N = 0, R = 0, N2d = 0;
+    double sigma2x = 0.0;  // 2 n_m sigma^2
+    bool normalized = false;  // softmax ran on this buffer after the likelihood
+    bool have_total = false;
+    double total = 0.0;       // sum P log P from the softmax pass
+  } like;
+  DBuf d_phi_used;   // float [N]: scaling used by the likelihood
+  DBuf d_smax;       // float  [chunks][N] partial maxima
+  DBuf d_ssum;       // double [chunks][N] partial sums
+  DBuf d_sacc;       // double [4][chunks][N] partial P-weighted sums
+  DBuf d_phiden;     // double [N]: sum_r P_ri C_ir
+  DBuf d_red;        // double scratch for reductions
+  DBuf d_invphi;     // float [N]: 1/scaling for the M-step
+
+  // ---- M-step ----
+  DBuf d_T;          // double [R]
+  DBuf d_Tf;         // float [R]
+  DBuf d_Pscale;     // float [R]: FP16 M-step operand row scale
+  DBuf d_Sc;         // float [R_pad][n_mp]: updated compact slices for insertion
+  DBuf d_wts;        // float [R]: rotation weights (uploaded per call)
+  DBuf d_wT;         // float [R]: weights[r] * T_r
+
+  // ---- insertion map for the deterministic mode ----
+  struct {
+    const float* rot = nullptr;
+    const float *x = nullptr, *y = nullptr, *z = nullptr;
+    const int* mask = nullptr;
+    int R = 0, S = 0, interp = -1;
+    uint64_t stamp = 0;
+    bool valid = false;
+  } map_key;
+  DBuf d_ins_pix;    // int [n_ins]: inserted pixels in x-major order
+  DBuf d_ins_col;    // int [n_ins]: their compact column
+  DBuf d_map_off;    // uint32 [S^3 + 1]
+  DBuf d_map_ent;    // uint32 [entries]: entry id e = r * n_ins + t
+  DBuf d_map_wt;     // float [entries]: trilinear weight (trilinear only)
+  size_t map_entries = 0;
+
+  uint64_t invalidate_stamp = 1;
+
+  // ---- timing ----
+  double phase_ms[PH_COUNT] = {0};
+  long long phase_launches[PH_COUNT] = {0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  bool init();
+  void release_all();
+};
+
+Engine& engine();          // initialised on first use (device = current)
+bool ready();              // engine initialised and no sticky error
+
+// Phase timer: records events around a region when timing is on.
+struct PhaseTimer {
+  int phase;
+  bool on;
+  explicit PhaseTimer(int ph);
+  ~PhaseTimer();
+};
+
+// ---- kernel launchers (defined in the k_*.hip files) ----------------------
+// expand
+void launch_expand(const float* model, int S, const float* rot, int R, const float* x,
+                   const float* y, const float* z, float* slices, int interp, hipStream_t st);
+// pack patterns: compact columns, B_i, FP32 and FP16 copies (+ transposes)
+void launch_pack_mask(const int* mask, int N2d, int* pix, int n_mp, hipStream_t st);
+void launch_pack_images_f32(const float* images, const int* pix, int N, int N2d, int n_mp,
+                            int N_pad, float* K32, float* Kt32, double* B, hipStream_t st);
+void launch_pack_images_f16(const float* images, const int* pix, int N, int N2d, int n_mp,
+                            int N_pad, float kscale, bool split, _Float16* K16, _Float16* K16lo,
+                            _Float16* Kt16, _Float16* Kt16lo, double* B, hipStream_t st);
+void launch_images_stats(const float* images, const int* pix, int N, int N2d, int n_mp,
+                         unsigned int* flags, hipStream_t st);
+// pack slices: compact FP32 or FP16 hi/lo, per-rotation scale, A_r
+void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d, int n_mp,
+                            int R_pad, float* S32, double* A, hipStream_t st);
+void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d, int n_mp,
+                            int R_pad, _Float16* S16, _Float16* S16lo, float* scale, double* A,
+                            hipStream_t st);
+
+// GEMMs (k_gemm.hip)
+struct LikeEpilogue {
+  float* L;               // [R][N] output, or nullptr
+  const float* P;         // [R][N] responsibilities (PC mode), or nullptr
+  double* D;              // [N] sum_r P_ri C_ir accumulator (PC mode)
+  const double* A;        // [R]
+  const double* B;        // [N]
+  const float* phi;       // [N]
+  const float* cscale;    // [R] C = acc * cscale[r] (FP16 path), or nullptr
+  float kscale_inv;       // 1 / k_scale (FP16 path)
+  double inv_2nms2;       // 1 / (2 n_m sigma^2)
+  int N, R;
+  int vecL;               // L rows 16-byte aligned (N % 4 == 0, aligned base)
+};
+void launch_estep(int precision, bool ksplit, const void* K, const void* Klo, const void* S,
+                  const void* Slo, int N_pad, int R_pad, int n_mp, const LikeEpilogue& ep,
+                  hipStream_t st);
+struct MstepArgs {
+  const float* P;         // [R][N]
+  const float* invphi;    // [N]
+  const float* rowscale;  // [R] (FP16 path)
+  const float* Tf;        // [R]
+  const int* pix;         // [n_mp]
+  float* slices;          // [R][N2d] full output (masked columns written)
+  float* Sc;              // [R_pad][n_mp] compact output
+  float kscale_inv;
+  int N, R, N_pad, R_pad, n_mp, n_m, N2d;
+  int vecP;               // P rows 16-byte aligned (N % 4 == 0, aligned base)
+};
+void launch_mstep(int precision, bool ksplit, const void* Kt, const void* Ktlo,
+                  const MstepArgs& a, hipStream_t st);
+
+// softmax / totals / scaling (k_softmax.hip)
+int softmax_chunks(int N, int R);
+void launch_softmax(float* L, int R, int N, int chunks, float* smax, double* ssum, double* sacc,
+                    const double* A, bool fuse_phi, hipStream_t st);
+void launch_softmax_reduce(int N, int chunks, const double* sacc, bool fuse_phi,
+                           const double* B, const float* phi_used, double sigma2x,
+                           double* phiden, double* red, hipStream_t st);
+void launch_total_respons(const float* P, size_t n, double* red, hipStream_t st);
+void launch_scaling_finish(int N, const double* B, const double* phiden, float* scaling,
+                           float* invphi, hipStream_t st);
+void launch_reciprocal(const float* s, int N, float* inv, hipStream_t st);
+// model helpers (k_model.hip)
+void launch_rowsum(const float* P, int R, int N, const float* invphi, double* T, float* Tf,
+                   float* rowscale, bool want_scale, hipStream_t st);
+void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* mask,
+                          const float* wT, const float* rot, int R, int S, const float* x,
+                          const float* y, const float* z, float* model, float* weight,
+                          int interp, hipStream_t st);
+void launch_build_map(const int* ins_pix, int n_ins, const float* rot, int R, int S,
+                      const float* x, const float* y, const float* z, int interp,
+                      Engine& eng, hipStream_t st);
+void launch_insert_gather(const float* Sc, int n_mp, const int* ins_col, int n_ins,
+                          const float* wT, const uint32_t* off, const uint32_t* ent,
+                          const float* ent_wt, int S, int interp, float* model, float* weight,
+                          hipStream_t st);
+void launch_set_wT(const float* wts, const float* Tf, int R, float* wT, hipStream_t st);
+void launch_normalize_model(float* model, const float* weight, int n, double* red,
+                            hipStream_t st);
+void launch_model_max(const float* model, int n, float* out, hipStream_t st);
+void launch_fill(float* p, size_t n, float v, hipStream_t st);
+
+}  // namespace emc

@@ -1,0 +1,269 @@
+// Expand (slicing the model at every rotation) and operand packing kernels.
+//
+// Slicing follows emc_cuda.cu:51-111: the pixel coordinate is rotated by the
+// quaternion's rotation matrix, written exactly as the reference writes it in
+// float, and the voxel index along each axis is roundf(S/2 + 0.5 + c).  FMA
+// contraction is disabled in these functions so the index matches the
+// oracle's float evaluation bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+#include "geom.hpp"
+#include "reduce.hpp"
+
+namespace emc {
+
+// One workgroup per rotation; lanes walk the pixels in order so slice writes
+// and coordinate reads are coalesced; model reads are gathers served by L2.
+__global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict__ model, int S,
+                                                        const float* __restrict__ rot,
+                                                        const float* __restrict__ x,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ z,
+                                                        float* __restrict__ slices) {
+  const int r = blockIdx.x;
+  float e[9];
+  rot_matrix(&rot[4 * (size_t)r], e);
+  const float half = (float)S / 2.0f + 0.5f;
+  const int N2d = S * S;
+  float* out = slices + (size_t)r * N2d;
+  for (int p = threadIdx.x; p < N2d; p += blockDim.x) {
+    float ux, uy, uz;
+    voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+    const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
+    float v = 0.0f;
+    if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S)
+      v = model[((size_t)vz * S + vy) * S + vx];
+    out[p] = v;
+  }
+}
+
+// Trilinear slicing (not in the reference): the 8 voxels around the
+// continuous coordinate, those inside 0 <= v < S, weighted by the usual
+// products of fractional distances.
+__global__ __launch_bounds__(256) void k_expand_trilinear(const float* __restrict__ model, int S,
+                                                          const float* __restrict__ rot,
+                                                          const float* __restrict__ x,
+                                                          const float* __restrict__ y,
+                                                          const float* __restrict__ z,
+                                                          float* __restrict__ slices) {
+  const int r = blockIdx.x;
+  float e[9];
+  rot_matrix(&rot[4 * (size_t)r], e);
+  const float half = (float)S / 2.0f + 0.5f;
+  const int N2d = S * S;
+  float* out = slices + (size_t)r * N2d;
+  for (int p = threadIdx.x; p < N2d; p += blockDim.x) {
+    float ux, uy, uz;
+    voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      int v;
+      float w;
+      if (tri_corner(c, ux, uy, uz, S, v, w)) acc += w * model[v];
+    }
+    out[p] = acc;
+  }
+}
+
+void launch_expand(const float* model, int S, const float* rot, int R, const float* x,
+                   const float* y, const float* z, float* slices, int interp, hipStream_t st) {
+  if (R <= 0) return;
+  if (interp == INTERP_TRILINEAR)
+    hipLaunchKernelGGL(k_expand_trilinear, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+  else
+    hipLaunchKernelGGL(k_expand_nearest, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+  EMC_LAUNCHED();
+}
+
+// ---------------------------------------------------------------------------
+// Pattern packing: gather the masked pixels of every pattern into a dense,
+// padded row (the GEMM operand), and sum_p m K^2 in double.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_images_stats(const float* __restrict__ images,
+                                                      const int* __restrict__ pix, int N, int N2d,
+                                                      int n_mp, unsigned int* flags) {
+  const int i = blockIdx.x;
+  unsigned int inexact = 0, maxbits = 0;
+  for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
+    const int p = pix[k];
+    if (p < 0) continue;
+    const float v = images[(size_t)i * N2d + p];
+    const float a = fabsf(v);
+    if (!(v == rintf(v)) || !(a <= 2048.0f)) inexact = 1;
+    if (a == a) maxbits = max(maxbits, __float_as_uint(a));
+  }
+  inexact = block_max_u32(inexact);
+  maxbits = block_max_u32(maxbits);
+  if (threadIdx.x == 0) {
+    if (inexact) atomicOr(&flags[0], 1u);
+    atomicMax(&flags[1], maxbits);
+  }
+}
+
+void launch_images_stats(const float* images, const int* pix, int N, int N2d, int n_mp,
+                         unsigned int* flags, hipStream_t st) {
+  hipLaunchKernelGGL(k_images_stats, dim3(N), dim3(256), 0, st, images, pix, N, N2d, n_mp, flags);
+  EMC_LAUNCHED();
+}
+
+__global__ __launch_bounds__(256) void k_pack_images_f32(const float* __restrict__ images,
+                                                         const int* __restrict__ pix, int N, int N2d,
+                                                         int n_mp, float* __restrict__ K32,
+                                                         double* __restrict__ B) {
+  const int i = blockIdx.x;  // 0 .. N_pad-1
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
+    const int p = pix[k];
+    float v = 0.0f;
+    if (i < N && p >= 0) v = images[(size_t)i * N2d + p];
+    K32[(size_t)i * n_mp + k] = v;
+    acc += (double)v * v;
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0 && i < N) B[i] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_pack_images_f16(const float* __restrict__ images,
+                                                         const int* __restrict__ pix, int N, int N2d,
+                                                         int n_mp, float kscale, int split,
+                                                         _Float16* __restrict__ K16,
+                                                         _Float16* __restrict__ K16lo,
+                                                         double* __restrict__ B) {
+  const int i = blockIdx.x;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
+    const int p = pix[k];
+    float v = 0.0f;
+    if (i < N && p >= 0) v = images[(size_t)i * N2d + p];
+    const float s = v * kscale;
+    const _Float16 hi = (_Float16)s;
+    K16[(size_t)i * n_mp + k] = hi;
+    if (split) K16lo[(size_t)i * n_mp + k] = (_Float16)(s - (float)hi);
+    acc += (double)v * v;
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0 && i < N) B[i] = acc;
+}
+
+// out[c][r] = in[r][c] for an rows x cols matrix, 64x64 tiles through LDS.
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(const T* __restrict__ in, T* __restrict__ out,
+                                                   int rows, int cols) {
+  __shared__ T tile[64][65];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+  for (int j = ty; j < 64; j += 4) {
+    const int r = r0 + j, c = c0 + tx;
+    tile[j][tx] = (r < rows && c < cols) ? in[(size_t)r * cols + c] : T(0);
+  }
+  __syncthreads();
+  for (int j = ty; j < 64; j += 4) {
+    const int c = c0 + j, r = r0 + tx;
+    if (c < cols && r < rows) out[(size_t)c * rows + r] = tile[tx][j];
+  }
+}
+
+template <class T>
+static void transpose(const T* in, T* out, int rows, int cols, hipStream_t st) {
+  dim3 grid((cols + 63) / 64, (rows + 63) / 64);
+  hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), 0, st, in, out, rows, cols);
+  EMC_LAUNCHED();
+}
+
+void launch_pack_images_f32(const float* images, const int* pix, int N, int N2d, int n_mp,
+                            int N_pad, float* K32, float* Kt32, double* B, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_images_f32, dim3(N_pad), dim3(256), 0, st, images, pix, N, N2d, n_mp, K32, B);
+  EMC_LAUNCHED();
+  transpose<float>(K32, Kt32, N_pad, n_mp, st);
+}
+
+void launch_pack_images_f16(const float* images, const int* pix, int N, int N2d, int n_mp,
+                            int N_pad, float kscale, bool split, _Float16* K16, _Float16* K16lo,
+                            _Float16* Kt16, _Float16* Kt16lo, double* B, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_images_f16, dim3(N_pad), dim3(256), 0, st, images, pix, N, N2d, n_mp,
+                     kscale, (int)split, K16, K16lo, B);
+  EMC_LAUNCHED();
+  transpose<_Float16>(K16, Kt16, N_pad, n_mp, st);
+  if (split) transpose<_Float16>(K16lo, Kt16lo, N_pad, n_mp, st);
+}
+
+// ---------------------------------------------------------------------------
+// Slice packing: compact masked columns, A_r = sum_p m S^2 in double.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_pack_slices_f32(const float* __restrict__ slices,
+                                                         const int* __restrict__ pix, int R, int N2d,
+                                                         int n_mp, float* __restrict__ S32,
+                                                         double* __restrict__ A) {
+  const int r = blockIdx.x;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
+    const int p = pix[k];
+    float v = 0.0f;
+    if (r < R && p >= 0) v = slices[(size_t)r * N2d + p];
+    S32[(size_t)r * n_mp + k] = v;
+    acc += (double)v * v;
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0 && r < R) A[r] = acc;
+}
+
+// Per-rotation power-of-two scale so that max |S| * scale <= 2^14, then the
+// FP16 pair hi = fp16(S*scale), lo = fp16(S*scale - hi).
+__global__ __launch_bounds__(256) void k_pack_slices_f16(const float* __restrict__ slices,
+                                                         const int* __restrict__ pix, int R, int N2d,
+                                                         int n_mp, _Float16* __restrict__ S16,
+                                                         _Float16* __restrict__ S16lo,
+                                                         float* __restrict__ scale,
+                                                         double* __restrict__ A) {
+  const int r = blockIdx.x;
+  const bool live = r < R;
+  float mx = 0.0f;
+  if (live)
+    for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
+      const int p = pix[k];
+      if (p >= 0) mx = fmaxf(mx, fabsf(slices[(size_t)r * N2d + p]));
+    }
+  mx = block_max_f32(mx);
+  float sc = 1.0f;
+  if (mx > 0.0f && mx < 3.0e38f) {
+    int e;
+    frexpf(mx, &e);  // mx < 2^e
+    sc = ldexpf(1.0f, max(-120, min(120, 14 - e)));
+  }
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
+    const int p = pix[k];
+    float v = 0.0f;
+    if (live && p >= 0) v = slices[(size_t)r * N2d + p];
+    const float s = v * sc;
+    const _Float16 hi = (_Float16)s;
+    S16[(size_t)r * n_mp + k] = hi;
+    S16lo[(size_t)r * n_mp + k] = (_Float16)(s - (float)hi);
+    acc += (double)v * v;
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0 && live) {
+    A[r] = acc;
+    scale[r] = 1.0f / sc;  // exact: sc is a power of two
+  }
+}
+
+void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d, int n_mp,
+                            int R_pad, float* S32, double* A, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_slices_f32, dim3(R_pad), dim3(256), 0, st, slices, pix, R, N2d, n_mp, S32, A);
+  EMC_LAUNCHED();
+}
+
+void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d, int n_mp,
+                            int R_pad, _Float16* S16, _Float16* S16lo, float* scale, double* A,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_slices_f16, dim3(R_pad), dim3(256), 0, st, slices, pix, R, N2d, n_mp, S16,
+                     S16lo, scale, A);
+  EMC_LAUNCHED();
+}
+
+}  // namespace emc

@@ -1,0 +1,371 @@
+// Compress: per-rotation totals, insertion of the updated slices into the
+// model and weight accumulators, model normalisation and maximum.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "engine.hpp"
+#include "geom.hpp"
+#include "reduce.hpp"
+
+namespace emc {
+
+// T_r = sum_i P_ri (emc_atomic.cu:99-101, FP64 here) and, for the FP16
+// M-step, a power-of-two row scale with max_i P_ri/phi_i * scale <= 2^14.
+__global__ __launch_bounds__(256) void k_rowsum(const float* __restrict__ P, int R, int N,
+                                                const float* __restrict__ invphi, double* __restrict__ T,
+                                                float* __restrict__ Tf, float* __restrict__ rowscale, int want) {
+  const int r = blockIdx.x;
+  const float* row = P + (size_t)r * N;
+  double t = 0.0;
+  float mx = 0.0f;
+  for (int i = threadIdx.x; i < N; i += 256) {
+    const float p = row[i];
+    t += p;
+    if (want) mx = fmaxf(mx, fabsf(p * invphi[i]));
+  }
+  t = block_sum_f64(t);
+  if (want) mx = block_max_f32(mx);
+  if (threadIdx.x == 0) {
+    T[r] = t;
+    Tf[r] = (float)t;
+    if (want) {
+      float sc = 1.0f;
+      if (mx > 0.0f && mx < 3.0e38f) {
+        int e;
+        frexpf(mx, &e);
+        sc = ldexpf(1.0f, max(-120, min(120, 14 - e)));
+      }
+      rowscale[r] = sc;
+    }
+  }
+}
+
+void launch_rowsum(const float* P, int R, int N, const float* invphi, double* T, float* Tf, float* rowscale,
+                   bool want_scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_rowsum, dim3(R), dim3(256), 0, st, P, R, N, invphi, T, Tf, rowscale, (int)want_scale);
+  EMC_LAUNCHED();
+}
+
+// w_r T_r as the reference forms it: weights[i_slice] * total_respons
+// (emc_atomic.cu:129), float.
+__global__ void k_set_wT(const float* __restrict__ w, const float* __restrict__ Tf, int R, float* __restrict__ wT) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r < R) wT[r] = w[r] * Tf[r];
+}
+
+void launch_set_wT(const float* wts, const float* Tf, int R, float* wT, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_wT, dim3((R + 255) / 256), dim3(256), 0, st, wts, Tf, R, wT);
+  EMC_LAUNCHED();
+}
+
+// Atomic insertion (emc_atomic.cu:17-73,115-133): for rotations with
+// T_r > 1e-10 and pixels whose mask is 1, model[v] += w_r T_r S'_rp and
+// weight[v] += w_r T_r at the nearest voxel (bounds 0 <= v < S), or spread
+// over the trilinear footprint.  The order of the float additions is not
+// fixed.
+__global__ __launch_bounds__(256) void k_insert_atomic(const float* __restrict__ Sc, int n_mp, int n_m,
+                                                       const int* __restrict__ pix, const int* __restrict__ mask,
+                                                       const float* __restrict__ wT, const float* __restrict__ Tf,
+                                                       const float* __restrict__ rot, int S,
+                                                       const float* __restrict__ x, const float* __restrict__ y,
+                                                       const float* __restrict__ z, float* __restrict__ model,
+                                                       float* __restrict__ weight, int interp) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x;
+  if (!(Tf[r] > 1e-10f)) return;
+  const float w = wT[r];
+  float e[9];
+  rot_matrix(&rot[4 * (size_t)r], e);
+  const float half = (float)S / 2.0f + 0.5f;
+  for (int k = threadIdx.x; k < n_m; k += 256) {
+    const int p = pix[k];
+    if (mask[p] != 1) continue;
+    float ux, uy, uz;
+    voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+    const float val = w * Sc[(size_t)r * n_mp + k];
+    if (interp == INTERP_NEAREST) {
+      const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
+      if (vx < 0 || vx >= S || vy < 0 || vy >= S || vz < 0 || vz >= S) continue;
+      const size_t v = ((size_t)vz * S + vy) * S + vx;
+      atomicAdd(&model[v], val);
+      atomicAdd(&weight[v], w);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        int v;
+        float cw;
+        if (!tri_corner(c, ux, uy, uz, S, v, cw)) continue;
+        atomicAdd(&model[v], cw * val);
+        atomicAdd(&weight[v], cw * w);
+      }
+    }
+  }
+}
+
+void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* mask, const float* wT,
+                          const float* rot, int R, int S, const float* x, const float* y, const float* z,
+                          float* model, float* weight, int interp, hipStream_t st) {
+  Engine& eng = engine();
+  hipLaunchKernelGGL(k_insert_atomic, dim3(R), dim3(256), 0, st, Sc, n_mp, eng.n_m, pix, mask, wT,
+                     eng.d_Tf.as<float>(), rot, S, x, y, z, model, weight, interp);
+  EMC_LAUNCHED();
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic insertion.  The voxel hit by (rotation r, pixel t) depends
+// only on geometry, so the inverse map voxel -> entries is built once: entry
+// e = r * n_ins + t (t enumerates inserted pixels x-major, the reference's
+// loop order), keyed by its voxel and stably sorted, which leaves every
+// voxel's entries in increasing e.  Each iteration then sums every voxel's
+// contributions sequentially in that order -- no atomics, the same float
+// additions in the same order on every run (and the order of the reference's
+// sequential CPU insertion, emc.c:491-539 called per rotation).
+// Trilinear entries carry (e, corner) as 8 e + c.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_map_keys(const int* __restrict__ ins_pix, int n_ins, int R,
+                                                  const float* __restrict__ rot, int S, const float* __restrict__ x,
+                                                  const float* __restrict__ y, const float* __restrict__ z, int interp,
+                                                  uint32_t sentinel, uint32_t* __restrict__ keys) {
+  const size_t e_id = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e_id >= (size_t)R * n_ins) return;
+  const int r = (int)(e_id / n_ins);
+  const int t = (int)(e_id % n_ins);
+  float e[9];
+  rot_matrix(&rot[4 * (size_t)r], e);
+  const float half = (float)S / 2.0f + 0.5f;
+  const int p = ins_pix[t];
+  float ux, uy, uz;
+  voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+  const size_t ent = (size_t)r * n_ins + t;
+  if (interp == INTERP_NEAREST) {
+    const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
+    uint32_t key = sentinel;
+    if (vx >= 0 && vx < S && vy >= 0 && vy < S && vz >= 0 && vz < S) key = (uint32_t)((vz * S + vy) * S + vx);
+    keys[ent] = key;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      int v;
+      float cw;
+      keys[ent * 8 + c] = tri_corner(c, ux, uy, uz, S, v, cw) ? (uint32_t)v : sentinel;
+    }
+  }
+}
+
+__global__ void k_map_count(const uint32_t* __restrict__ sorted_keys, size_t n, uint32_t nvox, uint32_t* __restrict__ off) {
+  // off[v] = first position with key >= v, for v in [0, nvox]; keys sorted.
+  const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t k = sorted_keys[j];
+  const uint32_t prev = j == 0 ? 0u : min(sorted_keys[j - 1], nvox) + 1u;
+  const uint32_t hi = min(k, nvox);
+  if (j == 0)
+    for (uint32_t v = 0; v <= hi; v++) off[v] = 0;
+  else
+    for (uint32_t v = prev; v <= hi; v++) off[v] = (uint32_t)j;
+}
+
+__global__ void k_map_tail(const uint32_t* __restrict__ sorted_keys, size_t n, uint32_t nvox, uint32_t* __restrict__ off) {
+  // voxels above the largest key: off = n (or the sentinel start)
+  const uint32_t last = n ? min(sorted_keys[n - 1], nvox) : 0u;
+  const uint32_t from = n ? last + 1u : 0u;
+  for (uint32_t v = from + threadIdx.x; v <= nvox; v += 256) off[v] = (uint32_t)n;
+}
+
+__global__ void k_map_weights(const uint32_t* __restrict__ ent, size_t n, const int* __restrict__ ins_pix, int n_ins,
+                              const float* __restrict__ rot, int S, const float* __restrict__ x,
+                              const float* __restrict__ y, const float* __restrict__ z, float* __restrict__ wt) {
+  const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t code = ent[j];
+  const uint32_t eidx = code >> 3;
+  const int c = (int)(code & 7u);
+  const int r = (int)(eidx / (uint32_t)n_ins), t = (int)(eidx % (uint32_t)n_ins);
+  float e[9];
+  rot_matrix(&rot[4 * (size_t)r], e);
+  const float half = (float)S / 2.0f + 0.5f;
+  const int p = ins_pix[t];
+  float ux, uy, uz;
+  voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+  int v;
+  float cw = 0.0f;
+  tri_corner(c, ux, uy, uz, S, v, cw);
+  wt[j] = cw;
+}
+
+void launch_build_map(const int* ins_pix, int n_ins, const float* rot, int R, int S, const float* x,
+                      const float* y, const float* z, int interp, Engine& eng, hipStream_t st) {
+  const size_t per = interp == INTERP_NEAREST ? 1 : 8;
+  const size_t n = (size_t)R * n_ins * per;
+  const uint32_t nvox = (uint32_t)S * S * S;
+  eng.map_entries = 0;
+  if (n == 0 || n >= 0xffffffffull) {
+    if (n) set_error("insertion map: %zu entries exceed 32-bit entry ids", n);
+    return;
+  }
+  DBuf keys, keys_sorted, temp;
+  if (!keys.ensure(n * 4) || !keys_sorted.ensure(n * 4) || !eng.d_map_ent.ensure(n * 4) ||
+      !eng.d_map_off.ensure(((size_t)nvox + 1) * 4)) {
+    keys.release();
+    keys_sorted.release();
+    return;
+  }
+  hipLaunchKernelGGL(k_map_keys, dim3((unsigned)(((size_t)R * n_ins + 255) / 256)), dim3(256), 0, st, ins_pix, n_ins, R,
+                     rot, S, x, y, z, interp, nvox, keys.as<uint32_t>());
+  EMC_LAUNCHED();
+  unsigned int bits = 1;
+  while ((1ull << bits) <= nvox) bits++;
+  rocprim::counting_iterator<uint32_t> ids(0u);
+  size_t tbytes = 0;
+  EMC_CHECK(rocprim::radix_sort_pairs(nullptr, tbytes, keys.as<uint32_t>(), keys_sorted.as<uint32_t>(), ids,
+                                      eng.d_map_ent.as<uint32_t>(), n, 0, bits, st));
+  if (!temp.ensure(tbytes)) {
+    keys.release();
+    keys_sorted.release();
+    return;
+  }
+  EMC_CHECK(rocprim::radix_sort_pairs(temp.p, tbytes, keys.as<uint32_t>(), keys_sorted.as<uint32_t>(), ids,
+                                      eng.d_map_ent.as<uint32_t>(), n, 0, bits, st));
+  hipLaunchKernelGGL(k_map_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys_sorted.as<uint32_t>(), n,
+                     nvox, eng.d_map_off.as<uint32_t>());
+  EMC_LAUNCHED();
+  hipLaunchKernelGGL(k_map_tail, dim3(1), dim3(256), 0, st, keys_sorted.as<uint32_t>(), n, nvox,
+                     eng.d_map_off.as<uint32_t>());
+  EMC_LAUNCHED();
+  if (interp != INTERP_NEAREST) {
+    if (eng.d_map_wt.ensure(n * 4)) {
+      hipLaunchKernelGGL(k_map_weights, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                         eng.d_map_ent.as<uint32_t>(), n, ins_pix, n_ins, rot, S, x, y, z, eng.d_map_wt.as<float>());
+      EMC_LAUNCHED();
+    }
+  }
+  EMC_CHECK(hipStreamSynchronize(st));
+  temp.release();
+  keys.release();
+  keys_sorted.release();
+  eng.map_entries = n;
+}
+
+// One thread per voxel, contributions added sequentially in entry order,
+// starting from the accumulator's current value (the caller may not have
+// reset it).
+__global__ __launch_bounds__(256) void k_insert_gather(const float* __restrict__ Sc, int n_mp,
+                                                       const int* __restrict__ ins_col, int n_ins,
+                                                       const float* __restrict__ wT, const float* __restrict__ Tf,
+                                                       const uint32_t* __restrict__ off, const uint32_t* __restrict__ ent,
+                                                       const float* __restrict__ ent_wt, int nvox, int tri,
+                                                       float* __restrict__ model, float* __restrict__ weight) {
+#pragma clang fp contract(off)
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= nvox) return;
+  const uint32_t b = off[v], en = off[v + 1];
+  if (b == en) return;
+  float m = model[v], wsum = weight[v];
+  for (uint32_t j = b; j < en; j++) {
+    const uint32_t code = ent[j];
+    const uint32_t eidx = tri ? code >> 3 : code;
+    const int r = (int)(eidx / (uint32_t)n_ins), t = (int)(eidx % (uint32_t)n_ins);
+    if (!(Tf[r] > 1e-10f)) continue;
+    const float w = wT[r];
+    const float val = w * Sc[(size_t)r * n_mp + ins_col[t]];
+    if (tri) {
+      const float cw = ent_wt[j];
+      m += cw * val;
+      wsum += cw * w;
+    } else {
+      m += val;
+      wsum += w;
+    }
+  }
+  model[v] = m;
+  weight[v] = wsum;
+}
+
+void launch_insert_gather(const float* Sc, int n_mp, const int* ins_col, int n_ins, const float* wT,
+                          const uint32_t* off, const uint32_t* ent, const float* ent_wt, int S, int interp,
+                          float* model, float* weight, hipStream_t st) {
+  const int nvox = S * S * S;
+  hipLaunchKernelGGL(k_insert_gather, dim3((nvox + 255) / 256), dim3(256), 0, st, Sc, n_mp, ins_col, n_ins, wT,
+                     engine().d_Tf.as<float>(), off, ent, ent_wt, nvox, (int)(interp != INTERP_NEAREST), model, weight);
+  EMC_LAUNCHED();
+}
+
+// ---------------------------------------------------------------------------
+// Normalisation (emc_cuda.cu:414-433): model = weight > 0 ? model/weight : 0,
+// then model *= 1 / (sum(model) / n).  The sum is FP64 in a fixed order.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_norm_divide(float* __restrict__ model, const float* __restrict__ weight, int n,
+                                                     double* __restrict__ red) {
+  double s = 0.0;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+    const float w = weight[k];
+    const float m = w > 0.0f ? model[k] / w : 0.0f;
+    model[k] = m;
+    s += m;
+  }
+  s = block_sum_f64(s);
+  if (threadIdx.x == 0) red[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_norm_factor(const double* __restrict__ red, int nb, int n, float* __restrict__ fac) {
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 256) s += red[k];
+  s = block_sum_f64(s);
+  if (threadIdx.x == 0) {
+    float msum = (float)s;
+    msum /= (float)n;
+    fac[0] = 1.0f / msum;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_norm_scale(float* __restrict__ model, int n, const float* __restrict__ fac) {
+  const float f = fac[0];
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) model[k] *= f;
+}
+
+// red must hold >= 1024 + 2 doubles.
+void launch_normalize_model(float* model, const float* weight, int n, double* red, hipStream_t st) {
+  const int nb = std::min(1024, (n + 255) / 256);
+  float* fac = reinterpret_cast<float*>(red + 1024);
+  hipLaunchKernelGGL(k_norm_divide, dim3(nb), dim3(256), 0, st, model, weight, n, red);
+  EMC_LAUNCHED();
+  hipLaunchKernelGGL(k_norm_factor, dim3(1), dim3(256), 0, st, red, nb, n, fac);
+  EMC_LAUNCHED();
+  hipLaunchKernelGGL(k_norm_scale, dim3(nb), dim3(256), 0, st, model, n, fac);
+  EMC_LAUNCHED();
+}
+
+// max(0, max_v model[v]) (emc_cuda.cu:364-368).  out[0] must be 0 on entry
+// (as a bit pattern); non-negative floats order like their bit patterns.
+__global__ __launch_bounds__(256) void k_model_max(const float* __restrict__ model, int n, unsigned int* __restrict__ out) {
+  float m = 0.0f;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) m = fmaxf(m, model[k]);
+  m = block_max_f32(m);
+  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(m));
+}
+
+void launch_model_max(const float* model, int n, float* out, hipStream_t st) {
+  const int nb = std::min(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(k_model_max, dim3(nb), dim3(256), 0, st, model, n, reinterpret_cast<unsigned int*>(out));
+  EMC_LAUNCHED();
+}
+
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) p[k] = v;
+}
+
+void launch_fill(float* p, size_t n, float v, hipStream_t st) {
+  if (!n) return;
+  const unsigned nb = (unsigned)std::min<size_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, st, p, n, v);
+  EMC_LAUNCHED();
+}
+
+}  // namespace emc

@@ -1,0 +1,88 @@
+/*
+ * emc_hip.h -- additions to the EMC C-ABI that the reference does not have.
+ *
+ * None of these has a counterpart in FilipeMaia/emc's emc.h; they add what a
+ * production caller needs around the reference entry points (error codes,
+ * freeing, synchronisation, option selection, phase timing) without changing
+ * the signatures of those entry points.
+ *
+ * Threading: the library keeps one HIP stream on the current device and is
+ * not thread safe.  All entry points of emc.h and this header enqueue work on
+ * that stream.
+ */
+#ifndef EMC_HIP_H
+#define EMC_HIP_H
+
+#include <stddef.h>
+
+#include "emc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status of the last failed call: returns 0 if no error has occurred since
+ * the last emc_hip_clear_error(), else a non-zero code, and copies the
+ * message into buf (n bytes, may be NULL).  Errors are sticky: after one,
+ * entry points return without doing work (and print the message once to
+ * stderr) until emc_hip_clear_error() is called.  With the environment
+ * variable EMC_HIP_ABORT_ON_ERROR=1 the process aborts at the first error. */
+int emc_hip_last_error(char *buf, size_t n);
+void emc_hip_clear_error(void);
+
+/* Free a device buffer returned by any cuda_allocate_* function. */
+void emc_hip_free(void *d_ptr);
+/* Wait for all work enqueued by the library. */
+void emc_hip_synchronize(void);
+/* Release every internal buffer and the stream (optional, at exit). */
+void emc_hip_shutdown(void);
+
+/* Device buffer of `bytes` bytes (the size_t form of cuda_allocate_real). */
+void emc_hip_allocate_bytes(void **d_ptr, size_t bytes);
+/* Copies between host and device on the library stream; they return after
+ * the copy is complete.  Return 0 on success. */
+int emc_hip_memcpy_to_host(void *dst, const void *d_src, size_t bytes);
+int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
+
+/* Options (key, value), all strings; returns 0 if accepted.
+ *   "precision"  "fp32" (default): the reference's method, FP32 arithmetic
+ *                 on FP32 matrix cores;
+ *                "fp16": the pattern x slice and M-step contractions on FP16
+ *                 matrix cores with hi/lo operand splitting and FP32
+ *                 accumulation, FP64 normalisations.
+ *   "insert"     "atomic" (default): float atomics, order not fixed;
+ *                "deterministic": per-voxel gather in a fixed order,
+ *                 bit-reproducible run to run.
+ *   "interp"     "nearest" (default, the reference's rule) or "trilinear".
+ *   "timing"     "1" records per-phase GPU time (adds a synchronisation per
+ *                 entry point), "0" off.
+ *   "verbose"    "1" prints the reference's per-call diagnostics.
+ */
+int emc_hip_set_option(const char *key, const char *value);
+
+/* Invalidate every cached derivative of caller buffers (packed patterns,
+ * mask index, insertion map).  Needed only if the caller writes images, the
+ * mask, rotations or coordinates by means other than this library. */
+void emc_hip_invalidate(void);
+
+/* Per-phase GPU time accumulated while "timing" is on, in milliseconds, and
+ * the number of kernel launches per phase.  Phases:
+ *   0 expand, 1 pack (slice operands), 2 likelihood GEMM, 3 softmax,
+ *   4 scaling, 5 M-step row sums, 6 M-step GEMM, 7 insert, 8 normalise,
+ *   9 other.
+ * Returns the number of phases written (<= n). */
+int emc_hip_phase_times(double *ms, long long *launches, int n);
+void emc_hip_reset_phase_times(void);
+/* Name of phase k, or NULL. */
+const char *emc_hip_phase_name(int k);
+
+/* Information on the packed pattern operand of the last E-step:
+ * out[0] = masked pixel count n_m, out[1] = padded count, out[2] = 1 if the
+ * FP16 operand holds the counts exactly, out[3] = GEMM passes per product. */
+int emc_hip_pack_info(int *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMC_HIP_H */

@@ -1,0 +1,303 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Each entry point of emc.h is run on the GPU and compared with the oracle's
+restatement of the reference on the same inputs:
+  - expand (nearest voxel): bit-identical;
+  - likelihood, scaling, M-step, insertion, normalisation: within the FP32
+    rounding of the two evaluation orders (tolerances stated per test);
+  - responsibilities: 1e-4 absolute; model: 1e-4 relative (the north-star
+    bar), one iteration at a time from the same inputs.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import emc_amd as E
+from oracle import oracle as O
+from tests import problems
+
+pytestmark = pytest.mark.gpu
+
+P_TOL = 1e-4      # responsibilities, absolute
+M_TOL = 1e-4      # model, relative
+
+
+def options(precision="fp32", insert="atomic", interp="nearest"):
+    E.set_option("precision", precision)
+    E.set_option("insert", insert)
+    E.set_option("interp", interp)
+
+
+def rel_err(a, b, floor_frac=1e-3):
+    """max relative error over entries above floor_frac * max|b|."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    sig = np.abs(b) > floor_frac * np.abs(b).max()
+    return float(np.max(np.abs(a[sig] - b[sig]) / np.abs(b[sig]))) if sig.any() else 0.0
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    return problems.tiny()
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    return problems.config0_like(n_patterns=1000)
+
+
+@pytest.fixture(autouse=True)
+def _reset():
+    E.lib().emc_hip_clear_error()
+    options()
+    yield
+    options()
+
+
+def one_step(P, prob, sigma=0.4):
+    """GPU steps one at a time, each compared with the FP32 restatement fed
+    the GPU's own inputs of that step."""
+    out = {}
+    prob.get_slices()
+    out["slices"] = prob.slices()
+    Lh = np.zeros((prob.R, prob.N), np.float32)
+    prob.likelihood(sigma, Lh)
+    out["L"] = prob.respons()
+    out["L_host"] = Lh
+    prob.normalize_respons()
+    out["P"] = prob.respons()
+    out["total"] = prob.total_respons()
+    prob.reset_model()
+    prob.update_scaling()
+    out["scaling"] = prob.scaling()
+    out["scaling_host"] = prob.scaling_host.copy()
+    out["overall"] = prob.update_slices()
+    out["slices_new"] = prob.slices()
+    out["model_raw"] = prob.model()
+    out["weight"] = prob.weight()
+    out["model_host_raw"] = prob.model_host.copy()
+    prob.normalize_model()
+    out["model"] = prob.model()
+    return out
+
+
+def check_one_step(P, prob, out, sigma=0.4, l_rtol=2e-5, fp16=False):
+    model_in = P.model
+    S = P.side
+    sl = O.r32_expand(model_in, S, P.q, P.x, P.y, P.z)
+    assert np.array_equal(out["slices"], sl), "expand must be bit-identical"
+    sc_in = np.ones(P.N, np.float32)
+    L = O.r32_likelihood(sl, P.images, P.mask, sigma, sc_in)
+    assert np.array_equal(out["L"], out["L_host"])
+    scale = np.abs(L).max()
+    assert np.abs(out["L"] - L).max() <= l_rtol * scale, np.abs(out["L"] - L).max() / scale
+    # responsibilities: GPU softmax of GPU L vs oracle softmax of oracle L
+    Pref = O.r32_softmax(L)
+    assert np.abs(out["P"] - Pref).max() <= P_TOL
+    assert np.abs(out["P"] - O.r32_softmax(out["L"])).max() <= 2e-6
+    t = O.r32_total_respons(out["P"])
+    assert abs(out["total"] - t) <= 1e-5 * abs(t) + 1e-3
+    phi = O.r32_update_scaling(P.images, sl, P.mask, out["P"])
+    assert np.allclose(out["scaling"], phi, rtol=2e-5)
+    assert np.array_equal(out["scaling"], out["scaling_host"])
+    m = np.zeros(S ** 3, np.float32)
+    wg = np.zeros(S ** 3, np.float32)
+    ov, sl2, T = O.r32_update_slices(P.images, sl, P.mask, out["P"], out["scaling"], S, m, wg, P.x, P.y, P.z,
+                                     P.q, P.w)
+    assert abs(out["overall"] - ov) <= 1e-5 * abs(ov)
+    msk = P.mask != 0
+    assert rel_err(out["slices_new"][:, msk], sl2[:, msk]) <= (2e-4 if fp16 else 2e-5)
+    assert np.array_equal(out["slices_new"][:, ~msk], sl[:, ~msk])  # unmasked pixels untouched
+    assert rel_err(out["model_raw"], m) <= (1e-4 if fp16 else 2e-5)
+    assert np.array_equal(out["model_raw"], out["model_host_raw"])
+    assert rel_err(out["weight"], wg) <= 1e-5
+    mn = O.r32_normalize_model(out["model_raw"], out["weight"])
+    assert rel_err(out["model"], mn) <= 1e-6
+    # end to end against the oracle's own chain from the same start
+    ref_m, _, ref_s, ref_P, _ = O.r32_iteration(model_in, P.images, P.mask, sc_in, S, P.x, P.y, P.z, P.q, P.w,
+                                                sigma)
+    assert np.abs(out["P"] - ref_P).max() <= P_TOL
+    assert rel_err(out["model"], ref_m) <= M_TOL
+
+
+def test_expand_bitexact(tiny):
+    options()
+    prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+    prob.get_slices()
+    assert np.array_equal(prob.slices(), O.r32_expand(tiny.model, tiny.side, tiny.q, tiny.x, tiny.y, tiny.z))
+    prob.free()
+
+
+def test_one_step_fp32_tiny(tiny):
+    options("fp32")
+    prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+    out = one_step(tiny, prob)
+    check_one_step(tiny, prob, out)
+    prob.free()
+
+
+def test_one_step_fp32_ragged():
+    # N not a multiple of 4 (scalar load/store paths), R not a multiple of 128
+    P = problems.tiny(n_patterns=37, n_rot=2, seed=5)
+    options("fp32")
+    prob = E.EmcProblem(P.side, P.images, P.mask, P.q[:301], P.w[:301], P.model)
+    Pq = problems.Problem(P.side, P.images, P.mask, P.q[:301].copy(), P.w[:301].copy(), P.x, P.y, P.z, P.model,
+                          P.truth, P.phi_true)
+    out = one_step(Pq, prob)
+    check_one_step(Pq, prob, out)
+    prob.free()
+
+
+def test_one_step_fp16_tiny(tiny):
+    options("fp16")
+    prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+    out = one_step(tiny, prob)
+    info = (C.c_int * 4)()
+    E.lib().emc_hip_pack_info(info, 4)
+    assert info[2] == 1  # integer counts are exact in FP16
+    check_one_step(tiny, prob, out, fp16=True)
+    prob.free()
+
+
+def test_one_step_fp16_inexact_counts(tiny):
+    # non-integer patterns force the split of K into hi + lo halves
+    imgs = tiny.images * np.float32(1.37)
+    P = problems.Problem(tiny.side, imgs, tiny.mask, tiny.q, tiny.w, tiny.x, tiny.y, tiny.z, tiny.model,
+                         tiny.truth, tiny.phi_true)
+    options("fp16")
+    prob = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
+    out = one_step(P, prob)
+    info = (C.c_int * 4)()
+    E.lib().emc_hip_pack_info(info, 4)
+    assert info[2] == 0 and info[3] == 3
+    check_one_step(P, prob, out, fp16=True)
+    prob.free()
+
+
+def test_scaling_fallback_matches_fused(tiny):
+    """update_scaling recomputes sum_p K S when the likelihood's partial sums
+    are not available (the reference's method); both paths agree."""
+    options("fp32")
+    prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+    prob.get_slices()
+    prob.likelihood(0.4)
+    prob.normalize_respons()
+    P = prob.respons()
+    E.lib().emc_hip_invalidate()
+    prob.update_scaling()
+    phi_fb = prob.scaling()
+    sl = prob.slices()
+    ref = O.r32_update_scaling(tiny.images, sl, tiny.mask, P)
+    assert np.allclose(phi_fb, ref, rtol=2e-5)
+    prob.free()
+
+
+def test_deterministic_insert_bit_reproducible(tiny):
+    models = []
+    for _ in range(2):
+        options("fp32", "deterministic")
+        prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+        for _ in range(2):
+            prob.iteration()
+        models.append((prob.model(), prob.weight()))
+        prob.free()
+        E.lib().emc_hip_invalidate()
+    assert np.array_equal(models[0][0], models[1][0])
+    assert np.array_equal(models[0][1], models[1][1])
+    options("fp32", "atomic")
+    prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+    for _ in range(2):
+        prob.iteration()
+    assert rel_err(prob.model(), models[0][0]) <= 1e-4
+    prob.free()
+
+
+def test_deterministic_insert_matches_sequential_oracle(tiny):
+    """The deterministic mode adds each voxel's contributions in the oracle's
+    sequential order (rotation, then x, then y), so with the GPU's own
+    updated slices and totals the accumulators agree bit for bit."""
+    options("fp32", "deterministic")
+    prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+    prob.get_slices()
+    prob.likelihood(0.4)
+    prob.normalize_respons()
+    prob.reset_model()
+    prob.update_scaling()
+    Pm = prob.respons()
+    prob.update_slices()
+    sl2 = prob.slices()
+    m_gpu, w_gpu = prob.model(), prob.weight()
+    # the GPU's T_r is the float rounding of an FP64 sum of P over patterns
+    T = Pm.astype(np.float64).sum(axis=1).astype(np.float32)
+    S = tiny.side
+    m = np.zeros(S ** 3, np.float32)
+    wg = np.zeros(S ** 3, np.float32)
+    lib = O.lib()
+    for r in range(tiny.R):
+        if T[r] > 1e-10:
+            lib.r32_insert_slice(O.f(m), O.f(wg), S, O.f(np.ascontiguousarray(sl2[r])), O.i(tiny.mask),
+                                 float(np.float32(tiny.w[r]) * T[r]), O.f(np.ascontiguousarray(tiny.q[r])),
+                                 O.f(tiny.x), O.f(tiny.y), O.f(tiny.z))
+    agree = np.mean(m_gpu == m)
+    assert agree > 0.999, agree  # T may differ by one ulp for a handful of rotations
+    assert rel_err(m_gpu, m) <= 1e-6
+    assert rel_err(w_gpu, wg) <= 1e-6
+    prob.free()
+
+
+def test_trilinear_one_step_vs_fp64(tiny):
+    options("fp32", "atomic", "trilinear")
+    prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+    prob.iteration()
+    ref = O.o64_iteration(tiny.model.astype(np.float64), tiny.images, tiny.mask, np.ones(tiny.N), tiny.side,
+                          tiny.q, tiny.w, tiny.x, tiny.y, tiny.z, 0.4, O.TRILINEAR)
+    assert np.abs(prob.respons() - ref["P"]).max() <= P_TOL
+    assert rel_err(prob.model(), ref["model"]) <= M_TOL
+    prob.free()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_config0_one_step(cfg0, precision):
+    """Configuration 0's shapes (32^3, 1,000 patterns, 3,240 rotations)."""
+    options(precision)
+    prob = E.EmcProblem(cfg0.side, cfg0.images, cfg0.mask, cfg0.q, cfg0.w, cfg0.model)
+    prob.iteration()
+    ref_m, _, ref_s, ref_P, _ = O.r32_iteration(cfg0.model, cfg0.images, cfg0.mask, np.ones(cfg0.N, np.float32),
+                                                cfg0.side, cfg0.x, cfg0.y, cfg0.z, cfg0.q, cfg0.w, 0.4)
+    dp = np.abs(prob.respons() - ref_P).max()
+    dm = rel_err(prob.model(), ref_m)
+    print(f"config0 {precision}: max|dP| = {dp:.3g}, model rel = {dm:.3g}")
+    assert dp <= P_TOL
+    assert dm <= M_TOL
+    assert np.allclose(prob.scaling(), ref_s, rtol=1e-4)
+    prob.free()
+
+
+def test_multi_iteration_tracks_oracle(tiny):
+    """Three iterations; each GPU iteration is compared with the oracle's
+    iteration started from the GPU's model and scaling of the step before."""
+    options("fp32")
+    prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+    model = tiny.model.copy()
+    sc = np.ones(tiny.N, np.float32)
+    for it in range(3):
+        prob.iteration()
+        ref_m, _, ref_s, ref_P, _ = O.r32_iteration(model, tiny.images, tiny.mask, sc, tiny.side, tiny.x, tiny.y,
+                                                    tiny.z, tiny.q, tiny.w, 0.4)
+        assert np.abs(prob.respons() - ref_P).max() <= P_TOL, it
+        assert rel_err(prob.model(), ref_m) <= M_TOL, it
+        model = prob.model()
+        sc = prob.scaling()
+    prob.free()
+
+
+def test_error_reporting_invalid_arguments():
+    L = E.lib()
+    L.emc_hip_clear_error()
+    L.cuda_normalize_responsabilities(None, 10, 10)
+    buf = C.create_string_buffer(256)
+    assert L.emc_hip_last_error(buf, 256) != 0
+    assert b"invalid argument" in buf.value
+    L.emc_hip_clear_error()
+    assert L.emc_hip_last_error(buf, 256) == 0
