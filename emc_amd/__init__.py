@@ -96,6 +96,7 @@ _SIGS = [
     ("emc_hip_free", None, [C.c_void_p]),
     ("emc_hip_synchronize", None, []),
     ("emc_hip_shutdown", None, []),
+    ("emc_hip_total_respons_n", C.c_float, [C.c_void_p, C.c_size_t]),
     ("emc_hip_allocate_bytes", None, [c_void_pp, C.c_size_t]),
     ("emc_hip_memcpy_to_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("emc_hip_memcpy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -321,7 +322,11 @@ class EmcProblem:
         check()
 
     def total_respons(self) -> float:
-        v = lib().cuda_total_respons(self.d_respons.vp, None, self.N * self.R)
+        n = self.N * self.R
+        if n < 2 ** 31:
+            v = lib().cuda_total_respons(self.d_respons.vp, None, n)
+        else:  # the reference's int argument cannot hold it
+            v = lib().emc_hip_total_respons_n(self.d_respons.vp, n)
         check()
         return v
 

@@ -381,13 +381,12 @@ void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images
   }
 }
 
-real cuda_total_respons(real* d_respons, real* respons, int n) {
-  (void)respons;
+real emc_hip_total_respons_n(const real* d_respons, size_t n) {
   if (!ready()) return 0.0f;
   Engine& e = engine();
   auto& lk = e.like;
-  if (lk.have_total && lk.respons == d_respons && (size_t)n == (size_t)lk.N * lk.R) return (real)lk.total;
-  if (!d_respons || n < 0) {
+  if (lk.have_total && lk.respons == d_respons && n == (size_t)lk.N * lk.R) return (real)lk.total;
+  if (!d_respons) {
     set_error("cuda_total_respons: invalid argument");
     return 0.0f;
   }
@@ -395,11 +394,27 @@ real cuda_total_respons(real* d_respons, real* respons, int n) {
   double total = 0.0;
   {
     PhaseTimer pt(PH_OTHER);
-    launch_total_respons(d_respons, (size_t)n, e.d_red.as<double>(), e.stream);
+    launch_total_respons(d_respons, n, e.d_red.as<double>(), e.stream);
   }
   EMC_CHECK(hipMemcpyAsync(&total, e.d_red.as<double>() + 1024, sizeof(double), hipMemcpyDeviceToHost, e.stream));
   sync();
   return (real)total;
+}
+
+real cuda_total_respons(real* d_respons, real* respons, int n) {
+  (void)respons;
+  if (!ready()) return 0.0f;
+  auto& lk = engine().like;
+  // The reference passes N_images * N_slices through this int; beyond 2^31
+  // the caller's value has wrapped.  If it agrees modulo 2^32 with the shape
+  // of the responsibilities this buffer holds, it means that whole matrix.
+  const size_t full = (size_t)lk.N * lk.R;
+  if (lk.respons == d_respons && (uint32_t)n == (uint32_t)full && full > 0) return emc_hip_total_respons_n(d_respons, full);
+  if (n < 0) {
+    set_error("cuda_total_respons: invalid argument (n = %d; use emc_hip_total_respons_n for sizes >= 2^31)", n);
+    return 0.0f;
+  }
+  return emc_hip_total_respons_n(d_respons, (size_t)n);
 }
 
 void cuda_update_scaling(real* d_images, real* slices, int* d_mask, real* d_respons, real* d_scaling, int N_images,

@@ -37,6 +37,11 @@ void emc_hip_synchronize(void);
 /* Release every internal buffer and the stream (optional, at exit). */
 void emc_hip_shutdown(void);
 
+/* sum over n entries of P log P: the size_t form of cuda_total_respons, for
+ * N_images * N_slices >= 2^31 (cuda_total_respons also accepts the wrapped
+ * int the reference's driver passes when it matches the buffer's shape). */
+real emc_hip_total_respons_n(const real *d_respons, size_t n);
+
 /* Device buffer of `bytes` bytes (the size_t form of cuda_allocate_real). */
 void emc_hip_allocate_bytes(void **d_ptr, size_t bytes);
 /* Copies between host and device on the library stream; they return after

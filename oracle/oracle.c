@@ -504,12 +504,10 @@ void r32_insert_slice(float *model, float *weight, int S, const float *slice, co
     }
 }
 
-/* emc_atomic.cu:76-133 (M-step then insert).  Float atomics in the reference
- * have no fixed order; here contributions are added in rotation order. */
-float r32_update_slices(const float *images, float *slices, const int *mask, const float *respons,
-                        const float *scaling, int N, int R, int N2d, int S, float *model,
-                        float *weight, const float *x, const float *y, const float *z,
-                        const float *q, const float *weights, float *T) {
+/* emc_atomic.cu:76-113: the M-step alone (slices overwritten at masked
+ * pixels, T written). */
+void r32_mstep(const float *images, float *slices, const int *mask, const float *respons,
+               const float *scaling, int N, int R, int N2d, float *T) {
 #pragma omp parallel for schedule(dynamic, 1)
   for (int r = 0; r < R; r++) {
     const float *pr = &respons[(size_t)r * N];
@@ -527,9 +525,53 @@ float r32_update_slices(const float *images, float *slices, const int *mask, con
       for (int p = 0; p < N2d; p++)
         if (mask[p] != 0) sl[p] /= tot;
   }
+}
+
+/* emc_atomic.cu:115-133: insert every slice with T_r > 1e-10, weight
+ * weights[r] * T_r.  Float atomics in the reference have no fixed order;
+ * here contributions are added in rotation order, sequentially. */
+void r32_insert_all(float *model, float *weight, int S, const float *slices, const int *mask,
+                    const float *T, const float *q, const float *weights, int R, const float *x,
+                    const float *y, const float *z) {
+  const size_t N2d = (size_t)S * S;
   for (int r = 0; r < R; r++)
     if (T[r] > 1e-10f)
       r32_insert_slice(model, weight, S, &slices[(size_t)r * N2d], mask, weights[r] * T[r], &q[4 * r], x, y, z);
+}
+
+/* The same insertion spread over host threads (rotations split between
+ * threads, thread-private accumulators summed at the end): used to time the
+ * reference's method on all host cores.  Summation order differs from
+ * r32_insert_all, as the reference's atomics' order does. */
+void r32_insert_all_par(float *model, float *weight, int S, const float *slices, const int *mask,
+                        const float *T, const float *q, const float *weights, int R, const float *x,
+                        const float *y, const float *z) {
+  const size_t N2d = (size_t)S * S, n3 = (size_t)S * S * S;
+#pragma omp parallel
+  {
+    float *m = (float *)calloc(n3, sizeof(float));
+    float *w = (float *)calloc(n3, sizeof(float));
+#pragma omp for schedule(dynamic, 16)
+    for (int r = 0; r < R; r++)
+      if (T[r] > 1e-10f)
+        r32_insert_slice(m, w, S, &slices[(size_t)r * N2d], mask, weights[r] * T[r], &q[4 * r], x, y, z);
+#pragma omp critical
+    for (size_t k = 0; k < n3; k++) {
+      model[k] += m[k];
+      weight[k] += w[k];
+    }
+    free(m);
+    free(w);
+  }
+}
+
+/* emc_atomic.cu:76-133 and emc_cuda.cu:348-351 (M-step, insert, sum T). */
+float r32_update_slices(const float *images, float *slices, const int *mask, const float *respons,
+                        const float *scaling, int N, int R, int N2d, int S, float *model,
+                        float *weight, const float *x, const float *y, const float *z,
+                        const float *q, const float *weights, float *T) {
+  r32_mstep(images, slices, mask, respons, scaling, N, R, N2d, T);
+  r32_insert_all(model, weight, S, slices, mask, T, q, weights, R, x, y, z);
   float overall = 0.0f; /* emc_cuda.cu:348-351 */
   for (int r = 0; r < R; r++) overall += T[r];
   return overall;

@@ -58,6 +58,14 @@ void r32_softmax(float *respons, int R, int N);
 double r32_total_respons(const float *respons, size_t n);
 void r32_update_scaling(const float *images, const float *slices, const int *mask,
                         const float *respons, float *scaling, int N, int R, int N2d);
+void r32_mstep(const float *images, float *slices, const int *mask, const float *respons,
+               const float *scaling, int N, int R, int N2d, float *T);
+void r32_insert_all(float *model, float *weight, int S, const float *slices, const int *mask,
+                    const float *T, const float *q, const float *weights, int R, const float *x,
+                    const float *y, const float *z);
+void r32_insert_all_par(float *model, float *weight, int S, const float *slices, const int *mask,
+                        const float *T, const float *q, const float *weights, int R, const float *x,
+                        const float *y, const float *z);
 /* M-step (slices overwritten at masked pixels, T[r] written), then insert.
  * Returns sum_r T_r.  Insert order: r, then x outer, y inner, sequential. */
 float r32_update_slices(const float *images, float *slices, const int *mask, const float *respons,

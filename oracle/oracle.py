@@ -35,6 +35,9 @@ _SIGS = [
     ("r32_update_scaling", None, [_f, _f, _i, _f, _f, C.c_int, C.c_int, C.c_int]),
     ("r32_update_slices", C.c_float, [_f, _f, _i, _f, _f, C.c_int, C.c_int, C.c_int, C.c_int, _f, _f, _f, _f,
                                       _f, _f, _f, _f]),
+    ("r32_mstep", None, [_f, _f, _i, _f, _f, C.c_int, C.c_int, C.c_int, _f]),
+    ("r32_insert_all", None, [_f, _f, C.c_int, _f, _i, _f, _f, _f, C.c_int, _f, _f, _f]),
+    ("r32_insert_all_par", None, [_f, _f, C.c_int, _f, _i, _f, _f, _f, C.c_int, _f, _f, _f]),
     ("r32_normalize_model", None, [_f, _f, C.c_int]),
     ("r32_model_max", C.c_float, [_f, C.c_int]),
     ("r32_insert_slice", None, [_f, _f, C.c_int, _f, _i, C.c_float, _f, _f, _f, _f]),
@@ -188,6 +191,51 @@ def r32_iteration(model, images, mask, scaling, S, x, y, z, q, weights, sigma=0.
     tot = lib().r32_iteration(f(m), f(wgt), f(sl), f(c32(images)), i(ci(mask)), f(sc), f(P), N, R, S, f(x), f(y),
                               f(z), f(c32(q)), f(c32(weights)), sigma, nthreads)
     return m, wgt, sc, P, tot
+
+
+def time_reference_iteration(model, images, mask, S, x, y, z, q, weights, n_full, sigma=0.4):
+    """Wall time of one FP32-restatement iteration (emc.c:989-1084 order) on
+    the pattern sample `images`, phase by phase, and its extrapolation to
+    n_full patterns: the per-pattern phases (likelihood, softmax, sum P log P,
+    scaling, M-step) scale with the pattern count, expand, insertion and
+    normalisation do not."""
+    import time
+
+    L = lib()
+    R = q.shape[0]
+    Ns = images.shape[0]
+    N2d = S * S
+    images = c32(images)
+    q = c32(q)
+    weights = c32(weights)
+    mask = ci(mask)
+    sl = np.zeros((R, N2d), np.float32)
+    P = np.zeros((R, Ns), np.float32)
+    sc = np.ones(Ns, np.float32)
+    m = c32(model).copy()
+    wg = np.zeros_like(m)
+    T = np.zeros(R, np.float32)
+    t = {}
+
+    def run(name, fn):
+        t0 = time.perf_counter()
+        fn()
+        t[name] = time.perf_counter() - t0
+
+    run("expand", lambda: L.r32_expand(f(m), S, f(q), R, f(x), f(y), f(z), f(sl)))
+    run("likelihood", lambda: L.r32_likelihood(f(sl), f(images), i(mask), sigma, f(sc), f(P), N2d, Ns, R))
+    run("softmax", lambda: L.r32_softmax(f(P), R, Ns))
+    run("total", lambda: L.r32_total_respons(f(P), P.size))
+    m[:] = 0
+    run("scaling", lambda: L.r32_update_scaling(f(images), f(sl), i(mask), f(P), f(sc), Ns, R, N2d))
+    run("mstep", lambda: L.r32_mstep(f(images), f(sl), i(mask), f(P), f(sc), Ns, R, N2d, f(T)))
+    run("insert", lambda: L.r32_insert_all_par(f(m), f(wg), S, f(sl), i(mask), f(T), f(q), f(weights), R, f(x), f(y),
+                                           f(z)))
+    run("normalize", lambda: L.r32_normalize_model(f(m), f(wg), m.size))
+    dep = t["likelihood"] + t["softmax"] + t["total"] + t["scaling"] + t["mstep"]
+    indep = t["expand"] + t["insert"] + t["normalize"]
+    return {"phases": t, "sample_s": dep + indep, "full_s": indep + dep * n_full / Ns,
+            "threads": L.o_max_threads()}
 
 
 # ---- FP64 oracle ----
