@@ -167,7 +167,7 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
   e.n_m = (int)pix.size();
   e.n_mp = (int)round_up(std::max(e.n_m, 1), kPadK);
   e.n_ins = (int)ins.size();
-  e.N_pad = (int)round_up(N, kTileM);
+  e.N_pad = (int)round_up(N, kPadN);
   std::vector<int> pixp(e.n_mp, -1), inscol(std::max(e.n_ins, 1));
   std::copy(pix.begin(), pix.end(), pixp.begin());
   for (int t = 0; t < e.n_ins; t++) inscol[t] = col_of[ins[t]];

@@ -45,9 +45,12 @@ enum InsertMode { INSERT_ATOMIC = 0, INSERT_DETERMINISTIC = 1 };
 enum Interp { INTERP_NEAREST = 0, INTERP_TRILINEAR = 1 };
 
 // Tile sizes shared by the GEMM kernels and the padding of packed operands.
-constexpr int kTileM = 128;
+constexpr int kTileM = 128;    // FP32 kernels: 128 x 128
 constexpr int kTileN = 128;
-constexpr int kPadK = 128;  // pixel-dimension padding: multiple of every BK and of kTileN
+constexpr int kTileM16 = 256;  // FP16 E-step: 256 patterns x 128 rotations
+constexpr int kPadK = 256;     // pixel padding: multiple of every BK, of kTileN and of
+                               // the FP16 M-step's 256-pixel tile
+constexpr int kPadN = 256;     // pattern padding: multiple of every pattern tile
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
@@ -212,6 +215,10 @@ struct MstepArgs {
 };
 void launch_mstep(int precision, bool ksplit, const void* Kt, const void* Ktlo,
                   const MstepArgs& a, hipStream_t st);
+// FP16 hi/lo forms (k_gemm16.hip)
+void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, const void* Slo, int N_pad,
+                    int R_pad, int n_mp, const LikeEpilogue& ep, hipStream_t st);
+void launch_mstep16(bool ksplit, const void* Kt, const void* Ktlo, const MstepArgs& a, hipStream_t st);
 
 // softmax / totals / scaling (k_softmax.hip)
 int softmax_chunks(int N, int R);

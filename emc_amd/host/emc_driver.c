@@ -1,0 +1,266 @@
+/*
+ * emc -- the EMC reconstruction driver (emc.c:712-1088 in the reference).
+ *
+ * Same structure as the reference's main(): build the rotation table and the
+ * detector coordinates, load the patterns and the mask, initialise the
+ * model, allocate device buffers through the cuda_allocate_* entry points,
+ * then iterate expand -> likelihood -> responsibilities -> scaling ->
+ * M-step/insertion -> normalisation through the emc.h C-ABI, writing
+ * likelihood.data (sum P log P per iteration), scale_error.data (RMS error of
+ * the mean-normalised scaling factors against the true ones) and the model.
+ *
+ * Differences from the reference, all additive:
+ *   - the configuration comes from the command line instead of compile-time
+ *     constants (emc.c:714-736,753,766-778);
+ *   - patterns are synthetic (the reference reads Data/image_%04d.h5 through
+ *     libspimage, which is not available); --config selects the workload;
+ *   - the host copy of the raw log-likelihoods (N*R floats) is made only
+ *     with --dump-respons;
+ *   - models are written as raw float32 volumes (model_%04d.raw).
+ */
+#include <errno.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include "emc.h"
+#include "emc_hip.h"
+#include "emc_host.h"
+
+typedef struct {
+  int side, n_blobs, n_patterns, n_rot, iterations, gap_h, gap_cross;
+  float blob_radius, phi_sigma, photons, beamstop;
+} preset;
+
+/* The five benchmark workloads (synthetic). */
+static const preset PRESETS[5] = {
+    {32, 20, 1000, 4, 10, 0, 0, 6.0f, 0.2f, 2000.0f, 3.0f},
+    {64, 40, 10000, 8, 20, 2, 0, 12.0f, 0.3f, 5000.0f, 4.0f},
+    {128, 80, 50000, 10, 20, 0, 3, 24.0f, 0.3f, 10000.0f, 6.0f},
+    {128, 80, 50000, 10, 20, 0, 3, 24.0f, 0.3f, 10000.0f, 6.0f},
+    {256, 150, 20000, 8, 30, 0, 8, 48.0f, 0.5f, 300.0f, 12.0f},
+};
+
+static double now_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec * 1e3 + ts.tv_nsec / 1e6;
+}
+
+static void die_if_error(const char *where) {
+  char buf[512];
+  if (emc_hip_last_error(buf, sizeof buf)) {
+    fprintf(stderr, "emc: %s: %s\n", where, buf);
+    exit(1);
+  }
+}
+
+static void usage(void) {
+  fprintf(stderr,
+          "usage: emc [--config K] [--patterns N] [--iterations I] [--sigma S]\n"
+          "           [--precision fp32|fp16] [--insert atomic|deterministic]\n"
+          "           [--interp nearest|trilinear] [--sampling n] [--full-table]\n"
+          "           [--model random|truth] [--seed S] [--out DIR] [--dump-respons]\n"
+          "           [--save-every K] [--quiet]\n");
+}
+
+static void write_raw(const char *dir, const char *name, const float *v, size_t n) {
+  char path[1024];
+  snprintf(path, sizeof path, "%s/%s", dir, name);
+  FILE *f = fopen(path, "wb");
+  if (!f) {
+    fprintf(stderr, "emc: cannot write %s: %s\n", path, strerror(errno));
+    return;
+  }
+  fwrite(v, sizeof(float), n, f);
+  fclose(f);
+}
+
+int main(int argc, char **argv) {
+  int cfg = 0, N_images = -1, iterations = -1, n_rot = -1, full_table = 0, dump = 0, quiet = 0;
+  int save_every = 0, model_truth = 0;
+  float sigma = 0.4f; /* emc.c:734-735 */
+  uint64_t seed = 0;
+  const char *out = "output", *precision = "fp32", *insert = "atomic", *interp = "nearest";
+  for (int a = 1; a < argc; a++) {
+    const char *k = argv[a];
+    const char *v = a + 1 < argc ? argv[a + 1] : NULL;
+#define TAKE(var, conv) do { if (!v) { usage(); return 2; } var = conv(v); a++; } while (0)
+    if (!strcmp(k, "--config")) TAKE(cfg, atoi);
+    else if (!strcmp(k, "--patterns")) TAKE(N_images, atoi);
+    else if (!strcmp(k, "--iterations")) TAKE(iterations, atoi);
+    else if (!strcmp(k, "--sampling")) TAKE(n_rot, atoi);
+    else if (!strcmp(k, "--sigma")) TAKE(sigma, (float)atof);
+    else if (!strcmp(k, "--seed")) TAKE(seed, (uint64_t)atoll);
+    else if (!strcmp(k, "--save-every")) TAKE(save_every, atoi);
+    else if (!strcmp(k, "--precision")) { if (!v) { usage(); return 2; } precision = v; a++; }
+    else if (!strcmp(k, "--insert")) { if (!v) { usage(); return 2; } insert = v; a++; }
+    else if (!strcmp(k, "--interp")) { if (!v) { usage(); return 2; } interp = v; a++; }
+    else if (!strcmp(k, "--out")) { if (!v) { usage(); return 2; } out = v; a++; }
+    else if (!strcmp(k, "--model")) { if (!v) { usage(); return 2; } model_truth = !strcmp(v, "truth"); a++; }
+    else if (!strcmp(k, "--full-table")) full_table = 1;
+    else if (!strcmp(k, "--dump-respons")) dump = 1;
+    else if (!strcmp(k, "--quiet")) quiet = 1;
+    else { usage(); return 2; }
+#undef TAKE
+  }
+  if (cfg < 0 || cfg > 4) { usage(); return 2; }
+  const preset P = PRESETS[cfg];
+  if (N_images <= 0) N_images = P.n_patterns;
+  if (iterations < 0) iterations = P.iterations;
+  if (n_rot <= 0) n_rot = P.n_rot;
+  if (emc_hip_set_option("precision", precision) || emc_hip_set_option("insert", insert) ||
+      emc_hip_set_option("interp", interp)) {
+    fprintf(stderr, "emc: bad option value\n");
+    return 2;
+  }
+  if (!quiet) emc_hip_set_option("verbose", "0");
+  mkdir(out, 0755);
+
+  /* rotations (emc.c:753-757) */
+  const int nfull = emc_n_to_samples(n_rot);
+  float *qbuf = malloc(sizeof(float) * 4 * (size_t)nfull);
+  real *weights = malloc(sizeof(real) * (size_t)nfull);
+  const int N_slices = emc_rotation_table(n_rot, full_table ? 0 : 1, qbuf, weights);
+  if (N_slices <= 0) { fprintf(stderr, "emc: rotation table failed\n"); return 1; }
+  Quaternion **rotations = malloc(sizeof(Quaternion *) * (size_t)N_slices);
+  for (int r = 0; r < N_slices; r++) {
+    rotations[r] = malloc(sizeof(Quaternion));
+    memcpy(rotations[r]->q, &qbuf[4 * (size_t)r], sizeof(float) * 4);
+  }
+  printf("%d rotations sampled (n = %d%s)\n", N_slices, n_rot, full_table ? "" : ", antipodal half");
+
+  /* setup (emc.c:766-771), coordinates (emc.c:872-875) */
+  Setup setup = {P.side, 5.6e-9f, 16e-6f, 512, 0.15f};
+  const int side = setup.side;
+  const int N_2d = side * side;
+  const size_t N_model = (size_t)side * side * side;
+  sp_matrix xc = {side, side, malloc(sizeof(real) * N_2d)}, yc = {side, side, malloc(sizeof(real) * N_2d)},
+            zc = {side, side, malloc(sizeof(real) * N_2d)};
+  calculate_coordinates(setup, &xc, &yc, &zc);
+
+  /* mask and patterns */
+  sp_imatrix mask = {side, side, malloc(sizeof(int) * N_2d)};
+  const int n_m = emc_make_mask(side, P.beamstop, P.gap_h, P.gap_cross, 0, mask.data);
+  emc_synth_params sp = {side, P.n_blobs, P.blob_radius, 1.0f, 2.0f, N_images, P.phi_sigma, P.photons, seed};
+  float *truth = malloc(sizeof(float) * N_model);
+  float *image_data = malloc(sizeof(float) * (size_t)N_images * N_2d);
+  float *intensities = malloc(sizeof(float) * N_images);
+  double t0 = now_ms();
+  emc_synth_intensity(&sp, truth);
+  emc_synth_patterns(&sp, truth, xc.data, yc.data, zc.data, mask.data, image_data, NULL, intensities);
+  printf("%d patterns of %dx%d, %d masked pixels (synthetic data in %.1f s)\n", N_images, side, side, n_m,
+         (now_ms() - t0) / 1e3);
+  sp_matrix *image_mats = malloc(sizeof(sp_matrix) * (size_t)N_images);
+  sp_matrix **images = malloc(sizeof(sp_matrix *) * (size_t)N_images);
+  for (int i = 0; i < N_images; i++) {
+    image_mats[i].rows = side;
+    image_mats[i].cols = side;
+    image_mats[i].data = image_data + (size_t)i * N_2d;
+    images[i] = &image_mats[i];
+  }
+
+  /* model init (emc.c:888-928): uniform random, or the true intensity */
+  sp_3matrix model = {side, side, side, malloc(sizeof(real) * N_model)};
+  sp_3matrix weight = {side, side, side, calloc(N_model, sizeof(real))};
+  if (model_truth) memcpy(model.data, truth, sizeof(float) * N_model);
+  else emc_synth_random_model(side, seed + 12345, model.data);
+  write_raw(out, "model_init.raw", model.data, N_model);
+  real *scaling = malloc(sizeof(real) * N_images);
+  for (int i = 0; i < N_images; i++) scaling[i] = 1.0f;
+  real *respons = dump ? malloc(sizeof(real) * (size_t)N_slices * N_images) : NULL;
+
+  char path[1024];
+  snprintf(path, sizeof path, "%s/scale_error.data", out);
+  FILE *scale = fopen(path, "w");
+  snprintf(path, sizeof path, "%s/likelihood.data", out);
+  FILE *likelihood = fopen(path, "w");
+
+  /* device buffers (emc.c:961-985) */
+  real *slices = NULL, *d_model = NULL, *d_weight = NULL, *d_rotations = NULL, *d_x = NULL, *d_y = NULL,
+       *d_z = NULL, *d_images = NULL, *d_respons = NULL, *d_scaling = NULL;
+  int *d_mask = NULL;
+  cuda_allocate_slices(&slices, setup, N_slices);
+  cuda_allocate_model(&d_model, &model);
+  cuda_allocate_model(&d_weight, &weight);
+  cuda_allocate_mask(&d_mask, &mask);
+  cuda_allocate_rotations(&d_rotations, rotations, N_slices);
+  cuda_allocate_coords(&d_x, &d_y, &d_z, &xc, &yc, &zc);
+  cuda_allocate_images(&d_images, images, N_images);
+  /* N_slices*N_images*sizeof(real) bytes: the reference's int argument
+   * (emc.c:983) cannot hold it beyond 2 GiB, so use the size_t form */
+  emc_hip_allocate_bytes((void **)&d_respons, sizeof(real) * (size_t)N_slices * N_images);
+  cuda_allocate_scaling(&d_scaling, N_images);
+  die_if_error("allocation");
+
+  double t_total = 0;
+  for (int iteration = 0; iteration < iterations; iteration++) {
+    const double t_it = now_ms();
+    real mx = cuda_model_max(d_model, (int)N_model);
+    if (!quiet) printf("model max = %g\niteration %d\n", mx, iteration);
+    cuda_get_slices(&model, d_model, slices, d_rotations, d_x, d_y, d_z, N_slices);
+    double t_i = now_ms();
+    cuda_calculate_responsabilities(slices, d_images, d_mask, sigma, d_scaling, d_respons, N_2d, N_images,
+                                    N_slices, respons);
+    cuda_normalize_responsabilities(d_respons, N_slices, N_images);
+    double t_e = now_ms();
+    if (!quiet) printf("Expansion time = %fms\n", t_e - t_i);
+    t_i = now_ms();
+    const size_t nr = (size_t)N_images * N_slices;
+    real total_respons = nr < 2147483648ull ? cuda_total_respons(d_respons, respons, (int)nr)
+                                            : emc_hip_total_respons_n(d_respons, nr);
+    fprintf(likelihood, "%g\n", total_respons);
+    fflush(likelihood);
+    if (!quiet) printf("likelihood = %g\n", total_respons);
+    cuda_reset_model(&model, d_model);
+    cuda_reset_model(&weight, d_weight);
+    cuda_update_scaling(d_images, slices, d_mask, d_respons, d_scaling, N_images, N_slices, N_2d, scaling);
+    double scale_sum = 0.0;
+    for (int i = 0; i < N_images; i++) scale_sum += scaling[i];
+    scale_sum /= N_images;
+    double err = 0.0;
+    for (int i = 0; i < N_images; i++) err += pow(scaling[i] / scale_sum - intensities[i], 2);
+    err = sqrt(err / N_images);
+    if (!quiet) printf("scaling error = %g\n", err);
+    fprintf(scale, "%g\n", err);
+    fflush(scale);
+    real overall = cuda_update_slices(d_images, slices, d_mask, d_respons, d_scaling, N_images, N_slices, N_2d,
+                                      &model, d_model, d_x, d_y, d_z, d_rotations, weights, d_weight, setup, images);
+    t_e = now_ms();
+    if (!quiet) printf("Maximize time = %fms (total respons %g)\n", t_e - t_i, overall);
+    t_i = now_ms();
+    cuda_normalize_model(&model, d_model, d_weight);
+    t_e = now_ms();
+    if (!quiet) printf("Compression time = %fms\n", t_e - t_i);
+    die_if_error("iteration");
+    const double dt = now_ms() - t_it;
+    t_total += dt;
+    printf("iteration %d: %.1f ms, %.3g pattern x rotation pairs/s, likelihood %g, scaling error %g\n", iteration,
+           dt, (double)N_images * N_slices / (dt / 1e3), total_respons, err);
+    if (save_every > 0 && (iteration % save_every == 0 || iteration == iterations - 1)) {
+      /* the host model holds the accumulators before normalisation
+       * (emc_cuda.cu:352 copies it before cuda_normalize_model) */
+      snprintf(path, sizeof path, "model_%04d.raw", iteration);
+      write_raw(out, path, model.data, N_model);
+    }
+    if (respons) {
+      snprintf(path, sizeof path, "respons_%04d.raw", iteration);
+      write_raw(out, path, respons, (size_t)N_slices * N_images);
+    }
+  }
+  /* final normalised model */
+  float *final_model = malloc(sizeof(float) * N_model);
+  emc_hip_memcpy_to_host(final_model, d_model, sizeof(float) * N_model);
+  write_raw(out, "model_final.raw", final_model, N_model);
+  write_raw(out, "truth.raw", truth, N_model);
+  printf("%d iterations in %.1f s\n", iterations, t_total / 1e3);
+  fclose(scale);
+  fclose(likelihood);
+  die_if_error("end");
+  emc_hip_shutdown();
+  return 0;
+}
