@@ -83,7 +83,7 @@ bool Engine::init() {
 void Engine::release_all() {
   DBuf* bufs[] = {&d_pix, &d_B, &d_K32, &d_Kt32, &d_K16, &d_K16lo, &d_Kt16, &d_Kt16lo, &d_flags,
                   &d_S32, &d_S16, &d_S16lo, &d_Sscale, &d_A, &d_phi_used, &d_smax, &d_ssum, &d_sacc,
-                  &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Pscale, &d_Sc, &d_wts, &d_wT,
+                  &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Pscale, &d_P16, &d_P16lo, &d_Sc, &d_wts, &d_wT,
                   &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
   for (DBuf* b : bufs) b->release();
   pack_key = {};
@@ -231,7 +231,7 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
 // Packed slice operand for the current precision from an [R][N2d] buffer.
 static bool pack_slices(const float* slices, int R, int N2d) {
   Engine& e = engine();
-  e.R_pad = (int)round_up(R, kTileN);
+  e.R_pad = (int)round_up(R, kPadR);
   const size_t ns = (size_t)e.R_pad * e.n_mp;
   if (!e.d_A.ensure(sizeof(double) * R)) return false;
   PhaseTimer pt(PH_PACK);
@@ -467,21 +467,18 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   }
   if (!ensure_pack(d_images, d_mask, N_images, N_2d)) return 0.0f;
   const int R = N_slices, N = N_images;
-  e.R_pad = (int)round_up(R, kTileN);
+  e.R_pad = (int)round_up(R, kPadR);
+  const bool fp16 = e.precision == PREC_FP16;
   if (!e.d_invphi.ensure(sizeof(float) * N) || !e.d_T.ensure(sizeof(double) * R) || !e.d_Tf.ensure(sizeof(float) * R) ||
       !e.d_Pscale.ensure(sizeof(float) * R) || !e.d_Sc.ensure(sizeof(float) * (size_t)e.R_pad * e.n_mp) ||
       !e.d_wts.ensure(sizeof(float) * R) || !e.d_wT.ensure(sizeof(float) * R))
+    return 0.0f;
+  if (fp16 && (!e.d_P16.ensure(2 * (size_t)e.R_pad * e.N_pad) || !e.d_P16lo.ensure(2 * (size_t)e.R_pad * e.N_pad)))
     return 0.0f;
   EMC_CHECK(hipMemcpyAsync(e.d_wts.p, weights, sizeof(float) * R, hipMemcpyHostToDevice, e.stream));
   // 1/phi from the current scaling: the M-step divides by the updated
   // factors (emc_atomic.cu:94)
   launch_reciprocal(d_scaling, N, e.d_invphi.as<float>(), e.stream);
-  const bool fp16 = e.precision == PREC_FP16;
-  {
-    PhaseTimer pt(PH_ROWSUM);
-    launch_rowsum(d_respons, R, N, e.d_invphi.as<float>(), e.d_T.as<double>(), e.d_Tf.as<float>(),
-                  e.d_Pscale.as<float>(), fp16, e.stream);
-  }
   {
     MstepArgs a{};
     a.P = d_respons;
@@ -500,11 +497,18 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
     a.n_m = e.n_m;
     a.N2d = N_2d;
     a.vecP = (N % 4 == 0) && ((uintptr_t)d_respons % 16 == 0);
+    {
+      // T_r, and for the FP16 form the row scales and the hi/lo operand
+      PhaseTimer pt(PH_ROWSUM);
+      launch_rowsum(d_respons, R, N, e.d_invphi.as<float>(), e.d_T.as<double>(), e.d_Tf.as<float>(),
+                    e.d_Pscale.as<float>(), fp16, e.stream);
+      if (fp16) launch_convert_p(a, e.d_P16.as<_Float16>(), e.d_P16lo.as<_Float16>(), e.stream);
+    }
     PhaseTimer pt(PH_MSTEP);
     if (fp16)
-      launch_mstep(PREC_FP16, !e.k_exact, e.d_Kt16.p, e.d_Kt16lo.p, a, e.stream);
+      launch_mstep16(!e.k_exact, e.d_P16.p, e.d_P16lo.p, e.d_Kt16.p, e.d_Kt16lo.p, a, e.stream);
     else
-      launch_mstep(PREC_FP32, false, e.d_Kt32.p, nullptr, a, e.stream);
+      launch_mstep32(e.d_Kt32.as<float>(), a, e.stream);
   }
   g_slices_epoch++;
   launch_set_wT(e.d_wts.as<float>(), e.d_Tf.as<float>(), R, e.d_wT.as<float>(), e.stream);
@@ -762,7 +766,7 @@ void emc_hip_invalidate(void) {
 }
 
 static const char* kPhaseNames[PH_COUNT] = {"expand", "pack", "likelihood", "softmax", "scaling",
-                                            "rowsum", "mstep", "insert", "normalize", "other"};
+                                            "mstep_prep", "mstep", "insert", "normalize", "other"};
 
 int emc_hip_phase_times(double* ms, long long* launches, int n) {
   Engine& e = engine();

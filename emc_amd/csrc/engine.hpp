@@ -47,10 +47,10 @@ enum Interp { INTERP_NEAREST = 0, INTERP_TRILINEAR = 1 };
 // Tile sizes shared by the GEMM kernels and the padding of packed operands.
 constexpr int kTileM = 128;    // FP32 kernels: 128 x 128
 constexpr int kTileN = 128;
-constexpr int kTileM16 = 256;  // FP16 E-step: 256 patterns x 128 rotations
-constexpr int kPadK = 256;     // pixel padding: multiple of every BK, of kTileN and of
-                               // the FP16 M-step's 256-pixel tile
+constexpr int kTile16 = 256;   // FP16 kernels: 256 x 256
+constexpr int kPadK = 256;     // pixel padding: multiple of every BK and pixel tile
 constexpr int kPadN = 256;     // pattern padding: multiple of every pattern tile
+constexpr int kPadR = 256;     // rotation padding: multiple of every rotation tile
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
@@ -122,6 +122,7 @@ struct Engine {
   DBuf d_T;          // double [R]
   DBuf d_Tf;         // float [R]
   DBuf d_Pscale;     // float [R]: FP16 M-step operand row scale
+  DBuf d_P16, d_P16lo;  // half [R_pad][N_pad]: FP16 M-step operand (P/phi * row scale) hi/lo
   DBuf d_Sc;         // float [R_pad][n_mp]: updated compact slices for insertion
   DBuf d_wts;        // float [R]: rotation weights (uploaded per call)
   DBuf d_wT;         // float [R]: weights[r] * T_r
@@ -213,12 +214,13 @@ struct MstepArgs {
   int N, R, N_pad, R_pad, n_mp, n_m, N2d;
   int vecP;               // P rows 16-byte aligned (N % 4 == 0, aligned base)
 };
-void launch_mstep(int precision, bool ksplit, const void* Kt, const void* Ktlo,
-                  const MstepArgs& a, hipStream_t st);
+void launch_mstep32(const float* Kt, const MstepArgs& a, hipStream_t st);
 // FP16 hi/lo forms (k_gemm16.hip)
 void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, const void* Slo, int N_pad,
                     int R_pad, int n_mp, const LikeEpilogue& ep, hipStream_t st);
-void launch_mstep16(bool ksplit, const void* Kt, const void* Ktlo, const MstepArgs& a, hipStream_t st);
+void launch_convert_p(const MstepArgs& a, _Float16* hi, _Float16* lo, hipStream_t st);
+void launch_mstep16(bool ksplit, const void* Phi, const void* Plo, const void* Kt, const void* Ktlo,
+                    const MstepArgs& a, hipStream_t st);
 
 // softmax / totals / scaling (k_softmax.hip)
 int softmax_chunks(int N, int R);

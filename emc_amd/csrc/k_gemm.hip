@@ -209,15 +209,10 @@ __global__ __launch_bounds__(256) void k_mstep_f32(const float* __restrict__ Kt,
     for (int y = 0; y < 4; y++) mstep_store(a, m0 + wm * 64 + x * 16 + 4 * g, n0 + wn * 64 + y * 16 + fr, acc[x][y], 1.0f);
 }
 
-void launch_mstep(int precision, bool ksplit, const void* Kt, const void* Ktlo, const MstepArgs& a,
-                  hipStream_t st) {
-  if (precision == PREC_FP16) {
-    launch_mstep16(ksplit, Kt, Ktlo, a, st);
-    return;
-  }
+void launch_mstep32(const float* Kt, const MstepArgs& a, hipStream_t st) {
   const int tiles_m = a.R_pad / BM, tiles_n = a.n_mp / BN;  // n_mp is a multiple of BN
   const dim3 grid(tiles_m * tiles_n), block(256);
-  hipLaunchKernelGGL(k_mstep_f32, grid, block, 0, st, (const float*)Kt, tiles_m, tiles_n, a);
+  hipLaunchKernelGGL(k_mstep_f32, grid, block, 0, st, Kt, tiles_m, tiles_n, a);
   EMC_LAUNCHED();
 }
 

@@ -1,15 +1,18 @@
 // FP16 matrix-core contractions (precision form) of the E-step and M-step.
 //
-// C[m][n] = sum_k A[m][k] B[n][k] with both operands row-major along k.  The
-// operand that is not exact in FP16 is carried as a hi + lo pair
-// (x = hi + lo to ~22 bits); the products hi*hi + hi*lo (+ lo*hi when both
-// sides are split) are accumulated in FP32 by v_mfma_f32_16x16x32_f16.
+// C[m][n] = sum_k A[m][k] B[n][k] with both operands FP16 row-major along k.
+// The operand that is not exact in FP16 is carried as a hi + lo pair
+// (x = hi + lo to ~22 bits); hi*hi + hi*lo (+ lo*hi when both sides are
+// split) accumulate in FP32 through v_mfma_f32_16x16x32_f16.
 //
-// Workgroup: 4 wave64s in a 2 x 2 grid; wave tile (16 TM) x (16 TN); the
-// split operand sits on the 128-wide side so each stage stages 256 + 2 x 128
-// (or 2 x 128 + 256) rows of 32 halves: 64 MFMAs per wave per stage against
-// 32 KB of global reads.  LDS double buffered (80 KB, two workgroups per
-// CU), next stage's global loads in flight during the current MFMAs.
+// Workgroup: 8 wave64s, 256 x 256 output tile, wave tile 128 x 64 (E-step,
+// 2 x 4 waves) or 64 x 128 (M-step, 4 x 2 waves).  Operands reach LDS by
+// direct global->LDS loads (global_load_lds_dwordx4): each wave-level load
+// writes 1 KB = 16 rows of 32 halves; no staging registers.  Three LDS stages
+// (48 KB each) keep two stages of loads in flight while the third is
+// consumed; one barrier per stage.  LDS rows are 64 B with the four 16-byte
+// chunks of a row XOR-swizzled by the row index, so the 16-byte fragment
+// reads of 16 consecutive rows hit distinct banks.
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
@@ -20,112 +23,93 @@ namespace emc {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
+#define EMC_LDS __attribute__((address_space(3)))
 
-constexpr int BK = 32;      // halves per stage along k
-constexpr int LDK = BK + 8;  // padded LDS row pitch (80 bytes: conflict-free b128 reads)
+constexpr int BK = 32;  // halves per stage along k (one MFMA k-step)
 
-// Plain FP16 rows: global [rows][ld] (+ optional lo copy) -> LDS.
-template <int ROWS, int NS>
-struct RowsF16 {
-  static constexpr int PER = ROWS / 64;  // 16-byte chunks per thread per split
-  const _Float16* g[NS];
-  uint4 r[NS][PER];
-  int lr, lc;
-  __device__ void init(const _Float16* hi, const _Float16* lo, size_t row0, size_t ld) {
-    lr = threadIdx.x >> 2;
-    lc = (threadIdx.x & 3) * 8;
-    g[0] = hi + (row0 + lr) * ld + lc;
-    if (NS > 1) g[NS - 1] = lo + (row0 + lr) * ld + lc;
-    ldm = ld;
-  }
-  size_t ldm;
-  __device__ __forceinline__ void load(int k0) {
-#pragma unroll
-    for (int s = 0; s < NS; s++)
-#pragma unroll
-      for (int j = 0; j < PER; j++) r[s][j] = *reinterpret_cast<const uint4*>(g[s] + (size_t)j * 64 * ldm + k0);
-  }
-  __device__ __forceinline__ void store(_Float16 (*lds)[ROWS][LDK]) {
-#pragma unroll
-    for (int s = 0; s < NS; s++)
-#pragma unroll
-      for (int j = 0; j < PER; j++) *reinterpret_cast<uint4*>(&lds[s][lr + 64 * j][lc]) = r[s][j];
-  }
+__device__ __forceinline__ int chunk_swz(int row) { return ((row >> 1) ^ (row >> 2)) & 3; }
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// Operand description for the main loop: NA splits of A (rows m0..m0+BM),
+// NB splits of B (rows n0..n0+BN), row pitch ld (halves).
+struct Operands {
+  const _Float16* a[2];
+  const _Float16* b[2];
+  size_t lda, ldb;
 };
 
-// M-step A operand: responsibilities P[r][i] (fp32), scaled by invphi[i] and
-// a per-row power of two, split into hi/lo halves on the way to LDS.
-template <int ROWS>
-struct RowsP {
-  static constexpr int PER = ROWS / 32;  // float4 per thread
-  const MstepArgs* a;
-  floatx4 r[PER];
-  float rs[PER];
-  int ar, ac, m0;
-  __device__ void init(const MstepArgs* args, int row0) {
-    a = args;
-    m0 = row0;
-    ar = threadIdx.x >> 3;
-    ac = (threadIdx.x & 7) * 4;
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-      const int row = m0 + ar + 32 * j;
-      rs[j] = row < a->R ? a->rowscale[row] : 0.0f;
-    }
-  }
-  __device__ __forceinline__ void load(int k0) {
-#pragma unroll
-    for (int j = 0; j < PER; j++) r[j] = load_p4(*a, m0 + ar + 32 * j, k0 + ac, rs[j]);
-  }
-  __device__ __forceinline__ void store(_Float16 (*lds)[ROWS][LDK]) {
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-      halfx4 hi, lo;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        hi[c] = (_Float16)r[j][c];
-        lo[c] = (_Float16)(r[j][c] - (float)hi[c]);
-      }
-      *reinterpret_cast<halfx4*>(&lds[0][ar + 32 * j][ac]) = hi;
-      *reinterpret_cast<halfx4*>(&lds[1][ar + 32 * j][ac]) = lo;
-    }
-  }
-};
+// Main loop.  WGM x WGN waves, each TM x TN blocks of 16x16; STAGES LDS
+// buffers.  On return acc holds this wave's block of C.
+template <int TM, int TN, int WGM, int WGN, int NA, int NB, int STAGES>
+__device__ __forceinline__ void dl_mma_loop(const Operands& op, int m0, int n0, int nk, floatx4 (&acc)[TM][TN],
+                                            _Float16* lds) {
+  constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN, NW = WGM * WGN;
+  constexpr int ROWS = NA * BM + NB * BN;  // rows of 32 halves per stage
+  constexpr int STAGE_HALVES = ROWS * 32;
+  constexpr int PERW = ROWS / 16 / NW;     // 1 KB loads per wave per stage
+  static_assert((ROWS / 16) % NW == 0, "stage rows must split evenly over waves");
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w / WGN, wn = w % WGN;
 
-// Main loop.  NA / NB: splits of A / B (1 or 2).  Returns the accumulators.
-template <int TM, int TN, int NA, int NB, class LA, class LB>
-__device__ __forceinline__ void mma_loop(LA& la, LB& lb, int nk, floatx4 (&acc)[TM][TN],
-                                         _Float16 (*sA)[NA][32 * TM][LDK], _Float16 (*sB)[NB][32 * TN][LDK]) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
-  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  // global source of each of this wave's loads: row r of the stage, chunk
+  // (lane & 3) ^ swizzle(r) so that LDS slot lane holds the swizzled chunk
+  const _Float16* src[PERW];
+#pragma unroll
+  for (int j = 0; j < PERW; j++) {
+    const int row = (w * PERW + j) * 16 + (lane >> 2);
+    const int c = (lane & 3) ^ chunk_swz(row);
+    const _Float16* p;
+    if (row < NA * BM) {
+      const int s = row / BM, m = row - s * BM;
+      p = op.a[s] + (size_t)(m0 + m) * op.lda;
+    } else {
+      const int rb = row - NA * BM;
+      const int s = rb / BN, n = rb - s * BN;
+      p = op.b[s] + (size_t)(n0 + n) * op.ldb;
+    }
+    src[j] = p + c * 8;
+  }
+  auto issue = [&](int stage, int k0) {
+#pragma unroll
+    for (int j = 0; j < PERW; j++)
+      __builtin_amdgcn_global_load_lds((const void*)(src[j] + k0),
+                                       (EMC_LDS void*)(lds + stage * STAGE_HALVES + (w * PERW + j) * 16 * 32), 16, 0, 0);
+  };
 #pragma unroll
   for (int x = 0; x < TM; x++)
 #pragma unroll
     for (int y = 0; y < TN; y++) acc[x][y] = floatx4{0, 0, 0, 0};
-  la.load(0);
-  lb.load(0);
-  la.store(sA[0]);
-  lb.store(sB[0]);
-  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < STAGES - 1; s++)
+    if (s < nk) issue(s, s * BK);
+  const int fr = lane & 15, g = lane >> 4;
   for (int kt = 0; kt < nk; kt++) {
-    const int buf = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) {
-      la.load((kt + 1) * BK);
-      lb.load((kt + 1) * BK);
-    }
+    // this wave's loads of stage kt are complete (later stages may still be
+    // in flight); the barrier makes every wave's part visible and also
+    // guarantees that all waves finished reading the buffer refilled below
+    if (kt + STAGES - 2 < nk) wait_vmcnt<(STAGES - 2) * PERW>(); else wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (kt + STAGES - 1 < nk) issue((kt + STAGES - 1) % STAGES, (kt + STAGES - 1) * BK);
+    const _Float16* L = lds + (kt % STAGES) * STAGE_HALVES;
     halfx8 fb[NB][TN];
 #pragma unroll
     for (int s = 0; s < NB; s++)
 #pragma unroll
-      for (int y = 0; y < TN; y++)
-        fb[s][y] = *reinterpret_cast<const halfx8*>(&sB[buf][s][wn * 16 * TN + y * 16 + fr][fk]);
+      for (int y = 0; y < TN; y++) {
+        const int row = NA * BM + s * BN + wn * 16 * TN + y * 16 + fr;
+        fb[s][y] = *reinterpret_cast<const halfx8*>(&L[row * 32 + ((g ^ chunk_swz(row)) * 8)]);
+      }
 #pragma unroll
     for (int x = 0; x < TM; x++) {
       halfx8 fa[NA];
 #pragma unroll
-      for (int s = 0; s < NA; s++)
-        fa[s] = *reinterpret_cast<const halfx8*>(&sA[buf][s][wm * 16 * TM + x * 16 + fr][fk]);
+      for (int s = 0; s < NA; s++) {
+        const int row = s * BM + wm * 16 * TM + x * 16 + fr;
+        fa[s] = *reinterpret_cast<const halfx8*>(&L[row * 32 + ((g ^ chunk_swz(row)) * 8)]);
+      }
 #pragma unroll
       for (int y = 0; y < TN; y++) {
         acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[0][y], acc[x][y], 0, 0, 0);
@@ -133,35 +117,27 @@ __device__ __forceinline__ void mma_loop(LA& la, LB& lb, int nk, floatx4 (&acc)[
         if (NA > 1) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[NA - 1], fb[0][y], acc[x][y], 0, 0, 0);
       }
     }
-    if (more) {
-      la.store(sA[buf ^ 1]);
-      lb.store(sB[buf ^ 1]);
-    }
-    __syncthreads();
   }
+  wait_vmcnt<0>();
 }
 
-// E-step: A = packed counts (rows = patterns, 256 per tile), B = packed
-// slices hi/lo (rows = rotations, 128 per tile).
+// ---------------------------------------------------------------------------
+// E-step: A = packed counts (256 patterns), B = packed slices hi/lo (256
+// rotations).  KSPLIT: counts not exact in FP16, A carried hi/lo too (then
+// two stages to stay within LDS).
+// ---------------------------------------------------------------------------
 template <bool KSPLIT, bool PC>
-__global__ __launch_bounds__(256, 2) void k_estep16(const _Float16* __restrict__ K, const _Float16* __restrict__ Klo,
-                                                 const _Float16* __restrict__ S, const _Float16* __restrict__ Slo,
-                                                 int n_mp, int tiles_m, int tiles_n, LikeEpilogue ep) {
-  constexpr int TM = 8, TN = 4, NA = KSPLIT ? 2 : 1, NB = 2;
-  constexpr int BMM = 32 * TM, BNN = 32 * TN;
-  __shared__ __attribute__((aligned(16))) _Float16 sA[2][NA][BMM][LDK];
-  __shared__ __attribute__((aligned(16))) _Float16 sB[2][NB][BNN][LDK];
+__global__ __launch_bounds__(512, 1) void k_estep16(Operands op, int n_mp, int tiles_m, int tiles_n, LikeEpilogue ep) {
+  constexpr int TM = 8, TN = 4, WGM = 2, WGN = 4, NA = KSPLIT ? 2 : 1, NB = 2, STAGES = KSPLIT ? 2 : 3;
+  constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
+  __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
   int tm, tn;
   tile_of_block(tiles_m, tiles_n, tm, tn);
-  const int m0 = tm * BMM, n0 = tn * BNN;
-  RowsF16<BMM, NA> la;
-  RowsF16<BNN, NB> lb;
-  la.init(K, Klo, (size_t)m0, (size_t)n_mp);
-  lb.init(S, Slo, (size_t)n0, (size_t)n_mp);
+  const int m0 = tm * BM, n0 = tn * BN;
   floatx4 acc[TM][TN];
-  mma_loop<TM, TN, NA, NB>(la, lb, n_mp / BK, acc, sA, sB);
+  dl_mma_loop<TM, TN, WGM, WGN, NA, NB, STAGES>(op, m0, n0, n_mp / BK, acc, lds);
 
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w / WGN, wn = w % WGN;
   const int g = lane >> 4, fr = lane & 15;
   if (!PC) {
 #pragma unroll
@@ -188,26 +164,22 @@ __global__ __launch_bounds__(256, 2) void k_estep16(const _Float16* __restrict__
   }
 }
 
-// M-step: A = P/phi hi/lo (rows = rotations, 128 per tile), B = transposed
-// packed counts (rows = pixels, 256 per tile).
+// ---------------------------------------------------------------------------
+// M-step: A = (P/phi * rowscale) hi/lo (256 rotations), B = transposed packed
+// counts (256 pixels), reduction over patterns.
+// ---------------------------------------------------------------------------
 template <bool KSPLIT>
-__global__ __launch_bounds__(256, 2) void k_mstep16(const _Float16* __restrict__ Kt, const _Float16* __restrict__ Ktlo,
-                                                 int tiles_m, int tiles_n, MstepArgs a) {
-  constexpr int TM = 4, TN = 8, NA = 2, NB = KSPLIT ? 2 : 1;
-  constexpr int BMM = 32 * TM, BNN = 32 * TN;
-  __shared__ __attribute__((aligned(16))) _Float16 sA[2][NA][BMM][LDK];
-  __shared__ __attribute__((aligned(16))) _Float16 sB[2][NB][BNN][LDK];
+__global__ __launch_bounds__(512, 1) void k_mstep16(Operands op, int tiles_m, int tiles_n, MstepArgs a) {
+  constexpr int TM = 4, TN = 8, WGM = 4, WGN = 2, NA = 2, NB = KSPLIT ? 2 : 1, STAGES = KSPLIT ? 2 : 3;
+  constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
+  __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
   int tm, tn;
   tile_of_block(tiles_m, tiles_n, tm, tn);
-  const int m0 = tm * BMM, n0 = tn * BNN;
-  RowsP<BMM> la;
-  RowsF16<BNN, NB> lb;
-  la.init(&a, m0);
-  lb.init(Kt, Ktlo, (size_t)n0, (size_t)a.N_pad);
+  const int m0 = tm * BM, n0 = tn * BN;
   floatx4 acc[TM][TN];
-  mma_loop<TM, TN, NA, NB>(la, lb, a.N_pad / BK, acc, sA, sB);
+  dl_mma_loop<TM, TN, WGM, WGN, NA, NB, STAGES>(op, m0, n0, a.N_pad / BK, acc, lds);
 
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w / WGN, wn = w % WGN;
   const int g = lane >> 4, fr = lane & 15;
 #pragma unroll
   for (int x = 0; x < TM; x++)
@@ -216,30 +188,77 @@ __global__ __launch_bounds__(256, 2) void k_mstep16(const _Float16* __restrict__
       mstep_store(a, m0 + wm * 16 * TM + x * 16 + 4 * g, n0 + wn * 16 * TN + y * 16 + fr, acc[x][y], a.kscale_inv);
 }
 
+// P'[r][i] = P_ri / phi_i * rowscale_r split into hi/lo halves, zero padded to
+// [R_pad][N_pad] (the M-step's A operand).
+__global__ __launch_bounds__(256) void k_convert_p(const float* __restrict__ P, int R, int N, int N_pad,
+                                                   const float* __restrict__ invphi, const float* __restrict__ rowscale,
+                                                   int vecP, _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
+  const int r = blockIdx.x;
+  const float rs = r < R ? rowscale[r] : 0.0f;
+  const float* row = P + (size_t)r * N;
+  for (int i = threadIdx.x * 4; i < N_pad; i += 256 * 4) {
+    floatx4 v = {0, 0, 0, 0};
+    if (r < R) {
+      if (vecP && i + 3 < N) {
+        v = *reinterpret_cast<const floatx4*>(row + i) * *reinterpret_cast<const floatx4*>(invphi + i) * rs;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (i + c < N) v[c] = row[i + c] * invphi[i + c] * rs;
+      }
+    }
+    halfx4 h, l;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      h[c] = (_Float16)v[c];
+      l[c] = (_Float16)(v[c] - (float)h[c]);
+    }
+    *reinterpret_cast<halfx4*>(hi + (size_t)r * N_pad + i) = h;
+    *reinterpret_cast<halfx4*>(lo + (size_t)r * N_pad + i) = l;
+  }
+}
+
+void launch_convert_p(const MstepArgs& a, _Float16* hi, _Float16* lo, hipStream_t st) {
+  hipLaunchKernelGGL(k_convert_p, dim3(a.R_pad), dim3(256), 0, st, a.P, a.R, a.N, a.N_pad, a.invphi, a.rowscale,
+                     a.vecP, hi, lo);
+  EMC_LAUNCHED();
+}
+
 void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, const void* Slo, int N_pad,
                     int R_pad, int n_mp, const LikeEpilogue& ep, hipStream_t st) {
-  const int tiles_m = N_pad / kTileM16, tiles_n = R_pad / kTileN;
-  const dim3 grid(tiles_m * tiles_n), block(256);
+  const int tiles_m = N_pad / kTile16, tiles_n = R_pad / kTile16;
+  const dim3 grid(tiles_m * tiles_n), block(512);
+  Operands op{};
+  op.a[0] = (const _Float16*)K;
+  op.a[1] = (const _Float16*)Klo;
+  op.b[0] = (const _Float16*)S;
+  op.b[1] = (const _Float16*)Slo;
+  op.lda = op.ldb = (size_t)n_mp;
   const bool pc = ep.P != nullptr;
-  const _Float16 *k = (const _Float16*)K, *kl = (const _Float16*)Klo, *s = (const _Float16*)S,
-                 *sl = (const _Float16*)Slo;
   if (ksplit) {
-    if (pc) hipLaunchKernelGGL((k_estep16<true, true>), grid, block, 0, st, k, kl, s, sl, n_mp, tiles_m, tiles_n, ep);
-    else hipLaunchKernelGGL((k_estep16<true, false>), grid, block, 0, st, k, kl, s, sl, n_mp, tiles_m, tiles_n, ep);
+    if (pc) hipLaunchKernelGGL((k_estep16<true, true>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, ep);
+    else hipLaunchKernelGGL((k_estep16<true, false>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, ep);
   } else {
-    if (pc) hipLaunchKernelGGL((k_estep16<false, true>), grid, block, 0, st, k, kl, s, sl, n_mp, tiles_m, tiles_n, ep);
-    else hipLaunchKernelGGL((k_estep16<false, false>), grid, block, 0, st, k, kl, s, sl, n_mp, tiles_m, tiles_n, ep);
+    if (pc) hipLaunchKernelGGL((k_estep16<false, true>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, ep);
+    else hipLaunchKernelGGL((k_estep16<false, false>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, ep);
   }
   EMC_LAUNCHED();
 }
 
-void launch_mstep16(bool ksplit, const void* Kt, const void* Ktlo, const MstepArgs& a, hipStream_t st) {
-  const int tiles_m = a.R_pad / kTileN, tiles_n = a.n_mp / kPadK;
-  const dim3 grid(tiles_m * tiles_n), block(256);
+void launch_mstep16(bool ksplit, const void* Phi, const void* Plo, const void* Kt, const void* Ktlo,
+                    const MstepArgs& a, hipStream_t st) {
+  const int tiles_m = a.R_pad / kTile16, tiles_n = a.n_mp / kTile16;
+  const dim3 grid(tiles_m * tiles_n), block(512);
+  Operands op{};
+  op.a[0] = (const _Float16*)Phi;
+  op.a[1] = (const _Float16*)Plo;
+  op.b[0] = (const _Float16*)Kt;
+  op.b[1] = (const _Float16*)Ktlo;
+  op.lda = op.ldb = (size_t)a.N_pad;
   if (ksplit)
-    hipLaunchKernelGGL(k_mstep16<true>, grid, block, 0, st, (const _Float16*)Kt, (const _Float16*)Ktlo, tiles_m, tiles_n, a);
+    hipLaunchKernelGGL(k_mstep16<true>, grid, block, 0, st, op, tiles_m, tiles_n, a);
   else
-    hipLaunchKernelGGL(k_mstep16<false>, grid, block, 0, st, (const _Float16*)Kt, (const _Float16*)Ktlo, tiles_m, tiles_n, a);
+    hipLaunchKernelGGL(k_mstep16<false>, grid, block, 0, st, op, tiles_m, tiles_n, a);
   EMC_LAUNCHED();
 }
 
