@@ -106,6 +106,13 @@ _SIGS = [
     ("emc_hip_reset_phase_times", None, []),
     ("emc_hip_phase_name", C.c_char_p, [C.c_int]),
     ("emc_hip_pack_info", C.c_int, [c_int_p, C.c_int]),
+    ("emc_hip_cnn_init", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong]),
+    ("emc_hip_cnn_param_count", C.c_long, []),
+    ("emc_hip_cnn_get", C.c_int, [c_float_p, c_float_p]),
+    ("emc_hip_cnn_set", C.c_int, [c_float_p]),
+    ("emc_hip_cnn_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, c_int_p, C.c_int, c_float_p]),
+    ("emc_hip_cnn_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, c_int_p, C.c_int, c_float_p, C.c_float,
+                                   C.POINTER(C.c_double)]),
     # emc_host.h
     ("emc_n_to_samples", C.c_int, [C.c_int]),
     ("emc_rotation_table", C.c_int, [C.c_int, C.c_int, c_float_p, c_float_p]),

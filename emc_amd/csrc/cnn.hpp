@@ -1,0 +1,75 @@
+// Rotation-proposal network: GEMM descriptor, kernel launchers and the
+// network object (weights, Adam state, activations).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace emc {
+
+// C(m, n) = sum_k A(m, k) B(n, k) [+ bias_n[n] + bias_m[m]] [ReLU] [+= C]
+// with A(m, k) = A[m*sam + k*sak], B(n, k) = B[n*sbn + k*sbk],
+// C(m, n) = C[m*scm + n*scn].
+struct GemmDesc {
+  const float* A = nullptr;
+  long sam = 0, sak = 0;
+  const float* B = nullptr;
+  long sbn = 0, sbk = 0;
+  float* C = nullptr;
+  long scm = 0, scn = 0;
+  const float* bias_n = nullptr;
+  const float* bias_m = nullptr;
+  int M = 0, N = 0, K = 0;
+  int relu = 0;
+  int accumulate = 0;
+};
+
+void gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st);
+void cnn_input(const float* images, const int* mask, int N2d, const int* idx, int B, float* x, hipStream_t st);
+void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t st);
+void col2im(const float* col, int B, int H, int W, int C, float* in, hipStream_t st);
+void relu_bwd(float* g, const float* a, long n, hipStream_t st);
+void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, hipStream_t st);
+void softmax_xent(const float* logits, int C, int B, const float* targets, long Nt, const int* tidx, float scale,
+                  float* grad, double* loss, hipStream_t st);
+void adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps, int step,
+          hipStream_t st);
+void topk(const float* logits, int C, int B, int k, const int* out_rows, int* out, hipStream_t st);
+void coarse_targets(const float* P, int N, const int* cell_off, const int* cell_rot, int C, float* T, hipStream_t st);
+
+// The network.  Parameters in one FP32 buffer:
+//   conv l (l = 0..3): W [OC][3][3][IC], b [OC]; dense1 W [H][F], b [H];
+//   dense2 W [C][H], b [C]  (F = 64 * (S/16)^2 flattened NHWC features).
+struct Cnn {
+  int side = 0, ncell = 0, hidden = 256, batch = 0;
+  static constexpr int kConvs = 4;
+  int ic[kConvs] = {1, 16, 32, 64};
+  int oc[kConvs] = {16, 32, 64, 64};
+  int hin[kConvs] = {0, 0, 0, 0};  // input height of conv l
+  long off_w[kConvs + 2] = {0}, off_b[kConvs + 2] = {0};
+  long nparam = 0;
+  int feat = 0;
+  int step = 0;
+  DBuf params, grads, adam_m, adam_v;
+  // activations for one batch
+  DBuf x, col[kConvs], act[kConvs], h, logits;
+  DBuf dact, dcol, dh, dlogits, scratch, idx, loss;
+  size_t scratch_floats = 0;
+
+  bool init(int side, int ncell, int batch, uint64_t seed);
+  void release();
+  // forward for patterns idx_host[0..B) (B <= batch) into logits [C][B]
+  void forward(const float* images, const int* mask, int N2d, const int* d_idx, int B, hipStream_t st);
+  // backward from dlogits [C][B] and Adam step
+  void backward_and_step(int B, float lr, hipStream_t st);
+  float* W(int l) const { return params.as<float>() + off_w[l]; }
+  float* Bv(int l) const { return params.as<float>() + off_b[l]; }
+  float* dW(int l) const { return grads.as<float>() + off_w[l]; }
+  float* dB(int l) const { return grads.as<float>() + off_b[l]; }
+};
+
+}  // namespace emc

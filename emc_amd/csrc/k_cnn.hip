@@ -1,0 +1,315 @@
+// Kernels of the rotation-proposal network (no counterpart in the reference).
+//
+// Network: x = log1p(counts) on the S x S detector (unmasked pixels 0);
+// four 3x3 / stride-2 / pad-1 convolutions with ReLU (channels 16, 32, 64,
+// 64), flatten, dense 256 + ReLU, dense C (coarse rotation cells), softmax.
+// Activations are NHWC FP32; convolutions run as im2col + GEMM; every GEMM
+// rounds its operands to BF16 and accumulates in FP32 on
+// v_mfma_f32_16x16x32_bf16 (BF16 compute, FP32 master weights).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "engine.hpp"
+#include "cnn.hpp"
+#include "reduce.hpp"
+
+namespace emc {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------
+// Generic strided GEMM: C(m, n) = sum_k A(m, k) B(n, k), operands FP32 in
+// memory (any strides), rounded to BF16 on the way to LDS.  64 x 64 tiles,
+// 4 waves of 32 x 32, BK = 32.  Split-K over gridDim.z writes partial tiles
+// to `partial` ([z][M][N]); gemm_finish adds them in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int GT = 64, GK = 32, GLD = GK + 8;
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmDesc d, float* __restrict__ partial, int ksplit_len) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[GT][GLD];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[GT][GLD];
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  const int kz0 = blockIdx.z * ksplit_len, kz1 = min(d.K, kz0 + ksplit_len);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = floatx4{0, 0, 0, 0};
+  // element (row, k) of each thread's 8 loads: k-contiguous operands are read
+  // along k, the others along the row
+  const bool a_kfast = d.sak == 1, b_kfast = d.sbk == 1;
+  for (int k0 = kz0; k0 < kz1; k0 += GK) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      int r, k;
+      if (a_kfast) { r = t >> 2; k = (t & 3) * 8 + e; }
+      else { k = t >> 3; r = (t & 7) * 8 + e; }
+      const int gm = m0 + r, gk = k0 + k;
+      float v = 0.0f;
+      if (gm < d.M && gk < kz1) v = d.A[(long)gm * d.sam + (long)gk * d.sak];
+      sA[r][k] = (__bf16)v;
+      int rn, kn;
+      if (b_kfast) { rn = t >> 2; kn = (t & 3) * 8 + e; }
+      else { kn = t >> 3; rn = (t & 7) * 8 + e; }
+      const int gn = n0 + rn, gk2 = k0 + kn;
+      float u = 0.0f;
+      if (gn < d.N && gk2 < kz1) u = d.B[(long)gn * d.sbn + (long)gk2 * d.sbk];
+      sB[rn][kn] = (__bf16)u;
+    }
+    __syncthreads();
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    bf16x8 fa[2], fb[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) fa[a] = *reinterpret_cast<const bf16x8*>(&sA[wm * 32 + a * 16 + fr][fk]);
+#pragma unroll
+    for (int b = 0; b < 2; b++) fb[b] = *reinterpret_cast<const bf16x8*>(&sB[wn * 32 + b * 16 + fr][fk]);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    __syncthreads();
+  }
+  const int g = lane >> 4, fr = lane & 15;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int m = m0 + wm * 32 + a * 16 + 4 * g + j, n = n0 + wn * 32 + b * 16 + fr;
+        if (m >= d.M || n >= d.N) continue;
+        if (partial) {
+          partial[((long)blockIdx.z * d.M + m) * d.N + n] = acc[a][b][j];
+        } else {
+          float v = acc[a][b][j];
+          if (d.bias_n) v += d.bias_n[n];
+          if (d.bias_m) v += d.bias_m[m];
+          if (d.relu) v = fmaxf(v, 0.0f);
+          float* c = d.C + (long)m * d.scm + (long)n * d.scn;
+          *c = d.accumulate ? *c + v : v;
+        }
+      }
+}
+
+__global__ void k_gemm_finish(GemmDesc d, const float* __restrict__ partial, int nz) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)d.M * d.N) return;
+  const int m = (int)(idx / d.N), n = (int)(idx % d.N);
+  float v = 0.0f;
+  for (int z = 0; z < nz; z++) v += partial[(long)z * d.M * d.N + idx];
+  if (d.bias_n) v += d.bias_n[n];
+  if (d.bias_m) v += d.bias_m[m];
+  if (d.relu) v = fmaxf(v, 0.0f);
+  float* c = d.C + (long)m * d.scm + (long)n * d.scn;
+  *c = d.accumulate ? *c + v : v;
+}
+
+void gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st) {
+  if (d.M <= 0 || d.N <= 0) return;
+  const int gm = (d.M + GT - 1) / GT, gn = (d.N + GT - 1) / GT;
+  // split K so that a long reduction still spreads over the chip
+  int nz = 1;
+  const long tiles = (long)gm * gn;
+  while (tiles * nz < 512 && d.K / (nz * 2) >= 4 * GK && (size_t)(nz * 2) * d.M * d.N <= scratch_floats) nz *= 2;
+  const int klen = ((d.K + nz - 1) / nz + GK - 1) / GK * GK;
+  nz = (d.K + klen - 1) / klen;
+  if (nz <= 1) {
+    hipLaunchKernelGGL(k_gemm_bf16, dim3(gn, gm, 1), dim3(256), 0, st, d, nullptr, d.K);
+  } else {
+    hipLaunchKernelGGL(k_gemm_bf16, dim3(gn, gm, nz), dim3(256), 0, st, d, scratch, klen);
+    EMC_LAUNCHED();
+    const long n = (long)d.M * d.N;
+    hipLaunchKernelGGL(k_gemm_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, scratch, nz);
+  }
+  EMC_LAUNCHED();
+}
+
+// ---------------------------------------------------------------------------
+// Layers
+// ---------------------------------------------------------------------------
+
+// x[b][y][x] = log1p(max(K, 0)) on masked-in pixels, 0 elsewhere.
+__global__ void k_cnn_input(const float* __restrict__ images, const int* __restrict__ mask, int N2d,
+                            const int* __restrict__ idx, int B, float* __restrict__ x) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)B * N2d) return;
+  const int b = (int)(t / N2d), p = (int)(t % N2d);
+  const float k = images[(long)idx[b] * N2d + p];
+  x[t] = mask[p] != 0 ? log1pf(fmaxf(k, 0.0f)) : 0.0f;
+}
+
+// col[(b*OH + oy)*OW + ox][(ky*3 + kx)*C + c] = in[b][2oy+ky-1][2ox+kx-1][c]
+__global__ void k_im2col(const float* __restrict__ in, int B, int H, int W, int C, float* __restrict__ col) {
+  const int OH = H / 2, OW = W / 2, K = 9 * C;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)B * OH * OW * K) return;
+  const long m = t / K;
+  const int kk = (int)(t % K);
+  const int c = kk % C, kx = (kk / C) % 3, ky = kk / (3 * C);
+  const int ox = (int)(m % OW), oy = (int)((m / OW) % OH), b = (int)(m / ((long)OW * OH));
+  const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
+  col[t] = (y >= 0 && y < H && x >= 0 && x < W) ? in[(((long)b * H + y) * W + x) * C + c] : 0.0f;
+}
+
+// Adjoint of k_im2col as a gather (each input element sums the col entries
+// that copied it): deterministic, no atomics.
+__global__ void k_col2im(const float* __restrict__ col, int B, int H, int W, int C, float* __restrict__ in) {
+  const int OH = H / 2, OW = W / 2, K = 9 * C;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)B * H * W * C) return;
+  const int c = (int)(t % C), x = (int)((t / C) % W), y = (int)((t / ((long)C * W)) % H);
+  const int b = (int)(t / ((long)C * W * H));
+  float s = 0.0f;
+  for (int ky = 0; ky < 3; ky++) {
+    const int ty = y + 1 - ky;
+    if (ty < 0 || (ty & 1)) continue;
+    const int oy = ty >> 1;
+    if (oy >= OH) continue;
+    for (int kx = 0; kx < 3; kx++) {
+      const int tx = x + 1 - kx;
+      if (tx < 0 || (tx & 1)) continue;
+      const int ox = tx >> 1;
+      if (ox >= OW) continue;
+      s += col[(((long)b * OH + oy) * OW + ox) * K + (ky * 3 + kx) * C + c];
+    }
+  }
+  in[t] = s;
+}
+
+// g *= (a > 0)
+__global__ void k_relu_bwd(float* __restrict__ g, const float* __restrict__ a, long n) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n && !(a[t] > 0.0f)) g[t] = 0.0f;
+}
+
+// out[j] = sum_i G[i*si + j*sj], i < rows (fixed order per column).
+__global__ void k_reduce_rows(const float* __restrict__ G, long rows, int cols, long si, long sj, float* __restrict__ out) {
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (long i = threadIdx.x; i < rows; i += 256) s += G[i * si + (long)j * sj];
+  s = block_sum_f64(s);
+  if (threadIdx.x == 0) out[j] = (float)s;
+}
+
+// Softmax cross-entropy over logits stored [C][B] (lane = pattern).
+// targets: [C][Nt] at columns tidx[b]; grad = (p - t) * scale.
+__global__ void k_softmax_xent(const float* __restrict__ logits, int C, int B, const float* __restrict__ targets,
+                               long Nt, const int* __restrict__ tidx, float scale, float* __restrict__ grad,
+                               double* __restrict__ loss) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  double l = 0.0;
+  if (b < B) {
+    float m = -3.0e38f;
+    for (int c = 0; c < C; c++) m = fmaxf(m, logits[(long)c * B + b]);
+    double s = 0.0;
+    for (int c = 0; c < C; c++) s += exp((double)logits[(long)c * B + b] - m);
+    const double ls = log(s);
+    const long col = tidx[b];
+    for (int c = 0; c < C; c++) {
+      const double lp = (double)logits[(long)c * B + b] - m - ls;
+      const float tc = targets[(long)c * Nt + col];
+      if (tc > 0.0f) l -= tc * lp;
+      if (grad) grad[(long)c * B + b] = (float)((exp(lp) - tc) * scale);
+    }
+  }
+  l = block_sum_f64(l);
+  if (threadIdx.x == 0 && loss) atomicAdd(loss, l);
+}
+
+// Adam on FP32 master weights (bias-corrected).
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       long n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float gt = g[t];
+  const float mt = b1 * m[t] + (1.0f - b1) * gt;
+  const float vt = b2 * v[t] + (1.0f - b2) * gt * gt;
+  m[t] = mt;
+  v[t] = vt;
+  p[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+}
+
+// The k largest logits of each pattern (logits [C][B]), descending, ties by
+// lower index; k repeated maximum scans below the previous pick.
+__global__ void k_topk(const float* __restrict__ logits, int C, int B, int k, const int* __restrict__ out_rows,
+                       int* __restrict__ out) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= B) return;
+  float last_v = 3.0e38f;
+  int last_i = -1;
+  const long row = out_rows ? out_rows[b] : b;
+  for (int j = 0; j < k; j++) {
+    float bv = -3.0e38f;
+    int bi = -1;
+    for (int c = 0; c < C; c++) {
+      const float v = logits[(long)c * B + b];
+      const bool below = v < last_v || (v == last_v && c > last_i);
+      if (below && (bi < 0 || v > bv)) {
+        bv = v;
+        bi = c;
+      }
+    }
+    out[row * k + j] = bi;
+    last_v = bv;
+    last_i = bi;
+  }
+}
+
+// Coarse targets: T[c][i] = sum over fine rotations r of cell c of P[r][i]
+// (P in the ABI's [R][N] layout; cells as CSR lists of fine rotations).
+__global__ void k_coarse_targets(const float* __restrict__ P, int N, const int* __restrict__ cell_off,
+                                 const int* __restrict__ cell_rot, int C, float* __restrict__ T) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= N || c >= C) return;
+  double s = 0.0;
+  for (int j = cell_off[c]; j < cell_off[c + 1]; j++) s += P[(long)cell_rot[j] * N + i];
+  T[(long)c * N + i] = (float)s;
+}
+
+static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+void cnn_input(const float* images, const int* mask, int N2d, const int* idx, int B, float* x, hipStream_t st) {
+  hipLaunchKernelGGL(k_cnn_input, dim3(nblk((long)B * N2d)), dim3(256), 0, st, images, mask, N2d, idx, B, x);
+  EMC_LAUNCHED();
+}
+void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t st) {
+  hipLaunchKernelGGL(k_im2col, dim3(nblk((long)B * (H / 2) * (W / 2) * 9 * C)), dim3(256), 0, st, in, B, H, W, C, col);
+  EMC_LAUNCHED();
+}
+void col2im(const float* col, int B, int H, int W, int C, float* in, hipStream_t st) {
+  hipLaunchKernelGGL(k_col2im, dim3(nblk((long)B * H * W * C)), dim3(256), 0, st, col, B, H, W, C, in);
+  EMC_LAUNCHED();
+}
+void relu_bwd(float* g, const float* a, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n)), dim3(256), 0, st, g, a, n);
+  EMC_LAUNCHED();
+}
+void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_rows, dim3(cols), dim3(256), 0, st, G, rows, cols, si, sj, out);
+  EMC_LAUNCHED();
+}
+void softmax_xent(const float* logits, int C, int B, const float* targets, long Nt, const int* tidx, float scale,
+                  float* grad, double* loss, hipStream_t st) {
+  hipLaunchKernelGGL(k_softmax_xent, dim3(nblk(B)), dim3(256), 0, st, logits, C, B, targets, Nt, tidx, scale, grad, loss);
+  EMC_LAUNCHED();
+}
+void adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps, int step,
+          hipStream_t st) {
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, st, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+  EMC_LAUNCHED();
+}
+void topk(const float* logits, int C, int B, int k, const int* out_rows, int* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_topk, dim3(nblk(B)), dim3(256), 0, st, logits, C, B, k, out_rows, out);
+  EMC_LAUNCHED();
+}
+void coarse_targets(const float* P, int N, const int* cell_off, const int* cell_rot, int C, float* T, hipStream_t st) {
+  hipLaunchKernelGGL(k_coarse_targets, dim3(nblk(N), C), dim3(256), 0, st, P, N, cell_off, cell_rot, C, T);
+  EMC_LAUNCHED();
+}
+
+}  // namespace emc

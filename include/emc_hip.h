@@ -81,6 +81,27 @@ void emc_hip_reset_phase_times(void);
 /* Name of phase k, or NULL. */
 const char *emc_hip_phase_name(int k);
 
+/* ---- Rotation-proposal network (no counterpart in the reference) ----
+ * A small CNN maps a pattern (log1p of its counts on the detector grid,
+ * unmasked pixels 0) to a distribution over coarse rotation cells: four
+ * 3x3 / stride-2 convolutions (16, 32, 64, 64 channels, ReLU), dense 256 +
+ * ReLU, dense n_cells, softmax.  BF16 operands, FP32 accumulation, FP32
+ * master weights, Adam.  These calls expose it for testing; the proposal
+ * workflow below drives it. */
+int emc_hip_cnn_init(int side, int n_cells, int batch, unsigned long long seed);
+long emc_hip_cnn_param_count(void);
+/* Copy the parameters and/or the last gradients (either pointer may be NULL). */
+int emc_hip_cnn_get(float *params, float *grads);
+int emc_hip_cnn_set(const float *params);
+/* Logits of patterns idx[0..B) of d_images, written [n_cells][B]. */
+int emc_hip_cnn_forward(const real *d_images, const int *d_mask, int N_2d, const int *idx, int B,
+                        float *logits);
+/* One step of softmax cross-entropy training against targets [n_cells][B]
+ * (rows sum to 1): forward, backward, Adam with learning rate lr (0 keeps
+ * the weights, the gradients are still computed).  *loss = mean loss. */
+int emc_hip_cnn_step(const real *d_images, const int *d_mask, int N_2d, const int *idx, int B,
+                     const float *targets, float lr, double *loss);
+
 /* Information on the packed pattern operand of the last E-step:
  * out[0] = masked pixel count n_m, out[1] = padded count, out[2] = 1 if the
  * FP16 operand holds the counts exactly, out[3] = GEMM passes per product. */
