@@ -113,9 +113,18 @@ _SIGS = [
     ("emc_hip_cnn_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, c_int_p, C.c_int, c_float_p]),
     ("emc_hip_cnn_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, c_int_p, C.c_int, c_float_p, C.c_float,
                                    C.POINTER(C.c_double)]),
+    ("emc_hip_proposal_init", C.c_int, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]),
+    ("emc_hip_proposal_cells", C.c_int, [c_int_p, c_int_p]),
+    ("emc_hip_proposal_train", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                         C.c_float, C.POINTER(C.c_double)]),
+    ("emc_hip_proposal_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+    ("emc_hip_proposal_set_candidates", C.c_int, [c_int_p, C.c_int, C.c_int]),
+    ("emc_hip_proposal_get", C.c_int, [c_int_p, c_int_p, c_float_p, C.c_int]),
+    ("emc_hip_proposal_stats", C.c_int, [C.POINTER(C.c_double), C.c_int]),
     # emc_host.h
     ("emc_n_to_samples", C.c_int, [C.c_int]),
     ("emc_rotation_table", C.c_int, [C.c_int, C.c_int, c_float_p, c_float_p]),
+    ("emc_assign_cells", C.c_int, [c_float_p, C.c_int, c_float_p, C.c_int, c_int_p]),
     ("generate_rotation_list", C.c_int, [C.c_int, C.c_void_p, C.c_void_p]),
     ("emc_coordinates", None, [Setup, c_float_p, c_float_p, c_float_p]),
     ("calculate_coordinates", None, [Setup, C.POINTER(SpMatrix), C.POINTER(SpMatrix), C.POINTER(SpMatrix)]),

@@ -38,7 +38,7 @@ void softmax_xent(const float* logits, int C, int B, const float* targets, long 
                   float* grad, double* loss, hipStream_t st);
 void adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps, int step,
           hipStream_t st);
-void topk(const float* logits, int C, int B, int k, const int* out_rows, int* out, hipStream_t st);
+void topk(const float* logits, int C, int B, int k, const int* out_rows, int* out, float* mass, hipStream_t st);
 void coarse_targets(const float* P, int N, const int* cell_off, const int* cell_rot, int C, float* T, hipStream_t st);
 
 // The network.  Parameters in one FP32 buffer:

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "proposal.hpp"
 
 namespace emc {
 
@@ -317,7 +318,16 @@ void cuda_calculate_responsabilities(real* slices, real* images, int* d_mask, re
   if (!pack_slices(slices, N_slices, N_2d)) return;
   if (!e.d_phi_used.ensure(sizeof(float) * N_images)) return;
   EMC_CHECK(hipMemcpyAsync(e.d_phi_used.p, d_scaling, sizeof(float) * N_images, hipMemcpyDeviceToDevice, e.stream));
-  run_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), d_respons, nullptr, nullptr);
+  const bool prop = proposal_applies(N_images, N_slices);
+  if (prop) {
+    if (respons) {
+      set_error("cuda_calculate_responsabilities: no host copy of the log-likelihoods in proposal mode");
+      return;
+    }
+    if (!proposal_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>())) return;
+  } else {
+    run_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), d_respons, nullptr, nullptr);
+  }
   g_respons_epoch++;
   auto& lk = e.like;
   lk.valid = !failed();
@@ -331,6 +341,7 @@ void cuda_calculate_responsabilities(real* slices, real* images, int* d_mask, re
   lk.sigma2x = 2.0 * (double)(float)e.n_m * ((double)sigma * (double)sigma);
   lk.normalized = false;
   lk.have_total = false;
+  lk.proposal = prop;
   if (respons) {
     EMC_CHECK(hipMemcpyAsync(respons, d_respons, sizeof(float) * (size_t)N_images * N_slices,
                              hipMemcpyDeviceToHost, e.stream));
@@ -352,6 +363,18 @@ void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images
     return;
   }
   auto& lk = e.like;
+  if (lk.valid && lk.proposal && !lk.normalized && lk.respons == d_respons && lk.N == N_images &&
+      lk.R == N_slices) {
+    // responsibilities over each pattern's candidates (whole table for the
+    // audited and fallback patterns), written densely into d_respons
+    double total = 0.0;
+    if (!e.d_phiden.ensure(sizeof(double) * N_images)) return;
+    if (!proposal_softmax(d_respons, N_images, N_slices, lk.sigma2x, &total)) return;
+    lk.normalized = true;
+    lk.have_total = true;
+    lk.total = total;
+    return;
+  }
   const bool fuse = lk.valid && !lk.normalized && lk.respons == d_respons && lk.N == N_images && lk.R == N_slices &&
                     e.d_A.p && e.d_B.p;
   const int chunks = softmax_chunks(N_images, N_slices);
@@ -469,17 +492,24 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   const int R = N_slices, N = N_images;
   e.R_pad = (int)round_up(R, kPadR);
   const bool fp16 = e.precision == PREC_FP16;
+  const Proposal& prp = proposal();
+  const bool prop = proposal_applies(N, R) && prp.have_p && prp.respons == d_respons && e.like.valid &&
+                    e.like.proposal && e.like.normalized && e.like.respons == d_respons;
   if (!e.d_invphi.ensure(sizeof(float) * N) || !e.d_T.ensure(sizeof(double) * R) || !e.d_Tf.ensure(sizeof(float) * R) ||
       !e.d_Pscale.ensure(sizeof(float) * R) || !e.d_Sc.ensure(sizeof(float) * (size_t)e.R_pad * e.n_mp) ||
       !e.d_wts.ensure(sizeof(float) * R) || !e.d_wT.ensure(sizeof(float) * R))
     return 0.0f;
-  if (fp16 && (!e.d_P16.ensure(2 * (size_t)e.R_pad * e.N_pad) || !e.d_P16lo.ensure(2 * (size_t)e.R_pad * e.N_pad)))
+  if (fp16 && !prop &&
+      (!e.d_P16.ensure(2 * (size_t)e.R_pad * e.N_pad) || !e.d_P16lo.ensure(2 * (size_t)e.R_pad * e.N_pad)))
     return 0.0f;
   EMC_CHECK(hipMemcpyAsync(e.d_wts.p, weights, sizeof(float) * R, hipMemcpyHostToDevice, e.stream));
   // 1/phi from the current scaling: the M-step divides by the updated
   // factors (emc_atomic.cu:94)
   launch_reciprocal(d_scaling, N, e.d_invphi.as<float>(), e.stream);
-  {
+  if (prop) {
+    // candidate-restricted M-step (+ the whole-table subset)
+    if (!proposal_mstep(N, R, slices, N_2d)) return 0.0f;
+  } else {
     MstepArgs a{};
     a.P = d_respons;
     a.invphi = e.d_invphi.as<float>();
@@ -752,6 +782,19 @@ int emc_hip_set_option(const char* key, const char* value) {
     e.timing = value[0] == '1';
   } else if (is(key, "verbose")) {
     e.verbose = value[0] == '1';
+  } else if (is(key, "proposal")) {
+    if (is(value, "on")) proposal().active = true;
+    else if (is(value, "off")) proposal().active = false;
+    else return 1;
+  } else if (is(key, "audit_fraction") || is(key, "min_mass")) {
+    char* end = nullptr;
+    const double v = strtod(value, &end);
+    if (!end || *end != 0 || !(v >= 0.0 && v <= 1.0)) return 1;
+    (is(key, "min_mass") ? proposal().min_mass : proposal().audit_frac) = (float)v;
+  } else if (is(key, "proposal_gate")) {
+    if (is(value, "predicted")) proposal().gate_predicted = true;
+    else if (is(value, "audit")) proposal().gate_predicted = false;
+    else return 1;
   } else {
     return 1;
   }
@@ -766,7 +809,7 @@ void emc_hip_invalidate(void) {
 }
 
 static const char* kPhaseNames[PH_COUNT] = {"expand", "pack", "likelihood", "softmax", "scaling",
-                                            "mstep_prep", "mstep", "insert", "normalize", "other"};
+                                            "mstep_prep", "mstep", "insert", "normalize", "other", "audit"};
 
 int emc_hip_phase_times(double* ms, long long* launches, int n) {
   Engine& e = engine();

@@ -37,7 +37,7 @@ struct DBuf {
 // ---- phases for timing -----------------------------------------------------
 enum Phase {
   PH_EXPAND = 0, PH_PACK, PH_LIKE, PH_SOFTMAX, PH_SCALING, PH_ROWSUM, PH_MSTEP,
-  PH_INSERT, PH_NORMALIZE, PH_OTHER, PH_COUNT
+  PH_INSERT, PH_NORMALIZE, PH_OTHER, PH_AUDIT, PH_COUNT
 };
 
 enum Precision { PREC_FP32 = 0, PREC_FP16 = 1 };
@@ -109,6 +109,7 @@ struct Engine {
     bool normalized = false;  // softmax ran on this buffer after the likelihood
     bool have_total = false;
     double total = 0.0;       // sum P log P from the softmax pass
+    bool proposal = false;    // L came from the candidate-restricted E-step
   } like;
   DBuf d_phi_used;   // float [N]: scaling used by the likelihood
   DBuf d_smax;       // float  [chunks][N] partial maxima
@@ -213,6 +214,7 @@ struct MstepArgs {
   float kscale_inv;
   int N, R, N_pad, R_pad, n_mp, n_m, N2d;
   int vecP;               // P rows 16-byte aligned (N % 4 == 0, aligned base)
+  int raw = 0;            // write the unnormalised sums U to Sc only
 };
 void launch_mstep32(const float* Kt, const MstepArgs& a, hipStream_t st);
 // FP16 hi/lo forms (k_gemm16.hip)
@@ -233,6 +235,8 @@ void launch_total_respons(const float* P, size_t n, double* red, hipStream_t st)
 void launch_scaling_finish(int N, const double* B, const double* phiden, float* scaling,
                            float* invphi, hipStream_t st);
 void launch_reciprocal(const float* s, int N, float* inv, hipStream_t st);
+// out[0] = sum of v[0..n) in a fixed order (one workgroup)
+void launch_sum_f64(const double* v, int n, double* out, hipStream_t st);
 // model helpers (k_model.hip)
 void launch_rowsum(const float* P, int R, int N, const float* invphi, double* T, float* Tf,
                    float* rowscale, bool want_scale, hipStream_t st);

@@ -94,7 +94,8 @@ __device__ __forceinline__ floatx4_t load_p4(const MstepArgs& a, int r, int i, f
 
 // M-step epilogue: rows r0..r0+3 (rotations), column k (packed pixel).
 // U = acc * extra / rowscale[r]; S' = T_r > 1e-10 ? U / T_r : U written to
-// the full slice and to the packed copy used by insertion.
+// the full slice and to the packed copy used by insertion (raw: U itself to
+// the packed copy only).
 __device__ __forceinline__ void mstep_store(const MstepArgs& a, int r0, int k, floatx4_t acc, float extra) {
   if (k >= a.n_m) return;
   const int p = a.pix[k];
@@ -104,6 +105,10 @@ __device__ __forceinline__ void mstep_store(const MstepArgs& a, int r0, int k, f
     if (r >= a.R) continue;
     float u = acc[j] * extra;
     if (a.rowscale) u = u / a.rowscale[r];
+    if (a.raw) {
+      a.Sc[(size_t)r * a.n_mp + k] = u;
+      continue;
+    }
     const float T = a.Tf[r];
     const float s = T > 1e-10f ? u / T : u;
     a.slices[(size_t)r * a.N2d + p] = s;

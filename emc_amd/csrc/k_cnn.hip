@@ -233,14 +233,17 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // The k largest logits of each pattern (logits [C][B]), descending, ties by
-// lower index; k repeated maximum scans below the previous pick.
+// lower index; k repeated maximum scans below the previous pick.  With mass
+// != nullptr, mass[b] = softmax probability held by the k picks.
 __global__ void k_topk(const float* __restrict__ logits, int C, int B, int k, const int* __restrict__ out_rows,
-                       int* __restrict__ out) {
+                       int* __restrict__ out, float* __restrict__ mass) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= B) return;
   float last_v = 3.0e38f;
   int last_i = -1;
   const long row = out_rows ? out_rows[b] : b;
+  float top = -3.0e38f;
+  double picked = 0.0;
   for (int j = 0; j < k; j++) {
     float bv = -3.0e38f;
     int bi = -1;
@@ -253,8 +256,15 @@ __global__ void k_topk(const float* __restrict__ logits, int C, int B, int k, co
       }
     }
     out[row * k + j] = bi;
+    if (j == 0) top = bv;
+    if (bi >= 0) picked += exp((double)bv - (double)top);
     last_v = bv;
     last_i = bi;
+  }
+  if (mass) {
+    double z = 0.0;
+    for (int c = 0; c < C; c++) z += exp((double)logits[(long)c * B + b] - (double)top);
+    mass[row] = (float)(picked / z);
   }
 }
 
@@ -303,8 +313,8 @@ void adam(float* p, const float* g, float* m, float* v, long n, float lr, float 
   hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, st, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
   EMC_LAUNCHED();
 }
-void topk(const float* logits, int C, int B, int k, const int* out_rows, int* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_topk, dim3(nblk(B)), dim3(256), 0, st, logits, C, B, k, out_rows, out);
+void topk(const float* logits, int C, int B, int k, const int* out_rows, int* out, float* mass, hipStream_t st) {
+  hipLaunchKernelGGL(k_topk, dim3(nblk(B)), dim3(256), 0, st, logits, C, B, k, out_rows, out, mass);
   EMC_LAUNCHED();
 }
 void coarse_targets(const float* P, int N, const int* cell_off, const int* cell_rot, int C, float* T, hipStream_t st) {

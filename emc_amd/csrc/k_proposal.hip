@@ -1,0 +1,511 @@
+// Kernels of the candidate-restricted iteration (learned rotation proposal).
+//
+// Every pattern i has topk candidate coarse cells cand[i][j]; each cell c
+// holds the fine rotations cell_rot[cell_off[c] .. cell_off[c+1]) (at most
+// kSlot).  Work is organised by cell: the entries (pattern, j) whose
+// candidate j is c form the cell's list ent_off[c] .. ent_off[c+1], patterns
+// ascending.  Per-candidate values live in Lc[i][j][s] (s = slot = position
+// of the rotation in its cell), first log-likelihoods, then responsibilities.
+//
+// E-step: one workgroup = one cell x 128 entries; the gathered count rows
+// and the cell's slice rows (hi/lo) stream through the FP16 main loop by
+// direct global->LDS loads.  M-step: one workgroup = one cell x 256 pixels;
+// the responsibility panel (hi/lo, prepared per cell) and the gathered count
+// rows are staged in LDS, two stages.  Each rotation belongs to exactly one
+// cell, so every M-step output is owned by one workgroup: no atomics.
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+#include "gemm16.hpp"
+#include "gemm_common.hpp"
+#include "proposal_kernels.hpp"
+#include "reduce.hpp"
+
+namespace emc {
+
+// ---------------------------------------------------------------------------
+// E-step over the candidates
+// ---------------------------------------------------------------------------
+template <bool KSPLIT>
+__global__ __launch_bounds__(256) void k_cand_estep(CandEstepArgs a) {
+  constexpr int TM = 2, TN = 2, WGM = 4, WGN = 1, NA = KSPLIT ? 2 : 1, NB = 2, STAGES = 3;
+  constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
+  static_assert(BM == kCandRows && BN == kSlot, "tile must match the work items");
+  __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
+  int wi, unused;
+  tile_of_block(a.n_work, 1, wi, unused);
+  const int2 w = a.work[wi];
+  const int c = w.x, e0 = w.y;
+  const int e1 = min(e0 + BM, a.ent_off[c + 1]);
+  const int r0 = a.cell_off[c], nr = a.cell_off[c + 1] - r0;
+  Operands op{};
+  op.a[0] = a.K;
+  op.a[1] = a.Klo;
+  op.b[0] = a.S;
+  op.b[1] = a.Slo;
+  op.lda = op.ldb = (size_t)a.n_mp;
+  op.arow = a.ent_pat + e0;
+  op.arow_n = e1 - e0;
+  op.brow = a.cell_rot + r0;
+  op.brow_n = nr;
+  op.zero = a.zero;
+  floatx4 acc[TM][TN];
+  dl_mma_loop<TM, TN, WGM, WGN, NA, NB, STAGES>(op, 0, 0, a.n_mp / BK, acc, lds);
+
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, fr = lane & 15;
+#pragma unroll
+  for (int y = 0; y < TN; y++) {
+    const int s = y * 16 + fr;
+    if (s >= nr) continue;
+    const int r = a.cell_rot[r0 + s];
+    const double cs = (double)a.sscale[r] * (double)a.kscale_inv;
+    const double Ar = a.A[r];
+#pragma unroll
+    for (int x = 0; x < TM; x++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int e = e0 + wv * 16 * TM + x * 16 + 4 * g + j;
+        if (e >= e1) continue;
+        const int i = a.ent_pat[e];
+        const double phi = (double)a.phi[i];
+        const double cc = (double)acc[x][y][j] * cs;
+        const float L = (float)(-(Ar - 2.0 * cc / phi + a.B[i] / (phi * phi)) * a.inv_2nms2);
+        a.Lc[((size_t)i * a.topk + a.ent_j[e]) * kSlot + s] = L;
+      }
+  }
+}
+
+void launch_cand_estep(bool ksplit, const CandEstepArgs& a, hipStream_t st) {
+  if (a.n_work <= 0) return;
+  if (ksplit)
+    hipLaunchKernelGGL(k_cand_estep<true>, dim3(a.n_work), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_cand_estep<false>, dim3(a.n_work), dim3(256), 0, st, a);
+  EMC_LAUNCHED();
+}
+
+// ---------------------------------------------------------------------------
+// Softmax over each pattern's candidates (emc_cuda.cu:445-476 semantics:
+// m = max(-1e10, max L), P = exp(L - m) if L - m > -1e10 else 0, P /= sum),
+// one wave per pattern.  Also: sum P log P of the pattern, the scaling
+// denominator sum_r P_ri C_ir from L (C_ir = (phi/2)(A_r + B_i/phi^2 +
+// 2 n_m sigma^2 L_ri)), and the responsibilities scattered into the dense
+// [R][N] matrix (zeroed beforehand).  Patterns scored on the whole table
+// this iteration (insub) get zero candidate responsibilities.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cand_softmax(CandSoftmaxArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= a.N) return;
+  float* row = a.Lc + (size_t)i * a.topk * kSlot;
+  const int n = a.topk * kSlot;
+  const int* cand = a.cand + (size_t)i * a.topk;
+  if (a.insub[i]) {
+    for (int t = lane; t < n; t += 64) row[t] = 0.0f;
+    if (lane == 0) a.plogp[i] = 0.0;
+    return;
+  }
+  float m = -1.0e10f;
+  for (int t = lane; t < n; t += 64) {
+    const int j = t / kSlot, s = t - j * kSlot;
+    const int c = cand[j];
+    if (s < a.cell_off[c + 1] - a.cell_off[c]) m = fmaxf(m, row[t]);
+  }
+  m = wave_max(m);
+  double sum = 0.0;
+  for (int t = lane; t < n; t += 64) {
+    const int j = t / kSlot, s = t - j * kSlot;
+    const int c = cand[j];
+    if (s < a.cell_off[c + 1] - a.cell_off[c]) {
+      const float d = row[t] - m;
+      if (d > -1.0e10f) sum += (double)expf(d);
+    }
+  }
+  sum = wave_sum(sum);
+  const double inv = 1.0 / sum;
+  double plogp = 0.0, ps = 0.0, pa = 0.0, pl = 0.0;
+  for (int t = lane; t < n; t += 64) {
+    const int j = t / kSlot, s = t - j * kSlot;
+    const int c = cand[j];
+    const int o = a.cell_off[c];
+    float p = 0.0f;
+    if (s < a.cell_off[c + 1] - o) {
+      const float v = row[t];
+      const float d = v - m;
+      const float ex = d > -1.0e10f ? expf(d) : 0.0f;
+      p = (float)((double)ex * inv);
+      const int r = a.cell_rot[o + s];
+      if (p > 0.0f) plogp += (double)(p * logf(p));
+      ps += p;
+      pa += (double)p * a.A[r];
+      pl += (double)p * (double)v;
+      if (a.respons) a.respons[(size_t)r * a.N + i] = p;
+    }
+    row[t] = p;
+  }
+  plogp = wave_sum(plogp);
+  ps = wave_sum(ps);
+  pa = wave_sum(pa);
+  pl = wave_sum(pl);
+  if (lane == 0) {
+    const double f = (double)a.phi[i];
+    a.phiden[i] = 0.5 * f * (pa + a.B[i] / (f * f) * ps + a.sigma2x * pl);
+    a.plogp[i] = plogp;
+  }
+}
+
+void launch_cand_softmax(const CandSoftmaxArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_cand_softmax, dim3((a.N + 3) / 4), dim3(256), 0, st, a);
+  EMC_LAUNCHED();
+}
+
+// ---------------------------------------------------------------------------
+// Whole-table subset (audited patterns and patterns in full search)
+// ---------------------------------------------------------------------------
+// Rows sub[k] of the packed counts (hi, and lo if given) -> [n_sub_pad][n_mp],
+// zero rows from n_sub on.
+__global__ __launch_bounds__(256) void k_gather_rows(const _Float16* __restrict__ K, const _Float16* __restrict__ Klo,
+                                                     const int* __restrict__ sub, int n_sub, int n_mp,
+                                                     _Float16* __restrict__ Ks, _Float16* __restrict__ Kslo) {
+  const int k = blockIdx.x;
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const long src = k < n_sub ? (long)sub[k] * n_mp : -1;
+  for (int q = threadIdx.x; q < n_mp / 8; q += 256) {
+    u32x4 v = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
+    if (src >= 0) {
+      v = *reinterpret_cast<const u32x4*>(K + src + q * 8);
+      if (Klo) vl = *reinterpret_cast<const u32x4*>(Klo + src + q * 8);
+    }
+    *reinterpret_cast<u32x4*>(Ks + (long)k * n_mp + q * 8) = v;
+    if (Klo) *reinterpret_cast<u32x4*>(Kslo + (long)k * n_mp + q * 8) = vl;
+  }
+}
+
+// Transposed gather: Kt[p][k] = K[sub[k]][p], [n_mp][n_sub_pad], zero for
+// k >= n_sub.  64 x 64 tiles through LDS.
+__global__ __launch_bounds__(256) void k_gather_rows_t(const _Float16* __restrict__ K, const int* __restrict__ sub,
+                                                       int n_sub, int n_sub_pad, int n_mp, _Float16* __restrict__ Kt) {
+  __shared__ _Float16 tile[64][66];
+  const int k0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int kk = idx >> 6, pp = idx & 63;
+    const int k = k0 + kk;
+    tile[kk][pp] = k < n_sub ? K[(long)sub[k] * n_mp + p0 + pp] : (_Float16)0.0f;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int pp = idx >> 6, kk = idx & 63;
+    Kt[(long)(p0 + pp) * n_sub_pad + k0 + kk] = tile[kk][pp];
+  }
+}
+
+__global__ void k_gather_f64(const double* __restrict__ v, const int* __restrict__ sub, int n, double* __restrict__ out) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) out[k] = v[sub[k]];
+}
+__global__ void k_gather_f32(const float* __restrict__ v, const int* __restrict__ sub, int n, float* __restrict__ out) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) out[k] = v[sub[k]];
+}
+
+void launch_gather_rows(const _Float16* K, const _Float16* Klo, const int* sub, int n_sub, int n_sub_pad, int n_mp,
+                        _Float16* Ks, _Float16* Kslo, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_rows, dim3(n_sub_pad), dim3(256), 0, st, K, Klo, sub, n_sub, n_mp, Ks, Kslo);
+  EMC_LAUNCHED();
+}
+void launch_gather_rows_t(const _Float16* K, const int* sub, int n_sub, int n_sub_pad, int n_mp, _Float16* Kt,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_rows_t, dim3(n_sub_pad / 64, n_mp / 64), dim3(256), 0, st, K, sub, n_sub, n_sub_pad,
+                     n_mp, Kt);
+  EMC_LAUNCHED();
+}
+void launch_gather_f64(const double* v, const int* sub, int n, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_f64, dim3((n + 255) / 256), dim3(256), 0, st, v, sub, n, out);
+  EMC_LAUNCHED();
+}
+void launch_gather_f32(const float* v, const int* sub, int n, float* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_f32, dim3((n + 255) / 256), dim3(256), 0, st, v, sub, n, out);
+  EMC_LAUNCHED();
+}
+
+// Per subset pattern k (one wave each): its scaling denominator to the
+// pattern's slot, and the whole-table posterior mass its candidates hold.
+__global__ __launch_bounds__(256) void k_sub_finish(const float* __restrict__ Psub, int n_sub,
+                                                    const int* __restrict__ sub, const double* __restrict__ phiden_sub,
+                                                    double* __restrict__ phiden, const int* __restrict__ cand, int topk,
+                                                    const int* __restrict__ cell_off, const int* __restrict__ cell_rot,
+                                                    float* __restrict__ mass) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n_sub) return;
+  const int i = sub[k];
+  double s = 0.0;
+  for (int t = lane; t < topk * kSlot; t += 64) {
+    const int j = t / kSlot, q = t - j * kSlot;
+    const int c = cand[(size_t)i * topk + j];
+    const int o = cell_off[c];
+    if (q < cell_off[c + 1] - o) s += Psub[(size_t)cell_rot[o + q] * n_sub + k];
+  }
+  s = wave_sum(s);
+  if (lane == 0) {
+    phiden[i] = phiden_sub[k];
+    mass[k] = (float)s;
+  }
+}
+
+void launch_sub_finish(const float* Psub, int n_sub, const int* sub, const double* phiden_sub, double* phiden,
+                       const int* cand, int topk, const int* cell_off, const int* cell_rot, float* mass,
+                       hipStream_t st) {
+  if (n_sub <= 0) return;
+  hipLaunchKernelGGL(k_sub_finish, dim3((n_sub + 3) / 4), dim3(256), 0, st, Psub, n_sub, sub, phiden_sub, phiden,
+                     cand, topk, cell_off, cell_rot, mass);
+  EMC_LAUNCHED();
+}
+
+// respons[r][sub[k]] = Psub[r][k] for every rotation r (one block per r).
+__global__ __launch_bounds__(256) void k_sub_scatter(const float* __restrict__ Psub, int n_sub,
+                                                     const int* __restrict__ sub, int N, float* __restrict__ respons) {
+  const int r = blockIdx.x;
+  for (int k = threadIdx.x; k < n_sub; k += 256) respons[(size_t)r * N + sub[k]] = Psub[(size_t)r * n_sub + k];
+}
+
+void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int N, float* respons, hipStream_t st) {
+  if (n_sub <= 0) return;
+  hipLaunchKernelGGL(k_sub_scatter, dim3(R), dim3(256), 0, st, Psub, n_sub, sub, N, respons);
+  EMC_LAUNCHED();
+}
+
+// ---------------------------------------------------------------------------
+// M-step over the candidates
+// ---------------------------------------------------------------------------
+// One block per cell.  T_r = Tsub_r + sum over the cell's entries of P (fixed
+// order), the row scale of the FP16 operand (power of two placing the row's
+// largest P/phi near 2^14), and the operand itself: per chunk of 32 entries
+// a [32 slots][32 entries] panel of P/phi * scale in hi and lo halves.
+__global__ __launch_bounds__(256) void k_cand_prep(CandPrepArgs a) {
+  __shared__ double tpart[8][kSlot];
+  __shared__ float mpart[8][kSlot];
+  __shared__ float scale[kSlot];
+  __shared__ __attribute__((aligned(16))) _Float16 panel[2][kSlot * 32];
+  const int c = blockIdx.x;
+  const int eb = a.ent_off[c], ee = a.ent_off[c + 1];
+  const int r0 = a.cell_off[c], nr = a.cell_off[c + 1] - r0;
+  const int tid = threadIdx.x;
+  {
+    const int s = tid & (kSlot - 1), grp = tid >> 5;
+    double t = 0.0;
+    float mx = 0.0f;
+    for (int e = eb + grp; e < ee; e += 8) {
+      const int i = a.ent_pat[e];
+      const float p = a.Pc[((size_t)i * a.topk + a.ent_j[e]) * kSlot + s];
+      t += p;
+      mx = fmaxf(mx, fabsf(p * a.invphi[i]));
+    }
+    tpart[grp][s] = t;
+    mpart[grp][s] = mx;
+  }
+  __syncthreads();
+  if (tid < kSlot) {
+    const int s = tid;
+    double t = 0.0;
+    float mx = 0.0f;
+    for (int g = 0; g < 8; g++) {
+      t += tpart[g][s];
+      mx = fmaxf(mx, mpart[g][s]);
+    }
+    float sc = 1.0f;
+    if (mx > 0.0f && mx < 3.0e38f) {
+      int ex;
+      frexpf(mx, &ex);
+      sc = ldexpf(1.0f, max(-120, min(120, 14 - ex)));
+    }
+    scale[s] = sc;
+    if (s < nr) {
+      const int r = a.cell_rot[r0 + s];
+      const double T = (a.Tsub ? a.Tsub[r] : 0.0) + t;
+      a.T[r] = T;
+      a.Tf[r] = (float)T;
+      a.rowscale[r] = sc;
+    }
+  }
+  __syncthreads();
+  // panels: thread (entry el = tid & 31, slots 4 * (tid >> 5) .. +3)
+  const int el = tid & 31, s4 = (tid >> 5) * 4;
+  const long chunk0 = a.chunk_off[c];
+  const int nch = (ee - eb + 31) / 32;
+  for (int q = 0; q < nch; q++) {
+    const int e = eb + q * 32 + el;
+    floatx4 v = {0, 0, 0, 0};
+    if (e < ee) {
+      const int i = a.ent_pat[e];
+      v = *reinterpret_cast<const floatx4*>(a.Pc + ((size_t)i * a.topk + a.ent_j[e]) * kSlot + s4);
+      v = v * a.invphi[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const float x = v[u] * scale[s4 + u];
+      const _Float16 h = (_Float16)x;
+      panel[0][(s4 + u) * 32 + el] = h;
+      panel[1][(s4 + u) * 32 + el] = (_Float16)(x - (float)h);
+    }
+    __syncthreads();
+    // 2 x 2 KB out, 16 bytes per thread
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const int h = tid >> 7, o = (tid & 127) * 8;
+    *reinterpret_cast<u32x4*>(a.panels + ((chunk0 + q) * 2 + h) * (kSlot * 32) + o) =
+        *reinterpret_cast<const u32x4*>(&panel[h][o]);
+    __syncthreads();
+  }
+}
+
+void launch_cand_prep(const CandPrepArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_cand_prep, dim3(a.C), dim3(256), 0, st, a);
+  EMC_LAUNCHED();
+}
+
+// XOR swizzle of the 16-byte chunks of a count row in LDS: the four groups
+// of 8 entries read by one MFMA B fragment land in distinct banks.
+__device__ __forceinline__ int kswz(int row) { return ((row >> 3) & 3) * 2; }
+
+// U[r][k] = sum over the cell's entries e of (P/phi)_e,r K[pat(e)][k]
+// (+ the subset's U already in Sc), then S' = T > 1e-10 ? U / T : U to Sc and
+// to the full slices.  Workgroup: one cell x 256 pixels, 4 waves of 64
+// pixels; chunk of 32 entries = one MFMA k-step.
+template <bool KSPLIT>
+__global__ __launch_bounds__(256) void k_cand_mstep(CandMstepArgs a) {
+  constexpr int EC = 32, PT = 256, NBK = KSPLIT ? 2 : 1;
+  __shared__ __attribute__((aligned(1024))) _Float16 kb[2][NBK][EC * PT];
+  __shared__ __attribute__((aligned(1024))) _Float16 ab[2][2][kSlot * EC];
+  int cell, pt;
+  {
+    int tm, tn;
+    tile_of_block(a.C * a.n_ptiles, 1, tm, tn);
+    cell = tm % a.C;
+    pt = tm / a.C;
+  }
+  const int p0 = pt * PT;
+  const int eb = a.ent_off[cell], ee = a.ent_off[cell + 1];
+  const int r0 = a.cell_off[cell], nr = a.cell_off[cell + 1] - r0;
+  const int nch = (ee - eb + EC - 1) / EC;
+  const long chunk0 = a.chunk_off[cell];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, fr = lane & 15;
+
+  floatx4 acc[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) acc[x][y] = floatx4{0, 0, 0, 0};
+
+  // pattern of each of this lane's 4 rows for a chunk
+  auto rows_of = [&](int q, int (&pat)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int row = (wv * 4 + j) * 2 + (lane >> 5);
+      const int e = eb + q * EC + row;
+      pat[j] = e < ee ? a.ent_pat[e] : -1;
+    }
+  };
+  // loads of one chunk: 16 wave loads per count split (2 rows of 512 B
+  // each), 4 for the two 2 KB panels; a wave issues 4 * NBK + 1
+  auto dma_chunk = [&](int q, int st, const int (&pat)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int ld = wv * 4 + j;
+      const int row = ld * 2 + (lane >> 5);
+      const int ch = (lane & 31) ^ kswz(row);
+#pragma unroll
+      for (int sp = 0; sp < NBK; sp++) {
+        const _Float16* base = sp ? a.Klo : a.K;
+        const _Float16* src = pat[j] >= 0 ? base + (size_t)pat[j] * a.n_mp + p0 + ch * 8 : a.zero + ch * 8;
+        __builtin_amdgcn_global_load_lds((const void*)src, (EMC_LDS void*)(&kb[st][sp][ld * 512]), 16, 0, 0);
+      }
+    }
+    // panels: hi (2 KB) and lo (2 KB) = 4 wave loads; wave wv takes load wv
+    {
+      const int h = wv >> 1, part = wv & 1;
+      const _Float16* src = a.panels + ((chunk0 + q) * 2 + h) * (kSlot * EC) + part * 512 + lane * 8;
+      __builtin_amdgcn_global_load_lds((const void*)src, (EMC_LDS void*)(&ab[st][h][part * 512]), 16, 0, 0);
+    }
+  };
+
+  int pat_next[4];
+  if (nch > 0) {
+    int pat0[4];
+    rows_of(0, pat0);
+    dma_chunk(0, 0, pat0);
+    if (nch > 1) rows_of(1, pat_next);
+  }
+  for (int q = 0; q < nch; q++) {
+    const int st = q & 1;
+    wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (q + 1 < nch) {
+      dma_chunk(q + 1, st ^ 1, pat_next);
+      if (q + 2 < nch) rows_of(q + 2, pat_next);
+    }
+    // B fragments: K[e = 8g + jj][pixel wv*64 + 16y + fr]
+    halfx8 bf[NBK][4];
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int px = wv * 64 + y * 16 + fr;
+      const int chk = px >> 3, within = px & 7;
+#pragma unroll
+      for (int sp = 0; sp < NBK; sp++)
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++) {
+          const int row = g * 8 + jj;
+          bf[sp][y][jj] = kb[st][sp][row * PT + ((chk ^ kswz(row)) * 8) + within];
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const halfx8 ah = *reinterpret_cast<const halfx8*>(&ab[st][0][(x * 16 + fr) * EC + g * 8]);
+      const halfx8 al = *reinterpret_cast<const halfx8*>(&ab[st][1][(x * 16 + fr) * EC + g * 8]);
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bf[0][y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bf[0][y], acc[x][y], 0, 0, 0);
+        if (KSPLIT) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bf[NBK - 1][y], acc[x][y], 0, 0, 0);
+      }
+    }
+  }
+  wait_vmcnt<0>();
+
+  // epilogue: rows = slots 16x + 4g + j, column = pixel
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int s = x * 16 + 4 * g + j;
+      if (s >= nr) continue;
+      const int r = a.cell_rot[r0 + s];
+      const float inv_scale = a.kscale_inv / a.rowscale[r];
+      const float T = a.Tf[r];
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const int k = p0 + wv * 64 + y * 16 + fr;
+        if (k >= a.n_m) continue;
+        float u = acc[x][y][j] * inv_scale;
+        float* dst = a.Sc + (size_t)r * a.n_mp + k;
+        if (a.has_sub) u += *dst;
+        const float sv = T > 1e-10f ? u / T : u;
+        *dst = sv;
+        a.slices[(size_t)r * a.N2d + a.pix[k]] = sv;
+      }
+    }
+}
+
+void launch_cand_mstep(bool ksplit, const CandMstepArgs& a, hipStream_t st) {
+  const dim3 grid(a.C * a.n_ptiles);
+  if (ksplit)
+    hipLaunchKernelGGL(k_cand_mstep<true>, grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_cand_mstep<false>, grid, dim3(256), 0, st, a);
+  EMC_LAUNCHED();
+}
+
+}  // namespace emc

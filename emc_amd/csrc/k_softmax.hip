@@ -154,6 +154,11 @@ __global__ __launch_bounds__(256) void k_plogp_partial(const float* __restrict__
   if (threadIdx.x == 0) red[blockIdx.x] = s;
 }
 
+void launch_sum_f64(const double* v, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, st, v, n, out);
+  EMC_LAUNCHED();
+}
+
 // red must hold 1025 doubles; red[1024] = result.
 void launch_total_respons(const float* P, size_t n, double* red, hipStream_t st) {
   const int nb = 1024;

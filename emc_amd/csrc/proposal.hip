@@ -5,21 +5,351 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <random>
 #include <vector>
 
 #include "cnn.hpp"
 #include "emc_host.h"
 #include "engine.hpp"
 #include "proposal.hpp"
+#include "proposal_kernels.hpp"
 
 namespace emc {
 
 Proposal& proposal() {
   static Proposal* p = new Proposal();
   return *p;
+}
+
+void Proposal::release() {
+  DBuf* bufs[] = {&d_cell_off, &d_cell_rot, &d_cand, &d_pmass, &d_ent_off, &d_ent_pat, &d_ent_j, &d_work,
+                  &d_chunk_off, &d_Lc, &d_panels, &d_crowscale, &d_plogp, &d_zero, &d_sub, &d_insub, &d_Ksub,
+                  &d_Ksublo, &d_Ktsub, &d_Ktsublo, &d_Bsub, &d_phisub, &d_invphisub, &d_Lsub, &d_phiden_sub,
+                  &d_Psub16, &d_Psub16lo, &d_mass, &d_tot};
+  for (DBuf* b : bufs) b->release();
+  cnn.release();
+  configured = predicted = have_p = false;
+}
+
+template <class T>
+static bool upload(DBuf& d, const std::vector<T>& h, hipStream_t st) {
+  if (!d.ensure(sizeof(T) * std::max<size_t>(h.size(), 1))) return false;
+  if (!h.empty()) EMC_CHECK(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+  return !failed();
+}
+
+static void sync_stream() { EMC_CHECK(hipStreamSynchronize(engine().stream)); }
+
+// Entries of each cell (patterns ascending), E-step work items, 32-entry
+// chunks of the M-step, candidate rotation count of each pattern.
+static bool build_lists(Proposal& p) {
+  Engine& e = engine();
+  const int N = p.N, K = p.topk, C = p.C;
+  p.h_ent_off.assign(C + 1, 0);
+  for (size_t t = 0; t < p.h_cand.size(); t++) p.h_ent_off[p.h_cand[t] + 1]++;
+  for (int c = 0; c < C; c++) p.h_ent_off[c + 1] += p.h_ent_off[c];
+  p.n_entries = p.h_ent_off[C];
+  std::vector<int> fill(p.h_ent_off.begin(), p.h_ent_off.end() - 1), pat(p.n_entries), jj(p.n_entries);
+  p.h_ncand.assign(N, 0);
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < K; j++) {
+      const int c = p.h_cand[(size_t)i * K + j];
+      const int at = fill[c]++;
+      pat[at] = i;
+      jj[at] = j;
+      p.h_ncand[i] += p.h_cell_off[c + 1] - p.h_cell_off[c];
+    }
+  std::vector<int2> work;
+  p.h_chunk_off.assign(C + 1, 0);
+  for (int c = 0; c < C; c++) {
+    for (int e0 = p.h_ent_off[c]; e0 < p.h_ent_off[c + 1]; e0 += kCandRows) work.push_back(make_int2(c, e0));
+    p.h_chunk_off[c + 1] = p.h_chunk_off[c] + (p.h_ent_off[c + 1] - p.h_ent_off[c] + 31) / 32;
+  }
+  p.n_work = (int)work.size();
+  p.n_chunks = p.h_chunk_off[C];
+  if (!upload(p.d_ent_off, p.h_ent_off, e.stream) || !upload(p.d_ent_pat, pat, e.stream) ||
+      !upload(p.d_ent_j, jj, e.stream) || !upload(p.d_work, work, e.stream) ||
+      !upload(p.d_chunk_off, p.h_chunk_off, e.stream) || !upload(p.d_cand, p.h_cand, e.stream))
+    return false;
+  if (!p.d_Lc.ensure(sizeof(float) * (size_t)N * K * kSlot) ||
+      !p.d_panels.ensure(2 * (size_t)std::max<long>(p.n_chunks, 1) * 2 * kSlot * 32) ||
+      !p.d_plogp.ensure(sizeof(double) * N) || !p.d_crowscale.ensure(sizeof(float) * p.R))
+    return false;
+  sync_stream();
+  return !failed();
+}
+
+static void reset_modes(Proposal& p) {
+  p.h_mode.assign(p.N, 0);
+  p.st_gated = 0;
+  if (p.gate_predicted)
+    for (int i = 0; i < p.N; i++)
+      if (p.h_pmass[i] < p.min_mass) {
+        p.h_mode[i] = 1;
+        p.st_gated++;
+      }
+}
+
+bool proposal_applies(int N, int R) {
+  const Proposal& p = proposal();
+  return p.active && p.predicted && p.N == N && p.R == R && engine().precision == PREC_FP16;
+}
+
+bool proposal_estep(int N, int R, float sigma, const float* phi_used) {
+  Engine& e = engine();
+  Proposal& p = proposal();
+  hipStream_t st = e.stream;
+  p.iteration++;
+  p.have_p = false;
+  // whole-table subset: the audit sample and the patterns in full search
+  p.h_sub.clear();
+  p.h_insub.assign(N, 0);
+  p.h_audit.assign(N, 0);
+  p.st_full = 0;
+  p.st_audited = 0;
+  for (int i = 0; i < N; i++) {
+    const bool aud = emc_uniform(p.seed, 0xA0D17ULL, (uint64_t)p.iteration, (uint64_t)i) < (double)p.audit_frac;
+    const bool full = p.h_mode[i] != 0;
+    if (aud || full) {
+      p.h_insub[i] = 1;
+      p.h_audit[i] = aud && !full;
+      p.h_sub.push_back(i);
+      p.st_full += full;
+      p.st_audited += aud && !full;
+    }
+  }
+  p.n_sub = (int)p.h_sub.size();
+  p.n_sub_pad = (int)round_up(std::max(p.n_sub, 1), kPadN);
+  if (!upload(p.d_sub, p.h_sub, st) || !upload(p.d_insub, p.h_insub, st)) return false;
+  if (!p.d_zero.ensure(2 * (size_t)e.n_mp)) return false;
+  launch_fill(p.d_zero.as<float>(), (size_t)e.n_mp / 2, 0.0f, st);
+  const bool split = !e.k_exact;
+  const double inv_2nms2 = 1.0 / (2.0 * (double)(float)e.n_m * ((double)sigma * (double)sigma));
+  {
+    PhaseTimer pt(PH_LIKE);
+    CandEstepArgs a{};
+    a.K = e.d_K16.as<_Float16>();
+    a.Klo = split ? e.d_K16lo.as<_Float16>() : nullptr;
+    a.S = e.d_S16.as<_Float16>();
+    a.Slo = e.d_S16lo.as<_Float16>();
+    a.zero = p.d_zero.as<_Float16>();
+    a.work = p.d_work.as<int2>();
+    a.n_work = p.n_work;
+    a.cell_off = p.d_cell_off.as<int>();
+    a.cell_rot = p.d_cell_rot.as<int>();
+    a.ent_off = p.d_ent_off.as<int>();
+    a.ent_pat = p.d_ent_pat.as<int>();
+    a.ent_j = p.d_ent_j.as<int>();
+    a.A = e.d_A.as<double>();
+    a.B = e.d_B.as<double>();
+    a.phi = phi_used;
+    a.sscale = e.d_Sscale.as<float>();
+    a.kscale_inv = 1.0f / e.k_scale;
+    a.inv_2nms2 = inv_2nms2;
+    a.n_mp = e.n_mp;
+    a.topk = p.topk;
+    a.Lc = p.d_Lc.as<float>();
+    launch_cand_estep(split, a, st);
+  }
+  if (p.n_sub > 0) {
+    PhaseTimer pt(PH_AUDIT);
+    const size_t rows = (size_t)p.n_sub_pad * e.n_mp;
+    if (!p.d_Ksub.ensure(2 * rows) || (split && !p.d_Ksublo.ensure(2 * rows)) ||
+        !p.d_Bsub.ensure(sizeof(double) * p.n_sub) || !p.d_phisub.ensure(sizeof(float) * p.n_sub) ||
+        !p.d_Lsub.ensure(sizeof(float) * (size_t)R * p.n_sub))
+      return false;
+    launch_gather_rows(e.d_K16.as<_Float16>(), split ? e.d_K16lo.as<_Float16>() : nullptr, p.d_sub.as<int>(), p.n_sub,
+                       p.n_sub_pad, e.n_mp, p.d_Ksub.as<_Float16>(), split ? p.d_Ksublo.as<_Float16>() : nullptr, st);
+    launch_gather_f64(e.d_B.as<double>(), p.d_sub.as<int>(), p.n_sub, p.d_Bsub.as<double>(), st);
+    launch_gather_f32(phi_used, p.d_sub.as<int>(), p.n_sub, p.d_phisub.as<float>(), st);
+    LikeEpilogue ep{};
+    ep.L = p.d_Lsub.as<float>();
+    ep.A = e.d_A.as<double>();
+    ep.B = p.d_Bsub.as<double>();
+    ep.phi = p.d_phisub.as<float>();
+    ep.cscale = e.d_Sscale.as<float>();
+    ep.kscale_inv = 1.0f / e.k_scale;
+    ep.inv_2nms2 = inv_2nms2;
+    ep.N = p.n_sub;
+    ep.R = R;
+    ep.vecL = p.n_sub % 4 == 0;
+    launch_estep16(split, p.d_Ksub.p, p.d_Ksublo.p, e.d_S16.p, e.d_S16lo.p, p.n_sub_pad, e.R_pad, e.n_mp, ep, st);
+  }
+  return !failed();
+}
+
+bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* total) {
+  Engine& e = engine();
+  Proposal& p = proposal();
+  hipStream_t st = e.stream;
+  if (!p.d_tot.ensure(sizeof(double) * 2) || !p.d_mass.ensure(sizeof(float) * std::max(p.n_sub, 1))) return false;
+  {
+    PhaseTimer pt(PH_SOFTMAX);
+    launch_fill(respons, (size_t)N * R, 0.0f, st);
+    CandSoftmaxArgs a{};
+    a.Lc = p.d_Lc.as<float>();
+    a.cand = p.d_cand.as<int>();
+    a.cell_off = p.d_cell_off.as<int>();
+    a.cell_rot = p.d_cell_rot.as<int>();
+    a.insub = p.d_insub.as<unsigned char>();
+    a.A = e.d_A.as<double>();
+    a.B = e.d_B.as<double>();
+    a.phi = e.d_phi_used.as<float>();
+    a.sigma2x = sigma2x;
+    a.N = N;
+    a.topk = p.topk;
+    a.respons = respons;
+    a.phiden = e.d_phiden.as<double>();
+    a.plogp = p.d_plogp.as<double>();
+    launch_cand_softmax(a, st);
+    launch_sum_f64(p.d_plogp.as<double>(), N, p.d_tot.as<double>(), st);
+  }
+  double tot[2] = {0.0, 0.0};
+  std::vector<float> mass(p.n_sub);
+  if (p.n_sub > 0) {
+    PhaseTimer pt(PH_AUDIT);
+    const int ns = p.n_sub;
+    const int chunks = softmax_chunks(ns, R);
+    const size_t cn = (size_t)chunks * ns;
+    const int nb = (ns + 255) / 256;
+    if (!e.d_smax.ensure(std::max(e.d_smax.bytes, cn * 4)) || !e.d_ssum.ensure(std::max(e.d_ssum.bytes, cn * 8)) ||
+        !e.d_sacc.ensure(std::max(e.d_sacc.bytes, 4 * cn * 8)) || !p.d_phiden_sub.ensure(sizeof(double) * ns) ||
+        !e.d_red.ensure(std::max(e.d_red.bytes, sizeof(double) * (nb + 1))))
+      return false;
+    launch_softmax(p.d_Lsub.as<float>(), R, ns, chunks, e.d_smax.as<float>(), e.d_ssum.as<double>(),
+                   e.d_sacc.as<double>(), e.d_A.as<double>(), true, st);
+    launch_softmax_reduce(ns, chunks, e.d_sacc.as<double>(), true, p.d_Bsub.as<double>(), p.d_phisub.as<float>(),
+                          sigma2x, p.d_phiden_sub.as<double>(), e.d_red.as<double>(), st);
+    launch_sub_finish(p.d_Lsub.as<float>(), ns, p.d_sub.as<int>(), p.d_phiden_sub.as<double>(),
+                      e.d_phiden.as<double>(), p.d_cand.as<int>(), p.topk, p.d_cell_off.as<int>(),
+                      p.d_cell_rot.as<int>(), p.d_mass.as<float>(), st);
+    launch_sub_scatter(p.d_Lsub.as<float>(), R, ns, p.d_sub.as<int>(), N, respons, st);
+    EMC_CHECK(hipMemcpyAsync(&tot[1], e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, st));
+    EMC_CHECK(hipMemcpyAsync(mass.data(), p.d_mass.p, sizeof(float) * ns, hipMemcpyDeviceToHost, st));
+  }
+  EMC_CHECK(hipMemcpyAsync(&tot[0], p.d_tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  sync_stream();
+  if (failed()) return false;
+  // search mode of every pattern scored on the whole table: full search
+  // while its candidates hold less than min_mass of its posterior
+  double fails = 0, mn = 1.0, sum = 0.0;
+  for (int k = 0; k < p.n_sub; k++) {
+    const int i = p.h_sub[k];
+    const bool low = mass[k] < p.min_mass;
+    if (p.h_audit[i]) {
+      fails += low;
+      mn = std::min(mn, (double)mass[k]);
+      sum += mass[k];
+    }
+    p.h_mode[i] = low ? 1 : 0;
+  }
+  p.st_fail = fails;
+  p.st_min_mass = p.st_audited > 0 ? mn : 1.0;
+  p.st_mean_mass = p.st_audited > 0 ? sum / p.st_audited : 1.0;
+  double pairs = (double)p.n_sub * R;
+  for (int i = 0; i < N; i++)
+    if (!p.h_insub[i]) pairs += p.h_ncand[i];
+  p.st_pairs = pairs;
+  *total = tot[0] + tot[1];
+  p.have_p = true;
+  p.respons = respons;
+  return true;
+}
+
+bool proposal_mstep(int N, int R, float* slices, int N2d) {
+  (void)N;
+  Engine& e = engine();
+  Proposal& p = proposal();
+  hipStream_t st = e.stream;
+  const bool split = !e.k_exact;
+  const bool has_sub = p.n_sub > 0;
+  if (has_sub) {
+    PhaseTimer pt(PH_AUDIT);
+    const size_t pk = (size_t)e.R_pad * p.n_sub_pad, kt = (size_t)e.n_mp * p.n_sub_pad;
+    if (!p.d_invphisub.ensure(sizeof(float) * p.n_sub) || !p.d_Psub16.ensure(2 * pk) || !p.d_Psub16lo.ensure(2 * pk) ||
+        !p.d_Ktsub.ensure(2 * kt) || (split && !p.d_Ktsublo.ensure(2 * kt)))
+      return false;
+    launch_gather_f32(e.d_invphi.as<float>(), p.d_sub.as<int>(), p.n_sub, p.d_invphisub.as<float>(), st);
+    launch_rowsum(p.d_Lsub.as<float>(), R, p.n_sub, p.d_invphisub.as<float>(), e.d_T.as<double>(), e.d_Tf.as<float>(),
+                  e.d_Pscale.as<float>(), true, st);
+    MstepArgs ms{};
+    ms.P = p.d_Lsub.as<float>();
+    ms.invphi = p.d_invphisub.as<float>();
+    ms.rowscale = e.d_Pscale.as<float>();
+    ms.Tf = e.d_Tf.as<float>();
+    ms.pix = e.d_pix.as<int>();
+    ms.slices = slices;
+    ms.Sc = e.d_Sc.as<float>();
+    ms.kscale_inv = 1.0f / e.k_scale;
+    ms.N = p.n_sub;
+    ms.R = R;
+    ms.N_pad = p.n_sub_pad;
+    ms.R_pad = e.R_pad;
+    ms.n_mp = e.n_mp;
+    ms.n_m = e.n_m;
+    ms.N2d = N2d;
+    ms.vecP = p.n_sub % 4 == 0;
+    ms.raw = 1;
+    launch_convert_p(ms, p.d_Psub16.as<_Float16>(), p.d_Psub16lo.as<_Float16>(), st);
+    launch_gather_rows_t(e.d_K16.as<_Float16>(), p.d_sub.as<int>(), p.n_sub, p.n_sub_pad, e.n_mp,
+                         p.d_Ktsub.as<_Float16>(), st);
+    if (split)
+      launch_gather_rows_t(e.d_K16lo.as<_Float16>(), p.d_sub.as<int>(), p.n_sub, p.n_sub_pad, e.n_mp,
+                           p.d_Ktsublo.as<_Float16>(), st);
+    launch_mstep16(split, p.d_Psub16.p, p.d_Psub16lo.p, p.d_Ktsub.p, p.d_Ktsublo.p, ms, st);
+  }
+  {
+    PhaseTimer pt(PH_ROWSUM);
+    CandPrepArgs a{};
+    a.Pc = p.d_Lc.as<float>();
+    a.ent_off = p.d_ent_off.as<int>();
+    a.ent_pat = p.d_ent_pat.as<int>();
+    a.ent_j = p.d_ent_j.as<int>();
+    a.cell_off = p.d_cell_off.as<int>();
+    a.cell_rot = p.d_cell_rot.as<int>();
+    a.chunk_off = p.d_chunk_off.as<long>();
+    a.invphi = e.d_invphi.as<float>();
+    a.Tsub = has_sub ? e.d_T.as<double>() : nullptr;
+    a.T = e.d_T.as<double>();
+    a.Tf = e.d_Tf.as<float>();
+    a.rowscale = p.d_crowscale.as<float>();
+    a.panels = p.d_panels.as<_Float16>();
+    a.C = p.C;
+    a.topk = p.topk;
+    launch_cand_prep(a, st);
+  }
+  {
+    PhaseTimer pt(PH_MSTEP);
+    CandMstepArgs a{};
+    a.K = e.d_K16.as<_Float16>();
+    a.Klo = split ? e.d_K16lo.as<_Float16>() : nullptr;
+    a.zero = p.d_zero.as<_Float16>();
+    a.panels = p.d_panels.as<_Float16>();
+    a.chunk_off = p.d_chunk_off.as<long>();
+    a.ent_off = p.d_ent_off.as<int>();
+    a.ent_pat = p.d_ent_pat.as<int>();
+    a.cell_off = p.d_cell_off.as<int>();
+    a.cell_rot = p.d_cell_rot.as<int>();
+    a.pix = e.d_pix.as<int>();
+    a.rowscale = p.d_crowscale.as<float>();
+    a.Tf = e.d_Tf.as<float>();
+    a.Sc = e.d_Sc.as<float>();
+    a.slices = slices;
+    a.kscale_inv = 1.0f / e.k_scale;
+    a.has_sub = has_sub;
+    a.C = p.C;
+    a.n_ptiles = e.n_mp / 256;
+    a.n_mp = e.n_mp;
+    a.n_m = e.n_m;
+    a.N2d = N2d;
+    launch_cand_mstep(split, a, st);
+  }
+  p.have_p = false;
+  return !failed();
 }
 
 }  // namespace emc
@@ -103,6 +433,206 @@ int emc_hip_cnn_step(const real* d_images, const int* d_mask, int N2d, const int
   t.release();
   ti.release();
   return failed() ? 1 : 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// proposal workflow
+// ---------------------------------------------------------------------------
+int emc_hip_proposal_init(const real* q_fine, int n_rot, int n_coarse, int side, int topk, int batch,
+                          unsigned long long seed) {
+  if (!ready()) return 1;
+  Proposal& p = proposal();
+  Engine& e = engine();
+  if (!q_fine || n_rot <= 0 || n_coarse < 1 || topk <= 0 || batch <= 0) {
+    set_error("emc_hip_proposal_init: invalid argument");
+    return 1;
+  }
+  const int C = emc_n_to_samples(n_coarse) / 2;
+  std::vector<float> qc(4 * (size_t)C), wc(C);
+  if (emc_rotation_table(n_coarse, 1, qc.data(), wc.data()) != C) {
+    set_error("emc_hip_proposal_init: coarse table n=%d failed", n_coarse);
+    return 1;
+  }
+  if (topk > C) {
+    set_error("emc_hip_proposal_init: topk %d exceeds the %d coarse cells", topk, C);
+    return 1;
+  }
+  std::vector<int> cell(n_rot);
+  emc_assign_cells(q_fine, n_rot, qc.data(), C, cell.data());
+  std::vector<int> off(C + 1, 0), rot(n_rot);
+  for (int r = 0; r < n_rot; r++) off[cell[r] + 1]++;
+  int largest = 0;
+  for (int c = 0; c < C; c++) {
+    largest = std::max(largest, off[c + 1]);
+    off[c + 1] += off[c];
+  }
+  if (largest > kSlot) {
+    set_error("emc_hip_proposal_init: a coarse cell holds %d fine rotations (at most %d); use a finer coarse table",
+              largest, kSlot);
+    return 1;
+  }
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int r = 0; r < n_rot; r++) rot[fill[cell[r]]++] = r;
+  p.release();
+  p.C = C;
+  p.R = n_rot;
+  p.topk = topk;
+  p.batch = batch;
+  p.seed = seed;
+  p.iteration = 0;
+  p.h_cell_off = off;
+  p.h_cell_rot = rot;
+  if (!upload(p.d_cell_off, off, e.stream) || !upload(p.d_cell_rot, rot, e.stream)) return 1;
+  sync_stream();
+  if (!p.cnn.init(side, C, batch, seed)) return 1;
+  p.configured = true;
+  return failed() ? 1 : 0;
+}
+
+int emc_hip_proposal_cells(int* cell_off, int* cell_rot) {
+  const Proposal& p = proposal();
+  if (!p.configured) return -1;
+  if (cell_off) std::copy(p.h_cell_off.begin(), p.h_cell_off.end(), cell_off);
+  if (cell_rot) std::copy(p.h_cell_rot.begin(), p.h_cell_rot.end(), cell_rot);
+  return p.C;
+}
+
+int emc_hip_proposal_train(const real* d_images, const int* d_mask, int N_2d, const real* d_respons, int N_images,
+                           int N_slices, int epochs, float lr, double* loss) {
+  if (!ready()) return 1;
+  Proposal& p = proposal();
+  Engine& e = engine();
+  Cnn& n = p.cnn;
+  if (!p.configured || !d_images || !d_mask || !d_respons || N_images <= 0 || N_slices != p.R || epochs < 0 ||
+      N_2d != n.side * n.side) {
+    set_error("emc_hip_proposal_train: invalid argument (proposal configured for %d rotations, side %d)", p.R,
+              n.side);
+    return 1;
+  }
+  hipStream_t st = e.stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int N = N_images, B = p.batch;
+  DBuf T, perm;
+  if (!T.ensure(sizeof(float) * (size_t)p.C * N) || !perm.ensure(sizeof(int) * N)) return 1;
+  coarse_targets(d_respons, N, p.d_cell_off.as<int>(), p.d_cell_rot.as<int>(), p.C, T.as<float>(), st);
+  std::vector<int> order(N);
+  std::iota(order.begin(), order.end(), 0);
+  std::mt19937_64 gen(p.seed ^ (0x9e3779b97f4a7c15ULL * (uint64_t)(n.step + 1)));
+  double last = 0.0;
+  for (int ep = 0; ep < epochs; ep++) {
+    std::shuffle(order.begin(), order.end(), gen);
+    EMC_CHECK(hipMemcpyAsync(perm.p, order.data(), sizeof(int) * N, hipMemcpyHostToDevice, st));
+    launch_fill(n.loss.as<float>(), 2, 0.0f, st);
+    for (int b0 = 0; b0 < N; b0 += B) {
+      const int nb = std::min(B, N - b0);
+      const int* idx = perm.as<int>() + b0;
+      n.forward(d_images, d_mask, N_2d, idx, nb, st);
+      softmax_xent(n.logits.as<float>(), p.C, nb, T.as<float>(), N, idx, 1.0f / nb, n.dlogits.as<float>(),
+                   n.loss.as<double>(), st);
+      n.backward_and_step(nb, lr, st);
+    }
+    EMC_CHECK(hipMemcpyAsync(&last, n.loss.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    sync_stream();
+    if (failed()) break;
+    last /= N;
+  }
+  T.release();
+  perm.release();
+  p.st_train_loss = last;
+  p.st_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (loss) *loss = last;
+  return failed() ? 1 : 0;
+}
+
+int emc_hip_proposal_predict(const real* d_images, const int* d_mask, int N_2d, int N_images) {
+  if (!ready()) return 1;
+  Proposal& p = proposal();
+  Engine& e = engine();
+  Cnn& n = p.cnn;
+  if (!p.configured || !d_images || !d_mask || N_images <= 0 || N_2d != n.side * n.side) {
+    set_error("emc_hip_proposal_predict: invalid argument");
+    return 1;
+  }
+  hipStream_t st = e.stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int N = N_images, B = p.batch;
+  std::vector<int> iota(N);
+  std::iota(iota.begin(), iota.end(), 0);
+  DBuf idx;
+  if (!upload(idx, iota, st) || !p.d_cand.ensure(sizeof(int) * (size_t)N * p.topk) ||
+      !p.d_pmass.ensure(sizeof(float) * N))
+    return 1;
+  for (int b0 = 0; b0 < N; b0 += B) {
+    const int nb = std::min(B, N - b0);
+    n.forward(d_images, d_mask, N_2d, idx.as<int>() + b0, nb, st);
+    topk(n.logits.as<float>(), p.C, nb, p.topk, nullptr, p.d_cand.as<int>() + (size_t)b0 * p.topk,
+         p.d_pmass.as<float>() + b0, st);
+  }
+  p.N = N;
+  p.h_cand.resize((size_t)N * p.topk);
+  p.h_pmass.resize(N);
+  EMC_CHECK(hipMemcpyAsync(p.h_cand.data(), p.d_cand.p, sizeof(int) * p.h_cand.size(), hipMemcpyDeviceToHost, st));
+  EMC_CHECK(hipMemcpyAsync(p.h_pmass.data(), p.d_pmass.p, sizeof(float) * N, hipMemcpyDeviceToHost, st));
+  sync_stream();
+  idx.release();
+  if (failed()) return 1;
+  reset_modes(p);
+  if (!build_lists(p)) return 1;
+  p.predicted = true;
+  p.iteration = 0;
+  p.st_predict_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int emc_hip_proposal_set_candidates(const int* cand, int N_images, int topk) {
+  if (!ready()) return 1;
+  Proposal& p = proposal();
+  if (!p.configured || !cand || N_images <= 0 || topk <= 0 || topk > p.C) {
+    set_error("emc_hip_proposal_set_candidates: invalid argument");
+    return 1;
+  }
+  std::vector<char> seen(p.C, 0);
+  for (int i = 0; i < N_images; i++) {
+    for (int j = 0; j < topk; j++) {
+      const int c = cand[(size_t)i * topk + j];
+      if (c < 0 || c >= p.C || seen[c]) {
+        set_error("emc_hip_proposal_set_candidates: pattern %d: cell %d out of range or repeated", i, c);
+        return 1;
+      }
+      seen[c] = 1;
+    }
+    for (int j = 0; j < topk; j++) seen[cand[(size_t)i * topk + j]] = 0;
+  }
+  p.N = N_images;
+  p.topk = topk;
+  p.h_cand.assign(cand, cand + (size_t)N_images * topk);
+  p.h_pmass.assign(N_images, 1.0f);
+  reset_modes(p);
+  if (!build_lists(p)) return 1;
+  p.predicted = true;
+  p.iteration = 0;
+  return 0;
+}
+
+int emc_hip_proposal_get(int* cand, int* mode, float* mass, int N_images) {
+  const Proposal& p = proposal();
+  if (!p.predicted || N_images != p.N) return 1;
+  if (cand) std::copy(p.h_cand.begin(), p.h_cand.end(), cand);
+  if (mode)
+    for (int i = 0; i < p.N; i++) mode[i] = p.h_mode[i];
+  if (mass) std::copy(p.h_pmass.begin(), p.h_pmass.end(), mass);
+  return 0;
+}
+
+int emc_hip_proposal_stats(double* out, int n) {
+  const Proposal& p = proposal();
+  const double v[12] = {(double)p.iteration, p.st_full,     p.st_audited,  p.st_fail,
+                        p.st_min_mass,       p.st_mean_mass, p.st_pairs,    p.st_gated,
+                        p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub};
+  int k = 0;
+  for (; k < n && k < 12; k++) out[k] = v[k];
+  return k;
 }
 
 }  // extern "C"

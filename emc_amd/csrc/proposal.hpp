@@ -1,4 +1,5 @@
-// State of the learned rotation proposal.
+// State of the learned rotation proposal and the candidate-restricted
+// iteration.
 #pragma once
 
 #include <cstdint>
@@ -12,49 +13,61 @@ namespace emc {
 struct Proposal {
   Cnn cnn;
   bool configured = false;
-  bool predicted = false;
-  int C = 0;           // coarse cells
-  int R = 0;           // fine rotations
-  int N = 0;           // patterns (set by predict)
+  bool predicted = false;  // candidates present
+  bool active = false;     // option "proposal" on
+  int C = 0;               // coarse cells
+  int R = 0;               // fine rotations
+  int N = 0;               // patterns of the candidate lists
   int topk = 32;
-  int slot = 32;       // per-cell rotation slots (>= largest cell, multiple of 16)
+  int batch = 256;         // network batch
   float audit_frac = 0.02f;
   float min_mass = 0.999f;
+  bool gate_predicted = true;  // full search when the network's top-k mass < min_mass
   uint64_t seed = 0;
-  int iteration = 0;   // candidate-restricted iterations run
+  int iteration = 0;       // candidate-restricted E-steps run
 
-  // coarse cells: fine rotations of each cell (CSR), cell of each fine rotation
+  // coarse cells: fine rotations of each cell (CSR)
   std::vector<int> h_cell_off, h_cell_rot;
-  DBuf d_cell_off, d_cell_rot, d_rot_cell, d_rot_slot;
-  // per pattern: top-k cells (row-major [N][topk]) and search mode (1 = full)
-  std::vector<int> h_cand, h_mode;
-  DBuf d_cand;
-  // work lists of the candidate-restricted contractions, rebuilt when the
-  // candidates or the modes change
-  bool lists_valid = false;
-  std::vector<int> h_ent_off;        // [C+1] entries per cell
-  DBuf d_ent_off, d_ent_pat, d_ent_j;  // entry -> pattern, position in its top-k
-  DBuf d_work;                       // int2 (cell, first entry) per E-step workgroup
+  DBuf d_cell_off, d_cell_rot;
+  // per pattern: candidate cells [N][topk], network top-k mass, search mode
+  std::vector<int> h_cand, h_ncand;
+  std::vector<float> h_pmass;
+  std::vector<unsigned char> h_mode;  // 1 = whole table
+  DBuf d_cand, d_pmass;
+  // entries per cell (CSR, patterns ascending) and E-step work items
+  std::vector<int> h_ent_off;
+  std::vector<long> h_chunk_off;
+  DBuf d_ent_off, d_ent_pat, d_ent_j, d_work, d_chunk_off;
   int n_work = 0;
-  long n_entries = 0;
-  // candidate log-likelihoods -> responsibilities [N][topk][slot]
-  DBuf d_Lc;
-  // patterns scored on the whole table this iteration (audit + full search)
+  long n_entries = 0, n_chunks = 0;
+  DBuf d_Lc;      // float [N][topk][32]: L, then P
+  DBuf d_panels;  // half [chunks][2][32][32] M-step operand
+  DBuf d_crowscale;
+  DBuf d_plogp;   // double [N]
+  DBuf d_zero;    // zero halves (gather target of padding rows)
+  // whole-table subset of this iteration (audited + full search)
   std::vector<int> h_sub;
+  std::vector<unsigned char> h_insub, h_audit;
   int n_sub = 0, n_sub_pad = 0;
-  std::vector<unsigned char> h_audit;  // pattern audited this iteration
-  DBuf d_sub, d_sub_is_audit, d_Ksub, d_Ktsub, d_Bsub, d_phisub, d_Lsub, d_Psub16, d_Psub16lo, d_invphisub;
-  DBuf d_mass;          // captured mass of audited patterns
-  DBuf d_psum;          // per-pattern sum P log P (candidate patterns)
-  DBuf d_zero_row;      // n_mp zero halves (gather target for padding rows)
-  DBuf d_U;             // float [R_pad][n_mp] accumulators of the subset M-step
-  bool have_estep = false;
-  double last_total = 0.0;
+  DBuf d_sub, d_insub, d_Ksub, d_Ksublo, d_Ktsub, d_Ktsublo, d_Bsub, d_phisub, d_invphisub, d_Lsub, d_phiden_sub,
+      d_Psub16, d_Psub16lo, d_mass, d_tot;
+  // state shared between the calls of one iteration
+  const float* respons = nullptr;  // buffer of the current proposal responsibilities
+  bool have_p = false;             // softmax done, M-step may run on the candidates
   // statistics of the last iteration
   double st_full = 0, st_audited = 0, st_fail = 0, st_min_mass = 1, st_mean_mass = 1, st_pairs = 0,
-         st_train_loss = 0;
+         st_gated = 0, st_train_loss = 0, st_train_ms = 0, st_predict_ms = 0;
+
+  void release();
 };
 
 Proposal& proposal();
+
+// Candidate-restricted steps, called by the emc.h entry points when the
+// proposal is active for (N, R).  Packed patterns and slices are current.
+bool proposal_applies(int N, int R);
+bool proposal_estep(int N, int R, float sigma, const float* phi_used);
+bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* total);
+bool proposal_mstep(int N, int R, float* slices, int N2d);
 
 }  // namespace emc

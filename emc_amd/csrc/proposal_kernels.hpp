@@ -1,0 +1,83 @@
+// Launch arguments of the candidate-restricted iteration's kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+
+namespace emc {
+
+constexpr int kSlot = 32;       // rotation slots per coarse cell (largest cell must fit)
+constexpr int kCandRows = 128;  // entries per E-step workgroup
+
+struct CandEstepArgs {
+  const _Float16 *K, *Klo;          // [N_pad][n_mp] packed counts (lo only when split)
+  const _Float16 *S, *Slo;          // [R_pad][n_mp] packed slices hi / lo
+  const _Float16* zero;             // >= n_mp zero halves
+  const int2* work;                 // (cell, first entry) per workgroup
+  int n_work;
+  const int *cell_off, *cell_rot;   // fine rotations of each cell (CSR)
+  const int *ent_off, *ent_pat, *ent_j;  // entries of each cell (CSR)
+  const double *A, *B;              // sum_p m S^2 [R], sum_p m K^2 [N]
+  const float* phi;                 // [N] scaling used by the likelihood
+  const float* sscale;              // [R] 1 / FP16 slice row scale
+  float kscale_inv;
+  double inv_2nms2;
+  int n_mp, topk;
+  float* Lc;                        // [N][topk][kSlot]
+};
+
+struct CandSoftmaxArgs {
+  float* Lc;                        // L in, P out
+  const int* cand;                  // [N][topk]
+  const int *cell_off, *cell_rot;
+  const unsigned char* insub;       // [N] pattern scored on the whole table
+  const double *A, *B;
+  const float* phi;
+  double sigma2x;                   // 2 n_m sigma^2
+  int N, topk;
+  float* respons;                   // dense [R][N] (zeroed) or nullptr
+  double *phiden, *plogp;           // [N]
+};
+
+struct CandPrepArgs {
+  const float* Pc;                  // [N][topk][kSlot]
+  const int *ent_off, *ent_pat, *ent_j, *cell_off, *cell_rot;
+  const long* chunk_off;            // [C] first 32-entry chunk of each cell
+  const float* invphi;
+  const double* Tsub;               // [R] subset row sums or nullptr
+  double* T;
+  float *Tf, *rowscale;
+  _Float16* panels;                 // [chunks][2][kSlot][32]
+  int C, topk;
+};
+
+struct CandMstepArgs {
+  const _Float16 *K, *Klo;
+  const _Float16* zero;
+  const _Float16* panels;
+  const long* chunk_off;
+  const int *ent_off, *ent_pat, *cell_off, *cell_rot, *pix;
+  const float *rowscale, *Tf;
+  float* Sc;                        // [R_pad][n_mp]: subset U in (if has_sub), S' out
+  float* slices;                    // [R][N2d]
+  float kscale_inv;
+  int has_sub, C, n_ptiles, n_mp, n_m, N2d;
+};
+
+void launch_cand_estep(bool ksplit, const CandEstepArgs& a, hipStream_t st);
+void launch_cand_softmax(const CandSoftmaxArgs& a, hipStream_t st);
+void launch_cand_prep(const CandPrepArgs& a, hipStream_t st);
+void launch_cand_mstep(bool ksplit, const CandMstepArgs& a, hipStream_t st);
+void launch_gather_rows(const _Float16* K, const _Float16* Klo, const int* sub, int n_sub, int n_sub_pad, int n_mp,
+                        _Float16* Ks, _Float16* Kslo, hipStream_t st);
+void launch_gather_rows_t(const _Float16* K, const int* sub, int n_sub, int n_sub_pad, int n_mp, _Float16* Kt,
+                          hipStream_t st);
+void launch_gather_f64(const double* v, const int* sub, int n, double* out, hipStream_t st);
+void launch_gather_f32(const float* v, const int* sub, int n, float* out, hipStream_t st);
+void launch_sub_finish(const float* Psub, int n_sub, const int* sub, const double* phiden_sub, double* phiden,
+                       const int* cand, int topk, const int* cell_off, const int* cell_rot, float* mass,
+                       hipStream_t st);
+void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int N, float* respons, hipStream_t st);
+
+}  // namespace emc

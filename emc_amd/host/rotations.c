@@ -354,3 +354,23 @@ int generate_rotation_list(const int n, Quaternion ***return_list, real **return
   *return_weights = w;
   return ns;
 }
+
+int emc_assign_cells(const float *q_fine, int n_fine, const float *q_coarse, int n_coarse, int *cell) {
+  if (!q_fine || !q_coarse || !cell || n_fine < 0 || n_coarse <= 0) return -1;
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < n_fine; r++) {
+    const float *a = &q_fine[4 * (size_t)r];
+    double best = -1.0;
+    int bi = 0;
+    for (int c = 0; c < n_coarse; c++) {
+      const float *b = &q_coarse[4 * (size_t)c];
+      const double d = fabs((double)a[0] * b[0] + (double)a[1] * b[1] + (double)a[2] * b[2] + (double)a[3] * b[3]);
+      if (d > best) {
+        best = d;
+        bi = c;
+      }
+    }
+    cell[r] = bi;
+  }
+  return 0;
+}

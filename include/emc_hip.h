@@ -62,6 +62,18 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *   "timing"     "1" records per-phase GPU time (adds a synchronisation per
  *                 entry point), "0" off.
  *   "verbose"    "1" prints the reference's per-call diagnostics.
+ *   "proposal"   "on": once candidates exist for this pattern count (see
+ *                 the proposal workflow below) and precision is "fp16", the
+ *                 E-step scores each pattern only against the fine rotations
+ *                 of its candidate coarse cells; "off" (default).
+ *   "audit_fraction"  fraction of patterns (default "0.02") scored on the
+ *                 whole table every proposal iteration, chosen at random.
+ *   "min_mass"   captured posterior mass (default "0.999") below which a
+ *                 pattern returns to whole-table search.
+ *   "proposal_gate"  "predicted" (default): a pattern also starts in
+ *                 whole-table search when the network's top-k cells hold less
+ *                 than min_mass of its predicted distribution; "audit": only
+ *                 whole-table scoring moves patterns to full search.
  */
 int emc_hip_set_option(const char *key, const char *value);
 
@@ -74,7 +86,8 @@ void emc_hip_invalidate(void);
  * the number of kernel launches per phase.  Phases:
  *   0 expand, 1 pack (slice operands), 2 likelihood GEMM, 3 softmax,
  *   4 scaling, 5 M-step row sums, 6 M-step GEMM, 7 insert, 8 normalise,
- *   9 other.
+ *   9 other, 10 audit (proposal mode: everything done for the patterns
+ *   scored on the whole table).
  * Returns the number of phases written (<= n). */
 int emc_hip_phase_times(double *ms, long long *launches, int n);
 void emc_hip_reset_phase_times(void);
@@ -101,6 +114,45 @@ int emc_hip_cnn_forward(const real *d_images, const int *d_mask, int N_2d, const
  * the weights, the gradients are still computed).  *loss = mean loss. */
 int emc_hip_cnn_step(const real *d_images, const int *d_mask, int N_2d, const int *idx, int B,
                      const float *targets, float lr, double *loss);
+
+/* ---- Proposal workflow (no counterpart in the reference) ----
+ * Coarse cells: the antipodal half of the sampling-n_coarse table; every
+ * fine rotation belongs to the cell of the nearest coarse rotation (at most
+ * 32 per cell).  Typical use, with the emc.h calls unchanged:
+ *   emc_hip_proposal_init(q, R, 4, side, 32, 256, seed);
+ *   iterations 1-5 (dense), each followed by
+ *     emc_hip_proposal_train(..., d_respons, ...)   (P summed per cell as targets)
+ *   emc_hip_proposal_predict(...);  emc_hip_set_option("proposal", "on");
+ *   further iterations score topk cells per pattern (+ the audit subset). */
+/* Configure for the fine table q_fine [n_rot][4] (host); (re)initialises the
+ * network for side x side patterns and the given batch. */
+int emc_hip_proposal_init(const real *q_fine, int n_rot, int n_coarse, int side, int topk, int batch,
+                          unsigned long long seed);
+/* Cells as CSR: cell_off [C+1], cell_rot [n_rot] (either may be NULL);
+ * returns C, or -1 if not configured. */
+int emc_hip_proposal_cells(int *cell_off, int *cell_rot);
+/* `epochs` passes of Adam (learning rate lr) over all patterns against the
+ * per-cell sums of the responsibilities d_respons [N_slices][N_images];
+ * *loss = mean cross-entropy of the last pass. */
+int emc_hip_proposal_train(const real *d_images, const int *d_mask, int N_2d, const real *d_respons, int N_images,
+                           int N_slices, int epochs, float lr, double *loss);
+/* Candidate cells of every pattern from the network (its topk largest
+ * outputs); resets the search modes. */
+int emc_hip_proposal_predict(const real *d_images, const int *d_mask, int N_2d, int N_images);
+/* Candidate cells given by the caller, cand [N_images][topk] (distinct per
+ * pattern); resets the search modes. */
+int emc_hip_proposal_set_candidates(const int *cand, int N_images, int topk);
+/* Current candidates [N][topk], search modes (1 = whole table) and the
+ * network's top-k mass (1 for caller-given candidates); NULL skips. */
+int emc_hip_proposal_get(int *cand, int *mode, float *mass, int N_images);
+/* Statistics of the last proposal iteration: out[0] iteration, [1] patterns
+ * in whole-table search, [2] audited patterns, [3] audited below min_mass,
+ * [4] smallest and [5] mean captured mass of the audited patterns, [6]
+ * pattern x rotation pairs scored, [7] patterns gated to full search at
+ * prediction, [8] last training loss, [9] last training time (ms), [10] last
+ * prediction time (ms), [11] patterns scored on the whole table.  Returns
+ * the number of values written (<= n). */
+int emc_hip_proposal_stats(double *out, int n);
 
 /* Information on the packed pattern operand of the last E-step:
  * out[0] = masked pixel count n_m, out[1] = padded count, out[2] = 1 if the

@@ -32,6 +32,12 @@ int emc_n_to_samples(int n);
  * -1 on failure (n < 1 or an internal consistency check failing). */
 int emc_rotation_table(int n, int half, float *q, float *w);
 
+/* Coarse cell of each fine rotation (no counterpart in the reference): the
+ * coarse entry c maximising |q_fine[r] . q_coarse[c]| (the smallest rotation
+ * angle between the two; q and -q are the same rotation), lowest c on ties.
+ * Returns 0, or -1 on invalid arguments. */
+int emc_assign_cells(const float *q_fine, int n_fine, const float *q_coarse, int n_coarse, int *cell);
+
 /* Same as generate_rotation_list (emc.c:49) with the reference's signature:
  * allocates the Quaternion* array and the weights with malloc. */
 int generate_rotation_list(const int n, Quaternion ***return_list, real **return_weights);

@@ -312,3 +312,60 @@ def o64_iteration(model, images, mask, scaling, S, q, weights, x, y, z, sigma=0.
     mn = o64_normalize_model(m, wg)
     return {"slices": sl, "L": L, "P": P, "total": tot, "scaling": phi, "slices_new": sl2, "T": T,
             "model_raw": m, "weight": wg, "model": mn}
+
+
+# ---- learned proposal: candidate-restricted iteration (numpy, FP64) ----
+def o_assign_cells(q_fine, q_coarse):
+    """Coarse cell of each fine rotation: argmax_c |q_fine[r] . q_coarse[c]|,
+    lowest c on ties, products summed in the order q0..q3 in double."""
+    a = np.asarray(q_fine, np.float64)
+    b = np.asarray(q_coarse, np.float64)
+    out = np.empty(a.shape[0], np.int64)
+    for s in range(0, a.shape[0], 2048):
+        blk = a[s:s + 2048]
+        dots = np.abs(((blk[:, None, 0] * b[None, :, 0] + blk[:, None, 1] * b[None, :, 1])
+                       + blk[:, None, 2] * b[None, :, 2]) + blk[:, None, 3] * b[None, :, 3])
+        out[s:s + 2048] = dots.argmax(axis=1)
+    return out
+
+
+def o64_restricted_softmax(L, allowed):
+    """Responsibilities of each pattern over its allowed rotations only, with
+    the reference's rule (emc_cuda.cu:445-476): m = max(-1e10, max L),
+    P = exp(L - m) where L - m > -1e10, normalised; zero where not allowed."""
+    L = np.asarray(L, np.float64)
+    allowed = np.asarray(allowed, bool)
+    Lm = np.where(allowed, L, -np.inf)
+    m = np.maximum(-1e10, Lm.max(axis=0))
+    d = Lm - m[None, :]
+    e = np.where(allowed & (d > -1e10), np.exp(np.where(allowed, d, 0.0)), 0.0)
+    return e / e.sum(axis=0, keepdims=True)
+
+
+def allowed_from_candidates(cand, cell_off, cell_rot, R, sub=None):
+    """[R][N] mask of the rotations a pattern is scored on: those of its
+    candidate cells, or every rotation for the patterns in `sub`."""
+    N = cand.shape[0]
+    A = np.zeros((R, N), bool)
+    for i in range(N):
+        for c in cand[i]:
+            A[cell_rot[cell_off[c]:cell_off[c + 1]], i] = True
+    if sub is not None:
+        A[:, np.asarray(sub, np.int64)] = True
+    return A
+
+
+def o64_proposal_iteration(model, images, mask, scaling, S, q, weights, x, y, z, allowed, sigma=0.4,
+                           interp=NEAREST):
+    """One FP64 iteration in the driver order with the responsibilities
+    restricted to `allowed` ([R][N] bool)."""
+    sl = o64_expand(model, S, q, x, y, z, interp)
+    L = o64_likelihood(sl, images, mask, sigma, scaling)
+    P = o64_restricted_softmax(L, allowed)
+    tot = o64_total_respons(P)
+    phi = o64_update_scaling(images, sl, mask, P)
+    sl2, T = o64_mstep(images, mask, P, phi, sl)
+    m, wg = o64_insert(sl2, mask, T, weights, S, q, x, y, z, interp)
+    mn = o64_normalize_model(m, wg)
+    return {"slices": sl, "L": L, "P": P, "total": tot, "scaling": phi, "slices_new": sl2, "T": T,
+            "model_raw": m, "weight": wg, "model": mn}
