@@ -1,23 +1,36 @@
 """EMC iteration throughput on one MI355X: pattern x rotation pairs per second.
 
 Workload (default): the 128^3 model / 128x128 patterns / 50,000 patterns /
-50,100 rotations configuration (n = 10 rotation table, antipodal half), with
-synthetic data.  One step = one EMC iteration exactly as the reference driver
-sequences it (emc.c:989-1084), through the emc.h C-ABI: model max, expand,
-likelihood, responsibilities, sum P log P, model reset, scaling update, M-step
-+ insertion, normalisation.  Inputs are resident in HBM before timing starts.
+50,100 rotations configuration (n = 10 rotation table, antipodal half) with
+the learned rotation proposal, on synthetic data.  One step = one EMC
+iteration exactly as the reference driver sequences it (emc.c:989-1084),
+through the emc.h C-ABI: model max, expand, likelihood, responsibilities,
+sum P log P, model reset, scaling update, M-step + insertion, normalisation.
+Inputs are resident in HBM before timing starts.
 
-Printed: one JSON line.  `value` is the fast form's (FP16 matrix cores with
-hi/lo splitting) pairs/s; the plain FP32 form (the reference's method on FP32
-matrix cores) is timed in the same run and reported beside it, with the
-correlation of the two final models.  `roofline` is the likelihood GEMM,
-timed with HIP events on the library's stream during the timed iterations;
-`cpu_baseline` is the oracle's FP32 restatement of the reference on the host
-cores, on a pattern sample, extrapolated to the full pattern count.
+Schedule of the learned form, from the random initial model: iterations
+1..T dense (FP16 hi/lo matrix cores), the proposal network trained after
+each on that iteration's responsibilities; prediction of the top-k coarse
+cells of every pattern; then proposal iterations (candidates scored exactly,
+a random 2% audited on the whole table, fallback below 0.999 captured mass).
+The timed steps are proposal iterations after `warmup` untimed ones.  The
+plain FP32 form (the reference's method on FP32 matrix cores) runs the same
+number of iterations from the same start; its last `steps` are timed; the
+final models are correlated.
+
+Printed: one JSON line.  `value` = N * R / (seconds per timed iteration), the
+effective pattern x rotation pairs per second; the pairs actually scored are
+reported beside it.  `roofline` is the dominant matrix-core kernel (the
+candidate E-step in proposal iterations, the dense likelihood GEMM
+otherwise), timed with HIP events on the library's stream during the timed
+iterations.  `cpu_baseline` is the oracle's FP32 restatement of the
+reference on the host cores, on a pattern sample, extrapolated to the full
+pattern count.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -31,6 +44,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP16 = 2.5e15     # MI355X dense FP16 MFMA, FLOP/s
 PEAK_FP32 = 157.3e12   # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12      # bytes/s
+NPHASE = 11
 
 
 def log(*a):
@@ -38,40 +52,117 @@ def log(*a):
 
 
 def phase_times(L):
-    import ctypes as C
-    ms = (C.c_double * 10)()
-    n = (C.c_longlong * 10)()
-    k = L.emc_hip_phase_times(ms, n, 10)
+    ms = (C.c_double * NPHASE)()
+    n = (C.c_longlong * NPHASE)()
+    k = L.emc_hip_phase_times(ms, n, NPHASE)
     return {L.emc_hip_phase_name(j).decode(): (ms[j], n[j]) for j in range(k)}
 
 
-def run_form(E, W, precision, warmup, steps, sigma=0.4):
-    """Returns (seconds per iteration, phase times over the timed steps,
-    final model)."""
+def proposal_stats(L):
+    out = (C.c_double * 13)()
+    k = L.emc_hip_proposal_stats(out, 13)
+    keys = ["iteration", "full_search", "audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass",
+            "pairs_scored", "gated_at_prediction", "train_loss", "train_ms", "predict_ms", "whole_table",
+            "candidate_pairs"]
+    return {keys[j]: out[j] for j in range(k)}
+
+
+def timed(E, prob, sigma, steps, learned=False):
     L = E.lib()
-    E.set_option("precision", precision)
-    E.set_option("insert", "atomic")
-    E.set_option("timing", "0")
-    prob = E.EmcProblem(W.cfg.side, W.images, W.mask, W.q, W.w, W.model0)
-    for it in range(warmup):
-        prob.iteration(sigma)
-        log(f"  {precision} warmup {it} done")
     E.set_option("timing", "1")
     L.emc_hip_reset_phase_times()
     L.emc_hip_synchronize()
     t0 = time.perf_counter()
-    for it in range(steps):
+    per = []
+    for _ in range(steps):
         prob.iteration(sigma)
+        if learned:
+            per.append(proposal_stats(L))
     L.emc_hip_synchronize()
     dt = (time.perf_counter() - t0) / steps
     E.check()
     ph = phase_times(L)
     E.set_option("timing", "0")
+    return dt, ph, per
+
+
+def run_dense(E, W, precision, warmup, steps, sigma, pre=0):
+    """`pre` + `warmup` untimed iterations, then `steps` timed ones."""
+    L = E.lib()
+    E.set_option("precision", precision)
+    E.set_option("insert", "atomic")
+    E.set_option("proposal", "off")
+    E.set_option("timing", "0")
+    prob = E.EmcProblem(W.cfg.side, W.images, W.mask, W.q, W.w, W.model0)
+    for it in range(pre + warmup):
+        prob.iteration(sigma)
+        log(f"  {precision} untimed iteration {it + 1} done")
+    dt, ph, _ = timed(E, prob, sigma, steps)
     model = prob.model()
     prob.free()
     L.emc_hip_invalidate()
     log(f"  {precision}: {dt * 1e3:.1f} ms/iteration")
-    return dt, ph, model
+    return {"dt": dt, "phases": ph, "model": model}
+
+
+def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coarse, seed=0):
+    L = E.lib()
+    E.set_option("precision", "fp16")
+    E.set_option("insert", "atomic")
+    E.set_option("proposal", "off")
+    E.set_option("audit_fraction", "0.02")
+    E.set_option("min_mass", "0.999")
+    E.set_option("proposal_gate", "predicted")
+    E.set_option("timing", "0")
+    N, R, side = W.N, W.R, W.cfg.side
+    prob = E.EmcProblem(side, W.images, W.mask, W.q, W.w, W.model0)
+    t_start = time.perf_counter()
+    if L.emc_hip_proposal_init(E.fptr(W.q), R, n_coarse, side, topk, 256, seed) != 0:
+        E.check()
+    dense_s = train_s = 0.0
+    losses = []
+    for it in range(train_iters):
+        t0 = time.perf_counter()
+        prob.iteration(sigma)
+        L.emc_hip_synchronize()
+        t1 = time.perf_counter()
+        loss = C.c_double()
+        if L.emc_hip_proposal_train(prob.d_images.vp, prob.d_mask.vp, side * side, prob.d_respons.vp, N, R, epochs,
+                                    lr, C.byref(loss)) != 0:
+            E.check()
+        t2 = time.perf_counter()
+        dense_s += t1 - t0
+        train_s += t2 - t1
+        losses.append(loss.value)
+        log(f"  dense iteration {it + 1}: {(t1 - t0) * 1e3:.0f} ms, training {(t2 - t1) * 1e3:.0f} ms, "
+            f"loss {loss.value:.4f}")
+    t0 = time.perf_counter()
+    if L.emc_hip_proposal_predict(prob.d_images.vp, prob.d_mask.vp, side * side, N) != 0:
+        E.check()
+    predict_s = time.perf_counter() - t0
+    gated = proposal_stats(L)["gated_at_prediction"]
+    log(f"  prediction {predict_s * 1e3:.0f} ms; {gated:.0f} patterns start in whole-table search")
+    E.set_option("proposal", "on")
+    for it in range(warmup):
+        prob.iteration(sigma)
+        st = proposal_stats(L)
+        log(f"  proposal warmup {it + 1}: whole table {st['whole_table']:.0f}, audit below min mass "
+            f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f}, min mass {st['audit_min_mass']:.6f}")
+    dt, ph, per = timed(E, prob, sigma, steps, learned=True)
+    total_s = time.perf_counter() - t_start
+    model = prob.model()
+    prob.free()
+    E.set_option("proposal", "off")
+    L.emc_hip_invalidate()
+    log(f"  learned: {dt * 1e3:.1f} ms/iteration")
+    return {"dt": dt, "phases": ph, "model": model, "per_step": per, "dense_s": dense_s, "train_s": train_s,
+            "predict_s": predict_s, "losses": losses, "gated": gated, "total_s": total_s}
+
+
+def cc(a, b):
+    a = a.astype(np.float64) - a.mean()
+    b = b.astype(np.float64) - b.mean()
+    return float((a * b).sum() / np.sqrt((a * a).sum() * (b * b).sum()))
 
 
 def copy_calibration(mib):
@@ -101,20 +192,20 @@ def read_traffic(kernel, workload):
     return None
 
 
-def cc(a, b):
-    a = a.astype(np.float64) - a.mean()
-    b = b.astype(np.float64) - b.mean()
-    return float((a * b).sum() / np.sqrt((a * a).sum() * (b * b).sum()))
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=3, help="workload preset (emc_amd/configs.py)")
     ap.add_argument("--patterns", type=int, default=0, help="override the pattern count")
+    ap.add_argument("--sigma", type=float, default=0.4)
     ap.add_argument("--plain-steps", type=int, default=-1, help="timed FP32 iterations (default = steps)")
+    ap.add_argument("--train-iters", type=int, default=5, help="dense iterations whose responsibilities train")
+    ap.add_argument("--epochs", type=int, default=2, help="training passes after each dense iteration")
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--topk", type=int, default=32)
+    ap.add_argument("--coarse-n", type=int, default=4)
     ap.add_argument("--cpu-patterns", type=int, default=8, help="pattern sample for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -135,25 +226,37 @@ def main():
     t0 = time.perf_counter()
     W = configs.build(args.config, args.patterns or None)
     N, R, n_m = W.N, W.R, W.n_m
+    learned = W.cfg.name.endswith("proposal")
     log(f"workload {W.cfg.name}: N={N} R={R} side={W.cfg.side} n_m={n_m} "
         f"(data {time.perf_counter() - t0:.1f}s)")
 
     plain_steps = args.steps if args.plain_steps < 0 else args.plain_steps
-    dt_fast, ph_fast, model_fast = run_form(E, W, "fp16", args.warmup, args.steps)
-    res_plain = None
+    pre = args.train_iters if learned else 0
+    if learned:
+        fast = run_learned(E, W, args.warmup, args.steps, args.sigma, args.train_iters, args.epochs, args.lr,
+                           args.topk, args.coarse_n)
+    else:
+        fast = run_dense(E, W, "fp16", args.warmup, args.steps, args.sigma)
+    plain = None
     if not args.no_plain and plain_steps > 0:
-        res_plain = run_form(E, W, "fp32", args.warmup, plain_steps)
+        plain = run_dense(E, W, "fp32", args.warmup, plain_steps, args.sigma, pre=pre)
 
     pairs = float(N) * R
-    value = pairs / dt_fast
-    flop_gemm = 2.0 * N * R * n_m  # algorithmic FLOP of one E-step contraction
-    like_ms, like_n = ph_fast["likelihood"]
+    dt_fast = fast["dt"]
+    like_ms, like_n = fast["phases"]["likelihood"]
     like_avg = like_ms / 1e3 / max(like_n, 1)
-    achieved = flop_gemm / like_avg
+    if learned:
+        flop = 2.0 * fast["per_step"][-1]["candidate_pairs"] * n_m
+        kernel, kname = "candidate E-step (k_cand_estep)", "k_cand_estep"
+    else:
+        flop = 2.0 * N * R * n_m
+        kernel, kname = "likelihood GEMM (k_estep16)", "k_estep16"
+    achieved = flop / like_avg
 
     out = {
-        "metric": "pattern x rotation pairs/s per EMC iteration (FP16-MFMA hi/lo vs plain FP32)",
-        "value": value,
+        "metric": "pattern x rotation pairs/s per EMC iteration (FP16-MFMA + learned proposal vs plain FP32)"
+        if learned else "pattern x rotation pairs/s per EMC iteration (FP16-MFMA hi/lo vs plain FP32)",
+        "value": pairs / dt_fast,
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -163,24 +266,42 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp16 (hi/lo split) x fp32 accumulate; fp64 normalisation",
-        "data": "synthetic (Gaussian-blob object, Poisson patterns), inputs resident in HBM",
+        "data": "synthetic (Gaussian-blob object, Poisson patterns), random initial model, inputs resident in HBM",
         "config": {"workload": W.cfg.name, "side": W.cfg.side, "patterns": N, "rotations": R,
-                   "masked_pixels": n_m, "sampling_n": W.cfg.n_rot, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": "likelihood GEMM (k_estep16)", "achieved": achieved / 1e12,
-                     "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP16,
-                     "traffic": read_traffic("k_estep16", W.cfg.name), "launch_ms": like_avg * 1e3,
-                     "algorithmic_flop_per_launch": flop_gemm},
-        "phases_ms_per_step": {k: v[0] / args.steps for k, v in ph_fast.items()},
+                   "masked_pixels": n_m, "sampling_n": W.cfg.n_rot, "sigma": args.sigma,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": kernel, "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP16,
+                     "traffic": read_traffic(kname, W.cfg.name), "launch_ms": like_avg * 1e3,
+                     "algorithmic_flop_per_launch": flop},
+        "phases_ms_per_step": {k: v[0] / args.steps for k, v in fast["phases"].items()},
     }
-    if res_plain is not None:
-        dt_plain, ph_plain, model_plain = res_plain
-        out["plain_fp32"] = {"value": pairs / dt_plain, "ms_per_step": dt_plain * 1e3,
-                             "speedup_fast_vs_plain": dt_plain / dt_fast,
-                             "phases_ms_per_step": {k: v[0] / plain_steps for k, v in ph_plain.items()},
-                             "likelihood_tflops": flop_gemm / (ph_plain["likelihood"][0] / 1e3 /
-                                                               max(ph_plain["likelihood"][1], 1)) / 1e12}
+    if learned:
+        steps_st = fast["per_step"]
+        scored = float(np.mean([s["pairs_scored"] for s in steps_st]))
+        out["proposal"] = {
+            "schedule": f"{args.train_iters} dense FP16 iterations, network trained after each "
+                        f"({args.epochs} epochs, Adam lr {args.lr}); top-{args.topk} of the n={args.coarse_n} "
+                        f"coarse cells; {args.warmup} untimed + {args.steps} timed proposal iterations",
+            "pairs_scored_per_step": scored,
+            "scored_pairs_per_s": scored / dt_fast,
+            "whole_table_patterns_per_step": float(np.mean([s["whole_table"] for s in steps_st])),
+            "full_search_patterns_per_step": float(np.mean([s["full_search"] for s in steps_st])),
+            "audited_per_step": float(np.mean([s["audited"] for s in steps_st])),
+            "audit_below_min_mass_per_step": float(np.mean([s["audit_below_min_mass"] for s in steps_st])),
+            "audit_min_captured_mass": float(min(s["audit_min_mass"] for s in steps_st)),
+            "audit_mean_captured_mass": float(np.mean([s["audit_mean_mass"] for s in steps_st])),
+            "gated_at_prediction": fast["gated"],
+            "train_loss_per_dense_iteration": fast["losses"],
+            "dense_iterations_s": fast["dense_s"], "training_s": fast["train_s"], "prediction_s": fast["predict_s"],
+            "whole_schedule_s": fast["total_s"],
+        }
+    if plain is not None:
+        out["plain_fp32"] = {"value": pairs / plain["dt"], "ms_per_step": plain["dt"] * 1e3,
+                             "speedup_fast_vs_plain": plain["dt"] / dt_fast,
+                             "phases_ms_per_step": {k: v[0] / plain_steps for k, v in plain["phases"].items()}}
         if plain_steps == args.steps:
-            out["model_cc_fast_vs_plain"] = cc(model_fast, model_plain)
+            out["model_cc_fast_vs_plain"] = cc(fast["model"], plain["model"])
 
     if not args.no_cpu:
         from oracle import oracle as O

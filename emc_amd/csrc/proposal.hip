@@ -49,6 +49,12 @@ static void sync_stream() { EMC_CHECK(hipStreamSynchronize(engine().stream)); }
 static bool build_lists(Proposal& p) {
   Engine& e = engine();
   const int N = p.N, K = p.topk, C = p.C;
+  for (size_t t = 0; t < p.h_cand.size(); t++)
+    if (p.h_cand[t] < 0 || p.h_cand[t] >= C) {
+      set_error("proposal: pattern %zu has no valid candidate cell at rank %zu (network output not finite?)",
+                t / K, t % K);
+      return false;
+    }
   p.h_ent_off.assign(C + 1, 0);
   for (size_t t = 0; t < p.h_cand.size(); t++) p.h_ent_off[p.h_cand[t] + 1]++;
   for (int c = 0; c < C; c++) p.h_ent_off[c + 1] += p.h_ent_off[c];
@@ -63,6 +69,8 @@ static bool build_lists(Proposal& p) {
       jj[at] = j;
       p.h_ncand[i] += p.h_cell_off[c + 1] - p.h_cell_off[c];
     }
+  p.st_cand_pairs = 0;
+  for (int i = 0; i < N; i++) p.st_cand_pairs += p.h_ncand[i];
   std::vector<int2> work;
   p.h_chunk_off.assign(C + 1, 0);
   for (int c = 0; c < C; c++) {
@@ -627,11 +635,12 @@ int emc_hip_proposal_get(int* cand, int* mode, float* mass, int N_images) {
 
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  const double v[12] = {(double)p.iteration, p.st_full,     p.st_audited,  p.st_fail,
-                        p.st_min_mass,       p.st_mean_mass, p.st_pairs,    p.st_gated,
-                        p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub};
+  const double v[13] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
+                        p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
+                        p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
+                        p.st_cand_pairs};
   int k = 0;
-  for (; k < n && k < 12; k++) out[k] = v[k];
+  for (; k < n && k < 13; k++) out[k] = v[k];
   return k;
 }
 

@@ -65,7 +65,11 @@ static void usage(void) {
           "           [--precision fp32|fp16] [--insert atomic|deterministic]\n"
           "           [--interp nearest|trilinear] [--sampling n] [--full-table]\n"
           "           [--model random|truth] [--seed S] [--out DIR] [--dump-respons]\n"
-          "           [--save-every K] [--quiet]\n");
+          "           [--save-every K] [--quiet]\n"
+          "           [--proposal T] [--topk K] [--coarse-n n] [--epochs E] [--lr LR]\n"
+          "  --proposal T: train the rotation-proposal network on iterations 1..T,\n"
+          "                then score only its top-K coarse cells (default: 5 for\n"
+          "                --config 3, else 0 = off)\n");
 }
 
 static void write_raw(const char *dir, const char *name, const float *v, size_t n) {
@@ -82,10 +86,11 @@ static void write_raw(const char *dir, const char *name, const float *v, size_t 
 
 int main(int argc, char **argv) {
   int cfg = 0, N_images = -1, iterations = -1, n_rot = -1, full_table = 0, dump = 0, quiet = 0;
-  int save_every = 0, model_truth = 0;
+  int save_every = 0, model_truth = 0, prop_iters = -1, topk = 32, coarse_n = 4, epochs = 2;
+  float lr = 1e-3f;
   float sigma = 0.4f; /* emc.c:734-735 */
   uint64_t seed = 0;
-  const char *out = "output", *precision = "fp32", *insert = "atomic", *interp = "nearest";
+  const char *out = "output", *precision = NULL, *insert = "atomic", *interp = "nearest";
   for (int a = 1; a < argc; a++) {
     const char *k = argv[a];
     const char *v = a + 1 < argc ? argv[a + 1] : NULL;
@@ -97,6 +102,11 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "--sigma")) TAKE(sigma, (float)atof);
     else if (!strcmp(k, "--seed")) TAKE(seed, (uint64_t)atoll);
     else if (!strcmp(k, "--save-every")) TAKE(save_every, atoi);
+    else if (!strcmp(k, "--proposal")) TAKE(prop_iters, atoi);
+    else if (!strcmp(k, "--topk")) TAKE(topk, atoi);
+    else if (!strcmp(k, "--coarse-n")) TAKE(coarse_n, atoi);
+    else if (!strcmp(k, "--epochs")) TAKE(epochs, atoi);
+    else if (!strcmp(k, "--lr")) TAKE(lr, (float)atof);
     else if (!strcmp(k, "--precision")) { if (!v) { usage(); return 2; } precision = v; a++; }
     else if (!strcmp(k, "--insert")) { if (!v) { usage(); return 2; } insert = v; a++; }
     else if (!strcmp(k, "--interp")) { if (!v) { usage(); return 2; } interp = v; a++; }
@@ -113,6 +123,12 @@ int main(int argc, char **argv) {
   if (N_images <= 0) N_images = P.n_patterns;
   if (iterations < 0) iterations = P.iterations;
   if (n_rot <= 0) n_rot = P.n_rot;
+  if (!precision) precision = cfg >= 2 ? "fp16" : "fp32"; /* the precision form from 128^3 on */
+  if (prop_iters < 0) prop_iters = cfg == 3 ? 5 : 0;
+  if (prop_iters > 0 && strcmp(precision, "fp16") != 0) {
+    fprintf(stderr, "emc: --proposal runs with --precision fp16\n");
+    return 2;
+  }
   if (emc_hip_set_option("precision", precision) || emc_hip_set_option("insert", insert) ||
       emc_hip_set_option("interp", interp)) {
     fprintf(stderr, "emc: bad option value\n");
@@ -197,6 +213,8 @@ int main(int argc, char **argv) {
   cuda_allocate_scaling(&d_scaling, N_images);
   die_if_error("allocation");
 
+  if (prop_iters > 0 && emc_hip_proposal_init(qbuf, N_slices, coarse_n, P.side, topk, 256, seed) != 0)
+    die_if_error("proposal setup");
   double t_total = 0;
   for (int iteration = 0; iteration < iterations; iteration++) {
     const double t_it = now_ms();
@@ -237,6 +255,24 @@ int main(int argc, char **argv) {
     t_e = now_ms();
     if (!quiet) printf("Compression time = %fms\n", t_e - t_i);
     die_if_error("iteration");
+    if (iteration < prop_iters) {
+      /* the proposal network learns from this iteration's responsibilities;
+       * after the last training iteration it predicts every pattern's cells */
+      double loss = 0.0;
+      emc_hip_proposal_train(d_images, d_mask, N_2d, d_respons, N_images, N_slices, epochs, lr, &loss);
+      if (!quiet) printf("proposal training loss = %g\n", loss);
+      if (iteration == prop_iters - 1) {
+        emc_hip_proposal_predict(d_images, d_mask, N_2d, N_images);
+        emc_hip_set_option("proposal", "on");
+      }
+      die_if_error("proposal");
+    } else if (prop_iters > 0) {
+      double st[13];
+      emc_hip_proposal_stats(st, 13);
+      if (!quiet)
+        printf("proposal: %.3g pairs scored, %.0f patterns on the whole table, %.0f of %.0f audited below "
+               "min mass (min captured %.6f)\n", st[6], st[11], st[3], st[2], st[4]);
+    }
     const double dt = now_ms() - t_it;
     t_total += dt;
     printf("iteration %d: %.1f ms, %.3g pattern x rotation pairs/s, likelihood %g, scaling error %g\n", iteration,

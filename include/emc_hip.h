@@ -150,7 +150,8 @@ int emc_hip_proposal_get(int *cand, int *mode, float *mass, int N_images);
  * [4] smallest and [5] mean captured mass of the audited patterns, [6]
  * pattern x rotation pairs scored, [7] patterns gated to full search at
  * prediction, [8] last training loss, [9] last training time (ms), [10] last
- * prediction time (ms), [11] patterns scored on the whole table.  Returns
+ * prediction time (ms), [11] patterns scored on the whole table, [12]
+ * candidate pairs of all patterns (the candidate E-step's work).  Returns
  * the number of values written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 

@@ -1,0 +1,234 @@
+"""The candidate-restricted iteration of the learned proposal, through the
+C-ABI:
+  - with every coarse cell as a candidate it reproduces the dense FP16
+    iteration (same contractions, summation order aside);
+  - with top-k candidates, the audit subset and whole-table fallback it
+    matches the FP64 oracle restricted to the same rotations (1e-4 absolute
+    on responsibilities, 1e-4 relative on scaling and model);
+  - the search mode follows the audited captured mass, over two iterations;
+  - a network trained on EMC's own responsibilities captures far more of the
+    posterior in its top-k cells than the untrained network.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import emc_amd as E
+from oracle import oracle as O
+from tests import problems
+
+pytestmark = pytest.mark.gpu
+
+SEED = 11
+
+
+@pytest.fixture(scope="module")
+def prob():
+    # 32^3 model, 32x32 patterns, n=4 fine table (3,240), truth-model start
+    return problems.make(side=32, n_patterns=300, n_rot=4, n_blobs=20, radius=6.0, photons=2000.0,
+                         phi_sigma=0.2, beamstop=3.0, seed=3, model="truth")
+
+
+@pytest.fixture(autouse=True)
+def _reset():
+    E.lib().emc_hip_clear_error()
+    opts(proposal="off")
+    yield
+    opts(proposal="off")
+    E.lib().emc_hip_clear_error()
+
+
+def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", min_mass="0.999", gate="audit"):
+    E.set_option("precision", precision)
+    E.set_option("insert", insert)
+    E.set_option("interp", "nearest")
+    E.set_option("proposal", proposal)
+    E.set_option("audit_fraction", audit)
+    E.set_option("min_mass", min_mass)
+    E.set_option("proposal_gate", gate)
+
+
+def init_cells(P, topk, n_coarse=2, batch=64):
+    L = E.lib()
+    assert L.emc_hip_proposal_init(E.fptr(P.q), P.R, n_coarse, P.side, topk, batch, SEED) == 0
+    E.check()
+    nc = L.emc_hip_proposal_cells(None, None)
+    off = np.zeros(nc + 1, np.int32)
+    rot = np.zeros(P.R, np.int32)
+    assert L.emc_hip_proposal_cells(E.iptr(off), E.iptr(rot)) == nc
+    return nc, off, rot
+
+
+def stats():
+    out = (C.c_double * 12)()
+    E.lib().emc_hip_proposal_stats(out, 12)
+    return list(out)
+
+
+def audited(N, iteration, frac):
+    L = E.lib()
+    return np.array([i for i in range(N) if L.emc_uniform(SEED, 0xA0D17, iteration, i) < frac], np.int64)
+
+
+def run(P, model=None, scaling=None, iterations=1):
+    prob = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model if model is None else model,
+                        scaling=scaling)
+    out = []
+    for _ in range(iterations):
+        it = prob.iteration(0.4)
+        out.append({"P": prob.respons(), "scaling": prob.scaling(), "model": prob.model(), "it": it,
+                    "stats": stats()})
+    prob.free()
+    return out
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def cell_mass(Pm, off, rot):
+    """[C][N] posterior mass per coarse cell."""
+    return np.stack([Pm[rot[off[c]:off[c + 1]]].sum(axis=0) for c in range(len(off) - 1)])
+
+
+def test_all_cells_reproduce_dense(prob):
+    P = prob
+    opts()
+    dense = run(P)[0]
+    nc, off, rot = init_cells(P, topk=8)
+    cand = np.tile(np.arange(nc, dtype=np.int32), (P.N, 1))
+    assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, nc) == 0
+    opts(proposal="on")
+    cand_run = run(P)[0]
+    st = cand_run["stats"]
+    assert st[0] == 1 and st[11] == 0 and st[6] == P.N * P.R
+    dP = np.abs(cand_run["P"] - dense["P"]).max()
+    ds = rel(cand_run["scaling"], dense["scaling"])
+    dm = rel(cand_run["model"], dense["model"])
+    dl = abs(cand_run["it"]["likelihood"] - dense["it"]["likelihood"]) / abs(dense["it"]["likelihood"])
+    print(f"all cells vs dense FP16: P {dP:.2e}, scaling {ds:.2e}, model {dm:.2e}, sum PlogP {dl:.2e}")
+    assert dP < 1e-6 and ds < 1e-6 and dm < 1e-5 and dl < 1e-6
+
+
+def oracle_candidates(P, off, rot, topk):
+    ref = O.o64_iteration(P.model.astype(np.float64), P.images, P.mask, np.ones(P.N), P.side, P.q, P.w,
+                          P.x, P.y, P.z)
+    cm = cell_mass(ref["P"], off, rot)
+    return np.argsort(-cm, axis=0, kind="stable")[:topk].T.astype(np.int32).copy(), ref
+
+
+@pytest.mark.parametrize("audit", ["0", "0.3"])
+def test_topk_with_audit_matches_oracle(prob, audit):
+    P = prob
+    topk = 6
+    nc, off, rot = init_cells(P, topk)
+    cand, ref_dense = oracle_candidates(P, off, rot, topk)
+    assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
+    opts(proposal="on", audit=audit)
+    g = run(P)[0]
+    sub = audited(P.N, 1, float(audit))
+    allowed = O.allowed_from_candidates(cand, off, rot, P.R, sub)
+    ref = O.o64_proposal_iteration(P.model.astype(np.float64), P.images, P.mask, np.ones(P.N), P.side, P.q, P.w,
+                                   P.x, P.y, P.z, allowed)
+    st = g["stats"]
+    dP = np.abs(g["P"] - ref["P"]).max()
+    ds = rel(g["scaling"], ref["scaling"])
+    dm = rel(g["model"], ref["model"])
+    print(f"top-{topk}, audit {audit}: {len(sub)} audited, pairs {st[6]:.0f} of {P.N * P.R}; "
+          f"P {dP:.2e}, scaling {ds:.2e}, model {dm:.2e}; captured mass min {st[4]:.6f} mean {st[5]:.6f}")
+    assert st[2] == len(sub) and st[11] == len(sub)
+    assert st[6] == allowed.sum()
+    assert dP < 1e-4 and ds < 1e-4 and dm < 1e-4
+    # search modes: audited patterns whose candidates hold < 0.999 of the
+    # whole-table posterior go to full search
+    mode = np.zeros(P.N, np.int32)
+    assert E.lib().emc_hip_proposal_get(None, E.iptr(mode), None, P.N) == 0
+    mass = np.array([ref_dense["P"][allowed_from(cand[i], off, rot), i].sum() for i in range(P.N)])
+    clear = np.abs(mass - 0.999) > 1e-4
+    expect = np.zeros(P.N, np.int32)
+    expect[sub] = (mass[sub] < 0.999)
+    assert np.array_equal(mode[clear], expect[clear])
+    assert st[3] == expect.sum() or not clear[sub].all()
+
+
+def allowed_from(cells, off, rot):
+    return np.concatenate([rot[off[c]:off[c + 1]] for c in cells])
+
+
+def test_fallback_over_two_iterations(prob):
+    P = prob
+    topk = 3
+    nc, off, rot = init_cells(P, topk)
+    # deliberately poor candidates: cells ranked 2..4 by posterior mass
+    cm = cell_mass(O.o64_iteration(P.model.astype(np.float64), P.images, P.mask, np.ones(P.N), P.side, P.q, P.w,
+                                   P.x, P.y, P.z)["P"], off, rot)
+    cand = np.argsort(-cm, axis=0, kind="stable")[2:2 + topk].T.astype(np.int32).copy()
+    assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
+    opts(proposal="on", audit="0.5")
+    g1, g2 = run(P, iterations=2)
+    mode1 = np.zeros(P.N, np.int32)
+    # modes after iteration 2; reconstruct those after iteration 1 from the
+    # oracle to predict the subset of iteration 2
+    sub1 = audited(P.N, 1, 0.5)
+    allowed1 = O.allowed_from_candidates(cand, off, rot, P.R, sub1)
+    ref1 = O.o64_proposal_iteration(P.model.astype(np.float64), P.images, P.mask, np.ones(P.N), P.side, P.q, P.w,
+                                    P.x, P.y, P.z, allowed1)
+    full1 = np.zeros(P.N, bool)
+    for i in sub1:
+        full1[i] = ref1["P"][allowed_from(cand[i], off, rot), i].sum() < 0.999
+    print(f"iteration 1: {len(sub1)} audited, {full1.sum()} to full search (GPU {g1['stats'][3]:.0f})")
+    assert g1["stats"][3] == full1.sum()
+    assert g2["stats"][1] == full1.sum()  # whole-table search in iteration 2
+    sub2 = np.union1d(audited(P.N, 2, 0.5), np.nonzero(full1)[0])
+    assert g2["stats"][11] == len(sub2)
+    # iteration 2 from the GPU's own iteration-1 model and scaling
+    allowed2 = O.allowed_from_candidates(cand, off, rot, P.R, sub2)
+    ref2 = O.o64_proposal_iteration(g1["model"].astype(np.float64), P.images, P.mask, g1["scaling"], P.side, P.q,
+                                    P.w, P.x, P.y, P.z, allowed2)
+    dP = np.abs(g2["P"] - ref2["P"]).max()
+    dm = rel(g2["model"], ref2["model"])
+    print(f"iteration 2: {len(sub2)} on the whole table; P {dP:.2e}, model {dm:.2e}")
+    assert dP < 1e-4 and dm < 1e-4
+    del mode1
+
+
+def test_trained_network_gain(prob):
+    P = prob
+    topk = 8
+    nc, off, rot = init_cells(P, topk, batch=64)
+    L = E.lib()
+    opts()
+    p = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
+    p.get_slices()
+    p.likelihood(0.4)
+    p.normalize_respons()
+    Pm = p.respons().astype(np.float64)
+    cm = cell_mass(Pm, off, rot)
+
+    def captured():
+        cand = np.zeros((P.N, topk), np.int32)
+        pm = np.zeros(P.N, np.float32)
+        assert L.emc_hip_proposal_predict(p.d_images.vp, p.d_mask.vp, P.side ** 2, P.N) == 0
+        assert L.emc_hip_proposal_get(E.iptr(cand), None, E.fptr(pm), P.N) == 0
+        return np.take_along_axis(cm, cand.T, axis=0).sum(axis=0), pm
+
+    m0, pm0 = captured()
+    loss = C.c_double()
+    losses = []
+    for _ in range(16):
+        assert L.emc_hip_proposal_train(p.d_images.vp, p.d_mask.vp, P.side ** 2, p.d_respons.vp, P.N, P.R, 5, 2e-3,
+                                        C.byref(loss)) == 0
+        losses.append(loss.value)
+    m1, pm1 = captured()
+    p.free()
+    E.check()
+    print(f"training loss {losses[0]:.3f} -> {losses[-1]:.3f}; captured mass in top-{topk} of {nc} cells: "
+          f"untrained mean {m0.mean():.3f} (>=0.999: {np.mean(m0 >= 0.999):.3f}), "
+          f"trained mean {m1.mean():.3f} (>=0.999: {np.mean(m1 >= 0.999):.3f}); "
+          f"predicted top-k mass {pm0.mean():.3f} -> {pm1.mean():.3f}")
+    assert losses[-1] < 0.5 * losses[0]
+    assert m1.mean() > 0.9 and m1.mean() > m0.mean() + 0.5
+    assert np.mean(m1 >= 0.99) > np.mean(m0 >= 0.99) + 0.5
