@@ -73,9 +73,9 @@ def timed(E, prob, sigma, steps, learned=False):
     L.emc_hip_reset_phase_times()
     L.emc_hip_synchronize()
     t0 = time.perf_counter()
-    per = []
+    per, hist = [], []
     for _ in range(steps):
-        prob.iteration(sigma)
+        hist.append(prob.iteration(sigma))
         if learned:
             per.append(proposal_stats(L))
     L.emc_hip_synchronize()
@@ -83,7 +83,7 @@ def timed(E, prob, sigma, steps, learned=False):
     E.check()
     ph = phase_times(L)
     E.set_option("timing", "0")
-    return dt, ph, per
+    return dt, ph, per, hist
 
 
 def run_dense(E, W, precision, warmup, steps, sigma, pre=0):
@@ -94,15 +94,17 @@ def run_dense(E, W, precision, warmup, steps, sigma, pre=0):
     E.set_option("proposal", "off")
     E.set_option("timing", "0")
     prob = E.EmcProblem(W.cfg.side, W.images, W.mask, W.q, W.w, W.model0)
+    history = []
     for it in range(pre + warmup):
-        prob.iteration(sigma)
+        history.append(prob.iteration(sigma))
         log(f"  {precision} untimed iteration {it + 1} done")
-    dt, ph, _ = timed(E, prob, sigma, steps)
+    dt, ph, _, hist = timed(E, prob, sigma, steps)
+    history += hist
     model = prob.model()
     prob.free()
     L.emc_hip_invalidate()
     log(f"  {precision}: {dt * 1e3:.1f} ms/iteration")
-    return {"dt": dt, "phases": ph, "model": model}
+    return {"dt": dt, "phases": ph, "model": model, "history": history}
 
 
 def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coarse, seed=0):
@@ -120,14 +122,15 @@ def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coa
     if L.emc_hip_proposal_init(E.fptr(W.q), R, n_coarse, side, topk, 256, seed) != 0:
         E.check()
     dense_s = train_s = 0.0
-    losses = []
+    losses, history = [], []
     for it in range(train_iters):
         t0 = time.perf_counter()
-        prob.iteration(sigma)
+        history.append(prob.iteration(sigma))
         L.emc_hip_synchronize()
         t1 = time.perf_counter()
         loss = C.c_double()
-        if L.emc_hip_proposal_train(prob.d_images.vp, prob.d_mask.vp, side * side, prob.d_respons.vp, N, R, epochs,
+        ep = epochs[min(it, len(epochs) - 1)]
+        if L.emc_hip_proposal_train(prob.d_images.vp, prob.d_mask.vp, side * side, prob.d_respons.vp, N, R, ep,
                                     lr, C.byref(loss)) != 0:
             E.check()
         t2 = time.perf_counter()
@@ -141,14 +144,19 @@ def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coa
         E.check()
     predict_s = time.perf_counter() - t0
     gated = proposal_stats(L)["gated_at_prediction"]
-    log(f"  prediction {predict_s * 1e3:.0f} ms; {gated:.0f} patterns start in whole-table search")
+    pmass = np.zeros(N, np.float32)
+    L.emc_hip_proposal_get(None, None, E.fptr(pmass), N)
+    pq = np.percentile(pmass, [1, 10, 50]).tolist()
+    log(f"  prediction {predict_s * 1e3:.0f} ms; {gated:.0f} patterns start in whole-table search; "
+        f"network top-k mass q1/q10/q50 {pq}")
     E.set_option("proposal", "on")
     for it in range(warmup):
-        prob.iteration(sigma)
+        history.append(prob.iteration(sigma))
         st = proposal_stats(L)
         log(f"  proposal warmup {it + 1}: whole table {st['whole_table']:.0f}, audit below min mass "
             f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f}, min mass {st['audit_min_mass']:.6f}")
-    dt, ph, per = timed(E, prob, sigma, steps, learned=True)
+    dt, ph, per, hist = timed(E, prob, sigma, steps, learned=True)
+    history += hist
     total_s = time.perf_counter() - t_start
     model = prob.model()
     prob.free()
@@ -156,7 +164,8 @@ def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coa
     L.emc_hip_invalidate()
     log(f"  learned: {dt * 1e3:.1f} ms/iteration")
     return {"dt": dt, "phases": ph, "model": model, "per_step": per, "dense_s": dense_s, "train_s": train_s,
-            "predict_s": predict_s, "losses": losses, "gated": gated, "total_s": total_s}
+            "predict_s": predict_s, "losses": losses, "gated": gated, "total_s": total_s,
+            "predicted_mass_q1_q10_q50": pq, "history": history}
 
 
 def cc(a, b):
@@ -195,15 +204,16 @@ def read_traffic(kernel, workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=3, help="workload preset (emc_amd/configs.py)")
     ap.add_argument("--patterns", type=int, default=0, help="override the pattern count")
     ap.add_argument("--sigma", type=float, default=0.4)
     ap.add_argument("--plain-steps", type=int, default=-1, help="timed FP32 iterations (default = steps)")
     ap.add_argument("--train-iters", type=int, default=5, help="dense iterations whose responsibilities train")
-    ap.add_argument("--epochs", type=int, default=2, help="training passes after each dense iteration")
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--epochs", default="1,1,1,2,6",
+                    help="training passes after each dense iteration (comma list; the last repeats)")
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--topk", type=int, default=32)
     ap.add_argument("--coarse-n", type=int, default=4)
     ap.add_argument("--cpu-patterns", type=int, default=8, help="pattern sample for the CPU baseline")
@@ -233,8 +243,8 @@ def main():
     plain_steps = args.steps if args.plain_steps < 0 else args.plain_steps
     pre = args.train_iters if learned else 0
     if learned:
-        fast = run_learned(E, W, args.warmup, args.steps, args.sigma, args.train_iters, args.epochs, args.lr,
-                           args.topk, args.coarse_n)
+        fast = run_learned(E, W, args.warmup, args.steps, args.sigma, args.train_iters,
+                           [int(v) for v in args.epochs.split(",")], args.lr, args.topk, args.coarse_n)
     else:
         fast = run_dense(E, W, "fp16", args.warmup, args.steps, args.sigma)
     plain = None
@@ -281,7 +291,7 @@ def main():
         scored = float(np.mean([s["pairs_scored"] for s in steps_st]))
         out["proposal"] = {
             "schedule": f"{args.train_iters} dense FP16 iterations, network trained after each "
-                        f"({args.epochs} epochs, Adam lr {args.lr}); top-{args.topk} of the n={args.coarse_n} "
+                        f"(epochs {args.epochs}, Adam lr {args.lr}); top-{args.topk} of the n={args.coarse_n} "
                         f"coarse cells; {args.warmup} untimed + {args.steps} timed proposal iterations",
             "pairs_scored_per_step": scored,
             "scored_pairs_per_s": scored / dt_fast,
@@ -295,11 +305,14 @@ def main():
             "train_loss_per_dense_iteration": fast["losses"],
             "dense_iterations_s": fast["dense_s"], "training_s": fast["train_s"], "prediction_s": fast["predict_s"],
             "whole_schedule_s": fast["total_s"],
+            "network_topk_mass_q1_q10_q50": fast["predicted_mass_q1_q10_q50"],
+            "sum_plogp_per_iteration": [h["likelihood"] for h in fast["history"]],
         }
     if plain is not None:
         out["plain_fp32"] = {"value": pairs / plain["dt"], "ms_per_step": plain["dt"] * 1e3,
                              "speedup_fast_vs_plain": plain["dt"] / dt_fast,
-                             "phases_ms_per_step": {k: v[0] / plain_steps for k, v in plain["phases"].items()}}
+                             "phases_ms_per_step": {k: v[0] / plain_steps for k, v in plain["phases"].items()},
+                             "sum_plogp_per_iteration": [h["likelihood"] for h in plain["history"]]}
         if plain_steps == args.steps:
             out["model_cc_fast_vs_plain"] = cc(fast["model"], plain["model"])
 

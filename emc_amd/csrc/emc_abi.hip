@@ -84,8 +84,8 @@ bool Engine::init() {
 void Engine::release_all() {
   DBuf* bufs[] = {&d_pix, &d_B, &d_K32, &d_Kt32, &d_K16, &d_K16lo, &d_Kt16, &d_Kt16lo, &d_flags,
                   &d_S32, &d_S16, &d_S16lo, &d_Sscale, &d_A, &d_phi_used, &d_smax, &d_ssum, &d_sacc,
-                  &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Pscale, &d_P16, &d_P16lo, &d_Sc, &d_wts, &d_wT,
-                  &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
+                  &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Tphi, &d_ones, &d_Pscale, &d_P16, &d_P16lo,
+                  &d_Sc, &d_wts, &d_wT, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
   for (DBuf* b : bufs) b->release();
   pack_key = {};
   map_key = {};
@@ -139,7 +139,7 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
   Engine& e = engine();
   auto& k = e.pack_key;
   if (k.images == images && k.mask == mask && k.N == N && k.N2d == N2d && k.precision == e.precision &&
-      k.stamp == e.invalidate_stamp)
+      k.likelihood == e.likelihood && k.stamp == e.invalidate_stamp)
     return true;
   k = {};
   if (N <= 0 || N2d <= 0) {
@@ -186,7 +186,7 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
     e.d_K16.release(); e.d_K16lo.release(); e.d_Kt16.release(); e.d_Kt16lo.release();
     if (!e.d_K32.ensure(nk * 4) || !e.d_Kt32.ensure(nk * 4)) return false;
     launch_pack_images_f32(images, e.d_pix.as<int>(), N, N2d, e.n_mp, e.N_pad, e.d_K32.as<float>(),
-                           e.d_Kt32.as<float>(), e.d_B.as<double>(), e.stream);
+                           e.d_Kt32.as<float>(), e.d_B.as<double>(), e.stream, e.likelihood == LIKE_POISSON);
     e.k_exact = true;
     e.k_scale = 1.0f;
   } else {
@@ -216,7 +216,7 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
     }
     launch_pack_images_f16(images, e.d_pix.as<int>(), N, N2d, e.n_mp, e.N_pad, e.k_scale, !e.k_exact,
                            e.d_K16.as<_Float16>(), e.d_K16lo.as<_Float16>(), e.d_Kt16.as<_Float16>(),
-                           e.d_Kt16lo.as<_Float16>(), e.d_B.as<double>(), e.stream);
+                           e.d_Kt16lo.as<_Float16>(), e.d_B.as<double>(), e.stream, e.likelihood == LIKE_POISSON);
   }
   if (failed()) return false;
   k.images = images;
@@ -224,6 +224,7 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
   k.N = N;
   k.N2d = N2d;
   k.precision = e.precision;
+  k.likelihood = e.likelihood;
   k.stamp = e.invalidate_stamp;
   e.like.valid = false;
   return true;
@@ -239,11 +240,12 @@ static bool pack_slices(const float* slices, int R, int N2d) {
   if (e.precision == PREC_FP32) {
     if (!e.d_S32.ensure(ns * 4)) return false;
     launch_pack_slices_f32(slices, e.d_pix.as<int>(), R, N2d, e.n_mp, e.R_pad, e.d_S32.as<float>(),
-                           e.d_A.as<double>(), e.stream);
+                           e.d_A.as<double>(), e.stream, e.likelihood == LIKE_POISSON);
   } else {
     if (!e.d_S16.ensure(ns * 2) || !e.d_S16lo.ensure(ns * 2) || !e.d_Sscale.ensure(sizeof(float) * R)) return false;
     launch_pack_slices_f16(slices, e.d_pix.as<int>(), R, N2d, e.n_mp, e.R_pad, e.d_S16.as<_Float16>(),
-                           e.d_S16lo.as<_Float16>(), e.d_Sscale.as<float>(), e.d_A.as<double>(), e.stream);
+                           e.d_S16lo.as<_Float16>(), e.d_Sscale.as<float>(), e.d_A.as<double>(), e.stream,
+                           e.likelihood == LIKE_POISSON);
   }
   return !failed();
 }
@@ -267,6 +269,7 @@ static void run_estep(int N, int R, float sigma, const float* phi, float* L, con
   ep.inv_2nms2 = 1.0 / (2.0 * (double)(float)e.n_m * ((double)sigma * (double)sigma));
   ep.N = N;
   ep.R = R;
+  ep.poisson = e.likelihood == LIKE_POISSON;
   ep.vecL = (N % 4 == 0) && L && ((uintptr_t)L % 16 == 0);
   PhaseTimer pt(PH_LIKE);
   if (e.precision == PREC_FP32)
@@ -389,7 +392,7 @@ void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images
                    e.d_sacc.as<double>(), fuse ? e.d_A.as<double>() : nullptr, fuse, e.stream);
     launch_softmax_reduce(N_images, chunks, e.d_sacc.as<double>(), fuse, fuse ? e.d_B.as<double>() : nullptr,
                           fuse ? e.d_phi_used.as<float>() : nullptr, lk.sigma2x, e.d_phiden.as<double>(),
-                          e.d_red.as<double>(), e.stream);
+                          e.d_red.as<double>(), e.stream, e.likelihood == LIKE_POISSON);
   }
   double total = 0.0;
   EMC_CHECK(hipMemcpyAsync(&total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, e.stream));
@@ -455,10 +458,15 @@ void cuda_update_scaling(real* d_images, real* slices, int* d_mask, real* d_resp
                      lk.R == N_slices && lk.N2d == N_2d;
   if (!e.d_phiden.ensure(sizeof(double) * N_images) || !e.d_invphi.ensure(sizeof(float) * N_images)) return;
   if (!fused) {
-    // the reference's method: recompute sum_p m K S for every (pattern, rotation)
     if (!pack_slices(slices, N_slices, N_2d)) return;
-    launch_fill(reinterpret_cast<float*>(e.d_phiden.p), (size_t)N_images * 2, 0.0f, e.stream);
-    run_estep(N_images, N_slices, 1.0f, d_scaling, nullptr, d_respons, e.d_phiden.as<double>());
+    if (e.likelihood == LIKE_POISSON) {
+      // Poisson: sum_r P_ri sum_p m S_rp
+      launch_p_dot_a(d_respons, N_slices, N_images, e.d_A.as<double>(), e.d_phiden.as<double>(), e.stream);
+    } else {
+      // the reference's method: recompute sum_p m K S for every (pattern, rotation)
+      launch_fill(reinterpret_cast<float*>(e.d_phiden.p), (size_t)N_images * 2, 0.0f, e.stream);
+      run_estep(N_images, N_slices, 1.0f, d_scaling, nullptr, d_respons, e.d_phiden.as<double>());
+    }
   }
   {
     PhaseTimer pt(PH_SCALING);
@@ -492,12 +500,14 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   const int R = N_slices, N = N_images;
   e.R_pad = (int)round_up(R, kPadR);
   const bool fp16 = e.precision == PREC_FP16;
+  const bool poisson = e.likelihood == LIKE_POISSON;
   const Proposal& prp = proposal();
   const bool prop = proposal_applies(N, R) && prp.have_p && prp.respons == d_respons && e.like.valid &&
                     e.like.proposal && e.like.normalized && e.like.respons == d_respons;
   if (!e.d_invphi.ensure(sizeof(float) * N) || !e.d_T.ensure(sizeof(double) * R) || !e.d_Tf.ensure(sizeof(float) * R) ||
       !e.d_Pscale.ensure(sizeof(float) * R) || !e.d_Sc.ensure(sizeof(float) * (size_t)e.R_pad * e.n_mp) ||
-      !e.d_wts.ensure(sizeof(float) * R) || !e.d_wT.ensure(sizeof(float) * R))
+      !e.d_wts.ensure(sizeof(float) * R) || !e.d_wT.ensure(sizeof(float) * R) ||
+      (poisson && (!e.d_Tphi.ensure(sizeof(float) * R) || !e.d_ones.ensure(sizeof(float) * N))))
     return 0.0f;
   if (fp16 && !prop &&
       (!e.d_P16.ensure(2 * (size_t)e.R_pad * e.N_pad) || !e.d_P16lo.ensure(2 * (size_t)e.R_pad * e.N_pad)))
@@ -505,16 +515,21 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   EMC_CHECK(hipMemcpyAsync(e.d_wts.p, weights, sizeof(float) * R, hipMemcpyHostToDevice, e.stream));
   // 1/phi from the current scaling: the M-step divides by the updated
   // factors (emc_atomic.cu:94)
-  launch_reciprocal(d_scaling, N, e.d_invphi.as<float>(), e.stream);
+  // Gaussian: U = sum_i P K / phi; Poisson: U = sum_i P K, divided by
+  // sum_i P phi instead of T
+  if (poisson) launch_fill(e.d_ones.as<float>(), N, 1.0f, e.stream);
+  else launch_reciprocal(d_scaling, N, e.d_invphi.as<float>(), e.stream);
+  const float* opscale = poisson ? e.d_ones.as<float>() : e.d_invphi.as<float>();
   if (prop) {
     // candidate-restricted M-step (+ the whole-table subset)
-    if (!proposal_mstep(N, R, slices, N_2d)) return 0.0f;
+    if (!proposal_mstep(N, R, slices, N_2d, d_scaling)) return 0.0f;
   } else {
     MstepArgs a{};
     a.P = d_respons;
-    a.invphi = e.d_invphi.as<float>();
+    a.invphi = opscale;
     a.rowscale = fp16 ? e.d_Pscale.as<float>() : nullptr;
-    a.Tf = e.d_Tf.as<float>();
+    a.Tf = poisson ? e.d_Tphi.as<float>() : e.d_Tf.as<float>();
+    a.poisson = poisson;
     a.pix = e.d_pix.as<int>();
     a.slices = slices;
     a.Sc = e.d_Sc.as<float>();
@@ -530,8 +545,8 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
     {
       // T_r, and for the FP16 form the row scales and the hi/lo operand
       PhaseTimer pt(PH_ROWSUM);
-      launch_rowsum(d_respons, R, N, e.d_invphi.as<float>(), e.d_T.as<double>(), e.d_Tf.as<float>(),
-                    e.d_Pscale.as<float>(), fp16, e.stream);
+      launch_rowsum(d_respons, R, N, opscale, e.d_T.as<double>(), e.d_Tf.as<float>(), e.d_Pscale.as<float>(), fp16,
+                    e.stream, poisson ? d_scaling : nullptr, poisson ? e.d_Tphi.as<float>() : nullptr);
       if (fp16) launch_convert_p(a, e.d_P16.as<_Float16>(), e.d_P16lo.as<_Float16>(), e.stream);
     }
     PhaseTimer pt(PH_MSTEP);
@@ -782,6 +797,10 @@ int emc_hip_set_option(const char* key, const char* value) {
     e.timing = value[0] == '1';
   } else if (is(key, "verbose")) {
     e.verbose = value[0] == '1';
+  } else if (is(key, "likelihood")) {
+    if (is(value, "gaussian")) e.likelihood = LIKE_GAUSSIAN;
+    else if (is(value, "poisson")) e.likelihood = LIKE_POISSON;
+    else return 1;
   } else if (is(key, "proposal")) {
     if (is(value, "on")) proposal().active = true;
     else if (is(value, "off")) proposal().active = false;

@@ -43,6 +43,11 @@ enum Phase {
 enum Precision { PREC_FP32 = 0, PREC_FP16 = 1 };
 enum InsertMode { INSERT_ATOMIC = 0, INSERT_DETERMINISTIC = 1 };
 enum Interp { INTERP_NEAREST = 0, INTERP_TRILINEAR = 1 };
+// Gaussian: the reference's L = -sum m (S - K/phi)^2 / (2 n_m sigma^2).
+// Poisson: L = sum m (K log(phi max(S, kSlogFloor)) - phi S); the floor bounds
+// the range of the FP16 log-slice operand.
+enum Likelihood { LIKE_GAUSSIAN = 0, LIKE_POISSON = 1 };
+constexpr float kSlogFloor = 1e-6f;
 
 // Tile sizes shared by the GEMM kernels and the padding of packed operands.
 constexpr int kTileM = 128;    // FP32 kernels: 128 x 128
@@ -62,6 +67,7 @@ struct Engine {
   int precision = PREC_FP32;
   int insert_mode = INSERT_ATOMIC;
   int interp = INTERP_NEAREST;
+  int likelihood = LIKE_GAUSSIAN;
   bool timing = false;
   bool verbose = false;
 
@@ -73,7 +79,7 @@ struct Engine {
   struct {
     const float* images = nullptr;
     const int* mask = nullptr;
-    int N = 0, N2d = 0, precision = -1;
+    int N = 0, N2d = 0, precision = -1, likelihood = -1;
     uint64_t stamp = 0;  // invalidation stamp at build time
   } pack_key;
   int n_m = 0;      // masked pixels (mask != 0)
@@ -83,7 +89,7 @@ struct Engine {
   bool k_exact = true;   // FP16 operand holds the counts exactly
   float k_scale = 1.0f;  // FP16 operand = K * k_scale (power of two)
   DBuf d_pix;       // int [n_mp]: pixel index of compact column, -1 padding
-  DBuf d_B;         // double [N]: sum_p m K^2
+  DBuf d_B;         // double [N]: sum_p m K^2 (Poisson: sum_p m K)
   DBuf d_K32;       // float [N_pad][n_mp]
   DBuf d_Kt32;      // float [n_mp][N_pad]
   DBuf d_K16, d_K16lo;    // half [N_pad][n_mp]
@@ -95,7 +101,7 @@ struct Engine {
   DBuf d_S32;           // float [R_pad][n_mp]
   DBuf d_S16, d_S16lo;  // half [R_pad][n_mp]
   DBuf d_Sscale;        // float [R]: 1/scale_r, where the FP16 operand is S * scale_r (power of two)
-  DBuf d_A;             // double [R]: sum_p m S^2
+  DBuf d_A;             // double [R]: sum_p m S^2 (Poisson: sum_p m S)
 
   // ---- likelihood -> softmax -> scaling fusion state ----
   struct {
@@ -122,6 +128,8 @@ struct Engine {
   // ---- M-step ----
   DBuf d_T;          // double [R]
   DBuf d_Tf;         // float [R]
+  DBuf d_Tphi;       // float [R]: sum_i P_ri phi_i (Poisson M-step divisor)
+  DBuf d_ones;       // float [N]: 1 (Poisson: the M-step operand is P itself)
   DBuf d_Pscale;     // float [R]: FP16 M-step operand row scale
   DBuf d_P16, d_P16lo;  // half [R_pad][N_pad]: FP16 M-step operand (P/phi * row scale) hi/lo
   DBuf d_Sc;         // float [R_pad][n_mp]: updated compact slices for insertion
@@ -172,19 +180,23 @@ void launch_expand(const float* model, int S, const float* rot, int R, const flo
                    const float* y, const float* z, float* slices, int interp, hipStream_t st);
 // pack patterns: compact columns, B_i, FP32 and FP16 copies (+ transposes)
 void launch_pack_mask(const int* mask, int N2d, int* pix, int n_mp, hipStream_t st);
+// B = sum_p m K^2, or sum_p m K when poisson
 void launch_pack_images_f32(const float* images, const int* pix, int N, int N2d, int n_mp,
-                            int N_pad, float* K32, float* Kt32, double* B, hipStream_t st);
+                            int N_pad, float* K32, float* Kt32, double* B, hipStream_t st,
+                            bool poisson = false);
 void launch_pack_images_f16(const float* images, const int* pix, int N, int N2d, int n_mp,
                             int N_pad, float kscale, bool split, _Float16* K16, _Float16* K16lo,
-                            _Float16* Kt16, _Float16* Kt16lo, double* B, hipStream_t st);
+                            _Float16* Kt16, _Float16* Kt16lo, double* B, hipStream_t st,
+                            bool poisson = false);
 void launch_images_stats(const float* images, const int* pix, int N, int N2d, int n_mp,
                          unsigned int* flags, hipStream_t st);
-// pack slices: compact FP32 or FP16 hi/lo, per-rotation scale, A_r
+// pack slices: compact FP32 or FP16 hi/lo, per-rotation scale, A_r; with
+// poisson the operand is log(max(S, kSlogFloor)) and A_r = sum_p m S
 void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d, int n_mp,
-                            int R_pad, float* S32, double* A, hipStream_t st);
+                            int R_pad, float* S32, double* A, hipStream_t st, bool poisson = false);
 void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, _Float16* S16, _Float16* S16lo, float* scale, double* A,
-                            hipStream_t st);
+                            hipStream_t st, bool poisson = false);
 
 // GEMMs (k_gemm.hip)
 struct LikeEpilogue {
@@ -198,6 +210,7 @@ struct LikeEpilogue {
   float kscale_inv;       // 1 / k_scale (FP16 path)
   double inv_2nms2;       // 1 / (2 n_m sigma^2)
   int N, R;
+  int poisson;            // L = C + B_i log phi_i - phi_i A_r (C over log S)
   int vecL;               // L rows 16-byte aligned (N % 4 == 0, aligned base)
 };
 void launch_estep(int precision, bool ksplit, const void* K, const void* Klo, const void* S,
@@ -207,7 +220,7 @@ struct MstepArgs {
   const float* P;         // [R][N]
   const float* invphi;    // [N]
   const float* rowscale;  // [R] (FP16 path)
-  const float* Tf;        // [R]
+  const float* Tf;        // [R] divisor of U
   const int* pix;         // [n_mp]
   float* slices;          // [R][N2d] full output (masked columns written)
   float* Sc;              // [R_pad][n_mp] compact output
@@ -215,6 +228,7 @@ struct MstepArgs {
   int N, R, N_pad, R_pad, n_mp, n_m, N2d;
   int vecP;               // P rows 16-byte aligned (N % 4 == 0, aligned base)
   int raw = 0;            // write the unnormalised sums U to Sc only
+  int poisson = 0;        // S' = Tf > 1e-30 ? U / Tf : 0 (else the reference's rule)
 };
 void launch_mstep32(const float* Kt, const MstepArgs& a, hipStream_t st);
 // FP16 hi/lo forms (k_gemm16.hip)
@@ -230,7 +244,7 @@ void launch_softmax(float* L, int R, int N, int chunks, float* smax, double* ssu
                     const double* A, bool fuse_phi, hipStream_t st);
 void launch_softmax_reduce(int N, int chunks, const double* sacc, bool fuse_phi,
                            const double* B, const float* phi_used, double sigma2x,
-                           double* phiden, double* red, hipStream_t st);
+                           double* phiden, double* red, hipStream_t st, bool poisson = false);
 void launch_total_respons(const float* P, size_t n, double* red, hipStream_t st);
 void launch_scaling_finish(int N, const double* B, const double* phiden, float* scaling,
                            float* invphi, hipStream_t st);
@@ -239,7 +253,10 @@ void launch_reciprocal(const float* s, int N, float* inv, hipStream_t st);
 void launch_sum_f64(const double* v, int n, double* out, hipStream_t st);
 // model helpers (k_model.hip)
 void launch_rowsum(const float* P, int R, int N, const float* invphi, double* T, float* Tf,
-                   float* rowscale, bool want_scale, hipStream_t st);
+                   float* rowscale, bool want_scale, hipStream_t st, const float* phi = nullptr,
+                   float* Tphi = nullptr);
+// den[i] = sum_r P[r][i] A[r] (Poisson scaling denominator)
+void launch_p_dot_a(const float* P, int R, int N, const double* A, double* den, hipStream_t st);
 void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* mask,
                           const float* wT, const float* rot, int R, int S, const float* x,
                           const float* y, const float* z, float* model, float* weight,

@@ -33,7 +33,8 @@ __device__ __forceinline__ void tile_of_block(int tiles_m, int tiles_n, int& tm,
 
 // Likelihood epilogue for one lane block: rows i0..i0+3 (patterns), column r
 // (rotation).  C = acc * cscale[r] / kscale, then in FP64
-// L = -(A_r - 2 C/phi_i + B_i/phi_i^2) / (2 n_m sigma^2), stored to L[r][i].
+// L = -(A_r - 2 C/phi_i + B_i/phi_i^2) / (2 n_m sigma^2) (Gaussian) or
+// L = C + B_i log phi_i - phi_i A_r (Poisson, C over log S), stored to L[r][i].
 __device__ __forceinline__ void like_store(const LikeEpilogue& ep, int i0, int r, floatx4_t acc) {
   if (r >= ep.R) return;
   const double cs = (double)(ep.cscale ? ep.cscale[r] : 1.0f) * (double)ep.kscale_inv;
@@ -46,7 +47,8 @@ __device__ __forceinline__ void like_store(const LikeEpilogue& ep, int i0, int r
     if (i < ep.N) {
       const double c = (double)acc[j] * cs;
       const double phi = (double)ep.phi[i];
-      out[j] = (float)(-(a - 2.0 * c / phi + ep.B[i] / (phi * phi)) * ep.inv_2nms2);
+      out[j] = ep.poisson ? (float)(c + ep.B[i] * log(phi) - phi * a)
+                          : (float)(-(a - 2.0 * c / phi + ep.B[i] / (phi * phi)) * ep.inv_2nms2);
     }
   }
   float* dst = ep.L + (size_t)r * ep.N + i0;
@@ -110,7 +112,7 @@ __device__ __forceinline__ void mstep_store(const MstepArgs& a, int r0, int k, f
       continue;
     }
     const float T = a.Tf[r];
-    const float s = T > 1e-10f ? u / T : u;
+    const float s = a.poisson ? (T > 1e-30f ? u / T : 0.0f) : (T > 1e-10f ? u / T : u);
     a.slices[(size_t)r * a.N2d + p] = s;
     a.Sc[(size_t)r * a.n_mp + k] = s;
   }

@@ -233,29 +233,42 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // The k largest logits of each pattern (logits [C][B]), descending, ties by
-// lower index; k repeated maximum scans below the previous pick.  With mass
-// != nullptr, mass[b] = softmax probability held by the k picks.
-__global__ void k_topk(const float* __restrict__ logits, int C, int B, int k, const int* __restrict__ out_rows,
-                       int* __restrict__ out, float* __restrict__ mass) {
-  const int b = blockIdx.x * 256 + threadIdx.x;
+// lower index: one wave per pattern, k rounds of a wave-wide (value, index)
+// maximum below the previous pick.  With mass != nullptr, mass[b] = softmax
+// probability held by the k picks.
+__global__ __launch_bounds__(256) void k_topk(const float* __restrict__ logits, int C, int B, int k,
+                                              const int* __restrict__ out_rows, int* __restrict__ out,
+                                              float* __restrict__ mass) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (b >= B) return;
+  const long row = out_rows ? out_rows[b] : b;
   float last_v = 3.0e38f;
   int last_i = -1;
-  const long row = out_rows ? out_rows[b] : b;
-  float top = -3.0e38f;
+  float top = 0.0f;
   double picked = 0.0;
   for (int j = 0; j < k; j++) {
     float bv = -3.0e38f;
-    int bi = -1;
-    for (int c = 0; c < C; c++) {
+    int bi = 0x7fffffff;
+    for (int c = lane; c < C; c += 64) {
       const float v = logits[(long)c * B + b];
       const bool below = v < last_v || (v == last_v && c > last_i);
-      if (below && (bi < 0 || v > bv)) {
+      if (below && (v > bv || (v == bv && c < bi))) {
         bv = v;
         bi = c;
       }
     }
-    out[row * k + j] = bi;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float ov = __shfl_xor(bv, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    if (bi == 0x7fffffff) bi = -1;
+    if (lane == 0) out[row * k + j] = bi;
     if (j == 0) top = bv;
     if (bi >= 0) picked += exp((double)bv - (double)top);
     last_v = bv;
@@ -263,8 +276,10 @@ __global__ void k_topk(const float* __restrict__ logits, int C, int B, int k, co
   }
   if (mass) {
     double z = 0.0;
-    for (int c = 0; c < C; c++) z += exp((double)logits[(long)c * B + b] - (double)top);
-    mass[row] = (float)(picked / z);
+    for (int c = lane; c < C; c += 64) z += exp((double)logits[(long)c * B + b] - (double)top);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
+    if (lane == 0) mass[row] = (float)(picked / z);
   }
 }
 
@@ -314,7 +329,7 @@ void adam(float* p, const float* g, float* m, float* v, long n, float lr, float 
   EMC_LAUNCHED();
 }
 void topk(const float* logits, int C, int B, int k, const int* out_rows, int* out, float* mass, hipStream_t st) {
-  hipLaunchKernelGGL(k_topk, dim3(nblk(B)), dim3(256), 0, st, logits, C, B, k, out_rows, out, mass);
+  hipLaunchKernelGGL(k_topk, dim3((B + 3) / 4), dim3(256), 0, st, logits, C, B, k, out_rows, out, mass);
   EMC_LAUNCHED();
 }
 void coarse_targets(const float* P, int N, const int* cell_off, const int* cell_rot, int C, float* T, hipStream_t st) {

@@ -112,7 +112,7 @@ void launch_images_stats(const float* images, const int* pix, int N, int N2d, in
 __global__ __launch_bounds__(256) void k_pack_images_f32(const float* __restrict__ images,
                                                          const int* __restrict__ pix, int N, int N2d,
                                                          int n_mp, float* __restrict__ K32,
-                                                         double* __restrict__ B) {
+                                                         double* __restrict__ B, int poisson) {
   const int i = blockIdx.x;  // 0 .. N_pad-1
   double acc = 0.0;
   for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_pack_images_f32(const float* __restrict
     float v = 0.0f;
     if (i < N && p >= 0) v = images[(size_t)i * N2d + p];
     K32[(size_t)i * n_mp + k] = v;
-    acc += (double)v * v;
+    acc += poisson ? (double)v : (double)v * v;
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0 && i < N) B[i] = acc;
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_pack_images_f16(const float* __restrict
                                                          int n_mp, float kscale, int split,
                                                          _Float16* __restrict__ K16,
                                                          _Float16* __restrict__ K16lo,
-                                                         double* __restrict__ B) {
+                                                         double* __restrict__ B, int poisson) {
   const int i = blockIdx.x;
   double acc = 0.0;
   for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_pack_images_f16(const float* __restrict
     const _Float16 hi = (_Float16)s;
     K16[(size_t)i * n_mp + k] = hi;
     if (split) K16lo[(size_t)i * n_mp + k] = (_Float16)(s - (float)hi);
-    acc += (double)v * v;
+    acc += poisson ? (double)v : (double)v * v;
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0 && i < N) B[i] = acc;
@@ -174,38 +174,48 @@ static void transpose(const T* in, T* out, int rows, int cols, hipStream_t st) {
 }
 
 void launch_pack_images_f32(const float* images, const int* pix, int N, int N2d, int n_mp,
-                            int N_pad, float* K32, float* Kt32, double* B, hipStream_t st) {
-  hipLaunchKernelGGL(k_pack_images_f32, dim3(N_pad), dim3(256), 0, st, images, pix, N, N2d, n_mp, K32, B);
+                            int N_pad, float* K32, float* Kt32, double* B, hipStream_t st, bool poisson) {
+  hipLaunchKernelGGL(k_pack_images_f32, dim3(N_pad), dim3(256), 0, st, images, pix, N, N2d, n_mp, K32, B,
+                     (int)poisson);
   EMC_LAUNCHED();
   transpose<float>(K32, Kt32, N_pad, n_mp, st);
 }
 
 void launch_pack_images_f16(const float* images, const int* pix, int N, int N2d, int n_mp,
                             int N_pad, float kscale, bool split, _Float16* K16, _Float16* K16lo,
-                            _Float16* Kt16, _Float16* Kt16lo, double* B, hipStream_t st) {
+                            _Float16* Kt16, _Float16* Kt16lo, double* B, hipStream_t st, bool poisson) {
   hipLaunchKernelGGL(k_pack_images_f16, dim3(N_pad), dim3(256), 0, st, images, pix, N, N2d, n_mp,
-                     kscale, (int)split, K16, K16lo, B);
+                     kscale, (int)split, K16, K16lo, B, (int)poisson);
   EMC_LAUNCHED();
   transpose<_Float16>(K16, Kt16, N_pad, n_mp, st);
   if (split) transpose<_Float16>(K16lo, Kt16lo, N_pad, n_mp, st);
 }
 
 // ---------------------------------------------------------------------------
-// Slice packing: compact masked columns, A_r = sum_p m S^2 in double.
+// Slice packing: compact masked columns, A_r = sum_p m S^2 in double.  For
+// the Poisson likelihood the operand is log(max(S, kSlogFloor)) and
+// A_r = sum_p m S.
 // ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float slice_operand(float v, int poisson) {
+  return poisson ? logf(fmaxf(v, kSlogFloor)) : v;
+}
 
 __global__ __launch_bounds__(256) void k_pack_slices_f32(const float* __restrict__ slices,
                                                          const int* __restrict__ pix, int R, int N2d,
                                                          int n_mp, float* __restrict__ S32,
-                                                         double* __restrict__ A) {
+                                                         double* __restrict__ A, int poisson) {
   const int r = blockIdx.x;
   double acc = 0.0;
   for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
     const int p = pix[k];
-    float v = 0.0f;
-    if (r < R && p >= 0) v = slices[(size_t)r * N2d + p];
-    S32[(size_t)r * n_mp + k] = v;
-    acc += (double)v * v;
+    float v = 0.0f, o = 0.0f;
+    if (r < R && p >= 0) {
+      v = slices[(size_t)r * N2d + p];
+      o = slice_operand(v, poisson);
+    }
+    S32[(size_t)r * n_mp + k] = o;
+    acc += poisson ? (double)v : (double)v * v;
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0 && r < R) A[r] = acc;
@@ -218,14 +228,14 @@ __global__ __launch_bounds__(256) void k_pack_slices_f16(const float* __restrict
                                                          int n_mp, _Float16* __restrict__ S16,
                                                          _Float16* __restrict__ S16lo,
                                                          float* __restrict__ scale,
-                                                         double* __restrict__ A) {
+                                                         double* __restrict__ A, int poisson) {
   const int r = blockIdx.x;
   const bool live = r < R;
   float mx = 0.0f;
   if (live)
     for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
       const int p = pix[k];
-      if (p >= 0) mx = fmaxf(mx, fabsf(slices[(size_t)r * N2d + p]));
+      if (p >= 0) mx = fmaxf(mx, fabsf(slice_operand(slices[(size_t)r * N2d + p], poisson)));
     }
   mx = block_max_f32(mx);
   float sc = 1.0f;
@@ -237,13 +247,16 @@ __global__ __launch_bounds__(256) void k_pack_slices_f16(const float* __restrict
   double acc = 0.0;
   for (int k = threadIdx.x; k < n_mp; k += blockDim.x) {
     const int p = pix[k];
-    float v = 0.0f;
-    if (live && p >= 0) v = slices[(size_t)r * N2d + p];
-    const float s = v * sc;
+    float v = 0.0f, o = 0.0f;
+    if (live && p >= 0) {
+      v = slices[(size_t)r * N2d + p];
+      o = slice_operand(v, poisson);
+    }
+    const float s = o * sc;
     const _Float16 hi = (_Float16)s;
     S16[(size_t)r * n_mp + k] = hi;
     S16lo[(size_t)r * n_mp + k] = (_Float16)(s - (float)hi);
-    acc += (double)v * v;
+    acc += poisson ? (double)v : (double)v * v;
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0 && live) {
@@ -253,16 +266,17 @@ __global__ __launch_bounds__(256) void k_pack_slices_f16(const float* __restrict
 }
 
 void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d, int n_mp,
-                            int R_pad, float* S32, double* A, hipStream_t st) {
-  hipLaunchKernelGGL(k_pack_slices_f32, dim3(R_pad), dim3(256), 0, st, slices, pix, R, N2d, n_mp, S32, A);
+                            int R_pad, float* S32, double* A, hipStream_t st, bool poisson) {
+  hipLaunchKernelGGL(k_pack_slices_f32, dim3(R_pad), dim3(256), 0, st, slices, pix, R, N2d, n_mp, S32, A,
+                     (int)poisson);
   EMC_LAUNCHED();
 }
 
 void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, _Float16* S16, _Float16* S16lo, float* scale, double* A,
-                            hipStream_t st) {
+                            hipStream_t st, bool poisson) {
   hipLaunchKernelGGL(k_pack_slices_f16, dim3(R_pad), dim3(256), 0, st, slices, pix, R, N2d, n_mp, S16,
-                     S16lo, scale, A);
+                     S16lo, scale, A, (int)poisson);
   EMC_LAUNCHED();
 }
 

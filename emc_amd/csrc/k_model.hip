@@ -18,21 +18,25 @@ namespace emc {
 // M-step, a power-of-two row scale with max_i P_ri/phi_i * scale <= 2^14.
 __global__ __launch_bounds__(256) void k_rowsum(const float* __restrict__ P, int R, int N,
                                                 const float* __restrict__ invphi, double* __restrict__ T,
-                                                float* __restrict__ Tf, float* __restrict__ rowscale, int want) {
+                                                float* __restrict__ Tf, float* __restrict__ rowscale, int want,
+                                                const float* __restrict__ phi, float* __restrict__ Tphi) {
   const int r = blockIdx.x;
   const float* row = P + (size_t)r * N;
-  double t = 0.0;
+  double t = 0.0, tp = 0.0;
   float mx = 0.0f;
   for (int i = threadIdx.x; i < N; i += 256) {
     const float p = row[i];
     t += p;
+    if (phi) tp += (double)p * (double)phi[i];
     if (want) mx = fmaxf(mx, fabsf(p * invphi[i]));
   }
   t = block_sum_f64(t);
+  if (phi) tp = block_sum_f64(tp);
   if (want) mx = block_max_f32(mx);
   if (threadIdx.x == 0) {
     T[r] = t;
     Tf[r] = (float)t;
+    if (phi) Tphi[r] = (float)tp;
     if (want) {
       float sc = 1.0f;
       if (mx > 0.0f && mx < 3.0e38f) {
@@ -46,8 +50,24 @@ __global__ __launch_bounds__(256) void k_rowsum(const float* __restrict__ P, int
 }
 
 void launch_rowsum(const float* P, int R, int N, const float* invphi, double* T, float* Tf, float* rowscale,
-                   bool want_scale, hipStream_t st) {
-  hipLaunchKernelGGL(k_rowsum, dim3(R), dim3(256), 0, st, P, R, N, invphi, T, Tf, rowscale, (int)want_scale);
+                   bool want_scale, hipStream_t st, const float* phi, float* Tphi) {
+  hipLaunchKernelGGL(k_rowsum, dim3(R), dim3(256), 0, st, P, R, N, invphi, T, Tf, rowscale, (int)want_scale, phi,
+                     Tphi);
+  EMC_LAUNCHED();
+}
+
+// den[i] = sum_r P[r][i] A[r] in FP64 (lanes along patterns: coalesced rows).
+__global__ __launch_bounds__(256) void k_p_dot_a(const float* __restrict__ P, int R, int N,
+                                                 const double* __restrict__ A, double* __restrict__ den) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  double s = 0.0;
+  for (int r = 0; r < R; r++) s += (double)P[(size_t)r * N + i] * A[r];
+  den[i] = s;
+}
+
+void launch_p_dot_a(const float* P, int R, int N, const double* A, double* den, hipStream_t st) {
+  hipLaunchKernelGGL(k_p_dot_a, dim3((N + 255) / 256), dim3(256), 0, st, P, R, N, A, den);
   EMC_LAUNCHED();
 }
 

@@ -32,9 +32,10 @@ __global__ __launch_bounds__(256) void k_cand_estep(CandEstepArgs a) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   static_assert(BM == kCandRows && BN == kSlot, "tile must match the work items");
   __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
-  int wi, unused;
-  tile_of_block(a.n_work, 1, wi, unused);
-  const int2 w = a.work[wi];
+  // work items in cell order, round robin over the XCDs: the whole chip
+  // works on one window of neighbouring cells, whose patterns' count rows
+  // then stay in the memory-side cache
+  const int2 w = a.work[blockIdx.x];
   const int c = w.x, e0 = w.y;
   const int e1 = min(e0 + BM, a.ent_off[c + 1]);
   const int r0 = a.cell_off[c], nr = a.cell_off[c + 1] - r0;
@@ -70,7 +71,8 @@ __global__ __launch_bounds__(256) void k_cand_estep(CandEstepArgs a) {
         const int i = a.ent_pat[e];
         const double phi = (double)a.phi[i];
         const double cc = (double)acc[x][y][j] * cs;
-        const float L = (float)(-(Ar - 2.0 * cc / phi + a.B[i] / (phi * phi)) * a.inv_2nms2);
+        const float L = a.poisson ? (float)(cc + a.B[i] * log(phi) - phi * Ar)
+                                  : (float)(-(Ar - 2.0 * cc / phi + a.B[i] / (phi * phi)) * a.inv_2nms2);
         a.Lc[((size_t)i * a.topk + a.ent_j[e]) * kSlot + s] = L;
       }
   }
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(256) void k_cand_softmax(CandSoftmaxArgs a) {
   pl = wave_sum(pl);
   if (lane == 0) {
     const double f = (double)a.phi[i];
-    a.phiden[i] = 0.5 * f * (pa + a.B[i] / (f * f) * ps + a.sigma2x * pl);
+    a.phiden[i] = a.poisson ? pa : 0.5 * f * (pa + a.B[i] / (f * f) * ps + a.sigma2x * pl);
     a.plogp[i] = plogp;
   }
 }
@@ -286,7 +288,7 @@ void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int
 // largest P/phi near 2^14), and the operand itself: per chunk of 32 entries
 // a [32 slots][32 entries] panel of P/phi * scale in hi and lo halves.
 __global__ __launch_bounds__(256) void k_cand_prep(CandPrepArgs a) {
-  __shared__ double tpart[8][kSlot];
+  __shared__ double tpart[8][kSlot], ppart[8][kSlot];
   __shared__ float mpart[8][kSlot];
   __shared__ float scale[kSlot];
   __shared__ __attribute__((aligned(16))) _Float16 panel[2][kSlot * 32];
@@ -296,24 +298,27 @@ __global__ __launch_bounds__(256) void k_cand_prep(CandPrepArgs a) {
   const int tid = threadIdx.x;
   {
     const int s = tid & (kSlot - 1), grp = tid >> 5;
-    double t = 0.0;
+    double t = 0.0, tp = 0.0;
     float mx = 0.0f;
     for (int e = eb + grp; e < ee; e += 8) {
       const int i = a.ent_pat[e];
       const float p = a.Pc[((size_t)i * a.topk + a.ent_j[e]) * kSlot + s];
       t += p;
+      if (a.phi) tp += (double)p * (double)a.phi[i];
       mx = fmaxf(mx, fabsf(p * a.invphi[i]));
     }
     tpart[grp][s] = t;
+    ppart[grp][s] = tp;
     mpart[grp][s] = mx;
   }
   __syncthreads();
   if (tid < kSlot) {
     const int s = tid;
-    double t = 0.0;
+    double t = 0.0, tp = 0.0;
     float mx = 0.0f;
     for (int g = 0; g < 8; g++) {
       t += tpart[g][s];
+      tp += ppart[g][s];
       mx = fmaxf(mx, mpart[g][s]);
     }
     float sc = 1.0f;
@@ -329,6 +334,7 @@ __global__ __launch_bounds__(256) void k_cand_prep(CandPrepArgs a) {
       a.T[r] = T;
       a.Tf[r] = (float)T;
       a.rowscale[r] = sc;
+      if (a.phi) a.Tphi[r] = (float)((a.Tphisub ? (double)a.Tphisub[r] : 0.0) + tp);
     }
   }
   __syncthreads();
@@ -492,7 +498,7 @@ __global__ __launch_bounds__(256) void k_cand_mstep(CandMstepArgs a) {
         float u = acc[x][y][j] * inv_scale;
         float* dst = a.Sc + (size_t)r * a.n_mp + k;
         if (a.has_sub) u += *dst;
-        const float sv = T > 1e-10f ? u / T : u;
+        const float sv = a.poisson ? (T > 1e-30f ? u / T : 0.0f) : (T > 1e-10f ? u / T : u);
         *dst = sv;
         a.slices[(size_t)r * a.N2d + a.pix[k]] = sv;
       }

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_softmax_finish(float* __restrict__ L, i
 __global__ __launch_bounds__(256) void k_softmax_reduce(int N, int chunks, const double* __restrict__ sacc, int fuse,
                                                         const double* __restrict__ B, const float* __restrict__ phi,
                                                         double sigma2x, double* __restrict__ phiden,
-                                                        double* __restrict__ red) {
+                                                        double* __restrict__ red, int poisson) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   double plogp = 0.0;
   if (i < N) {
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(256) void k_softmax_reduce(int N, int chunks, const
     }
     if (fuse) {
       const double f = (double)phi[i];
-      phiden[i] = 0.5 * f * (pa + B[i] / (f * f) * ps + sigma2x * pl);
+      // Poisson: the denominator is sum_r P_ri A_r itself
+      phiden[i] = poisson ? pa : 0.5 * f * (pa + B[i] / (f * f) * ps + sigma2x * pl);
     }
   }
   plogp = block_sum_f64(plogp);
@@ -134,10 +135,10 @@ void launch_softmax(float* L, int R, int N, int chunks, float* smax, double* ssu
 // red must hold ceil(N/256) + 1 doubles; red[ceil(N/256)] = sum P log P.
 void launch_softmax_reduce(int N, int chunks, const double* sacc, bool fuse_phi, const double* B,
                            const float* phi_used, double sigma2x, double* phiden, double* red,
-                           hipStream_t st) {
+                           hipStream_t st, bool poisson) {
   const int nb = (N + 255) / 256;
   hipLaunchKernelGGL(k_softmax_reduce, dim3(nb), dim3(256), 0, st, N, chunks, sacc, (int)fuse_phi, B, phi_used,
-                     sigma2x, phiden, red);
+                     sigma2x, phiden, red, (int)poisson);
   EMC_LAUNCHED();
   hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, st, red, nb, red + nb);
   EMC_LAUNCHED();

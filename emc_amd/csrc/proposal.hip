@@ -160,6 +160,7 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used) {
     a.inv_2nms2 = inv_2nms2;
     a.n_mp = e.n_mp;
     a.topk = p.topk;
+    a.poisson = e.likelihood == LIKE_POISSON;
     a.Lc = p.d_Lc.as<float>();
     launch_cand_estep(split, a, st);
   }
@@ -184,6 +185,7 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used) {
     ep.inv_2nms2 = inv_2nms2;
     ep.N = p.n_sub;
     ep.R = R;
+    ep.poisson = e.likelihood == LIKE_POISSON;
     ep.vecL = p.n_sub % 4 == 0;
     launch_estep16(split, p.d_Ksub.p, p.d_Ksublo.p, e.d_S16.p, e.d_S16lo.p, p.n_sub_pad, e.R_pad, e.n_mp, ep, st);
   }
@@ -210,6 +212,7 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
     a.sigma2x = sigma2x;
     a.N = N;
     a.topk = p.topk;
+    a.poisson = e.likelihood == LIKE_POISSON;
     a.respons = respons;
     a.phiden = e.d_phiden.as<double>();
     a.plogp = p.d_plogp.as<double>();
@@ -231,7 +234,8 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
     launch_softmax(p.d_Lsub.as<float>(), R, ns, chunks, e.d_smax.as<float>(), e.d_ssum.as<double>(),
                    e.d_sacc.as<double>(), e.d_A.as<double>(), true, st);
     launch_softmax_reduce(ns, chunks, e.d_sacc.as<double>(), true, p.d_Bsub.as<double>(), p.d_phisub.as<float>(),
-                          sigma2x, p.d_phiden_sub.as<double>(), e.d_red.as<double>(), st);
+                          sigma2x, p.d_phiden_sub.as<double>(), e.d_red.as<double>(), st,
+                          e.likelihood == LIKE_POISSON);
     launch_sub_finish(p.d_Lsub.as<float>(), ns, p.d_sub.as<int>(), p.d_phiden_sub.as<double>(),
                       e.d_phiden.as<double>(), p.d_cand.as<int>(), p.topk, p.d_cell_off.as<int>(),
                       p.d_cell_rot.as<int>(), p.d_mass.as<float>(), st);
@@ -268,22 +272,27 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   return true;
 }
 
-bool proposal_mstep(int N, int R, float* slices, int N2d) {
+bool proposal_mstep(int N, int R, float* slices, int N2d, const float* scaling) {
   (void)N;
   Engine& e = engine();
   Proposal& p = proposal();
   hipStream_t st = e.stream;
   const bool split = !e.k_exact;
   const bool has_sub = p.n_sub > 0;
+  // Gaussian: operand P/phi, divisor T; Poisson: operand P, divisor sum P phi
+  const bool poisson = e.likelihood == LIKE_POISSON;
+  const float* opscale = poisson ? e.d_ones.as<float>() : e.d_invphi.as<float>();
   if (has_sub) {
     PhaseTimer pt(PH_AUDIT);
     const size_t pk = (size_t)e.R_pad * p.n_sub_pad, kt = (size_t)e.n_mp * p.n_sub_pad;
     if (!p.d_invphisub.ensure(sizeof(float) * p.n_sub) || !p.d_Psub16.ensure(2 * pk) || !p.d_Psub16lo.ensure(2 * pk) ||
         !p.d_Ktsub.ensure(2 * kt) || (split && !p.d_Ktsublo.ensure(2 * kt)))
       return false;
-    launch_gather_f32(e.d_invphi.as<float>(), p.d_sub.as<int>(), p.n_sub, p.d_invphisub.as<float>(), st);
+    launch_gather_f32(opscale, p.d_sub.as<int>(), p.n_sub, p.d_invphisub.as<float>(), st);
+    if (poisson) launch_gather_f32(scaling, p.d_sub.as<int>(), p.n_sub, p.d_phisub.as<float>(), st);
     launch_rowsum(p.d_Lsub.as<float>(), R, p.n_sub, p.d_invphisub.as<float>(), e.d_T.as<double>(), e.d_Tf.as<float>(),
-                  e.d_Pscale.as<float>(), true, st);
+                  e.d_Pscale.as<float>(), true, st, poisson ? p.d_phisub.as<float>() : nullptr,
+                  poisson ? e.d_Tphi.as<float>() : nullptr);
     MstepArgs ms{};
     ms.P = p.d_Lsub.as<float>();
     ms.invphi = p.d_invphisub.as<float>();
@@ -320,10 +329,13 @@ bool proposal_mstep(int N, int R, float* slices, int N2d) {
     a.cell_off = p.d_cell_off.as<int>();
     a.cell_rot = p.d_cell_rot.as<int>();
     a.chunk_off = p.d_chunk_off.as<long>();
-    a.invphi = e.d_invphi.as<float>();
+    a.invphi = opscale;
+    a.phi = poisson ? scaling : nullptr;
     a.Tsub = has_sub ? e.d_T.as<double>() : nullptr;
+    a.Tphisub = has_sub && poisson ? e.d_Tphi.as<float>() : nullptr;
     a.T = e.d_T.as<double>();
     a.Tf = e.d_Tf.as<float>();
+    a.Tphi = e.d_Tphi.as<float>();
     a.rowscale = p.d_crowscale.as<float>();
     a.panels = p.d_panels.as<_Float16>();
     a.C = p.C;
@@ -344,7 +356,8 @@ bool proposal_mstep(int N, int R, float* slices, int N2d) {
     a.cell_rot = p.d_cell_rot.as<int>();
     a.pix = e.d_pix.as<int>();
     a.rowscale = p.d_crowscale.as<float>();
-    a.Tf = e.d_Tf.as<float>();
+    a.Tf = poisson ? e.d_Tphi.as<float>() : e.d_Tf.as<float>();
+    a.poisson = poisson;
     a.Sc = e.d_Sc.as<float>();
     a.slices = slices;
     a.kscale_inv = 1.0f / e.k_scale;
@@ -468,6 +481,33 @@ int emc_hip_proposal_init(const real* q_fine, int n_rot, int n_coarse, int side,
   }
   std::vector<int> cell(n_rot);
   emc_assign_cells(q_fine, n_rot, qc.data(), C, cell.data());
+  // Number the cells along a nearest-neighbour walk through the coarse
+  // rotations: cells with nearby numbers are nearby in orientation, so the
+  // cell-major contractions work on cells that share patterns at the same
+  // time (their count rows stay in cache).
+  {
+    std::vector<int> walk_id(C, -1);
+    int cur = 0;
+    for (int pos = 0; pos < C; pos++) {
+      walk_id[cur] = pos;
+      const float* a = &qc[4 * (size_t)cur];
+      double best = -1.0;
+      int next = -1;
+      for (int c = 0; c < C; c++) {
+        if (walk_id[c] >= 0) continue;
+        const float* b = &qc[4 * (size_t)c];
+        const double d = std::fabs((double)a[0] * b[0] + (double)a[1] * b[1] + (double)a[2] * b[2] +
+                                   (double)a[3] * b[3]);
+        if (d > best) {
+          best = d;
+          next = c;
+        }
+      }
+      if (next < 0) break;
+      cur = next;
+    }
+    for (int r = 0; r < n_rot; r++) cell[r] = walk_id[cell[r]];
+  }
   std::vector<int> off(C + 1, 0), rot(n_rot);
   for (int r = 0; r < n_rot; r++) off[cell[r] + 1]++;
   int largest = 0;

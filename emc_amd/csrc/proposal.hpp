@@ -68,6 +68,6 @@ Proposal& proposal();
 bool proposal_applies(int N, int R);
 bool proposal_estep(int N, int R, float sigma, const float* phi_used);
 bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* total);
-bool proposal_mstep(int N, int R, float* slices, int N2d);
+bool proposal_mstep(int N, int R, float* slices, int N2d, const float* scaling);
 
 }  // namespace emc

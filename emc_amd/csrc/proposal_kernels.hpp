@@ -24,6 +24,7 @@ struct CandEstepArgs {
   float kscale_inv;
   double inv_2nms2;
   int n_mp, topk;
+  int poisson;                      // Poisson epilogue (C over log S)
   float* Lc;                        // [N][topk][kSlot]
 };
 
@@ -36,6 +37,7 @@ struct CandSoftmaxArgs {
   const float* phi;
   double sigma2x;                   // 2 n_m sigma^2
   int N, topk;
+  int poisson;                      // scaling denominator sum_r P_ri A_r
   float* respons;                   // dense [R][N] (zeroed) or nullptr
   double *phiden, *plogp;           // [N]
 };
@@ -44,10 +46,13 @@ struct CandPrepArgs {
   const float* Pc;                  // [N][topk][kSlot]
   const int *ent_off, *ent_pat, *ent_j, *cell_off, *cell_rot;
   const long* chunk_off;            // [C] first 32-entry chunk of each cell
-  const float* invphi;
+  const float* invphi;              // operand P * invphi (ones for Poisson)
+  const float* phi;                 // Poisson: also Tphi_r = sum P phi, else nullptr
   const double* Tsub;               // [R] subset row sums or nullptr
+  const float* Tphisub;             // [R] subset sum P phi (Poisson) or nullptr
   double* T;
   float *Tf, *rowscale;
+  float* Tphi;                      // [R] (Poisson)
   _Float16* panels;                 // [chunks][2][kSlot][32]
   int C, topk;
 };
@@ -58,11 +63,12 @@ struct CandMstepArgs {
   const _Float16* panels;
   const long* chunk_off;
   const int *ent_off, *ent_pat, *cell_off, *cell_rot, *pix;
-  const float *rowscale, *Tf;
+  const float *rowscale, *Tf;        // Tf: divisor of U (T_r, or Tphi_r for Poisson)
   float* Sc;                        // [R_pad][n_mp]: subset U in (if has_sub), S' out
   float* slices;                    // [R][N2d]
   float kscale_inv;
   int has_sub, C, n_ptiles, n_mp, n_m, N2d;
+  int poisson;
 };
 
 void launch_cand_estep(bool ksplit, const CandEstepArgs& a, hipStream_t st);

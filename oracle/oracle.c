@@ -717,8 +717,9 @@ void o64_likelihood(const double *slices, const float *images, const int *mask, 
       } else {
         for (int p = 0; p < N2d; p++)
           if (mask[p] != 0) {
-            double lam = scaling[i] * sl[p];
-            s += (double)img[p] * log(lam > 1e-30 ? lam : 1e-30) - lam;
+            /* log(phi max(S, 1e-6)) - phi S: the floor is on the slice value */
+            const double sv = sl[p] > 1e-6 ? sl[p] : 1e-6;
+            s += (double)img[p] * (log(scaling[i]) + log(sv)) - scaling[i] * sl[p];
           }
         L[(size_t)r * N + i] = s;
       }

@@ -79,6 +79,12 @@ void r32_insert_slice(float *model, float *weight, int S, const float *slice, co
 
 /* ---- FP64 oracle ---- */
 enum { O_NEAREST = 0, O_TRILINEAR = 1 };
+/* O_GAUSSIAN: the reference's L = -sum m (S - K/phi)^2 / (2 n_m sigma^2);
+ * phi = sum m K^2 / sum_r P sum m K S; S' = sum_i P K/phi / T (T > 1e-10).
+ * O_POISSON (no counterpart in the reference's GPU path):
+ * L = sum m [K (log phi + log max(S, 1e-6)) - phi S];
+ * phi = sum m K / sum_r P sum m S; S' = sum_i P K / sum_i P phi (> 1e-30, else 0).
+ * Insertion weights are w_r T_r in both. */
 enum { O_GAUSSIAN = 0, O_POISSON = 1 };
 void o64_expand(const double *model, int S, const float *q, int R, const float *x,
                 const float *y, const float *z, int interp, double *slices);

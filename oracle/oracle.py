@@ -300,14 +300,14 @@ def o64_normalize_model(model, weight):
     return m
 
 
-def o64_iteration(model, images, mask, scaling, S, q, weights, x, y, z, sigma=0.4, interp=NEAREST):
+def o64_iteration(model, images, mask, scaling, S, q, weights, x, y, z, sigma=0.4, interp=NEAREST, kind=GAUSSIAN):
     """One FP64 iteration in the driver order; returns a dict of every output."""
     sl = o64_expand(model, S, q, x, y, z, interp)
-    L = o64_likelihood(sl, images, mask, sigma, scaling)
+    L = o64_likelihood(sl, images, mask, sigma, scaling, kind)
     P = o64_softmax(L)
     tot = o64_total_respons(P)
-    phi = o64_update_scaling(images, sl, mask, P)
-    sl2, T = o64_mstep(images, mask, P, phi, sl)
+    phi = o64_update_scaling(images, sl, mask, P, kind)
+    sl2, T = o64_mstep(images, mask, P, phi, sl, kind)
     m, wg = o64_insert(sl2, mask, T, weights, S, q, x, y, z, interp)
     mn = o64_normalize_model(m, wg)
     return {"slices": sl, "L": L, "P": P, "total": tot, "scaling": phi, "slices_new": sl2, "T": T,
@@ -356,15 +356,15 @@ def allowed_from_candidates(cand, cell_off, cell_rot, R, sub=None):
 
 
 def o64_proposal_iteration(model, images, mask, scaling, S, q, weights, x, y, z, allowed, sigma=0.4,
-                           interp=NEAREST):
+                           interp=NEAREST, kind=GAUSSIAN):
     """One FP64 iteration in the driver order with the responsibilities
     restricted to `allowed` ([R][N] bool)."""
     sl = o64_expand(model, S, q, x, y, z, interp)
-    L = o64_likelihood(sl, images, mask, sigma, scaling)
+    L = o64_likelihood(sl, images, mask, sigma, scaling, kind)
     P = o64_restricted_softmax(L, allowed)
     tot = o64_total_respons(P)
-    phi = o64_update_scaling(images, sl, mask, P)
-    sl2, T = o64_mstep(images, mask, P, phi, sl)
+    phi = o64_update_scaling(images, sl, mask, P, kind)
+    sl2, T = o64_mstep(images, mask, P, phi, sl, kind)
     m, wg = o64_insert(sl2, mask, T, weights, S, q, x, y, z, interp)
     mn = o64_normalize_model(m, wg)
     return {"slices": sl, "L": L, "P": P, "total": tot, "scaling": phi, "slices_new": sl2, "T": T,

@@ -232,3 +232,32 @@ def test_trained_network_gain(prob):
     assert losses[-1] < 0.5 * losses[0]
     assert m1.mean() > 0.9 and m1.mean() > m0.mean() + 0.5
     assert np.mean(m1 >= 0.99) > np.mean(m0 >= 0.99) + 0.5
+
+
+def test_prediction_is_topk_of_network_output(prob):
+    """Candidates = the network's k largest outputs (descending, ties to the
+    lower cell), mass = softmax probability they hold."""
+    P = prob
+    topk = 8
+    nc, off, rot = init_cells(P, topk, batch=64)
+    L = E.lib()
+    d_img = E.DeviceArray.from_host(P.images.reshape(-1))
+    d_mask = E.DeviceArray.from_host(P.mask)
+    n = 40
+    assert L.emc_hip_proposal_predict(d_img.vp, d_mask.vp, P.side ** 2, P.N) == 0
+    cand = np.zeros((P.N, topk), np.int32)
+    mass = np.zeros(P.N, np.float32)
+    assert L.emc_hip_proposal_get(E.iptr(cand), None, E.fptr(mass), P.N) == 0
+    idx = np.arange(5, 5 + n, dtype=np.int32)
+    logits = np.zeros((nc, n), np.float32)
+    assert L.emc_hip_cnn_forward(d_img.vp, d_mask.vp, P.side ** 2, E.iptr(idx), n, E.fptr(logits)) == 0
+    d_img.free()
+    d_mask.free()
+    E.check()
+    ref = np.argsort(-logits, axis=0, kind="stable")[:topk].T
+    assert np.array_equal(cand[idx], ref)
+    lg = logits.astype(np.float64)
+    p = np.exp(lg - lg.max(axis=0))
+    p /= p.sum(axis=0)
+    ref_mass = np.take_along_axis(p, ref.T, axis=0).sum(axis=0)
+    assert np.allclose(mass[idx], ref_mass, rtol=1e-5, atol=1e-6)
