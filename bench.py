@@ -114,7 +114,7 @@ def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coa
     E.set_option("proposal", "off")
     E.set_option("audit_fraction", "0.02")
     E.set_option("min_mass", "0.999")
-    E.set_option("proposal_gate", "predicted")
+    E.set_option("proposal_gate", "audit")
     E.set_option("timing", "0")
     N, R, side = W.N, W.R, W.cfg.side
     prob = E.EmcProblem(side, W.images, W.mask, W.q, W.w, W.model0)
@@ -145,27 +145,51 @@ def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coa
     predict_s = time.perf_counter() - t0
     gated = proposal_stats(L)["gated_at_prediction"]
     pmass = np.zeros(N, np.float32)
-    L.emc_hip_proposal_get(None, None, E.fptr(pmass), N)
+    cand = np.zeros((N, topk), np.int32)
+    L.emc_hip_proposal_get(E.iptr(cand), None, E.fptr(pmass), N)
     pq = np.percentile(pmass, [1, 10, 50]).tolist()
+    per_cell = np.bincount(cand.reshape(-1), minlength=L.emc_hip_proposal_cells(None, None))
     log(f"  prediction {predict_s * 1e3:.0f} ms; {gated:.0f} patterns start in whole-table search; "
-        f"network top-k mass q1/q10/q50 {pq}")
+        f"network top-k mass q1/q10/q50 {pq}; entries per cell mean {per_cell.mean():.0f} "
+        f"p99 {np.percentile(per_cell, 99):.0f} max {per_cell.max()}")
     E.set_option("proposal", "on")
+    first_audit = None
     for it in range(warmup):
+        # the first proposal iteration audits every pattern: the search mode
+        # of each then follows its measured captured mass
+        E.set_option("audit_fraction", "1" if it == 0 else "0.02")
         history.append(prob.iteration(sigma))
         st = proposal_stats(L)
+        if it == 0:
+            first_audit = st
         log(f"  proposal warmup {it + 1}: whole table {st['whole_table']:.0f}, audit below min mass "
-            f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f}, min mass {st['audit_min_mass']:.6f}")
+            f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f}, min mass {st['audit_min_mass']:.6f}, "
+            f"mean {st['audit_mean_mass']:.6f}")
+    E.set_option("audit_fraction", "0.02")
     dt, ph, per, hist = timed(E, prob, sigma, steps, learned=True)
     history += hist
     total_s = time.perf_counter() - t_start
     model = prob.model()
+    # one more iteration from this state, learned and plain FP32, for the
+    # accuracy of a single iteration (free of the run-to-run divergence of
+    # the iterated map)
+    state_model, state_scaling = model.copy(), prob.scaling().copy()
+    prob.iteration(sigma)
+    one_learned = prob.model()
     prob.free()
     E.set_option("proposal", "off")
+    L.emc_hip_invalidate()
+    E.set_option("precision", "fp32")
+    ref = E.EmcProblem(side, W.images, W.mask, W.q, W.w, state_model, scaling=state_scaling)
+    ref.iteration(sigma)
+    one_plain = ref.model()
+    ref.free()
     L.emc_hip_invalidate()
     log(f"  learned: {dt * 1e3:.1f} ms/iteration")
     return {"dt": dt, "phases": ph, "model": model, "per_step": per, "dense_s": dense_s, "train_s": train_s,
             "predict_s": predict_s, "losses": losses, "gated": gated, "total_s": total_s,
-            "predicted_mass_q1_q10_q50": pq, "history": history}
+            "predicted_mass_q1_q10_q50": pq, "history": history, "first_audit": first_audit,
+            "one_step_cc": cc(one_learned, one_plain)}
 
 
 def cc(a, b):
@@ -213,7 +237,7 @@ def main():
     ap.add_argument("--train-iters", type=int, default=5, help="dense iterations whose responsibilities train")
     ap.add_argument("--epochs", default="1,1,1,2,6",
                     help="training passes after each dense iteration (comma list; the last repeats)")
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--topk", type=int, default=32)
     ap.add_argument("--coarse-n", type=int, default=4)
     ap.add_argument("--cpu-patterns", type=int, default=8, help="pattern sample for the CPU baseline")
@@ -306,6 +330,10 @@ def main():
             "dense_iterations_s": fast["dense_s"], "training_s": fast["train_s"], "prediction_s": fast["predict_s"],
             "whole_schedule_s": fast["total_s"],
             "network_topk_mass_q1_q10_q50": fast["predicted_mass_q1_q10_q50"],
+            "full_audit_first_iteration": {k: fast["first_audit"][k] for k in
+                                           ("audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass")}
+            if fast["first_audit"] else None,
+            "model_cc_one_iteration_vs_plain_fp32": fast["one_step_cc"],
             "sum_plogp_per_iteration": [h["likelihood"] for h in fast["history"]],
         }
     if plain is not None:

@@ -85,7 +85,7 @@ void Engine::release_all() {
   DBuf* bufs[] = {&d_pix, &d_B, &d_K32, &d_Kt32, &d_K16, &d_K16lo, &d_Kt16, &d_Kt16lo, &d_flags,
                   &d_S32, &d_S16, &d_S16lo, &d_Sscale, &d_A, &d_phi_used, &d_smax, &d_ssum, &d_sacc,
                   &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Tphi, &d_ones, &d_Pscale, &d_P16, &d_P16lo,
-                  &d_Sc, &d_wts, &d_wT, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
+                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
   for (DBuf* b : bufs) b->release();
   pack_key = {};
   map_key = {};
@@ -139,7 +139,7 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
   Engine& e = engine();
   auto& k = e.pack_key;
   if (k.images == images && k.mask == mask && k.N == N && k.N2d == N2d && k.precision == e.precision &&
-      k.likelihood == e.likelihood && k.stamp == e.invalidate_stamp)
+      k.likelihood == e.likelihood && k.sparse == (int)e.sparse && k.stamp == e.invalidate_stamp)
     return true;
   k = {};
   if (N <= 0 || N2d <= 0) {
@@ -217,6 +217,29 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
     launch_pack_images_f16(images, e.d_pix.as<int>(), N, N2d, e.n_mp, e.N_pad, e.k_scale, !e.k_exact,
                            e.d_K16.as<_Float16>(), e.d_K16lo.as<_Float16>(), e.d_Kt16.as<_Float16>(),
                            e.d_Kt16lo.as<_Float16>(), e.d_B.as<double>(), e.stream, e.likelihood == LIKE_POISSON);
+    if (e.sparse && e.likelihood == LIKE_POISSON) {
+      // CSR copy of the counts for the sparse E-step; the dense FP16 operands
+      // used beside it (M-step, candidates) need the counts exact as well
+      if (!e.k_exact) {
+        set_error("sparse mode needs integer photon counts <= 2048");
+        return false;
+      }
+      std::vector<int> nnz(N), off(N + 1, 0);
+      if (!e.d_csr_off.ensure(sizeof(int) * (N + 1))) return false;
+      launch_csr_row_nnz(images, e.d_pix.as<int>(), N, N2d, e.n_mp, e.d_csr_off.as<int>(), e.stream);
+      EMC_CHECK(hipMemcpyAsync(nnz.data(), e.d_csr_off.p, sizeof(int) * N, hipMemcpyDeviceToHost, e.stream));
+      sync();
+      if (failed()) return false;
+      for (int i = 0; i < N; i++) off[i + 1] = off[i] + nnz[i];
+      e.csr_nnz = (size_t)off[N];
+      if (!e.d_csr_col.ensure(sizeof(int) * std::max<size_t>(e.csr_nnz, 1)) ||
+          !e.d_csr_cnt.ensure(sizeof(unsigned short) * std::max<size_t>(e.csr_nnz, 1)))
+        return false;
+      EMC_CHECK(hipMemcpyAsync(e.d_csr_off.p, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, e.stream));
+      launch_csr_fill(images, e.d_pix.as<int>(), N, N2d, e.n_mp, e.d_csr_off.as<int>(), e.d_csr_col.as<int>(),
+                      e.d_csr_cnt.as<unsigned short>(), e.stream);
+      sync();
+    }
   }
   if (failed()) return false;
   k.images = images;
@@ -225,9 +248,16 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
   k.N2d = N2d;
   k.precision = e.precision;
   k.likelihood = e.likelihood;
+  k.sparse = (int)e.sparse;
   k.stamp = e.invalidate_stamp;
   e.like.valid = false;
   return true;
+}
+
+// The Poisson E-step reads the CSR counts (FP16 log-slice operand).
+static bool sparse_estep() {
+  const Engine& e = engine();
+  return e.sparse && e.likelihood == LIKE_POISSON && e.precision == PREC_FP16;
 }
 
 // Packed slice operand for the current precision from an [R][N2d] buffer.
@@ -246,6 +276,10 @@ static bool pack_slices(const float* slices, int R, int N2d) {
     launch_pack_slices_f16(slices, e.d_pix.as<int>(), R, N2d, e.n_mp, e.R_pad, e.d_S16.as<_Float16>(),
                            e.d_S16lo.as<_Float16>(), e.d_Sscale.as<float>(), e.d_A.as<double>(), e.stream,
                            e.likelihood == LIKE_POISSON);
+    if (sparse_estep()) {
+      if (!e.d_LSt.ensure(4 * ns)) return false;
+      launch_transpose_hilo(e.d_S16.as<_Float16>(), e.d_S16lo.as<_Float16>(), e.R_pad, e.n_mp, e.d_LSt.p, e.stream);
+    }
   }
   return !failed();
 }
@@ -272,6 +306,11 @@ static void run_estep(int N, int R, float sigma, const float* phi, float* L, con
   ep.poisson = e.likelihood == LIKE_POISSON;
   ep.vecL = (N % 4 == 0) && L && ((uintptr_t)L % 16 == 0);
   PhaseTimer pt(PH_LIKE);
+  if (L && !P && sparse_estep()) {
+    launch_estep_csr(e.d_csr_off.as<int>(), e.d_csr_col.as<int>(), e.d_csr_cnt.as<unsigned short>(), e.d_LSt.p,
+                     e.R_pad, ep, e.stream);
+    return;
+  }
   if (e.precision == PREC_FP32)
     launch_estep(PREC_FP32, false, e.d_K32.p, nullptr, e.d_S32.p, nullptr, e.N_pad, e.R_pad, e.n_mp, ep, e.stream);
   else
@@ -801,6 +840,10 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (is(value, "gaussian")) e.likelihood = LIKE_GAUSSIAN;
     else if (is(value, "poisson")) e.likelihood = LIKE_POISSON;
     else return 1;
+  } else if (is(key, "sparse")) {
+    if (is(value, "on")) e.sparse = true;
+    else if (is(value, "off")) e.sparse = false;
+    else return 1;
   } else if (is(key, "proposal")) {
     if (is(value, "on")) proposal().active = true;
     else if (is(value, "off")) proposal().active = false;
@@ -849,6 +892,42 @@ void emc_hip_reset_phase_times(void) {
 }
 
 const char* emc_hip_phase_name(int k) { return k >= 0 && k < PH_COUNT ? kPhaseNames[k] : nullptr; }
+
+int emc_hip_images_from_csr(real* d_images, const int* row_off, const int* pix, const unsigned short* counts,
+                            int N_images, int N_2d) {
+  if (!ready()) return 1;
+  Engine& e = engine();
+  if (!d_images || !row_off || N_images <= 0 || N_2d <= 0 || row_off[0] != 0) {
+    set_error("emc_hip_images_from_csr: invalid argument");
+    return 1;
+  }
+  const size_t nnz = (size_t)row_off[N_images];
+  for (int i = 0; i < N_images; i++)
+    if (row_off[i + 1] < row_off[i]) {
+      set_error("emc_hip_images_from_csr: row offsets decrease at pattern %d", i);
+      return 1;
+    }
+  for (size_t j = 0; j < nnz; j++)
+    if (pix[j] < 0 || pix[j] >= N_2d) {
+      set_error("emc_hip_images_from_csr: pixel index %d out of range", pix[j]);
+      return 1;
+    }
+  DBuf d_off, d_pix, d_cnt;
+  if (!d_off.ensure(sizeof(int) * (N_images + 1)) || !d_pix.ensure(sizeof(int) * std::max<size_t>(nnz, 1)) ||
+      !d_cnt.ensure(sizeof(unsigned short) * std::max<size_t>(nnz, 1)))
+    return 1;
+  EMC_CHECK(hipMemcpyAsync(d_off.p, row_off, sizeof(int) * (N_images + 1), hipMemcpyHostToDevice, e.stream));
+  if (nnz) {
+    EMC_CHECK(hipMemcpyAsync(d_pix.p, pix, sizeof(int) * nnz, hipMemcpyHostToDevice, e.stream));
+    EMC_CHECK(hipMemcpyAsync(d_cnt.p, counts, sizeof(unsigned short) * nnz, hipMemcpyHostToDevice, e.stream));
+  }
+  launch_images_from_csr(d_off.as<int>(), d_pix.as<int>(), d_cnt.as<unsigned short>(), N_images, N_2d, d_images,
+                         e.stream);
+  sync();
+  e.invalidate_stamp++;  // the patterns at d_images changed
+  e.like.valid = false;
+  return failed() ? 1 : 0;
+}
 
 int emc_hip_pack_info(int* out, int n) {
   Engine& e = engine();

@@ -68,6 +68,7 @@ struct Engine {
   int insert_mode = INSERT_ATOMIC;
   int interp = INTERP_NEAREST;
   int likelihood = LIKE_GAUSSIAN;
+  bool sparse = false;  // Poisson E-step from CSR counts (see k_sparse.hip)
   bool timing = false;
   bool verbose = false;
 
@@ -79,7 +80,7 @@ struct Engine {
   struct {
     const float* images = nullptr;
     const int* mask = nullptr;
-    int N = 0, N2d = 0, precision = -1, likelihood = -1;
+    int N = 0, N2d = 0, precision = -1, likelihood = -1, sparse = -1;
     uint64_t stamp = 0;  // invalidation stamp at build time
   } pack_key;
   int n_m = 0;      // masked pixels (mask != 0)
@@ -95,6 +96,10 @@ struct Engine {
   DBuf d_K16, d_K16lo;    // half [N_pad][n_mp]
   DBuf d_Kt16, d_Kt16lo;  // half [n_mp][N_pad]
   DBuf d_flags;     // scratch flags for the pack kernels
+  // CSR copy of the counts (sparse mode): row offsets [N+1], compact pixel
+  // index and count of every nonzero
+  DBuf d_csr_off, d_csr_col, d_csr_cnt;
+  size_t csr_nnz = 0;
 
   // ---- packed slices (per likelihood call) ----
   int R_pad = 0;
@@ -102,6 +107,7 @@ struct Engine {
   DBuf d_S16, d_S16lo;  // half [R_pad][n_mp]
   DBuf d_Sscale;        // float [R]: 1/scale_r, where the FP16 operand is S * scale_r (power of two)
   DBuf d_A;             // double [R]: sum_p m S^2 (Poisson: sum_p m S)
+  DBuf d_LSt;           // half2 [n_mp][R_pad]: (hi, lo) transposed (sparse mode)
 
   // ---- likelihood -> softmax -> scaling fusion state ----
   struct {
@@ -273,5 +279,14 @@ void launch_normalize_model(float* model, const float* weight, int n, double* re
                             hipStream_t st);
 void launch_model_max(const float* model, int n, float* out, hipStream_t st);
 void launch_fill(float* p, size_t n, float v, hipStream_t st);
+// CSR counts and the sparse Poisson E-step (k_sparse.hip)
+void launch_csr_row_nnz(const float* images, const int* pix, int N, int N2d, int n_mp, int* nnz, hipStream_t st);
+void launch_csr_fill(const float* images, const int* pix, int N, int N2d, int n_mp, const int* off, int* col,
+                     unsigned short* cnt, hipStream_t st);
+void launch_transpose_hilo(const _Float16* hi, const _Float16* lo, int R_pad, int n_mp, void* out, hipStream_t st);
+void launch_estep_csr(const int* off, const int* col, const unsigned short* cnt, const void* LSt, int R_pad,
+                      const LikeEpilogue& ep, hipStream_t st);
+void launch_images_from_csr(const int* off, const int* pix, const unsigned short* cnt, int N, int N2d, float* images,
+                            hipStream_t st);
 
 }  // namespace emc

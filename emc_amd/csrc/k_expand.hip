@@ -13,8 +13,17 @@
 
 namespace emc {
 
-// One workgroup per rotation; lanes walk the pixels in order so slice writes
-// and coordinate reads are coalesced; model reads are gathers served by L2.
+// Pixel p handled by lane slot t of a workgroup: each wave takes an 8 x 8
+// pixel tile, so the voxels it gathers lie on a small patch of the rotated
+// plane (few cache lines per wave) while its slice writes stay 32-byte rows.
+// Needs S % 8 == 0; otherwise pixels in row order.
+__device__ __forceinline__ int expand_pixel(int t, int S) {
+  if (S & 7) return t;
+  const int tile = t >> 6, l = t & 63, tiles_x = S >> 3;
+  return ((tile / tiles_x) * 8 + (l >> 3)) * S + (tile % tiles_x) * 8 + (l & 7);
+}
+
+// One workgroup per rotation; model reads are gathers served by L2.
 __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict__ model, int S,
                                                         const float* __restrict__ rot,
                                                         const float* __restrict__ x,
@@ -27,7 +36,8 @@ __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict_
   const float half = (float)S / 2.0f + 0.5f;
   const int N2d = S * S;
   float* out = slices + (size_t)r * N2d;
-  for (int p = threadIdx.x; p < N2d; p += blockDim.x) {
+  for (int t = threadIdx.x; t < N2d; t += blockDim.x) {
+    const int p = expand_pixel(t, S);
     float ux, uy, uz;
     voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
     const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
@@ -53,7 +63,8 @@ __global__ __launch_bounds__(256) void k_expand_trilinear(const float* __restric
   const float half = (float)S / 2.0f + 0.5f;
   const int N2d = S * S;
   float* out = slices + (size_t)r * N2d;
-  for (int p = threadIdx.x; p < N2d; p += blockDim.x) {
+  for (int t = threadIdx.x; t < N2d; t += blockDim.x) {
+    const int p = expand_pixel(t, S);
     float ux, uy, uz;
     voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
     float acc = 0.0f;

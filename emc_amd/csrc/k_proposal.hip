@@ -283,120 +283,162 @@ void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int
 // ---------------------------------------------------------------------------
 // M-step over the candidates
 // ---------------------------------------------------------------------------
-// One block per cell.  T_r = Tsub_r + sum over the cell's entries of P (fixed
-// order), the row scale of the FP16 operand (power of two placing the row's
-// largest P/phi near 2^14), and the operand itself: per chunk of 32 entries
-// a [32 slots][32 entries] panel of P/phi * scale in hi and lo halves.
-__global__ __launch_bounds__(256) void k_cand_prep(CandPrepArgs a) {
+// M-step preparation, in three load-balanced passes (cells differ widely in
+// how many patterns chose them):
+//  1. per 32-entry chunk: partial sums over its entries of P, P phi and the
+//     largest |P * opscale| for each of the 32 slots;
+//  2. per cell: those partials added in chunk order (fixed, so T_r is
+//     reproducible) + the subset's sums -> T_r, Tphi_r, and the power-of-two
+//     row scale placing the row's largest operand near 2^14;
+//  3. per chunk: the FP16 operand panel [32 slots][32 entries] in hi and lo.
+__device__ __forceinline__ int chunk_first_entry(const CandPrepArgs& a, long q, int& cell) {
+  cell = a.chunk_cell[q];
+  return a.ent_off[cell] + (int)(q - a.chunk_off[cell]) * 32;
+}
+
+__global__ __launch_bounds__(256) void k_prep_partial(CandPrepArgs a) {
   __shared__ double tpart[8][kSlot], ppart[8][kSlot];
   __shared__ float mpart[8][kSlot];
-  __shared__ float scale[kSlot];
-  __shared__ __attribute__((aligned(16))) _Float16 panel[2][kSlot * 32];
-  const int c = blockIdx.x;
-  const int eb = a.ent_off[c], ee = a.ent_off[c + 1];
-  const int r0 = a.cell_off[c], nr = a.cell_off[c + 1] - r0;
-  const int tid = threadIdx.x;
-  {
-    const int s = tid & (kSlot - 1), grp = tid >> 5;
-    double t = 0.0, tp = 0.0;
-    float mx = 0.0f;
-    for (int e = eb + grp; e < ee; e += 8) {
-      const int i = a.ent_pat[e];
-      const float p = a.Pc[((size_t)i * a.topk + a.ent_j[e]) * kSlot + s];
-      t += p;
-      if (a.phi) tp += (double)p * (double)a.phi[i];
-      mx = fmaxf(mx, fabsf(p * a.invphi[i]));
-    }
-    tpart[grp][s] = t;
-    ppart[grp][s] = tp;
-    mpart[grp][s] = mx;
+  const long q = blockIdx.x;
+  int cell;
+  const int e0 = chunk_first_entry(a, q, cell);
+  const int ee = min(e0 + 32, a.ent_off[cell + 1]);
+  const int tid = threadIdx.x, s = tid & (kSlot - 1), grp = tid >> 5;
+  double t = 0.0, tp = 0.0;
+  float mx = 0.0f;
+  for (int e = e0 + grp; e < ee; e += 8) {
+    const int i = a.ent_pat[e];
+    const float p = a.Pc[((size_t)i * a.topk + a.ent_j[e]) * kSlot + s];
+    t += p;
+    if (a.phi) tp += (double)p * (double)a.phi[i];
+    mx = fmaxf(mx, fabsf(p * a.invphi[i]));
   }
+  tpart[grp][s] = t;
+  ppart[grp][s] = tp;
+  mpart[grp][s] = mx;
   __syncthreads();
   if (tid < kSlot) {
-    const int s = tid;
-    double t = 0.0, tp = 0.0;
-    float mx = 0.0f;
+    double st = 0.0, sp = 0.0;
+    float m = 0.0f;
     for (int g = 0; g < 8; g++) {
-      t += tpart[g][s];
-      tp += ppart[g][s];
-      mx = fmaxf(mx, mpart[g][s]);
+      st += tpart[g][tid];
+      sp += ppart[g][tid];
+      m = fmaxf(m, mpart[g][tid]);
     }
-    float sc = 1.0f;
-    if (mx > 0.0f && mx < 3.0e38f) {
-      int ex;
-      frexpf(mx, &ex);
-      sc = ldexpf(1.0f, max(-120, min(120, 14 - ex)));
-    }
-    scale[s] = sc;
-    if (s < nr) {
-      const int r = a.cell_rot[r0 + s];
-      const double T = (a.Tsub ? a.Tsub[r] : 0.0) + t;
-      a.T[r] = T;
-      a.Tf[r] = (float)T;
-      a.rowscale[r] = sc;
-      if (a.phi) a.Tphi[r] = (float)((a.Tphisub ? (double)a.Tphisub[r] : 0.0) + tp);
-    }
-  }
-  __syncthreads();
-  // panels: thread (entry el = tid & 31, slots 4 * (tid >> 5) .. +3)
-  const int el = tid & 31, s4 = (tid >> 5) * 4;
-  const long chunk0 = a.chunk_off[c];
-  const int nch = (ee - eb + 31) / 32;
-  for (int q = 0; q < nch; q++) {
-    const int e = eb + q * 32 + el;
-    floatx4 v = {0, 0, 0, 0};
-    if (e < ee) {
-      const int i = a.ent_pat[e];
-      v = *reinterpret_cast<const floatx4*>(a.Pc + ((size_t)i * a.topk + a.ent_j[e]) * kSlot + s4);
-      v = v * a.invphi[i];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const float x = v[u] * scale[s4 + u];
-      const _Float16 h = (_Float16)x;
-      panel[0][(s4 + u) * 32 + el] = h;
-      panel[1][(s4 + u) * 32 + el] = (_Float16)(x - (float)h);
-    }
-    __syncthreads();
-    // 2 x 2 KB out, 16 bytes per thread
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    const int h = tid >> 7, o = (tid & 127) * 8;
-    *reinterpret_cast<u32x4*>(a.panels + ((chunk0 + q) * 2 + h) * (kSlot * 32) + o) =
-        *reinterpret_cast<const u32x4*>(&panel[h][o]);
-    __syncthreads();
+    a.part_T[q * kSlot + tid] = st;
+    a.part_Tphi[q * kSlot + tid] = sp;
+    a.part_max[q * kSlot + tid] = m;
   }
 }
 
-void launch_cand_prep(const CandPrepArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_cand_prep, dim3(a.C), dim3(256), 0, st, a);
+__global__ __launch_bounds__(64) void k_prep_reduce(CandPrepArgs a) {
+  const int c = blockIdx.x, s = threadIdx.x;
+  if (s >= kSlot) return;
+  const int r0 = a.cell_off[c], nr = a.cell_off[c + 1] - r0;
+  double t = 0.0, tp = 0.0;
+  float mx = 0.0f;
+  for (long q = a.chunk_off[c]; q < a.chunk_off[c + 1]; q++) {
+    t += a.part_T[q * kSlot + s];
+    tp += a.part_Tphi[q * kSlot + s];
+    mx = fmaxf(mx, a.part_max[q * kSlot + s]);
+  }
+  float sc = 1.0f;
+  if (mx > 0.0f && mx < 3.0e38f) {
+    int ex;
+    frexpf(mx, &ex);
+    sc = ldexpf(1.0f, max(-120, min(120, 14 - ex)));
+  }
+  if (s < nr) {
+    const int r = a.cell_rot[r0 + s];
+    const double T = (a.Tsub ? a.Tsub[r] : 0.0) + t;
+    a.T[r] = T;
+    a.Tf[r] = (float)T;
+    a.rowscale[r] = sc;
+    if (a.phi) a.Tphi[r] = (float)((a.Tphisub ? (double)a.Tphisub[r] : 0.0) + tp);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_prep_panels(CandPrepArgs a) {
+  __shared__ __attribute__((aligned(16))) _Float16 panel[2][kSlot * 32];
+  const long q = blockIdx.x;
+  int cell;
+  const int e0 = chunk_first_entry(a, q, cell);
+  const int ee = a.ent_off[cell + 1];
+  const int r0 = a.cell_off[cell], nr = a.cell_off[cell + 1] - r0;
+  const int tid = threadIdx.x;
+  // thread (entry el = tid & 31, slots 4 * (tid >> 5) .. +3)
+  const int el = tid & 31, s4 = (tid >> 5) * 4;
+  const int e = e0 + el;
+  floatx4 v = {0, 0, 0, 0};
+  if (e < ee) {
+    const int i = a.ent_pat[e];
+    v = *reinterpret_cast<const floatx4*>(a.Pc + ((size_t)i * a.topk + a.ent_j[e]) * kSlot + s4);
+    v = v * a.invphi[i];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int s = s4 + u;
+    const float sc = s < nr ? a.rowscale[a.cell_rot[r0 + s]] : 1.0f;
+    const float x = v[u] * sc;
+    const _Float16 h = (_Float16)x;
+    panel[0][s * 32 + el] = h;
+    panel[1][s * 32 + el] = (_Float16)(x - (float)h);
+  }
+  __syncthreads();
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const int h = tid >> 7, o = (tid & 127) * 8;
+  *reinterpret_cast<u32x4*>(a.panels + (q * 2 + h) * (kSlot * 32) + o) =
+      *reinterpret_cast<const u32x4*>(&panel[h][o]);
+}
+
+void launch_cand_prep(const CandPrepArgs& a, long n_chunks, hipStream_t st) {
+  if (n_chunks > 0) {
+    hipLaunchKernelGGL(k_prep_partial, dim3(n_chunks), dim3(256), 0, st, a);
+    EMC_LAUNCHED();
+  }
+  hipLaunchKernelGGL(k_prep_reduce, dim3(a.C), dim3(64), 0, st, a);
   EMC_LAUNCHED();
+  if (n_chunks > 0) {
+    hipLaunchKernelGGL(k_prep_panels, dim3(n_chunks), dim3(256), 0, st, a);
+    EMC_LAUNCHED();
+  }
 }
 
 // XOR swizzle of the 16-byte chunks of a count row in LDS: the four groups
 // of 8 entries read by one MFMA B fragment land in distinct banks.
 __device__ __forceinline__ int kswz(int row) { return ((row >> 3) & 3) * 2; }
 
-// U[r][k] = sum over the cell's entries e of (P/phi)_e,r K[pat(e)][k]
-// (+ the subset's U already in Sc), then S' = T > 1e-10 ? U / T : U to Sc and
-// to the full slices.  Workgroup: one cell x 256 pixels, 4 waves of 64
+// S' = U / divisor under the likelihood's rule.
+__device__ __forceinline__ float finish_slice(float u, float T, int poisson) {
+  return poisson ? (T > 1e-30f ? u / T : 0.0f) : (T > 1e-10f ? u / T : u);
+}
+
+// U[r][k] = sum over a segment of the cell's entries e of
+// (P/phi)_e,r K[pat(e)][k].  A cell whose entries fit one segment adds the
+// subset's U already in Sc and writes S' = U / T to Sc and the full slices;
+// the segments of a larger cell write partial sums, added in segment order by
+// k_cand_finalize.  Workgroup: one segment x 256 pixels, 4 waves of 64
 // pixels; chunk of 32 entries = one MFMA k-step.
 template <bool KSPLIT>
 __global__ __launch_bounds__(256) void k_cand_mstep(CandMstepArgs a) {
   constexpr int EC = 32, PT = 256, NBK = KSPLIT ? 2 : 1;
   __shared__ __attribute__((aligned(1024))) _Float16 kb[2][NBK][EC * PT];
   __shared__ __attribute__((aligned(1024))) _Float16 ab[2][2][kSlot * EC];
-  int cell, pt;
+  int seg, pt;
   {
     int tm, tn;
-    tile_of_block(a.C * a.n_ptiles, 1, tm, tn);
-    cell = tm % a.C;
-    pt = tm / a.C;
+    tile_of_block(a.n_seg * a.n_ptiles, 1, tm, tn);
+    seg = tm % a.n_seg;
+    pt = tm / a.n_seg;
   }
+  const int cell = a.seg_cell[seg];
   const int p0 = pt * PT;
-  const int eb = a.ent_off[cell], ee = a.ent_off[cell + 1];
+  const long chunk0 = a.seg_chunk0[seg];
+  const int nch = a.seg_nch[seg];
+  const int part = a.seg_part[seg];
+  const int eb = a.ent_off[cell] + (int)(chunk0 - a.chunk_off[cell]) * EC;
+  const int ee = min(eb + nch * EC, a.ent_off[cell + 1]);
   const int r0 = a.cell_off[cell], nr = a.cell_off[cell + 1] - r0;
-  const int nch = (ee - eb + EC - 1) / EC;
-  const long chunk0 = a.chunk_off[cell];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, fr = lane & 15;
 
@@ -496,22 +538,50 @@ __global__ __launch_bounds__(256) void k_cand_mstep(CandMstepArgs a) {
         const int k = p0 + wv * 64 + y * 16 + fr;
         if (k >= a.n_m) continue;
         float u = acc[x][y][j] * inv_scale;
+        if (part >= 0) {
+          a.part[((size_t)part * kSlot + s) * a.n_mp + k] = u;
+          continue;
+        }
         float* dst = a.Sc + (size_t)r * a.n_mp + k;
         if (a.has_sub) u += *dst;
-        const float sv = a.poisson ? (T > 1e-30f ? u / T : 0.0f) : (T > 1e-10f ? u / T : u);
+        const float sv = finish_slice(u, T, a.poisson);
         *dst = sv;
         a.slices[(size_t)r * a.N2d + a.pix[k]] = sv;
       }
     }
 }
 
+// Cells split into several segments: U = (subset U) + the segments' partial
+// sums in segment order, then S' as above.  One block per (cell, 256 pixels).
+__global__ __launch_bounds__(256) void k_cand_finalize(CandMstepArgs a) {
+  const int mc = blockIdx.x % a.n_mcell, pt = blockIdx.x / a.n_mcell;
+  const int cell = a.mcell[mc];
+  const int k = pt * 256 + threadIdx.x;
+  if (k >= a.n_m) return;
+  const int r0 = a.cell_off[cell], nr = a.cell_off[cell + 1] - r0;
+  const int q0 = a.mcell_part[mc], nq = a.mcell_part[mc + 1] - q0;
+  for (int s = 0; s < nr; s++) {
+    const int r = a.cell_rot[r0 + s];
+    float* dst = a.Sc + (size_t)r * a.n_mp + k;
+    float u = a.has_sub ? *dst : 0.0f;
+    for (int q = 0; q < nq; q++) u += a.part[((size_t)(q0 + q) * kSlot + s) * a.n_mp + k];
+    const float sv = finish_slice(u, a.Tf[r], a.poisson);
+    *dst = sv;
+    a.slices[(size_t)r * a.N2d + a.pix[k]] = sv;
+  }
+}
+
 void launch_cand_mstep(bool ksplit, const CandMstepArgs& a, hipStream_t st) {
-  const dim3 grid(a.C * a.n_ptiles);
+  const dim3 grid(a.n_seg * a.n_ptiles);
   if (ksplit)
     hipLaunchKernelGGL(k_cand_mstep<true>, grid, dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(k_cand_mstep<false>, grid, dim3(256), 0, st, a);
   EMC_LAUNCHED();
+  if (a.n_mcell > 0) {
+    hipLaunchKernelGGL(k_cand_finalize, dim3(a.n_mcell * a.n_ptiles), dim3(256), 0, st, a);
+    EMC_LAUNCHED();
+  }
 }
 
 }  // namespace emc

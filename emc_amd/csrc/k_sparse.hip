@@ -1,0 +1,150 @@
+// Patterns stored as CSR (compact pixel index int32, photon count uint16) and
+// the sparse-count x log-slice contraction of the Poisson E-step:
+//   C_ir = sum over the nonzero counts K_ik of pattern i of K_ik LS[r][k],
+// with LS the FP16 hi/lo log-slice operand (per-rotation power-of-two scale),
+// read transposed ([k][r], rotations contiguous) so that one nonzero feeds
+// 256 rotations from one coalesced 1 KB row segment.
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+#include "gemm_common.hpp"
+#include "reduce.hpp"
+
+namespace emc {
+
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+// Nonzero counts of each pattern over the compact pixels.
+__global__ __launch_bounds__(256) void k_csr_row_nnz(const float* __restrict__ images, const int* __restrict__ pix,
+                                                     int N2d, int n_mp, int* __restrict__ nnz) {
+  const int i = blockIdx.x;
+  int c = 0;
+  for (int k = threadIdx.x; k < n_mp; k += 256) {
+    const int p = pix[k];
+    if (p >= 0 && images[(size_t)i * N2d + p] != 0.0f) c++;
+  }
+  c = (int)block_sum_f32((float)c);
+  if (threadIdx.x == 0) nnz[i] = c;
+}
+
+// One wave per pattern: its nonzeros in increasing compact pixel order.
+__global__ __launch_bounds__(256) void k_csr_fill(const float* __restrict__ images, const int* __restrict__ pix,
+                                                  int N, int N2d, int n_mp, const int* __restrict__ off,
+                                                  int* __restrict__ col, unsigned short* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= N) return;
+  int at = off[i];
+  for (int k0 = 0; k0 < n_mp; k0 += 64) {
+    const int k = k0 + lane;
+    const int p = pix[k];
+    const float v = p >= 0 ? images[(size_t)i * N2d + p] : 0.0f;
+    const bool nz = v != 0.0f;
+    const unsigned long long m = __ballot(nz);
+    if (nz) {
+      const int pos = at + __popcll(m & ((1ull << lane) - 1ull));
+      col[pos] = k;
+      cnt[pos] = (unsigned short)v;
+    }
+    at += __popcll(m);
+  }
+}
+
+// LSt[k][r] = (hi, lo) of S16 / S16lo [R_pad][n_mp]; 64 x 64 tiles.
+__global__ __launch_bounds__(256) void k_transpose_hilo(const _Float16* __restrict__ hi, const _Float16* __restrict__ lo,
+                                                        int R_pad, int n_mp, half2v* __restrict__ out) {
+  __shared__ half2v tile[64][65];
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int rr = idx >> 6, kk = idx & 63;
+    const size_t src = (size_t)(r0 + rr) * n_mp + k0 + kk;
+    tile[rr][kk] = half2v{hi[src], lo[src]};
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int kk = idx >> 6, rr = idx & 63;
+    out[(size_t)(k0 + kk) * R_pad + r0 + rr] = tile[rr][kk];
+  }
+}
+
+// One workgroup = 16 patterns x 256 rotations; thread = rotation.  The
+// patterns' nonzeros are wave-uniform (scalar loads); each feeds one FMA per
+// thread from a coalesced row of LSt.  Epilogue: the likelihood of
+// like_store (Poisson form).
+constexpr int kCsrPatterns = 16;
+
+__global__ __launch_bounds__(256) void k_estep_csr(const int* __restrict__ off, const int* __restrict__ col,
+                                                   const unsigned short* __restrict__ cnt,
+                                                   const half2v* __restrict__ LSt, int R_pad, LikeEpilogue ep) {
+  const int r = blockIdx.y * 256 + threadIdx.x;
+  const int i0 = blockIdx.x * kCsrPatterns;
+  float acc[kCsrPatterns];
+#pragma unroll
+  for (int b = 0; b < kCsrPatterns; b++) {
+    acc[b] = 0.0f;
+    const int i = i0 + b;
+    if (i >= ep.N) continue;
+    const int j1 = off[i + 1];
+    float s = 0.0f;
+    for (int j = off[i]; j < j1; j++) {
+      const half2v v = LSt[(size_t)col[j] * R_pad + r];
+      s += (float)cnt[j] * ((float)v.x + (float)v.y);
+    }
+    acc[b] = s;
+  }
+  if (r >= ep.R) return;
+  const double cs = (double)ep.cscale[r];
+  const double a = ep.A[r];
+#pragma unroll
+  for (int b = 0; b < kCsrPatterns; b++) {
+    const int i = i0 + b;
+    if (i >= ep.N) break;
+    const double c = (double)acc[b] * cs;
+    const double phi = (double)ep.phi[i];
+    ep.L[(size_t)r * ep.N + i] = (float)(c + ep.B[i] * log(phi) - phi * a);
+  }
+}
+
+// Dense images from CSR: images[i][pix] = count (zero elsewhere, filled first).
+__global__ __launch_bounds__(256) void k_images_from_csr(const int* __restrict__ off, const int* __restrict__ pix,
+                                                         const unsigned short* __restrict__ cnt, int N, int N2d,
+                                                         float* __restrict__ images) {
+  const int i = blockIdx.x;
+  for (int j = off[i] + threadIdx.x; j < off[i + 1]; j += 256) {
+    const int p = pix[j];
+    if (p >= 0 && p < N2d) images[(size_t)i * N2d + p] = (float)cnt[j];
+  }
+}
+
+void launch_csr_row_nnz(const float* images, const int* pix, int N, int N2d, int n_mp, int* nnz, hipStream_t st) {
+  hipLaunchKernelGGL(k_csr_row_nnz, dim3(N), dim3(256), 0, st, images, pix, N2d, n_mp, nnz);
+  EMC_LAUNCHED();
+}
+
+void launch_csr_fill(const float* images, const int* pix, int N, int N2d, int n_mp, const int* off, int* col,
+                     unsigned short* cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_csr_fill, dim3((N + 3) / 4), dim3(256), 0, st, images, pix, N, N2d, n_mp, off, col, cnt);
+  EMC_LAUNCHED();
+}
+
+void launch_transpose_hilo(const _Float16* hi, const _Float16* lo, int R_pad, int n_mp, void* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_transpose_hilo, dim3(n_mp / 64, R_pad / 64), dim3(256), 0, st, hi, lo, R_pad, n_mp,
+                     (half2v*)out);
+  EMC_LAUNCHED();
+}
+
+void launch_estep_csr(const int* off, const int* col, const unsigned short* cnt, const void* LSt, int R_pad,
+                      const LikeEpilogue& ep, hipStream_t st) {
+  const dim3 grid((ep.N + kCsrPatterns - 1) / kCsrPatterns, R_pad / 256);
+  hipLaunchKernelGGL(k_estep_csr, grid, dim3(256), 0, st, off, col, cnt, (const half2v*)LSt, R_pad, ep);
+  EMC_LAUNCHED();
+}
+
+void launch_images_from_csr(const int* off, const int* pix, const unsigned short* cnt, int N, int N2d, float* images,
+                            hipStream_t st) {
+  launch_fill(images, (size_t)N * N2d, 0.0f, st);
+  hipLaunchKernelGGL(k_images_from_csr, dim3(N), dim3(256), 0, st, off, pix, cnt, N, N2d, images);
+  EMC_LAUNCHED();
+}
+
+}  // namespace emc

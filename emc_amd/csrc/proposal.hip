@@ -27,7 +27,8 @@ Proposal& proposal() {
 
 void Proposal::release() {
   DBuf* bufs[] = {&d_cell_off, &d_cell_rot, &d_cand, &d_pmass, &d_ent_off, &d_ent_pat, &d_ent_j, &d_work,
-                  &d_chunk_off, &d_Lc, &d_panels, &d_crowscale, &d_plogp, &d_zero, &d_sub, &d_insub, &d_Ksub,
+                  &d_chunk_off, &d_chunk_cell, &d_seg_cell, &d_seg_chunk0, &d_seg_nch, &d_seg_part, &d_mcell,
+                  &d_mcell_part, &d_part, &d_part_T, &d_part_Tphi, &d_part_max, &d_Lc, &d_panels, &d_crowscale, &d_plogp, &d_zero, &d_sub, &d_insub, &d_Ksub,
                   &d_Ksublo, &d_Ktsub, &d_Ktsublo, &d_Bsub, &d_phisub, &d_invphisub, &d_Lsub, &d_phiden_sub,
                   &d_Psub16, &d_Psub16lo, &d_mass, &d_tot};
   for (DBuf* b : bufs) b->release();
@@ -79,10 +80,40 @@ static bool build_lists(Proposal& p) {
   }
   p.n_work = (int)work.size();
   p.n_chunks = p.h_chunk_off[C];
+  std::vector<int> chunk_cell(std::max<long>(p.n_chunks, 1), 0);
+  // M-step segments of at most kSegChunks chunks; a cell with one segment
+  // (or none: no entries) finishes its slices in the M-step kernel
+  std::vector<int> seg_cell, seg_nch, seg_part, mcell, mcell_part(1, 0);
+  std::vector<long> seg_chunk0;
+  int parts = 0;
+  for (int c = 0; c < C; c++) {
+    const long q0 = p.h_chunk_off[c], nq = p.h_chunk_off[c + 1] - q0;
+    for (long q = q0; q < q0 + nq; q++) chunk_cell[q] = c;
+    const bool split = nq > kSegChunks;
+    for (long s0 = 0; s0 < std::max<long>(nq, 1); s0 += kSegChunks) {
+      seg_cell.push_back(c);
+      seg_chunk0.push_back(q0 + s0);
+      seg_nch.push_back((int)std::min<long>(kSegChunks, nq - s0 > 0 ? nq - s0 : 0));
+      seg_part.push_back(split ? parts++ : -1);
+    }
+    if (split) {
+      mcell.push_back(c);
+      mcell_part.push_back(parts);
+    }
+  }
+  p.n_seg = (int)seg_cell.size();
+  p.n_mcell = (int)mcell.size();
+  p.n_parts = parts;
   if (!upload(p.d_ent_off, p.h_ent_off, e.stream) || !upload(p.d_ent_pat, pat, e.stream) ||
       !upload(p.d_ent_j, jj, e.stream) || !upload(p.d_work, work, e.stream) ||
-      !upload(p.d_chunk_off, p.h_chunk_off, e.stream) || !upload(p.d_cand, p.h_cand, e.stream))
+      !upload(p.d_chunk_off, p.h_chunk_off, e.stream) || !upload(p.d_cand, p.h_cand, e.stream) ||
+      !upload(p.d_chunk_cell, chunk_cell, e.stream) || !upload(p.d_seg_cell, seg_cell, e.stream) ||
+      !upload(p.d_seg_chunk0, seg_chunk0, e.stream) || !upload(p.d_seg_nch, seg_nch, e.stream) ||
+      !upload(p.d_seg_part, seg_part, e.stream) || !upload(p.d_mcell, mcell, e.stream) ||
+      !upload(p.d_mcell_part, mcell_part, e.stream))
     return false;
+  const size_t nq = (size_t)std::max<long>(p.n_chunks, 1) * kSlot;
+  if (!p.d_part_T.ensure(8 * nq) || !p.d_part_Tphi.ensure(8 * nq) || !p.d_part_max.ensure(4 * nq)) return false;
   if (!p.d_Lc.ensure(sizeof(float) * (size_t)N * K * kSlot) ||
       !p.d_panels.ensure(2 * (size_t)std::max<long>(p.n_chunks, 1) * 2 * kSlot * 32) ||
       !p.d_plogp.ensure(sizeof(double) * N) || !p.d_crowscale.ensure(sizeof(float) * p.R))
@@ -338,10 +369,16 @@ bool proposal_mstep(int N, int R, float* slices, int N2d, const float* scaling) 
     a.Tphi = e.d_Tphi.as<float>();
     a.rowscale = p.d_crowscale.as<float>();
     a.panels = p.d_panels.as<_Float16>();
+    a.chunk_cell = p.d_chunk_cell.as<int>();
+    a.part_T = p.d_part_T.as<double>();
+    a.part_Tphi = p.d_part_Tphi.as<double>();
+    a.part_max = p.d_part_max.as<float>();
     a.C = p.C;
     a.topk = p.topk;
-    launch_cand_prep(a, st);
+    launch_cand_prep(a, p.n_chunks, st);
   }
+  // partial sums of the split cells: [parts][32 slots][n_mp]
+  if (!p.d_part.ensure(sizeof(float) * (size_t)std::max(p.n_parts, 1) * kSlot * e.n_mp)) return false;
   {
     PhaseTimer pt(PH_MSTEP);
     CandMstepArgs a{};
@@ -350,6 +387,15 @@ bool proposal_mstep(int N, int R, float* slices, int N2d, const float* scaling) 
     a.zero = p.d_zero.as<_Float16>();
     a.panels = p.d_panels.as<_Float16>();
     a.chunk_off = p.d_chunk_off.as<long>();
+    a.seg_cell = p.d_seg_cell.as<int>();
+    a.seg_chunk0 = p.d_seg_chunk0.as<long>();
+    a.seg_nch = p.d_seg_nch.as<int>();
+    a.seg_part = p.d_seg_part.as<int>();
+    a.n_seg = p.n_seg;
+    a.part = p.d_part.as<float>();
+    a.mcell = p.d_mcell.as<int>();
+    a.mcell_part = p.d_mcell_part.as<int>();
+    a.n_mcell = p.n_mcell;
     a.ent_off = p.d_ent_off.as<int>();
     a.ent_pat = p.d_ent_pat.as<int>();
     a.cell_off = p.d_cell_off.as<int>();

@@ -37,9 +37,13 @@ struct Proposal {
   // entries per cell (CSR, patterns ascending) and E-step work items
   std::vector<int> h_ent_off;
   std::vector<long> h_chunk_off;
-  DBuf d_ent_off, d_ent_pat, d_ent_j, d_work, d_chunk_off;
+  DBuf d_ent_off, d_ent_pat, d_ent_j, d_work, d_chunk_off, d_chunk_cell;
   int n_work = 0;
   long n_entries = 0, n_chunks = 0;
+  // M-step segments (cells with many entries are split) and their partials
+  DBuf d_seg_cell, d_seg_chunk0, d_seg_nch, d_seg_part, d_mcell, d_mcell_part, d_part;
+  DBuf d_part_T, d_part_Tphi, d_part_max;
+  int n_seg = 0, n_mcell = 0, n_parts = 0;
   DBuf d_Lc;      // float [N][topk][32]: L, then P
   DBuf d_panels;  // half [chunks][2][32][32] M-step operand
   DBuf d_crowscale;

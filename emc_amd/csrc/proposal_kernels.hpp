@@ -9,6 +9,7 @@ namespace emc {
 
 constexpr int kSlot = 32;       // rotation slots per coarse cell (largest cell must fit)
 constexpr int kCandRows = 128;  // entries per E-step workgroup
+constexpr int kSegChunks = 24;  // 32-entry chunks per M-step segment
 
 struct CandEstepArgs {
   const _Float16 *K, *Klo;          // [N_pad][n_mp] packed counts (lo only when split)
@@ -45,7 +46,10 @@ struct CandSoftmaxArgs {
 struct CandPrepArgs {
   const float* Pc;                  // [N][topk][kSlot]
   const int *ent_off, *ent_pat, *ent_j, *cell_off, *cell_rot;
-  const long* chunk_off;            // [C] first 32-entry chunk of each cell
+  const long* chunk_off;            // [C+1] first 32-entry chunk of each cell
+  const int* chunk_cell;            // [chunks] cell of each chunk
+  double *part_T, *part_Tphi;       // [chunks][kSlot] partial sums
+  float* part_max;                  // [chunks][kSlot]
   const float* invphi;              // operand P * invphi (ones for Poisson)
   const float* phi;                 // Poisson: also Tphi_r = sum P phi, else nullptr
   const double* Tsub;               // [R] subset row sums or nullptr
@@ -62,6 +66,15 @@ struct CandMstepArgs {
   const _Float16* zero;
   const _Float16* panels;
   const long* chunk_off;
+  // segments: cell, first chunk, chunk count, partial-sum slot (-1: the
+  // cell's only segment, which finishes S' itself)
+  const int* seg_cell;
+  const long* seg_chunk0;
+  const int *seg_nch, *seg_part;
+  int n_seg;
+  float* part;                      // [parts][kSlot][n_mp]
+  const int *mcell, *mcell_part;    // cells split into segments, their parts (CSR)
+  int n_mcell;
   const int *ent_off, *ent_pat, *cell_off, *cell_rot, *pix;
   const float *rowscale, *Tf;        // Tf: divisor of U (T_r, or Tphi_r for Poisson)
   float* Sc;                        // [R_pad][n_mp]: subset U in (if has_sub), S' out
@@ -73,7 +86,7 @@ struct CandMstepArgs {
 
 void launch_cand_estep(bool ksplit, const CandEstepArgs& a, hipStream_t st);
 void launch_cand_softmax(const CandSoftmaxArgs& a, hipStream_t st);
-void launch_cand_prep(const CandPrepArgs& a, hipStream_t st);
+void launch_cand_prep(const CandPrepArgs& a, long n_chunks, hipStream_t st);
 void launch_cand_mstep(bool ksplit, const CandMstepArgs& a, hipStream_t st);
 void launch_gather_rows(const _Float16* K, const _Float16* Klo, const int* sub, int n_sub, int n_sub_pad, int n_mp,
                         _Float16* Ks, _Float16* Kslo, hipStream_t st);

@@ -62,6 +62,12 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *   "timing"     "1" records per-phase GPU time (adds a synchronisation per
  *                 entry point), "0" off.
  *   "verbose"    "1" prints the reference's per-call diagnostics.
+ *   "likelihood" "gaussian" (default, the reference's); "poisson":
+ *                 L = sum_p m (K log(phi max(S, 1e-6)) - phi S), scaling
+ *                 phi = sum K / sum_r P sum S, S' = sum_i P K / sum_i P phi.
+ *   "sparse"     "on" (Poisson, precision "fp16"): the whole-table E-step
+ *                 reads the counts as CSR (int32 pixel, uint16 count) and
+ *                 contracts them with the log-slice operand; "off" (default).
  *   "proposal"   "on": once candidates exist for this pattern count (see
  *                 the proposal workflow below) and precision is "fp16", the
  *                 E-step scores each pattern only against the fine rotations
@@ -154,6 +160,14 @@ int emc_hip_proposal_get(int *cand, int *mode, float *mass, int N_images);
  * candidate pairs of all patterns (the candidate E-step's work).  Returns
  * the number of values written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
+
+/* Patterns given as CSR (no counterpart in the reference): pattern i holds
+ * counts[j] photons at pixel pix[j] (p = y * side + x) for row_off[i] <= j <
+ * row_off[i+1]; every other pixel 0.  Fills the dense [N_images][N_2d] buffer
+ * d_images the emc.h calls take (so a caller uploads only the nonzeros).
+ * Returns 0 on success. */
+int emc_hip_images_from_csr(real *d_images, const int *row_off, const int *pix, const unsigned short *counts,
+                            int N_images, int N_2d);
 
 /* Information on the packed pattern operand of the last E-step:
  * out[0] = masked pixel count n_m, out[1] = padded count, out[2] = 1 if the
