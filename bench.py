@@ -59,11 +59,11 @@ def phase_times(L):
 
 
 def proposal_stats(L):
-    out = (C.c_double * 13)()
-    k = L.emc_hip_proposal_stats(out, 13)
+    out = (C.c_double * 14)()
+    k = L.emc_hip_proposal_stats(out, 14)
     keys = ["iteration", "full_search", "audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass",
             "pairs_scored", "gated_at_prediction", "train_loss", "train_ms", "predict_ms", "whole_table",
-            "candidate_pairs"]
+            "candidate_pairs", "recentred"]
     return {keys[j]: out[j] for j in range(k)}
 
 
@@ -235,7 +235,7 @@ def main():
     ap.add_argument("--sigma", type=float, default=0.4)
     ap.add_argument("--plain-steps", type=int, default=-1, help="timed FP32 iterations (default = steps)")
     ap.add_argument("--train-iters", type=int, default=5, help="dense iterations whose responsibilities train")
-    ap.add_argument("--epochs", default="1,1,1,2,6",
+    ap.add_argument("--epochs", default="0,0,0,2,8",
                     help="training passes after each dense iteration (comma list; the last repeats)")
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--topk", type=int, default=32)
@@ -325,6 +325,7 @@ def main():
             "audit_below_min_mass_per_step": float(np.mean([s["audit_below_min_mass"] for s in steps_st])),
             "audit_min_captured_mass": float(min(s["audit_min_mass"] for s in steps_st)),
             "audit_mean_captured_mass": float(np.mean([s["audit_mean_mass"] for s in steps_st])),
+            "recentred_patterns_per_step": float(np.mean([s["recentred"] for s in steps_st])),
             "gated_at_prediction": fast["gated"],
             "train_loss_per_dense_iteration": fast["losses"],
             "dense_iterations_s": fast["dense_s"], "training_s": fast["train_s"], "prediction_s": fast["predict_s"],

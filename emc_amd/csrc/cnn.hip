@@ -10,6 +10,8 @@
 
 namespace emc {
 
+static constexpr size_t kRedDoubles = 1 << 16;  // partial sums of the bias gradients
+
 bool Cnn::init(int side_, int ncell_, int batch_, uint64_t seed) {
   release();
   if (side_ < 16 || side_ % 16 || ncell_ <= 0 || batch_ <= 0) {
@@ -72,13 +74,15 @@ bool Cnn::init(int side_, int ncell_, int batch_, uint64_t seed) {
   scratch_floats = std::max<size_t>((size_t)16 << 20, (size_t)4 * hidden * feat);
   if (!h.ensure(4 * B * hidden) || !logits.ensure(4 * (long)ncell * B) || !dact.ensure(4 * 2 * maxact) ||
       !dcol.ensure(4 * maxcol) || !dh.ensure(4 * B * hidden) || !dlogits.ensure(4 * (long)ncell * B) ||
-      !scratch.ensure(4 * scratch_floats) || !idx.ensure(4 * B) || !loss.ensure(8))
+      !scratch.ensure(4 * scratch_floats) || !idx.ensure(4 * B) || !loss.ensure(8) ||
+      !red.ensure(8 * kRedDoubles))
     return false;
   return !failed();
 }
 
 void Cnn::release() {
-  DBuf* bufs[] = {&params, &grads, &adam_m, &adam_v, &x, &h, &logits, &dact, &dcol, &dh, &dlogits, &scratch, &idx, &loss};
+  DBuf* bufs[] = {&params, &grads, &adam_m, &adam_v, &x, &h, &logits, &dact, &dcol, &dh, &dlogits, &scratch, &idx, &loss,
+                  &red};
   for (DBuf* b : bufs) b->release();
   for (int l = 0; l < kConvs; l++) {
     col[l].release();
@@ -129,7 +133,7 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     d.C = dW(5); d.scm = hidden; d.scn = 1;
     d.M = ncell; d.N = hidden; d.K = B;
     gemm_bf16(d, sc, scratch_floats, st);
-    reduce_rows(dl, B, ncell, 1, B, dB(5), st);
+    reduce_rows(dl, B, ncell, 1, B, dB(5), red.as<double>(), kRedDoubles, st);
   }
   {  // dh[b][k] = sum_c dlog[c][b] W5[c][k], then ReLU'
     GemmDesc d;
@@ -151,7 +155,7 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     d.C = dW(4); d.scm = feat; d.scn = 1;
     d.M = hidden; d.N = feat; d.K = B;
     gemm_bf16(d, sc, scratch_floats, st);
-    reduce_rows(dh.as<float>(), B, hidden, hidden, 1, dB(4), st);
+    reduce_rows(dh.as<float>(), B, hidden, hidden, 1, dB(4), red.as<double>(), kRedDoubles, st);
     GemmDesc e;
     e.A = dh.as<float>(); e.sam = hidden; e.sak = 1;
     e.B = W(4); e.sbn = 1; e.sbk = feat;
@@ -170,7 +174,7 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     d.C = dW(l); d.scm = K; d.scn = 1;
     d.M = OC; d.N = K; d.K = (int)M;
     gemm_bf16(d, sc, scratch_floats, st);
-    reduce_rows(dz, M, OC, OC, 1, dB(l), st);
+    reduce_rows(dz, M, OC, OC, 1, dB(l), red.as<double>(), kRedDoubles, st);
     if (l > 0) {
       GemmDesc e;  // dcol[m][kk] = sum_oc dz[m][oc] W[oc][kk]
       e.A = dz; e.sam = OC; e.sak = 1;

@@ -33,7 +33,10 @@ void cnn_input(const float* images, const int* mask, int N2d, const int* idx, in
 void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t st);
 void col2im(const float* col, int B, int H, int W, int C, float* in, hipStream_t st);
 void relu_bwd(float* g, const float* a, long n, hipStream_t st);
-void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, hipStream_t st);
+// out[j] = sum over rows i of G[i*si + j*sj]; scratch holds the per-split
+// partial sums (cols * splits doubles)
+void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, double* scratch,
+                 size_t scratch_doubles, hipStream_t st);
 void softmax_xent(const float* logits, int C, int B, const float* targets, long Nt, const int* tidx, float scale,
                   float* grad, double* loss, hipStream_t st);
 void adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps, int step,
@@ -57,7 +60,7 @@ struct Cnn {
   DBuf params, grads, adam_m, adam_v;
   // activations for one batch
   DBuf x, col[kConvs], act[kConvs], h, logits;
-  DBuf dact, dcol, dh, dlogits, scratch, idx, loss;
+  DBuf dact, dcol, dh, dlogits, scratch, idx, loss, red;
   size_t scratch_floats = 0;
 
   bool init(int side, int ncell, int batch, uint64_t seed);

@@ -85,7 +85,7 @@ void Engine::release_all() {
   DBuf* bufs[] = {&d_pix, &d_B, &d_K32, &d_Kt32, &d_K16, &d_K16lo, &d_Kt16, &d_Kt16lo, &d_flags,
                   &d_S32, &d_S16, &d_S16lo, &d_Sscale, &d_A, &d_phi_used, &d_smax, &d_ssum, &d_sacc,
                   &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Tphi, &d_ones, &d_Pscale, &d_P16, &d_P16lo,
-                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
+                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_tiled, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
   for (DBuf* b : bufs) b->release();
   pack_key = {};
   map_key = {};
@@ -169,16 +169,31 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
   e.n_mp = (int)round_up(std::max(e.n_m, 1), kPadK);
   e.n_ins = (int)ins.size();
   e.N_pad = (int)round_up(N, kPadN);
-  std::vector<int> pixp(e.n_mp, -1), inscol(std::max(e.n_ins, 1));
+  std::vector<int> pixp(e.n_mp, -1), inscol(std::max(e.n_ins, 1)), tiled;
   std::copy(pix.begin(), pix.end(), pixp.begin());
   for (int t = 0; t < e.n_ins; t++) inscol[t] = col_of[ins[t]];
+  // the same pixels in 8 x 8 detector tiles (atomic insertion order)
+  if (S % 8 == 0) {
+    for (int ty = 0; ty < S / 8; ty++)
+      for (int tx = 0; tx < S / 8; tx++)
+        for (int ly = 0; ly < 8; ly++)
+          for (int lx = 0; lx < 8; lx++) {
+            const int p = (ty * 8 + ly) * S + tx * 8 + lx;
+            if (hmask[p] == 1) tiled.push_back(col_of[p]);
+          }
+  } else {
+    for (int p = 0; p < N2d; p++)
+      if (hmask[p] == 1) tiled.push_back(col_of[p]);
+  }
   if (!e.d_pix.ensure(sizeof(int) * e.n_mp) || !e.d_ins_pix.ensure(sizeof(int) * std::max(e.n_ins, 1)) ||
+      !e.d_ins_tiled.ensure(sizeof(int) * std::max(e.n_ins, 1)) ||
       !e.d_ins_col.ensure(sizeof(int) * std::max(e.n_ins, 1)) || !e.d_B.ensure(sizeof(double) * N))
     return false;
   EMC_CHECK(hipMemcpyAsync(e.d_pix.p, pixp.data(), sizeof(int) * e.n_mp, hipMemcpyHostToDevice, e.stream));
   if (e.n_ins) {
     EMC_CHECK(hipMemcpyAsync(e.d_ins_pix.p, ins.data(), sizeof(int) * e.n_ins, hipMemcpyHostToDevice, e.stream));
     EMC_CHECK(hipMemcpyAsync(e.d_ins_col.p, inscol.data(), sizeof(int) * e.n_ins, hipMemcpyHostToDevice, e.stream));
+    EMC_CHECK(hipMemcpyAsync(e.d_ins_tiled.p, tiled.data(), sizeof(int) * e.n_ins, hipMemcpyHostToDevice, e.stream));
   }
   const size_t nk = (size_t)e.N_pad * e.n_mp;
   PhaseTimer pt(PH_PACK);
@@ -625,7 +640,8 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
                          e.stream);
   } else {
     PhaseTimer pt(PH_INSERT);
-    launch_insert_atomic(e.d_Sc.as<float>(), e.n_mp, e.d_pix.as<int>(), d_mask, e.d_wT.as<float>(), d_rotations, R, S,
+    launch_insert_atomic(e.d_Sc.as<float>(), e.n_mp, e.d_pix.as<int>(), e.d_ins_tiled.as<int>(), e.d_wT.as<float>(),
+                         d_rotations, R, S,
                          x_coordinates, y_coordinates, z_coordinates, d_model, d_weight, e.interp, e.stream);
   }
   std::vector<float> T(R);
@@ -853,6 +869,10 @@ int emc_hip_set_option(const char* key, const char* value) {
     const double v = strtod(value, &end);
     if (!end || *end != 0 || !(v >= 0.0 && v <= 1.0)) return 1;
     (is(key, "min_mass") ? proposal().min_mass : proposal().audit_frac) = (float)v;
+  } else if (is(key, "proposal_track")) {
+    if (is(value, "on")) proposal().track = true;
+    else if (is(value, "off")) proposal().track = false;
+    else return 1;
   } else if (is(key, "proposal_gate")) {
     if (is(value, "predicted")) proposal().gate_predicted = true;
     else if (is(value, "audit")) proposal().gate_predicted = false;

@@ -153,6 +153,7 @@ struct Engine {
   } map_key;
   DBuf d_ins_pix;    // int [n_ins]: inserted pixels in x-major order
   DBuf d_ins_col;    // int [n_ins]: their compact column
+  DBuf d_ins_tiled;  // int [n_ins]: compact columns of the inserted pixels in 8 x 8 tiles
   DBuf d_map_off;    // uint32 [S^3 + 1]
   DBuf d_map_ent;    // uint32 [entries]: entry id e = r * n_ins + t
   DBuf d_map_wt;     // float [entries]: trilinear weight (trilinear only)
@@ -263,7 +264,8 @@ void launch_rowsum(const float* P, int R, int N, const float* invphi, double* T,
                    float* Tphi = nullptr);
 // den[i] = sum_r P[r][i] A[r] (Poisson scaling denominator)
 void launch_p_dot_a(const float* P, int R, int N, const double* A, double* den, hipStream_t st);
-void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* mask,
+// ins_k: compact indices of the inserted pixels (mask == 1) in 8 x 8 tiles
+void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* ins_k,
                           const float* wT, const float* rot, int R, int S, const float* x,
                           const float* y, const float* z, float* model, float* weight,
                           int interp, hipStream_t st);

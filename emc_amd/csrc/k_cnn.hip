@@ -186,37 +186,57 @@ __global__ void k_relu_bwd(float* __restrict__ g, const float* __restrict__ a, l
 }
 
 // out[j] = sum_i G[i*si + j*sj], i < rows (fixed order per column).
-__global__ void k_reduce_rows(const float* __restrict__ G, long rows, int cols, long si, long sj, float* __restrict__ out) {
-  const int j = blockIdx.x;
+// Column sums in two passes: partial sums of row ranges (grid cols x
+// splits), then the splits added in order -- fixed order, and enough
+// workgroups for the million-row activations of the first layers.
+__global__ void k_reduce_rows_part(const float* __restrict__ G, long rows, int cols, long si, long sj, long per,
+                                   double* __restrict__ part) {
+  const int j = blockIdx.x, sp = blockIdx.y;
+  const long r0 = (long)sp * per, r1 = min(rows, r0 + per);
   double s = 0.0;
-  for (long i = threadIdx.x; i < rows; i += 256) s += G[i * si + (long)j * sj];
+  for (long i = r0 + threadIdx.x; i < r1; i += 256) s += G[i * si + (long)j * sj];
   s = block_sum_f64(s);
-  if (threadIdx.x == 0) out[j] = (float)s;
+  if (threadIdx.x == 0) part[(long)sp * cols + j] = s;
 }
 
-// Softmax cross-entropy over logits stored [C][B] (lane = pattern).
-// targets: [C][Nt] at columns tidx[b]; grad = (p - t) * scale.
-__global__ void k_softmax_xent(const float* __restrict__ logits, int C, int B, const float* __restrict__ targets,
-                               long Nt, const int* __restrict__ tidx, float scale, float* __restrict__ grad,
-                               double* __restrict__ loss) {
-  const int b = blockIdx.x * 256 + threadIdx.x;
+__global__ void k_reduce_rows_fin(const double* __restrict__ part, int nsplit, int cols, float* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= cols) return;
+  double s = 0.0;
+  for (int sp = 0; sp < nsplit; sp++) s += part[(long)sp * cols + j];
+  out[j] = (float)s;
+}
+
+// Softmax cross-entropy over logits stored [C][B], one wave per pattern
+// (lanes over classes).  targets: [C][Nt] at columns tidx[b];
+// grad = (p - t) * scale; loss accumulates -sum t log p.
+__global__ __launch_bounds__(256) void k_softmax_xent(const float* __restrict__ logits, int C, int B,
+                                                      const float* __restrict__ targets, long Nt,
+                                                      const int* __restrict__ tidx, float scale,
+                                                      float* __restrict__ grad, double* __restrict__ loss) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= B) return;
+  float m = -3.0e38f;
+  for (int c = lane; c < C; c += 64) m = fmaxf(m, logits[(long)c * B + b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  double s = 0.0;
+  for (int c = lane; c < C; c += 64) s += exp((double)logits[(long)c * B + b] - m);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  const double ls = log(s);
+  const long col = tidx[b];
   double l = 0.0;
-  if (b < B) {
-    float m = -3.0e38f;
-    for (int c = 0; c < C; c++) m = fmaxf(m, logits[(long)c * B + b]);
-    double s = 0.0;
-    for (int c = 0; c < C; c++) s += exp((double)logits[(long)c * B + b] - m);
-    const double ls = log(s);
-    const long col = tidx[b];
-    for (int c = 0; c < C; c++) {
-      const double lp = (double)logits[(long)c * B + b] - m - ls;
-      const float tc = targets[(long)c * Nt + col];
-      if (tc > 0.0f) l -= tc * lp;
-      if (grad) grad[(long)c * B + b] = (float)((exp(lp) - tc) * scale);
-    }
+  for (int c = lane; c < C; c += 64) {
+    const double lp = (double)logits[(long)c * B + b] - m - ls;
+    const float tc = targets[(long)c * Nt + col];
+    if (tc > 0.0f) l -= tc * lp;
+    if (grad) grad[(long)c * B + b] = (float)((exp(lp) - tc) * scale);
   }
-  l = block_sum_f64(l);
-  if (threadIdx.x == 0 && loss) atomicAdd(loss, l);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+  if (lane == 0 && loss) atomicAdd(loss, l);
 }
 
 // Adam on FP32 master weights (bias-corrected).
@@ -313,13 +333,20 @@ void relu_bwd(float* g, const float* a, long n, hipStream_t st) {
   hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n)), dim3(256), 0, st, g, a, n);
   EMC_LAUNCHED();
 }
-void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_rows, dim3(cols), dim3(256), 0, st, G, rows, cols, si, sj, out);
+void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, double* scratch,
+                 size_t scratch_doubles, hipStream_t st) {
+  long nsplit = std::max<long>(1, std::min<long>((rows + 8191) / 8192, (long)(scratch_doubles / std::max(cols, 1))));
+  nsplit = std::min<long>(nsplit, 65535);
+  const long per = (rows + nsplit - 1) / nsplit;
+  hipLaunchKernelGGL(k_reduce_rows_part, dim3(cols, nsplit), dim3(256), 0, st, G, rows, cols, si, sj, per, scratch);
+  EMC_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce_rows_fin, dim3((cols + 255) / 256), dim3(256), 0, st, scratch, (int)nsplit, cols, out);
   EMC_LAUNCHED();
 }
 void softmax_xent(const float* logits, int C, int B, const float* targets, long Nt, const int* tidx, float scale,
                   float* grad, double* loss, hipStream_t st) {
-  hipLaunchKernelGGL(k_softmax_xent, dim3(nblk(B)), dim3(256), 0, st, logits, C, B, targets, Nt, tidx, scale, grad, loss);
+  hipLaunchKernelGGL(k_softmax_xent, dim3((B + 3) / 4), dim3(256), 0, st, logits, C, B, targets, Nt, tidx, scale, grad,
+                     loss);
   EMC_LAUNCHED();
 }
 void adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps, int step,

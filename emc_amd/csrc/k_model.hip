@@ -87,9 +87,11 @@ void launch_set_wT(const float* wts, const float* Tf, int R, float* wT, hipStrea
 // T_r > 1e-10 and pixels whose mask is 1, model[v] += w_r T_r S'_rp and
 // weight[v] += w_r T_r at the nearest voxel (bounds 0 <= v < S), or spread
 // over the trilinear footprint.  The order of the float additions is not
-// fixed.
-__global__ __launch_bounds__(256) void k_insert_atomic(const float* __restrict__ Sc, int n_mp, int n_m,
-                                                       const int* __restrict__ pix, const int* __restrict__ mask,
+// fixed.  The inserted pixels (compact indices ins_k) come in 8 x 8 detector
+// tiles, so the atomics of one wave land on a small patch of the rotated
+// plane (few cache lines).
+__global__ __launch_bounds__(256) void k_insert_atomic(const float* __restrict__ Sc, int n_mp, int n_ins,
+                                                       const int* __restrict__ pix, const int* __restrict__ ins_k,
                                                        const float* __restrict__ wT, const float* __restrict__ Tf,
                                                        const float* __restrict__ rot, int S,
                                                        const float* __restrict__ x, const float* __restrict__ y,
@@ -102,9 +104,9 @@ __global__ __launch_bounds__(256) void k_insert_atomic(const float* __restrict__
   float e[9];
   rot_matrix(&rot[4 * (size_t)r], e);
   const float half = (float)S / 2.0f + 0.5f;
-  for (int k = threadIdx.x; k < n_m; k += 256) {
+  for (int j = threadIdx.x; j < n_ins; j += 256) {
+    const int k = ins_k[j];
     const int p = pix[k];
-    if (mask[p] != 1) continue;
     float ux, uy, uz;
     voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
     const float val = w * Sc[(size_t)r * n_mp + k];
@@ -127,11 +129,12 @@ __global__ __launch_bounds__(256) void k_insert_atomic(const float* __restrict__
   }
 }
 
-void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* mask, const float* wT,
+void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* ins_k, const float* wT,
                           const float* rot, int R, int S, const float* x, const float* y, const float* z,
                           float* model, float* weight, int interp, hipStream_t st) {
   Engine& eng = engine();
-  hipLaunchKernelGGL(k_insert_atomic, dim3(R), dim3(256), 0, st, Sc, n_mp, eng.n_m, pix, mask, wT,
+  if (eng.n_ins <= 0) return;
+  hipLaunchKernelGGL(k_insert_atomic, dim3(R), dim3(256), 0, st, Sc, n_mp, eng.n_ins, pix, ins_k, wT,
                      eng.d_Tf.as<float>(), rot, S, x, y, z, model, weight, interp);
   EMC_LAUNCHED();
 }

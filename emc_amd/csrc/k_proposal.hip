@@ -585,3 +585,121 @@ void launch_cand_mstep(bool ksplit, const CandMstepArgs& a, hipStream_t st) {
 }
 
 }  // namespace emc
+
+namespace emc {
+
+// ---------------------------------------------------------------------------
+// Posterior cells for re-centring the candidates
+// ---------------------------------------------------------------------------
+// Wave per pattern scored on its candidates: mass of each candidate cell
+// (sum over its slots), the kTop largest (ties to the lower rank).
+__global__ __launch_bounds__(256) void k_cand_topcells(const float* __restrict__ Pc, int N, int topk,
+                                                       const unsigned char* __restrict__ insub,
+                                                       const int* __restrict__ cand, int ktop,
+                                                       int* __restrict__ top, float* __restrict__ topm) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= N || insub[i]) return;
+  float m = -1.0f;
+  if (lane < topk) {
+    const float* row = Pc + ((size_t)i * topk + lane) * kSlot;
+    m = 0.0f;
+    for (int s = 0; s < kSlot; s++) m += row[s];
+  }
+  for (int t = 0; t < ktop; t++) {
+    float bv = m;
+    int bj = lane < topk ? lane : 0x7fffffff;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float ov = __shfl_xor(bv, o, 64);
+      const int oj = __shfl_xor(bj, o, 64);
+      if (ov > bv || (ov == bv && oj < bj)) {
+        bv = ov;
+        bj = oj;
+      }
+    }
+    if (lane == 0) {
+      top[(size_t)i * ktop + t] = bj < topk && bv >= 0.0f ? cand[(size_t)i * topk + bj] : -1;
+      topm[(size_t)i * ktop + t] = bj < topk ? fmaxf(bv, 0.0f) : 0.0f;
+    }
+    if (lane == bj) m = -1.0f;
+  }
+}
+
+// Patterns scored on the whole table: cell masses from the columns of Psub
+// (lanes over consecutive subset columns, so every row read is coalesced),
+// in two passes: the 4 largest of every group of 64 cells, then the ktop
+// largest of those (cells in increasing order, ties to the lower cell).
+constexpr int kTopGroup = 64;
+
+__device__ __forceinline__ void top4_insert(float m, int c, float (&lm)[4], int (&lc)[4]) {
+  if (!(m > lm[3])) return;
+  int at = 3;
+  while (at > 0 && m > lm[at - 1]) {
+    lm[at] = lm[at - 1];
+    lc[at] = lc[at - 1];
+    at--;
+  }
+  lm[at] = m;
+  lc[at] = c;
+}
+
+__global__ __launch_bounds__(64) void k_sub_topcells_part(const float* __restrict__ Psub, int n_sub, int C,
+                                                          const int* __restrict__ cell_off,
+                                                          const int* __restrict__ cell_rot,
+                                                          int* __restrict__ pc, float* __restrict__ pm) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= n_sub) return;
+  const int g = blockIdx.y, ng = gridDim.y;
+  float lm[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
+  int lc[4] = {-1, -1, -1, -1};
+  const int c1 = min(C, (g + 1) * kTopGroup);
+  for (int c = g * kTopGroup; c < c1; c++) {
+    float m = 0.0f;
+    for (int j = cell_off[c]; j < cell_off[c + 1]; j++) m += Psub[(size_t)cell_rot[j] * n_sub + k];
+    top4_insert(m, c, lm, lc);
+  }
+  const size_t o = ((size_t)k * ng + g) * 4;
+  for (int t = 0; t < 4; t++) {
+    pc[o + t] = lc[t];
+    pm[o + t] = lm[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sub_topcells_merge(const int* __restrict__ pc, const float* __restrict__ pm,
+                                                            int n_sub, int ng, const int* __restrict__ sub, int ktop,
+                                                            int* __restrict__ top, float* __restrict__ topm) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n_sub) return;
+  float lm[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
+  int lc[4] = {-1, -1, -1, -1};
+  for (int g = 0; g < ng; g++)
+    for (int t = 0; t < 4; t++) {
+      const size_t o = ((size_t)k * ng + g) * 4 + t;
+      if (pc[o] >= 0) top4_insert(pm[o], pc[o], lm, lc);
+    }
+  const int i = sub[k];
+  for (int t = 0; t < ktop; t++) {
+    top[(size_t)i * ktop + t] = lc[t];
+    topm[(size_t)i * ktop + t] = fmaxf(lm[t], 0.0f);
+  }
+}
+
+void launch_topcells(const float* Pc, int N, int topk, const unsigned char* insub, const int* cand,
+                     const float* Psub, int n_sub, const int* sub, int C, const int* cell_off, const int* cell_rot,
+                     int ktop, int* top, float* topm, int* part_c, float* part_m, hipStream_t st) {
+  hipLaunchKernelGGL(k_cand_topcells, dim3((N + 3) / 4), dim3(256), 0, st, Pc, N, topk, insub, cand, ktop, top,
+                     topm);
+  EMC_LAUNCHED();
+  if (n_sub > 0) {
+    const int ng = (C + kTopGroup - 1) / kTopGroup;
+    hipLaunchKernelGGL(k_sub_topcells_part, dim3((n_sub + 63) / 64, ng), dim3(64), 0, st, Psub, n_sub, C, cell_off,
+                       cell_rot, part_c, part_m);
+    EMC_LAUNCHED();
+    hipLaunchKernelGGL(k_sub_topcells_merge, dim3((n_sub + 255) / 256), dim3(256), 0, st, part_c, part_m, n_sub, ng,
+                       sub, ktop, top, topm);
+    EMC_LAUNCHED();
+  }
+}
+
+}  // namespace emc

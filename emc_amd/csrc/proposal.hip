@@ -26,7 +26,7 @@ Proposal& proposal() {
 }
 
 void Proposal::release() {
-  DBuf* bufs[] = {&d_cell_off, &d_cell_rot, &d_cand, &d_pmass, &d_ent_off, &d_ent_pat, &d_ent_j, &d_work,
+  DBuf* bufs[] = {&d_cell_off, &d_cell_rot, &d_cand, &d_pmass, &d_top, &d_topm, &d_ent_off, &d_ent_pat, &d_ent_j, &d_work,
                   &d_chunk_off, &d_chunk_cell, &d_seg_cell, &d_seg_chunk0, &d_seg_nch, &d_seg_part, &d_mcell,
                   &d_mcell_part, &d_part, &d_part_T, &d_part_Tphi, &d_part_max, &d_Lc, &d_panels, &d_crowscale, &d_plogp, &d_zero, &d_sub, &d_insub, &d_Ksub,
                   &d_Ksublo, &d_Ktsub, &d_Ktsublo, &d_Bsub, &d_phisub, &d_invphisub, &d_Lsub, &d_phiden_sub,
@@ -138,10 +138,68 @@ bool proposal_applies(int N, int R) {
   return p.active && p.predicted && p.N == N && p.R == R && engine().precision == PREC_FP16;
 }
 
+// Re-centre every pattern's candidates on its last posterior: the kTop cells
+// holding the most mass (from its candidates, or from the whole table for
+// the patterns scored there), then their nearest cells in mass order, then
+// the network's cells in rank order, up to topk distinct cells.
+static bool recentre(Proposal& p) {
+  Engine& e = engine();
+  hipStream_t st = e.stream;
+  const int N = p.N, K = Proposal::kTop, ng = (p.C + 63) / 64;
+  if (!p.d_top.ensure(sizeof(int) * (size_t)N * K) || !p.d_topm.ensure(sizeof(float) * (size_t)N * K)) return false;
+  DBuf part_c, part_m;
+  const size_t np = (size_t)std::max(p.n_sub, 1) * ng * 4;
+  if (!part_c.ensure(sizeof(int) * np) || !part_m.ensure(sizeof(float) * np)) return false;
+  launch_topcells(p.d_Lc.as<float>(), N, p.topk, p.d_insub.as<unsigned char>(), p.d_cand.as<int>(),
+                  p.d_Lsub.as<float>(), p.n_sub, p.d_sub.as<int>(), p.C, p.d_cell_off.as<int>(),
+                  p.d_cell_rot.as<int>(), K, p.d_top.as<int>(), p.d_topm.as<float>(), part_c.as<int>(),
+                  part_m.as<float>(), st);
+  std::vector<int> top((size_t)N * K);
+  std::vector<float> topm((size_t)N * K);
+  EMC_CHECK(hipMemcpyAsync(top.data(), p.d_top.p, sizeof(int) * top.size(), hipMemcpyDeviceToHost, st));
+  EMC_CHECK(hipMemcpyAsync(topm.data(), p.d_topm.p, sizeof(float) * topm.size(), hipMemcpyDeviceToHost, st));
+  sync_stream();
+  if (failed()) return false;
+  std::vector<int> stamp(p.C, -1), next;
+  next.reserve(p.topk);
+  double moved = 0;
+  for (int i = 0; i < N; i++) {
+    next.clear();
+    auto add = [&](int c) {
+      if (c >= 0 && c < p.C && stamp[c] != i && (int)next.size() < p.topk) {
+        stamp[c] = i;
+        next.push_back(c);
+      }
+    };
+    const int* ti = &top[(size_t)i * K];
+    const float* mi = &topm[(size_t)i * K];
+    int used = 0;
+    for (int t = 0; t < K; t++)
+      if (ti[t] >= 0 && (t == 0 || mi[t] > 1e-6f * mi[0])) {
+        add(ti[t]);
+        used = t + 1;
+      }
+    for (int t = 0; t < used; t++)
+      for (int g = 0; g < Proposal::kNbr; g++) add(p.h_nbr[(size_t)ti[t] * Proposal::kNbr + g]);
+    for (int j = 0; j < p.topk; j++) add(p.h_net_cand[(size_t)i * p.topk + j]);
+    int* ci = &p.h_cand[(size_t)i * p.topk];
+    bool same = true;
+    for (int j = 0; j < p.topk; j++) same = same && ci[j] == next[j];
+    if (!same) {
+      std::copy(next.begin(), next.end(), ci);
+      moved++;
+    }
+  }
+  p.st_moved = moved;
+  return moved == 0 || build_lists(p);
+}
+
 bool proposal_estep(int N, int R, float sigma, const float* phi_used) {
   Engine& e = engine();
   Proposal& p = proposal();
   hipStream_t st = e.stream;
+  if (p.track && p.have_post && !recentre(p)) return false;
+  p.have_post = false;
   p.iteration++;
   p.have_p = false;
   // whole-table subset: the audit sample and the patterns in full search
@@ -299,6 +357,7 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   p.st_pairs = pairs;
   *total = tot[0] + tot[1];
   p.have_p = true;
+  p.have_post = true;
   p.respons = respons;
   return true;
 }
@@ -553,6 +612,23 @@ int emc_hip_proposal_init(const real* q_fine, int n_rot, int n_coarse, int side,
       cur = next;
     }
     for (int r = 0; r < n_rot; r++) cell[r] = walk_id[cell[r]];
+    // nearest cells of every cell (by rotation angle), in the new numbering
+    std::vector<float> qw(4 * (size_t)C);
+    for (int t = 0; t < C; t++) std::copy(&qc[4 * (size_t)t], &qc[4 * (size_t)t] + 4, &qw[4 * (size_t)walk_id[t]]);
+    const int G = std::min(Proposal::kNbr, C - 1);
+    std::vector<int> nbr((size_t)C * Proposal::kNbr, -1);
+    std::vector<std::pair<double, int>> d(C);
+    for (int c = 0; c < C; c++) {
+      const float* a = &qw[4 * (size_t)c];
+      for (int o = 0; o < C; o++) {
+        const float* b = &qw[4 * (size_t)o];
+        d[o] = {-std::fabs((double)a[0] * b[0] + (double)a[1] * b[1] + (double)a[2] * b[2] + (double)a[3] * b[3]), o};
+      }
+      d[c].first = 1e30;  // not its own neighbour
+      std::partial_sort(d.begin(), d.begin() + G, d.end());
+      for (int g = 0; g < G; g++) nbr[(size_t)c * Proposal::kNbr + g] = d[g].second;
+    }
+    proposal().h_nbr.swap(nbr);
   }
   std::vector<int> off(C + 1, 0), rot(n_rot);
   for (int r = 0; r < n_rot; r++) off[cell[r] + 1]++;
@@ -672,6 +748,8 @@ int emc_hip_proposal_predict(const real* d_images, const int* d_mask, int N_2d, 
   idx.release();
   if (failed()) return 1;
   reset_modes(p);
+  p.h_net_cand = p.h_cand;
+  p.have_post = false;
   if (!build_lists(p)) return 1;
   p.predicted = true;
   p.iteration = 0;
@@ -703,6 +781,8 @@ int emc_hip_proposal_set_candidates(const int* cand, int N_images, int topk) {
   p.h_cand.assign(cand, cand + (size_t)N_images * topk);
   p.h_pmass.assign(N_images, 1.0f);
   reset_modes(p);
+  p.h_net_cand = p.h_cand;
+  p.have_post = false;
   if (!build_lists(p)) return 1;
   p.predicted = true;
   p.iteration = 0;
@@ -721,12 +801,12 @@ int emc_hip_proposal_get(int* cand, int* mode, float* mass, int N_images) {
 
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  const double v[13] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
+  const double v[14] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
                         p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
                         p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
-                        p.st_cand_pairs};
+                        p.st_cand_pairs,     p.st_moved};
   int k = 0;
-  for (; k < n && k < 13; k++) out[k] = v[k];
+  for (; k < n && k < 14; k++) out[k] = v[k];
   return k;
 }
 

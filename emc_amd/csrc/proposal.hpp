@@ -23,11 +23,17 @@ struct Proposal {
   float audit_frac = 0.02f;
   float min_mass = 0.999f;
   bool gate_predicted = true;  // full search when the network's top-k mass < min_mass
+  bool track = true;       // re-centre the candidates on the last posterior
   uint64_t seed = 0;
   int iteration = 0;       // candidate-restricted E-steps run
 
-  // coarse cells: fine rotations of each cell (CSR)
-  std::vector<int> h_cell_off, h_cell_rot;
+  // coarse cells: fine rotations of each cell (CSR), nearest cells of each
+  std::vector<int> h_cell_off, h_cell_rot, h_nbr;
+  static constexpr int kNbr = 12;
+  static constexpr int kTop = 4;     // posterior cells per pattern used to re-centre
+  std::vector<int> h_net_cand;       // the network's candidates [N][topk]
+  DBuf d_top, d_topm;                // [N][kTop] cells, masses of the last posterior
+  bool have_post = false;            // Lc / Lsub hold the last iteration's posterior
   DBuf d_cell_off, d_cell_rot;
   // per pattern: candidate cells [N][topk], network top-k mass, search mode
   std::vector<int> h_cand, h_ncand;
@@ -60,7 +66,8 @@ struct Proposal {
   bool have_p = false;             // softmax done, M-step may run on the candidates
   // statistics of the last iteration
   double st_full = 0, st_audited = 0, st_fail = 0, st_min_mass = 1, st_mean_mass = 1, st_pairs = 0,
-         st_gated = 0, st_train_loss = 0, st_train_ms = 0, st_predict_ms = 0, st_cand_pairs = 0;
+         st_gated = 0, st_train_loss = 0, st_train_ms = 0, st_predict_ms = 0, st_cand_pairs = 0,
+         st_moved = 0;
 
   void release();
 };

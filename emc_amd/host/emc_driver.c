@@ -67,9 +67,11 @@ static void usage(void) {
           "           [--model random|truth] [--seed S] [--out DIR] [--dump-respons]\n"
           "           [--save-every K] [--quiet]\n"
           "           [--proposal T] [--topk K] [--coarse-n n] [--epochs E] [--lr LR]\n"
+          "           [--likelihood gaussian|poisson] [--sparse]\n"
           "  --proposal T: train the rotation-proposal network on iterations 1..T,\n"
           "                then score only its top-K coarse cells (default: 5 for\n"
-          "                --config 3, else 0 = off)\n");
+          "                --config 3 and 4, else 0 = off)\n"
+          "  --likelihood poisson --sparse: the low-photon form (default of --config 4)\n");
 }
 
 static void write_raw(const char *dir, const char *name, const float *v, size_t n) {
@@ -90,7 +92,8 @@ int main(int argc, char **argv) {
   float lr = 1e-3f;
   float sigma = 0.4f; /* emc.c:734-735 */
   uint64_t seed = 0;
-  const char *out = "output", *precision = NULL, *insert = "atomic", *interp = "nearest";
+  const char *out = "output", *precision = NULL, *insert = "atomic", *interp = "nearest", *like_kind = NULL;
+  int sparse = -1;
   for (int a = 1; a < argc; a++) {
     const char *k = argv[a];
     const char *v = a + 1 < argc ? argv[a + 1] : NULL;
@@ -111,6 +114,8 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "--insert")) { if (!v) { usage(); return 2; } insert = v; a++; }
     else if (!strcmp(k, "--interp")) { if (!v) { usage(); return 2; } interp = v; a++; }
     else if (!strcmp(k, "--out")) { if (!v) { usage(); return 2; } out = v; a++; }
+    else if (!strcmp(k, "--likelihood")) { if (!v) { usage(); return 2; } like_kind = v; a++; }
+    else if (!strcmp(k, "--sparse")) sparse = 1;
     else if (!strcmp(k, "--model")) { if (!v) { usage(); return 2; } model_truth = !strcmp(v, "truth"); a++; }
     else if (!strcmp(k, "--full-table")) full_table = 1;
     else if (!strcmp(k, "--dump-respons")) dump = 1;
@@ -124,13 +129,16 @@ int main(int argc, char **argv) {
   if (iterations < 0) iterations = P.iterations;
   if (n_rot <= 0) n_rot = P.n_rot;
   if (!precision) precision = cfg >= 2 ? "fp16" : "fp32"; /* the precision form from 128^3 on */
-  if (prop_iters < 0) prop_iters = cfg == 3 ? 5 : 0;
+  if (prop_iters < 0) prop_iters = cfg >= 3 ? 5 : 0;
+  if (!like_kind) like_kind = cfg == 4 ? "poisson" : "gaussian";
+  if (sparse < 0) sparse = cfg == 4;
   if (prop_iters > 0 && strcmp(precision, "fp16") != 0) {
     fprintf(stderr, "emc: --proposal runs with --precision fp16\n");
     return 2;
   }
   if (emc_hip_set_option("precision", precision) || emc_hip_set_option("insert", insert) ||
-      emc_hip_set_option("interp", interp)) {
+      emc_hip_set_option("interp", interp) || emc_hip_set_option("likelihood", like_kind) ||
+      emc_hip_set_option("sparse", sparse ? "on" : "off")) {
     fprintf(stderr, "emc: bad option value\n");
     return 2;
   }

@@ -76,6 +76,11 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 whole table every proposal iteration, chosen at random.
  *   "min_mass"   captured posterior mass (default "0.999") below which a
  *                 pattern returns to whole-table search.
+ *   "proposal_track" "on" (default): before every proposal iteration each
+ *                 pattern's candidates are re-centred on its last posterior
+ *                 (the cells holding most of its mass and their nearest
+ *                 cells, then the network's cells); "off": the network's
+ *                 cells throughout.
  *   "proposal_gate"  "predicted" (default): a pattern also starts in
  *                 whole-table search when the network's top-k cells hold less
  *                 than min_mass of its predicted distribution; "audit": only
@@ -157,7 +162,8 @@ int emc_hip_proposal_get(int *cand, int *mode, float *mass, int N_images);
  * pattern x rotation pairs scored, [7] patterns gated to full search at
  * prediction, [8] last training loss, [9] last training time (ms), [10] last
  * prediction time (ms), [11] patterns scored on the whole table, [12]
- * candidate pairs of all patterns (the candidate E-step's work).  Returns
+ * candidate pairs of all patterns (the candidate E-step's work), [13]
+ * patterns whose candidates were re-centred before the iteration.  Returns
  * the number of values written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 

@@ -39,7 +39,8 @@ def _reset():
     E.lib().emc_hip_clear_error()
 
 
-def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", min_mass="0.999", gate="audit"):
+def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", min_mass="0.999", gate="audit",
+         track="off"):
     E.set_option("precision", precision)
     E.set_option("insert", insert)
     E.set_option("interp", "nearest")
@@ -47,6 +48,7 @@ def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", mi
     E.set_option("audit_fraction", audit)
     E.set_option("min_mass", min_mass)
     E.set_option("proposal_gate", gate)
+    E.set_option("proposal_track", track)
 
 
 def init_cells(P, topk, n_coarse=2, batch=64):
@@ -61,8 +63,8 @@ def init_cells(P, topk, n_coarse=2, batch=64):
 
 
 def stats():
-    out = (C.c_double * 12)()
-    E.lib().emc_hip_proposal_stats(out, 12)
+    out = (C.c_double * 14)()
+    assert E.lib().emc_hip_proposal_stats(out, 14) == 14
     return list(out)
 
 
@@ -167,7 +169,7 @@ def test_fallback_over_two_iterations(prob):
                                    P.x, P.y, P.z)["P"], off, rot)
     cand = np.argsort(-cm, axis=0, kind="stable")[2:2 + topk].T.astype(np.int32).copy()
     assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
-    opts(proposal="on", audit="0.5")
+    opts(proposal="on", audit="0.5")  # fixed candidates: the fallback alone
     g1, g2 = run(P, iterations=2)
     mode1 = np.zeros(P.N, np.int32)
     # modes after iteration 2; reconstruct those after iteration 1 from the
@@ -261,3 +263,37 @@ def test_prediction_is_topk_of_network_output(prob):
     p /= p.sum(axis=0)
     ref_mass = np.take_along_axis(p, ref.T, axis=0).sum(axis=0)
     assert np.allclose(mass[idx], ref_mass, rtol=1e-5, atol=1e-6)
+
+
+def test_tracking_recentres_on_the_posterior(prob):
+    """With deliberately poor candidates (cells ranked 3..8 by posterior mass)
+    and no audit, the first proposal iteration misses most of the mass; the
+    candidates of the second are re-centred on the cells that hold the mass
+    of the first iteration's posterior (whole-table posterior for the patterns
+    scored there), so they capture far more of the next whole-table posterior."""
+    P = prob
+    topk = 6
+    nc, off, rot = init_cells(P, topk)
+    ref = O.o64_iteration(P.model.astype(np.float64), P.images, P.mask, np.ones(P.N), P.side, P.q, P.w,
+                          P.x, P.y, P.z)
+    cm = cell_mass(ref["P"], off, rot)
+    order = np.argsort(-cm, axis=0, kind="stable")
+    cand = order[2:2 + topk].T.astype(np.int32).copy()
+    assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
+    opts(proposal="on", audit="1", track="on")  # every pattern scored on the whole table
+    g1, g2 = run(P, iterations=2)
+    new = np.zeros((P.N, topk), np.int32)
+    assert E.lib().emc_hip_proposal_get(E.iptr(new), None, None, P.N) == 0
+    st2 = g2["stats"]
+    # the re-centred candidates hold each pattern's best cell of iteration 1
+    # (whole-table posterior, computed here from the GPU's iteration-1 P)
+    cm1 = cell_mass(g1["P"].astype(np.float64), off, rot)
+    best1 = cm1.argmax(axis=0)
+    hit = np.mean([best1[i] in new[i] for i in range(P.N)])
+    old_mass = np.take_along_axis(cm1, cand.T, axis=0).sum(axis=0)
+    new_mass = np.take_along_axis(cm1, new.T, axis=0).sum(axis=0)
+    print(f"re-centred {st2[13]:.0f} patterns; best cell kept {hit:.3f}; iteration-1 mass in old candidates "
+          f"{old_mass.mean():.3f}, in re-centred {new_mass.mean():.3f}")
+    assert st2[13] > 0.5 * P.N
+    assert hit == 1.0
+    assert new_mass.mean() > old_mass.mean() + 0.3
