@@ -2,30 +2,37 @@
 
 Workload (default): the 128^3 model / 128x128 patterns / 50,000 patterns /
 50,100 rotations configuration (n = 10 rotation table, antipodal half) with
-the learned rotation proposal, on synthetic data.  One step = one EMC
-iteration exactly as the reference driver sequences it (emc.c:989-1084),
-through the emc.h C-ABI: model max, expand, likelihood, responsibilities,
-sum P log P, model reset, scaling update, M-step + insertion, normalisation.
-Inputs are resident in HBM before timing starts.
+the learned rotation proposal, on synthetic data, 20 iterations from the
+random initial model, with the workload's likelihood (Poisson: photon counts
+contracted with the log of the slices).  One step = one EMC iteration exactly
+as the reference driver sequences it (emc.c:989-1084), through the emc.h
+C-ABI: model max, expand, likelihood, responsibilities, sum P log P, model
+reset, scaling update, M-step + insertion, normalisation.  Inputs are
+resident in HBM before timing starts.
 
-Schedule of the learned form, from the random initial model: iterations
-1..T dense (FP16 hi/lo matrix cores), the proposal network trained after
-each on that iteration's responsibilities; prediction of the top-k coarse
-cells of every pattern; then proposal iterations (candidates scored exactly,
-a random 2% audited on the whole table, fallback below 0.999 captured mass).
-The timed steps are proposal iterations after `warmup` untimed ones.  The
-plain FP32 form (the reference's method on FP32 matrix cores) runs the same
-number of iterations from the same start; its last `steps` are timed; the
-final models are correlated.
+Schedule of the learned form: iterations 1..T dense (FP16 hi/lo matrix
+cores), the proposal network trained after each on that iteration's
+responsibilities; prediction of the top-k coarse cells of every pattern;
+then proposal iterations to the end of the run.  In each, a random 2% of the
+patterns (and those in full search) are scored on the whole table first;
+when more than `--max-fail` of the audited patterns hold less than 0.999 of
+their posterior in their candidates, the iteration runs on the whole table,
+else every other pattern is scored on the fine rotations of its candidate
+cells (re-centred on its last posterior, then the network's cells).  The
+first proposal iteration (untimed) audits every pattern; the rest are timed.
+The plain FP32 form (the same equations on FP32 matrix cores) runs the same
+number of iterations from the same start, its last `steps` timed; the final
+models are correlated, and one more iteration of each from the same state
+gives the accuracy of a single iteration.
 
 Printed: one JSON line.  `value` = N * R / (seconds per timed iteration), the
 effective pattern x rotation pairs per second; the pairs actually scored are
-reported beside it.  `roofline` is the dominant matrix-core kernel (the
-candidate E-step in proposal iterations, the dense likelihood GEMM
-otherwise), timed with HIP events on the library's stream during the timed
-iterations.  `cpu_baseline` is the oracle's FP32 restatement of the
-reference on the host cores, on a pattern sample, extrapolated to the full
-pattern count.
+reported beside it.  `roofline` is the matrix-core kernel that took most of
+the likelihood time (the candidate E-step, or the dense likelihood GEMM),
+timed with HIP events on the library's stream during the timed iterations.
+`cpu_baseline` is the oracle's FP32 restatement of the reference's CPU path
+on the host cores, on a pattern sample, extrapolated to the full pattern
+count.
 """
 from __future__ import annotations
 
@@ -59,36 +66,48 @@ def phase_times(L):
 
 
 def proposal_stats(L):
-    out = (C.c_double * 14)()
-    k = L.emc_hip_proposal_stats(out, 14)
+    out = (C.c_double * 15)()
+    k = L.emc_hip_proposal_stats(out, 15)
     keys = ["iteration", "full_search", "audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass",
             "pairs_scored", "gated_at_prediction", "train_loss", "train_ms", "predict_ms", "whole_table",
-            "candidate_pairs", "recentred"]
+            "candidate_pairs", "recentred", "whole_table_iteration"]
     return {keys[j]: out[j] for j in range(k)}
 
 
 def timed(E, prob, sigma, steps, learned=False):
+    """`steps` iterations; total and per-iteration wall time (every iteration
+    ends with a host read of sum P log P, so the stream is drained), phase
+    times, and for the learned form each iteration's proposal statistics with
+    its own likelihood-phase time."""
     L = E.lib()
     E.set_option("timing", "1")
     L.emc_hip_reset_phase_times()
     L.emc_hip_synchronize()
     t0 = time.perf_counter()
-    per, hist = [], []
+    per, hist, its = [], [], []
+    like_prev = 0.0
     for _ in range(steps):
+        t1 = time.perf_counter()
         hist.append(prob.iteration(sigma))
+        its.append(time.perf_counter() - t1)
         if learned:
-            per.append(proposal_stats(L))
+            st = proposal_stats(L)
+            like = phase_times(L)["likelihood"][0]
+            st["likelihood_ms"] = like - like_prev
+            like_prev = like
+            per.append(st)
     L.emc_hip_synchronize()
     dt = (time.perf_counter() - t0) / steps
     E.check()
     ph = phase_times(L)
     E.set_option("timing", "0")
-    return dt, ph, per, hist
+    return dt, ph, per, hist, its
 
 
-def run_dense(E, W, precision, warmup, steps, sigma, pre=0):
+def run_dense(E, W, precision, warmup, steps, sigma, likelihood, pre=0):
     """`pre` + `warmup` untimed iterations, then `steps` timed ones."""
     L = E.lib()
+    E.set_option("likelihood", likelihood)
     E.set_option("precision", precision)
     E.set_option("insert", "atomic")
     E.set_option("proposal", "off")
@@ -98,23 +117,27 @@ def run_dense(E, W, precision, warmup, steps, sigma, pre=0):
     for it in range(pre + warmup):
         history.append(prob.iteration(sigma))
         log(f"  {precision} untimed iteration {it + 1} done")
-    dt, ph, _, hist = timed(E, prob, sigma, steps)
+    dt, ph, _, hist, its = timed(E, prob, sigma, steps)
     history += hist
     model = prob.model()
     prob.free()
     L.emc_hip_invalidate()
     log(f"  {precision}: {dt * 1e3:.1f} ms/iteration")
-    return {"dt": dt, "phases": ph, "model": model, "history": history}
+    return {"dt": dt, "phases": ph, "model": model, "history": history, "iteration_s": its}
 
 
-def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coarse, seed=0):
+def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr, topk, n_coarse, max_fail,
+                seed=0):
     L = E.lib()
+    E.set_option("likelihood", likelihood)
     E.set_option("precision", "fp16")
     E.set_option("insert", "atomic")
     E.set_option("proposal", "off")
     E.set_option("audit_fraction", "0.02")
     E.set_option("min_mass", "0.999")
+    E.set_option("audit_max_fail", str(max_fail))
     E.set_option("proposal_gate", "audit")
+    E.set_option("proposal_track", "on")
     E.set_option("timing", "0")
     N, R, side = W.N, W.R, W.cfg.side
     prob = E.EmcProblem(side, W.images, W.mask, W.q, W.w, W.model0)
@@ -164,9 +187,14 @@ def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coa
             first_audit = st
         log(f"  proposal warmup {it + 1}: whole table {st['whole_table']:.0f}, audit below min mass "
             f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f}, min mass {st['audit_min_mass']:.6f}, "
-            f"mean {st['audit_mean_mass']:.6f}")
+            f"mean {st['audit_mean_mass']:.6f}, whole-table iteration {st['whole_table_iteration']:.0f}")
     E.set_option("audit_fraction", "0.02")
-    dt, ph, per, hist = timed(E, prob, sigma, steps, learned=True)
+    dt, ph, per, hist, its = timed(E, prob, sigma, steps, learned=True)
+    for k, st in enumerate(per):
+        log(f"  proposal iteration {train_iters + warmup + k + 1}: {its[k] * 1e3:.1f} ms, "
+            f"{'whole table' if st['whole_table_iteration'] else 'candidates'}, audit below min mass "
+            f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f} (mean mass {st['audit_mean_mass']:.5f}), "
+            f"full search {st['full_search']:.0f}, re-centred {st['recentred']:.0f}")
     history += hist
     total_s = time.perf_counter() - t_start
     model = prob.model()
@@ -175,6 +203,7 @@ def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coa
     # the iterated map)
     state_model, state_scaling = model.copy(), prob.scaling().copy()
     prob.iteration(sigma)
+    one_st = proposal_stats(L)
     one_learned = prob.model()
     prob.free()
     E.set_option("proposal", "off")
@@ -186,10 +215,12 @@ def run_learned(E, W, warmup, steps, sigma, train_iters, epochs, lr, topk, n_coa
     ref.free()
     L.emc_hip_invalidate()
     log(f"  learned: {dt * 1e3:.1f} ms/iteration")
-    return {"dt": dt, "phases": ph, "model": model, "per_step": per, "dense_s": dense_s, "train_s": train_s,
+    return {"dt": dt, "phases": ph, "model": model, "per_step": per, "iteration_s": its, "dense_s": dense_s,
+            "train_s": train_s,
             "predict_s": predict_s, "losses": losses, "gated": gated, "total_s": total_s,
             "predicted_mass_q1_q10_q50": pq, "history": history, "first_audit": first_audit,
-            "one_step_cc": cc(one_learned, one_plain)}
+            "one_step_cc": cc(one_learned, one_plain),
+            "one_step_whole_table": one_st["whole_table_iteration"]}
 
 
 def cc(a, b):
@@ -228,11 +259,18 @@ def read_traffic(kernel, workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed iterations (default: the rest of the workload's iteration count for the "
+                         "learned form, 3 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed iterations before them (default: 1 proposal iteration, 2 otherwise)")
     ap.add_argument("--config", type=int, default=3, help="workload preset (emc_amd/configs.py)")
     ap.add_argument("--patterns", type=int, default=0, help="override the pattern count")
-    ap.add_argument("--sigma", type=float, default=0.4)
+    ap.add_argument("--likelihood", choices=["gaussian", "poisson"], default=None,
+                    help="default: the workload's (poisson = count x log-slice contraction)")
+    ap.add_argument("--sigma", type=float, default=0.4, help="noise level of the gaussian likelihood")
+    ap.add_argument("--max-fail", type=float, default=0.005,
+                    help="audited fraction below min mass that sends an iteration to the whole table")
     ap.add_argument("--plain-steps", type=int, default=-1, help="timed FP32 iterations (default = steps)")
     ap.add_argument("--train-iters", type=int, default=5, help="dense iterations whose responsibilities train")
     ap.add_argument("--epochs", default="0,0,0,2,8",
@@ -264,27 +302,45 @@ def main():
     log(f"workload {W.cfg.name}: N={N} R={R} side={W.cfg.side} n_m={n_m} "
         f"(data {time.perf_counter() - t0:.1f}s)")
 
+    like = args.likelihood or W.cfg.likelihood
+    if args.warmup is None:
+        args.warmup = 1 if learned else 2
+    if args.steps is None:
+        args.steps = max(W.cfg.iterations - args.train_iters - args.warmup, 1) if learned else 3
     plain_steps = args.steps if args.plain_steps < 0 else args.plain_steps
     pre = args.train_iters if learned else 0
+    sigma = args.sigma if like == "gaussian" else None
+    log(f"likelihood {like}; {pre} training + {args.warmup} untimed + {args.steps} timed iterations")
     if learned:
-        fast = run_learned(E, W, args.warmup, args.steps, args.sigma, args.train_iters,
-                           [int(v) for v in args.epochs.split(",")], args.lr, args.topk, args.coarse_n)
+        fast = run_learned(E, W, args.warmup, args.steps, args.sigma, like, args.train_iters,
+                           [int(v) for v in args.epochs.split(",")], args.lr, args.topk, args.coarse_n,
+                           args.max_fail)
     else:
-        fast = run_dense(E, W, "fp16", args.warmup, args.steps, args.sigma)
+        fast = run_dense(E, W, "fp16", args.warmup, args.steps, args.sigma, like)
     plain = None
     if not args.no_plain and plain_steps > 0:
-        plain = run_dense(E, W, "fp32", args.warmup, plain_steps, args.sigma, pre=pre)
+        plain = run_dense(E, W, "fp32", args.warmup, plain_steps, args.sigma, like, pre=pre)
 
     pairs = float(N) * R
     dt_fast = fast["dt"]
     like_ms, like_n = fast["phases"]["likelihood"]
     like_avg = like_ms / 1e3 / max(like_n, 1)
+    kernel, kname = "likelihood GEMM (k_estep16)", "k_estep16"
+    flop = 2.0 * N * R * n_m
     if learned:
-        flop = 2.0 * fast["per_step"][-1]["candidate_pairs"] * n_m
-        kernel, kname = "candidate E-step (k_cand_estep)", "k_cand_estep"
-    else:
-        flop = 2.0 * N * R * n_m
-        kernel, kname = "likelihood GEMM (k_estep16)", "k_estep16"
+        # the kernel that took the most likelihood time over the timed
+        # iterations: the candidate E-step, or the dense GEMM of the
+        # iterations the audit sent to the whole table
+        cand_st = [st for st in fast["per_step"] if not st["whole_table_iteration"]]
+        dense_st = [st for st in fast["per_step"] if st["whole_table_iteration"]]
+        cand_ms = sum(st["likelihood_ms"] for st in cand_st)
+        dense_ms = sum(st["likelihood_ms"] for st in dense_st)
+        if cand_st and cand_ms >= dense_ms:
+            kernel, kname = "candidate E-step (k_cand_estep)", "k_cand_estep"
+            flop = 2.0 * float(np.mean([st["candidate_pairs"] for st in cand_st])) * n_m
+            like_avg = cand_ms / 1e3 / len(cand_st)
+        else:
+            like_avg = dense_ms / 1e3 / len(dense_st)
     achieved = flop / like_avg
 
     out = {
@@ -302,8 +358,8 @@ def main():
         "dtype": "fp16 (hi/lo split) x fp32 accumulate; fp64 normalisation",
         "data": "synthetic (Gaussian-blob object, Poisson patterns), random initial model, inputs resident in HBM",
         "config": {"workload": W.cfg.name, "side": W.cfg.side, "patterns": N, "rotations": R,
-                   "masked_pixels": n_m, "sampling_n": W.cfg.n_rot, "sigma": args.sigma,
-                   "parallelism": "single GPU"},
+                   "masked_pixels": n_m, "sampling_n": W.cfg.n_rot, "likelihood": like, "sigma": sigma,
+                   "iterations": pre + args.warmup + args.steps, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "kernel": kernel, "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP16,
                      "traffic": read_traffic(kname, W.cfg.name), "launch_ms": like_avg * 1e3,
@@ -317,6 +373,13 @@ def main():
             "schedule": f"{args.train_iters} dense FP16 iterations, network trained after each "
                         f"(epochs {args.epochs}, Adam lr {args.lr}); top-{args.topk} of the n={args.coarse_n} "
                         f"coarse cells; {args.warmup} untimed + {args.steps} timed proposal iterations",
+            "whole_table_iterations_timed": float(sum(s["whole_table_iteration"] for s in steps_st)),
+            "audit_max_fail": args.max_fail,
+            "per_iteration": [{"iteration": pre + args.warmup + k + 1, "ms": fast["iteration_s"][k] * 1e3,
+                               "whole_table": bool(st["whole_table_iteration"]),
+                               "audit_below_min_mass": st["audit_below_min_mass"], "audited": st["audited"],
+                               "audit_mean_mass": st["audit_mean_mass"], "full_search": st["full_search"]}
+                              for k, st in enumerate(steps_st)],
             "pairs_scored_per_step": scored,
             "scored_pairs_per_s": scored / dt_fast,
             "whole_table_patterns_per_step": float(np.mean([s["whole_table"] for s in steps_st])),
@@ -335,6 +398,7 @@ def main():
                                            ("audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass")}
             if fast["first_audit"] else None,
             "model_cc_one_iteration_vs_plain_fp32": fast["one_step_cc"],
+            "one_iteration_ran_on_whole_table": bool(fast["one_step_whole_table"]),
             "sum_plogp_per_iteration": [h["likelihood"] for h in fast["history"]],
         }
     if plain is not None:
@@ -352,9 +416,11 @@ def main():
         tr = O.time_reference_iteration(W.model0, W.images[:Ns], W.mask, W.cfg.side, W.x, W.y, W.z, W.q, W.w, N)
         out["cpu_baseline"] = {
             "value": pairs / tr["full_s"], "unit": "pairs/s", "cores": tr["threads"], "kind": "port",
-            "sample": (f"one iteration of the oracle's FP32 restatement on {Ns} of {N} patterns x all {R} "
-                       f"rotations ({tr['sample_s']:.1f} s); per-pattern phases scaled by {N}/{Ns}, "
-                       f"expand/insert/normalise counted once: {tr['full_s']:.1f} s per full iteration"),
+            "sample": (f"one iteration of the oracle's FP32 restatement of the reference's CPU path "
+                       f"(its absolute-difference likelihood, the same pattern x pixel x rotation "
+                       f"contraction) on {Ns} of {N} patterns x all {R} rotations ({tr['sample_s']:.1f} s); "
+                       f"per-pattern phases scaled by {N}/{Ns}, expand/insert/normalise counted once: "
+                       f"{tr['full_s']:.1f} s per full iteration"),
             "phases_s": tr["phases"],
         }
     print(json.dumps(out), flush=True)

@@ -106,6 +106,7 @@ _SIGS = [
     ("emc_hip_reset_phase_times", None, []),
     ("emc_hip_phase_name", C.c_char_p, [C.c_int]),
     ("emc_hip_pack_info", C.c_int, [c_int_p, C.c_int]),
+    ("emc_hip_mstep_nnz", C.c_longlong, []),
     ("emc_hip_cnn_init", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong]),
     ("emc_hip_cnn_param_count", C.c_long, []),
     ("emc_hip_cnn_get", C.c_int, [c_float_p, c_float_p]),

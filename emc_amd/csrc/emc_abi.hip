@@ -375,16 +375,17 @@ void cuda_calculate_responsabilities(real* slices, real* images, int* d_mask, re
   if (!pack_slices(slices, N_slices, N_2d)) return;
   if (!e.d_phi_used.ensure(sizeof(float) * N_images)) return;
   EMC_CHECK(hipMemcpyAsync(e.d_phi_used.p, d_scaling, sizeof(float) * N_images, hipMemcpyDeviceToDevice, e.stream));
-  const bool prop = proposal_applies(N_images, N_slices);
+  bool prop = proposal_applies(N_images, N_slices);
   if (prop) {
     if (respons) {
       set_error("cuda_calculate_responsabilities: no host copy of the log-likelihoods in proposal mode");
       return;
     }
-    if (!proposal_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>())) return;
-  } else {
-    run_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), d_respons, nullptr, nullptr);
+    bool dense = false;
+    if (!proposal_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), &dense)) return;
+    prop = !dense;
   }
+  if (!prop) run_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), d_respons, nullptr, nullptr);
   g_respons_epoch++;
   auto& lk = e.like;
   lk.valid = !failed();
@@ -456,6 +457,8 @@ void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images
     lk.normalized = fuse;
     lk.have_total = true;
     lk.total = total;
+    // a whole-table iteration while the proposal tracks the posterior
+    if (fuse && proposal_tracks(N_images, N_slices)) proposal_dense_posterior(d_respons, N_images, N_slices);
   } else {
     lk.valid = false;
   }
@@ -563,9 +566,6 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
       !e.d_wts.ensure(sizeof(float) * R) || !e.d_wT.ensure(sizeof(float) * R) ||
       (poisson && (!e.d_Tphi.ensure(sizeof(float) * R) || !e.d_ones.ensure(sizeof(float) * N))))
     return 0.0f;
-  if (fp16 && !prop &&
-      (!e.d_P16.ensure(2 * (size_t)e.R_pad * e.N_pad) || !e.d_P16lo.ensure(2 * (size_t)e.R_pad * e.N_pad)))
-    return 0.0f;
   EMC_CHECK(hipMemcpyAsync(e.d_wts.p, weights, sizeof(float) * R, hipMemcpyHostToDevice, e.stream));
   // 1/phi from the current scaling: the M-step divides by the updated
   // factors (emc_atomic.cu:94)
@@ -596,15 +596,57 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
     a.n_m = e.n_m;
     a.N2d = N_2d;
     a.vecP = (N % 4 == 0) && ((uintptr_t)d_respons % 16 == 0);
+    // FP16 form: the M-step over the nonzero responsibilities when they are
+    // few (concentrated posteriors), else the dense matrix-core M-step
+    const bool try_sparse = fp16 && e.mstep_mode != MSTEP_DENSE;
+    bool sparse_m = false;
+    e.p_nnz = -1;
     {
       // T_r, and for the FP16 form the row scales and the hi/lo operand
       PhaseTimer pt(PH_ROWSUM);
+      if (try_sparse && (!e.d_pnnz.ensure(sizeof(int) * R) || !e.d_poff.ensure(sizeof(int) * (R + 1))))
+        return 0.0f;
       launch_rowsum(d_respons, R, N, opscale, e.d_T.as<double>(), e.d_Tf.as<float>(), e.d_Pscale.as<float>(), fp16,
-                    e.stream, poisson ? d_scaling : nullptr, poisson ? e.d_Tphi.as<float>() : nullptr);
-      if (fp16) launch_convert_p(a, e.d_P16.as<_Float16>(), e.d_P16lo.as<_Float16>(), e.stream);
+                    e.stream, poisson ? d_scaling : nullptr, poisson ? e.d_Tphi.as<float>() : nullptr,
+                    try_sparse ? e.d_pnnz.as<int>() : nullptr);
+      if (try_sparse) {
+        e.h_pnnz.resize(R);
+        EMC_CHECK(hipMemcpyAsync(e.h_pnnz.data(), e.d_pnnz.p, sizeof(int) * R, hipMemcpyDeviceToHost, e.stream));
+        sync();
+        if (failed()) return 0.0f;
+        long total = 0;
+        for (int r = 0; r < R; r++) total += e.h_pnnz[r];
+        sparse_m = e.mstep_mode == MSTEP_SPARSE || total * kSparseMstepRatio <= (long)R * N;
+        if (sparse_m) {
+          if (total > (long)INT32_MAX) {
+            set_error("cuda_update_slices: %ld nonzero responsibilities exceed the sparse M-step's 32-bit lists",
+                      total);
+            return 0.0f;
+          }
+          e.h_poff.resize(R + 1);
+          e.h_poff[0] = 0;
+          for (int r = 0; r < R; r++) e.h_poff[r + 1] = e.h_poff[r] + e.h_pnnz[r];
+          if (!e.d_pcol.ensure(sizeof(int) * std::max(total, 1L)) ||
+              !e.d_pval.ensure(sizeof(float) * std::max(total, 1L)))
+            return 0.0f;
+          EMC_CHECK(hipMemcpyAsync(e.d_poff.p, e.h_poff.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice,
+                                   e.stream));
+          launch_p_row_fill(d_respons, R, N, opscale, e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(),
+                            e.stream);
+          e.p_nnz = total;
+        }
+      }
+      if (fp16 && !sparse_m) {
+        if (!e.d_P16.ensure(2 * (size_t)e.R_pad * e.N_pad) || !e.d_P16lo.ensure(2 * (size_t)e.R_pad * e.N_pad))
+          return 0.0f;
+        launch_convert_p(a, e.d_P16.as<_Float16>(), e.d_P16lo.as<_Float16>(), e.stream);
+      }
     }
     PhaseTimer pt(PH_MSTEP);
-    if (fp16)
+    if (sparse_m)
+      launch_mstep_sparse(e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(), e.d_K16.as<_Float16>(),
+                          e.k_exact ? nullptr : e.d_K16lo.as<_Float16>(), a, e.stream);
+    else if (fp16)
       launch_mstep16(!e.k_exact, e.d_P16.p, e.d_P16lo.p, e.d_Kt16.p, e.d_Kt16lo.p, a, e.stream);
     else
       launch_mstep32(e.d_Kt32.as<float>(), a, e.stream);
@@ -860,15 +902,21 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (is(value, "on")) e.sparse = true;
     else if (is(value, "off")) e.sparse = false;
     else return 1;
+  } else if (is(key, "mstep")) {
+    if (is(value, "auto")) e.mstep_mode = MSTEP_AUTO;
+    else if (is(value, "dense")) e.mstep_mode = MSTEP_DENSE;
+    else if (is(value, "sparse")) e.mstep_mode = MSTEP_SPARSE;
+    else return 1;
   } else if (is(key, "proposal")) {
     if (is(value, "on")) proposal().active = true;
     else if (is(value, "off")) proposal().active = false;
     else return 1;
-  } else if (is(key, "audit_fraction") || is(key, "min_mass")) {
+  } else if (is(key, "audit_fraction") || is(key, "min_mass") || is(key, "audit_max_fail")) {
     char* end = nullptr;
     const double v = strtod(value, &end);
     if (!end || *end != 0 || !(v >= 0.0 && v <= 1.0)) return 1;
-    (is(key, "min_mass") ? proposal().min_mass : proposal().audit_frac) = (float)v;
+    Proposal& p = proposal();
+    (is(key, "min_mass") ? p.min_mass : is(key, "audit_max_fail") ? p.max_fail : p.audit_frac) = (float)v;
   } else if (is(key, "proposal_track")) {
     if (is(value, "on")) proposal().track = true;
     else if (is(value, "off")) proposal().track = false;
@@ -956,5 +1004,7 @@ int emc_hip_pack_info(int* out, int n) {
   for (; k < n && k < 4; k++) out[k] = v[k];
   return k;
 }
+
+long long emc_hip_mstep_nnz(void) { return engine().p_nnz; }
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "emc.h"
 #include "emc_hip.h"
@@ -42,6 +43,10 @@ enum Phase {
 
 enum Precision { PREC_FP32 = 0, PREC_FP16 = 1 };
 enum InsertMode { INSERT_ATOMIC = 0, INSERT_DETERMINISTIC = 1 };
+// auto: the nonzero-list M-step when at most 1 in kSparseMstepRatio
+// responsibilities is nonzero, else the dense matrix-core M-step
+enum MstepMode { MSTEP_AUTO = 0, MSTEP_DENSE = 1, MSTEP_SPARSE = 2 };
+constexpr long kSparseMstepRatio = 200;
 enum Interp { INTERP_NEAREST = 0, INTERP_TRILINEAR = 1 };
 // Gaussian: the reference's L = -sum m (S - K/phi)^2 / (2 n_m sigma^2).
 // Poisson: L = sum m (K log(phi max(S, kSlogFloor)) - phi S); the floor bounds
@@ -69,6 +74,7 @@ struct Engine {
   int interp = INTERP_NEAREST;
   int likelihood = LIKE_GAUSSIAN;
   bool sparse = false;  // Poisson E-step from CSR counts (see k_sparse.hip)
+  int mstep_mode = MSTEP_AUTO;  // FP16 form: M-step over the nonzero responsibilities
   bool timing = false;
   bool verbose = false;
 
@@ -138,6 +144,11 @@ struct Engine {
   DBuf d_ones;       // float [N]: 1 (Poisson: the M-step operand is P itself)
   DBuf d_Pscale;     // float [R]: FP16 M-step operand row scale
   DBuf d_P16, d_P16lo;  // half [R_pad][N_pad]: FP16 M-step operand (P/phi * row scale) hi/lo
+  DBuf d_pnnz;       // int [R]: nonzero responsibilities of each rotation
+  DBuf d_poff;       // int [R+1]: their offsets in the lists below
+  DBuf d_pcol, d_pval;  // int / float [nnz]: pattern, P * operand scale (patterns ascending)
+  long p_nnz = -1;      // nonzeros of the last M-step (-1: dense form)
+  std::vector<int> h_pnnz, h_poff;
   DBuf d_Sc;         // float [R_pad][n_mp]: updated compact slices for insertion
   DBuf d_wts;        // float [R]: rotation weights (uploaded per call)
   DBuf d_wT;         // float [R]: weights[r] * T_r
@@ -261,7 +272,7 @@ void launch_sum_f64(const double* v, int n, double* out, hipStream_t st);
 // model helpers (k_model.hip)
 void launch_rowsum(const float* P, int R, int N, const float* invphi, double* T, float* Tf,
                    float* rowscale, bool want_scale, hipStream_t st, const float* phi = nullptr,
-                   float* Tphi = nullptr);
+                   float* Tphi = nullptr, int* nnz = nullptr);
 // den[i] = sum_r P[r][i] A[r] (Poisson scaling denominator)
 void launch_p_dot_a(const float* P, int R, int N, const double* A, double* den, hipStream_t st);
 // ins_k: compact indices of the inserted pixels (mask == 1) in 8 x 8 tiles
@@ -288,6 +299,12 @@ void launch_csr_fill(const float* images, const int* pix, int N, int N2d, int n_
 void launch_transpose_hilo(const _Float16* hi, const _Float16* lo, int R_pad, int n_mp, void* out, hipStream_t st);
 void launch_estep_csr(const int* off, const int* col, const unsigned short* cnt, const void* LSt, int R_pad,
                       const LikeEpilogue& ep, hipStream_t st);
+// M-step over the nonzero responsibilities (k_sparse.hip): lists of each
+// rotation's nonzero P (patterns ascending), then S' from them
+void launch_p_row_fill(const float* P, int R, int N, const float* opscale, const int* off, int* col, float* val,
+                       hipStream_t st);
+void launch_mstep_sparse(const int* off, const int* col, const float* val, const _Float16* K, const _Float16* Klo,
+                         const MstepArgs& a, hipStream_t st);
 void launch_images_from_csr(const int* off, const int* pix, const unsigned short* cnt, int N, int N2d, float* images,
                             hipStream_t st);
 

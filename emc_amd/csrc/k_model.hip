@@ -19,24 +19,29 @@ namespace emc {
 __global__ __launch_bounds__(256) void k_rowsum(const float* __restrict__ P, int R, int N,
                                                 const float* __restrict__ invphi, double* __restrict__ T,
                                                 float* __restrict__ Tf, float* __restrict__ rowscale, int want,
-                                                const float* __restrict__ phi, float* __restrict__ Tphi) {
+                                                const float* __restrict__ phi, float* __restrict__ Tphi,
+                                                int* __restrict__ nnz) {
   const int r = blockIdx.x;
   const float* row = P + (size_t)r * N;
   double t = 0.0, tp = 0.0;
   float mx = 0.0f;
+  int nz = 0;
   for (int i = threadIdx.x; i < N; i += 256) {
     const float p = row[i];
     t += p;
+    nz += p != 0.0f;
     if (phi) tp += (double)p * (double)phi[i];
     if (want) mx = fmaxf(mx, fabsf(p * invphi[i]));
   }
   t = block_sum_f64(t);
   if (phi) tp = block_sum_f64(tp);
   if (want) mx = block_max_f32(mx);
+  if (nnz) nz = (int)block_sum_f64((double)nz);
   if (threadIdx.x == 0) {
     T[r] = t;
     Tf[r] = (float)t;
     if (phi) Tphi[r] = (float)tp;
+    if (nnz) nnz[r] = nz;
     if (want) {
       float sc = 1.0f;
       if (mx > 0.0f && mx < 3.0e38f) {
@@ -50,9 +55,9 @@ __global__ __launch_bounds__(256) void k_rowsum(const float* __restrict__ P, int
 }
 
 void launch_rowsum(const float* P, int R, int N, const float* invphi, double* T, float* Tf, float* rowscale,
-                   bool want_scale, hipStream_t st, const float* phi, float* Tphi) {
+                   bool want_scale, hipStream_t st, const float* phi, float* Tphi, int* nnz) {
   hipLaunchKernelGGL(k_rowsum, dim3(R), dim3(256), 0, st, P, R, N, invphi, T, Tf, rowscale, (int)want_scale, phi,
-                     Tphi);
+                     Tphi, nnz);
   EMC_LAUNCHED();
 }
 

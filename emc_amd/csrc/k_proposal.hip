@@ -688,9 +688,11 @@ __global__ __launch_bounds__(256) void k_sub_topcells_merge(const int* __restric
 void launch_topcells(const float* Pc, int N, int topk, const unsigned char* insub, const int* cand,
                      const float* Psub, int n_sub, const int* sub, int C, const int* cell_off, const int* cell_rot,
                      int ktop, int* top, float* topm, int* part_c, float* part_m, hipStream_t st) {
-  hipLaunchKernelGGL(k_cand_topcells, dim3((N + 3) / 4), dim3(256), 0, st, Pc, N, topk, insub, cand, ktop, top,
-                     topm);
-  EMC_LAUNCHED();
+  if (Pc) {
+    hipLaunchKernelGGL(k_cand_topcells, dim3((N + 3) / 4), dim3(256), 0, st, Pc, N, topk, insub, cand, ktop, top,
+                       topm);
+    EMC_LAUNCHED();
+  }
   if (n_sub > 0) {
     const int ng = (C + kTopGroup - 1) / kTopGroup;
     hipLaunchKernelGGL(k_sub_topcells_part, dim3((n_sub + 63) / 64, ng), dim3(64), 0, st, Psub, n_sub, C, cell_off,

@@ -105,6 +105,92 @@ __global__ __launch_bounds__(256) void k_estep_csr(const int* __restrict__ off, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// M-step over the nonzero responsibilities.  Concentrated posteriors leave
+// almost every P_ri exactly zero, so U_r = sum_i P_ri w_i K_i (w = 1/phi, or
+// 1 for Poisson) is a short sum per rotation: lists of each rotation's
+// nonzero (pattern, P_ri w_i), patterns ascending, then one FP32 FMA per
+// (nonzero, pixel) against the packed count rows.  The dropped terms are
+// exact zeros, so S' is the dense M-step's sum in another order (and with P
+// in FP32 instead of the FP16 hi/lo operand).
+// ---------------------------------------------------------------------------
+
+// One wave per rotation: the nonzeros of row r of P [R][N] in pattern order.
+__global__ __launch_bounds__(256) void k_p_row_fill(const float* __restrict__ P, int R, int N,
+                                                    const float* __restrict__ opscale, const int* __restrict__ off,
+                                                    int* __restrict__ col, float* __restrict__ val) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= R) return;
+  const float* row = P + (size_t)r * N;
+  int at = off[r];
+  for (int i0 = 0; i0 < N; i0 += 64) {
+    const int i = i0 + lane;
+    const float p = i < N ? row[i] : 0.0f;
+    const bool nz = p != 0.0f;
+    const unsigned long long m = __ballot(nz);
+    if (nz) {
+      const int pos = at + __popcll(m & ((1ull << lane) - 1ull));
+      col[pos] = i;
+      val[pos] = p * opscale[i];
+    }
+    at += __popcll(m);
+  }
+}
+
+// One workgroup = one rotation x 2,048 compact pixels (8 per thread, one
+// 16-byte load of FP16 counts per nonzero); the epilogue of mstep_store.
+constexpr int kSpPix = 8;
+
+__global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ off, const int* __restrict__ col,
+                                                      const float* __restrict__ val, const _Float16* __restrict__ K,
+                                                      const _Float16* __restrict__ Klo, MstepArgs a) {
+  typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+  const int r = blockIdx.x;
+  const int k0 = (blockIdx.y * 256 + threadIdx.x) * kSpPix;
+  if (k0 >= a.n_mp) return;
+  float acc[kSpPix];
+#pragma unroll
+  for (int t = 0; t < kSpPix; t++) acc[t] = 0.0f;
+  const int j1 = off[r + 1];
+  for (int j = off[r]; j < j1; j++) {
+    const size_t base = (size_t)col[j] * a.n_mp + k0;
+    const float w = val[j];
+    const half8 h = *reinterpret_cast<const half8*>(K + base);
+    if (Klo) {
+      const half8 l = *reinterpret_cast<const half8*>(Klo + base);
+#pragma unroll
+      for (int t = 0; t < kSpPix; t++) acc[t] += w * ((float)h[t] + (float)l[t]);
+    } else {
+#pragma unroll
+      for (int t = 0; t < kSpPix; t++) acc[t] += w * (float)h[t];
+    }
+  }
+  const float T = a.Tf[r];
+#pragma unroll
+  for (int t = 0; t < kSpPix; t++) {
+    const int k = k0 + t;
+    if (k >= a.n_m) break;
+    const float u = acc[t] * a.kscale_inv;
+    const float s = a.poisson ? (T > 1e-30f ? u / T : 0.0f) : (T > 1e-10f ? u / T : u);
+    a.slices[(size_t)r * a.N2d + a.pix[k]] = s;
+    a.Sc[(size_t)r * a.n_mp + k] = s;
+  }
+}
+
+void launch_p_row_fill(const float* P, int R, int N, const float* opscale, const int* off, int* col, float* val,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_p_row_fill, dim3((R + 3) / 4), dim3(256), 0, st, P, R, N, opscale, off, col, val);
+  EMC_LAUNCHED();
+}
+
+void launch_mstep_sparse(const int* off, const int* col, const float* val, const _Float16* K, const _Float16* Klo,
+                         const MstepArgs& a, hipStream_t st) {
+  const dim3 grid(a.R, (a.n_mp + 256 * kSpPix - 1) / (256 * kSpPix));
+  hipLaunchKernelGGL(k_mstep_sparse, grid, dim3(256), 0, st, off, col, val, K, Klo, a);
+  EMC_LAUNCHED();
+}
+
 // Dense images from CSR: images[i][pix] = count (zero elsewhere, filled first).
 __global__ __launch_bounds__(256) void k_images_from_csr(const int* __restrict__ off, const int* __restrict__ pix,
                                                          const unsigned short* __restrict__ cnt, int N, int N2d,

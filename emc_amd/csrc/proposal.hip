@@ -30,10 +30,11 @@ void Proposal::release() {
                   &d_chunk_off, &d_chunk_cell, &d_seg_cell, &d_seg_chunk0, &d_seg_nch, &d_seg_part, &d_mcell,
                   &d_mcell_part, &d_part, &d_part_T, &d_part_Tphi, &d_part_max, &d_Lc, &d_panels, &d_crowscale, &d_plogp, &d_zero, &d_sub, &d_insub, &d_Ksub,
                   &d_Ksublo, &d_Ktsub, &d_Ktsublo, &d_Bsub, &d_phisub, &d_invphisub, &d_Lsub, &d_phiden_sub,
-                  &d_Psub16, &d_Psub16lo, &d_mass, &d_tot};
+                  &d_Psub16, &d_Psub16lo, &d_mass, &d_tot, &d_iota, &d_part_c, &d_part_m};
   for (DBuf* b : bufs) b->release();
   cnn.release();
   configured = predicted = have_p = false;
+  post = POST_NONE;
 }
 
 template <class T>
@@ -147,13 +148,16 @@ static bool recentre(Proposal& p) {
   hipStream_t st = e.stream;
   const int N = p.N, K = Proposal::kTop, ng = (p.C + 63) / 64;
   if (!p.d_top.ensure(sizeof(int) * (size_t)N * K) || !p.d_topm.ensure(sizeof(float) * (size_t)N * K)) return false;
-  DBuf part_c, part_m;
-  const size_t np = (size_t)std::max(p.n_sub, 1) * ng * 4;
-  if (!part_c.ensure(sizeof(int) * np) || !part_m.ensure(sizeof(float) * np)) return false;
-  launch_topcells(p.d_Lc.as<float>(), N, p.topk, p.d_insub.as<unsigned char>(), p.d_cand.as<int>(),
-                  p.d_Lsub.as<float>(), p.n_sub, p.d_sub.as<int>(), p.C, p.d_cell_off.as<int>(),
-                  p.d_cell_rot.as<int>(), K, p.d_top.as<int>(), p.d_topm.as<float>(), part_c.as<int>(),
-                  part_m.as<float>(), st);
+  if (p.post == Proposal::POST_CAND) {
+    const size_t np = (size_t)std::max(p.n_sub, 1) * ng * 4;
+    if (!p.d_part_c.ensure(sizeof(int) * np) || !p.d_part_m.ensure(sizeof(float) * np)) return false;
+    launch_topcells(p.d_Lc.as<float>(), N, p.topk, p.d_insub.as<unsigned char>(), p.d_cand.as<int>(),
+                    p.d_Lsub.as<float>(), p.n_sub, p.d_sub.as<int>(), p.C, p.d_cell_off.as<int>(),
+                    p.d_cell_rot.as<int>(), K, p.d_top.as<int>(), p.d_topm.as<float>(), p.d_part_c.as<int>(),
+                    p.d_part_m.as<float>(), st);
+  } else if (p.post != Proposal::POST_TOP || p.post_N != N) {
+    return true;
+  }
   std::vector<int> top((size_t)N * K);
   std::vector<float> topm((size_t)N * K);
   EMC_CHECK(hipMemcpyAsync(top.data(), p.d_top.p, sizeof(int) * top.size(), hipMemcpyDeviceToHost, st));
@@ -175,12 +179,13 @@ static bool recentre(Proposal& p) {
     const float* mi = &topm[(size_t)i * K];
     int used = 0;
     for (int t = 0; t < K; t++)
-      if (ti[t] >= 0 && (t == 0 || mi[t] > 1e-6f * mi[0])) {
+      if (ti[t] >= 0 && ti[t] < p.C && (t == 0 || mi[t] > 1e-6f * mi[0])) {
         add(ti[t]);
         used = t + 1;
       }
     for (int t = 0; t < used; t++)
-      for (int g = 0; g < Proposal::kNbr; g++) add(p.h_nbr[(size_t)ti[t] * Proposal::kNbr + g]);
+      if (ti[t] >= 0 && ti[t] < p.C)
+        for (int g = 0; g < Proposal::kNbr; g++) add(p.h_nbr[(size_t)ti[t] * Proposal::kNbr + g]);
     for (int j = 0; j < p.topk; j++) add(p.h_net_cand[(size_t)i * p.topk + j]);
     int* ci = &p.h_cand[(size_t)i * p.topk];
     bool same = true;
@@ -194,12 +199,13 @@ static bool recentre(Proposal& p) {
   return moved == 0 || build_lists(p);
 }
 
-bool proposal_estep(int N, int R, float sigma, const float* phi_used) {
+bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dense) {
   Engine& e = engine();
   Proposal& p = proposal();
   hipStream_t st = e.stream;
-  if (p.track && p.have_post && !recentre(p)) return false;
-  p.have_post = false;
+  *dense = false;
+  if (p.track && p.post != Proposal::POST_NONE && !recentre(p)) return false;
+  p.post = Proposal::POST_NONE;
   p.iteration++;
   p.have_p = false;
   // whole-table subset: the audit sample and the patterns in full search
@@ -208,6 +214,7 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used) {
   p.h_audit.assign(N, 0);
   p.st_full = 0;
   p.st_audited = 0;
+  p.st_dense = 0;
   for (int i = 0; i < N; i++) {
     const bool aud = emc_uniform(p.seed, 0xA0D17ULL, (uint64_t)p.iteration, (uint64_t)i) < (double)p.audit_frac;
     const bool full = p.h_mode[i] != 0;
@@ -222,10 +229,86 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used) {
   p.n_sub = (int)p.h_sub.size();
   p.n_sub_pad = (int)round_up(std::max(p.n_sub, 1), kPadN);
   if (!upload(p.d_sub, p.h_sub, st) || !upload(p.d_insub, p.h_insub, st)) return false;
-  if (!p.d_zero.ensure(2 * (size_t)e.n_mp)) return false;
+  if (!p.d_zero.ensure(2 * (size_t)e.n_mp) || !e.d_phiden.ensure(sizeof(double) * N)) return false;
   launch_fill(p.d_zero.as<float>(), (size_t)e.n_mp / 2, 0.0f, st);
   const bool split = !e.k_exact;
+  const bool poisson = e.likelihood == LIKE_POISSON;
   const double inv_2nms2 = 1.0 / (2.0 * (double)(float)e.n_m * ((double)sigma * (double)sigma));
+  const double sigma2x = 2.0 * (double)(float)e.n_m * ((double)sigma * (double)sigma);
+  // the subset first: its posterior measures the candidates' captured mass
+  // before the candidates are scored
+  p.sub_total = 0.0;
+  p.st_fail = 0;
+  p.st_min_mass = p.st_mean_mass = 1.0;
+  if (p.n_sub > 0) {
+    PhaseTimer pt(PH_AUDIT);
+    const int ns = p.n_sub;
+    const size_t rows = (size_t)p.n_sub_pad * e.n_mp;
+    const int chunks = softmax_chunks(ns, R);
+    const size_t cn = (size_t)chunks * ns;
+    const int nb = (ns + 255) / 256;
+    if (!p.d_Ksub.ensure(2 * rows) || (split && !p.d_Ksublo.ensure(2 * rows)) ||
+        !p.d_Bsub.ensure(sizeof(double) * ns) || !p.d_phisub.ensure(sizeof(float) * ns) ||
+        !p.d_Lsub.ensure(sizeof(float) * (size_t)R * ns) || !p.d_mass.ensure(sizeof(float) * ns) ||
+        !e.d_smax.ensure(std::max(e.d_smax.bytes, cn * 4)) || !e.d_ssum.ensure(std::max(e.d_ssum.bytes, cn * 8)) ||
+        !e.d_sacc.ensure(std::max(e.d_sacc.bytes, 4 * cn * 8)) || !p.d_phiden_sub.ensure(sizeof(double) * ns) ||
+        !e.d_red.ensure(std::max(e.d_red.bytes, sizeof(double) * (nb + 1))))
+      return false;
+    launch_gather_rows(e.d_K16.as<_Float16>(), split ? e.d_K16lo.as<_Float16>() : nullptr, p.d_sub.as<int>(), ns,
+                       p.n_sub_pad, e.n_mp, p.d_Ksub.as<_Float16>(), split ? p.d_Ksublo.as<_Float16>() : nullptr, st);
+    launch_gather_f64(e.d_B.as<double>(), p.d_sub.as<int>(), ns, p.d_Bsub.as<double>(), st);
+    launch_gather_f32(phi_used, p.d_sub.as<int>(), ns, p.d_phisub.as<float>(), st);
+    LikeEpilogue ep{};
+    ep.L = p.d_Lsub.as<float>();
+    ep.A = e.d_A.as<double>();
+    ep.B = p.d_Bsub.as<double>();
+    ep.phi = p.d_phisub.as<float>();
+    ep.cscale = e.d_Sscale.as<float>();
+    ep.kscale_inv = 1.0f / e.k_scale;
+    ep.inv_2nms2 = inv_2nms2;
+    ep.N = ns;
+    ep.R = R;
+    ep.poisson = poisson;
+    ep.vecL = ns % 4 == 0;
+    launch_estep16(split, p.d_Ksub.p, p.d_Ksublo.p, e.d_S16.p, e.d_S16lo.p, p.n_sub_pad, e.R_pad, e.n_mp, ep, st);
+    launch_softmax(p.d_Lsub.as<float>(), R, ns, chunks, e.d_smax.as<float>(), e.d_ssum.as<double>(),
+                   e.d_sacc.as<double>(), e.d_A.as<double>(), true, st);
+    launch_softmax_reduce(ns, chunks, e.d_sacc.as<double>(), true, p.d_Bsub.as<double>(), p.d_phisub.as<float>(),
+                          sigma2x, p.d_phiden_sub.as<double>(), e.d_red.as<double>(), st, poisson);
+    launch_sub_finish(p.d_Lsub.as<float>(), ns, p.d_sub.as<int>(), p.d_phiden_sub.as<double>(),
+                      e.d_phiden.as<double>(), p.d_cand.as<int>(), p.topk, p.d_cell_off.as<int>(),
+                      p.d_cell_rot.as<int>(), p.d_mass.as<float>(), st);
+    std::vector<float> mass(ns);
+    EMC_CHECK(hipMemcpyAsync(&p.sub_total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, st));
+    EMC_CHECK(hipMemcpyAsync(mass.data(), p.d_mass.p, sizeof(float) * ns, hipMemcpyDeviceToHost, st));
+    sync_stream();
+    if (failed()) return false;
+    // search mode of every pattern scored on the whole table: full search
+    // while its candidates hold less than min_mass of its posterior
+    double fails = 0, mn = 1.0, sum = 0.0;
+    for (int k = 0; k < ns; k++) {
+      const int i = p.h_sub[k];
+      const bool low = mass[k] < p.min_mass;
+      if (p.h_audit[i]) {
+        fails += low;
+        mn = std::min(mn, (double)mass[k]);
+        sum += mass[k];
+      }
+      p.h_mode[i] = low ? 1 : 0;
+    }
+    p.st_fail = fails;
+    p.st_min_mass = p.st_audited > 0 ? mn : 1.0;
+    p.st_mean_mass = p.st_audited > 0 ? sum / p.st_audited : 1.0;
+  }
+  if (p.st_audited > 0 && p.st_fail > (double)p.max_fail * p.st_audited) {
+    // the audit says the candidates miss too much of the posterior: this
+    // iteration runs on the whole table
+    p.st_dense = 1;
+    p.st_pairs = (double)N * R;
+    p.st_cand_pairs = 0;
+    *dense = true;
+    return true;
+  }
   {
     PhaseTimer pt(PH_LIKE);
     CandEstepArgs a{};
@@ -249,34 +332,9 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used) {
     a.inv_2nms2 = inv_2nms2;
     a.n_mp = e.n_mp;
     a.topk = p.topk;
-    a.poisson = e.likelihood == LIKE_POISSON;
+    a.poisson = poisson;
     a.Lc = p.d_Lc.as<float>();
     launch_cand_estep(split, a, st);
-  }
-  if (p.n_sub > 0) {
-    PhaseTimer pt(PH_AUDIT);
-    const size_t rows = (size_t)p.n_sub_pad * e.n_mp;
-    if (!p.d_Ksub.ensure(2 * rows) || (split && !p.d_Ksublo.ensure(2 * rows)) ||
-        !p.d_Bsub.ensure(sizeof(double) * p.n_sub) || !p.d_phisub.ensure(sizeof(float) * p.n_sub) ||
-        !p.d_Lsub.ensure(sizeof(float) * (size_t)R * p.n_sub))
-      return false;
-    launch_gather_rows(e.d_K16.as<_Float16>(), split ? e.d_K16lo.as<_Float16>() : nullptr, p.d_sub.as<int>(), p.n_sub,
-                       p.n_sub_pad, e.n_mp, p.d_Ksub.as<_Float16>(), split ? p.d_Ksublo.as<_Float16>() : nullptr, st);
-    launch_gather_f64(e.d_B.as<double>(), p.d_sub.as<int>(), p.n_sub, p.d_Bsub.as<double>(), st);
-    launch_gather_f32(phi_used, p.d_sub.as<int>(), p.n_sub, p.d_phisub.as<float>(), st);
-    LikeEpilogue ep{};
-    ep.L = p.d_Lsub.as<float>();
-    ep.A = e.d_A.as<double>();
-    ep.B = p.d_Bsub.as<double>();
-    ep.phi = p.d_phisub.as<float>();
-    ep.cscale = e.d_Sscale.as<float>();
-    ep.kscale_inv = 1.0f / e.k_scale;
-    ep.inv_2nms2 = inv_2nms2;
-    ep.N = p.n_sub;
-    ep.R = R;
-    ep.poisson = e.likelihood == LIKE_POISSON;
-    ep.vecL = p.n_sub % 4 == 0;
-    launch_estep16(split, p.d_Ksub.p, p.d_Ksublo.p, e.d_S16.p, e.d_S16lo.p, p.n_sub_pad, e.R_pad, e.n_mp, ep, st);
   }
   return !failed();
 }
@@ -285,7 +343,7 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   Engine& e = engine();
   Proposal& p = proposal();
   hipStream_t st = e.stream;
-  if (!p.d_tot.ensure(sizeof(double) * 2) || !p.d_mass.ensure(sizeof(float) * std::max(p.n_sub, 1))) return false;
+  if (!p.d_tot.ensure(sizeof(double) * 2)) return false;
   {
     PhaseTimer pt(PH_SOFTMAX);
     launch_fill(respons, (size_t)N * R, 0.0f, st);
@@ -308,57 +366,55 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
     launch_cand_softmax(a, st);
     launch_sum_f64(p.d_plogp.as<double>(), N, p.d_tot.as<double>(), st);
   }
-  double tot[2] = {0.0, 0.0};
-  std::vector<float> mass(p.n_sub);
   if (p.n_sub > 0) {
+    // the subset's posterior (computed with the E-step) into its columns
     PhaseTimer pt(PH_AUDIT);
-    const int ns = p.n_sub;
-    const int chunks = softmax_chunks(ns, R);
-    const size_t cn = (size_t)chunks * ns;
-    const int nb = (ns + 255) / 256;
-    if (!e.d_smax.ensure(std::max(e.d_smax.bytes, cn * 4)) || !e.d_ssum.ensure(std::max(e.d_ssum.bytes, cn * 8)) ||
-        !e.d_sacc.ensure(std::max(e.d_sacc.bytes, 4 * cn * 8)) || !p.d_phiden_sub.ensure(sizeof(double) * ns) ||
-        !e.d_red.ensure(std::max(e.d_red.bytes, sizeof(double) * (nb + 1))))
-      return false;
-    launch_softmax(p.d_Lsub.as<float>(), R, ns, chunks, e.d_smax.as<float>(), e.d_ssum.as<double>(),
-                   e.d_sacc.as<double>(), e.d_A.as<double>(), true, st);
-    launch_softmax_reduce(ns, chunks, e.d_sacc.as<double>(), true, p.d_Bsub.as<double>(), p.d_phisub.as<float>(),
-                          sigma2x, p.d_phiden_sub.as<double>(), e.d_red.as<double>(), st,
-                          e.likelihood == LIKE_POISSON);
-    launch_sub_finish(p.d_Lsub.as<float>(), ns, p.d_sub.as<int>(), p.d_phiden_sub.as<double>(),
-                      e.d_phiden.as<double>(), p.d_cand.as<int>(), p.topk, p.d_cell_off.as<int>(),
-                      p.d_cell_rot.as<int>(), p.d_mass.as<float>(), st);
-    launch_sub_scatter(p.d_Lsub.as<float>(), R, ns, p.d_sub.as<int>(), N, respons, st);
-    EMC_CHECK(hipMemcpyAsync(&tot[1], e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, st));
-    EMC_CHECK(hipMemcpyAsync(mass.data(), p.d_mass.p, sizeof(float) * ns, hipMemcpyDeviceToHost, st));
+    launch_sub_scatter(p.d_Lsub.as<float>(), R, p.n_sub, p.d_sub.as<int>(), N, respons, st);
   }
-  EMC_CHECK(hipMemcpyAsync(&tot[0], p.d_tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  double tot = 0.0;
+  EMC_CHECK(hipMemcpyAsync(&tot, p.d_tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
   sync_stream();
   if (failed()) return false;
-  // search mode of every pattern scored on the whole table: full search
-  // while its candidates hold less than min_mass of its posterior
-  double fails = 0, mn = 1.0, sum = 0.0;
-  for (int k = 0; k < p.n_sub; k++) {
-    const int i = p.h_sub[k];
-    const bool low = mass[k] < p.min_mass;
-    if (p.h_audit[i]) {
-      fails += low;
-      mn = std::min(mn, (double)mass[k]);
-      sum += mass[k];
-    }
-    p.h_mode[i] = low ? 1 : 0;
-  }
-  p.st_fail = fails;
-  p.st_min_mass = p.st_audited > 0 ? mn : 1.0;
-  p.st_mean_mass = p.st_audited > 0 ? sum / p.st_audited : 1.0;
   double pairs = (double)p.n_sub * R;
   for (int i = 0; i < N; i++)
     if (!p.h_insub[i]) pairs += p.h_ncand[i];
   p.st_pairs = pairs;
-  *total = tot[0] + tot[1];
+  *total = tot + p.sub_total;
   p.have_p = true;
-  p.have_post = true;
+  p.post = Proposal::POST_CAND;
   p.respons = respons;
+  return true;
+}
+
+bool proposal_tracks(int N, int R) {
+  const Proposal& p = proposal();
+  return p.configured && p.track && p.R == R && N > 0 && engine().precision == PREC_FP16;
+}
+
+bool proposal_dense_posterior(const float* respons, int N, int R) {
+  Engine& e = engine();
+  Proposal& p = proposal();
+  hipStream_t st = e.stream;
+  const int K = Proposal::kTop, ng = (p.C + 63) / 64;
+  p.post = Proposal::POST_NONE;
+  std::vector<int> iota(N);
+  for (int i = 0; i < N; i++) iota[i] = i;
+  if (!upload(p.d_iota, iota, st)) return false;
+  const size_t np = (size_t)N * ng * 4;
+  if (!p.d_top.ensure(sizeof(int) * (size_t)N * K) || !p.d_topm.ensure(sizeof(float) * (size_t)N * K) ||
+      !p.d_part_c.ensure(sizeof(int) * np) || !p.d_part_m.ensure(sizeof(float) * np))
+    return false;
+  {
+    PhaseTimer pt(PH_AUDIT);
+    launch_topcells(nullptr, N, p.topk, nullptr, nullptr, respons, N, p.d_iota.as<int>(), p.C,
+                    p.d_cell_off.as<int>(), p.d_cell_rot.as<int>(), K, p.d_top.as<int>(), p.d_topm.as<float>(),
+                    p.d_part_c.as<int>(), p.d_part_m.as<float>(), st);
+    sync_stream();
+  }
+  if (failed()) return false;
+  (void)R;
+  p.post = Proposal::POST_TOP;
+  p.post_N = N;
   return true;
 }
 
@@ -749,7 +805,7 @@ int emc_hip_proposal_predict(const real* d_images, const int* d_mask, int N_2d, 
   if (failed()) return 1;
   reset_modes(p);
   p.h_net_cand = p.h_cand;
-  p.have_post = false;
+  if (p.post == Proposal::POST_CAND) p.post = Proposal::POST_NONE;
   if (!build_lists(p)) return 1;
   p.predicted = true;
   p.iteration = 0;
@@ -782,7 +838,7 @@ int emc_hip_proposal_set_candidates(const int* cand, int N_images, int topk) {
   p.h_pmass.assign(N_images, 1.0f);
   reset_modes(p);
   p.h_net_cand = p.h_cand;
-  p.have_post = false;
+  if (p.post == Proposal::POST_CAND) p.post = Proposal::POST_NONE;
   if (!build_lists(p)) return 1;
   p.predicted = true;
   p.iteration = 0;
@@ -801,12 +857,12 @@ int emc_hip_proposal_get(int* cand, int* mode, float* mass, int N_images) {
 
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  const double v[14] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
+  const double v[15] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
                         p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
                         p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
-                        p.st_cand_pairs,     p.st_moved};
+                        p.st_cand_pairs,     p.st_moved,     p.st_dense};
   int k = 0;
-  for (; k < n && k < 14; k++) out[k] = v[k];
+  for (; k < n && k < 15; k++) out[k] = v[k];
   return k;
 }
 

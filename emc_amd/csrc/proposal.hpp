@@ -22,6 +22,9 @@ struct Proposal {
   int batch = 256;         // network batch
   float audit_frac = 0.02f;
   float min_mass = 0.999f;
+  // fraction of the audited patterns below min_mass above which the whole
+  // iteration runs on the whole table (1 = never)
+  float max_fail = 0.005f;
   bool gate_predicted = true;  // full search when the network's top-k mass < min_mass
   bool track = true;       // re-centre the candidates on the last posterior
   uint64_t seed = 0;
@@ -33,7 +36,13 @@ struct Proposal {
   static constexpr int kTop = 4;     // posterior cells per pattern used to re-centre
   std::vector<int> h_net_cand;       // the network's candidates [N][topk]
   DBuf d_top, d_topm;                // [N][kTop] cells, masses of the last posterior
-  bool have_post = false;            // Lc / Lsub hold the last iteration's posterior
+  // where the last posterior is: none, Lc / Lsub (candidate iteration), or
+  // already reduced into d_top / d_topm (whole-table iteration of post_N patterns)
+  enum { POST_NONE = 0, POST_CAND = 1, POST_TOP = 2 };
+  int post = POST_NONE;
+  int post_N = 0;
+  DBuf d_iota;                       // 0..N-1 (whole-table posterior as a subset)
+  DBuf d_part_c, d_part_m;           // top-cell partials of the subset columns
   DBuf d_cell_off, d_cell_rot;
   // per pattern: candidate cells [N][topk], network top-k mass, search mode
   std::vector<int> h_cand, h_ncand;
@@ -64,10 +73,11 @@ struct Proposal {
   // state shared between the calls of one iteration
   const float* respons = nullptr;  // buffer of the current proposal responsibilities
   bool have_p = false;             // softmax done, M-step may run on the candidates
+  double sub_total = 0;            // sum P log P over the whole-table subset
   // statistics of the last iteration
   double st_full = 0, st_audited = 0, st_fail = 0, st_min_mass = 1, st_mean_mass = 1, st_pairs = 0,
          st_gated = 0, st_train_loss = 0, st_train_ms = 0, st_predict_ms = 0, st_cand_pairs = 0,
-         st_moved = 0;
+         st_moved = 0, st_dense = 0;
 
   void release();
 };
@@ -77,8 +87,16 @@ Proposal& proposal();
 // Candidate-restricted steps, called by the emc.h entry points when the
 // proposal is active for (N, R).  Packed patterns and slices are current.
 bool proposal_applies(int N, int R);
-bool proposal_estep(int N, int R, float sigma, const float* phi_used);
+// Scores the whole-table subset first; when more than max_fail of the
+// audited patterns fall below min_mass, *dense is set and the caller runs
+// this iteration on the whole table instead of the candidates.
+bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dense);
 bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* total);
 bool proposal_mstep(int N, int R, float* slices, int N2d, const float* scaling);
+// Whole-table responsibilities [R][N] of an iteration run without the
+// candidates (a fallback iteration, or a dense one before prediction):
+// reduced to each pattern's top cells for re-centring, when tracking.
+bool proposal_tracks(int N, int R);
+bool proposal_dense_posterior(const float* respons, int N, int R);
 
 }  // namespace emc

@@ -97,8 +97,9 @@ void launch_gather_f32(const float* v, const int* sub, int n, float* out, hipStr
 void launch_sub_finish(const float* Psub, int n_sub, const int* sub, const double* phiden_sub, double* phiden,
                        const int* cand, int topk, const int* cell_off, const int* cell_rot, float* mass,
                        hipStream_t st);
-// the ktop (<= 4) cells holding the most posterior mass of every pattern;
-// part_c / part_m: n_sub * ceil(C / 64) * 4 scratch entries
+// the ktop (<= 4) cells holding the most posterior mass of every pattern
+// (Pc == nullptr: the subset columns only); part_c / part_m:
+// n_sub * ceil(C / 64) * 4 scratch entries
 void launch_topcells(const float* Pc, int N, int topk, const unsigned char* insub, const int* cand,
                      const float* Psub, int n_sub, const int* sub, int C, const int* cell_off, const int* cell_rot,
                      int ktop, int* top, float* topm, int* part_c, float* part_m, hipStream_t st);

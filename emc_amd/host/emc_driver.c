@@ -71,7 +71,9 @@ static void usage(void) {
           "  --proposal T: train the rotation-proposal network on iterations 1..T,\n"
           "                then score only its top-K coarse cells (default: 5 for\n"
           "                --config 3 and 4, else 0 = off)\n"
-          "  --likelihood poisson --sparse: the low-photon form (default of --config 4)\n");
+          "  --likelihood: poisson (count x log-slice) by default for --config 2..4,\n"
+          "                gaussian (the reference's absolute form) for 0 and 1\n"
+          "  --sparse: patterns held as CSR (default of --config 4)\n");
 }
 
 static void write_raw(const char *dir, const char *name, const float *v, size_t n) {
@@ -130,7 +132,7 @@ int main(int argc, char **argv) {
   if (n_rot <= 0) n_rot = P.n_rot;
   if (!precision) precision = cfg >= 2 ? "fp16" : "fp32"; /* the precision form from 128^3 on */
   if (prop_iters < 0) prop_iters = cfg >= 3 ? 5 : 0;
-  if (!like_kind) like_kind = cfg == 4 ? "poisson" : "gaussian";
+  if (!like_kind) like_kind = cfg >= 2 ? "poisson" : "gaussian";
   if (sparse < 0) sparse = cfg == 4;
   if (prop_iters > 0 && strcmp(precision, "fp16") != 0) {
     fprintf(stderr, "emc: --proposal runs with --precision fp16\n");

@@ -68,6 +68,11 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *   "sparse"     "on" (Poisson, precision "fp16"): the whole-table E-step
  *                 reads the counts as CSR (int32 pixel, uint16 count) and
  *                 contracts them with the log-slice operand; "off" (default).
+ *   "mstep"      FP16 form only.  "auto" (default): when at most 1 in 200
+ *                 responsibilities is nonzero, the M-step sums each rotation's
+ *                 nonzero P_ri (lists built from d_respons, P kept in FP32)
+ *                 instead of the dense contraction; "dense", "sparse": always
+ *                 that form.  The dropped terms are exact zeros.
  *   "proposal"   "on": once candidates exist for this pattern count (see
  *                 the proposal workflow below) and precision is "fp16", the
  *                 E-step scores each pattern only against the fine rotations
@@ -76,6 +81,10 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 whole table every proposal iteration, chosen at random.
  *   "min_mass"   captured posterior mass (default "0.999") below which a
  *                 pattern returns to whole-table search.
+ *   "audit_max_fail"  fraction of the audited patterns (default "0.005")
+ *                 allowed below min_mass: the subset is scored before the
+ *                 candidates, and when more of it falls below, the iteration
+ *                 runs on the whole table instead ("1": never).
  *   "proposal_track" "on" (default): before every proposal iteration each
  *                 pattern's candidates are re-centred on its last posterior
  *                 (the cells holding most of its mass and their nearest
@@ -163,8 +172,9 @@ int emc_hip_proposal_get(int *cand, int *mode, float *mass, int N_images);
  * prediction, [8] last training loss, [9] last training time (ms), [10] last
  * prediction time (ms), [11] patterns scored on the whole table, [12]
  * candidate pairs of all patterns (the candidate E-step's work), [13]
- * patterns whose candidates were re-centred before the iteration.  Returns
- * the number of values written (<= n). */
+ * patterns whose candidates were re-centred before the iteration, [14] 1 if
+ * the audit sent the iteration to the whole table.  Returns the number of
+ * values written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 
 /* Patterns given as CSR (no counterpart in the reference): pattern i holds
@@ -179,6 +189,11 @@ int emc_hip_images_from_csr(real *d_images, const int *row_off, const int *pix, 
  * out[0] = masked pixel count n_m, out[1] = padded count, out[2] = 1 if the
  * FP16 operand holds the counts exactly, out[3] = GEMM passes per product. */
 int emc_hip_pack_info(int *out, int n);
+
+/* Nonzero responsibilities the last dense-form M-step summed over (no
+ * counterpart in the reference): >= 0 when it ran over the per-rotation lists
+ * of nonzero P (option "mstep"), -1 when it ran as the dense contraction. */
+long long emc_hip_mstep_nnz(void);
 
 #ifdef __cplusplus
 }

@@ -7,7 +7,10 @@ C-ABI:
     on responsibilities, 1e-4 relative on scaling and model);
   - the search mode follows the audited captured mass, over two iterations;
   - a network trained on EMC's own responsibilities captures far more of the
-    posterior in its top-k cells than the untrained network.
+    posterior in its top-k cells than the untrained network;
+  - when more of the audit than audit_max_fail misses min_mass, the iteration
+    runs on the whole table (identical to the dense FP16 iteration), and the
+    next candidates are re-centred on that whole-table posterior.
 """
 import ctypes as C
 
@@ -40,7 +43,7 @@ def _reset():
 
 
 def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", min_mass="0.999", gate="audit",
-         track="off"):
+         track="off", max_fail="1"):
     E.set_option("precision", precision)
     E.set_option("insert", insert)
     E.set_option("interp", "nearest")
@@ -49,6 +52,7 @@ def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", mi
     E.set_option("min_mass", min_mass)
     E.set_option("proposal_gate", gate)
     E.set_option("proposal_track", track)
+    E.set_option("audit_max_fail", max_fail)
 
 
 def init_cells(P, topk, n_coarse=2, batch=64):
@@ -63,8 +67,8 @@ def init_cells(P, topk, n_coarse=2, batch=64):
 
 
 def stats():
-    out = (C.c_double * 14)()
-    assert E.lib().emc_hip_proposal_stats(out, 14) == 14
+    out = (C.c_double * 15)()
+    assert E.lib().emc_hip_proposal_stats(out, 15) == 15
     return list(out)
 
 
@@ -297,3 +301,39 @@ def test_tracking_recentres_on_the_posterior(prob):
     assert st2[13] > 0.5 * P.N
     assert hit == 1.0
     assert new_mass.mean() > old_mass.mean() + 0.3
+
+
+def test_audit_sends_iteration_to_whole_table(prob):
+    """Poor candidates (cells ranked 3..8): far more than audit_max_fail of
+    the audited patterns miss min_mass, so the iteration runs on the whole
+    table and equals the dense FP16 iteration; the failing audited patterns
+    go to full search; the next iteration's candidates are re-centred on the
+    whole-table posterior and hold every pattern's best cell of it."""
+    P = prob
+    topk = 6
+    opts()
+    dense = run(P)[0]
+    nc, off, rot = init_cells(P, topk)
+    cm = cell_mass(O.o64_iteration(P.model.astype(np.float64), P.images, P.mask, np.ones(P.N), P.side, P.q, P.w,
+                                   P.x, P.y, P.z)["P"], off, rot)
+    cand = np.argsort(-cm, axis=0, kind="stable")[2:2 + topk].T.astype(np.int32).copy()
+    assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
+    opts(proposal="on", audit="0.2", track="on", max_fail="0.005")
+    g1, g2 = run(P, iterations=2)
+    st1 = g1["stats"]
+    sub = audited(P.N, 1, 0.2)
+    print(f"iteration 1: {st1[2]:.0f} audited, {st1[3]:.0f} below min mass, whole table {st1[14]:.0f}")
+    assert st1[2] == len(sub) and st1[3] > 0.005 * st1[2]
+    assert st1[14] == 1 and st1[6] == P.N * P.R
+    dP = np.abs(g1["P"] - dense["P"]).max()
+    ds = rel(g1["scaling"], dense["scaling"])
+    dm = rel(g1["model"], dense["model"])
+    print(f"whole-table iteration vs dense FP16: P {dP:.2e}, scaling {ds:.2e}, model {dm:.2e}")
+    assert dP == 0.0 and ds == 0.0 and dm == 0.0
+    assert g2["stats"][1] == st1[3]  # the failing audited patterns search the whole table
+    new = np.zeros((P.N, topk), np.int32)
+    assert E.lib().emc_hip_proposal_get(E.iptr(new), None, None, P.N) == 0
+    best1 = cell_mass(g1["P"].astype(np.float64), off, rot).argmax(axis=0)
+    hit = np.mean([best1[i] in new[i] for i in range(P.N)])
+    print(f"iteration 2: re-centred {g2['stats'][13]:.0f}; best cell of iteration 1 kept {hit:.3f}")
+    assert g2["stats"][13] > 0.5 * P.N and hit == 1.0
