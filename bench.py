@@ -66,11 +66,11 @@ def phase_times(L):
 
 
 def proposal_stats(L):
-    out = (C.c_double * 15)()
-    k = L.emc_hip_proposal_stats(out, 15)
+    out = (C.c_double * 16)()
+    k = L.emc_hip_proposal_stats(out, 16)
     keys = ["iteration", "full_search", "audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass",
             "pairs_scored", "gated_at_prediction", "train_loss", "train_ms", "predict_ms", "whole_table",
-            "candidate_pairs", "recentred", "whole_table_iteration"]
+            "candidate_pairs", "recentred", "whole_table_iteration", "shared_cells"]
     return {keys[j]: out[j] for j in range(k)}
 
 
@@ -194,7 +194,8 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
         log(f"  proposal iteration {train_iters + warmup + k + 1}: {its[k] * 1e3:.1f} ms, "
             f"{'whole table' if st['whole_table_iteration'] else 'candidates'}, audit below min mass "
             f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f} (mean mass {st['audit_mean_mass']:.5f}), "
-            f"full search {st['full_search']:.0f}, re-centred {st['recentred']:.0f}")
+            f"full search {st['full_search']:.0f}, re-centred {st['recentred']:.0f}, "
+            f"shared cells {st['shared_cells']:.0f}, pairs {st['pairs_scored']:.3g}")
     history += hist
     total_s = time.perf_counter() - t_start
     model = prob.model()
@@ -378,7 +379,8 @@ def main():
             "per_iteration": [{"iteration": pre + args.warmup + k + 1, "ms": fast["iteration_s"][k] * 1e3,
                                "whole_table": bool(st["whole_table_iteration"]),
                                "audit_below_min_mass": st["audit_below_min_mass"], "audited": st["audited"],
-                               "audit_mean_mass": st["audit_mean_mass"], "full_search": st["full_search"]}
+                               "audit_mean_mass": st["audit_mean_mass"], "full_search": st["full_search"],
+                               "shared_cells": st["shared_cells"], "pairs_scored": st["pairs_scored"]}
                               for k, st in enumerate(steps_st)],
             "pairs_scored_per_step": scored,
             "scored_pairs_per_s": scored / dt_fast,

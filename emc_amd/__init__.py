@@ -122,6 +122,7 @@ _SIGS = [
     ("emc_hip_proposal_set_candidates", C.c_int, [c_int_p, C.c_int, C.c_int]),
     ("emc_hip_proposal_get", C.c_int, [c_int_p, c_int_p, c_float_p, C.c_int]),
     ("emc_hip_proposal_stats", C.c_int, [C.POINTER(C.c_double), C.c_int]),
+    ("emc_hip_proposal_shared", C.c_int, [c_int_p, C.c_int]),
     ("emc_hip_images_from_csr", C.c_int, [C.c_void_p, c_int_p, c_int_p, C.POINTER(C.c_ushort), C.c_int, C.c_int]),
     # emc_host.h
     ("emc_n_to_samples", C.c_int, [C.c_int]),

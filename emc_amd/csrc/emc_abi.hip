@@ -574,74 +574,78 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   if (poisson) launch_fill(e.d_ones.as<float>(), N, 1.0f, e.stream);
   else launch_reciprocal(d_scaling, N, e.d_invphi.as<float>(), e.stream);
   const float* opscale = poisson ? e.d_ones.as<float>() : e.d_invphi.as<float>();
-  if (prop) {
-    // candidate-restricted M-step (+ the whole-table subset)
-    if (!proposal_mstep(N, R, slices, N_2d, d_scaling)) return 0.0f;
-  } else {
-    MstepArgs a{};
-    a.P = d_respons;
-    a.invphi = opscale;
-    a.rowscale = fp16 ? e.d_Pscale.as<float>() : nullptr;
-    a.Tf = poisson ? e.d_Tphi.as<float>() : e.d_Tf.as<float>();
-    a.poisson = poisson;
-    a.pix = e.d_pix.as<int>();
-    a.slices = slices;
-    a.Sc = e.d_Sc.as<float>();
-    a.kscale_inv = 1.0f / e.k_scale;
-    a.N = N;
-    a.R = R;
-    a.N_pad = e.N_pad;
-    a.R_pad = e.R_pad;
-    a.n_mp = e.n_mp;
-    a.n_m = e.n_m;
-    a.N2d = N_2d;
-    a.vecP = (N % 4 == 0) && ((uintptr_t)d_respons % 16 == 0);
-    // FP16 form: the M-step over the nonzero responsibilities when they are
-    // few (concentrated posteriors), else the dense matrix-core M-step
-    const bool try_sparse = fp16 && e.mstep_mode != MSTEP_DENSE;
-    bool sparse_m = false;
-    e.p_nnz = -1;
-    {
-      // T_r, and for the FP16 form the row scales and the hi/lo operand
-      PhaseTimer pt(PH_ROWSUM);
-      if (try_sparse && (!e.d_pnnz.ensure(sizeof(int) * R) || !e.d_poff.ensure(sizeof(int) * (R + 1))))
-        return 0.0f;
+  MstepArgs a{};
+  a.P = d_respons;
+  a.invphi = opscale;
+  a.rowscale = fp16 ? e.d_Pscale.as<float>() : nullptr;
+  a.Tf = poisson ? e.d_Tphi.as<float>() : e.d_Tf.as<float>();
+  a.poisson = poisson;
+  a.pix = e.d_pix.as<int>();
+  a.slices = slices;
+  a.Sc = e.d_Sc.as<float>();
+  a.kscale_inv = 1.0f / e.k_scale;
+  a.N = N;
+  a.R = R;
+  a.N_pad = e.N_pad;
+  a.R_pad = e.R_pad;
+  a.n_mp = e.n_mp;
+  a.n_m = e.n_m;
+  a.N2d = N_2d;
+  a.vecP = (N % 4 == 0) && ((uintptr_t)d_respons % 16 == 0);
+  // FP16 form: the M-step over the nonzero responsibilities when they are
+  // few (concentrated posteriors), else the dense matrix-core M-step, or the
+  // candidate M-step in a proposal iteration (d_respons holds the complete
+  // posterior either way: candidates and whole-table subset scattered)
+  // (the candidate M-step covers the candidates and the whole-table subset,
+  // not shared rotations: with those, the dense form runs instead)
+  const bool try_sparse = fp16 && e.mstep_mode != MSTEP_DENSE;
+  const bool cand_ok = prop && !proposal_has_shared();
+  bool sparse_m = false;
+  e.p_nnz = -1;
+  {
+    // T_r, and for the dense FP16 form the row scales and the hi/lo operand
+    PhaseTimer pt(PH_ROWSUM);
+    if (try_sparse && (!e.d_pnnz.ensure(sizeof(int) * R) || !e.d_poff.ensure(sizeof(int) * (R + 1)))) return 0.0f;
+    if (try_sparse || !cand_ok)
       launch_rowsum(d_respons, R, N, opscale, e.d_T.as<double>(), e.d_Tf.as<float>(), e.d_Pscale.as<float>(), fp16,
                     e.stream, poisson ? d_scaling : nullptr, poisson ? e.d_Tphi.as<float>() : nullptr,
                     try_sparse ? e.d_pnnz.as<int>() : nullptr);
-      if (try_sparse) {
-        e.h_pnnz.resize(R);
-        EMC_CHECK(hipMemcpyAsync(e.h_pnnz.data(), e.d_pnnz.p, sizeof(int) * R, hipMemcpyDeviceToHost, e.stream));
-        sync();
-        if (failed()) return 0.0f;
-        long total = 0;
-        for (int r = 0; r < R; r++) total += e.h_pnnz[r];
-        sparse_m = e.mstep_mode == MSTEP_SPARSE || total * kSparseMstepRatio <= (long)R * N;
-        if (sparse_m) {
-          if (total > (long)INT32_MAX) {
-            set_error("cuda_update_slices: %ld nonzero responsibilities exceed the sparse M-step's 32-bit lists",
-                      total);
-            return 0.0f;
-          }
-          e.h_poff.resize(R + 1);
-          e.h_poff[0] = 0;
-          for (int r = 0; r < R; r++) e.h_poff[r + 1] = e.h_poff[r] + e.h_pnnz[r];
-          if (!e.d_pcol.ensure(sizeof(int) * std::max(total, 1L)) ||
-              !e.d_pval.ensure(sizeof(float) * std::max(total, 1L)))
-            return 0.0f;
-          EMC_CHECK(hipMemcpyAsync(e.d_poff.p, e.h_poff.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice,
-                                   e.stream));
-          launch_p_row_fill(d_respons, R, N, opscale, e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(),
-                            e.stream);
-          e.p_nnz = total;
-        }
-      }
-      if (fp16 && !sparse_m) {
-        if (!e.d_P16.ensure(2 * (size_t)e.R_pad * e.N_pad) || !e.d_P16lo.ensure(2 * (size_t)e.R_pad * e.N_pad))
+    if (try_sparse) {
+      e.h_pnnz.resize(R);
+      EMC_CHECK(hipMemcpyAsync(e.h_pnnz.data(), e.d_pnnz.p, sizeof(int) * R, hipMemcpyDeviceToHost, e.stream));
+      sync();
+      if (failed()) return 0.0f;
+      long total = 0;
+      for (int r = 0; r < R; r++) total += e.h_pnnz[r];
+      sparse_m = e.mstep_mode == MSTEP_SPARSE || total * kSparseMstepRatio <= (long)R * N;
+      if (sparse_m) {
+        if (total > (long)INT32_MAX) {
+          set_error("cuda_update_slices: %ld nonzero responsibilities exceed the sparse M-step's 32-bit lists",
+                    total);
           return 0.0f;
-        launch_convert_p(a, e.d_P16.as<_Float16>(), e.d_P16lo.as<_Float16>(), e.stream);
+        }
+        e.h_poff.resize(R + 1);
+        e.h_poff[0] = 0;
+        for (int r = 0; r < R; r++) e.h_poff[r + 1] = e.h_poff[r] + e.h_pnnz[r];
+        if (!e.d_pcol.ensure(sizeof(int) * std::max(total, 1L)) || !e.d_pval.ensure(sizeof(float) * std::max(total, 1L)))
+          return 0.0f;
+        EMC_CHECK(hipMemcpyAsync(e.d_poff.p, e.h_poff.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice,
+                                 e.stream));
+        launch_p_row_fill(d_respons, R, N, opscale, e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(),
+                          e.stream);
+        e.p_nnz = total;
       }
     }
+    if (fp16 && !sparse_m && !cand_ok) {
+      if (!e.d_P16.ensure(2 * (size_t)e.R_pad * e.N_pad) || !e.d_P16lo.ensure(2 * (size_t)e.R_pad * e.N_pad))
+        return 0.0f;
+      launch_convert_p(a, e.d_P16.as<_Float16>(), e.d_P16lo.as<_Float16>(), e.stream);
+    }
+  }
+  if (cand_ok && !sparse_m) {
+    // candidate-restricted M-step (+ the whole-table subset)
+    if (!proposal_mstep(N, R, slices, N_2d, d_scaling)) return 0.0f;
+  } else {
     PhaseTimer pt(PH_MSTEP);
     if (sparse_m)
       launch_mstep_sparse(e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(), e.d_K16.as<_Float16>(),
@@ -917,6 +921,11 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (!end || *end != 0 || !(v >= 0.0 && v <= 1.0)) return 1;
     Proposal& p = proposal();
     (is(key, "min_mass") ? p.min_mass : is(key, "audit_max_fail") ? p.max_fail : p.audit_frac) = (float)v;
+  } else if (is(key, "proposal_shared")) {
+    char* end = nullptr;
+    const long v = strtol(value, &end, 10);
+    if (!end || *end != 0 || v < 0 || v > (1 << 20)) return 1;
+    proposal().max_shared = (int)v;
   } else if (is(key, "proposal_track")) {
     if (is(value, "on")) proposal().track = true;
     else if (is(value, "off")) proposal().track = false;

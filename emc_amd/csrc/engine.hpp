@@ -260,9 +260,15 @@ void launch_mstep16(bool ksplit, const void* Phi, const void* Plo, const void* K
 int softmax_chunks(int N, int R);
 void launch_softmax(float* L, int R, int N, int chunks, float* smax, double* ssum, double* sacc,
                     const double* A, bool fuse_phi, hipStream_t st);
+// the two passes separately: chunk partials, then P with nmerge >= chunks
+// partials merged (smax / ssum / sacc hold [nmerge][N])
+void launch_softmax_partial(const float* L, int R, int N, int chunks, float* smax, double* ssum, hipStream_t st);
+void launch_softmax_finish(float* L, int R, int N, int chunks, int nmerge, const float* smax, const double* ssum,
+                           double* sacc, const double* A, bool fuse_phi, hipStream_t st);
 void launch_softmax_reduce(int N, int chunks, const double* sacc, bool fuse_phi,
                            const double* B, const float* phi_used, double sigma2x,
-                           double* phiden, double* red, hipStream_t st, bool poisson = false);
+                           double* phiden, double* red, hipStream_t st, bool poisson = false,
+                           const unsigned char* skip = nullptr);
 void launch_total_respons(const float* P, size_t n, double* red, hipStream_t st);
 void launch_scaling_finish(int N, const double* B, const double* phiden, float* scaling,
                            float* invphi, hipStream_t st);

@@ -103,29 +103,55 @@ __global__ __launch_bounds__(256) void k_cand_softmax(CandSoftmaxArgs a) {
   float* row = a.Lc + (size_t)i * a.topk * kSlot;
   const int n = a.topk * kSlot;
   const int* cand = a.cand + (size_t)i * a.topk;
+  const size_t si = (size_t)a.slot * a.N + i;
   if (a.insub[i]) {
+    if (a.mode == 1) {
+      if (lane == 0) {
+        a.smax[si] = -1.0e10f;
+        a.ssum[si] = 0.0;
+      }
+      return;
+    }
     for (int t = lane; t < n; t += 64) row[t] = 0.0f;
-    if (lane == 0) a.plogp[i] = 0.0;
+    if (lane == 0) {
+      if (a.mode == 2)
+        for (int q = 0; q < 4; q++) a.sacc[(size_t)q * a.nmerge * a.N + si] = 0.0;
+      else
+        a.plogp[i] = 0.0;
+    }
     return;
   }
-  float m = -1.0e10f;
-  for (int t = lane; t < n; t += 64) {
-    const int j = t / kSlot, s = t - j * kSlot;
-    const int c = cand[j];
-    if (s < a.cell_off[c + 1] - a.cell_off[c]) m = fmaxf(m, row[t]);
-  }
-  m = wave_max(m);
-  double sum = 0.0;
-  for (int t = lane; t < n; t += 64) {
-    const int j = t / kSlot, s = t - j * kSlot;
-    const int c = cand[j];
-    if (s < a.cell_off[c + 1] - a.cell_off[c]) {
-      const float d = row[t] - m;
-      if (d > -1.0e10f) sum += (double)expf(d);
+  float m;
+  double inv;
+  if (a.mode == 2) {
+    merge_stats(a.smax, a.ssum, a.nmerge, a.N, i, m, inv);
+  } else {
+    m = -1.0e10f;
+    for (int t = lane; t < n; t += 64) {
+      const int j = t / kSlot, s = t - j * kSlot;
+      const int c = cand[j];
+      if (s < a.cell_off[c + 1] - a.cell_off[c]) m = fmaxf(m, row[t]);
     }
+    m = wave_max(m);
+    double sum = 0.0;
+    for (int t = lane; t < n; t += 64) {
+      const int j = t / kSlot, s = t - j * kSlot;
+      const int c = cand[j];
+      if (s < a.cell_off[c + 1] - a.cell_off[c]) {
+        const float d = row[t] - m;
+        if (d > -1.0e10f) sum += (double)expf(d);
+      }
+    }
+    sum = wave_sum(sum);
+    if (a.mode == 1) {
+      if (lane == 0) {
+        a.smax[si] = m;
+        a.ssum[si] = sum;
+      }
+      return;
+    }
+    inv = 1.0 / sum;
   }
-  sum = wave_sum(sum);
-  const double inv = 1.0 / sum;
   double plogp = 0.0, ps = 0.0, pa = 0.0, pl = 0.0;
   for (int t = lane; t < n; t += 64) {
     const int j = t / kSlot, s = t - j * kSlot;
@@ -151,9 +177,17 @@ __global__ __launch_bounds__(256) void k_cand_softmax(CandSoftmaxArgs a) {
   pa = wave_sum(pa);
   pl = wave_sum(pl);
   if (lane == 0) {
-    const double f = (double)a.phi[i];
-    a.phiden[i] = a.poisson ? pa : 0.5 * f * (pa + a.B[i] / (f * f) * ps + a.sigma2x * pl);
-    a.plogp[i] = plogp;
+    if (a.mode == 2) {
+      const size_t cn = (size_t)a.nmerge * a.N;
+      a.sacc[si] = plogp;
+      a.sacc[cn + si] = ps;
+      a.sacc[2 * cn + si] = pa;
+      a.sacc[3 * cn + si] = pl;
+    } else {
+      const double f = (double)a.phi[i];
+      a.phiden[i] = a.poisson ? pa : 0.5 * f * (pa + a.B[i] / (f * f) * ps + a.sigma2x * pl);
+      a.plogp[i] = plogp;
+    }
   }
 }
 
@@ -239,6 +273,7 @@ __global__ __launch_bounds__(256) void k_sub_finish(const float* __restrict__ Ps
                                                     const int* __restrict__ sub, const double* __restrict__ phiden_sub,
                                                     double* __restrict__ phiden, const int* __restrict__ cand, int topk,
                                                     const int* __restrict__ cell_off, const int* __restrict__ cell_rot,
+                                                    const int* __restrict__ sh_rot, int R_sh,
                                                     float* __restrict__ mass) {
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -251,6 +286,8 @@ __global__ __launch_bounds__(256) void k_sub_finish(const float* __restrict__ Ps
     const int o = cell_off[c];
     if (q < cell_off[c + 1] - o) s += Psub[(size_t)cell_rot[o + q] * n_sub + k];
   }
+  // the shared rotations (disjoint from the candidate cells)
+  for (int t = lane; t < R_sh; t += 64) s += Psub[(size_t)sh_rot[t] * n_sub + k];
   s = wave_sum(s);
   if (lane == 0) {
     phiden[i] = phiden_sub[k];
@@ -259,11 +296,32 @@ __global__ __launch_bounds__(256) void k_sub_finish(const float* __restrict__ Ps
 }
 
 void launch_sub_finish(const float* Psub, int n_sub, const int* sub, const double* phiden_sub, double* phiden,
-                       const int* cand, int topk, const int* cell_off, const int* cell_rot, float* mass,
-                       hipStream_t st) {
+                       const int* cand, int topk, const int* cell_off, const int* cell_rot, const int* sh_rot,
+                       int R_sh, float* mass, hipStream_t st) {
   if (n_sub <= 0) return;
   hipLaunchKernelGGL(k_sub_finish, dim3((n_sub + 3) / 4), dim3(256), 0, st, Psub, n_sub, sub, phiden_sub, phiden,
-                     cand, topk, cell_off, cell_rot, mass);
+                     cand, topk, cell_off, cell_rot, sh_rot, R_sh, mass);
+  EMC_LAUNCHED();
+}
+
+// Shared rotations: respons[sh_rot[r]][i] = Psh[r][i] for the patterns not
+// scored on the whole table (one block per shared rotation, lanes along
+// patterns: coalesced both sides).
+__global__ __launch_bounds__(256) void k_shared_scatter(const float* __restrict__ Psh, int N,
+                                                        const int* __restrict__ sh_rot,
+                                                        const unsigned char* __restrict__ insub,
+                                                        float* __restrict__ respons) {
+  const int r = blockIdx.x;
+  float* out = respons + (size_t)sh_rot[r] * N;
+  const float* in = Psh + (size_t)r * N;
+  for (int i = threadIdx.x; i < N; i += 256)
+    if (!insub[i]) out[i] = in[i];
+}
+
+void launch_shared_scatter(const float* Psh, int R_sh, int N, const int* sh_rot, const unsigned char* insub,
+                           float* respons, hipStream_t st) {
+  if (R_sh <= 0) return;
+  hipLaunchKernelGGL(k_shared_scatter, dim3(R_sh), dim3(256), 0, st, Psh, N, sh_rot, insub, respons);
   EMC_LAUNCHED();
 }
 
@@ -683,6 +741,35 @@ __global__ __launch_bounds__(256) void k_sub_topcells_merge(const int* __restric
     top[(size_t)i * ktop + t] = lc[t];
     topm[(size_t)i * ktop + t] = fmaxf(lm[t], 0.0f);
   }
+}
+
+// Merge of each pattern's ktop candidate cells (top / topm, in and out) with
+// its ktop cells among the shared ones (local numbers -> sh_cells), for the
+// patterns not scored on the whole table.
+__global__ __launch_bounds__(256) void k_merge_top(int N, int ktop, const unsigned char* __restrict__ insub,
+                                                   int* __restrict__ top, float* __restrict__ topm,
+                                                   const int* __restrict__ top_s, const float* __restrict__ topm_s,
+                                                   const int* __restrict__ sh_cells) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N || insub[i]) return;
+  float lm[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
+  int lc[4] = {-1, -1, -1, -1};
+  for (int t = 0; t < ktop; t++) {
+    const size_t o = (size_t)i * ktop + t;
+    if (top[o] >= 0) top4_insert(topm[o], top[o], lm, lc);
+    if (top_s[o] >= 0) top4_insert(topm_s[o], sh_cells[top_s[o]], lm, lc);
+  }
+  for (int t = 0; t < ktop; t++) {
+    top[(size_t)i * ktop + t] = lc[t];
+    topm[(size_t)i * ktop + t] = fmaxf(lm[t], 0.0f);
+  }
+}
+
+void launch_merge_top(int N, int ktop, const unsigned char* insub, int* top, float* topm, const int* top_s,
+                      const float* topm_s, const int* sh_cells, hipStream_t st) {
+  hipLaunchKernelGGL(k_merge_top, dim3((N + 255) / 256), dim3(256), 0, st, N, ktop, insub, top, topm, top_s, topm_s,
+                     sh_cells);
+  EMC_LAUNCHED();
 }
 
 void launch_topcells(const float* Pc, int N, int topk, const unsigned char* insub, const int* cand,

@@ -46,8 +46,11 @@ __global__ __launch_bounds__(256) void k_softmax_partial(const float* __restrict
   ssum[(size_t)c * N + i] = s;
 }
 
-// sacc layout: [4][chunks][N] = sum P log P, sum P, sum P A_r, sum P L_ri.
-__global__ __launch_bounds__(256) void k_softmax_finish(float* __restrict__ L, int R, int N, int rows, int chunks,
+// sacc layout: [4][nmerge][N] = sum P log P, sum P, sum P A_r, sum P L_ri.
+// The normalisation merges nmerge >= chunks (max, sum) partials: the chunks
+// of L, and beyond them partials of values held elsewhere (the candidate
+// cells of the rotation proposal), merged by merge_stats in one order.
+__global__ __launch_bounds__(256) void k_softmax_finish(float* __restrict__ L, int R, int N, int rows, int nmerge,
                                                         const float* __restrict__ smax,
                                                         const double* __restrict__ ssum,
                                                         double* __restrict__ sacc,
@@ -55,11 +58,9 @@ __global__ __launch_bounds__(256) void k_softmax_finish(float* __restrict__ L, i
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int c = blockIdx.y;
   if (i >= N) return;
-  float m = -1.0e10f;
-  for (int k = 0; k < chunks; k++) m = fmaxf(m, smax[(size_t)k * N + i]);
-  double s = 0.0;
-  for (int k = 0; k < chunks; k++) s += ssum[(size_t)k * N + i] * exp((double)smax[(size_t)k * N + i] - (double)m);
-  const double inv = 1.0 / s;
+  float m;
+  double inv;
+  merge_stats(smax, ssum, nmerge, N, i, m, inv);
   const int r0 = c * rows, r1 = min(R, r0 + rows);
   double plogp = 0.0, psum = 0.0, pa = 0.0, pl = 0.0;
   for (int r = r0; r < r1; r++) {
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(256) void k_softmax_finish(float* __restrict__ L, i
       pl += (double)p * (double)v;
     }
   }
-  const size_t cn = (size_t)chunks * N;
+  const size_t cn = (size_t)nmerge * N;
   sacc[0 * cn + (size_t)c * N + i] = plogp;
   sacc[1 * cn + (size_t)c * N + i] = psum;
   sacc[2 * cn + (size_t)c * N + i] = pa;
@@ -86,14 +87,16 @@ __global__ __launch_bounds__(256) void k_softmax_finish(float* __restrict__ L, i
 // Per pattern: merge chunk partials; with the likelihood's A_r, B_i and phi
 // known, sum_r P_ri C_ir follows from L without recomputing C:
 //   C_ir = (phi_i / 2) (A_r + B_i / phi_i^2 + 2 n_m sigma^2 L_ri).
-// red[blockIdx.x] receives the block's sum of P log P.
+// red[blockIdx.x] receives the block's sum of P log P.  Patterns with
+// skip[i] set are left out (no phiden written, no P log P counted).
 __global__ __launch_bounds__(256) void k_softmax_reduce(int N, int chunks, const double* __restrict__ sacc, int fuse,
                                                         const double* __restrict__ B, const float* __restrict__ phi,
                                                         double sigma2x, double* __restrict__ phiden,
-                                                        double* __restrict__ red, int poisson) {
+                                                        double* __restrict__ red, int poisson,
+                                                        const unsigned char* __restrict__ skip) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   double plogp = 0.0;
-  if (i < N) {
+  if (i < N && !(skip && skip[i])) {
     const size_t cn = (size_t)chunks * N;
     double ps = 0.0, pa = 0.0, pl = 0.0;
     for (int k = 0; k < chunks; k++) {
@@ -121,24 +124,35 @@ __global__ __launch_bounds__(256) void k_sum_f64(const double* __restrict__ in, 
   if (threadIdx.x == 0) out[0] = s;
 }
 
-void launch_softmax(float* L, int R, int N, int chunks, float* smax, double* ssum, double* sacc,
-                    const double* A, bool fuse_phi, hipStream_t st) {
+void launch_softmax_partial(const float* L, int R, int N, int chunks, float* smax, double* ssum, hipStream_t st) {
   const int rows = (R + chunks - 1) / chunks;
   const dim3 grid((N + 255) / 256, chunks);
   hipLaunchKernelGGL(k_softmax_partial, grid, dim3(256), 0, st, L, R, N, rows, smax, ssum);
   EMC_LAUNCHED();
-  hipLaunchKernelGGL(k_softmax_finish, grid, dim3(256), 0, st, L, R, N, rows, chunks, smax, ssum, sacc, A,
+}
+
+void launch_softmax_finish(float* L, int R, int N, int chunks, int nmerge, const float* smax, const double* ssum,
+                           double* sacc, const double* A, bool fuse_phi, hipStream_t st) {
+  const int rows = (R + chunks - 1) / chunks;
+  const dim3 grid((N + 255) / 256, chunks);
+  hipLaunchKernelGGL(k_softmax_finish, grid, dim3(256), 0, st, L, R, N, rows, nmerge, smax, ssum, sacc, A,
                      (int)fuse_phi);
   EMC_LAUNCHED();
+}
+
+void launch_softmax(float* L, int R, int N, int chunks, float* smax, double* ssum, double* sacc,
+                    const double* A, bool fuse_phi, hipStream_t st) {
+  launch_softmax_partial(L, R, N, chunks, smax, ssum, st);
+  launch_softmax_finish(L, R, N, chunks, chunks, smax, ssum, sacc, A, fuse_phi, st);
 }
 
 // red must hold ceil(N/256) + 1 doubles; red[ceil(N/256)] = sum P log P.
 void launch_softmax_reduce(int N, int chunks, const double* sacc, bool fuse_phi, const double* B,
                            const float* phi_used, double sigma2x, double* phiden, double* red,
-                           hipStream_t st, bool poisson) {
+                           hipStream_t st, bool poisson, const unsigned char* skip) {
   const int nb = (N + 255) / 256;
   hipLaunchKernelGGL(k_softmax_reduce, dim3(nb), dim3(256), 0, st, N, chunks, sacc, (int)fuse_phi, B, phi_used,
-                     sigma2x, phiden, red, (int)poisson);
+                     sigma2x, phiden, red, (int)poisson, skip);
   EMC_LAUNCHED();
   hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, st, red, nb, red + nb);
   EMC_LAUNCHED();

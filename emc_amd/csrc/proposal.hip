@@ -30,11 +30,13 @@ void Proposal::release() {
                   &d_chunk_off, &d_chunk_cell, &d_seg_cell, &d_seg_chunk0, &d_seg_nch, &d_seg_part, &d_mcell,
                   &d_mcell_part, &d_part, &d_part_T, &d_part_Tphi, &d_part_max, &d_Lc, &d_panels, &d_crowscale, &d_plogp, &d_zero, &d_sub, &d_insub, &d_Ksub,
                   &d_Ksublo, &d_Ktsub, &d_Ktsublo, &d_Bsub, &d_phisub, &d_invphisub, &d_Lsub, &d_phiden_sub,
-                  &d_Psub16, &d_Psub16lo, &d_mass, &d_tot, &d_iota, &d_part_c, &d_part_m};
+                  &d_Psub16, &d_Psub16lo, &d_mass, &d_tot, &d_iota, &d_part_c, &d_part_m, &d_sh_cells, &d_sh_off, &d_sh_rot,
+                  &d_Ssh, &d_Sshlo, &d_Ash, &d_sscale_sh, &d_Lsh, &d_top_s, &d_topm_s};
   for (DBuf* b : bufs) b->release();
   cnn.release();
   configured = predicted = have_p = false;
   post = POST_NONE;
+  clear_shared();
 }
 
 template <class T>
@@ -139,10 +141,20 @@ bool proposal_applies(int N, int R) {
   return p.active && p.predicted && p.N == N && p.R == R && engine().precision == PREC_FP16;
 }
 
+static bool ensure_iota(Proposal& p, int n) {
+  if (p.d_iota.bytes >= sizeof(int) * (size_t)n) return true;
+  std::vector<int> iota(n);
+  for (int i = 0; i < n; i++) iota[i] = i;
+  return upload(p.d_iota, iota, engine().stream);
+}
+
 // Re-centre every pattern's candidates on its last posterior: the kTop cells
-// holding the most mass (from its candidates, or from the whole table for
-// the patterns scored there), then their nearest cells in mass order, then
-// the network's cells in rank order, up to topk distinct cells.
+// holding the most mass (from its candidates and the shared cells, or from
+// the whole table for the patterns scored there), then their nearest cells
+// in mass order, then the network's cells in rank order, up to topk distinct
+// cells.  Before that, the shared cells of the next iteration: every cell
+// that is some pattern's best cell or holds >= kShareMass of its posterior,
+// if there are at most max_shared of them; candidate cells exclude them.
 static bool recentre(Proposal& p) {
   Engine& e = engine();
   hipStream_t st = e.stream;
@@ -155,7 +167,23 @@ static bool recentre(Proposal& p) {
                     p.d_Lsub.as<float>(), p.n_sub, p.d_sub.as<int>(), p.C, p.d_cell_off.as<int>(),
                     p.d_cell_rot.as<int>(), K, p.d_top.as<int>(), p.d_topm.as<float>(), p.d_part_c.as<int>(),
                     p.d_part_m.as<float>(), st);
+    if (p.R_sh > 0) {
+      // the shared cells of the last iteration: their top cells from the
+      // shared responsibilities, merged with the candidates' top cells
+      const int gs = (p.n_shc + 63) / 64;
+      const size_t ns = (size_t)N * gs * 4;
+      if (!ensure_iota(p, std::max(N, p.R_sh)) || !p.d_part_c.ensure(std::max(p.d_part_c.bytes, sizeof(int) * ns)) ||
+          !p.d_part_m.ensure(std::max(p.d_part_m.bytes, sizeof(float) * ns)) ||
+          !p.d_top_s.ensure(sizeof(int) * (size_t)N * K) || !p.d_topm_s.ensure(sizeof(float) * (size_t)N * K))
+        return false;
+      launch_topcells(nullptr, N, p.topk, nullptr, nullptr, p.d_Lsh.as<float>(), N, p.d_iota.as<int>(), p.n_shc,
+                      p.d_sh_off.as<int>(), p.d_iota.as<int>(), K, p.d_top_s.as<int>(), p.d_topm_s.as<float>(),
+                      p.d_part_c.as<int>(), p.d_part_m.as<float>(), st);
+      launch_merge_top(N, K, p.d_insub.as<unsigned char>(), p.d_top.as<int>(), p.d_topm.as<float>(),
+                       p.d_top_s.as<int>(), p.d_topm_s.as<float>(), p.d_sh_cells.as<int>(), st);
+    }
   } else if (p.post != Proposal::POST_TOP || p.post_N != N) {
+    p.clear_shared();
     return true;
   }
   std::vector<int> top((size_t)N * K);
@@ -164,13 +192,42 @@ static bool recentre(Proposal& p) {
   EMC_CHECK(hipMemcpyAsync(topm.data(), p.d_topm.p, sizeof(float) * topm.size(), hipMemcpyDeviceToHost, st));
   sync_stream();
   if (failed()) return false;
+  // shared cells of the next iteration
+  p.clear_shared();
+  p.h_shared.assign(p.C, 0);
+  if (p.max_shared > 0) {
+    int n = 0;
+    for (size_t t = 0; t < top.size(); t++) {
+      const int c = top[t];
+      if (c >= 0 && c < p.C && !p.h_shared[c] && (t % K == 0 || topm[t] >= Proposal::kShareMass)) {
+        p.h_shared[c] = 1;
+        n++;
+      }
+    }
+    if (n > p.max_shared || n >= p.C) {
+      p.h_shared.assign(p.C, 0);
+    } else {
+      for (int c = 0; c < p.C; c++)
+        if (p.h_shared[c]) {
+          p.h_sh_cells.push_back(c);
+          for (int j = p.h_cell_off[c]; j < p.h_cell_off[c + 1]; j++) p.h_sh_rot.push_back(p.h_cell_rot[j]);
+          p.h_sh_off.push_back((int)p.h_sh_rot.size());
+        }
+      p.n_shc = (int)p.h_sh_cells.size();
+      p.R_sh = (int)p.h_sh_rot.size();
+      p.R_sh_pad = (int)round_up(std::max(p.R_sh, 1), kPadR);
+      if (!upload(p.d_sh_cells, p.h_sh_cells, st) || !upload(p.d_sh_off, p.h_sh_off, st) ||
+          !upload(p.d_sh_rot, p.h_sh_rot, st))
+        return false;
+    }
+  }
   std::vector<int> stamp(p.C, -1), next;
   next.reserve(p.topk);
   double moved = 0;
   for (int i = 0; i < N; i++) {
     next.clear();
     auto add = [&](int c) {
-      if (c >= 0 && c < p.C && stamp[c] != i && (int)next.size() < p.topk) {
+      if (c >= 0 && c < p.C && !p.h_shared[c] && stamp[c] != i && (int)next.size() < p.topk) {
         stamp[c] = i;
         next.push_back(c);
       }
@@ -187,6 +244,11 @@ static bool recentre(Proposal& p) {
       if (ti[t] >= 0 && ti[t] < p.C)
         for (int g = 0; g < Proposal::kNbr; g++) add(p.h_nbr[(size_t)ti[t] * Proposal::kNbr + g]);
     for (int j = 0; j < p.topk; j++) add(p.h_net_cand[(size_t)i * p.topk + j]);
+    // the shared cells may leave fewer than topk: the nearest cells of the
+    // cells taken so far, then the lowest-numbered free cells
+    for (size_t t = 0; t < next.size() && (int)next.size() < p.topk; t++)
+      for (int g = 0; g < Proposal::kNbr; g++) add(p.h_nbr[(size_t)next[t] * Proposal::kNbr + g]);
+    for (int c = 0; c < p.C && (int)next.size() < p.topk; c++) add(c);
     int* ci = &p.h_cand[(size_t)i * p.topk];
     bool same = true;
     for (int j = 0; j < p.topk; j++) same = same && ci[j] == next[j];
@@ -204,8 +266,13 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
   Proposal& p = proposal();
   hipStream_t st = e.stream;
   *dense = false;
-  if (p.track && p.post != Proposal::POST_NONE && !recentre(p)) return false;
+  if (p.track && p.post != Proposal::POST_NONE) {
+    if (!recentre(p)) return false;
+  } else {
+    p.clear_shared();
+  }
   p.post = Proposal::POST_NONE;
+  p.st_shared = p.n_shc;
   p.iteration++;
   p.have_p = false;
   // whole-table subset: the audit sample and the patterns in full search
@@ -277,7 +344,7 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
                           sigma2x, p.d_phiden_sub.as<double>(), e.d_red.as<double>(), st, poisson);
     launch_sub_finish(p.d_Lsub.as<float>(), ns, p.d_sub.as<int>(), p.d_phiden_sub.as<double>(),
                       e.d_phiden.as<double>(), p.d_cand.as<int>(), p.topk, p.d_cell_off.as<int>(),
-                      p.d_cell_rot.as<int>(), p.d_mass.as<float>(), st);
+                      p.d_cell_rot.as<int>(), p.d_sh_rot.as<int>(), p.R_sh, p.d_mass.as<float>(), st);
     std::vector<float> mass(ns);
     EMC_CHECK(hipMemcpyAsync(&p.sub_total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, st));
     EMC_CHECK(hipMemcpyAsync(mass.data(), p.d_mass.p, sizeof(float) * ns, hipMemcpyDeviceToHost, st));
@@ -335,6 +402,30 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
     a.poisson = poisson;
     a.Lc = p.d_Lc.as<float>();
     launch_cand_estep(split, a, st);
+    if (p.R_sh > 0) {
+      // every pattern on the shared rotations: gathered slice rows, dense kernel
+      const size_t rows = (size_t)p.R_sh_pad * e.n_mp;
+      if (!p.d_Ssh.ensure(2 * rows) || !p.d_Sshlo.ensure(2 * rows) || !p.d_Ash.ensure(sizeof(double) * p.R_sh) ||
+          !p.d_sscale_sh.ensure(sizeof(float) * p.R_sh) || !p.d_Lsh.ensure(sizeof(float) * (size_t)p.R_sh * N))
+        return false;
+      launch_gather_rows(e.d_S16.as<_Float16>(), e.d_S16lo.as<_Float16>(), p.d_sh_rot.as<int>(), p.R_sh, p.R_sh_pad,
+                         e.n_mp, p.d_Ssh.as<_Float16>(), p.d_Sshlo.as<_Float16>(), st);
+      launch_gather_f64(e.d_A.as<double>(), p.d_sh_rot.as<int>(), p.R_sh, p.d_Ash.as<double>(), st);
+      launch_gather_f32(e.d_Sscale.as<float>(), p.d_sh_rot.as<int>(), p.R_sh, p.d_sscale_sh.as<float>(), st);
+      LikeEpilogue ep{};
+      ep.L = p.d_Lsh.as<float>();
+      ep.A = p.d_Ash.as<double>();
+      ep.B = e.d_B.as<double>();
+      ep.phi = phi_used;
+      ep.cscale = p.d_sscale_sh.as<float>();
+      ep.kscale_inv = 1.0f / e.k_scale;
+      ep.inv_2nms2 = inv_2nms2;
+      ep.N = N;
+      ep.R = p.R_sh;
+      ep.poisson = poisson;
+      ep.vecL = N % 4 == 0;
+      launch_estep16(split, e.d_K16.p, e.d_K16lo.p, p.d_Ssh.p, p.d_Sshlo.p, e.N_pad, p.R_sh_pad, e.n_mp, ep, st);
+    }
   }
   return !failed();
 }
@@ -344,6 +435,15 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   Proposal& p = proposal();
   hipStream_t st = e.stream;
   if (!p.d_tot.ensure(sizeof(double) * 2)) return false;
+  const int chunks = p.R_sh > 0 ? softmax_chunks(N, p.R_sh) : 0;
+  const int nmerge = chunks + 1, nb = (N + 255) / 256;
+  if (p.R_sh > 0) {
+    const size_t cn = (size_t)nmerge * N;
+    if (!e.d_smax.ensure(std::max(e.d_smax.bytes, cn * 4)) || !e.d_ssum.ensure(std::max(e.d_ssum.bytes, cn * 8)) ||
+        !e.d_sacc.ensure(std::max(e.d_sacc.bytes, 4 * cn * 8)) ||
+        !e.d_red.ensure(std::max(e.d_red.bytes, sizeof(double) * (nb + 1))))
+      return false;
+  }
   {
     PhaseTimer pt(PH_SOFTMAX);
     launch_fill(respons, (size_t)N * R, 0.0f, st);
@@ -363,8 +463,32 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
     a.respons = respons;
     a.phiden = e.d_phiden.as<double>();
     a.plogp = p.d_plogp.as<double>();
-    launch_cand_softmax(a, st);
-    launch_sum_f64(p.d_plogp.as<double>(), N, p.d_tot.as<double>(), st);
+    if (p.R_sh == 0) {
+      a.mode = 0;
+      launch_cand_softmax(a, st);
+      launch_sum_f64(p.d_plogp.as<double>(), N, p.d_tot.as<double>(), st);
+    } else {
+      // candidates and shared rotations normalised together: the shared
+      // rows' chunk partials plus the candidates' as partial `chunks`
+      a.smax = e.d_smax.as<float>();
+      a.ssum = e.d_ssum.as<double>();
+      a.sacc = e.d_sacc.as<double>();
+      a.nmerge = nmerge;
+      a.slot = chunks;
+      launch_softmax_partial(p.d_Lsh.as<float>(), p.R_sh, N, chunks, a.smax, a.ssum, st);
+      a.mode = 1;
+      launch_cand_softmax(a, st);
+      launch_softmax_finish(p.d_Lsh.as<float>(), p.R_sh, N, chunks, nmerge, a.smax, a.ssum, a.sacc,
+                            p.d_Ash.as<double>(), true, st);
+      a.mode = 2;
+      launch_cand_softmax(a, st);
+      launch_softmax_reduce(N, nmerge, a.sacc, true, e.d_B.as<double>(), e.d_phi_used.as<float>(), sigma2x,
+                            e.d_phiden.as<double>(), e.d_red.as<double>(), st, a.poisson != 0,
+                            p.d_insub.as<unsigned char>());
+      launch_shared_scatter(p.d_Lsh.as<float>(), p.R_sh, N, p.d_sh_rot.as<int>(), p.d_insub.as<unsigned char>(),
+                            respons, st);
+      EMC_CHECK(hipMemcpyAsync(p.d_tot.p, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
   }
   if (p.n_sub > 0) {
     // the subset's posterior (computed with the E-step) into its columns
@@ -375,7 +499,7 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   EMC_CHECK(hipMemcpyAsync(&tot, p.d_tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
   sync_stream();
   if (failed()) return false;
-  double pairs = (double)p.n_sub * R;
+  double pairs = (double)p.n_sub * R + (double)(N - p.n_sub) * p.R_sh;
   for (int i = 0; i < N; i++)
     if (!p.h_insub[i]) pairs += p.h_ncand[i];
   p.st_pairs = pairs;
@@ -385,6 +509,8 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   p.respons = respons;
   return true;
 }
+
+bool proposal_has_shared() { return proposal().R_sh > 0; }
 
 bool proposal_tracks(int N, int R) {
   const Proposal& p = proposal();
@@ -397,9 +523,7 @@ bool proposal_dense_posterior(const float* respons, int N, int R) {
   hipStream_t st = e.stream;
   const int K = Proposal::kTop, ng = (p.C + 63) / 64;
   p.post = Proposal::POST_NONE;
-  std::vector<int> iota(N);
-  for (int i = 0; i < N; i++) iota[i] = i;
-  if (!upload(p.d_iota, iota, st)) return false;
+  if (!ensure_iota(p, N)) return false;
   const size_t np = (size_t)N * ng * 4;
   if (!p.d_top.ensure(sizeof(int) * (size_t)N * K) || !p.d_topm.ensure(sizeof(float) * (size_t)N * K) ||
       !p.d_part_c.ensure(sizeof(int) * np) || !p.d_part_m.ensure(sizeof(float) * np))
@@ -855,14 +979,21 @@ int emc_hip_proposal_get(int* cand, int* mode, float* mass, int N_images) {
   return 0;
 }
 
+int emc_hip_proposal_shared(int* cells, int max_cells) {
+  const Proposal& p = proposal();
+  if (cells)
+    for (int k = 0; k < p.n_shc && k < max_cells; k++) cells[k] = p.h_sh_cells[k];
+  return p.n_shc;
+}
+
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  const double v[15] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
+  const double v[16] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
                         p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
                         p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
-                        p.st_cand_pairs,     p.st_moved,     p.st_dense};
+                        p.st_cand_pairs,     p.st_moved,     p.st_dense,      p.st_shared};
   int k = 0;
-  for (; k < n && k < 15; k++) out[k] = v[k];
+  for (; k < n && k < 16; k++) out[k] = v[k];
   return k;
 }
 

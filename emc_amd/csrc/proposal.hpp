@@ -41,7 +41,18 @@ struct Proposal {
   enum { POST_NONE = 0, POST_CAND = 1, POST_TOP = 2 };
   int post = POST_NONE;
   int post_N = 0;
-  DBuf d_iota;                       // 0..N-1 (whole-table posterior as a subset)
+  // shared cells: the cells holding a pattern's best cell (or >= kShareMass
+  // of some pattern's posterior) in the last posterior, when there are at
+  // most max_shared of them.  Every pattern is scored on all their rotations
+  // by the dense kernel, and the candidate cells exclude them.
+  int max_shared = 512;
+  static constexpr float kShareMass = 0.01f;
+  std::vector<int> h_sh_cells, h_sh_off, h_sh_rot;  // cells, CSR of their rotations
+  std::vector<unsigned char> h_shared;              // [C]
+  int n_shc = 0, R_sh = 0, R_sh_pad = 0;
+  DBuf d_sh_cells, d_sh_off, d_sh_rot, d_Ssh, d_Sshlo, d_Ash, d_sscale_sh, d_Lsh, d_top_s, d_topm_s;
+  void clear_shared() { h_sh_cells.clear(); h_sh_off.assign(1, 0); h_sh_rot.clear(); n_shc = R_sh = R_sh_pad = 0; }
+  DBuf d_iota;                       // 0..max(N, R_sh)-1
   DBuf d_part_c, d_part_m;           // top-cell partials of the subset columns
   DBuf d_cell_off, d_cell_rot;
   // per pattern: candidate cells [N][topk], network top-k mass, search mode
@@ -77,7 +88,7 @@ struct Proposal {
   // statistics of the last iteration
   double st_full = 0, st_audited = 0, st_fail = 0, st_min_mass = 1, st_mean_mass = 1, st_pairs = 0,
          st_gated = 0, st_train_loss = 0, st_train_ms = 0, st_predict_ms = 0, st_cand_pairs = 0,
-         st_moved = 0, st_dense = 0;
+         st_moved = 0, st_dense = 0, st_shared = 0;
 
   void release();
 };
@@ -97,6 +108,7 @@ bool proposal_mstep(int N, int R, float* slices, int N2d, const float* scaling);
 // candidates (a fallback iteration, or a dense one before prediction):
 // reduced to each pattern's top cells for re-centring, when tracking.
 bool proposal_tracks(int N, int R);
+bool proposal_has_shared();
 bool proposal_dense_posterior(const float* respons, int N, int R);
 
 }  // namespace emc

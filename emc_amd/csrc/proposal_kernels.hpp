@@ -41,6 +41,16 @@ struct CandSoftmaxArgs {
   int poisson;                      // scaling denominator sum_r P_ri A_r
   float* respons;                   // dense [R][N] (zeroed) or nullptr
   double *phiden, *plogp;           // [N]
+  // with the shared rotations scored beside the candidates, the candidates
+  // are one (max, sum) partial at index `slot` of [nmerge][N]:
+  // mode 0: normalise over the candidates alone (phiden, plogp written);
+  // mode 1: write the candidates' partial into smax / ssum;
+  // mode 2: normalise with all merged partials, fused sums into sacc
+  //         ([4][nmerge][N]: P log P, P, P A, P L) at `slot`
+  int mode;
+  float* smax;
+  double *ssum, *sacc;
+  int nmerge, slot;
 };
 
 struct CandPrepArgs {
@@ -94,9 +104,14 @@ void launch_gather_rows_t(const _Float16* K, const int* sub, int n_sub, int n_su
                           hipStream_t st);
 void launch_gather_f64(const double* v, const int* sub, int n, double* out, hipStream_t st);
 void launch_gather_f32(const float* v, const int* sub, int n, float* out, hipStream_t st);
+// captured mass: candidate cells' rotations plus the R_sh shared rotations
 void launch_sub_finish(const float* Psub, int n_sub, const int* sub, const double* phiden_sub, double* phiden,
-                       const int* cand, int topk, const int* cell_off, const int* cell_rot, float* mass,
-                       hipStream_t st);
+                       const int* cand, int topk, const int* cell_off, const int* cell_rot, const int* sh_rot,
+                       int R_sh, float* mass, hipStream_t st);
+void launch_shared_scatter(const float* Psh, int R_sh, int N, const int* sh_rot, const unsigned char* insub,
+                           float* respons, hipStream_t st);
+void launch_merge_top(int N, int ktop, const unsigned char* insub, int* top, float* topm, const int* top_s,
+                      const float* topm_s, const int* sh_cells, hipStream_t st);
 // the ktop (<= 4) cells holding the most posterior mass of every pattern
 // (Pc == nullptr: the subset columns only); part_c / part_m:
 // n_sub * ceil(C / 64) * 4 scratch entries

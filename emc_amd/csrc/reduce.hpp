@@ -52,4 +52,18 @@ __device__ __forceinline__ unsigned int block_max_u32(unsigned int v) {
   return block_reduce<unsigned int>(v, OpMax(), wave_max<unsigned int>);
 }
 
+// Per-pattern normalisation from nmerge (max, sum of exp(x - max)) partials
+// [nmerge][N]: the overall maximum m (at least -1e10, the reference's floor)
+// and 1 / sum exp(x - m), merged in partial order.  Every kernel that
+// normalises values of one pattern held in different places calls this on
+// the same partials and so uses bit-identical m and 1 / sum.
+__device__ __forceinline__ void merge_stats(const float* __restrict__ smax, const double* __restrict__ ssum, int nmerge,
+                                            int N, int i, float& m, double& inv) {
+  m = -1.0e10f;
+  for (int k = 0; k < nmerge; k++) m = fmaxf(m, smax[(size_t)k * N + i]);
+  double s = 0.0;
+  for (int k = 0; k < nmerge; k++) s += ssum[(size_t)k * N + i] * exp((double)smax[(size_t)k * N + i] - (double)m);
+  inv = 1.0 / s;
+}
+
 }  // namespace emc

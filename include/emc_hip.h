@@ -90,6 +90,12 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 (the cells holding most of its mass and their nearest
  *                 cells, then the network's cells); "off": the network's
  *                 cells throughout.
+ *   "proposal_shared" largest number of shared cells (default "512"; "0":
+ *                 none).  With tracking, the cells holding some pattern's
+ *                 best cell (or >= 1% of its posterior) in the last iteration
+ *                 are shared when there are at most this many: every pattern
+ *                 is scored on all their rotations (dense kernel) besides its
+ *                 topk candidate cells, which then exclude them.
  *   "proposal_gate"  "predicted" (default): a pattern also starts in
  *                 whole-table search when the network's top-k cells hold less
  *                 than min_mass of its predicted distribution; "audit": only
@@ -165,6 +171,10 @@ int emc_hip_proposal_set_candidates(const int *cand, int N_images, int topk);
 /* Current candidates [N][topk], search modes (1 = whole table) and the
  * network's top-k mass (1 for caller-given candidates); NULL skips. */
 int emc_hip_proposal_get(int *cand, int *mode, float *mass, int N_images);
+/* Shared cells of the current proposal iteration (cells every pattern is
+ * scored on, besides its candidate cells): their count; the first max_cells
+ * written to cells (NULL skips). */
+int emc_hip_proposal_shared(int *cells, int max_cells);
 /* Statistics of the last proposal iteration: out[0] iteration, [1] patterns
  * in whole-table search, [2] audited patterns, [3] audited below min_mass,
  * [4] smallest and [5] mean captured mass of the audited patterns, [6]
@@ -173,8 +183,8 @@ int emc_hip_proposal_get(int *cand, int *mode, float *mass, int N_images);
  * prediction time (ms), [11] patterns scored on the whole table, [12]
  * candidate pairs of all patterns (the candidate E-step's work), [13]
  * patterns whose candidates were re-centred before the iteration, [14] 1 if
- * the audit sent the iteration to the whole table.  Returns the number of
- * values written (<= n). */
+ * the audit sent the iteration to the whole table, [15] shared cells.
+ * Returns the number of values written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 
 /* Patterns given as CSR (no counterpart in the reference): pattern i holds

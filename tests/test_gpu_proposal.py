@@ -10,7 +10,11 @@ C-ABI:
     posterior in its top-k cells than the untrained network;
   - when more of the audit than audit_max_fail misses min_mass, the iteration
     runs on the whole table (identical to the dense FP16 iteration), and the
-    next candidates are re-centred on that whole-table posterior.
+    next candidates are re-centred on that whole-table posterior;
+  - with shared cells (the cells holding the patterns' last posteriors),
+    every pattern is also scored on the shared cells' rotations and the
+    iteration matches the FP64 oracle restricted to candidates + shared
+    rotations (+ the whole table for the audited patterns).
 """
 import ctypes as C
 
@@ -43,7 +47,7 @@ def _reset():
 
 
 def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", min_mass="0.999", gate="audit",
-         track="off", max_fail="1"):
+         track="off", max_fail="1", mstep="auto", shared="0"):
     E.set_option("precision", precision)
     E.set_option("insert", insert)
     E.set_option("interp", "nearest")
@@ -53,6 +57,8 @@ def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", mi
     E.set_option("proposal_gate", gate)
     E.set_option("proposal_track", track)
     E.set_option("audit_max_fail", max_fail)
+    E.set_option("mstep", mstep)
+    E.set_option("proposal_shared", shared)
 
 
 def init_cells(P, topk, n_coarse=2, batch=64):
@@ -67,8 +73,8 @@ def init_cells(P, topk, n_coarse=2, batch=64):
 
 
 def stats():
-    out = (C.c_double * 15)()
-    assert E.lib().emc_hip_proposal_stats(out, 15) == 15
+    out = (C.c_double * 16)()
+    assert E.lib().emc_hip_proposal_stats(out, 16) == 16
     return list(out)
 
 
@@ -126,14 +132,16 @@ def oracle_candidates(P, off, rot, topk):
     return np.argsort(-cm, axis=0, kind="stable")[:topk].T.astype(np.int32).copy(), ref
 
 
-@pytest.mark.parametrize("audit", ["0", "0.3"])
-def test_topk_with_audit_matches_oracle(prob, audit):
+@pytest.mark.parametrize("audit,mstep", [("0", "dense"), ("0.3", "dense"), ("0.3", "sparse")])
+def test_topk_with_audit_matches_oracle(prob, audit, mstep):
+    """mstep "dense": the candidate M-step; "sparse": the M-step over the
+    nonzero responsibilities of the scattered candidate + subset posterior."""
     P = prob
     topk = 6
     nc, off, rot = init_cells(P, topk)
     cand, ref_dense = oracle_candidates(P, off, rot, topk)
     assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
-    opts(proposal="on", audit=audit)
+    opts(proposal="on", audit=audit, mstep=mstep)
     g = run(P)[0]
     sub = audited(P.N, 1, float(audit))
     allowed = O.allowed_from_candidates(cand, off, rot, P.R, sub)
@@ -143,7 +151,8 @@ def test_topk_with_audit_matches_oracle(prob, audit):
     dP = np.abs(g["P"] - ref["P"]).max()
     ds = rel(g["scaling"], ref["scaling"])
     dm = rel(g["model"], ref["model"])
-    print(f"top-{topk}, audit {audit}: {len(sub)} audited, pairs {st[6]:.0f} of {P.N * P.R}; "
+    assert E.lib().emc_hip_mstep_nnz() == (int((g["P"] != 0).sum()) if mstep == "sparse" else -1)
+    print(f"top-{topk}, audit {audit}, mstep {mstep}: {len(sub)} audited, pairs {st[6]:.0f} of {P.N * P.R}; "
           f"P {dP:.2e}, scaling {ds:.2e}, model {dm:.2e}; captured mass min {st[4]:.6f} mean {st[5]:.6f}")
     assert st[2] == len(sub) and st[11] == len(sub)
     assert st[6] == allowed.sum()
@@ -337,3 +346,55 @@ def test_audit_sends_iteration_to_whole_table(prob):
     hit = np.mean([best1[i] in new[i] for i in range(P.N)])
     print(f"iteration 2: re-centred {g2['stats'][13]:.0f}; best cell of iteration 1 kept {hit:.3f}")
     assert g2["stats"][13] > 0.5 * P.N and hit == 1.0
+
+
+@pytest.mark.parametrize("mstep", ["dense", "sparse"])
+def test_shared_cells_match_oracle(prob, mstep):
+    """Iteration 1 scores every pattern on the whole table (audit 1); before
+    iteration 2 the cells holding the patterns' best cells become shared and
+    the candidates are re-centred around them (excluding them).  Iteration 2
+    (30% audit) must equal the FP64 oracle restricted to each pattern's
+    candidate cells + all shared rotations (whole table for the audited),
+    from the GPU's own iteration-1 model and scaling."""
+    P = prob
+    topk = 6
+    nc, off, rot = init_cells(P, topk)
+    cand, _ = oracle_candidates(P, off, rot, topk)
+    assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
+    opts(proposal="on", audit="1", min_mass="0", track="on", shared="1000", mstep=mstep)
+    prob_ = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
+    prob_.iteration(0.4)
+    g1 = {"P": prob_.respons(), "model": prob_.model(), "scaling": prob_.scaling()}
+    E.set_option("audit_fraction", "0.3")
+    it2 = prob_.iteration(0.4)
+    g2 = {"P": prob_.respons(), "model": prob_.model(), "scaling": prob_.scaling(), "stats": stats()}
+    prob_.free()
+    E.check()
+    L = E.lib()
+    n_sh = L.emc_hip_proposal_shared(None, 0)
+    shared = np.zeros(max(n_sh, 1), np.int32)
+    assert L.emc_hip_proposal_shared(E.iptr(shared), n_sh) == n_sh
+    shared = shared[:n_sh]
+    new = np.zeros((P.N, topk), np.int32)
+    assert L.emc_hip_proposal_get(E.iptr(new), None, None, P.N) == 0
+    best1 = cell_mass(g1["P"].astype(np.float64), off, rot).argmax(axis=0)
+    st = g2["stats"]
+    print(f"{n_sh} shared cells of {nc}; iteration-1 best cells {len(np.unique(best1))}; stats shared {st[15]:.0f}")
+    assert 0 < n_sh < nc and st[15] == n_sh
+    assert set(np.unique(best1)) <= set(shared.tolist())
+    assert not np.isin(new, shared).any()  # candidates exclude the shared cells
+    assert all(len(np.unique(new[i])) == topk for i in range(P.N))
+    sub2 = audited(P.N, 2, 0.3)
+    allowed = O.allowed_from_candidates(new, off, rot, P.R, sub2)
+    sh_rot = np.concatenate([rot[off[c]:off[c + 1]] for c in shared])
+    allowed[sh_rot, :] = True
+    ref = O.o64_proposal_iteration(g1["model"].astype(np.float64), P.images, P.mask, g1["scaling"], P.side, P.q,
+                                   P.w, P.x, P.y, P.z, allowed)
+    dP = np.abs(g2["P"] - ref["P"]).max()
+    ds = rel(g2["scaling"], ref["scaling"])
+    dm = rel(g2["model"], ref["model"])
+    dl = abs(it2["likelihood"] - ref["total"]) / abs(ref["total"])
+    print(f"shared, mstep {mstep}: pairs {st[6]:.0f} (allowed {allowed.sum()}); P {dP:.2e}, scaling {ds:.2e}, "
+          f"model {dm:.2e}, sum PlogP {dl:.2e}")
+    assert st[11] == len(sub2) and st[6] == allowed.sum()
+    assert dP < 1e-4 and ds < 1e-4 and dm < 1e-4 and dl < 1e-5
