@@ -51,7 +51,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP16 = 2.5e15     # MI355X dense FP16 MFMA, FLOP/s
 PEAK_FP32 = 157.3e12   # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12      # bytes/s
-NPHASE = 11
+NPHASE = 12
 
 
 def log(*a):
@@ -66,11 +66,11 @@ def phase_times(L):
 
 
 def proposal_stats(L):
-    out = (C.c_double * 16)()
-    k = L.emc_hip_proposal_stats(out, 16)
+    out = (C.c_double * 17)()
+    k = L.emc_hip_proposal_stats(out, 17)
     keys = ["iteration", "full_search", "audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass",
             "pairs_scored", "gated_at_prediction", "train_loss", "train_ms", "predict_ms", "whole_table",
-            "candidate_pairs", "recentred", "whole_table_iteration", "shared_cells"]
+            "candidate_pairs", "recentred", "whole_table_iteration", "shared_cells", "recentre_ms"]
     return {keys[j]: out[j] for j in range(k)}
 
 
@@ -195,7 +195,8 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
             f"{'whole table' if st['whole_table_iteration'] else 'candidates'}, audit below min mass "
             f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f} (mean mass {st['audit_mean_mass']:.5f}), "
             f"full search {st['full_search']:.0f}, re-centred {st['recentred']:.0f}, "
-            f"shared cells {st['shared_cells']:.0f}, pairs {st['pairs_scored']:.3g}")
+            f"shared cells {st['shared_cells']:.0f}, pairs {st['pairs_scored']:.3g}, "
+            f"re-centring {st['recentre_ms']:.1f} ms host")
     history += hist
     total_s = time.perf_counter() - t_start
     model = prob.model()
@@ -380,7 +381,8 @@ def main():
                                "whole_table": bool(st["whole_table_iteration"]),
                                "audit_below_min_mass": st["audit_below_min_mass"], "audited": st["audited"],
                                "audit_mean_mass": st["audit_mean_mass"], "full_search": st["full_search"],
-                               "shared_cells": st["shared_cells"], "pairs_scored": st["pairs_scored"]}
+                               "shared_cells": st["shared_cells"], "pairs_scored": st["pairs_scored"],
+                               "recentre_host_ms": st["recentre_ms"]}
                               for k, st in enumerate(steps_st)],
             "pairs_scored_per_step": scored,
             "scored_pairs_per_s": scored / dt_fast,

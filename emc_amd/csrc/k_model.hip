@@ -26,13 +26,23 @@ __global__ __launch_bounds__(256) void k_rowsum(const float* __restrict__ P, int
   double t = 0.0, tp = 0.0;
   float mx = 0.0f;
   int nz = 0;
-  for (int i = threadIdx.x; i < N; i += 256) {
-    const float p = row[i];
+  auto take = [&](float p, int i) {
     t += p;
     nz += p != 0.0f;
     if (phi) tp += (double)p * (double)phi[i];
     if (want) mx = fmaxf(mx, fabsf(p * invphi[i]));
+  };
+  // 16-byte loads when every row starts 16-byte aligned (N % 4 == 0)
+  const int nv = ((N & 3) == 0 && ((uintptr_t)P & 15) == 0) ? N / 4 : 0;
+  for (int v = threadIdx.x; v < nv; v += 256) {
+    const float4 q = reinterpret_cast<const float4*>(row)[v];
+    if (q.x == 0.0f && q.y == 0.0f && q.z == 0.0f && q.w == 0.0f) continue;
+    take(q.x, 4 * v);
+    take(q.y, 4 * v + 1);
+    take(q.z, 4 * v + 2);
+    take(q.w, 4 * v + 3);
   }
+  for (int i = 4 * nv + threadIdx.x; i < N; i += 256) take(row[i], i);
   t = block_sum_f64(t);
   if (phi) tp = block_sum_f64(tp);
   if (want) mx = block_max_f32(mx);

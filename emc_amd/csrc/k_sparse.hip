@@ -124,7 +124,34 @@ __global__ __launch_bounds__(256) void k_p_row_fill(const float* __restrict__ P,
   if (r >= R) return;
   const float* row = P + (size_t)r * N;
   int at = off[r];
-  for (int i0 = 0; i0 < N; i0 += 64) {
+  // 16-byte loads (4 patterns per lane, 256 per wave step) when rows are
+  // 16-byte aligned; positions from a wave prefix sum of the lanes' counts
+  const int nv = ((N & 3) == 0 && ((uintptr_t)P & 15) == 0) ? N / 4 : 0;
+  for (int v0 = 0; v0 < nv; v0 += 64) {
+    const int v = v0 + lane;
+    float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (v < nv) q = reinterpret_cast<const float4*>(row)[v];
+    const int c = (q.x != 0.0f) + (q.y != 0.0f) + (q.z != 0.0f) + (q.w != 0.0f);
+    if (__ballot(c != 0) == 0ull) continue;
+    int incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    int pos = at + incl - c;
+    const float e[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      if (e[t] != 0.0f) {
+        const int i = 4 * v + t;
+        col[pos] = i;
+        val[pos] = e[t] * opscale[i];
+        pos++;
+      }
+    at += __shfl(incl, 63, 64);
+  }
+  for (int i0 = 4 * nv; i0 < N; i0 += 64) {
     const int i = i0 + lane;
     const float p = i < N ? row[i] : 0.0f;
     const bool nz = p != 0.0f;
@@ -153,7 +180,26 @@ __global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ of
 #pragma unroll
   for (int t = 0; t < kSpPix; t++) acc[t] = 0.0f;
   const int j1 = off[r + 1];
-  for (int j = off[r]; j < j1; j++) {
+  int j = off[r];
+  // four nonzeros per step: their count rows are loaded together (the loop
+  // is latency-bound otherwise); accumulation stays in list order
+  constexpr int U = 4;
+  for (; j + U <= j1; j += U) {
+    half8 h[U], l[U];
+    float w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const size_t base = (size_t)col[j + u] * a.n_mp + k0;
+      w[u] = val[j + u];
+      h[u] = *reinterpret_cast<const half8*>(K + base);
+      if (Klo) l[u] = *reinterpret_cast<const half8*>(Klo + base);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int t = 0; t < kSpPix; t++) acc[t] += w[u] * (Klo ? (float)h[u][t] + (float)l[u][t] : (float)h[u][t]);
+  }
+  for (; j < j1; j++) {
     const size_t base = (size_t)col[j] * a.n_mp + k0;
     const float w = val[j];
     const half8 h = *reinterpret_cast<const half8*>(K + base);

@@ -266,11 +266,13 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
   Proposal& p = proposal();
   hipStream_t st = e.stream;
   *dense = false;
+  const auto t0 = std::chrono::steady_clock::now();
   if (p.track && p.post != Proposal::POST_NONE) {
     if (!recentre(p)) return false;
   } else {
     p.clear_shared();
   }
+  p.st_recentre_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   p.post = Proposal::POST_NONE;
   p.st_shared = p.n_shc;
   p.iteration++;
@@ -402,8 +404,11 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
     a.poisson = poisson;
     a.Lc = p.d_Lc.as<float>();
     launch_cand_estep(split, a, st);
-    if (p.R_sh > 0) {
+  }
+  if (p.R_sh > 0) {
+    {
       // every pattern on the shared rotations: gathered slice rows, dense kernel
+      PhaseTimer pt(PH_SHARED);
       const size_t rows = (size_t)p.R_sh_pad * e.n_mp;
       if (!p.d_Ssh.ensure(2 * rows) || !p.d_Sshlo.ensure(2 * rows) || !p.d_Ash.ensure(sizeof(double) * p.R_sh) ||
           !p.d_sscale_sh.ensure(sizeof(float) * p.R_sh) || !p.d_Lsh.ensure(sizeof(float) * (size_t)p.R_sh * N))
@@ -988,12 +993,13 @@ int emc_hip_proposal_shared(int* cells, int max_cells) {
 
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  const double v[16] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
+  const double v[17] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
                         p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
                         p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
-                        p.st_cand_pairs,     p.st_moved,     p.st_dense,      p.st_shared};
+                        p.st_cand_pairs,     p.st_moved,     p.st_dense,      p.st_shared,
+                        p.st_recentre_ms};
   int k = 0;
-  for (; k < n && k < 16; k++) out[k] = v[k];
+  for (; k < n && k < 17; k++) out[k] = v[k];
   return k;
 }
 

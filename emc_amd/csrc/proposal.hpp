@@ -88,7 +88,7 @@ struct Proposal {
   // statistics of the last iteration
   double st_full = 0, st_audited = 0, st_fail = 0, st_min_mass = 1, st_mean_mass = 1, st_pairs = 0,
          st_gated = 0, st_train_loss = 0, st_train_ms = 0, st_predict_ms = 0, st_cand_pairs = 0,
-         st_moved = 0, st_dense = 0, st_shared = 0;
+         st_moved = 0, st_dense = 0, st_shared = 0, st_recentre_ms = 0;
 
   void release();
 };

@@ -113,7 +113,8 @@ void emc_hip_invalidate(void);
  *   0 expand, 1 pack (slice operands), 2 likelihood GEMM, 3 softmax,
  *   4 scaling, 5 M-step row sums, 6 M-step GEMM, 7 insert, 8 normalise,
  *   9 other, 10 audit (proposal mode: everything done for the patterns
- *   scored on the whole table).
+ *   scored on the whole table), 11 shared (proposal mode: the likelihood of
+ *   every pattern on the shared cells' rotations).
  * Returns the number of phases written (<= n). */
 int emc_hip_phase_times(double *ms, long long *launches, int n);
 void emc_hip_reset_phase_times(void);
@@ -183,8 +184,9 @@ int emc_hip_proposal_shared(int *cells, int max_cells);
  * prediction time (ms), [11] patterns scored on the whole table, [12]
  * candidate pairs of all patterns (the candidate E-step's work), [13]
  * patterns whose candidates were re-centred before the iteration, [14] 1 if
- * the audit sent the iteration to the whole table, [15] shared cells.
- * Returns the number of values written (<= n). */
+ * the audit sent the iteration to the whole table, [15] shared cells, [16]
+ * host time of the re-centring and list building (ms).  Returns the number
+ * of values written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 
 /* Patterns given as CSR (no counterpart in the reference): pattern i holds

@@ -73,8 +73,8 @@ def init_cells(P, topk, n_coarse=2, batch=64):
 
 
 def stats():
-    out = (C.c_double * 16)()
-    assert E.lib().emc_hip_proposal_stats(out, 16) == 16
+    out = (C.c_double * 17)()
+    assert E.lib().emc_hip_proposal_stats(out, 17) == 17
     return list(out)
 
 
