@@ -190,6 +190,7 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
       !e.d_ins_col.ensure(sizeof(int) * std::max(e.n_ins, 1)) || !e.d_B.ensure(sizeof(double) * N))
     return false;
   EMC_CHECK(hipMemcpyAsync(e.d_pix.p, pixp.data(), sizeof(int) * e.n_mp, hipMemcpyHostToDevice, e.stream));
+  e.pix_gen++;
   if (e.n_ins) {
     EMC_CHECK(hipMemcpyAsync(e.d_ins_pix.p, ins.data(), sizeof(int) * e.n_ins, hipMemcpyHostToDevice, e.stream));
     EMC_CHECK(hipMemcpyAsync(e.d_ins_col.p, inscol.data(), sizeof(int) * e.n_ins, hipMemcpyHostToDevice, e.stream));
@@ -354,12 +355,31 @@ void cuda_get_slices(sp_3matrix* model, real* d_model, real* d_slices, real* d_r
     set_error("cuda_get_slices: model must be a cube (got %u x %u x %u)", model->x, model->y, model->z);
     return;
   }
-  {
+  const int S = (int)model->x;
+  // FP16 form with a compact pixel list from an earlier E-step of this
+  // detector: pack the slice operands in the same pass (the E-step then
+  // skips its pack while the slices, pixel list and likelihood are these)
+  const bool fuse = e.precision == PREC_FP16 && e.pix_gen > 0 && e.pack_key.N2d == S * S && !sparse_estep() &&
+                    expand_pack_fits(S, e.n_mp) && N_slices > 0;
+  e.slice_pack = {};
+  if (fuse) {
+    e.R_pad = (int)round_up(N_slices, kPadR);
+    const size_t ns = (size_t)e.R_pad * e.n_mp;
+    if (!e.d_A.ensure(sizeof(double) * N_slices) || !e.d_S16.ensure(ns * 2) || !e.d_S16lo.ensure(ns * 2) ||
+        !e.d_Sscale.ensure(sizeof(float) * N_slices))
+      return;
     PhaseTimer pt(PH_EXPAND);
-    launch_expand(d_model, (int)model->x, d_rot, N_slices, d_x_coordinates, d_y_coordinates, d_z_coordinates,
-                  d_slices, e.interp, e.stream);
+    launch_expand_pack_f16(d_model, S, d_rot, N_slices, e.R_pad, d_x_coordinates, d_y_coordinates, d_z_coordinates,
+                           d_slices, e.d_pix.as<int>(), e.n_mp, e.d_S16.as<_Float16>(), e.d_S16lo.as<_Float16>(),
+                           e.d_Sscale.as<float>(), e.d_A.as<double>(), e.interp, e.likelihood == LIKE_POISSON,
+                           e.stream);
+  } else {
+    PhaseTimer pt(PH_EXPAND);
+    launch_expand(d_model, S, d_rot, N_slices, d_x_coordinates, d_y_coordinates, d_z_coordinates, d_slices, e.interp,
+                  e.stream);
   }
   g_slices_epoch++;
+  if (fuse) e.slice_pack = {d_slices, g_slices_epoch, e.pix_gen, N_slices, e.likelihood};
   sync();
 }
 
@@ -372,7 +392,11 @@ void cuda_calculate_responsabilities(real* slices, real* images, int* d_mask, re
     return;
   }
   if (!ensure_pack(images, d_mask, N_images, N_2d)) return;
-  if (!pack_slices(slices, N_slices, N_2d)) return;
+  const auto& sp = e.slice_pack;
+  const bool packed = e.precision == PREC_FP16 && sp.slices == slices && sp.epoch == g_slices_epoch &&
+                      sp.pix_gen == e.pix_gen && sp.R == N_slices && sp.likelihood == e.likelihood &&
+                      !sparse_estep();
+  if (!packed && !pack_slices(slices, N_slices, N_2d)) return;
   if (!e.d_phi_used.ensure(sizeof(float) * N_images)) return;
   EMC_CHECK(hipMemcpyAsync(e.d_phi_used.p, d_scaling, sizeof(float) * N_images, hipMemcpyDeviceToDevice, e.stream));
   bool prop = proposal_applies(N_images, N_slices);
@@ -647,10 +671,11 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
     if (!proposal_mstep(N, R, slices, N_2d, d_scaling)) return 0.0f;
   } else {
     PhaseTimer pt(PH_MSTEP);
-    if (sparse_m)
-      launch_mstep_sparse(e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(), e.d_K16.as<_Float16>(),
-                          e.k_exact ? nullptr : e.d_K16lo.as<_Float16>(), a, e.stream);
-    else if (fp16)
+    if (sparse_m) {
+      if (!launch_mstep_sparse(e.h_poff, e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(),
+                               e.d_K16.as<_Float16>(), e.k_exact ? nullptr : e.d_K16lo.as<_Float16>(), a, e.stream))
+        return 0.0f;
+    } else if (fp16)
       launch_mstep16(!e.k_exact, e.d_P16.p, e.d_P16lo.p, e.d_Kt16.p, e.d_Kt16lo.p, a, e.stream);
     else
       launch_mstep32(e.d_Kt32.as<float>(), a, e.stream);
@@ -945,6 +970,7 @@ void emc_hip_invalidate(void) {
   Engine& e = engine();
   e.invalidate_stamp++;
   e.like.valid = false;
+  e.slice_pack = {};
 }
 
 static const char* kPhaseNames[PH_COUNT] = {"expand", "pack", "likelihood", "softmax", "scaling",

@@ -89,6 +89,14 @@ struct Engine {
     int N = 0, N2d = 0, precision = -1, likelihood = -1, sparse = -1;
     uint64_t stamp = 0;  // invalidation stamp at build time
   } pack_key;
+  uint64_t pix_gen = 0;  // bumped whenever the compact pixel list is rebuilt
+  // slice operands packed by the expand itself (FP16 form): valid for this
+  // slices buffer at this write epoch, pixel list and likelihood
+  struct {
+    const float* slices = nullptr;
+    uint64_t epoch = 0, pix_gen = 0;
+    int R = 0, likelihood = -1;
+  } slice_pack;
   int n_m = 0;      // masked pixels (mask != 0)
   int n_mp = 0;     // padded to kPadK
   int n_ins = 0;    // pixels with mask == 1 (inserted)
@@ -149,6 +157,11 @@ struct Engine {
   DBuf d_pcol, d_pval;  // int / float [nnz]: pattern, P * operand scale (patterns ascending)
   long p_nnz = -1;      // nonzeros of the last M-step (-1: dense form)
   std::vector<int> h_pnnz, h_poff;
+  // long rows of the list M-step: segments (first, end nonzero), rows,
+  // their first segment, partial sums [segments][n_mp]
+  std::vector<int2> h_sp_seg;
+  std::vector<int> h_sp_rows, h_sp_rowseg;
+  DBuf d_sp_seg, d_sp_rows, d_sp_rowseg, d_sp_part;
   DBuf d_Sc;         // float [R_pad][n_mp]: updated compact slices for insertion
   DBuf d_wts;        // float [R]: rotation weights (uploaded per call)
   DBuf d_wT;         // float [R]: weights[r] * T_r
@@ -212,6 +225,10 @@ void launch_images_stats(const float* images, const int* pix, int N, int N2d, in
 // poisson the operand is log(max(S, kSlogFloor)) and A_r = sum_p m S
 void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, float* S32, double* A, hipStream_t st, bool poisson = false);
+bool expand_pack_fits(int S, int n_mp);
+void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, int R_pad, const float* x,
+                            const float* y, const float* z, float* slices, const int* pix, int n_mp, _Float16* S16,
+                            _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st);
 void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, _Float16* S16, _Float16* S16lo, float* scale, double* A,
                             hipStream_t st, bool poisson = false);
@@ -309,8 +326,8 @@ void launch_estep_csr(const int* off, const int* col, const unsigned short* cnt,
 // rotation's nonzero P (patterns ascending), then S' from them
 void launch_p_row_fill(const float* P, int R, int N, const float* opscale, const int* off, int* col, float* val,
                        hipStream_t st);
-void launch_mstep_sparse(const int* off, const int* col, const float* val, const _Float16* K, const _Float16* Klo,
-                         const MstepArgs& a, hipStream_t st);
+bool launch_mstep_sparse(const std::vector<int>& h_off, const int* off, const int* col, const float* val,
+                         const _Float16* K, const _Float16* Klo, const MstepArgs& a, hipStream_t st);
 void launch_images_from_csr(const int* off, const int* pix, const unsigned short* cnt, int N, int N2d, float* images,
                             hipStream_t st);
 

@@ -276,6 +276,108 @@ __global__ __launch_bounds__(256) void k_pack_slices_f16(const float* __restrict
   }
 }
 
+// Expand and FP16 packing in one pass: one workgroup per rotation computes
+// its slice (written to slices, the caller's output, and kept in LDS), then
+// packs the masked pixels from LDS: operand (log for Poisson) once into
+// registers, the row maximum, the hi/lo pair and A_r.  Needs N2d floats of
+// LDS and n_mp <= 256 * kPackPer.
+constexpr int kPackPer = 64;
+
+template <bool TRI>
+__global__ __launch_bounds__(256) void k_expand_pack_f16(const float* __restrict__ model, int S,
+                                                         const float* __restrict__ rot,
+                                                         const float* __restrict__ x, const float* __restrict__ y,
+                                                         const float* __restrict__ z, float* __restrict__ slices,
+                                                         const int* __restrict__ pix, int R, int n_mp,
+                                                         _Float16* __restrict__ S16, _Float16* __restrict__ S16lo,
+                                                         float* __restrict__ scale, double* __restrict__ A,
+                                                         int poisson) {
+  extern __shared__ float sv[];
+  const int r = blockIdx.x;
+  const bool live = r < R;
+  const int N2d = S * S;
+  if (live) {
+    float e[9];
+    rot_matrix(&rot[4 * (size_t)r], e);
+    const float half = (float)S / 2.0f + 0.5f;
+    float* out = slices + (size_t)r * N2d;
+    for (int t = threadIdx.x; t < N2d; t += blockDim.x) {
+      const int p = expand_pixel(t, S);
+      float ux, uy, uz;
+      voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+      float v = 0.0f;
+      if (TRI) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          int vi;
+          float w;
+          if (tri_corner(c, ux, uy, uz, S, vi, w)) v += w * model[vi];
+        }
+      } else {
+        const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
+        if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S) v = model[((size_t)vz * S + vy) * S + vx];
+      }
+      out[p] = v;
+      sv[p] = v;
+    }
+  }
+  __syncthreads();
+  float o[kPackPer];
+  float mx = 0.0f;
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < kPackPer; q++) {
+    const int k = threadIdx.x + q * 256;
+    o[q] = 0.0f;
+    if (live && k < n_mp) {
+      const int p = pix[k];
+      if (p >= 0) {
+        const float v = sv[p];
+        o[q] = slice_operand(v, poisson);
+        mx = fmaxf(mx, fabsf(o[q]));
+        acc += poisson ? (double)v : (double)v * v;
+      }
+    }
+  }
+  mx = block_max_f32(mx);
+  float sc = 1.0f;
+  if (mx > 0.0f && mx < 3.0e38f) {
+    int ex;
+    frexpf(mx, &ex);  // mx < 2^ex
+    sc = ldexpf(1.0f, max(-120, min(120, 14 - ex)));
+  }
+#pragma unroll
+  for (int q = 0; q < kPackPer; q++) {
+    const int k = threadIdx.x + q * 256;
+    if (k < n_mp) {
+      const float sx = o[q] * sc;
+      const _Float16 hi = (_Float16)sx;
+      S16[(size_t)r * n_mp + k] = hi;
+      S16lo[(size_t)r * n_mp + k] = (_Float16)(sx - (float)hi);
+    }
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0 && live) {
+    A[r] = acc;
+    scale[r] = 1.0f / sc;  // exact: sc is a power of two
+  }
+}
+
+bool expand_pack_fits(int S, int n_mp) { return (size_t)S * S * 4 <= 65536 && n_mp <= 256 * kPackPer; }
+
+void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, int R_pad, const float* x,
+                            const float* y, const float* z, float* slices, const int* pix, int n_mp, _Float16* S16,
+                            _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st) {
+  const size_t lds = (size_t)S * S * sizeof(float);
+  if (interp == INTERP_TRILINEAR)
+    hipLaunchKernelGGL(k_expand_pack_f16<true>, dim3(R_pad), dim3(256), lds, st, model, S, rot, x, y, z, slices, pix,
+                       R, n_mp, S16, S16lo, scale, A, (int)poisson);
+  else
+    hipLaunchKernelGGL(k_expand_pack_f16<false>, dim3(R_pad), dim3(256), lds, st, model, S, rot, x, y, z, slices,
+                       pix, R, n_mp, S16, S16lo, scale, A, (int)poisson);
+  EMC_LAUNCHED();
+}
+
 void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, float* S32, double* A, hipStream_t st, bool poisson) {
   hipLaunchKernelGGL(k_pack_slices_f32, dim3(R_pad), dim3(256), 0, st, slices, pix, R, N2d, n_mp, S32, A,

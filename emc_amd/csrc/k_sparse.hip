@@ -167,20 +167,17 @@ __global__ __launch_bounds__(256) void k_p_row_fill(const float* __restrict__ P,
 
 // One workgroup = one rotation x 2,048 compact pixels (8 per thread, one
 // 16-byte load of FP16 counts per nonzero); the epilogue of mstep_store.
+// Rotations with more than kSpSeg nonzeros are split into segments of at
+// most kSpSeg (partial sums, then a fixed-order finish), so no workgroup
+// walks a long list alone.
 constexpr int kSpPix = 8;
+constexpr int kSpSeg = 128;
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 
-__global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ off, const int* __restrict__ col,
-                                                      const float* __restrict__ val, const _Float16* __restrict__ K,
-                                                      const _Float16* __restrict__ Klo, MstepArgs a) {
-  typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-  const int r = blockIdx.x;
-  const int k0 = (blockIdx.y * 256 + threadIdx.x) * kSpPix;
-  if (k0 >= a.n_mp) return;
-  float acc[kSpPix];
-#pragma unroll
-  for (int t = 0; t < kSpPix; t++) acc[t] = 0.0f;
-  const int j1 = off[r + 1];
-  int j = off[r];
+__device__ __forceinline__ void sparse_accumulate(const int* __restrict__ col, const float* __restrict__ val, int j,
+                                                  int j1, const _Float16* __restrict__ K,
+                                                  const _Float16* __restrict__ Klo, int n_mp, int k0,
+                                                  float (&acc)[kSpPix]) {
   // four nonzeros per step: their count rows are loaded together (the loop
   // is latency-bound otherwise); accumulation stays in list order
   constexpr int U = 4;
@@ -189,7 +186,7 @@ __global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ of
     float w[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const size_t base = (size_t)col[j + u] * a.n_mp + k0;
+      const size_t base = (size_t)col[j + u] * n_mp + k0;
       w[u] = val[j + u];
       h[u] = *reinterpret_cast<const half8*>(K + base);
       if (Klo) l[u] = *reinterpret_cast<const half8*>(Klo + base);
@@ -200,7 +197,7 @@ __global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ of
       for (int t = 0; t < kSpPix; t++) acc[t] += w[u] * (Klo ? (float)h[u][t] + (float)l[u][t] : (float)h[u][t]);
   }
   for (; j < j1; j++) {
-    const size_t base = (size_t)col[j] * a.n_mp + k0;
+    const size_t base = (size_t)col[j] * n_mp + k0;
     const float w = val[j];
     const half8 h = *reinterpret_cast<const half8*>(K + base);
     if (Klo) {
@@ -212,6 +209,9 @@ __global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ of
       for (int t = 0; t < kSpPix; t++) acc[t] += w * (float)h[t];
     }
   }
+}
+
+__device__ __forceinline__ void sparse_store(const MstepArgs& a, int r, int k0, const float (&acc)[kSpPix]) {
   const float T = a.Tf[r];
 #pragma unroll
   for (int t = 0; t < kSpPix; t++) {
@@ -224,17 +224,96 @@ __global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ of
   }
 }
 
+// rotations with at most kSpSeg nonzeros
+__global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ off, const int* __restrict__ col,
+                                                      const float* __restrict__ val, const _Float16* __restrict__ K,
+                                                      const _Float16* __restrict__ Klo, MstepArgs a) {
+  const int r = blockIdx.x;
+  const int k0 = (blockIdx.y * 256 + threadIdx.x) * kSpPix;
+  const int j0 = off[r], j1 = off[r + 1];
+  if (k0 >= a.n_mp || j1 - j0 > kSpSeg) return;
+  float acc[kSpPix];
+#pragma unroll
+  for (int t = 0; t < kSpPix; t++) acc[t] = 0.0f;
+  sparse_accumulate(col, val, j0, j1, K, Klo, a.n_mp, k0, acc);
+  sparse_store(a, r, k0, acc);
+}
+
+// one segment (nonzeros seg.x .. seg.y - 1 of a long row) -> part[s][n_mp]
+__global__ __launch_bounds__(256) void k_mstep_sparse_part(const int2* __restrict__ seg, const int* __restrict__ col,
+                                                           const float* __restrict__ val,
+                                                           const _Float16* __restrict__ K,
+                                                           const _Float16* __restrict__ Klo, int n_mp,
+                                                           float* __restrict__ part) {
+  const int sg = blockIdx.x;
+  const int k0 = (blockIdx.y * 256 + threadIdx.x) * kSpPix;
+  if (k0 >= n_mp) return;
+  float acc[kSpPix];
+#pragma unroll
+  for (int t = 0; t < kSpPix; t++) acc[t] = 0.0f;
+  const int2 s = seg[sg];
+  sparse_accumulate(col, val, s.x, s.y, K, Klo, n_mp, k0, acc);
+  float* out = part + (size_t)sg * n_mp + k0;
+#pragma unroll
+  for (int t = 0; t < kSpPix; t++) out[t] = acc[t];
+}
+
+// long rows: their segments' partials summed in segment order, then stored
+__global__ __launch_bounds__(256) void k_mstep_sparse_fin(const int* __restrict__ rows,
+                                                          const int* __restrict__ row_seg, const float* __restrict__ part,
+                                                          MstepArgs a) {
+  const int h = blockIdx.x;
+  const int k0 = (blockIdx.y * 256 + threadIdx.x) * kSpPix;
+  if (k0 >= a.n_mp) return;
+  float acc[kSpPix];
+#pragma unroll
+  for (int t = 0; t < kSpPix; t++) acc[t] = 0.0f;
+  for (int sg = row_seg[h]; sg < row_seg[h + 1]; sg++) {
+    const float* in = part + (size_t)sg * a.n_mp + k0;
+#pragma unroll
+    for (int t = 0; t < kSpPix; t++) acc[t] += in[t];
+  }
+  sparse_store(a, rows[h], k0, acc);
+}
+
 void launch_p_row_fill(const float* P, int R, int N, const float* opscale, const int* off, int* col, float* val,
                        hipStream_t st) {
   hipLaunchKernelGGL(k_p_row_fill, dim3((R + 3) / 4), dim3(256), 0, st, P, R, N, opscale, off, col, val);
   EMC_LAUNCHED();
 }
 
-void launch_mstep_sparse(const int* off, const int* col, const float* val, const _Float16* K, const _Float16* Klo,
-                         const MstepArgs& a, hipStream_t st) {
-  const dim3 grid(a.R, (a.n_mp + 256 * kSpPix - 1) / (256 * kSpPix));
-  hipLaunchKernelGGL(k_mstep_sparse, grid, dim3(256), 0, st, off, col, val, K, Klo, a);
+bool launch_mstep_sparse(const std::vector<int>& h_off, const int* off, const int* col, const float* val,
+                         const _Float16* K, const _Float16* Klo, const MstepArgs& a, hipStream_t st) {
+  Engine& e = engine();
+  const int chunks = (a.n_mp + 256 * kSpPix - 1) / (256 * kSpPix);
+  // long rows and their segments (host: the offsets are already here)
+  e.h_sp_seg.clear();
+  e.h_sp_rows.clear();
+  e.h_sp_rowseg.assign(1, 0);
+  for (int r = 0; r < a.R; r++) {
+    const int j0 = h_off[r], j1 = h_off[r + 1];
+    if (j1 - j0 <= kSpSeg) continue;
+    for (int j = j0; j < j1; j += kSpSeg) e.h_sp_seg.push_back(make_int2(j, std::min(j + kSpSeg, j1)));
+    e.h_sp_rows.push_back(r);
+    e.h_sp_rowseg.push_back((int)e.h_sp_seg.size());
+  }
+  hipLaunchKernelGGL(k_mstep_sparse, dim3(a.R, chunks), dim3(256), 0, st, off, col, val, K, Klo, a);
   EMC_LAUNCHED();
+  const int ns = (int)e.h_sp_seg.size(), nh = (int)e.h_sp_rows.size();
+  if (nh == 0) return !failed();
+  if (!e.d_sp_seg.ensure(sizeof(int2) * ns) || !e.d_sp_rows.ensure(sizeof(int) * nh) ||
+      !e.d_sp_rowseg.ensure(sizeof(int) * (nh + 1)) || !e.d_sp_part.ensure(sizeof(float) * (size_t)ns * a.n_mp))
+    return false;
+  EMC_CHECK(hipMemcpyAsync(e.d_sp_seg.p, e.h_sp_seg.data(), sizeof(int2) * ns, hipMemcpyHostToDevice, st));
+  EMC_CHECK(hipMemcpyAsync(e.d_sp_rows.p, e.h_sp_rows.data(), sizeof(int) * nh, hipMemcpyHostToDevice, st));
+  EMC_CHECK(hipMemcpyAsync(e.d_sp_rowseg.p, e.h_sp_rowseg.data(), sizeof(int) * (nh + 1), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_mstep_sparse_part, dim3(ns, chunks), dim3(256), 0, st, e.d_sp_seg.as<int2>(), col, val, K,
+                     Klo, a.n_mp, e.d_sp_part.as<float>());
+  EMC_LAUNCHED();
+  hipLaunchKernelGGL(k_mstep_sparse_fin, dim3(nh, chunks), dim3(256), 0, st, e.d_sp_rows.as<int>(),
+                     e.d_sp_rowseg.as<int>(), e.d_sp_part.as<float>(), a);
+  EMC_LAUNCHED();
+  return !failed();
 }
 
 // Dense images from CSR: images[i][pix] = count (zero elsewhere, filled first).

@@ -31,7 +31,8 @@ void Proposal::release() {
                   &d_mcell_part, &d_part, &d_part_T, &d_part_Tphi, &d_part_max, &d_Lc, &d_panels, &d_crowscale, &d_plogp, &d_zero, &d_sub, &d_insub, &d_Ksub,
                   &d_Ksublo, &d_Ktsub, &d_Ktsublo, &d_Bsub, &d_phisub, &d_invphisub, &d_Lsub, &d_phiden_sub,
                   &d_Psub16, &d_Psub16lo, &d_mass, &d_tot, &d_iota, &d_part_c, &d_part_m, &d_sh_cells, &d_sh_off, &d_sh_rot,
-                  &d_Ssh, &d_Sshlo, &d_Ash, &d_sscale_sh, &d_Lsh, &d_top_s, &d_topm_s};
+                  &d_Ssh, &d_Sshlo, &d_Ash, &d_sscale_sh, &d_Lsh, &d_top_s, &d_topm_s, &d_net_cand,
+                  &d_nbr, &d_mark, &d_shared, &d_count, &d_ncand, &d_moved, &d_keys_s, &d_vals_s, &d_sort_tmp};
   for (DBuf* b : bufs) b->release();
   cnn.release();
   configured = predicted = have_p = false;
@@ -49,32 +50,36 @@ static bool upload(DBuf& d, const std::vector<T>& h, hipStream_t st) {
 static void sync_stream() { EMC_CHECK(hipStreamSynchronize(engine().stream)); }
 
 // Entries of each cell (patterns ascending), E-step work items, 32-entry
-// chunks of the M-step, candidate rotation count of each pattern.
+// chunks of the M-step, candidate rotation count of each pattern, from the
+// candidate cells in d_cand: per-cell counts on the GPU, the entry lists by
+// a stable sort on the GPU, the small per-cell arrays on the host.
 static bool build_lists(Proposal& p) {
   Engine& e = engine();
+  hipStream_t st = e.stream;
   const int N = p.N, K = p.topk, C = p.C;
-  for (size_t t = 0; t < p.h_cand.size(); t++)
-    if (p.h_cand[t] < 0 || p.h_cand[t] >= C) {
-      set_error("proposal: pattern %zu has no valid candidate cell at rank %zu (network output not finite?)",
-                t / K, t % K);
-      return false;
-    }
+  const long n = (long)N * K;
+  if (!p.d_count.ensure(sizeof(int) * (C + 1)) || !p.d_ncand.ensure(sizeof(int) * N) ||
+      !p.d_ent_pat.ensure(sizeof(int) * std::max(n, 1L)) || !p.d_ent_j.ensure(sizeof(int) * std::max(n, 1L)))
+    return false;
+  launch_cand_hist(p.d_cand.as<int>(), N, K, C, p.d_cell_off.as<int>(), p.d_count.as<int>(), p.d_ncand.as<int>(), st);
+  p.h_count.resize(C + 1);
+  p.h_ncand.resize(N);
+  EMC_CHECK(hipMemcpyAsync(p.h_count.data(), p.d_count.p, sizeof(int) * (C + 1), hipMemcpyDeviceToHost, st));
+  EMC_CHECK(hipMemcpyAsync(p.h_ncand.data(), p.d_ncand.p, sizeof(int) * N, hipMemcpyDeviceToHost, st));
+  sync_stream();
+  if (failed()) return false;
+  if (p.h_count[C] > 0) {
+    set_error("proposal: %d candidate entries name no valid cell (network output not finite?)", p.h_count[C]);
+    return false;
+  }
   p.h_ent_off.assign(C + 1, 0);
-  for (size_t t = 0; t < p.h_cand.size(); t++) p.h_ent_off[p.h_cand[t] + 1]++;
-  for (int c = 0; c < C; c++) p.h_ent_off[c + 1] += p.h_ent_off[c];
+  for (int c = 0; c < C; c++) p.h_ent_off[c + 1] = p.h_ent_off[c] + p.h_count[c];
   p.n_entries = p.h_ent_off[C];
-  std::vector<int> fill(p.h_ent_off.begin(), p.h_ent_off.end() - 1), pat(p.n_entries), jj(p.n_entries);
-  p.h_ncand.assign(N, 0);
-  for (int i = 0; i < N; i++)
-    for (int j = 0; j < K; j++) {
-      const int c = p.h_cand[(size_t)i * K + j];
-      const int at = fill[c]++;
-      pat[at] = i;
-      jj[at] = j;
-      p.h_ncand[i] += p.h_cell_off[c + 1] - p.h_cell_off[c];
-    }
   p.st_cand_pairs = 0;
   for (int i = 0; i < N; i++) p.st_cand_pairs += p.h_ncand[i];
+  if (!sort_entries(p.d_cand.as<int>(), N, K, C, p.d_keys_s, p.d_vals_s, p.d_sort_tmp, p.d_ent_pat.as<int>(),
+                    p.d_ent_j.as<int>(), st))
+    return false;
   std::vector<int2> work;
   p.h_chunk_off.assign(C + 1, 0);
   for (int c = 0; c < C; c++) {
@@ -107,13 +112,11 @@ static bool build_lists(Proposal& p) {
   p.n_seg = (int)seg_cell.size();
   p.n_mcell = (int)mcell.size();
   p.n_parts = parts;
-  if (!upload(p.d_ent_off, p.h_ent_off, e.stream) || !upload(p.d_ent_pat, pat, e.stream) ||
-      !upload(p.d_ent_j, jj, e.stream) || !upload(p.d_work, work, e.stream) ||
-      !upload(p.d_chunk_off, p.h_chunk_off, e.stream) || !upload(p.d_cand, p.h_cand, e.stream) ||
-      !upload(p.d_chunk_cell, chunk_cell, e.stream) || !upload(p.d_seg_cell, seg_cell, e.stream) ||
-      !upload(p.d_seg_chunk0, seg_chunk0, e.stream) || !upload(p.d_seg_nch, seg_nch, e.stream) ||
-      !upload(p.d_seg_part, seg_part, e.stream) || !upload(p.d_mcell, mcell, e.stream) ||
-      !upload(p.d_mcell_part, mcell_part, e.stream))
+  if (!upload(p.d_ent_off, p.h_ent_off, st) || !upload(p.d_work, work, st) ||
+      !upload(p.d_chunk_off, p.h_chunk_off, st) || !upload(p.d_chunk_cell, chunk_cell, st) ||
+      !upload(p.d_seg_cell, seg_cell, st) || !upload(p.d_seg_chunk0, seg_chunk0, st) ||
+      !upload(p.d_seg_nch, seg_nch, st) || !upload(p.d_seg_part, seg_part, st) || !upload(p.d_mcell, mcell, st) ||
+      !upload(p.d_mcell_part, mcell_part, st))
     return false;
   const size_t nq = (size_t)std::max<long>(p.n_chunks, 1) * kSlot;
   if (!p.d_part_T.ensure(8 * nq) || !p.d_part_Tphi.ensure(8 * nq) || !p.d_part_max.ensure(4 * nq)) return false;
@@ -186,29 +189,24 @@ static bool recentre(Proposal& p) {
     p.clear_shared();
     return true;
   }
-  std::vector<int> top((size_t)N * K);
-  std::vector<float> topm((size_t)N * K);
-  EMC_CHECK(hipMemcpyAsync(top.data(), p.d_top.p, sizeof(int) * top.size(), hipMemcpyDeviceToHost, st));
-  EMC_CHECK(hipMemcpyAsync(topm.data(), p.d_topm.p, sizeof(float) * topm.size(), hipMemcpyDeviceToHost, st));
-  sync_stream();
-  if (failed()) return false;
-  // shared cells of the next iteration
+  // shared cells of the next iteration, from the support marks
   p.clear_shared();
-  p.h_shared.assign(p.C, 0);
+  const size_t cb = ((size_t)p.C + 3) / 4 * 4;
+  if (!p.d_mark.ensure(cb) || !p.d_shared.ensure(cb) || !p.d_moved.ensure(sizeof(int))) return false;
+  p.h_shared.assign(cb, 0);
   if (p.max_shared > 0) {
+    launch_mark_support(p.d_top.as<int>(), p.d_topm.as<float>(), N, K, Proposal::kShareMass, p.C,
+                        p.d_mark.as<unsigned char>(), st);
+    p.h_mark.resize(cb);
+    EMC_CHECK(hipMemcpyAsync(p.h_mark.data(), p.d_mark.p, cb, hipMemcpyDeviceToHost, st));
+    sync_stream();
+    if (failed()) return false;
     int n = 0;
-    for (size_t t = 0; t < top.size(); t++) {
-      const int c = top[t];
-      if (c >= 0 && c < p.C && !p.h_shared[c] && (t % K == 0 || topm[t] >= Proposal::kShareMass)) {
-        p.h_shared[c] = 1;
-        n++;
-      }
-    }
-    if (n > p.max_shared || n >= p.C) {
-      p.h_shared.assign(p.C, 0);
-    } else {
+    for (int c = 0; c < p.C; c++) n += p.h_mark[c] != 0;
+    if (n <= p.max_shared && n < p.C) {
       for (int c = 0; c < p.C; c++)
-        if (p.h_shared[c]) {
+        if (p.h_mark[c]) {
+          p.h_shared[c] = 1;
           p.h_sh_cells.push_back(c);
           for (int j = p.h_cell_off[c]; j < p.h_cell_off[c + 1]; j++) p.h_sh_rot.push_back(p.h_cell_rot[j]);
           p.h_sh_off.push_back((int)p.h_sh_rot.size());
@@ -221,42 +219,16 @@ static bool recentre(Proposal& p) {
         return false;
     }
   }
-  std::vector<int> stamp(p.C, -1), next;
-  next.reserve(p.topk);
-  double moved = 0;
-  for (int i = 0; i < N; i++) {
-    next.clear();
-    auto add = [&](int c) {
-      if (c >= 0 && c < p.C && !p.h_shared[c] && stamp[c] != i && (int)next.size() < p.topk) {
-        stamp[c] = i;
-        next.push_back(c);
-      }
-    };
-    const int* ti = &top[(size_t)i * K];
-    const float* mi = &topm[(size_t)i * K];
-    int used = 0;
-    for (int t = 0; t < K; t++)
-      if (ti[t] >= 0 && ti[t] < p.C && (t == 0 || mi[t] > 1e-6f * mi[0])) {
-        add(ti[t]);
-        used = t + 1;
-      }
-    for (int t = 0; t < used; t++)
-      if (ti[t] >= 0 && ti[t] < p.C)
-        for (int g = 0; g < Proposal::kNbr; g++) add(p.h_nbr[(size_t)ti[t] * Proposal::kNbr + g]);
-    for (int j = 0; j < p.topk; j++) add(p.h_net_cand[(size_t)i * p.topk + j]);
-    // the shared cells may leave fewer than topk: the nearest cells of the
-    // cells taken so far, then the lowest-numbered free cells
-    for (size_t t = 0; t < next.size() && (int)next.size() < p.topk; t++)
-      for (int g = 0; g < Proposal::kNbr; g++) add(p.h_nbr[(size_t)next[t] * Proposal::kNbr + g]);
-    for (int c = 0; c < p.C && (int)next.size() < p.topk; c++) add(c);
-    int* ci = &p.h_cand[(size_t)i * p.topk];
-    bool same = true;
-    for (int j = 0; j < p.topk; j++) same = same && ci[j] == next[j];
-    if (!same) {
-      std::copy(next.begin(), next.end(), ci);
-      moved++;
-    }
-  }
+  if (!upload(p.d_shared, p.h_shared, st)) return false;
+  // every pattern's candidates, re-centred (GPU), then the lists if any changed
+  launch_fill(p.d_moved.as<float>(), 1, 0.0f, st);
+  launch_recentre_cand(p.d_top.as<int>(), p.d_topm.as<float>(), N, K, p.d_nbr.as<int>(), Proposal::kNbr,
+                       p.d_net_cand.as<int>(), p.d_shared.as<unsigned char>(), p.C, p.topk, p.d_cand.as<int>(),
+                       p.d_moved.as<int>(), st);
+  int moved = 0;
+  EMC_CHECK(hipMemcpyAsync(&moved, p.d_moved.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  sync_stream();
+  if (failed()) return false;
   p.st_moved = moved;
   return moved == 0 || build_lists(p);
 }
@@ -267,7 +239,9 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
   hipStream_t st = e.stream;
   *dense = false;
   const auto t0 = std::chrono::steady_clock::now();
-  if (p.track && p.post != Proposal::POST_NONE) {
+  // (re-centring builds lists of at most kMaxTopk cells per pattern; longer
+  // caller-given lists stay as given)
+  if (p.track && p.post != Proposal::POST_NONE && p.topk <= kMaxTopk) {
     if (!recentre(p)) return false;
   } else {
     p.clear_shared();
@@ -838,7 +812,9 @@ int emc_hip_proposal_init(const real* q_fine, int n_rot, int n_coarse, int side,
   p.iteration = 0;
   p.h_cell_off = off;
   p.h_cell_rot = rot;
-  if (!upload(p.d_cell_off, off, e.stream) || !upload(p.d_cell_rot, rot, e.stream)) return 1;
+  if (!upload(p.d_cell_off, off, e.stream) || !upload(p.d_cell_rot, rot, e.stream) ||
+      !upload(p.d_nbr, p.h_nbr, e.stream))
+    return 1;
   sync_stream();
   if (!p.cnn.init(side, C, batch, seed)) return 1;
   p.configured = true;
@@ -925,15 +901,14 @@ int emc_hip_proposal_predict(const real* d_images, const int* d_mask, int N_2d, 
          p.d_pmass.as<float>() + b0, st);
   }
   p.N = N;
-  p.h_cand.resize((size_t)N * p.topk);
   p.h_pmass.resize(N);
-  EMC_CHECK(hipMemcpyAsync(p.h_cand.data(), p.d_cand.p, sizeof(int) * p.h_cand.size(), hipMemcpyDeviceToHost, st));
+  if (!p.d_net_cand.ensure(sizeof(int) * (size_t)N * p.topk)) return 1;
+  EMC_CHECK(hipMemcpyAsync(p.d_net_cand.p, p.d_cand.p, sizeof(int) * (size_t)N * p.topk, hipMemcpyDeviceToDevice, st));
   EMC_CHECK(hipMemcpyAsync(p.h_pmass.data(), p.d_pmass.p, sizeof(float) * N, hipMemcpyDeviceToHost, st));
   sync_stream();
   idx.release();
   if (failed()) return 1;
   reset_modes(p);
-  p.h_net_cand = p.h_cand;
   if (p.post == Proposal::POST_CAND) p.post = Proposal::POST_NONE;
   if (!build_lists(p)) return 1;
   p.predicted = true;
@@ -963,10 +938,10 @@ int emc_hip_proposal_set_candidates(const int* cand, int N_images, int topk) {
   }
   p.N = N_images;
   p.topk = topk;
-  p.h_cand.assign(cand, cand + (size_t)N_images * topk);
+  const std::vector<int> h(cand, cand + (size_t)N_images * topk);
+  if (!upload(p.d_cand, h, engine().stream) || !upload(p.d_net_cand, h, engine().stream)) return 1;
   p.h_pmass.assign(N_images, 1.0f);
   reset_modes(p);
-  p.h_net_cand = p.h_cand;
   if (p.post == Proposal::POST_CAND) p.post = Proposal::POST_NONE;
   if (!build_lists(p)) return 1;
   p.predicted = true;
@@ -977,7 +952,12 @@ int emc_hip_proposal_set_candidates(const int* cand, int N_images, int topk) {
 int emc_hip_proposal_get(int* cand, int* mode, float* mass, int N_images) {
   const Proposal& p = proposal();
   if (!p.predicted || N_images != p.N) return 1;
-  if (cand) std::copy(p.h_cand.begin(), p.h_cand.end(), cand);
+  if (cand) {
+    EMC_CHECK(hipMemcpyAsync(cand, p.d_cand.p, sizeof(int) * (size_t)p.N * p.topk, hipMemcpyDeviceToHost,
+                             engine().stream));
+    sync_stream();
+    if (failed()) return 1;
+  }
   if (mode)
     for (int i = 0; i < p.N; i++) mode[i] = p.h_mode[i];
   if (mass) std::copy(p.h_pmass.begin(), p.h_pmass.end(), mass);

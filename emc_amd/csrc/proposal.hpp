@@ -34,7 +34,13 @@ struct Proposal {
   std::vector<int> h_cell_off, h_cell_rot, h_nbr;
   static constexpr int kNbr = 12;
   static constexpr int kTop = 4;     // posterior cells per pattern used to re-centre
-  std::vector<int> h_net_cand;       // the network's candidates [N][topk]
+  DBuf d_net_cand;                   // the network's candidates [N][topk]
+  DBuf d_nbr;                        // [C][kNbr] nearest cells (-1: none)
+  DBuf d_mark, d_shared;             // [C] support marks, shared cells (bytes)
+  DBuf d_count, d_ncand, d_moved;    // entries per cell [C+1], rotations per pattern [N], changed lists
+  DBuf d_keys_s, d_vals_s, d_sort_tmp;  // cell-sorted entries and the sort's scratch
+  std::vector<int> h_count;
+  std::vector<unsigned char> h_mark;
   DBuf d_top, d_topm;                // [N][kTop] cells, masses of the last posterior
   // where the last posterior is: none, Lc / Lsub (candidate iteration), or
   // already reduced into d_top / d_topm (whole-table iteration of post_N patterns)
@@ -55,8 +61,9 @@ struct Proposal {
   DBuf d_iota;                       // 0..max(N, R_sh)-1
   DBuf d_part_c, d_part_m;           // top-cell partials of the subset columns
   DBuf d_cell_off, d_cell_rot;
-  // per pattern: candidate cells [N][topk], network top-k mass, search mode
-  std::vector<int> h_cand, h_ncand;
+  // per pattern: candidate rotation count, network top-k mass, search mode
+  // (the candidate cells [N][topk] live in d_cand)
+  std::vector<int> h_ncand;
   std::vector<float> h_pmass;
   std::vector<unsigned char> h_mode;  // 1 = whole table
   DBuf d_cand, d_pmass;

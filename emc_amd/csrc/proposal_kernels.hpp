@@ -10,6 +10,7 @@ namespace emc {
 constexpr int kSlot = 32;       // rotation slots per coarse cell (largest cell must fit)
 constexpr int kCandRows = 128;  // entries per E-step workgroup
 constexpr int kSegChunks = 24;  // 32-entry chunks per M-step segment
+constexpr int kMaxTopk = 64;    // candidate cells per pattern at most
 
 struct CandEstepArgs {
   const _Float16 *K, *Klo;          // [N_pad][n_mp] packed counts (lo only when split)
@@ -119,5 +120,18 @@ void launch_topcells(const float* Pc, int N, int topk, const unsigned char* insu
                      const float* Psub, int n_sub, const int* sub, int C, const int* cell_off, const int* cell_rot,
                      int ktop, int* top, float* topm, int* part_c, float* part_m, hipStream_t st);
 void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int N, float* respons, hipStream_t st);
+
+// candidate lists on the GPU (k_lists.hip)
+// mark: ceil(C / 4) * 4 bytes
+void launch_mark_support(const int* top, const float* topm, int N, int ktop, float share_mass, int C,
+                         unsigned char* mark, hipStream_t st);
+void launch_recentre_cand(const int* top, const float* topm, int N, int ktop, const int* nbr, int nnbr,
+                          const int* net, const unsigned char* shared, int C, int topk, int* cand, int* moved,
+                          hipStream_t st);
+// count: C + 1 ints (count[C] = entries naming no valid cell)
+void launch_cand_hist(const int* cand, int N, int topk, int C, const int* cell_off, int* count, int* ncand,
+                      hipStream_t st);
+bool sort_entries(const int* cand, int N, int topk, int C, DBuf& keys_sorted, DBuf& vals_sorted, DBuf& temp,
+                  int* ent_pat, int* ent_j, hipStream_t st);
 
 }  // namespace emc

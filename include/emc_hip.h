@@ -88,8 +88,8 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *   "proposal_track" "on" (default): before every proposal iteration each
  *                 pattern's candidates are re-centred on its last posterior
  *                 (the cells holding most of its mass and their nearest
- *                 cells, then the network's cells); "off": the network's
- *                 cells throughout.
+ *                 cells, then the network's cells; with topk <= 64);
+ *                 "off": the network's cells throughout.
  *   "proposal_shared" largest number of shared cells (default "512"; "0":
  *                 none).  With tracking, the cells holding some pattern's
  *                 best cell (or >= 1% of its posterior) in the last iteration
