@@ -160,8 +160,8 @@ struct Engine {
   // long rows of the list M-step: segments (first, end nonzero), rows,
   // their first segment, partial sums [segments][n_mp]
   std::vector<int2> h_sp_seg;
-  std::vector<int> h_sp_rows, h_sp_rowseg;
-  DBuf d_sp_seg, d_sp_rows, d_sp_rowseg, d_sp_part;
+  std::vector<int> h_sp_rows, h_sp_rowseg, h_sp_light, h_sp_zero;
+  DBuf d_sp_seg, d_sp_rows, d_sp_rowseg, d_sp_part, d_sp_light, d_sp_zero;
   DBuf d_Sc;         // float [R_pad][n_mp]: updated compact slices for insertion
   DBuf d_wts;        // float [R]: rotation weights (uploaded per call)
   DBuf d_wT;         // float [R]: weights[r] * T_r

@@ -224,19 +224,28 @@ __device__ __forceinline__ void sparse_store(const MstepArgs& a, int r, int k0, 
   }
 }
 
-// rotations with at most kSpSeg nonzeros
-__global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ off, const int* __restrict__ col,
-                                                      const float* __restrict__ val, const _Float16* __restrict__ K,
+// rotations with 1 .. kSpSeg nonzeros (rows[])
+__global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ rows, const int* __restrict__ off,
+                                                      const int* __restrict__ col, const float* __restrict__ val,
+                                                      const _Float16* __restrict__ K,
                                                       const _Float16* __restrict__ Klo, MstepArgs a) {
-  const int r = blockIdx.x;
+  const int r = rows[blockIdx.x];
   const int k0 = (blockIdx.y * 256 + threadIdx.x) * kSpPix;
   const int j0 = off[r], j1 = off[r + 1];
-  if (k0 >= a.n_mp || j1 - j0 > kSpSeg) return;
+  if (k0 >= a.n_mp) return;
   float acc[kSpPix];
 #pragma unroll
   for (int t = 0; t < kSpPix; t++) acc[t] = 0.0f;
   sparse_accumulate(col, val, j0, j1, K, Klo, a.n_mp, k0, acc);
   sparse_store(a, r, k0, acc);
+}
+
+// rotations without a nonzero responsibility: T_r = 0 and S' = 0 on the
+// masked pixels of the slice (the compact copy is not written: insertion
+// skips rotations with T_r <= 1e-10); lanes along pixels
+__global__ __launch_bounds__(256) void k_mstep_zero(const int* __restrict__ rows, MstepArgs a) {
+  float* out = a.slices + (size_t)rows[blockIdx.x] * a.N2d;
+  for (int k = threadIdx.x; k < a.n_m; k += 256) out[a.pix[k]] = 0.0f;
 }
 
 // one segment (nonzeros seg.x .. seg.y - 1 of a long row) -> part[s][n_mp]
@@ -286,19 +295,41 @@ bool launch_mstep_sparse(const std::vector<int>& h_off, const int* off, const in
                          const _Float16* K, const _Float16* Klo, const MstepArgs& a, hipStream_t st) {
   Engine& e = engine();
   const int chunks = (a.n_mp + 256 * kSpPix - 1) / (256 * kSpPix);
-  // long rows and their segments (host: the offsets are already here)
+  // rows by nonzero count (host: the offsets are already here): none,
+  // up to kSpSeg, more (split into segments)
   e.h_sp_seg.clear();
   e.h_sp_rows.clear();
   e.h_sp_rowseg.assign(1, 0);
+  e.h_sp_light.clear();
+  e.h_sp_zero.clear();
   for (int r = 0; r < a.R; r++) {
     const int j0 = h_off[r], j1 = h_off[r + 1];
-    if (j1 - j0 <= kSpSeg) continue;
+    if (j1 == j0) {
+      e.h_sp_zero.push_back(r);
+      continue;
+    }
+    if (j1 - j0 <= kSpSeg) {
+      e.h_sp_light.push_back(r);
+      continue;
+    }
     for (int j = j0; j < j1; j += kSpSeg) e.h_sp_seg.push_back(make_int2(j, std::min(j + kSpSeg, j1)));
     e.h_sp_rows.push_back(r);
     e.h_sp_rowseg.push_back((int)e.h_sp_seg.size());
   }
-  hipLaunchKernelGGL(k_mstep_sparse, dim3(a.R, chunks), dim3(256), 0, st, off, col, val, K, Klo, a);
-  EMC_LAUNCHED();
+  const int nl = (int)e.h_sp_light.size(), nz = (int)e.h_sp_zero.size();
+  if (!e.d_sp_light.ensure(sizeof(int) * std::max(nl, 1)) || !e.d_sp_zero.ensure(sizeof(int) * std::max(nz, 1)))
+    return false;
+  if (nl > 0) {
+    EMC_CHECK(hipMemcpyAsync(e.d_sp_light.p, e.h_sp_light.data(), sizeof(int) * nl, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_mstep_sparse, dim3(nl, chunks), dim3(256), 0, st, e.d_sp_light.as<int>(), off, col, val, K,
+                       Klo, a);
+    EMC_LAUNCHED();
+  }
+  if (nz > 0) {
+    EMC_CHECK(hipMemcpyAsync(e.d_sp_zero.p, e.h_sp_zero.data(), sizeof(int) * nz, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_mstep_zero, dim3(nz), dim3(256), 0, st, e.d_sp_zero.as<int>(), a);
+    EMC_LAUNCHED();
+  }
   const int ns = (int)e.h_sp_seg.size(), nh = (int)e.h_sp_rows.size();
   if (nh == 0) return !failed();
   if (!e.d_sp_seg.ensure(sizeof(int2) * ns) || !e.d_sp_rows.ensure(sizeof(int) * nh) ||
