@@ -18,6 +18,7 @@
 
 #include "engine.hpp"
 #include "proposal.hpp"
+#include "proposal_kernels.hpp"
 
 namespace emc {
 
@@ -627,10 +628,19 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   bool sparse_m = false;
   e.p_nnz = -1;
   {
-    // T_r, and for the dense FP16 form the row scales and the hi/lo operand
     PhaseTimer pt(PH_ROWSUM);
     if (try_sparse && (!e.d_pnnz.ensure(sizeof(int) * R) || !e.d_poff.ensure(sizeof(int) * (R + 1)))) return 0.0f;
-    if (try_sparse || !cand_ok)
+    // the nonzero count of every rotation: in a proposal iteration from the
+    // candidate slots, shared rows and subset columns, else from a scan of
+    // d_respons (with T_r and the row scales of the dense FP16 form)
+    const bool from_sources = prop && try_sparse && (uint64_t)R * (uint64_t)N < (1ull << 32);
+    NzSources src{};
+    if (from_sources) {
+      if (!proposal_nz_sources(src, N, R)) return 0.0f;
+      launch_nz_count(src, e.d_pnnz.as<int>(), e.stream);
+    }
+    const bool scan = !from_sources && (try_sparse || !cand_ok);
+    if (scan)
       launch_rowsum(d_respons, R, N, opscale, e.d_T.as<double>(), e.d_Tf.as<float>(), e.d_Pscale.as<float>(), fp16,
                     e.stream, poisson ? d_scaling : nullptr, poisson ? e.d_Tphi.as<float>() : nullptr,
                     try_sparse ? e.d_pnnz.as<int>() : nullptr);
@@ -655,9 +665,22 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
           return 0.0f;
         EMC_CHECK(hipMemcpyAsync(e.d_poff.p, e.h_poff.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice,
                                  e.stream));
-        launch_p_row_fill(d_respons, R, N, opscale, e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(),
-                          e.stream);
+        if (from_sources) {
+          // lists by a sort of the nonzeros; T_r (and Tphi_r) from them
+          if (!build_nz_lists(src, total, e.d_poff.as<int>(), opscale, poisson ? d_scaling : nullptr, e.d_nz_keys,
+                              e.d_nz_keys_s, e.d_nz_vals, e.d_nz_vals_s, e.d_nz_tail, e.d_nz_tmp,
+                              e.d_pcol.as<int>(), e.d_pval.as<float>(), e.d_T.as<double>(), e.d_Tf.as<float>(),
+                              poisson ? e.d_Tphi.as<float>() : nullptr, e.stream))
+            return 0.0f;
+        } else {
+          launch_p_row_fill(d_respons, R, N, opscale, e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(),
+                            e.stream);
+        }
         e.p_nnz = total;
+      } else if (from_sources && !cand_ok) {
+        // dense M-step after all: T_r and the row scales from d_respons
+        launch_rowsum(d_respons, R, N, opscale, e.d_T.as<double>(), e.d_Tf.as<float>(), e.d_Pscale.as<float>(), fp16,
+                      e.stream, poisson ? d_scaling : nullptr, poisson ? e.d_Tphi.as<float>() : nullptr, nullptr);
       }
     }
     if (fp16 && !sparse_m && !cand_ok) {

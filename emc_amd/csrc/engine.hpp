@@ -157,6 +157,7 @@ struct Engine {
   DBuf d_pcol, d_pval;  // int / float [nnz]: pattern, P * operand scale (patterns ascending)
   long p_nnz = -1;      // nonzeros of the last M-step (-1: dense form)
   std::vector<int> h_pnnz, h_poff;
+  DBuf d_nz_keys, d_nz_keys_s, d_nz_vals, d_nz_vals_s, d_nz_tail, d_nz_tmp;  // nonzero lists from sources
   // long rows of the list M-step: segments (first, end nonzero), rows,
   // their first segment, partial sums [segments][n_mp]
   std::vector<int2> h_sp_seg;

@@ -23,6 +23,13 @@ __device__ __forceinline__ int expand_pixel(int t, int S) {
   return ((tile / tiles_x) * 8 + (l >> 3)) * S + (tile % tiles_x) * 8 + (l & 7);
 }
 
+// Slice packing runs 1,024 threads per workgroup (rotation): the fused
+// expand + pack keeps the slice in 64 KB of LDS, and two such workgroups
+// still fill a CU with waves for the model gathers; the separate pack uses
+// the same count so that both sum A_r in the same order.
+constexpr int kPackThreads = 1024;
+constexpr int kPackPer = 16;
+
 // One workgroup per rotation; model reads are gathers served by L2.
 __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict__ model, int S,
                                                         const float* __restrict__ rot,
@@ -233,8 +240,9 @@ __global__ __launch_bounds__(256) void k_pack_slices_f32(const float* __restrict
 }
 
 // Per-rotation power-of-two scale so that max |S| * scale <= 2^14, then the
-// FP16 pair hi = fp16(S*scale), lo = fp16(S*scale - hi).
-__global__ __launch_bounds__(256) void k_pack_slices_f16(const float* __restrict__ slices,
+// FP16 pair hi = fp16(S*scale), lo = fp16(S*scale - hi).  Launched with the
+// fused kernel's thread count: A_r is summed in the same order.
+__global__ __launch_bounds__(kPackThreads) void k_pack_slices_f16(const float* __restrict__ slices,
                                                          const int* __restrict__ pix, int R, int N2d,
                                                          int n_mp, _Float16* __restrict__ S16,
                                                          _Float16* __restrict__ S16lo,
@@ -280,11 +288,10 @@ __global__ __launch_bounds__(256) void k_pack_slices_f16(const float* __restrict
 // its slice (written to slices, the caller's output, and kept in LDS), then
 // packs the masked pixels from LDS: operand (log for Poisson) once into
 // registers, the row maximum, the hi/lo pair and A_r.  Needs N2d floats of
-// LDS and n_mp <= 256 * kPackPer.
-constexpr int kPackPer = 64;
+// LDS and n_mp <= kPackThreads * kPackPer.
 
 template <bool TRI>
-__global__ __launch_bounds__(256) void k_expand_pack_f16(const float* __restrict__ model, int S,
+__global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* __restrict__ model, int S,
                                                          const float* __restrict__ rot,
                                                          const float* __restrict__ x, const float* __restrict__ y,
                                                          const float* __restrict__ z, float* __restrict__ slices,
@@ -327,7 +334,7 @@ __global__ __launch_bounds__(256) void k_expand_pack_f16(const float* __restrict
   double acc = 0.0;
 #pragma unroll
   for (int q = 0; q < kPackPer; q++) {
-    const int k = threadIdx.x + q * 256;
+    const int k = threadIdx.x + q * kPackThreads;
     o[q] = 0.0f;
     if (live && k < n_mp) {
       const int p = pix[k];
@@ -348,7 +355,7 @@ __global__ __launch_bounds__(256) void k_expand_pack_f16(const float* __restrict
   }
 #pragma unroll
   for (int q = 0; q < kPackPer; q++) {
-    const int k = threadIdx.x + q * 256;
+    const int k = threadIdx.x + q * kPackThreads;
     if (k < n_mp) {
       const float sx = o[q] * sc;
       const _Float16 hi = (_Float16)sx;
@@ -363,17 +370,17 @@ __global__ __launch_bounds__(256) void k_expand_pack_f16(const float* __restrict
   }
 }
 
-bool expand_pack_fits(int S, int n_mp) { return (size_t)S * S * 4 <= 65536 && n_mp <= 256 * kPackPer; }
+bool expand_pack_fits(int S, int n_mp) { return (size_t)S * S * 4 <= 65536 && n_mp <= kPackThreads * kPackPer; }
 
 void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, int R_pad, const float* x,
                             const float* y, const float* z, float* slices, const int* pix, int n_mp, _Float16* S16,
                             _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st) {
   const size_t lds = (size_t)S * S * sizeof(float);
   if (interp == INTERP_TRILINEAR)
-    hipLaunchKernelGGL(k_expand_pack_f16<true>, dim3(R_pad), dim3(256), lds, st, model, S, rot, x, y, z, slices, pix,
+    hipLaunchKernelGGL(k_expand_pack_f16<true>, dim3(R_pad), dim3(kPackThreads), lds, st, model, S, rot, x, y, z, slices, pix,
                        R, n_mp, S16, S16lo, scale, A, (int)poisson);
   else
-    hipLaunchKernelGGL(k_expand_pack_f16<false>, dim3(R_pad), dim3(256), lds, st, model, S, rot, x, y, z, slices,
+    hipLaunchKernelGGL(k_expand_pack_f16<false>, dim3(R_pad), dim3(kPackThreads), lds, st, model, S, rot, x, y, z, slices,
                        pix, R, n_mp, S16, S16lo, scale, A, (int)poisson);
   EMC_LAUNCHED();
 }
@@ -388,7 +395,7 @@ void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d,
 void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, _Float16* S16, _Float16* S16lo, float* scale, double* A,
                             hipStream_t st, bool poisson) {
-  hipLaunchKernelGGL(k_pack_slices_f16, dim3(R_pad), dim3(256), 0, st, slices, pix, R, N2d, n_mp, S16,
+  hipLaunchKernelGGL(k_pack_slices_f16, dim3(R_pad), dim3(kPackThreads), 0, st, slices, pix, R, N2d, n_mp, S16,
                      S16lo, scale, A, (int)poisson);
   EMC_LAUNCHED();
 }

@@ -143,4 +143,133 @@ bool sort_entries(const int* cand, int N, int topk, int C, DBuf& keys_sorted, DB
   return !failed();
 }
 
+// ---------------------------------------------------------------------------
+// Nonzero responsibilities of a proposal iteration, per rotation, from where
+// they were computed (candidate slots, shared rows, whole-table subset)
+// instead of a scan of the dense [R][N] matrix.  Pass 1 counts per rotation;
+// pass 2 appends (key = r * N + i, P) in any order; a radix sort by key then
+// gives every rotation's list in pattern order, independent of that order.
+// ---------------------------------------------------------------------------
+// mode 0: count into rownnz; mode 1: append to keys / vals at *tail
+__device__ __forceinline__ void nz_take(int mode, float p, int r, int i, int N, int* rownnz, unsigned* tail,
+                                        unsigned* keys, float* vals) {
+  if (p == 0.0f) return;
+  if (mode == 0) {
+    atomicAdd(&rownnz[r], 1);
+  } else {
+    const unsigned at = atomicAdd(tail, 1u);
+    keys[at] = (unsigned)r * (unsigned)N + (unsigned)i;
+    vals[at] = p;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nz_cand(NzSources a, int mode, int* rownnz, unsigned* tail, unsigned* keys,
+                                                 float* vals) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;  // (i, j, slot)
+  if (t >= (long)a.N * a.topk * kSlot) return;
+  const int s = (int)(t % kSlot);
+  const long ij = t / kSlot;
+  const int i = (int)(ij / a.topk);
+  if (a.insub[i]) return;
+  const int c = a.cand[ij];
+  const int o = a.cell_off[c];
+  if (s >= a.cell_off[c + 1] - o) return;
+  nz_take(mode, a.Pc[t], a.cell_rot[o + s], i, a.N, rownnz, tail, keys, vals);
+}
+
+__global__ __launch_bounds__(256) void k_nz_shared(NzSources a, int mode, int* rownnz, unsigned* tail,
+                                                   unsigned* keys, float* vals) {
+  const int rs = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.N || a.insub[i]) return;
+  nz_take(mode, a.Psh[(size_t)rs * a.N + i], a.sh_rot[rs], i, a.N, rownnz, tail, keys, vals);
+}
+
+__global__ __launch_bounds__(256) void k_nz_sub(NzSources a, int mode, int* rownnz, unsigned* tail, unsigned* keys,
+                                                float* vals) {
+  const int r = blockIdx.x;
+  const int k = blockIdx.y * 256 + threadIdx.x;
+  if (k >= a.n_sub) return;
+  nz_take(mode, a.Psub[(size_t)r * a.n_sub + k], r, a.sub[k], a.N, rownnz, tail, keys, vals);
+}
+
+static void nz_pass(const NzSources& a, int mode, int* rownnz, unsigned* tail, unsigned* keys, float* vals,
+                    hipStream_t st) {
+  const long nc = (long)a.N * a.topk * kSlot;
+  hipLaunchKernelGGL(k_nz_cand, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, a, mode, rownnz, tail, keys,
+                     vals);
+  EMC_LAUNCHED();
+  if (a.R_sh > 0) {
+    hipLaunchKernelGGL(k_nz_shared, dim3((a.N + 255) / 256, a.R_sh), dim3(256), 0, st, a, mode, rownnz, tail, keys,
+                       vals);
+    EMC_LAUNCHED();
+  }
+  if (a.n_sub > 0) {
+    hipLaunchKernelGGL(k_nz_sub, dim3(a.R, (a.n_sub + 255) / 256), dim3(256), 0, st, a, mode, rownnz, tail, keys,
+                       vals);
+    EMC_LAUNCHED();
+  }
+}
+
+void launch_nz_count(const NzSources& a, int* rownnz, hipStream_t st) {
+  launch_fill(reinterpret_cast<float*>(rownnz), (size_t)a.R, 0.0f, st);
+  nz_pass(a, 0, rownnz, nullptr, nullptr, nullptr, st);
+}
+
+// Sorted lists: col[e] = pattern, val[e] = P * opscale[pattern], and per
+// rotation T_r = sum P (FP64), Tf, and with phi Tphi_r = sum P phi.
+__global__ __launch_bounds__(256) void k_nz_split(const unsigned* __restrict__ keys, const float* __restrict__ P,
+                                                  long n, int N, const float* __restrict__ opscale,
+                                                  int* __restrict__ col, float* __restrict__ val) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  const int i = (int)(keys[e] % (unsigned)N);
+  col[e] = i;
+  val[e] = P[e] * opscale[i];
+}
+
+__global__ __launch_bounds__(256) void k_nz_rowsums(const int* __restrict__ off, const int* __restrict__ col,
+                                                    const float* __restrict__ P, int R, const float* __restrict__ phi,
+                                                    double* __restrict__ T, float* __restrict__ Tf,
+                                                    float* __restrict__ Tphi) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
+  double t = 0.0, tp = 0.0;
+  for (int e = off[r]; e < off[r + 1]; e++) {
+    t += P[e];
+    if (phi) tp += (double)P[e] * (double)phi[col[e]];
+  }
+  T[r] = t;
+  Tf[r] = (float)t;
+  if (phi) Tphi[r] = (float)tp;
+}
+
+bool build_nz_lists(const NzSources& a, long nnz, const int* off, const float* opscale, const float* phi,
+                    DBuf& keys, DBuf& keys_sorted, DBuf& vals, DBuf& vals_sorted, DBuf& tail, DBuf& temp, int* col,
+                    float* val, double* T, float* Tf, float* Tphi, hipStream_t st) {
+  const long n = std::max(nnz, 1L);
+  if (!keys.ensure(4 * n) || !keys_sorted.ensure(4 * n) || !vals.ensure(4 * n) || !vals_sorted.ensure(4 * n) ||
+      !tail.ensure(4))
+    return false;
+  launch_fill(tail.as<float>(), 1, 0.0f, st);
+  nz_pass(a, 1, nullptr, tail.as<unsigned>(), keys.as<unsigned>(), vals.as<float>(), st);
+  int bits = 1;
+  while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)a.R * (uint64_t)a.N) bits++;
+  size_t tbytes = 0;
+  EMC_CHECK(rocprim::radix_sort_pairs(nullptr, tbytes, keys.as<unsigned>(), keys_sorted.as<unsigned>(),
+                                      vals.as<float>(), vals_sorted.as<float>(), (size_t)nnz, 0, bits, st));
+  if (!temp.ensure(std::max(tbytes, (size_t)1))) return false;
+  EMC_CHECK(rocprim::radix_sort_pairs(temp.p, tbytes, keys.as<unsigned>(), keys_sorted.as<unsigned>(),
+                                      vals.as<float>(), vals_sorted.as<float>(), (size_t)nnz, 0, bits, st));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_nz_split, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, keys_sorted.as<unsigned>(),
+                       vals_sorted.as<float>(), nnz, a.N, opscale, col, val);
+    EMC_LAUNCHED();
+  }
+  hipLaunchKernelGGL(k_nz_rowsums, dim3((a.R + 255) / 256), dim3(256), 0, st, off, col, vals_sorted.as<float>(), a.R,
+                     phi, T, Tf, Tphi);
+  EMC_LAUNCHED();
+  return !failed();
+}
+
 }  // namespace emc

@@ -325,6 +325,34 @@ void launch_shared_scatter(const float* Psh, int R_sh, int N, const int* sh_rot,
   EMC_LAUNCHED();
 }
 
+// The dense responsibilities before the candidates are scattered: zero
+// everywhere except the columns of the patterns scored on the whole table,
+// which take their subset posterior (colmap[i] = subset index or -1).  One
+// pass over [R][N], 16-byte stores when N % 4 == 0.
+__global__ __launch_bounds__(256) void k_fill_subset(float* __restrict__ respons, int N,
+                                                     const int* __restrict__ colmap, const float* __restrict__ Psub,
+                                                     int n_sub) {
+  const int r = blockIdx.x;
+  const float* ps = Psub + (size_t)r * n_sub;
+  float4* out = reinterpret_cast<float4*>(respons + (size_t)r * N);
+  const int4* cm = reinterpret_cast<const int4*>(colmap);
+  for (int q = threadIdx.x; q < N / 4; q += 256) {
+    const int4 c = cm[q];
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (c.x >= 0) v.x = ps[c.x];
+    if (c.y >= 0) v.y = ps[c.y];
+    if (c.z >= 0) v.z = ps[c.z];
+    if (c.w >= 0) v.w = ps[c.w];
+    out[q] = v;
+  }
+}
+
+void launch_fill_subset(float* respons, int R, int N, const int* colmap, const float* Psub, int n_sub,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_subset, dim3(R), dim3(256), 0, st, respons, N, colmap, Psub, n_sub);
+  EMC_LAUNCHED();
+}
+
 // respons[r][sub[k]] = Psub[r][k] for every rotation r (one block per r).
 __global__ __launch_bounds__(256) void k_sub_scatter(const float* __restrict__ Psub, int n_sub,
                                                      const int* __restrict__ sub, int N, float* __restrict__ respons) {

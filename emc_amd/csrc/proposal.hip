@@ -32,7 +32,7 @@ void Proposal::release() {
                   &d_Ksublo, &d_Ktsub, &d_Ktsublo, &d_Bsub, &d_phisub, &d_invphisub, &d_Lsub, &d_phiden_sub,
                   &d_Psub16, &d_Psub16lo, &d_mass, &d_tot, &d_iota, &d_part_c, &d_part_m, &d_sh_cells, &d_sh_off, &d_sh_rot,
                   &d_Ssh, &d_Sshlo, &d_Ash, &d_sscale_sh, &d_Lsh, &d_top_s, &d_topm_s, &d_net_cand,
-                  &d_nbr, &d_mark, &d_shared, &d_count, &d_ncand, &d_moved, &d_keys_s, &d_vals_s, &d_sort_tmp};
+                  &d_nbr, &d_mark, &d_shared, &d_count, &d_ncand, &d_moved, &d_keys_s, &d_vals_s, &d_sort_tmp, &d_colmap};
   for (DBuf* b : bufs) b->release();
   cnn.release();
   configured = predicted = have_p = false;
@@ -414,6 +414,12 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   Proposal& p = proposal();
   hipStream_t st = e.stream;
   if (!p.d_tot.ensure(sizeof(double) * 2)) return false;
+  const bool vec_fill = N % 4 == 0 && (uintptr_t)respons % 16 == 0 && p.n_sub > 0;
+  if (vec_fill) {
+    p.h_colmap.assign(N, -1);
+    for (int k = 0; k < p.n_sub; k++) p.h_colmap[p.h_sub[k]] = k;
+    if (!upload(p.d_colmap, p.h_colmap, st)) return false;
+  }
   const int chunks = p.R_sh > 0 ? softmax_chunks(N, p.R_sh) : 0;
   const int nmerge = chunks + 1, nb = (N + 255) / 256;
   if (p.R_sh > 0) {
@@ -425,7 +431,10 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   }
   {
     PhaseTimer pt(PH_SOFTMAX);
-    launch_fill(respons, (size_t)N * R, 0.0f, st);
+    // zero, with the whole-table subset's columns written in the same pass
+    // when the rows allow 16-byte stores
+    if (vec_fill) launch_fill_subset(respons, R, N, p.d_colmap.as<int>(), p.d_Lsub.as<float>(), p.n_sub, st);
+    else launch_fill(respons, (size_t)N * R, 0.0f, st);
     CandSoftmaxArgs a{};
     a.Lc = p.d_Lc.as<float>();
     a.cand = p.d_cand.as<int>();
@@ -469,7 +478,7 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
       EMC_CHECK(hipMemcpyAsync(p.d_tot.p, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToDevice, st));
     }
   }
-  if (p.n_sub > 0) {
+  if (p.n_sub > 0 && !vec_fill) {
     // the subset's posterior (computed with the E-step) into its columns
     PhaseTimer pt(PH_AUDIT);
     launch_sub_scatter(p.d_Lsub.as<float>(), R, p.n_sub, p.d_sub.as<int>(), N, respons, st);
@@ -490,6 +499,30 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
 }
 
 bool proposal_has_shared() { return proposal().R_sh > 0; }
+
+bool proposal_nz_sources(NzSources& a, int N, int R) {
+  const Proposal& p = proposal();
+  if (!p.have_p || p.N != N || p.R != R) {
+    set_error("proposal: no candidate posterior for %d patterns x %d rotations", N, R);
+    return false;
+  }
+  a = {};
+  a.Pc = p.d_Lc.as<float>();
+  a.cand = p.d_cand.as<int>();
+  a.cell_off = p.d_cell_off.as<int>();
+  a.cell_rot = p.d_cell_rot.as<int>();
+  a.insub = p.d_insub.as<unsigned char>();
+  a.N = N;
+  a.topk = p.topk;
+  a.Psh = p.d_Lsh.as<float>();
+  a.sh_rot = p.d_sh_rot.as<int>();
+  a.R_sh = p.R_sh;
+  a.Psub = p.d_Lsub.as<float>();
+  a.sub = p.d_sub.as<int>();
+  a.R = R;
+  a.n_sub = p.n_sub;
+  return true;
+}
 
 bool proposal_tracks(int N, int R) {
   const Proposal& p = proposal();

@@ -39,7 +39,8 @@ struct Proposal {
   DBuf d_mark, d_shared;             // [C] support marks, shared cells (bytes)
   DBuf d_count, d_ncand, d_moved;    // entries per cell [C+1], rotations per pattern [N], changed lists
   DBuf d_keys_s, d_vals_s, d_sort_tmp;  // cell-sorted entries and the sort's scratch
-  std::vector<int> h_count;
+  std::vector<int> h_count, h_colmap;  // entries per cell; subset index of each pattern (-1: none)
+  DBuf d_colmap;
   std::vector<unsigned char> h_mark;
   DBuf d_top, d_topm;                // [N][kTop] cells, masses of the last posterior
   // where the last posterior is: none, Lc / Lsub (candidate iteration), or
@@ -116,6 +117,9 @@ bool proposal_mstep(int N, int R, float* slices, int N2d, const float* scaling);
 // reduced to each pattern's top cells for re-centring, when tracking.
 bool proposal_tracks(int N, int R);
 bool proposal_has_shared();
+// where the last proposal softmax left each part of the posterior
+struct NzSources;
+bool proposal_nz_sources(NzSources& src, int N, int R);
 bool proposal_dense_posterior(const float* respons, int N, int R);
 
 }  // namespace emc

@@ -120,6 +120,9 @@ void launch_topcells(const float* Pc, int N, int topk, const unsigned char* insu
                      const float* Psub, int n_sub, const int* sub, int C, const int* cell_off, const int* cell_rot,
                      int ktop, int* top, float* topm, int* part_c, float* part_m, hipStream_t st);
 void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int N, float* respons, hipStream_t st);
+// respons = 0 except the subset columns (colmap[i] >= 0: Psub column); N % 4 == 0, 16-byte aligned rows
+void launch_fill_subset(float* respons, int R, int N, const int* colmap, const float* Psub, int n_sub,
+                        hipStream_t st);
 
 // candidate lists on the GPU (k_lists.hip)
 // mark: ceil(C / 4) * 4 bytes
@@ -133,5 +136,25 @@ void launch_cand_hist(const int* cand, int N, int topk, int C, const int* cell_o
                       hipStream_t st);
 bool sort_entries(const int* cand, int N, int topk, int C, DBuf& keys_sorted, DBuf& vals_sorted, DBuf& temp,
                   int* ent_pat, int* ent_j, hipStream_t st);
+
+// nonzero responsibilities per rotation from the candidate slots, shared
+// rows and subset columns (k_lists.hip); R * N < 2^32
+struct NzSources {
+  const float* Pc;                 // [N][topk][kSlot]
+  const int *cand, *cell_off, *cell_rot;
+  const unsigned char* insub;      // [N]
+  int N, topk;
+  const float* Psh;                // [R_sh][N]
+  const int* sh_rot;
+  int R_sh;
+  const float* Psub;               // [R][n_sub]
+  const int* sub;
+  int R, n_sub;
+};
+void launch_nz_count(const NzSources& a, int* rownnz, hipStream_t st);
+// off: exclusive prefix of the counts ([R+1], device); col / val: [nnz]
+bool build_nz_lists(const NzSources& a, long nnz, const int* off, const float* opscale, const float* phi,
+                    DBuf& keys, DBuf& keys_sorted, DBuf& vals, DBuf& vals_sorted, DBuf& tail, DBuf& temp, int* col,
+                    float* val, double* T, float* Tf, float* Tphi, hipStream_t st);
 
 }  // namespace emc
