@@ -66,11 +66,11 @@ def phase_times(L):
 
 
 def proposal_stats(L):
-    out = (C.c_double * 17)()
-    k = L.emc_hip_proposal_stats(out, 17)
+    out = (C.c_double * 18)()
+    k = L.emc_hip_proposal_stats(out, 18)
     keys = ["iteration", "full_search", "audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass",
             "pairs_scored", "gated_at_prediction", "train_loss", "train_ms", "predict_ms", "whole_table",
-            "candidate_pairs", "recentred", "whole_table_iteration", "shared_cells", "recentre_ms"]
+            "candidate_pairs", "recentred", "whole_table_iteration", "shared_cells", "recentre_ms", "group_pairs"]
     return {keys[j]: out[j] for j in range(k)}
 
 
@@ -196,7 +196,7 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
             f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f} (mean mass {st['audit_mean_mass']:.5f}), "
             f"full search {st['full_search']:.0f}, re-centred {st['recentred']:.0f}, "
             f"shared cells {st['shared_cells']:.0f}, pairs {st['pairs_scored']:.3g}, "
-            f"re-centring {st['recentre_ms']:.1f} ms host")
+            f"re-centring {st['recentre_ms']:.1f} ms host, group-block pairs {st['group_pairs']:.3g}")
     history += hist
     total_s = time.perf_counter() - t_start
     model = prob.model()

@@ -363,6 +363,14 @@ void cuda_get_slices(sp_3matrix* model, real* d_model, real* d_slices, real* d_r
   const bool fuse = e.precision == PREC_FP16 && e.pix_gen > 0 && e.pack_key.N2d == S * S && !sparse_estep() &&
                     expand_pack_fits(S, e.n_mp) && N_slices > 0;
   e.slice_pack = {};
+  // gathers from a bricked copy of the model (made here: the model changes
+  // every iteration; 2 x S^3 x 4 bytes of traffic)
+  const bool bricked = S % 4 == 0 && e.d_model_brick.ensure(sizeof(float) * (size_t)S * S * S);
+  const float* gm = bricked ? e.d_model_brick.as<float>() : d_model;
+  if (bricked) {
+    PhaseTimer pt(PH_EXPAND);
+    launch_brick_model(d_model, S, e.d_model_brick.as<float>(), e.stream);
+  }
   if (fuse) {
     e.R_pad = (int)round_up(N_slices, kPadR);
     const size_t ns = (size_t)e.R_pad * e.n_mp;
@@ -370,14 +378,14 @@ void cuda_get_slices(sp_3matrix* model, real* d_model, real* d_slices, real* d_r
         !e.d_Sscale.ensure(sizeof(float) * N_slices))
       return;
     PhaseTimer pt(PH_EXPAND);
-    launch_expand_pack_f16(d_model, S, d_rot, N_slices, e.R_pad, d_x_coordinates, d_y_coordinates, d_z_coordinates,
+    launch_expand_pack_f16(gm, S, d_rot, N_slices, e.R_pad, d_x_coordinates, d_y_coordinates, d_z_coordinates,
                            d_slices, e.d_pix.as<int>(), e.n_mp, e.d_S16.as<_Float16>(), e.d_S16lo.as<_Float16>(),
                            e.d_Sscale.as<float>(), e.d_A.as<double>(), e.interp, e.likelihood == LIKE_POISSON,
-                           e.stream);
+                           e.stream, bricked);
   } else {
     PhaseTimer pt(PH_EXPAND);
-    launch_expand(d_model, S, d_rot, N_slices, d_x_coordinates, d_y_coordinates, d_z_coordinates, d_slices, e.interp,
-                  e.stream);
+    launch_expand(gm, S, d_rot, N_slices, d_x_coordinates, d_y_coordinates, d_z_coordinates, d_slices, e.interp,
+                  e.stream, bricked);
   }
   g_slices_epoch++;
   if (fuse) e.slice_pack = {d_slices, g_slices_epoch, e.pix_gen, N_slices, e.likelihood};
@@ -974,6 +982,11 @@ int emc_hip_set_option(const char* key, const char* value) {
     const long v = strtol(value, &end, 10);
     if (!end || *end != 0 || v < 0 || v > (1 << 20)) return 1;
     proposal().max_shared = (int)v;
+  } else if (is(key, "proposal_group_min")) {
+    char* end = nullptr;
+    const long v = strtol(value, &end, 10);
+    if (!end || *end != 0 || v < 0 || v > (1L << 30)) return 1;
+    proposal().group_min = (int)v;
   } else if (is(key, "proposal_track")) {
     if (is(value, "on")) proposal().track = true;
     else if (is(value, "off")) proposal().track = false;

@@ -164,6 +164,7 @@ struct Engine {
   std::vector<int> h_sp_rows, h_sp_rowseg, h_sp_light, h_sp_zero;
   DBuf d_sp_seg, d_sp_rows, d_sp_rowseg, d_sp_part, d_sp_light, d_sp_zero;
   DBuf d_Sc;         // float [R_pad][n_mp]: updated compact slices for insertion
+  DBuf d_model_brick;  // float [S^3]: the model in 4 x 4 x 4 bricks for the expand's gathers
   DBuf d_wts;        // float [R]: rotation weights (uploaded per call)
   DBuf d_wT;         // float [R]: weights[r] * T_r
 
@@ -208,8 +209,10 @@ struct PhaseTimer {
 
 // ---- kernel launchers (defined in the k_*.hip files) ----------------------
 // expand
+// (bricked: model is the 4 x 4 x 4-brick copy made by launch_brick_model)
 void launch_expand(const float* model, int S, const float* rot, int R, const float* x,
-                   const float* y, const float* z, float* slices, int interp, hipStream_t st);
+                   const float* y, const float* z, float* slices, int interp, hipStream_t st, bool bricked = false);
+void launch_brick_model(const float* model, int S, float* out, hipStream_t st);
 // pack patterns: compact columns, B_i, FP32 and FP16 copies (+ transposes)
 void launch_pack_mask(const int* mask, int N2d, int* pix, int n_mp, hipStream_t st);
 // B = sum_p m K^2, or sum_p m K when poisson
@@ -229,7 +232,8 @@ void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d,
 bool expand_pack_fits(int S, int n_mp);
 void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, int R_pad, const float* x,
                             const float* y, const float* z, float* slices, const int* pix, int n_mp, _Float16* S16,
-                            _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st);
+                            _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st,
+                            bool bricked = false);
 void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, _Float16* S16, _Float16* S16lo, float* scale, double* A,
                             hipStream_t st, bool poisson = false);

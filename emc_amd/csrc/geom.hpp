@@ -48,4 +48,32 @@ __device__ __forceinline__ bool tri_corner(int c, float ux, float uy, float uz, 
   return true;
 }
 
+// Model copy for the expand's gathers, in 4 x 4 x 4 voxel bricks (256 B,
+// two cache lines of 4 x 4 x 2): a slice plane crosses few bricks, where the
+// linear x-fastest layout puts nearly every voxel of a tilted plane in its
+// own cache line.  S % 4 == 0.
+__device__ __forceinline__ size_t brick_index(int x, int y, int z, int S) {
+  const int nb = S >> 2;
+  return ((((size_t)(z >> 2) * nb + (y >> 2)) * nb + (x >> 2)) << 6) + (((z & 3) * 4 + (y & 3)) * 4 + (x & 3));
+}
+
+// Model value at voxel (x, y, z), inside the grid, from the linear or the
+// bricked layout.
+template <bool BRICK>
+__device__ __forceinline__ float model_at(const float* __restrict__ m, int x, int y, int z, int S) {
+  return BRICK ? m[brick_index(x, y, z, S)] : m[((size_t)z * S + y) * S + x];
+}
+
+// Trilinear corner c as voxel coordinates (see tri_corner).
+__device__ __forceinline__ bool tri_corner_xyz(int c, float ux, float uy, float uz, int S, int& ix, int& iy, int& iz,
+                                               float& w) {
+  const float fx0 = floorf(ux), fy0 = floorf(uy), fz0 = floorf(uz);
+  ix = (int)fx0 + (c & 1);
+  iy = (int)fy0 + ((c >> 1) & 1);
+  iz = (int)fz0 + ((c >> 2) & 1);
+  const float fx = ux - fx0, fy = uy - fy0, fz = uz - fz0;
+  w = ((c & 1) ? fx : 1.0f - fx) * (((c >> 1) & 1) ? fy : 1.0f - fy) * (((c >> 2) & 1) ? fz : 1.0f - fz);
+  return !(ix < 0 || ix >= S || iy < 0 || iy >= S || iz < 0 || iz >= S);
+}
+
 }  // namespace emc

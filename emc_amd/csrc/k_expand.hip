@@ -30,7 +30,23 @@ __device__ __forceinline__ int expand_pixel(int t, int S) {
 constexpr int kPackThreads = 1024;
 constexpr int kPackPer = 16;
 
+// Bricked copy of the model for the gathers (brick_index).
+__global__ __launch_bounds__(256) void k_brick_model(const float* __restrict__ model, int S, float* __restrict__ out) {
+  const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t n = (size_t)S * S * S;
+  if (v >= n) return;
+  const int x = (int)(v % S), y = (int)((v / S) % S), z = (int)(v / ((size_t)S * S));
+  out[brick_index(x, y, z, S)] = model[v];
+}
+
+void launch_brick_model(const float* model, int S, float* out, hipStream_t st) {
+  const size_t n = (size_t)S * S * S;
+  hipLaunchKernelGGL(k_brick_model, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, model, S, out);
+  EMC_LAUNCHED();
+}
+
 // One workgroup per rotation; model reads are gathers served by L2.
+template <bool BRICK>
 __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict__ model, int S,
                                                         const float* __restrict__ rot,
                                                         const float* __restrict__ x,
@@ -49,8 +65,7 @@ __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict_
     voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
     const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
     float v = 0.0f;
-    if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S)
-      v = model[((size_t)vz * S + vy) * S + vx];
+    if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S) v = model_at<BRICK>(model, vx, vy, vz, S);
     out[p] = v;
   }
 }
@@ -58,6 +73,7 @@ __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict_
 // Trilinear slicing (not in the reference): the 8 voxels around the
 // continuous coordinate, those inside 0 <= v < S, weighted by the usual
 // products of fractional distances.
+template <bool BRICK>
 __global__ __launch_bounds__(256) void k_expand_trilinear(const float* __restrict__ model, int S,
                                                           const float* __restrict__ rot,
                                                           const float* __restrict__ x,
@@ -77,21 +93,28 @@ __global__ __launch_bounds__(256) void k_expand_trilinear(const float* __restric
     float acc = 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-      int v;
+      int ix, iy, iz;
       float w;
-      if (tri_corner(c, ux, uy, uz, S, v, w)) acc += w * model[v];
+      if (tri_corner_xyz(c, ux, uy, uz, S, ix, iy, iz, w)) acc += w * model_at<BRICK>(model, ix, iy, iz, S);
     }
     out[p] = acc;
   }
 }
 
 void launch_expand(const float* model, int S, const float* rot, int R, const float* x,
-                   const float* y, const float* z, float* slices, int interp, hipStream_t st) {
+                   const float* y, const float* z, float* slices, int interp, hipStream_t st, bool bricked) {
   if (R <= 0) return;
-  if (interp == INTERP_TRILINEAR)
-    hipLaunchKernelGGL(k_expand_trilinear, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
-  else
-    hipLaunchKernelGGL(k_expand_nearest, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+  if (interp == INTERP_TRILINEAR) {
+    if (bricked)
+      hipLaunchKernelGGL(k_expand_trilinear<true>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+    else
+      hipLaunchKernelGGL(k_expand_trilinear<false>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+  } else {
+    if (bricked)
+      hipLaunchKernelGGL(k_expand_nearest<true>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+    else
+      hipLaunchKernelGGL(k_expand_nearest<false>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+  }
   EMC_LAUNCHED();
 }
 
@@ -290,7 +313,7 @@ __global__ __launch_bounds__(kPackThreads) void k_pack_slices_f16(const float* _
 // registers, the row maximum, the hi/lo pair and A_r.  Needs N2d floats of
 // LDS and n_mp <= kPackThreads * kPackPer.
 
-template <bool TRI>
+template <bool TRI, bool BRICK>
 __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* __restrict__ model, int S,
                                                          const float* __restrict__ rot,
                                                          const float* __restrict__ x, const float* __restrict__ y,
@@ -316,13 +339,13 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
       if (TRI) {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-          int vi;
+          int ix, iy, iz;
           float w;
-          if (tri_corner(c, ux, uy, uz, S, vi, w)) v += w * model[vi];
+          if (tri_corner_xyz(c, ux, uy, uz, S, ix, iy, iz, w)) v += w * model_at<BRICK>(model, ix, iy, iz, S);
         }
       } else {
         const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
-        if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S) v = model[((size_t)vz * S + vy) * S + vx];
+        if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S) v = model_at<BRICK>(model, vx, vy, vz, S);
       }
       out[p] = v;
       sv[p] = v;
@@ -374,14 +397,26 @@ bool expand_pack_fits(int S, int n_mp) { return (size_t)S * S * 4 <= 65536 && n_
 
 void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, int R_pad, const float* x,
                             const float* y, const float* z, float* slices, const int* pix, int n_mp, _Float16* S16,
-                            _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st) {
+                            _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st,
+                            bool bricked) {
   const size_t lds = (size_t)S * S * sizeof(float);
-  if (interp == INTERP_TRILINEAR)
-    hipLaunchKernelGGL(k_expand_pack_f16<true>, dim3(R_pad), dim3(kPackThreads), lds, st, model, S, rot, x, y, z, slices, pix,
-                       R, n_mp, S16, S16lo, scale, A, (int)poisson);
-  else
-    hipLaunchKernelGGL(k_expand_pack_f16<false>, dim3(R_pad), dim3(kPackThreads), lds, st, model, S, rot, x, y, z, slices,
-                       pix, R, n_mp, S16, S16lo, scale, A, (int)poisson);
+  const dim3 g(R_pad), b(kPackThreads);
+  const int po = (int)poisson;
+  if (interp == INTERP_TRILINEAR) {
+    if (bricked)
+      hipLaunchKernelGGL((k_expand_pack_f16<true, true>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
+                         S16, S16lo, scale, A, po);
+    else
+      hipLaunchKernelGGL((k_expand_pack_f16<true, false>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
+                         S16, S16lo, scale, A, po);
+  } else {
+    if (bricked)
+      hipLaunchKernelGGL((k_expand_pack_f16<false, true>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
+                         S16, S16lo, scale, A, po);
+    else
+      hipLaunchKernelGGL((k_expand_pack_f16<false, false>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
+                         S16, S16lo, scale, A, po);
+  }
   EMC_LAUNCHED();
 }
 

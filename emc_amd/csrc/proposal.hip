@@ -32,7 +32,10 @@ void Proposal::release() {
                   &d_Ksublo, &d_Ktsub, &d_Ktsublo, &d_Bsub, &d_phisub, &d_invphisub, &d_Lsub, &d_phiden_sub,
                   &d_Psub16, &d_Psub16lo, &d_mass, &d_tot, &d_iota, &d_part_c, &d_part_m, &d_sh_cells, &d_sh_off, &d_sh_rot,
                   &d_Ssh, &d_Sshlo, &d_Ash, &d_sscale_sh, &d_Lsh, &d_top_s, &d_topm_s, &d_net_cand,
-                  &d_nbr, &d_mark, &d_shared, &d_count, &d_ncand, &d_moved, &d_keys_s, &d_vals_s, &d_sort_tmp, &d_colmap};
+                  &d_nbr, &d_mark, &d_shared, &d_count, &d_ncand, &d_moved, &d_keys_s, &d_vals_s, &d_sort_tmp, &d_colmap,
+                  &d_gkey, &d_gcount, &d_gpat, &d_gkey_s, &d_ghist, &d_gcore, &d_gncore, &d_gnrot, &d_gcofs, &d_crot,
+                  &d_ccell, &d_cslot, &d_rank, &d_rkey, &d_gwork, &d_rcount, &d_rncand, &d_rent_off, &d_rent_pat,
+                  &d_rent_j, &d_rwork, &d_rkeys_s, &d_rvals_s};
   for (DBuf* b : bufs) b->release();
   cnn.release();
   configured = predicted = have_p = false;
@@ -53,6 +56,8 @@ static void sync_stream() { EMC_CHECK(hipStreamSynchronize(engine().stream)); }
 // chunks of the M-step, candidate rotation count of each pattern, from the
 // candidate cells in d_cand: per-cell counts on the GPU, the entry lists by
 // a stable sort on the GPU, the small per-cell arrays on the host.
+static bool build_groups(Proposal& p);
+
 static bool build_lists(Proposal& p) {
   Engine& e = engine();
   hipStream_t st = e.stream;
@@ -124,6 +129,85 @@ static bool build_lists(Proposal& p) {
       !p.d_panels.ensure(2 * (size_t)std::max<long>(p.n_chunks, 1) * 2 * kSlot * 32) ||
       !p.d_plogp.ensure(sizeof(double) * N) || !p.d_crowscale.ensure(sizeof(float) * p.R))
     return false;
+  sync_stream();
+  return !failed() && build_groups(p);
+}
+
+// Pattern groups and their dense blocks; the per-cell E-step's entries are
+// those left outside the blocks.  With no group large enough, no blocks.
+static bool build_groups(Proposal& p) {
+  Engine& e = engine();
+  hipStream_t st = e.stream;
+  const int N = p.N, K = p.topk, C = p.C;
+  p.n_gwork = 0;
+  p.n_rwork = 0;
+  p.st_group_pairs = 0;
+  if (p.group_min <= 0) return true;
+  if (!p.d_gkey.ensure(sizeof(int) * N) || !p.d_gcount.ensure(sizeof(int) * C) || !p.d_gpat.ensure(sizeof(int) * N) ||
+      !p.d_ghist.ensure(sizeof(int) * (size_t)C * C) || !p.d_gcore.ensure(sizeof(int) * (size_t)C * kMaxCore) ||
+      !p.d_gncore.ensure(sizeof(int) * C) || !p.d_gnrot.ensure(sizeof(int) * C))
+    return false;
+  if (!build_group_keys(p.d_cand.as<int>(), N, K, C, p.d_gkey.as<int>(), p.d_gcount.as<int>(), p.d_gpat.as<int>(),
+                        p.d_gkey_s, p.d_sort_tmp, st))
+    return false;
+  launch_group_core(p.d_cand.as<int>(), p.d_gkey.as<int>(), p.d_gcount.as<int>(), N, K, C, p.group_min,
+                    p.d_cell_off.as<int>(), p.d_ghist.as<int>(), p.d_gcore.as<int>(), p.d_gncore.as<int>(),
+                    p.d_gnrot.as<int>(), st);
+  p.h_gcount.resize(C);
+  p.h_gncore.resize(C);
+  p.h_gnrot.resize(C);
+  EMC_CHECK(hipMemcpyAsync(p.h_gcount.data(), p.d_gcount.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
+  EMC_CHECK(hipMemcpyAsync(p.h_gncore.data(), p.d_gncore.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
+  EMC_CHECK(hipMemcpyAsync(p.h_gnrot.data(), p.d_gnrot.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
+  sync_stream();
+  if (failed()) return false;
+  // block work items: (first pattern in the sorted list, count, first core
+  // rotation, count), 128 x 128
+  p.h_gcofs.assign(C + 1, 0);
+  p.h_gwork.clear();
+  long pat0 = 0;
+  for (int g = 0; g < C; g++) {
+    p.h_gcofs[g + 1] = p.h_gcofs[g] + (p.h_gncore[g] > 0 ? p.h_gnrot[g] : 0);
+    if (p.h_gncore[g] > 0)
+      for (int a = 0; a < p.h_gcount[g]; a += kGroupTile)
+        for (int b = 0; b < p.h_gnrot[g]; b += kGroupTile) {
+          p.h_gwork.push_back(make_int4((int)(pat0 + a), std::min(kGroupTile, p.h_gcount[g] - a),
+                                        p.h_gcofs[g] + b, std::min(kGroupTile, p.h_gnrot[g] - b)));
+          p.st_group_pairs += (double)std::min(kGroupTile, p.h_gcount[g] - a) * std::min(kGroupTile, p.h_gnrot[g] - b);
+        }
+    pat0 += p.h_gcount[g];
+  }
+  p.n_gwork = (int)p.h_gwork.size();
+  if (p.n_gwork == 0) return true;
+  const int ncr = p.h_gcofs[C];
+  const long n = (long)N * K;
+  if (!upload(p.d_gcofs, p.h_gcofs, st) || !upload(p.d_gwork, p.h_gwork, st) || !p.d_crot.ensure(sizeof(int) * ncr) ||
+      !p.d_ccell.ensure(ncr) || !p.d_cslot.ensure(ncr) || !p.d_rank.ensure((size_t)N * kMaxCore) ||
+      !p.d_rkey.ensure(sizeof(int) * n) || !p.d_rcount.ensure(sizeof(int) * (C + 1)) ||
+      !p.d_rncand.ensure(sizeof(int) * N) || !p.d_rent_pat.ensure(sizeof(int) * n) ||
+      !p.d_rent_j.ensure(sizeof(int) * n))
+    return false;
+  launch_group_rots(p.d_gcore.as<int>(), p.d_gncore.as<int>(), p.d_gcofs.as<int>(), p.d_cell_off.as<int>(),
+                    p.d_cell_rot.as<int>(), C, p.d_crot.as<int>(), p.d_ccell.as<unsigned char>(),
+                    p.d_cslot.as<unsigned char>(), st);
+  launch_group_rank(p.d_cand.as<int>(), p.d_gkey.as<int>(), N, K, C, p.d_gcore.as<int>(), p.d_gncore.as<int>(),
+                    p.d_rank.as<signed char>(), p.d_rkey.as<int>(), st);
+  // the remaining entries, cell-major (key C = inside a block: sorted last)
+  launch_cand_hist(p.d_rkey.as<int>(), N, K, C, p.d_cell_off.as<int>(), p.d_rcount.as<int>(), p.d_rncand.as<int>(), st);
+  if (!sort_entries(p.d_rkey.as<int>(), N, K, C + 1, p.d_rkeys_s, p.d_rvals_s, p.d_sort_tmp, p.d_rent_pat.as<int>(),
+                    p.d_rent_j.as<int>(), st))
+    return false;
+  p.h_rcount.resize(C + 1);
+  EMC_CHECK(hipMemcpyAsync(p.h_rcount.data(), p.d_rcount.p, sizeof(int) * (C + 1), hipMemcpyDeviceToHost, st));
+  sync_stream();
+  if (failed()) return false;
+  p.h_rent_off.assign(C + 1, 0);
+  for (int c = 0; c < C; c++) p.h_rent_off[c + 1] = p.h_rent_off[c] + p.h_rcount[c];
+  std::vector<int2> rwork;
+  for (int c = 0; c < C; c++)
+    for (int e0 = p.h_rent_off[c]; e0 < p.h_rent_off[c + 1]; e0 += kCandRows) rwork.push_back(make_int2(c, e0));
+  p.n_rwork = (int)rwork.size();
+  if (!upload(p.d_rent_off, p.h_rent_off, st) || !upload(p.d_rwork, rwork, st)) return false;
   sync_stream();
   return !failed();
 }
@@ -360,13 +444,15 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
     a.S = e.d_S16.as<_Float16>();
     a.Slo = e.d_S16lo.as<_Float16>();
     a.zero = p.d_zero.as<_Float16>();
-    a.work = p.d_work.as<int2>();
-    a.n_work = p.n_work;
+    // with group blocks: the blocks, then the entries outside them
+    const bool groups = p.n_gwork > 0;
+    a.work = groups ? p.d_rwork.as<int2>() : p.d_work.as<int2>();
+    a.n_work = groups ? p.n_rwork : p.n_work;
     a.cell_off = p.d_cell_off.as<int>();
     a.cell_rot = p.d_cell_rot.as<int>();
-    a.ent_off = p.d_ent_off.as<int>();
-    a.ent_pat = p.d_ent_pat.as<int>();
-    a.ent_j = p.d_ent_j.as<int>();
+    a.ent_off = groups ? p.d_rent_off.as<int>() : p.d_ent_off.as<int>();
+    a.ent_pat = groups ? p.d_rent_pat.as<int>() : p.d_ent_pat.as<int>();
+    a.ent_j = groups ? p.d_rent_j.as<int>() : p.d_ent_j.as<int>();
     a.A = e.d_A.as<double>();
     a.B = e.d_B.as<double>();
     a.phi = phi_used;
@@ -377,6 +463,32 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
     a.topk = p.topk;
     a.poisson = poisson;
     a.Lc = p.d_Lc.as<float>();
+    if (groups) {
+      GroupEstepArgs ga{};
+      ga.K = a.K;
+      ga.Klo = a.Klo;
+      ga.S = a.S;
+      ga.Slo = a.Slo;
+      ga.zero = a.zero;
+      ga.work = p.d_gwork.as<int4>();
+      ga.n_work = p.n_gwork;
+      ga.gpat = p.d_gpat.as<int>();
+      ga.crot = p.d_crot.as<int>();
+      ga.ccell = p.d_ccell.as<unsigned char>();
+      ga.cslot = p.d_cslot.as<unsigned char>();
+      ga.rank = p.d_rank.as<signed char>();
+      ga.A = a.A;
+      ga.B = a.B;
+      ga.phi = a.phi;
+      ga.sscale = a.sscale;
+      ga.kscale_inv = a.kscale_inv;
+      ga.inv_2nms2 = a.inv_2nms2;
+      ga.n_mp = a.n_mp;
+      ga.topk = a.topk;
+      ga.poisson = a.poisson;
+      ga.Lc = a.Lc;
+      launch_group_estep(split, ga, st);
+    }
     launch_cand_estep(split, a, st);
   }
   if (p.R_sh > 0) {
@@ -1006,13 +1118,13 @@ int emc_hip_proposal_shared(int* cells, int max_cells) {
 
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  const double v[17] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
+  const double v[18] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
                         p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
                         p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
                         p.st_cand_pairs,     p.st_moved,     p.st_dense,      p.st_shared,
-                        p.st_recentre_ms};
+                        p.st_recentre_ms,    p.st_group_pairs};
   int k = 0;
-  for (; k < n && k < 17; k++) out[k] = v[k];
+  for (; k < n && k < 18; k++) out[k] = v[k];
   return k;
 }
 

@@ -78,6 +78,18 @@ struct Proposal {
   DBuf d_seg_cell, d_seg_chunk0, d_seg_nch, d_seg_part, d_mcell, d_mcell_part, d_part;
   DBuf d_part_T, d_part_Tphi, d_part_max;
   int n_seg = 0, n_mcell = 0, n_parts = 0;
+  // pattern groups of the candidate E-step: patterns sharing their first
+  // candidate cell, whose core cells (candidates of at least half of them)
+  // are scored as dense blocks; the per-cell E-step takes the other entries
+  int group_min = 64;                // smallest group scored as blocks (0: no groups)
+  DBuf d_gkey, d_gcount, d_gpat, d_gkey_s, d_ghist, d_gcore, d_gncore, d_gnrot, d_gcofs, d_crot, d_ccell, d_cslot,
+      d_rank, d_rkey, d_gwork, d_rcount, d_rncand;
+  std::vector<int> h_gcount, h_gncore, h_gnrot, h_gcofs, h_rcount, h_rent_off;
+  std::vector<int4> h_gwork;
+  int n_gwork = 0;
+  double st_group_pairs = 0;         // (pattern, rotation) pairs of the group blocks
+  DBuf d_rent_off, d_rent_pat, d_rent_j, d_rwork, d_rkeys_s, d_rvals_s;
+  int n_rwork = 0;
   DBuf d_Lc;      // float [N][topk][32]: L, then P
   DBuf d_panels;  // half [chunks][2][32][32] M-step operand
   DBuf d_crowscale;

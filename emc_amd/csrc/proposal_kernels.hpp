@@ -11,6 +11,8 @@ constexpr int kSlot = 32;       // rotation slots per coarse cell (largest cell 
 constexpr int kCandRows = 128;  // entries per E-step workgroup
 constexpr int kSegChunks = 24;  // 32-entry chunks per M-step segment
 constexpr int kMaxTopk = 64;    // candidate cells per pattern at most
+constexpr int kMaxCore = 64;    // core cells of a pattern group at most
+constexpr int kGroupTile = 128; // patterns and core rotations per group-block workgroup
 
 struct CandEstepArgs {
   const _Float16 *K, *Klo;          // [N_pad][n_mp] packed counts (lo only when split)
@@ -28,6 +30,24 @@ struct CandEstepArgs {
   int n_mp, topk;
   int poisson;                      // Poisson epilogue (C over log S)
   float* Lc;                        // [N][topk][kSlot]
+};
+
+struct GroupEstepArgs {
+  const _Float16 *K, *Klo, *S, *Slo, *zero;
+  const int4* work;                 // (first pattern in gpat, patterns, first core rotation, rotations)
+  int n_work;
+  const int* gpat;                  // patterns sorted by group
+  const int* crot;                  // core rotations of every group
+  const unsigned char *ccell, *cslot;  // core index of each one's cell, its slot in the cell
+  const signed char* rank;          // [N][kMaxCore] candidate rank of each core cell (-1: none)
+  const double *A, *B;
+  const float* phi;
+  const float* sscale;
+  float kscale_inv;
+  double inv_2nms2;
+  int n_mp, topk;
+  int poisson;
+  float* Lc;
 };
 
 struct CandSoftmaxArgs {
@@ -136,6 +156,18 @@ void launch_cand_hist(const int* cand, int N, int topk, int C, const int* cell_o
                       hipStream_t st);
 bool sort_entries(const int* cand, int N, int topk, int C, DBuf& keys_sorted, DBuf& vals_sorted, DBuf& temp,
                   int* ent_pat, int* ent_j, hipStream_t st);
+
+// pattern groups of the candidate E-step (k_groups.hip)
+bool build_group_keys(const int* cand, int N, int topk, int C, int* key, int* count, int* gpat, DBuf& key_sorted,
+                      DBuf& temp, hipStream_t st);
+// hist: C * C ints of scratch; core: [C][kMaxCore]
+void launch_group_core(const int* cand, const int* key, const int* count, int N, int topk, int C, int min_size,
+                       const int* cell_off, int* hist, int* core, int* ncore, int* nrot, hipStream_t st);
+void launch_group_rots(const int* core, const int* ncore, const int* cofs, const int* cell_off, const int* cell_rot,
+                       int C, int* crot, unsigned char* ccell, unsigned char* cslot, hipStream_t st);
+void launch_group_rank(const int* cand, const int* key, int N, int topk, int C, const int* core, const int* ncore,
+                       signed char* rank, int* rkey, hipStream_t st);
+void launch_group_estep(bool ksplit, const GroupEstepArgs& a, hipStream_t st);
 
 // nonzero responsibilities per rotation from the candidate slots, shared
 // rows and subset columns (k_lists.hip); R * N < 2^32

@@ -96,6 +96,11 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 are shared when there are at most this many: every pattern
  *                 is scored on all their rotations (dense kernel) besides its
  *                 topk candidate cells, which then exclude them.
+ *   "proposal_group_min" smallest group of patterns (sharing their first
+ *                 candidate cell) whose common candidate cells are scored as
+ *                 one dense block (default "64"; "0": every candidate cell
+ *                 by the per-cell kernel).  Takes effect when the candidate
+ *                 lists are next built.
  *   "proposal_gate"  "predicted" (default): a pattern also starts in
  *                 whole-table search when the network's top-k cells hold less
  *                 than min_mass of its predicted distribution; "audit": only
@@ -185,8 +190,9 @@ int emc_hip_proposal_shared(int *cells, int max_cells);
  * candidate pairs of all patterns (the candidate E-step's work), [13]
  * patterns whose candidates were re-centred before the iteration, [14] 1 if
  * the audit sent the iteration to the whole table, [15] shared cells, [16]
- * host time of the re-centring and list building (ms).  Returns the number
- * of values written (<= n). */
+ * host time of the re-centring and list building (ms), [17] (pattern,
+ * rotation) pairs of the group blocks.  Returns the number of values
+ * written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 
 /* Patterns given as CSR (no counterpart in the reference): pattern i holds

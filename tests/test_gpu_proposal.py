@@ -14,7 +14,9 @@ C-ABI:
   - with shared cells (the cells holding the patterns' last posteriors),
     every pattern is also scored on the shared cells' rotations and the
     iteration matches the FP64 oracle restricted to candidates + shared
-    rotations (+ the whole table for the audited patterns).
+    rotations (+ the whole table for the audited patterns);
+  - scoring a group's common candidate cells as one dense block gives the
+    same iteration, bit for bit, as scoring every candidate cell alone.
 """
 import ctypes as C
 
@@ -47,7 +49,7 @@ def _reset():
 
 
 def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", min_mass="0.999", gate="audit",
-         track="off", max_fail="1", mstep="auto", shared="0"):
+         track="off", max_fail="1", mstep="auto", shared="0", group_min="0"):
     E.set_option("precision", precision)
     E.set_option("insert", insert)
     E.set_option("interp", "nearest")
@@ -59,6 +61,7 @@ def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", mi
     E.set_option("audit_max_fail", max_fail)
     E.set_option("mstep", mstep)
     E.set_option("proposal_shared", shared)
+    E.set_option("proposal_group_min", group_min)
 
 
 def init_cells(P, topk, n_coarse=2, batch=64):
@@ -73,8 +76,8 @@ def init_cells(P, topk, n_coarse=2, batch=64):
 
 
 def stats():
-    out = (C.c_double * 17)()
-    assert E.lib().emc_hip_proposal_stats(out, 17) == 17
+    out = (C.c_double * 18)()
+    assert E.lib().emc_hip_proposal_stats(out, 18) == 18
     return list(out)
 
 
@@ -398,3 +401,36 @@ def test_shared_cells_match_oracle(prob, mstep):
           f"model {dm:.2e}, sum PlogP {dl:.2e}")
     assert st[11] == len(sub2) and st[6] == allowed.sum()
     assert dP < 1e-4 and ds < 1e-4 and dm < 1e-4 and dl < 1e-5
+
+
+def test_group_blocks_match_per_cell(prob):
+    """Every pattern's candidates start with the same three cells (one group
+    of 300 patterns, those cells its core), then its own best cells.  The
+    iteration with the group scored as a dense block equals the one with
+    every candidate cell scored by the per-cell kernel, bit for bit (same
+    products, same k order), and matches the restricted FP64 oracle."""
+    P = prob
+    topk = 6
+    nc, off, rot = init_cells(P, topk)
+    best, _ = oracle_candidates(P, off, rot, 12)
+    common = np.array([0, 7, 19], np.int32)
+    cand = np.zeros((P.N, topk), np.int32)
+    for i in range(P.N):
+        own = [c for c in best[i] if c not in common][:topk - 3]
+        cand[i] = np.concatenate([common, own])
+    out = {}
+    for gmin in ("0", "16"):
+        opts(proposal="on", audit="0.3", group_min=gmin)
+        assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
+        out[gmin] = run(P)[0]
+    g0, g1 = out["0"], out["16"]
+    print(f"group-block pairs {g1['stats'][17]:.0f} (none without groups: {g0['stats'][17]:.0f})")
+    core_rot = sum(off[c + 1] - off[c] for c in common)
+    assert g0["stats"][17] == 0 and g1["stats"][17] >= P.N * core_rot
+    for k in ("P", "scaling", "model"):
+        assert np.array_equal(g0[k], g1[k]), k
+    sub = audited(P.N, 1, 0.3)
+    allowed = O.allowed_from_candidates(cand, off, rot, P.R, sub)
+    ref = O.o64_proposal_iteration(P.model.astype(np.float64), P.images, P.mask, np.ones(P.N), P.side, P.q, P.w,
+                                   P.x, P.y, P.z, allowed)
+    assert np.abs(g1["P"] - ref["P"]).max() < 1e-4 and rel(g1["model"], ref["model"]) < 1e-4
