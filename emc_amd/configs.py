@@ -33,14 +33,18 @@ class Config:
     # "poisson" where the workload's likelihood is the count x log-slice
     # contraction; "gaussian" is the reference's absolute-difference form
     likelihood: str = "gaussian"
+    proposal: bool = False  # the learned rotation proposal after the training iterations
+    csr: bool = False       # patterns held as CSR, sparse-count E-step
 
 
 CONFIGS = {
     0: Config("s32-n1k-r3240", 32, 20, 6.0, 1000, 0.2, 2000.0, 3.0, 0, 0, 4, 10),
     1: Config("s64-n10k-r25680", 64, 40, 12.0, 10000, 0.3, 5000.0, 4.0, 2, 0, 8, 20),
     2: Config("s128-n50k-r50100", 128, 80, 24.0, 50000, 0.3, 10000.0, 6.0, 0, 3, 10, 20, "fp16", "poisson"),
-    3: Config("s128-n50k-r50100-proposal", 128, 80, 24.0, 50000, 0.3, 10000.0, 6.0, 0, 3, 10, 20, "fp16", "poisson"),
-    4: Config("s256-n20k-r25680-sparse", 256, 150, 48.0, 20000, 0.5, 300.0, 12.0, 0, 8, 8, 30, "fp16", "poisson"),
+    3: Config("s128-n50k-r50100-proposal", 128, 80, 24.0, 50000, 0.3, 10000.0, 6.0, 0, 3, 10, 20, "fp16", "poisson",
+              True),
+    4: Config("s256-n20k-r25680-sparse", 256, 150, 48.0, 20000, 0.5, 300.0, 12.0, 0, 8, 8, 30, "fp16", "poisson",
+              True, True),
 }
 
 

@@ -1011,7 +1011,7 @@ void emc_hip_invalidate(void) {
 
 static const char* kPhaseNames[PH_COUNT] = {"expand", "pack", "likelihood", "softmax", "scaling",
                                             "mstep_prep", "mstep", "insert", "normalize", "other", "audit",
-                                            "shared"};
+                                            "shared", "groups", "audit_gemm"};
 
 int emc_hip_phase_times(double* ms, long long* launches, int n) {
   Engine& e = engine();

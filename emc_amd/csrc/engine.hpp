@@ -38,7 +38,7 @@ struct DBuf {
 // ---- phases for timing -----------------------------------------------------
 enum Phase {
   PH_EXPAND = 0, PH_PACK, PH_LIKE, PH_SOFTMAX, PH_SCALING, PH_ROWSUM, PH_MSTEP,
-  PH_INSERT, PH_NORMALIZE, PH_OTHER, PH_AUDIT, PH_SHARED, PH_COUNT
+  PH_INSERT, PH_NORMALIZE, PH_OTHER, PH_AUDIT, PH_SHARED, PH_GROUPS, PH_AUDIT_GEMM, PH_COUNT
 };
 
 enum Precision { PREC_FP32 = 0, PREC_FP16 = 1 };

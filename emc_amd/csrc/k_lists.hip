@@ -10,6 +10,7 @@
 
 #include "engine.hpp"
 #include "proposal_kernels.hpp"
+#include "reduce.hpp"
 
 namespace emc {
 
@@ -86,31 +87,54 @@ void launch_recentre_cand(const int* top, const float* topm, int N, int ktop, co
   EMC_LAUNCHED();
 }
 
-// One thread per pattern: entry counts of every cell (count[C], zeroed
-// beforehand; count[C] counts invalid cells) and the pattern's candidate
-// rotation count.
+// One thread per pattern: its candidate rotation count; count[C] (zeroed
+// beforehand) counts entries naming no valid cell (c == C is allowed as the
+// no-entry key and counted there too).
 __global__ __launch_bounds__(256) void k_cand_hist(const int* __restrict__ cand, int N, int topk, int C,
                                                    const int* __restrict__ cell_off, int* __restrict__ count,
                                                    int* __restrict__ ncand) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= N) return;
-  int s = 0;
+  int s = 0, bad = 0;
   for (int j = 0; j < topk; j++) {
     const int c = cand[(size_t)i * topk + j];
     if (c < 0 || c >= C) {
-      atomicAdd(&count[C], 1);
+      bad++;
       continue;
     }
-    atomicAdd(&count[c], 1);
     s += cell_off[c + 1] - cell_off[c];
   }
   ncand[i] = s;
+  if (bad) atomicAdd(&count[C], bad);
 }
 
 void launch_cand_hist(const int* cand, int N, int topk, int C, const int* cell_off, int* count, int* ncand,
                       hipStream_t st) {
   launch_fill(reinterpret_cast<float*>(count), (size_t)C + 1, 0.0f, st);
   hipLaunchKernelGGL(k_cand_hist, dim3((N + 255) / 256), dim3(256), 0, st, cand, N, topk, C, cell_off, count, ncand);
+  EMC_LAUNCHED();
+}
+
+// count[c] for c < C from the sorted keys (first index with key > c minus
+// first index with key >= c); the entries were sorted already
+__global__ __launch_bounds__(256) void k_counts_sorted(const int* __restrict__ keys, long n, int C,
+                                                       int* __restrict__ count) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= C) return;
+  auto lower = [&](int v) {
+    long lo = 0, hi = n;
+    while (lo < hi) {
+      const long mid = (lo + hi) >> 1;
+      if (keys[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  count[c] = (int)(lower(c + 1) - lower(c));
+}
+
+void launch_counts_sorted(const int* keys, long n, int C, int* count, hipStream_t st) {
+  hipLaunchKernelGGL(k_counts_sorted, dim3((C + 255) / 256), dim3(256), 0, st, keys, n, C, count);
   EMC_LAUNCHED();
 }
 
@@ -228,20 +252,26 @@ __global__ __launch_bounds__(256) void k_nz_split(const unsigned* __restrict__ k
   val[e] = P[e] * opscale[i];
 }
 
+// one wave per rotation (lanes along its list, fixed-order wave sums)
 __global__ __launch_bounds__(256) void k_nz_rowsums(const int* __restrict__ off, const int* __restrict__ col,
                                                     const float* __restrict__ P, int R, const float* __restrict__ phi,
                                                     double* __restrict__ T, float* __restrict__ Tf,
                                                     float* __restrict__ Tphi) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= R) return;
   double t = 0.0, tp = 0.0;
-  for (int e = off[r]; e < off[r + 1]; e++) {
+  for (int e = off[r] + lane; e < off[r + 1]; e += 64) {
     t += P[e];
     if (phi) tp += (double)P[e] * (double)phi[col[e]];
   }
-  T[r] = t;
-  Tf[r] = (float)t;
-  if (phi) Tphi[r] = (float)tp;
+  t = wave_sum(t);
+  if (phi) tp = wave_sum(tp);
+  if (lane == 0) {
+    T[r] = t;
+    Tf[r] = (float)t;
+    if (phi) Tphi[r] = (float)tp;
+  }
 }
 
 bool build_nz_lists(const NzSources& a, long nnz, const int* off, const float* opscale, const float* phi,
@@ -266,7 +296,7 @@ bool build_nz_lists(const NzSources& a, long nnz, const int* off, const float* o
                        vals_sorted.as<float>(), nnz, a.N, opscale, col, val);
     EMC_LAUNCHED();
   }
-  hipLaunchKernelGGL(k_nz_rowsums, dim3((a.R + 255) / 256), dim3(256), 0, st, off, col, vals_sorted.as<float>(), a.R,
+  hipLaunchKernelGGL(k_nz_rowsums, dim3((a.R + 3) / 4), dim3(256), 0, st, off, col, vals_sorted.as<float>(), a.R,
                      phi, T, Tf, Tphi);
   EMC_LAUNCHED();
   return !failed();

@@ -332,18 +332,19 @@ void launch_shared_scatter(const float* Psh, int R_sh, int N, const int* sh_rot,
 __global__ __launch_bounds__(256) void k_fill_subset(float* __restrict__ respons, int N,
                                                      const int* __restrict__ colmap, const float* __restrict__ Psub,
                                                      int n_sub) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
   const int r = blockIdx.x;
   const float* ps = Psub + (size_t)r * n_sub;
-  float4* out = reinterpret_cast<float4*>(respons + (size_t)r * N);
+  f4* out = reinterpret_cast<f4*>(respons + (size_t)r * N);
   const int4* cm = reinterpret_cast<const int4*>(colmap);
   for (int q = threadIdx.x; q < N / 4; q += 256) {
     const int4 c = cm[q];
-    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    f4 v = {0.0f, 0.0f, 0.0f, 0.0f};
     if (c.x >= 0) v.x = ps[c.x];
     if (c.y >= 0) v.y = ps[c.y];
     if (c.z >= 0) v.z = ps[c.z];
     if (c.w >= 0) v.w = ps[c.w];
-    out[q] = v;
+    __builtin_nontemporal_store(v, out + q);  // streamed: not re-read before the candidates' scatter
   }
 }
 

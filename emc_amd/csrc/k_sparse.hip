@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_mstep_sparse(const int* __restrict__ ro
 // skips rotations with T_r <= 1e-10); lanes along pixels
 __global__ __launch_bounds__(256) void k_mstep_zero(const int* __restrict__ rows, MstepArgs a) {
   float* out = a.slices + (size_t)rows[blockIdx.x] * a.N2d;
-  for (int k = threadIdx.x; k < a.n_m; k += 256) out[a.pix[k]] = 0.0f;
+  for (int k = threadIdx.x; k < a.n_m; k += 256) __builtin_nontemporal_store(0.0f, out + a.pix[k]);
 }
 
 // one segment (nonzeros seg.x .. seg.y - 1 of a long row) -> part[s][n_mp]

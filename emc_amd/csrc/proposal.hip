@@ -67,6 +67,10 @@ static bool build_lists(Proposal& p) {
       !p.d_ent_pat.ensure(sizeof(int) * std::max(n, 1L)) || !p.d_ent_j.ensure(sizeof(int) * std::max(n, 1L)))
     return false;
   launch_cand_hist(p.d_cand.as<int>(), N, K, C, p.d_cell_off.as<int>(), p.d_count.as<int>(), p.d_ncand.as<int>(), st);
+  if (!sort_entries(p.d_cand.as<int>(), N, K, C, p.d_keys_s, p.d_vals_s, p.d_sort_tmp, p.d_ent_pat.as<int>(),
+                    p.d_ent_j.as<int>(), st))
+    return false;
+  launch_counts_sorted(p.d_keys_s.as<int>(), n, C, p.d_count.as<int>(), st);
   p.h_count.resize(C + 1);
   p.h_ncand.resize(N);
   EMC_CHECK(hipMemcpyAsync(p.h_count.data(), p.d_count.p, sizeof(int) * (C + 1), hipMemcpyDeviceToHost, st));
@@ -82,9 +86,6 @@ static bool build_lists(Proposal& p) {
   p.n_entries = p.h_ent_off[C];
   p.st_cand_pairs = 0;
   for (int i = 0; i < N; i++) p.st_cand_pairs += p.h_ncand[i];
-  if (!sort_entries(p.d_cand.as<int>(), N, K, C, p.d_keys_s, p.d_vals_s, p.d_sort_tmp, p.d_ent_pat.as<int>(),
-                    p.d_ent_j.as<int>(), st))
-    return false;
   std::vector<int2> work;
   p.h_chunk_off.assign(C + 1, 0);
   for (int c = 0; c < C; c++) {
@@ -142,6 +143,7 @@ static bool build_groups(Proposal& p) {
   p.n_gwork = 0;
   p.n_rwork = 0;
   p.st_group_pairs = 0;
+  p.st_resid_pairs = p.st_cand_pairs;
   if (p.group_min <= 0) return true;
   if (!p.d_gkey.ensure(sizeof(int) * N) || !p.d_gcount.ensure(sizeof(int) * C) || !p.d_gpat.ensure(sizeof(int) * N) ||
       !p.d_ghist.ensure(sizeof(int) * (size_t)C * C) || !p.d_gcore.ensure(sizeof(int) * (size_t)C * kMaxCore) ||
@@ -197,10 +199,15 @@ static bool build_groups(Proposal& p) {
   if (!sort_entries(p.d_rkey.as<int>(), N, K, C + 1, p.d_rkeys_s, p.d_rvals_s, p.d_sort_tmp, p.d_rent_pat.as<int>(),
                     p.d_rent_j.as<int>(), st))
     return false;
+  launch_counts_sorted(p.d_rkeys_s.as<int>(), n, C, p.d_rcount.as<int>(), st);
   p.h_rcount.resize(C + 1);
+  std::vector<int> rn(N);
   EMC_CHECK(hipMemcpyAsync(p.h_rcount.data(), p.d_rcount.p, sizeof(int) * (C + 1), hipMemcpyDeviceToHost, st));
+  EMC_CHECK(hipMemcpyAsync(rn.data(), p.d_rncand.p, sizeof(int) * N, hipMemcpyDeviceToHost, st));
   sync_stream();
   if (failed()) return false;
+  p.st_resid_pairs = 0;
+  for (int i = 0; i < N; i++) p.st_resid_pairs += rn[i];
   p.h_rent_off.assign(C + 1, 0);
   for (int c = 0; c < C; c++) p.h_rent_off[c + 1] = p.h_rent_off[c] + p.h_rcount[c];
   std::vector<int2> rwork;
@@ -368,7 +375,6 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
   p.st_fail = 0;
   p.st_min_mass = p.st_mean_mass = 1.0;
   if (p.n_sub > 0) {
-    PhaseTimer pt(PH_AUDIT);
     const int ns = p.n_sub;
     const size_t rows = (size_t)p.n_sub_pad * e.n_mp;
     const int chunks = softmax_chunks(ns, R);
@@ -381,23 +387,28 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
         !e.d_sacc.ensure(std::max(e.d_sacc.bytes, 4 * cn * 8)) || !p.d_phiden_sub.ensure(sizeof(double) * ns) ||
         !e.d_red.ensure(std::max(e.d_red.bytes, sizeof(double) * (nb + 1))))
       return false;
-    launch_gather_rows(e.d_K16.as<_Float16>(), split ? e.d_K16lo.as<_Float16>() : nullptr, p.d_sub.as<int>(), ns,
-                       p.n_sub_pad, e.n_mp, p.d_Ksub.as<_Float16>(), split ? p.d_Ksublo.as<_Float16>() : nullptr, st);
-    launch_gather_f64(e.d_B.as<double>(), p.d_sub.as<int>(), ns, p.d_Bsub.as<double>(), st);
-    launch_gather_f32(phi_used, p.d_sub.as<int>(), ns, p.d_phisub.as<float>(), st);
-    LikeEpilogue ep{};
-    ep.L = p.d_Lsub.as<float>();
-    ep.A = e.d_A.as<double>();
-    ep.B = p.d_Bsub.as<double>();
-    ep.phi = p.d_phisub.as<float>();
-    ep.cscale = e.d_Sscale.as<float>();
-    ep.kscale_inv = 1.0f / e.k_scale;
-    ep.inv_2nms2 = inv_2nms2;
-    ep.N = ns;
-    ep.R = R;
-    ep.poisson = poisson;
-    ep.vecL = ns % 4 == 0;
-    launch_estep16(split, p.d_Ksub.p, p.d_Ksublo.p, e.d_S16.p, e.d_S16lo.p, p.n_sub_pad, e.R_pad, e.n_mp, ep, st);
+    {
+      PhaseTimer gt(PH_AUDIT_GEMM);
+      launch_gather_rows(e.d_K16.as<_Float16>(), split ? e.d_K16lo.as<_Float16>() : nullptr, p.d_sub.as<int>(), ns,
+                         p.n_sub_pad, e.n_mp, p.d_Ksub.as<_Float16>(), split ? p.d_Ksublo.as<_Float16>() : nullptr,
+                         st);
+      launch_gather_f64(e.d_B.as<double>(), p.d_sub.as<int>(), ns, p.d_Bsub.as<double>(), st);
+      launch_gather_f32(phi_used, p.d_sub.as<int>(), ns, p.d_phisub.as<float>(), st);
+      LikeEpilogue ep{};
+      ep.L = p.d_Lsub.as<float>();
+      ep.A = e.d_A.as<double>();
+      ep.B = p.d_Bsub.as<double>();
+      ep.phi = p.d_phisub.as<float>();
+      ep.cscale = e.d_Sscale.as<float>();
+      ep.kscale_inv = 1.0f / e.k_scale;
+      ep.inv_2nms2 = inv_2nms2;
+      ep.N = ns;
+      ep.R = R;
+      ep.poisson = poisson;
+      ep.vecL = ns % 4 == 0;
+      launch_estep16(split, p.d_Ksub.p, p.d_Ksublo.p, e.d_S16.p, e.d_S16lo.p, p.n_sub_pad, e.R_pad, e.n_mp, ep, st);
+    }
+    PhaseTimer pt(PH_AUDIT);
     launch_softmax(p.d_Lsub.as<float>(), R, ns, chunks, e.d_smax.as<float>(), e.d_ssum.as<double>(),
                    e.d_sacc.as<double>(), e.d_A.as<double>(), true, st);
     launch_softmax_reduce(ns, chunks, e.d_sacc.as<double>(), true, p.d_Bsub.as<double>(), p.d_phisub.as<float>(),
@@ -437,7 +448,6 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
     return true;
   }
   {
-    PhaseTimer pt(PH_LIKE);
     CandEstepArgs a{};
     a.K = e.d_K16.as<_Float16>();
     a.Klo = split ? e.d_K16lo.as<_Float16>() : nullptr;
@@ -487,8 +497,10 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
       ga.topk = a.topk;
       ga.poisson = a.poisson;
       ga.Lc = a.Lc;
+      PhaseTimer gt(PH_GROUPS);
       launch_group_estep(split, ga, st);
     }
+    PhaseTimer pt(PH_LIKE);
     launch_cand_estep(split, a, st);
   }
   if (p.R_sh > 0) {
@@ -1118,13 +1130,13 @@ int emc_hip_proposal_shared(int* cells, int max_cells) {
 
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  const double v[18] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
+  const double v[20] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
                         p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
                         p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
                         p.st_cand_pairs,     p.st_moved,     p.st_dense,      p.st_shared,
-                        p.st_recentre_ms,    p.st_group_pairs};
+                        p.st_recentre_ms,    p.st_group_pairs, p.st_resid_pairs, (double)p.R_sh};
   int k = 0;
-  for (; k < n && k < 18; k++) out[k] = v[k];
+  for (; k < n && k < 20; k++) out[k] = v[k];
   return k;
 }
 

@@ -151,9 +151,11 @@ void launch_mark_support(const int* top, const float* topm, int N, int ktop, flo
 void launch_recentre_cand(const int* top, const float* topm, int N, int ktop, const int* nbr, int nnbr,
                           const int* net, const unsigned char* shared, int C, int topk, int* cand, int* moved,
                           hipStream_t st);
-// count: C + 1 ints (count[C] = entries naming no valid cell)
+// count[C] = entries naming no valid cell (count: C + 1 ints); ncand[i]
 void launch_cand_hist(const int* cand, int N, int topk, int C, const int* cell_off, int* count, int* ncand,
                       hipStream_t st);
+// count[c], c < C, of sorted keys [n]
+void launch_counts_sorted(const int* keys, long n, int C, int* count, hipStream_t st);
 bool sort_entries(const int* cand, int N, int topk, int C, DBuf& keys_sorted, DBuf& vals_sorted, DBuf& temp,
                   int* ent_pat, int* ent_j, hipStream_t st);
 

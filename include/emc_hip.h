@@ -119,7 +119,10 @@ void emc_hip_invalidate(void);
  *   4 scaling, 5 M-step row sums, 6 M-step GEMM, 7 insert, 8 normalise,
  *   9 other, 10 audit (proposal mode: everything done for the patterns
  *   scored on the whole table), 11 shared (proposal mode: the likelihood of
- *   every pattern on the shared cells' rotations).
+ *   every pattern on the shared cells' rotations), 12 groups (proposal mode:
+ *   the dense blocks of pattern groups; phase 2 is then the per-cell kernel
+ *   for the other candidate entries), 13 audit_gemm (proposal mode: the
+ *   whole-table likelihood of the subset; its other work is phase 10).
  * Returns the number of phases written (<= n). */
 int emc_hip_phase_times(double *ms, long long *launches, int n);
 void emc_hip_reset_phase_times(void);
@@ -191,7 +194,8 @@ int emc_hip_proposal_shared(int *cells, int max_cells);
  * patterns whose candidates were re-centred before the iteration, [14] 1 if
  * the audit sent the iteration to the whole table, [15] shared cells, [16]
  * host time of the re-centring and list building (ms), [17] (pattern,
- * rotation) pairs of the group blocks.  Returns the number of values
+ * rotation) pairs of the group blocks, [18] candidate pairs left to the
+ * per-cell kernel, [19] shared rotations.  Returns the number of values
  * written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 

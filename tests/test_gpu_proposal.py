@@ -76,8 +76,8 @@ def init_cells(P, topk, n_coarse=2, batch=64):
 
 
 def stats():
-    out = (C.c_double * 18)()
-    assert E.lib().emc_hip_proposal_stats(out, 18) == 18
+    out = (C.c_double * 20)()
+    assert E.lib().emc_hip_proposal_stats(out, 20) == 20
     return list(out)
 
 
