@@ -75,6 +75,8 @@ bool Engine::init() {
     set_error("no HIP device available");
     return false;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) n_cu = cus;
   if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
     return false;
@@ -186,11 +188,13 @@ static bool ensure_pack(const float* images, const int* mask, int N, int N2d) {
     for (int p = 0; p < N2d; p++)
       if (hmask[p] == 1) tiled.push_back(col_of[p]);
   }
-  if (!e.d_pix.ensure(sizeof(int) * e.n_mp) || !e.d_ins_pix.ensure(sizeof(int) * std::max(e.n_ins, 1)) ||
+  if (!e.d_pix.ensure(sizeof(int) * e.n_mp) || !e.d_kidx.ensure(sizeof(int) * N2d) ||
+      !e.d_ins_pix.ensure(sizeof(int) * std::max(e.n_ins, 1)) ||
       !e.d_ins_tiled.ensure(sizeof(int) * std::max(e.n_ins, 1)) ||
       !e.d_ins_col.ensure(sizeof(int) * std::max(e.n_ins, 1)) || !e.d_B.ensure(sizeof(double) * N))
     return false;
   EMC_CHECK(hipMemcpyAsync(e.d_pix.p, pixp.data(), sizeof(int) * e.n_mp, hipMemcpyHostToDevice, e.stream));
+  EMC_CHECK(hipMemcpyAsync(e.d_kidx.p, col_of.data(), sizeof(int) * N2d, hipMemcpyHostToDevice, e.stream));
   e.pix_gen++;
   if (e.n_ins) {
     EMC_CHECK(hipMemcpyAsync(e.d_ins_pix.p, ins.data(), sizeof(int) * e.n_ins, hipMemcpyHostToDevice, e.stream));
@@ -293,10 +297,7 @@ static bool pack_slices(const float* slices, int R, int N2d) {
     launch_pack_slices_f16(slices, e.d_pix.as<int>(), R, N2d, e.n_mp, e.R_pad, e.d_S16.as<_Float16>(),
                            e.d_S16lo.as<_Float16>(), e.d_Sscale.as<float>(), e.d_A.as<double>(), e.stream,
                            e.likelihood == LIKE_POISSON);
-    if (sparse_estep()) {
-      if (!e.d_LSt.ensure(4 * ns)) return false;
-      launch_transpose_hilo(e.d_S16.as<_Float16>(), e.d_S16lo.as<_Float16>(), e.R_pad, e.n_mp, e.d_LSt.p, e.stream);
-    }
+    e.s16_gen++;
   }
   return !failed();
 }
@@ -322,8 +323,18 @@ static void run_estep(int N, int R, float sigma, const float* phi, float* L, con
   ep.R = R;
   ep.poisson = e.likelihood == LIKE_POISSON;
   ep.vecL = (N % 4 == 0) && L && ((uintptr_t)L % 16 == 0);
+  const bool csr = L && !P && sparse_estep();
+  if (csr && e.lst_gen != e.s16_gen) {
+    // the CSR E-step reads the hi/lo operand transposed; built from the
+    // packed slices when they changed since (proposal iterations that stay
+    // on their candidates never need it)
+    PhaseTimer pp(PH_PACK);
+    if (!e.d_LSt.ensure(4 * (size_t)e.R_pad * e.n_mp)) return;
+    launch_transpose_hilo(e.d_S16.as<_Float16>(), e.d_S16lo.as<_Float16>(), e.R_pad, e.n_mp, e.d_LSt.p, e.stream);
+    e.lst_gen = e.s16_gen;
+  }
   PhaseTimer pt(PH_LIKE);
-  if (L && !P && sparse_estep()) {
+  if (csr) {
     launch_estep_csr(e.d_csr_off.as<int>(), e.d_csr_col.as<int>(), e.d_csr_cnt.as<unsigned short>(), e.d_LSt.p,
                      e.R_pad, ep, e.stream);
     return;
@@ -360,7 +371,7 @@ void cuda_get_slices(sp_3matrix* model, real* d_model, real* d_slices, real* d_r
   // FP16 form with a compact pixel list from an earlier E-step of this
   // detector: pack the slice operands in the same pass (the E-step then
   // skips its pack while the slices, pixel list and likelihood are these)
-  const bool fuse = e.precision == PREC_FP16 && e.pix_gen > 0 && e.pack_key.N2d == S * S && !sparse_estep() &&
+  const bool fuse = e.precision == PREC_FP16 && e.pix_gen > 0 && e.pack_key.N2d == S * S &&
                     expand_pack_fits(S, e.n_mp) && N_slices > 0;
   e.slice_pack = {};
   // gathers from a bricked copy of the model (made here: the model changes
@@ -379,9 +390,10 @@ void cuda_get_slices(sp_3matrix* model, real* d_model, real* d_slices, real* d_r
       return;
     PhaseTimer pt(PH_EXPAND);
     launch_expand_pack_f16(gm, S, d_rot, N_slices, e.R_pad, d_x_coordinates, d_y_coordinates, d_z_coordinates,
-                           d_slices, e.d_pix.as<int>(), e.n_mp, e.d_S16.as<_Float16>(), e.d_S16lo.as<_Float16>(),
-                           e.d_Sscale.as<float>(), e.d_A.as<double>(), e.interp, e.likelihood == LIKE_POISSON,
-                           e.stream, bricked);
+                           d_slices, e.d_pix.as<int>(), e.d_kidx.as<int>(), e.n_m, e.n_mp, e.d_S16.as<_Float16>(),
+                           e.d_S16lo.as<_Float16>(), e.d_Sscale.as<float>(), e.d_A.as<double>(), e.interp,
+                           e.likelihood == LIKE_POISSON, e.stream, bricked);
+    e.s16_gen++;
   } else {
     PhaseTimer pt(PH_EXPAND);
     launch_expand(gm, S, d_rot, N_slices, d_x_coordinates, d_y_coordinates, d_z_coordinates, d_slices, e.interp,
@@ -403,8 +415,7 @@ void cuda_calculate_responsabilities(real* slices, real* images, int* d_mask, re
   if (!ensure_pack(images, d_mask, N_images, N_2d)) return;
   const auto& sp = e.slice_pack;
   const bool packed = e.precision == PREC_FP16 && sp.slices == slices && sp.epoch == g_slices_epoch &&
-                      sp.pix_gen == e.pix_gen && sp.R == N_slices && sp.likelihood == e.likelihood &&
-                      !sparse_estep();
+                      sp.pix_gen == e.pix_gen && sp.R == N_slices && sp.likelihood == e.likelihood;
   if (!packed && !pack_slices(slices, N_slices, N_2d)) return;
   if (!e.d_phi_used.ensure(sizeof(float) * N_images)) return;
   EMC_CHECK(hipMemcpyAsync(e.d_phi_used.p, d_scaling, sizeof(float) * N_images, hipMemcpyDeviceToDevice, e.stream));
@@ -418,7 +429,10 @@ void cuda_calculate_responsabilities(real* slices, real* images, int* d_mask, re
     if (!proposal_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), &dense)) return;
     prop = !dense;
   }
-  if (!prop) run_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), d_respons, nullptr, nullptr);
+  if (!prop) {
+    run_estep(N_images, N_slices, sigma, e.d_phi_used.as<float>(), d_respons, nullptr, nullptr);
+    e.resp_nz.valid = false;
+  }
   g_respons_epoch++;
   auto& lk = e.like;
   lk.valid = !failed();
@@ -466,6 +480,7 @@ void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images
     lk.total = total;
     return;
   }
+  e.resp_nz.valid = false;  // normalised in place below
   const bool fuse = lk.valid && !lk.normalized && lk.respons == d_respons && lk.N == N_images && lk.R == N_slices &&
                     e.d_A.p && e.d_B.p;
   const int chunks = softmax_chunks(N_images, N_slices);
@@ -680,6 +695,13 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
                               e.d_pcol.as<int>(), e.d_pval.as<float>(), e.d_T.as<double>(), e.d_Tf.as<float>(),
                               poisson ? e.d_Tphi.as<float>() : nullptr, e.stream))
             return 0.0f;
+          // the keys list every nonzero the proposal iteration wrote to d_respons
+          e.resp_nz.ptr = d_respons;
+          e.resp_nz.N = N;
+          e.resp_nz.R = R;
+          e.resp_nz.nnz = total;
+          e.resp_nz.stamp = e.invalidate_stamp;
+          e.resp_nz.valid = true;
         } else {
           launch_p_row_fill(d_respons, R, N, opscale, e.d_poff.as<int>(), e.d_pcol.as<int>(), e.d_pval.as<float>(),
                             e.stream);
@@ -957,6 +979,14 @@ int emc_hip_set_option(const char* key, const char* value) {
   } else if (is(key, "likelihood")) {
     if (is(value, "gaussian")) e.likelihood = LIKE_GAUSSIAN;
     else if (is(value, "poisson")) e.likelihood = LIKE_POISSON;
+    else return 1;
+  } else if (is(key, "gemm_tail")) {
+    if (is(value, "split")) e.tail_split = true;
+    else if (is(value, "whole")) e.tail_split = false;
+    else return 1;
+  } else if (is(key, "respons_fill")) {
+    if (is(value, "delta")) e.respons_delta = true;
+    else if (is(value, "full")) e.respons_delta = false;
     else return 1;
   } else if (is(key, "sparse")) {
     if (is(value, "on")) e.sparse = true;

@@ -66,6 +66,8 @@ inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
 struct Engine {
   int device = -1;
+  int n_cu = 256;          // compute units of the device
+  DBuf d_tail_part;        // FP32 partial tiles of the split-k tail of the FP16 likelihood GEMM
   hipStream_t stream = nullptr;
 
   // options
@@ -75,6 +77,8 @@ struct Engine {
   int likelihood = LIKE_GAUSSIAN;
   bool sparse = false;  // Poisson E-step from CSR counts (see k_sparse.hip)
   int mstep_mode = MSTEP_AUTO;  // FP16 form: M-step over the nonzero responsibilities
+  bool respons_delta = true;    // proposal iterations rewrite only the changed responsibilities
+  bool tail_split = true;       // FP16 likelihood GEMM: last partial round of tiles split along k
   bool timing = false;
   bool verbose = false;
 
@@ -122,6 +126,11 @@ struct Engine {
   DBuf d_Sscale;        // float [R]: 1/scale_r, where the FP16 operand is S * scale_r (power of two)
   DBuf d_A;             // double [R]: sum_p m S^2 (Poisson: sum_p m S)
   DBuf d_LSt;           // half2 [n_mp][R_pad]: (hi, lo) transposed (sparse mode)
+  DBuf d_kidx;          // int [N2d]: packed index of each masked-in pixel (inverse of d_pix), -1 elsewhere
+  DBuf d_redo;          // int [R_pad]: rows the provisional-scale pack must redo
+  int sscale_R = -1;    // rotations whose row scales d_Sscale holds (-1: none)
+  uint64_t s16_gen = 0;  // bumped whenever d_S16 / d_S16lo are written
+  uint64_t lst_gen = 0;  // s16_gen of the slices d_LSt holds (built when the CSR E-step needs it)
 
   // ---- likelihood -> softmax -> scaling fusion state ----
   struct {
@@ -158,11 +167,25 @@ struct Engine {
   long p_nnz = -1;      // nonzeros of the last M-step (-1: dense form)
   std::vector<int> h_pnnz, h_poff;
   DBuf d_nz_keys, d_nz_keys_s, d_nz_vals, d_nz_vals_s, d_nz_tail, d_nz_tmp;  // nonzero lists from sources
+  // What a responsibility buffer holds, as far as this library wrote it:
+  // when valid, every nonzero of `ptr` ([R][N]) is at one of the nnz
+  // sorted keys r * N + i in d_nz_keys_s (the last M-step's lists, built
+  // from the posterior the library wrote there), zeros elsewhere.  The next
+  // proposal iteration then zeroes those entries instead of the whole matrix.
+  struct {
+    const float* ptr = nullptr;
+    int N = 0, R = 0;
+    long nnz = 0;
+    uint64_t stamp = 0;
+    bool valid = false;
+  } resp_nz;
   // long rows of the list M-step: segments (first, end nonzero), rows,
   // their first segment, partial sums [segments][n_mp]
   std::vector<int2> h_sp_seg;
   std::vector<int> h_sp_rows, h_sp_rowseg, h_sp_light, h_sp_zero;
   DBuf d_sp_seg, d_sp_rows, d_sp_rowseg, d_sp_part, d_sp_light, d_sp_zero;
+  DBuf d_mask4;          // int [N2d / 4]: bit q set when pixel 4v + q is masked in (compact pixel list pix_gen)
+  uint64_t mask4_gen = 0;
   DBuf d_Sc;         // float [R_pad][n_mp]: updated compact slices for insertion
   DBuf d_model_brick;  // float [S^3]: the model in 4 x 4 x 4 bricks for the expand's gathers
   DBuf d_wts;        // float [R]: rotation weights (uploaded per call)
@@ -230,10 +253,12 @@ void launch_images_stats(const float* images, const int* pix, int N, int N2d, in
 void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, float* S32, double* A, hipStream_t st, bool poisson = false);
 bool expand_pack_fits(int S, int n_mp);
+// kidx: packed index of every pixel (-1: masked out); n_m masked pixels,
+// packed entries n_m .. n_mp - 1 are padding
 void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, int R_pad, const float* x,
-                            const float* y, const float* z, float* slices, const int* pix, int n_mp, _Float16* S16,
-                            _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st,
-                            bool bricked = false);
+                            const float* y, const float* z, float* slices, const int* pix, const int* kidx, int n_m,
+                            int n_mp, _Float16* S16, _Float16* S16lo, float* scale, double* A, int interp,
+                            bool poisson, hipStream_t st, bool bricked = false);
 void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d, int n_mp,
                             int R_pad, _Float16* S16, _Float16* S16lo, float* scale, double* A,
                             hipStream_t st, bool poisson = false);

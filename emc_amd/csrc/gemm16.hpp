@@ -44,10 +44,11 @@ struct Operands {
 };
 
 // Main loop.  WGM x WGN waves, each TM x TN blocks of 16x16; STAGES LDS
-// buffers.  On return acc holds this wave's block of C.
+// buffers; nk steps of BK along k from k = kofs.  On return acc holds this
+// wave's block of C.
 template <int TM, int TN, int WGM, int WGN, int NA, int NB, int STAGES>
 __device__ __forceinline__ void dl_mma_loop(const Operands& op, int m0, int n0, int nk, floatx4 (&acc)[TM][TN],
-                                            _Float16* lds) {
+                                            _Float16* lds, int kofs = 0) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN, NW = WGM * WGN;
   constexpr int ROWS = NA * BM + NB * BN;  // rows of 32 halves per stage
   constexpr int STAGE_HALVES = ROWS * 32;
@@ -79,7 +80,7 @@ __device__ __forceinline__ void dl_mma_loop(const Operands& op, int m0, int n0, 
         p = op.b[s] + (size_t)(n0 + n) * op.ldb;
       }
     }
-    src[j] = p + c * 8;
+    src[j] = p + c * 8 + kofs;
   }
   auto issue = [&](int stage, int k0) {
 #pragma unroll

@@ -15,12 +15,17 @@ typedef float floatx4_t __attribute__((ext_vector_type(4)));
 // robin over the 8 XCDs, each with its own L2; remap so that every XCD works
 // on a contiguous range of tiles, and walk that range in groups of 8 tile
 // rows so neighbouring workgroups share operand panels in L2.
-__device__ __forceinline__ void tile_of_block(int tiles_m, int tiles_n, int& tm, int& tn) {
-  const int nt = tiles_m * tiles_n;
+// Logical tile of block blockIdx.x among nt blocks: XCD x takes the
+// consecutive logical tiles from x * nt / 8 on.
+__device__ __forceinline__ int logical_of_block(int nt) {
   const int bid = blockIdx.x;
   const int xcd = bid & 7, slot = bid >> 3;
   const int q = nt >> 3, rem = nt & 7;
-  const int logical = xcd * q + min(xcd, rem) + slot;
+  return xcd * q + min(xcd, rem) + slot;
+}
+
+// (tm, tn) of a logical tile: groups of 8 row tiles walked column by column.
+__device__ __forceinline__ void tile_of_logical(int logical, int tiles_m, int tiles_n, int& tm, int& tn) {
   constexpr int G = 8;
   const int per_group = G * tiles_n;
   const int group = logical / per_group;
@@ -29,6 +34,10 @@ __device__ __forceinline__ void tile_of_block(int tiles_m, int tiles_n, int& tm,
   const int in_group = logical - group * per_group;
   tm = first_m + in_group % gsize;
   tn = in_group / gsize;
+}
+
+__device__ __forceinline__ void tile_of_block(int tiles_m, int tiles_n, int& tm, int& tn) {
+  tile_of_logical(logical_of_block(tiles_m * tiles_n), tiles_m, tiles_n, tm, tn);
 }
 
 // Likelihood epilogue for one lane block: rows i0..i0+3 (patterns), column r

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
+#include "gemm_common.hpp"
 #include "geom.hpp"
 #include "reduce.hpp"
 
@@ -45,7 +46,10 @@ void launch_brick_model(const float* model, int S, float* out, hipStream_t st) {
   EMC_LAUNCHED();
 }
 
-// One workgroup per rotation; model reads are gathers served by L2.
+// One workgroup per rotation; model reads are gathers served by L2.  Each
+// XCD takes a contiguous range of rotations (logical_of_block): neighbouring
+// table entries are nearby orientations whose slices gather overlapping
+// model bricks, so that XCD's L2 serves them.
 template <bool BRICK>
 __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict__ model, int S,
                                                         const float* __restrict__ rot,
@@ -53,7 +57,7 @@ __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict_
                                                         const float* __restrict__ y,
                                                         const float* __restrict__ z,
                                                         float* __restrict__ slices) {
-  const int r = blockIdx.x;
+  const int r = logical_of_block(gridDim.x);
   float e[9];
   rot_matrix(&rot[4 * (size_t)r], e);
   const float half = (float)S / 2.0f + 0.5f;
@@ -80,7 +84,7 @@ __global__ __launch_bounds__(256) void k_expand_trilinear(const float* __restric
                                                           const float* __restrict__ y,
                                                           const float* __restrict__ z,
                                                           float* __restrict__ slices) {
-  const int r = blockIdx.x;
+  const int r = logical_of_block(gridDim.x);
   float e[9];
   rot_matrix(&rot[4 * (size_t)r], e);
   const float half = (float)S / 2.0f + 0.5f;
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
                                                          float* __restrict__ scale, double* __restrict__ A,
                                                          int poisson) {
   extern __shared__ float sv[];
-  const int r = blockIdx.x;
+  const int r = logical_of_block(gridDim.x);
   const bool live = r < R;
   const int N2d = S * S;
   if (live) {
@@ -393,15 +397,149 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
   }
 }
 
-bool expand_pack_fits(int S, int n_mp) { return (size_t)S * S * 4 <= 65536 && n_mp <= kPackThreads * kPackPer; }
+// Slices too large for LDS: the row scale is not known until the whole
+// slice is computed, so the pair is written with a provisional scale (the
+// rotation's scale from the previous pack, prov_inv = 1 / scale, 0 when
+// there is none) and the row is flagged when its own scale differs;
+// k_pack_fix then repacks the flagged rows from the FP32 slice.  The pairs
+// are the separate pack's either way (sc is a power of two); A_r is summed
+// over pixels in row order.  Pixels in row order: a wave's 64 packed
+// entries are then consecutive (whole 128-byte lines of the pair).
+template <bool TRI, bool BRICK>
+__global__ __launch_bounds__(256) void k_expand_pack_prov(const float* __restrict__ model, int S,
+                                                          const float* __restrict__ rot,
+                                                          const float* __restrict__ x, const float* __restrict__ y,
+                                                          const float* __restrict__ z, float* __restrict__ slices,
+                                                          const int* __restrict__ kidx, int R, int n_m, int n_mp,
+                                                          const float* __restrict__ prov_inv,
+                                                          _Float16* __restrict__ S16, _Float16* __restrict__ S16lo,
+                                                          float* __restrict__ scale, double* __restrict__ A,
+                                                          int* __restrict__ redo, int poisson) {
+  const int r = logical_of_block(gridDim.x);
+  const bool live = r < R;
+  const int N2d = S * S;
+  _Float16* hrow = S16 + (size_t)r * n_mp;
+  _Float16* lrow = S16lo + (size_t)r * n_mp;
+  for (int k = n_m + threadIdx.x; k < n_mp; k += 256) {
+    hrow[k] = (_Float16)0.0f;
+    lrow[k] = (_Float16)0.0f;
+  }
+  if (!live) {
+    for (int k = threadIdx.x; k < n_m; k += 256) {
+      hrow[k] = (_Float16)0.0f;
+      lrow[k] = (_Float16)0.0f;
+    }
+    return;
+  }
+  const float pv = prov_inv ? prov_inv[r] : 0.0f;
+  const float sc0 = pv > 0.0f ? 1.0f / pv : 1.0f;  // exact: pv is a power of two
+  float e[9];
+  rot_matrix(&rot[4 * (size_t)r], e);
+  const float half = (float)S / 2.0f + 0.5f;
+  float mx = 0.0f;
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < N2d; p += 256) {
+    float ux, uy, uz;
+    voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+    float v = 0.0f;
+    if (TRI) {
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        int ix, iy, iz;
+        float w;
+        if (tri_corner_xyz(c, ux, uy, uz, S, ix, iy, iz, w)) v += w * model_at<BRICK>(model, ix, iy, iz, S);
+      }
+    } else {
+      const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
+      if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S) v = model_at<BRICK>(model, vx, vy, vz, S);
+    }
+    slices[(size_t)r * N2d + p] = v;
+    const int k = kidx[p];
+    if (k >= 0) {
+      const float o = slice_operand(v, poisson);
+      mx = fmaxf(mx, fabsf(o));
+      acc += poisson ? (double)v : (double)v * v;
+      const float sx = o * sc0;
+      const _Float16 hi = (_Float16)sx;
+      hrow[k] = hi;
+      lrow[k] = (_Float16)(sx - (float)hi);
+    }
+  }
+  mx = block_max_f32(mx);
+  float sc = 1.0f;
+  if (mx > 0.0f && mx < 3.0e38f) {
+    int ex;
+    frexpf(mx, &ex);  // mx < 2^ex
+    sc = ldexpf(1.0f, max(-120, min(120, 14 - ex)));
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0) {
+    A[r] = acc;
+    scale[r] = 1.0f / sc;
+    redo[r] = sc != sc0;
+  }
+}
+
+// Rows whose provisional scale was not theirs: the pair again from the FP32
+// slice with the row's own scale.
+__global__ __launch_bounds__(256) void k_pack_fix(const float* __restrict__ slices, const int* __restrict__ pix,
+                                                  const int* __restrict__ redo, int R, int N2d, int n_m, int n_mp,
+                                                  const float* __restrict__ scale_inv, _Float16* __restrict__ S16,
+                                                  _Float16* __restrict__ S16lo, int poisson) {
+  const int r = blockIdx.x;
+  if (r >= R || !redo[r]) return;
+  const float sc = 1.0f / scale_inv[r];  // exact: a power of two
+  for (int k = threadIdx.x; k < n_m; k += 256) {
+    const float sx = slice_operand(slices[(size_t)r * N2d + pix[k]], poisson) * sc;
+    const _Float16 hi = (_Float16)sx;
+    S16[(size_t)r * n_mp + k] = hi;
+    S16lo[(size_t)r * n_mp + k] = (_Float16)(sx - (float)hi);
+  }
+}
+
+static bool expand_pack_lds(int S, int n_mp) {
+  return (size_t)S * S * 4 <= 65536 && n_mp <= kPackThreads * kPackPer;
+}
+
+bool expand_pack_fits(int S, int n_mp) {
+  (void)S;
+  (void)n_mp;
+  return true;  // the LDS form, else the provisional-scale form
+}
 
 void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, int R_pad, const float* x,
-                            const float* y, const float* z, float* slices, const int* pix, int n_mp, _Float16* S16,
-                            _Float16* S16lo, float* scale, double* A, int interp, bool poisson, hipStream_t st,
-                            bool bricked) {
+                            const float* y, const float* z, float* slices, const int* pix, const int* kidx, int n_m,
+                            int n_mp, _Float16* S16, _Float16* S16lo, float* scale, double* A, int interp,
+                            bool poisson, hipStream_t st, bool bricked) {
   const size_t lds = (size_t)S * S * sizeof(float);
   const dim3 g(R_pad), b(kPackThreads);
   const int po = (int)poisson;
+  if (!expand_pack_lds(S, n_mp)) {
+    Engine& e = engine();
+    if (!e.d_redo.ensure(sizeof(int) * (size_t)R_pad)) return;
+    // the previous pack's row scales, if they were for these rotations
+    const float* prov = e.sscale_R == R ? scale : nullptr;
+    const bool tri = interp == INTERP_TRILINEAR;
+    const dim3 gb(256);
+    int* redo = e.d_redo.as<int>();
+    if (tri && bricked)
+      hipLaunchKernelGGL((k_expand_pack_prov<true, true>), g, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R, n_m,
+                         n_mp, prov, S16, S16lo, scale, A, redo, po);
+    else if (tri)
+      hipLaunchKernelGGL((k_expand_pack_prov<true, false>), g, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R, n_m,
+                         n_mp, prov, S16, S16lo, scale, A, redo, po);
+    else if (bricked)
+      hipLaunchKernelGGL((k_expand_pack_prov<false, true>), g, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
+                         n_m, n_mp, prov, S16, S16lo, scale, A, redo, po);
+    else
+      hipLaunchKernelGGL((k_expand_pack_prov<false, false>), g, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
+                         n_m, n_mp, prov, S16, S16lo, scale, A, redo, po);
+    EMC_LAUNCHED();
+    hipLaunchKernelGGL(k_pack_fix, dim3(R), gb, 0, st, slices, pix, redo, R, S * S, n_m, n_mp, scale, S16, S16lo, po);
+    EMC_LAUNCHED();
+    e.sscale_R = R;
+    return;
+  }
   if (interp == INTERP_TRILINEAR) {
     if (bricked)
       hipLaunchKernelGGL((k_expand_pack_f16<true, true>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
@@ -418,6 +556,7 @@ void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, 
                          S16, S16lo, scale, A, po);
   }
   EMC_LAUNCHED();
+  engine().sscale_R = R;
 }
 
 void launch_pack_slices_f32(const float* slices, const int* pix, int R, int N2d, int n_mp,
@@ -433,6 +572,7 @@ void launch_pack_slices_f16(const float* slices, const int* pix, int R, int N2d,
   hipLaunchKernelGGL(k_pack_slices_f16, dim3(R_pad), dim3(kPackThreads), 0, st, slices, pix, R, N2d, n_mp, S16,
                      S16lo, scale, A, (int)poisson);
   EMC_LAUNCHED();
+  engine().sscale_R = R;
 }
 
 }  // namespace emc

@@ -15,6 +15,8 @@
 // reads of 16 consecutive rows hit distinct banks.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "engine.hpp"
 #include "gemm16.hpp"
 #include "gemm_common.hpp"
@@ -26,16 +28,46 @@ namespace emc {
 // rotations).  KSPLIT: counts not exact in FP16, A carried hi/lo too (then
 // two stages to stay within LDS).
 // ---------------------------------------------------------------------------
+// Tiles: the first n_main blocks take one whole tile each (XCD-aware order);
+// the last tiles_m * tiles_n - n_main tiles (the tail, fewer than the device
+// has compute units) are each split into `split` ranges along k, one block
+// per range, whose FP32 sums go to `part`; k_estep16_tail adds them in range
+// order and applies the epilogue.  One workgroup fills a compute unit (LDS),
+// so without the split a tail of a few tiles would take a whole tile time
+// on a few units while the rest of the device idles.
+constexpr int kE16TM = 8, kE16TN = 4, kE16WGM = 2, kE16WGN = 4;
+
 template <bool KSPLIT, bool PC>
-__global__ __launch_bounds__(512, 1) void k_estep16(Operands op, int n_mp, int tiles_m, int tiles_n, LikeEpilogue ep) {
-  constexpr int TM = 8, TN = 4, WGM = 2, WGN = 4, NA = KSPLIT ? 2 : 1, NB = 2, STAGES = KSPLIT ? 2 : 3;
+__global__ __launch_bounds__(512, 1) void k_estep16(Operands op, int n_mp, int tiles_m, int tiles_n, int n_main,
+                                                    int split, float* __restrict__ part, LikeEpilogue ep) {
+  constexpr int TM = kE16TM, TN = kE16TN, WGM = kE16WGM, WGN = kE16WGN, NA = KSPLIT ? 2 : 1, NB = 2,
+                STAGES = KSPLIT ? 2 : 3;
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
-  int tm, tn;
-  tile_of_block(tiles_m, tiles_n, tm, tn);
+  int tm, tn, logical, range = -1, k0 = 0, k1 = n_mp / BK;
+  if ((int)blockIdx.x < n_main) {
+    logical = logical_of_block(n_main);
+  } else {
+    const int u = (int)blockIdx.x - n_main, nk = n_mp / BK;
+    logical = n_main + u / split;
+    range = u % split;
+    k0 = range * nk / split;
+    k1 = (range + 1) * nk / split;
+  }
+  tile_of_logical(logical, tiles_m, tiles_n, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   floatx4 acc[TM][TN];
-  dl_mma_loop<TM, TN, WGM, WGN, NA, NB, STAGES>(op, m0, n0, n_mp / BK, acc, lds);
+  dl_mma_loop<TM, TN, WGM, WGN, NA, NB, STAGES>(op, m0, n0, k1 - k0, acc, lds, k0 * BK);
+  if (range >= 0) {
+    // [tail tile][range][x][y][thread] float4, coalesced over threads
+    float* out = part + ((size_t)(logical - n_main) * split + range) * (TM * TN * 512 * 4);
+#pragma unroll
+    for (int x = 0; x < TM; x++)
+#pragma unroll
+      for (int y = 0; y < TN; y++)
+        reinterpret_cast<floatx4*>(out)[(x * TN + y) * 512 + threadIdx.x] = acc[x][y];
+    return;
+  }
 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w / WGN, wn = w % WGN;
   const int g = lane >> 4, fr = lane & 15;
@@ -62,6 +94,27 @@ __global__ __launch_bounds__(512, 1) void k_estep16(Operands op, int n_mp, int t
       like_pc_flush(ep, i0, v);
     }
   }
+}
+
+// The split tail tiles: one block per tile, ranges added in order.
+__global__ __launch_bounds__(512) void k_estep16_tail(const float* __restrict__ part, int tiles_m, int tiles_n,
+                                                      int n_main, int split, LikeEpilogue ep) {
+  constexpr int TM = kE16TM, TN = kE16TN, WGM = kE16WGM, WGN = kE16WGN;
+  const int logical = n_main + (int)blockIdx.x;
+  int tm, tn;
+  tile_of_logical(logical, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * 16 * WGM * TM, n0 = tn * 16 * WGN * TN;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w / WGN, wn = w % WGN;
+  const int g = lane >> 4, fr = lane & 15;
+  const floatx4* in = reinterpret_cast<const floatx4*>(part) + (size_t)blockIdx.x * split * (TM * TN * 512);
+#pragma unroll
+  for (int x = 0; x < TM; x++)
+#pragma unroll
+    for (int y = 0; y < TN; y++) {
+      floatx4 acc = in[(x * TN + y) * 512 + threadIdx.x];
+      for (int r = 1; r < split; r++) acc += in[((size_t)r * TM * TN + x * TN + y) * 512 + threadIdx.x];
+      like_store(ep, m0 + wm * 16 * TM + x * 16 + 4 * g, n0 + wn * 16 * TN + y * 16 + fr, acc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -126,8 +179,7 @@ void launch_convert_p(const MstepArgs& a, _Float16* hi, _Float16* lo, hipStream_
 
 void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, const void* Slo, int N_pad,
                     int R_pad, int n_mp, const LikeEpilogue& ep, hipStream_t st) {
-  const int tiles_m = N_pad / kTile16, tiles_n = R_pad / kTile16;
-  const dim3 grid(tiles_m * tiles_n), block(512);
+  const int tiles_m = N_pad / kTile16, tiles_n = R_pad / kTile16, nt = tiles_m * tiles_n;
   Operands op{};
   op.a[0] = (const _Float16*)K;
   op.a[1] = (const _Float16*)Klo;
@@ -135,14 +187,37 @@ void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, 
   op.b[1] = (const _Float16*)Slo;
   op.lda = op.ldb = (size_t)n_mp;
   const bool pc = ep.P != nullptr;
+  // one workgroup per compute unit: a last round of at most half the units
+  // runs its tiles split along k over all of them
+  Engine& e = engine();
+  const int rounds_full = nt / e.n_cu, tail = nt - rounds_full * e.n_cu, nk = n_mp / BK;
+  int split = 1, n_main = nt;
+  if (e.tail_split && !pc && rounds_full > 0 && tail > 0 && 2 * tail <= e.n_cu) {
+    split = std::min(std::min(16, e.n_cu / tail), nk);
+    if (split > 1 &&
+        e.d_tail_part.ensure(sizeof(float) * (size_t)tail * split * kTile16 * kTile16))
+      n_main = nt - tail;
+    else
+      split = 1;
+  }
+  float* part = e.d_tail_part.as<float>();
+  const dim3 grid(n_main + (nt - n_main) * split), block(512);
   if (ksplit) {
-    if (pc) hipLaunchKernelGGL((k_estep16<true, true>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, ep);
-    else hipLaunchKernelGGL((k_estep16<true, false>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, ep);
+    if (pc) hipLaunchKernelGGL((k_estep16<true, true>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, n_main, split,
+                               part, ep);
+    else hipLaunchKernelGGL((k_estep16<true, false>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, n_main, split,
+                            part, ep);
   } else {
-    if (pc) hipLaunchKernelGGL((k_estep16<false, true>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, ep);
-    else hipLaunchKernelGGL((k_estep16<false, false>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, ep);
+    if (pc) hipLaunchKernelGGL((k_estep16<false, true>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, n_main, split,
+                               part, ep);
+    else hipLaunchKernelGGL((k_estep16<false, false>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, n_main, split,
+                            part, ep);
   }
   EMC_LAUNCHED();
+  if (n_main < nt) {
+    hipLaunchKernelGGL(k_estep16_tail, dim3(nt - n_main), dim3(512), 0, st, part, tiles_m, tiles_n, n_main, split, ep);
+    EMC_LAUNCHED();
+  }
 }
 
 void launch_mstep16(bool ksplit, const void* Phi, const void* Plo, const void* Kt, const void* Ktlo,

@@ -274,6 +274,18 @@ __global__ __launch_bounds__(256) void k_nz_rowsums(const int* __restrict__ off,
   }
 }
 
+__global__ __launch_bounds__(256) void k_zero_keys(const unsigned* __restrict__ keys, long n,
+                                                   float* __restrict__ respons) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) respons[keys[t]] = 0.0f;
+}
+
+void launch_zero_keys(const unsigned* keys, long n, float* respons, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n, respons);
+  EMC_LAUNCHED();
+}
+
 bool build_nz_lists(const NzSources& a, long nnz, const int* off, const float* opscale, const float* phi,
                     DBuf& keys, DBuf& keys_sorted, DBuf& vals, DBuf& vals_sorted, DBuf& tail, DBuf& temp, int* col,
                     float* val, double* T, float* Tf, float* Tphi, hipStream_t st) {

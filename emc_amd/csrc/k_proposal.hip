@@ -356,14 +356,19 @@ void launch_fill_subset(float* respons, int R, int N, const int* colmap, const f
 
 // respons[r][sub[k]] = Psub[r][k] for every rotation r (one block per r).
 __global__ __launch_bounds__(256) void k_sub_scatter(const float* __restrict__ Psub, int n_sub,
-                                                     const int* __restrict__ sub, int N, float* __restrict__ respons) {
+                                                     const int* __restrict__ sub, int N, int nonzero_only,
+                                                     float* __restrict__ respons) {
   const int r = blockIdx.x;
-  for (int k = threadIdx.x; k < n_sub; k += 256) respons[(size_t)r * N + sub[k]] = Psub[(size_t)r * n_sub + k];
+  for (int k = threadIdx.x; k < n_sub; k += 256) {
+    const float v = Psub[(size_t)r * n_sub + k];
+    if (!nonzero_only || v != 0.0f) respons[(size_t)r * N + sub[k]] = v;
+  }
 }
 
-void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int N, float* respons, hipStream_t st) {
+void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int N, float* respons, bool nonzero_only,
+                        hipStream_t st) {
   if (n_sub <= 0) return;
-  hipLaunchKernelGGL(k_sub_scatter, dim3(R), dim3(256), 0, st, Psub, n_sub, sub, N, respons);
+  hipLaunchKernelGGL(k_sub_scatter, dim3(R), dim3(256), 0, st, Psub, n_sub, sub, N, (int)nonzero_only, respons);
   EMC_LAUNCHED();
 }
 
@@ -717,7 +722,6 @@ __global__ __launch_bounds__(256) void k_cand_topcells(const float* __restrict__
 // (lanes over consecutive subset columns, so every row read is coalesced),
 // in two passes: the 4 largest of every group of 64 cells, then the ktop
 // largest of those (cells in increasing order, ties to the lower cell).
-constexpr int kTopGroup = 64;
 
 __device__ __forceinline__ void top4_insert(float m, int c, float (&lm)[4], int (&lc)[4]) {
   if (!(m > lm[3])) return;

@@ -248,6 +248,47 @@ __global__ __launch_bounds__(256) void k_mstep_zero(const int* __restrict__ rows
   for (int k = threadIdx.x; k < a.n_m; k += 256) __builtin_nontemporal_store(0.0f, out + a.pix[k]);
 }
 
+// the same with 16-byte stores: four pixels per lane, all four written at
+// once when all are masked in (m4: bit q for pixel 4v + q), else one by
+// one.  kZeroRows rows per workgroup share the mask words, loaded once.
+constexpr int kZeroRows = 4;
+
+__global__ __launch_bounds__(256) void k_mstep_zero4(const int* __restrict__ rows, int nrows,
+                                                     const int* __restrict__ m4, MstepArgs a) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const int nv = a.N2d / 4;
+  const f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int v0 = 0; v0 < nv; v0 += 256 * 4) {
+    int m[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int v = v0 + u * 256 + threadIdx.x;
+      m[u] = v < nv ? m4[v] : 0;
+    }
+    for (int h = 0; h < kZeroRows; h++) {
+      const int j = blockIdx.x * kZeroRows + h;
+      if (j >= nrows) break;
+      float* out = a.slices + (size_t)rows[j] * a.N2d;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int v = v0 + u * 256 + threadIdx.x;
+        if (m[u] == 15) {
+          reinterpret_cast<f4*>(out)[v] = z;
+        } else if (m[u] != 0) {
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (m[u] & (1 << q)) out[4 * v + q] = 0.0f;
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mask4(const int* __restrict__ pix, int n_m, int* __restrict__ m4) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n_m) atomicOr(&m4[pix[k] >> 2], 1 << (pix[k] & 3));
+}
+
 // one segment (nonzeros seg.x .. seg.y - 1 of a long row) -> part[s][n_mp]
 __global__ __launch_bounds__(256) void k_mstep_sparse_part(const int2* __restrict__ seg, const int* __restrict__ col,
                                                            const float* __restrict__ val,
@@ -327,7 +368,19 @@ bool launch_mstep_sparse(const std::vector<int>& h_off, const int* off, const in
   }
   if (nz > 0) {
     EMC_CHECK(hipMemcpyAsync(e.d_sp_zero.p, e.h_sp_zero.data(), sizeof(int) * nz, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_mstep_zero, dim3(nz), dim3(256), 0, st, e.d_sp_zero.as<int>(), a);
+    const bool vec = a.N2d % 4 == 0 && (uintptr_t)a.slices % 16 == 0;
+    if (vec && e.mask4_gen != e.pix_gen) {
+      if (!e.d_mask4.ensure(sizeof(int) * (a.N2d / 4))) return false;
+      launch_fill(e.d_mask4.as<float>(), (size_t)(a.N2d / 4), 0.0f, st);
+      hipLaunchKernelGGL(k_mask4, dim3((a.n_m + 255) / 256), dim3(256), 0, st, a.pix, a.n_m, e.d_mask4.as<int>());
+      EMC_LAUNCHED();
+      e.mask4_gen = e.pix_gen;
+    }
+    if (vec)
+      hipLaunchKernelGGL(k_mstep_zero4, dim3((nz + kZeroRows - 1) / kZeroRows), dim3(256), 0, st,
+                         e.d_sp_zero.as<int>(), nz, e.d_mask4.as<int>(), a);
+    else
+      hipLaunchKernelGGL(k_mstep_zero, dim3(nz), dim3(256), 0, st, e.d_sp_zero.as<int>(), a);
     EMC_LAUNCHED();
   }
   const int ns = (int)e.h_sp_seg.size(), nh = (int)e.h_sp_rows.size();

@@ -43,14 +43,58 @@ void Proposal::release() {
   clear_shared();
 }
 
+// Pinned staging for the host -> device uploads of the list building: a
+// copy from pinned memory is queued without blocking the host (one from
+// pageable memory is staged by the runtime first), so consecutive uploads
+// and the kernels after them follow each other on the GPU.  The space is
+// reused after the next stream synchronisation.
+struct PinnedArena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+};
+static PinnedArena g_pin;
+
+static void sync_stream() {
+  EMC_CHECK(hipStreamSynchronize(engine().stream));
+  g_pin.used = 0;
+}
+
+static void* pinned_take(size_t bytes) {
+  bytes = (bytes + 255) / 256 * 256;
+  if (g_pin.used + bytes > g_pin.cap) {
+    sync_stream();  // every queued copy from the arena is done
+    if (bytes > g_pin.cap) {
+      if (g_pin.base) (void)hipHostFree(g_pin.base);
+      g_pin.base = nullptr;
+      g_pin.cap = 0;
+      const size_t cap = std::max<size_t>(bytes, 4u << 20);
+      if (hipHostMalloc((void**)&g_pin.base, cap, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        g_pin.base = nullptr;
+        return nullptr;
+      }
+      g_pin.cap = cap;
+    }
+  }
+  void* at = g_pin.base + g_pin.used;
+  g_pin.used += bytes;
+  return at;
+}
+
 template <class T>
 static bool upload(DBuf& d, const std::vector<T>& h, hipStream_t st) {
   if (!d.ensure(sizeof(T) * std::max<size_t>(h.size(), 1))) return false;
-  if (!h.empty()) EMC_CHECK(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+  if (h.empty()) return !failed();
+  const size_t bytes = sizeof(T) * h.size();
+  void* pin = pinned_take(bytes);
+  if (pin) {
+    std::memcpy(pin, h.data(), bytes);
+    EMC_CHECK(hipMemcpyAsync(d.p, pin, bytes, hipMemcpyHostToDevice, st));
+  } else {
+    EMC_CHECK(hipMemcpyAsync(d.p, h.data(), bytes, hipMemcpyHostToDevice, st));
+  }
   return !failed();
 }
-
-static void sync_stream() { EMC_CHECK(hipStreamSynchronize(engine().stream)); }
 
 // Entries of each cell (patterns ascending), E-step work items, 32-entry
 // chunks of the M-step, candidate rotation count of each pattern, from the
@@ -252,7 +296,7 @@ static bool ensure_iota(Proposal& p, int n) {
 static bool recentre(Proposal& p) {
   Engine& e = engine();
   hipStream_t st = e.stream;
-  const int N = p.N, K = Proposal::kTop, ng = (p.C + 63) / 64;
+  const int N = p.N, K = Proposal::kTop, ng = (p.C + kTopGroup - 1) / kTopGroup;
   if (!p.d_top.ensure(sizeof(int) * (size_t)N * K) || !p.d_topm.ensure(sizeof(float) * (size_t)N * K)) return false;
   if (p.post == Proposal::POST_CAND) {
     const size_t np = (size_t)std::max(p.n_sub, 1) * ng * 4;
@@ -264,7 +308,7 @@ static bool recentre(Proposal& p) {
     if (p.R_sh > 0) {
       // the shared cells of the last iteration: their top cells from the
       // shared responsibilities, merged with the candidates' top cells
-      const int gs = (p.n_shc + 63) / 64;
+      const int gs = (p.n_shc + kTopGroup - 1) / kTopGroup;
       const size_t ns = (size_t)N * gs * 4;
       if (!ensure_iota(p, std::max(N, p.R_sh)) || !p.d_part_c.ensure(std::max(p.d_part_c.bytes, sizeof(int) * ns)) ||
           !p.d_part_m.ensure(std::max(p.d_part_m.bytes, sizeof(float) * ns)) ||
@@ -538,7 +582,14 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   Proposal& p = proposal();
   hipStream_t st = e.stream;
   if (!p.d_tot.ensure(sizeof(double) * 2)) return false;
-  const bool vec_fill = N % 4 == 0 && (uintptr_t)respons % 16 == 0 && p.n_sub > 0;
+  // The matrix still holds the last proposal iteration's posterior, whose
+  // nonzeros the M-step listed: zero those entries and write the subset's
+  // nonzeros, instead of rewriting all N * R entries.
+  auto& rec = e.resp_nz;
+  const bool delta = e.respons_delta && rec.valid && rec.ptr == respons && rec.N == N && rec.R == R &&
+                     rec.stamp == e.invalidate_stamp && e.d_nz_keys_s.bytes >= sizeof(unsigned) * (size_t)rec.nnz;
+  rec.valid = false;  // the keys describe the contents until this call writes them
+  const bool vec_fill = !delta && N % 4 == 0 && (uintptr_t)respons % 16 == 0 && p.n_sub > 0;
   if (vec_fill) {
     p.h_colmap.assign(N, -1);
     for (int k = 0; k < p.n_sub; k++) p.h_colmap[p.h_sub[k]] = k;
@@ -557,8 +608,15 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
     PhaseTimer pt(PH_SOFTMAX);
     // zero, with the whole-table subset's columns written in the same pass
     // when the rows allow 16-byte stores
-    if (vec_fill) launch_fill_subset(respons, R, N, p.d_colmap.as<int>(), p.d_Lsub.as<float>(), p.n_sub, st);
-    else launch_fill(respons, (size_t)N * R, 0.0f, st);
+    p.st_cleared = delta ? (double)rec.nnz : (double)N * R;
+    if (delta) {
+      launch_zero_keys(e.d_nz_keys_s.as<unsigned>(), rec.nnz, respons, st);
+      launch_sub_scatter(p.d_Lsub.as<float>(), R, p.n_sub, p.d_sub.as<int>(), N, respons, true, st);
+    } else if (vec_fill) {
+      launch_fill_subset(respons, R, N, p.d_colmap.as<int>(), p.d_Lsub.as<float>(), p.n_sub, st);
+    } else {
+      launch_fill(respons, (size_t)N * R, 0.0f, st);
+    }
     CandSoftmaxArgs a{};
     a.Lc = p.d_Lc.as<float>();
     a.cand = p.d_cand.as<int>();
@@ -602,10 +660,10 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
       EMC_CHECK(hipMemcpyAsync(p.d_tot.p, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToDevice, st));
     }
   }
-  if (p.n_sub > 0 && !vec_fill) {
+  if (p.n_sub > 0 && !vec_fill && !delta) {
     // the subset's posterior (computed with the E-step) into its columns
     PhaseTimer pt(PH_AUDIT);
-    launch_sub_scatter(p.d_Lsub.as<float>(), R, p.n_sub, p.d_sub.as<int>(), N, respons, st);
+    launch_sub_scatter(p.d_Lsub.as<float>(), R, p.n_sub, p.d_sub.as<int>(), N, respons, false, st);
   }
   double tot = 0.0;
   EMC_CHECK(hipMemcpyAsync(&tot, p.d_tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -657,7 +715,7 @@ bool proposal_dense_posterior(const float* respons, int N, int R) {
   Engine& e = engine();
   Proposal& p = proposal();
   hipStream_t st = e.stream;
-  const int K = Proposal::kTop, ng = (p.C + 63) / 64;
+  const int K = Proposal::kTop, ng = (p.C + kTopGroup - 1) / kTopGroup;
   p.post = Proposal::POST_NONE;
   if (!ensure_iota(p, N)) return false;
   const size_t np = (size_t)N * ng * 4;
@@ -1130,13 +1188,15 @@ int emc_hip_proposal_shared(int* cells, int max_cells) {
 
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  const double v[20] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
-                        p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
-                        p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
-                        p.st_cand_pairs,     p.st_moved,     p.st_dense,      p.st_shared,
-                        p.st_recentre_ms,    p.st_group_pairs, p.st_resid_pairs, (double)p.R_sh};
+  constexpr int kStats = 21;
+  const double v[kStats] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
+                            p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
+                            p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
+                            p.st_cand_pairs,     p.st_moved,     p.st_dense,      p.st_shared,
+                            p.st_recentre_ms,    p.st_group_pairs, p.st_resid_pairs, (double)p.R_sh,
+                            p.st_cleared};
   int k = 0;
-  for (; k < n && k < 20; k++) out[k] = v[k];
+  for (; k < n && k < kStats; k++) out[k] = v[k];
   return k;
 }
 

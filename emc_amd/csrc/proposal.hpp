@@ -89,6 +89,7 @@ struct Proposal {
   int n_gwork = 0;
   double st_group_pairs = 0;         // (pattern, rotation) pairs of the group blocks
   double st_resid_pairs = 0;         // candidate pairs left to the per-cell kernel
+  double st_cleared = 0;             // responsibility entries cleared by the last write (N * R unless delta)
   DBuf d_rent_off, d_rent_pat, d_rent_j, d_rwork, d_rkeys_s, d_rvals_s;
   int n_rwork = 0;
   DBuf d_Lc;      // float [N][topk][32]: L, then P

@@ -133,13 +133,19 @@ void launch_shared_scatter(const float* Psh, int R_sh, int N, const int* sh_rot,
                            float* respons, hipStream_t st);
 void launch_merge_top(int N, int ktop, const unsigned char* insub, int* top, float* topm, const int* top_s,
                       const float* topm_s, const int* sh_cells, hipStream_t st);
+// Cells per partial of the whole-table top cells (one wave per 64 patterns
+// and kTopGroup cells; small groups give enough waves to fill the device).
+constexpr int kTopGroup = 16;
 // the ktop (<= 4) cells holding the most posterior mass of every pattern
 // (Pc == nullptr: the subset columns only); part_c / part_m:
-// n_sub * ceil(C / 64) * 4 scratch entries
+// n_sub * ceil(C / kTopGroup) * 4 scratch entries
 void launch_topcells(const float* Pc, int N, int topk, const unsigned char* insub, const int* cand,
                      const float* Psub, int n_sub, const int* sub, int C, const int* cell_off, const int* cell_rot,
                      int ktop, int* top, float* topm, int* part_c, float* part_m, hipStream_t st);
-void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int N, float* respons, hipStream_t st);
+// the subset's posterior Psub [R][n_sub] into its columns of respons
+// (nonzero_only: only its nonzero entries, into a matrix already zero there)
+void launch_sub_scatter(const float* Psub, int R, int n_sub, const int* sub, int N, float* respons, bool nonzero_only,
+                        hipStream_t st);
 // respons = 0 except the subset columns (colmap[i] >= 0: Psub column); N % 4 == 0, 16-byte aligned rows
 void launch_fill_subset(float* respons, int R, int N, const int* colmap, const float* Psub, int n_sub,
                         hipStream_t st);
@@ -190,5 +196,7 @@ void launch_nz_count(const NzSources& a, int* rownnz, hipStream_t st);
 bool build_nz_lists(const NzSources& a, long nnz, const int* off, const float* opscale, const float* phi,
                     DBuf& keys, DBuf& keys_sorted, DBuf& vals, DBuf& vals_sorted, DBuf& tail, DBuf& temp, int* col,
                     float* val, double* T, float* Tf, float* Tphi, hipStream_t st);
+// respons[keys[t]] = 0 for the n sorted keys r * N + i
+void launch_zero_keys(const unsigned* keys, long n, float* respons, hipStream_t st);
 
 }  // namespace emc

@@ -73,6 +73,21 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 nonzero P_ri (lists built from d_respons, P kept in FP32)
  *                 instead of the dense contraction; "dense", "sparse": always
  *                 that form.  The dropped terms are exact zeros.
+ *   "gemm_tail"  "split" (default): when the FP16 likelihood GEMM's tiles
+ *                 leave a last round that fills at most half of the compute
+ *                 units, those tiles are split along the pixel sum over all
+ *                 units (partial sums added in a fixed order: results are
+ *                 reproducible, and equal the unsplit ones to FP32
+ *                 rounding); "whole": every tile in one workgroup.
+ *   "respons_fill"  proposal mode.  "delta" (default): when d_respons still
+ *                 holds the previous proposal iteration's responsibilities,
+ *                 as this library wrote them and its M-step listed their
+ *                 nonzeros, an iteration zeroes only those entries before
+ *                 writing its own, instead of all N_images * N_slices.  A
+ *                 caller that writes d_respons by other means than this
+ *                 library must call emc_hip_invalidate() (or choose "full")
+ *                 before the next iteration.  "full": every iteration writes
+ *                 the whole matrix.  The contents are the same either way.
  *   "proposal"   "on": once candidates exist for this pattern count (see
  *                 the proposal workflow below) and precision is "fp16", the
  *                 E-step scores each pattern only against the fine rotations
@@ -109,8 +124,9 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
 int emc_hip_set_option(const char *key, const char *value);
 
 /* Invalidate every cached derivative of caller buffers (packed patterns,
- * mask index, insertion map).  Needed only if the caller writes images, the
- * mask, rotations or coordinates by means other than this library. */
+ * mask index, insertion map, the record of which responsibilities are
+ * nonzero).  Needed only if the caller writes images, the mask, rotations,
+ * coordinates or responsibilities by means other than this library. */
 void emc_hip_invalidate(void);
 
 /* Per-phase GPU time accumulated while "timing" is on, in milliseconds, and
@@ -195,7 +211,10 @@ int emc_hip_proposal_shared(int *cells, int max_cells);
  * the audit sent the iteration to the whole table, [15] shared cells, [16]
  * host time of the re-centring and list building (ms), [17] (pattern,
  * rotation) pairs of the group blocks, [18] candidate pairs left to the
- * per-cell kernel, [19] shared rotations.  Returns the number of values
+ * per-cell kernel, [19] shared rotations, [20] responsibility entries the
+ * last iteration cleared before writing its own (N_images * N_slices, or the
+ * previous iteration's nonzeros with "respons_fill" "delta").  Returns the
+ * number of values
  * written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 

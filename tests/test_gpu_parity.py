@@ -329,3 +329,68 @@ def test_expand_pack_fused_matches_separate(tiny, likelihood, interp):
         E.set_option("likelihood", "gaussian")
     assert np.array_equal(sl_fused, sl_sep)
     assert np.array_equal(L_fused, L_sep)
+
+
+@pytest.mark.parametrize("side,n_patterns,n_rot", [(32, 600, 8), (16, 300, 9)])
+def test_gemm_tail_split_matches_whole(side, n_patterns, n_rot):
+    """FP16 likelihood GEMM with more tiles than compute units and a last
+    round of at most half of them (600 x 25,680: 303 tiles; 300 x 36,540:
+    286 tiles, 8 k steps, one per range): the tail tiles split along the
+    pixel sum give the likelihood of the unsplit GEMM to FP32 rounding, and
+    not bit for bit (the split ran)."""
+    P = problems.make(side=side, n_patterns=n_patterns, n_rot=n_rot, n_blobs=20, radius=side / 5.0,
+                      photons=1000.0, beamstop=2.0, seed=5)
+    options("fp16")
+    out = {}
+    try:
+        for mode in ("whole", "split"):
+            E.set_option("gemm_tail", mode)
+            prob = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
+            prob.get_slices()
+            L = np.zeros((prob.R, prob.N), np.float32)
+            prob.likelihood(0.4, L)
+            prob.free()
+            E.check()
+            out[mode] = L
+    finally:
+        E.set_option("gemm_tail", "split")
+    w, s = out["whole"].astype(np.float64), out["split"].astype(np.float64)
+    d = float(np.abs(s - w).max() / np.abs(w).max())
+    print(f"{n_patterns} x {P.R}: max |L_split - L_whole| / max |L| = {d:.2e}, "
+          f"{int((s != w).sum())} of {w.size} differ")
+    assert d < 1e-6
+    assert (s != w).any()
+
+
+@pytest.mark.parametrize("likelihood,interp", [("gaussian", "nearest"), ("poisson", "trilinear")])
+def test_expand_pack_large_slices_match_separate(likelihood, interp):
+    """Slices too large for the LDS form (136 x 136): the expand writes the
+    FP16 pair with each rotation's previous row scale and repacks the rows
+    whose scale changed.  Over iterations from a random model (row scales
+    changing) the slices are bit-identical to the separate pack's and the
+    likelihood agrees to the rounding of A_r's summation order."""
+    P = problems.make(side=136, n_patterns=64, n_rot=2, n_blobs=20, radius=24.0, photons=3000.0, beamstop=4.0,
+                      seed=7)
+    options("fp16", "atomic", interp)
+    E.set_option("likelihood", likelihood)
+    try:
+        prob = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
+        for it in range(3):
+            prob.iteration()
+            prob.get_slices()  # expand + pack with the previous row scales
+            sl_f = prob.slices()
+            L_f = np.zeros((prob.R, prob.N), np.float32)
+            prob.likelihood(0.4, L_f)
+            E.lib().emc_hip_invalidate()  # pixel list rebuilt: the E-step packs again
+            prob.get_slices()
+            sl_s = prob.slices()
+            L_s = np.zeros((prob.R, prob.N), np.float32)
+            prob.likelihood(0.4, L_s)
+            E.check()
+            d = float(np.abs(L_f.astype(np.float64) - L_s).max() / np.abs(L_s).max())
+            print(f"iteration {it + 1}: slices equal {np.array_equal(sl_f, sl_s)}, max |dL| / max |L| = {d:.1e}")
+            assert np.array_equal(sl_f, sl_s)
+            assert d < 1e-6
+        prob.free()
+    finally:
+        E.set_option("likelihood", "gaussian")

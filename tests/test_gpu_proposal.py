@@ -16,7 +16,10 @@ C-ABI:
     iteration matches the FP64 oracle restricted to candidates + shared
     rotations (+ the whole table for the audited patterns);
   - scoring a group's common candidate cells as one dense block gives the
-    same iteration, bit for bit, as scoring every candidate cell alone.
+    same iteration, bit for bit, as scoring every candidate cell alone;
+  - clearing only the previous iteration's nonzero responsibilities before
+    writing the new ones leaves the same matrix, bit for bit, as rewriting
+    all of it, over consecutive iterations.
 """
 import ctypes as C
 
@@ -49,7 +52,7 @@ def _reset():
 
 
 def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", min_mass="0.999", gate="audit",
-         track="off", max_fail="1", mstep="auto", shared="0", group_min="0"):
+         track="off", max_fail="1", mstep="auto", shared="0", group_min="0", fill="delta"):
     E.set_option("precision", precision)
     E.set_option("insert", insert)
     E.set_option("interp", "nearest")
@@ -62,6 +65,7 @@ def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", mi
     E.set_option("mstep", mstep)
     E.set_option("proposal_shared", shared)
     E.set_option("proposal_group_min", group_min)
+    E.set_option("respons_fill", fill)
 
 
 def init_cells(P, topk, n_coarse=2, batch=64):
@@ -76,8 +80,8 @@ def init_cells(P, topk, n_coarse=2, batch=64):
 
 
 def stats():
-    out = (C.c_double * 20)()
-    assert E.lib().emc_hip_proposal_stats(out, 20) == 20
+    out = (C.c_double * 21)()
+    assert E.lib().emc_hip_proposal_stats(out, 21) == 21
     return list(out)
 
 
@@ -434,3 +438,41 @@ def test_group_blocks_match_per_cell(prob):
     ref = O.o64_proposal_iteration(P.model.astype(np.float64), P.images, P.mask, np.ones(P.N), P.side, P.q, P.w,
                                    P.x, P.y, P.z, allowed)
     assert np.abs(g1["P"] - ref["P"]).max() < 1e-4 and rel(g1["model"], ref["model"]) < 1e-4
+
+
+def test_delta_fill_matches_full(prob):
+    """Four proposal iterations (whole-table audit first, then 30%, shared
+    cells, re-centring, M-step lists from the posterior's sources): with
+    "respons_fill" "delta" iterations 2-4 clear only the previous iteration's
+    nonzeros (stat 20 < N * R), and every iteration's responsibilities,
+    scaling and model equal those of the whole-matrix rewrite bit for bit."""
+    P = prob
+    topk = 6
+    nc, off, rot = init_cells(P, topk)
+    cand, _ = oracle_candidates(P, off, rot, topk)
+    out = {}
+    for fill in ("full", "delta"):
+        assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
+        opts(proposal="on", audit="1", min_mass="0", track="on", shared="1000", mstep="sparse", fill=fill)
+        prob_ = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
+        res = []
+        for it in range(4):
+            if it == 1:
+                E.set_option("audit_fraction", "0.3")
+            prob_.iteration(0.4)
+            res.append({"P": prob_.respons(), "scaling": prob_.scaling(), "model": prob_.model(),
+                        "cleared": stats()[20], "nnz": E.lib().emc_hip_mstep_nnz()})
+        prob_.free()
+        E.check()
+        out[fill] = res
+    for it, (f, d) in enumerate(zip(out["full"], out["delta"])):
+        print(f"iteration {it + 1}: cleared full {f['cleared']:.0f} delta {d['cleared']:.0f}, "
+              f"M-step nonzeros {d['nnz']}, P nonzeros {(d['P'] != 0).sum()}")
+        assert f["cleared"] == P.N * P.R
+        if it == 0:
+            assert d["cleared"] == P.N * P.R
+        else:
+            assert d["cleared"] == out["delta"][it - 1]["nnz"] < P.N * P.R
+        assert d["nnz"] == (d["P"] != 0).sum()
+        for k in ("P", "scaling", "model"):
+            assert np.array_equal(f[k], d[k]), (it, k)
