@@ -174,14 +174,30 @@ bool sort_entries(const int* cand, int N, int topk, int C, DBuf& keys_sorted, DB
 // pass 2 appends (key = r * N + i, P) in any order; a radix sort by key then
 // gives every rotation's list in pattern order, independent of that order.
 // ---------------------------------------------------------------------------
-// mode 0: count into rownnz; mode 1: append to keys / vals at *tail
-__device__ __forceinline__ void nz_take(int mode, float p, int r, int i, int N, int* rownnz, unsigned* tail,
-                                        unsigned* keys, float* vals) {
-  if (p == 0.0f) return;
+// mode 0: count into rownnz; mode 1: append to keys / vals at *tail.  Called
+// by every lane of the wave (take = lane has an entry): one atomic per wave
+// for the append position, and for the count when r is the same for the
+// whole wave (r_uniform).  The append order is irrelevant: the keys are
+// unique and sorted afterwards.
+__device__ __forceinline__ void nz_take(int mode, bool take, float p, int r, int i, int N, bool r_uniform,
+                                        int* rownnz, unsigned* tail, unsigned* keys, float* vals) {
+  const bool nz = take && p != 0.0f;
+  const unsigned long long m = __ballot(nz);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1, cnt = __popcll(m);
   if (mode == 0) {
-    atomicAdd(&rownnz[r], 1);
-  } else {
-    const unsigned at = atomicAdd(tail, 1u);
+    if (!r_uniform) {
+      if (nz) atomicAdd(&rownnz[r], 1);
+    } else if (lane == leader) {
+      atomicAdd(&rownnz[r], cnt);
+    }
+    return;
+  }
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(tail, (unsigned)cnt);
+  base = __shfl(base, leader);
+  if (nz) {
+    const unsigned at = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
     keys[at] = (unsigned)r * (unsigned)N + (unsigned)i;
     vals[at] = p;
   }
@@ -190,31 +206,40 @@ __device__ __forceinline__ void nz_take(int mode, float p, int r, int i, int N, 
 __global__ __launch_bounds__(256) void k_nz_cand(NzSources a, int mode, int* rownnz, unsigned* tail, unsigned* keys,
                                                  float* vals) {
   const long t = (long)blockIdx.x * 256 + threadIdx.x;  // (i, j, slot)
-  if (t >= (long)a.N * a.topk * kSlot) return;
-  const int s = (int)(t % kSlot);
-  const long ij = t / kSlot;
-  const int i = (int)(ij / a.topk);
-  if (a.insub[i]) return;
-  const int c = a.cand[ij];
-  const int o = a.cell_off[c];
-  if (s >= a.cell_off[c + 1] - o) return;
-  nz_take(mode, a.Pc[t], a.cell_rot[o + s], i, a.N, rownnz, tail, keys, vals);
+  bool take = t < (long)a.N * a.topk * kSlot;
+  int r = 0, i = 0;
+  float p = 0.0f;
+  if (take) {
+    const int s = (int)(t % kSlot);
+    const long ij = t / kSlot;
+    i = (int)(ij / a.topk);
+    const int c = a.cand[ij];
+    const int o = a.cell_off[c];
+    take = !a.insub[i] && s < a.cell_off[c + 1] - o;
+    if (take) {
+      p = a.Pc[t];
+      r = a.cell_rot[o + s];
+    }
+  }
+  nz_take(mode, take, p, r, i, a.N, false, rownnz, tail, keys, vals);
 }
 
 __global__ __launch_bounds__(256) void k_nz_shared(NzSources a, int mode, int* rownnz, unsigned* tail,
                                                    unsigned* keys, float* vals) {
   const int rs = blockIdx.y;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.N || a.insub[i]) return;
-  nz_take(mode, a.Psh[(size_t)rs * a.N + i], a.sh_rot[rs], i, a.N, rownnz, tail, keys, vals);
+  const bool take = i < a.N && !a.insub[i];
+  const float p = take ? a.Psh[(size_t)rs * a.N + i] : 0.0f;
+  nz_take(mode, take, p, a.sh_rot[rs], i, a.N, true, rownnz, tail, keys, vals);
 }
 
 __global__ __launch_bounds__(256) void k_nz_sub(NzSources a, int mode, int* rownnz, unsigned* tail, unsigned* keys,
                                                 float* vals) {
   const int r = blockIdx.x;
   const int k = blockIdx.y * 256 + threadIdx.x;
-  if (k >= a.n_sub) return;
-  nz_take(mode, a.Psub[(size_t)r * a.n_sub + k], r, a.sub[k], a.N, rownnz, tail, keys, vals);
+  const bool take = k < a.n_sub;
+  nz_take(mode, take, take ? a.Psub[(size_t)r * a.n_sub + k] : 0.0f, r, take ? a.sub[k] : 0, a.N, true, rownnz,
+          tail, keys, vals);
 }
 
 static void nz_pass(const NzSources& a, int mode, int* rownnz, unsigned* tail, unsigned* keys, float* vals,
