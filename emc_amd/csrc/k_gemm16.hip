@@ -33,8 +33,9 @@ namespace emc {
 // has compute units) are each split into `split` ranges along k, one block
 // per range, whose FP32 sums go to `part`; k_estep16_tail adds them in range
 // order and applies the epilogue.  One workgroup fills a compute unit (LDS),
-// so without the split a tail of a few tiles would take a whole tile time
-// on a few units while the rest of the device idles.
+// so without the split a tail of a few tiles (or a launch of fewer tiles
+// than units) would take a whole tile time on a few units while the rest of
+// the device idles.
 constexpr int kE16TM = 8, kE16TN = 4, kE16WGM = 2, kE16WGN = 4;
 
 template <bool KSPLIT, bool PC>
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(512, 1) void k_estep16(Operands op, int n_mp, int t
   }
 }
 
-// The split tail tiles: one block per tile, ranges added in order.
+// The split tail tiles: blocks (tile, x), ranges added in order.
 __global__ __launch_bounds__(512) void k_estep16_tail(const float* __restrict__ part, int tiles_m, int tiles_n,
                                                       int n_main, int split, LikeEpilogue ep) {
   constexpr int TM = kE16TM, TN = kE16TN, WGM = kE16WGM, WGN = kE16WGN;
@@ -107,14 +108,13 @@ __global__ __launch_bounds__(512) void k_estep16_tail(const float* __restrict__ 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w / WGN, wn = w % WGN;
   const int g = lane >> 4, fr = lane & 15;
   const floatx4* in = reinterpret_cast<const floatx4*>(part) + (size_t)blockIdx.x * split * (TM * TN * 512);
+  const int x = blockIdx.y;
 #pragma unroll
-  for (int x = 0; x < TM; x++)
-#pragma unroll
-    for (int y = 0; y < TN; y++) {
-      floatx4 acc = in[(x * TN + y) * 512 + threadIdx.x];
-      for (int r = 1; r < split; r++) acc += in[((size_t)r * TM * TN + x * TN + y) * 512 + threadIdx.x];
-      like_store(ep, m0 + wm * 16 * TM + x * 16 + 4 * g, n0 + wn * 16 * TN + y * 16 + fr, acc);
-    }
+  for (int y = 0; y < TN; y++) {
+    floatx4 acc = in[(x * TN + y) * 512 + threadIdx.x];
+    for (int r = 1; r < split; r++) acc += in[((size_t)r * TM * TN + x * TN + y) * 512 + threadIdx.x];
+    like_store(ep, m0 + wm * 16 * TM + x * 16 + 4 * g, n0 + wn * 16 * TN + y * 16 + fr, acc);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -187,15 +187,23 @@ void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, 
   op.b[1] = (const _Float16*)Slo;
   op.lda = op.ldb = (size_t)n_mp;
   const bool pc = ep.P != nullptr;
-  // one workgroup per compute unit: a last round of at most half the units
-  // runs its tiles split along k over all of them
+  // one workgroup per compute unit: the tiles of the last, partial round
+  // (all of them when there are fewer tiles than units) are split along k
+  // into the number of ranges that finishes them soonest, ceil(tail * split /
+  // units) rounds of 1 / split of a tile each
   Engine& e = engine();
   const int rounds_full = nt / e.n_cu, tail = nt - rounds_full * e.n_cu, nk = n_mp / BK;
   int split = 1, n_main = nt;
-  if (e.tail_split && !pc && rounds_full > 0 && tail > 0 && 2 * tail <= e.n_cu) {
-    split = std::min(std::min(16, e.n_cu / tail), nk);
-    if (split > 1 &&
-        e.d_tail_part.ensure(sizeof(float) * (size_t)tail * split * kTile16 * kTile16))
+  if (e.tail_split && !pc && tail > 0) {
+    double best = 1.0;
+    for (int sp = 2; sp <= std::min(16, nk); sp++) {
+      const double t = (double)((tail * sp + e.n_cu - 1) / e.n_cu) / sp;
+      if (t < best - 1e-9) {
+        best = t;
+        split = sp;
+      }
+    }
+    if (split > 1 && e.d_tail_part.ensure(sizeof(float) * (size_t)tail * split * kTile16 * kTile16))
       n_main = nt - tail;
     else
       split = 1;
@@ -215,7 +223,8 @@ void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, 
   }
   EMC_LAUNCHED();
   if (n_main < nt) {
-    hipLaunchKernelGGL(k_estep16_tail, dim3(nt - n_main), dim3(512), 0, st, part, tiles_m, tiles_n, n_main, split, ep);
+    hipLaunchKernelGGL(k_estep16_tail, dim3(nt - n_main, kE16TM), dim3(512), 0, st, part, tiles_m, tiles_n, n_main,
+                       split, ep);
     EMC_LAUNCHED();
   }
 }

@@ -73,10 +73,11 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 nonzero P_ri (lists built from d_respons, P kept in FP32)
  *                 instead of the dense contraction; "dense", "sparse": always
  *                 that form.  The dropped terms are exact zeros.
- *   "gemm_tail"  "split" (default): when the FP16 likelihood GEMM's tiles
- *                 leave a last round that fills at most half of the compute
- *                 units, those tiles are split along the pixel sum over all
- *                 units (partial sums added in a fixed order: results are
+ *   "gemm_tail"  "split" (default): the FP16 likelihood GEMM's tiles that
+ *                 do not fill a whole round of the compute units (all of
+ *                 them when there are fewer tiles than units) are split along
+ *                 the pixel sum into as many ranges as finish them soonest
+ *                 (partial sums added in a fixed order: results are
  *                 reproducible, and equal the unsplit ones to FP32
  *                 rounding); "whole": every tile in one workgroup.
  *   "respons_fill"  proposal mode.  "delta" (default): when d_respons still

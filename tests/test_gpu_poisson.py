@@ -26,7 +26,8 @@ def low():
 def _reset():
     E.lib().emc_hip_clear_error()
     yield
-    for k, v in (("likelihood", "gaussian"), ("proposal", "off"), ("precision", "fp32"), ("insert", "atomic")):
+    for k, v in (("likelihood", "gaussian"), ("proposal", "off"), ("precision", "fp32"), ("insert", "atomic"),
+                 ("gemm_tail", "split")):
         E.set_option(k, v)
     E.lib().emc_hip_clear_error()
 
@@ -64,7 +65,9 @@ def test_poisson_iteration_vs_fp64(low, precision):
 
 
 def test_poisson_candidates_all_cells_reproduce_dense(low):
+    """Same operands and k order (the dense GEMM unsplit along k)."""
     P = low
+    E.set_option("gemm_tail", "whole")
     dense = gpu_iteration(P, "fp16")
     L = E.lib()
     assert L.emc_hip_proposal_init(E.fptr(P.q), P.R, 2, P.side, 4, 64, 3) == 0

@@ -52,7 +52,7 @@ def _reset():
 
 
 def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", min_mass="0.999", gate="audit",
-         track="off", max_fail="1", mstep="auto", shared="0", group_min="0", fill="delta"):
+         track="off", max_fail="1", mstep="auto", shared="0", group_min="0", fill="delta", gemm_tail="split"):
     E.set_option("precision", precision)
     E.set_option("insert", insert)
     E.set_option("interp", "nearest")
@@ -66,6 +66,7 @@ def opts(precision="fp16", insert="deterministic", proposal="off", audit="0", mi
     E.set_option("proposal_shared", shared)
     E.set_option("proposal_group_min", group_min)
     E.set_option("respons_fill", fill)
+    E.set_option("gemm_tail", gemm_tail)
 
 
 def init_cells(P, topk, n_coarse=2, batch=64):
@@ -114,13 +115,16 @@ def cell_mass(Pm, off, rot):
 
 
 def test_all_cells_reproduce_dense(prob):
+    """Same operands and the same k order in both kernels (the dense GEMM
+    unsplit along k): the candidate iteration equals the dense one to FP32
+    rounding of the epilogues."""
     P = prob
-    opts()
+    opts(gemm_tail="whole")
     dense = run(P)[0]
     nc, off, rot = init_cells(P, topk=8)
     cand = np.tile(np.arange(nc, dtype=np.int32), (P.N, 1))
     assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, nc) == 0
-    opts(proposal="on")
+    opts(proposal="on", gemm_tail="whole")
     cand_run = run(P)[0]
     st = cand_run["stats"]
     assert st[0] == 1 and st[11] == 0 and st[6] == P.N * P.R
