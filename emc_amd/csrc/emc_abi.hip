@@ -44,20 +44,6 @@ void set_error(const char* fmt, ...) {
 
 bool failed() { return g_err_code != 0; }
 
-bool PinnedBuf::ensure(size_t b) {
-  if (p && bytes >= b) return true;
-  if (p) (void)hipHostFree(p);
-  p = nullptr;
-  bytes = 0;
-  if (hipHostMalloc(&p, b ? b : 1, hipHostMallocDefault) != hipSuccess) {
-    (void)hipGetLastError();
-    p = nullptr;
-    return false;
-  }
-  bytes = b;
-  return true;
-}
-
 bool DBuf::ensure(size_t b) {
   if (b == 0) b = 1;
   if (p && bytes >= b) return true;
@@ -784,14 +770,9 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   }
   std::vector<float> T(R);
   EMC_CHECK(hipMemcpyAsync(T.data(), e.d_Tf.p, sizeof(float) * R, hipMemcpyDeviceToHost, e.stream));
-  // the model to the host (emc_cuda.cu:352), through pinned staging and a
-  // threaded host copy
-  const size_t mbytes = sizeof(float) * (size_t)S * S * S;
-  const bool pinned = e.h_model_pin.ensure(mbytes);
-  EMC_CHECK(hipMemcpyAsync(pinned ? e.h_model_pin.p : (void*)model->data, d_model, mbytes, hipMemcpyDeviceToHost,
-                           e.stream));
+  // the model to the host (emc_cuda.cu:352)
+  EMC_CHECK(hipMemcpyAsync(model->data, d_model, sizeof(float) * (size_t)S * S * S, hipMemcpyDeviceToHost, e.stream));
   sync();
-  if (pinned && !failed()) emc_host_copy(model->data, e.h_model_pin.p, mbytes);
   e.like.valid = false;
   float overall = 0.0f;  // emc_cuda.cu:348-351
   for (int r = 0; r < R; r++) overall += T[r];
@@ -947,7 +928,6 @@ void emc_hip_shutdown(void) {
   if (!g_engine) return;
   if (g_engine->stream) (void)hipStreamSynchronize(g_engine->stream);
   g_engine->release_all();
-  if (g_engine->h_model_pin.p) (void)hipHostFree(g_engine->h_model_pin.p);
   if (g_engine->ev0) (void)hipEventDestroy(g_engine->ev0);
   if (g_engine->ev1) (void)hipEventDestroy(g_engine->ev1);
   if (g_engine->stream) (void)hipStreamDestroy(g_engine->stream);

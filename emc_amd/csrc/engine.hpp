@@ -35,17 +35,6 @@ struct DBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
-// Pinned host buffer (hipHostMalloc): device -> host copies into it run at
-// the link's rate; a failed allocation leaves p null (callers copy directly).
-struct PinnedBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  bool ensure(size_t b);
-};
-
-// host copy split over the OpenMP threads (emc_amd/host/geometry.c)
-extern "C" void emc_host_copy(void* dst, const void* src, size_t n);
-
 // ---- phases for timing -----------------------------------------------------
 enum Phase {
   PH_EXPAND = 0, PH_PACK, PH_LIKE, PH_SOFTMAX, PH_SCALING, PH_ROWSUM, PH_MSTEP,
@@ -79,7 +68,6 @@ struct Engine {
   int device = -1;
   int n_cu = 256;          // compute units of the device
   DBuf d_tail_part;        // FP32 partial tiles of the split-k tail of the FP16 likelihood GEMM
-  PinnedBuf h_model_pin;   // staging of the model copy update_slices hands to the host
   hipStream_t stream = nullptr;
 
   // options

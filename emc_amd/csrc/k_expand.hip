@@ -46,7 +46,8 @@ void launch_brick_model(const float* model, int S, float* out, hipStream_t st) {
   EMC_LAUNCHED();
 }
 
-// One workgroup per rotation; model reads are gathers served by L2.  Each
+// One workgroup per rotation; model reads are gathers served by L2 (the
+// pixel loops are unrolled so that several gathers per lane are in flight).  Each
 // XCD takes a contiguous range of rotations (logical_of_block): neighbouring
 // table entries are nearby orientations whose slices gather overlapping
 // model bricks, so that XCD's L2 serves them.
@@ -63,6 +64,7 @@ __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict_
   const float half = (float)S / 2.0f + 0.5f;
   const int N2d = S * S;
   float* out = slices + (size_t)r * N2d;
+#pragma unroll 4
   for (int t = threadIdx.x; t < N2d; t += blockDim.x) {
     const int p = expand_pixel(t, S);
     float ux, uy, uz;
@@ -90,6 +92,7 @@ __global__ __launch_bounds__(256) void k_expand_trilinear(const float* __restric
   const float half = (float)S / 2.0f + 0.5f;
   const int N2d = S * S;
   float* out = slices + (size_t)r * N2d;
+#pragma unroll 4
   for (int t = threadIdx.x; t < N2d; t += blockDim.x) {
     const int p = expand_pixel(t, S);
     float ux, uy, uz;
@@ -335,7 +338,8 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
     rot_matrix(&rot[4 * (size_t)r], e);
     const float half = (float)S / 2.0f + 0.5f;
     float* out = slices + (size_t)r * N2d;
-    for (int t = threadIdx.x; t < N2d; t += blockDim.x) {
+  #pragma unroll 4
+  for (int t = threadIdx.x; t < N2d; t += blockDim.x) {
       const int p = expand_pixel(t, S);
       float ux, uy, uz;
       voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
@@ -438,6 +442,7 @@ __global__ __launch_bounds__(256) void k_expand_pack_prov(const float* __restric
   const float half = (float)S / 2.0f + 0.5f;
   float mx = 0.0f;
   double acc = 0.0;
+#pragma unroll 4
   for (int p = threadIdx.x; p < N2d; p += 256) {
     float ux, uy, uz;
     voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);

@@ -720,13 +720,18 @@ __global__ __launch_bounds__(256) void k_cand_topcells(const float* __restrict__
 
 // Patterns scored on the whole table: cell masses from the columns of Psub
 // (lanes over consecutive subset columns, so every row read is coalesced),
-// in two passes: the 4 largest of every group of 64 cells, then the ktop
-// largest of those (cells in increasing order, ties to the lower cell).
+// in two passes: the 4 largest of every group of kTopGroup cells, then the
+// ktop largest of those.  Order: larger mass first, equal masses to the
+// lower cell, so the result does not depend on the order of insertion.
+
+__device__ __forceinline__ bool top_before(float m1, int c1, float m2, int c2) {
+  return m1 > m2 || (m1 == m2 && c1 >= 0 && (c2 < 0 || c1 < c2));
+}
 
 __device__ __forceinline__ void top4_insert(float m, int c, float (&lm)[4], int (&lc)[4]) {
-  if (!(m > lm[3])) return;
+  if (!top_before(m, c, lm[3], lc[3])) return;
   int at = 3;
-  while (at > 0 && m > lm[at - 1]) {
+  while (at > 0 && top_before(m, c, lm[at - 1], lc[at - 1])) {
     lm[at] = lm[at - 1];
     lc[at] = lc[at - 1];
     at--;
@@ -757,22 +762,40 @@ __global__ __launch_bounds__(64) void k_sub_topcells_part(const float* __restric
   }
 }
 
+// One wave per pattern: lanes merge the partial lists of groups lane,
+// lane + 64, ..., then the lanes' lists are merged pairwise down the wave.
 __global__ __launch_bounds__(256) void k_sub_topcells_merge(const int* __restrict__ pc, const float* __restrict__ pm,
                                                             int n_sub, int ng, const int* __restrict__ sub, int ktop,
                                                             int* __restrict__ top, float* __restrict__ topm) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= n_sub) return;
   float lm[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
   int lc[4] = {-1, -1, -1, -1};
-  for (int g = 0; g < ng; g++)
+  for (int g = lane; g < ng; g += 64)
     for (int t = 0; t < 4; t++) {
       const size_t o = ((size_t)k * ng + g) * 4 + t;
       if (pc[o] >= 0) top4_insert(pm[o], pc[o], lm, lc);
     }
-  const int i = sub[k];
-  for (int t = 0; t < ktop; t++) {
-    top[(size_t)i * ktop + t] = lc[t];
-    topm[(size_t)i * ktop + t] = fmaxf(lm[t], 0.0f);
+  for (int off = 32; off > 0; off >>= 1) {
+    float om[4];
+    int oc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      om[t] = __shfl_down(lm[t], off);
+      oc[t] = __shfl_down(lc[t], off);
+    }
+    if (lane < off)
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        if (oc[t] >= 0) top4_insert(om[t], oc[t], lm, lc);
+  }
+  if (lane == 0) {
+    const int i = sub[k];
+    for (int t = 0; t < ktop; t++) {
+      top[(size_t)i * ktop + t] = lc[t];
+      topm[(size_t)i * ktop + t] = fmaxf(lm[t], 0.0f);
+    }
   }
 }
 
@@ -818,7 +841,7 @@ void launch_topcells(const float* Pc, int N, int topk, const unsigned char* insu
     hipLaunchKernelGGL(k_sub_topcells_part, dim3((n_sub + 63) / 64, ng), dim3(64), 0, st, Psub, n_sub, C, cell_off,
                        cell_rot, part_c, part_m);
     EMC_LAUNCHED();
-    hipLaunchKernelGGL(k_sub_topcells_merge, dim3((n_sub + 255) / 256), dim3(256), 0, st, part_c, part_m, n_sub, ng,
+    hipLaunchKernelGGL(k_sub_topcells_merge, dim3((n_sub + 3) / 4), dim3(256), 0, st, part_c, part_m, n_sub, ng,
                        sub, ktop, top, topm);
     EMC_LAUNCHED();
   }

@@ -91,7 +91,9 @@ static bool upload(DBuf& d, const std::vector<T>& h, hipStream_t st) {
     std::memcpy(pin, h.data(), bytes);
     EMC_CHECK(hipMemcpyAsync(d.p, pin, bytes, hipMemcpyHostToDevice, st));
   } else {
+    // no pinned memory: copy from the caller's vector and wait for it
     EMC_CHECK(hipMemcpyAsync(d.p, h.data(), bytes, hipMemcpyHostToDevice, st));
+    EMC_CHECK(hipStreamSynchronize(st));
   }
   return !failed();
 }
@@ -174,7 +176,6 @@ static bool build_lists(Proposal& p) {
       !p.d_panels.ensure(2 * (size_t)std::max<long>(p.n_chunks, 1) * 2 * kSlot * 32) ||
       !p.d_plogp.ensure(sizeof(double) * N) || !p.d_crowscale.ensure(sizeof(float) * p.R))
     return false;
-  sync_stream();
   return !failed() && build_groups(p);
 }
 
@@ -259,7 +260,6 @@ static bool build_groups(Proposal& p) {
     for (int e0 = p.h_rent_off[c]; e0 < p.h_rent_off[c + 1]; e0 += kCandRows) rwork.push_back(make_int2(c, e0));
   p.n_rwork = (int)rwork.size();
   if (!upload(p.d_rent_off, p.h_rent_off, st) || !upload(p.d_rwork, rwork, st)) return false;
-  sync_stream();
   return !failed();
 }
 
@@ -360,12 +360,20 @@ static bool recentre(Proposal& p) {
   launch_recentre_cand(p.d_top.as<int>(), p.d_topm.as<float>(), N, K, p.d_nbr.as<int>(), Proposal::kNbr,
                        p.d_net_cand.as<int>(), p.d_shared.as<unsigned char>(), p.C, p.topk, p.d_cand.as<int>(),
                        p.d_moved.as<int>(), st);
-  int moved = 0;
-  EMC_CHECK(hipMemcpyAsync(&moved, p.d_moved.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  sync_stream();
-  if (failed()) return false;
-  p.st_moved = moved;
-  return moved == 0 || build_lists(p);
+  // the lists are rebuilt whether or not a pattern moved (almost every
+  // iteration some do), so the count is read after the list building's own
+  // synchronisation instead of with one of its own
+  static int* moved = nullptr;
+  if (!moved && hipHostMalloc((void**)&moved, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    moved = nullptr;
+  }
+  int moved_now = 0;
+  EMC_CHECK(hipMemcpyAsync(moved ? moved : &moved_now, p.d_moved.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  if (!moved) sync_stream();
+  if (failed() || !build_lists(p)) return false;  // it synchronises after the copy above
+  p.st_moved = moved ? *moved : moved_now;
+  return !failed();
 }
 
 bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dense) {

@@ -3,7 +3,6 @@
  */
 #include <math.h>
 #include <stdlib.h>
-#include <string.h>
 
 #include "emc_host.h"
 
@@ -102,14 +101,3 @@ void insert_slice(sp_3matrix *model, sp_3matrix *weight, sp_matrix *slice, sp_im
     }
 }
 
-/* n bytes from src to dst in 1 MB pieces over the OpenMP threads (the
- * library's model copy to the caller's array, out of pinned staging). */
-void emc_host_copy(void *dst, const void *src, size_t n) {
-  const size_t piece = (size_t)1 << 20;
-  const long np = (long)((n + piece - 1) / piece);
-#pragma omp parallel for schedule(static) if (np > 1)
-  for (long k = 0; k < np; k++) {
-    const size_t o = (size_t)k * piece;
-    memcpy((char *)dst + o, (const char *)src + o, n - o < piece ? n - o : piece);
-  }
-}
