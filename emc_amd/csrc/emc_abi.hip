@@ -981,6 +981,10 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (is(value, "gaussian")) e.likelihood = LIKE_GAUSSIAN;
     else if (is(value, "poisson")) e.likelihood = LIKE_POISSON;
     else return 1;
+  } else if (is(key, "gemm_tile")) {
+    if (is(value, "256")) e.gemm_small = false;
+    else if (is(value, "128")) e.gemm_small = true;
+    else return 1;
   } else if (is(key, "gemm_tail")) {
     if (is(value, "split")) e.tail_split = true;
     else if (is(value, "whole")) e.tail_split = false;

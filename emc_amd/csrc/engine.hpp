@@ -79,6 +79,7 @@ struct Engine {
   int mstep_mode = MSTEP_AUTO;  // FP16 form: M-step over the nonzero responsibilities
   bool respons_delta = true;    // proposal iterations rewrite only the changed responsibilities
   bool tail_split = true;       // FP16 likelihood GEMM: last partial round of tiles split along k
+  bool gemm_small = false;      // FP16 likelihood GEMM: 128 x 128 tiles, two workgroups per compute unit
   bool timing = false;
   bool verbose = false;
 

@@ -36,13 +36,15 @@ namespace emc {
 // so without the split a tail of a few tiles (or a launch of fewer tiles
 // than units) would take a whole tile time on a few units while the rest of
 // the device idles.
-constexpr int kE16TM = 8, kE16TN = 4, kE16WGM = 2, kE16WGN = 4;
-
-template <bool KSPLIT, bool PC>
-__global__ __launch_bounds__(512, 1) void k_estep16(Operands op, int n_mp, int tiles_m, int tiles_n, int n_main,
-                                                    int split, float* __restrict__ part, LikeEpilogue ep) {
-  constexpr int TM = kE16TM, TN = kE16TN, WGM = kE16WGM, WGN = kE16WGN, NA = KSPLIT ? 2 : 1, NB = 2,
-                STAGES = KSPLIT ? 2 : 3;
+// Tile shapes: 256 x 256 (8 waves of 128 x 64, one workgroup per compute
+// unit) or, with option gemm_tile "128", 128 x 128 (4 waves of 64 x 64, 72 KB
+// of LDS, two independent workgroups per compute unit, so one computes
+// while the other waits at its stage barrier).
+template <int TM, int TN, int WGM, int WGN, int MINB, bool KSPLIT, bool PC>
+__global__ __launch_bounds__(64 * WGM * WGN, MINB) void k_estep16(Operands op, int n_mp, int tiles_m, int tiles_n,
+                                                                  int n_main, int split, float* __restrict__ part,
+                                                                  LikeEpilogue ep) {
+  constexpr int NA = KSPLIT ? 2 : 1, NB = 2, STAGES = KSPLIT ? 2 : 3, NT = 64 * WGM * WGN;
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
   int tm, tn, logical, range = -1, k0 = 0, k1 = n_mp / BK;
@@ -61,12 +63,12 @@ __global__ __launch_bounds__(512, 1) void k_estep16(Operands op, int n_mp, int t
   dl_mma_loop<TM, TN, WGM, WGN, NA, NB, STAGES>(op, m0, n0, k1 - k0, acc, lds, k0 * BK);
   if (range >= 0) {
     // [tail tile][range][x][y][thread] float4, coalesced over threads
-    float* out = part + ((size_t)(logical - n_main) * split + range) * (TM * TN * 512 * 4);
+    float* out = part + ((size_t)(logical - n_main) * split + range) * (TM * TN * NT * 4);
 #pragma unroll
     for (int x = 0; x < TM; x++)
 #pragma unroll
       for (int y = 0; y < TN; y++)
-        reinterpret_cast<floatx4*>(out)[(x * TN + y) * 512 + threadIdx.x] = acc[x][y];
+        reinterpret_cast<floatx4*>(out)[(x * TN + y) * NT + threadIdx.x] = acc[x][y];
     return;
   }
 
@@ -98,21 +100,22 @@ __global__ __launch_bounds__(512, 1) void k_estep16(Operands op, int n_mp, int t
 }
 
 // The split tail tiles: blocks (tile, x), ranges added in order.
-__global__ __launch_bounds__(512) void k_estep16_tail(const float* __restrict__ part, int tiles_m, int tiles_n,
-                                                      int n_main, int split, LikeEpilogue ep) {
-  constexpr int TM = kE16TM, TN = kE16TN, WGM = kE16WGM, WGN = kE16WGN;
+template <int TM, int TN, int WGM, int WGN>
+__global__ __launch_bounds__(64 * WGM * WGN) void k_estep16_tail(const float* __restrict__ part, int tiles_m,
+                                                                 int tiles_n, int n_main, int split, LikeEpilogue ep) {
+  constexpr int NT = 64 * WGM * WGN;
   const int logical = n_main + (int)blockIdx.x;
   int tm, tn;
   tile_of_logical(logical, tiles_m, tiles_n, tm, tn);
   const int m0 = tm * 16 * WGM * TM, n0 = tn * 16 * WGN * TN;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w / WGN, wn = w % WGN;
   const int g = lane >> 4, fr = lane & 15;
-  const floatx4* in = reinterpret_cast<const floatx4*>(part) + (size_t)blockIdx.x * split * (TM * TN * 512);
+  const floatx4* in = reinterpret_cast<const floatx4*>(part) + (size_t)blockIdx.x * split * (TM * TN * NT);
   const int x = blockIdx.y;
 #pragma unroll
   for (int y = 0; y < TN; y++) {
-    floatx4 acc = in[(x * TN + y) * 512 + threadIdx.x];
-    for (int r = 1; r < split; r++) acc += in[((size_t)r * TM * TN + x * TN + y) * 512 + threadIdx.x];
+    floatx4 acc = in[(x * TN + y) * NT + threadIdx.x];
+    for (int r = 1; r < split; r++) acc += in[((size_t)r * TM * TN + x * TN + y) * NT + threadIdx.x];
     like_store(ep, m0 + wm * 16 * TM + x * 16 + 4 * g, n0 + wn * 16 * TN + y * 16 + fr, acc);
   }
 }
@@ -177,56 +180,71 @@ void launch_convert_p(const MstepArgs& a, _Float16* hi, _Float16* lo, hipStream_
   EMC_LAUNCHED();
 }
 
+template <int TM, int TN, int WGM, int WGN, int MINB>
+static void estep16_shape(bool ksplit, const Operands& op, int N_pad, int R_pad, int n_mp, const LikeEpilogue& ep,
+                          hipStream_t st) {
+  constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN, NT = 64 * WGM * WGN;
+  const int tiles_m = N_pad / BM, tiles_n = R_pad / BN, nt = tiles_m * tiles_n;
+  const bool pc = ep.P != nullptr;
+  // MINB workgroups per compute unit (LDS): the tiles of the last, partial
+  // round (all of them when there are fewer tiles than workgroup slots) are
+  // split along k into the number of ranges that finishes them soonest,
+  // ceil(tail * split / slots) rounds of 1 / split of a tile each
+  Engine& e = engine();
+  const int slots = e.n_cu * MINB;
+  const int rounds_full = nt / slots, tail = nt - rounds_full * slots, nk = n_mp / BK;
+  int split = 1, n_main = nt;
+  if (e.tail_split && !pc && tail > 0) {
+    double best = 1.0;
+    for (int sp = 2; sp <= std::min(16, nk); sp++) {
+      const double t = (double)((tail * sp + slots - 1) / slots) / sp;
+      if (t < best - 1e-9) {
+        best = t;
+        split = sp;
+      }
+    }
+    if (split > 1 && e.d_tail_part.ensure(sizeof(float) * (size_t)tail * split * BM * BN))
+      n_main = nt - tail;
+    else
+      split = 1;
+  }
+  float* part = e.d_tail_part.as<float>();
+  const dim3 grid(n_main + (nt - n_main) * split), block(NT);
+  if (ksplit) {
+    if (pc)
+      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, true, true>), grid, block, 0, st, op, n_mp, tiles_m,
+                         tiles_n, n_main, split, part, ep);
+    else
+      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, true, false>), grid, block, 0, st, op, n_mp, tiles_m,
+                         tiles_n, n_main, split, part, ep);
+  } else {
+    if (pc)
+      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, false, true>), grid, block, 0, st, op, n_mp, tiles_m,
+                         tiles_n, n_main, split, part, ep);
+    else
+      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, false, false>), grid, block, 0, st, op, n_mp, tiles_m,
+                         tiles_n, n_main, split, part, ep);
+  }
+  EMC_LAUNCHED();
+  if (n_main < nt) {
+    hipLaunchKernelGGL((k_estep16_tail<TM, TN, WGM, WGN>), dim3(nt - n_main, TM), dim3(NT), 0, st, part, tiles_m,
+                       tiles_n, n_main, split, ep);
+    EMC_LAUNCHED();
+  }
+}
+
 void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, const void* Slo, int N_pad,
                     int R_pad, int n_mp, const LikeEpilogue& ep, hipStream_t st) {
-  const int tiles_m = N_pad / kTile16, tiles_n = R_pad / kTile16, nt = tiles_m * tiles_n;
   Operands op{};
   op.a[0] = (const _Float16*)K;
   op.a[1] = (const _Float16*)Klo;
   op.b[0] = (const _Float16*)S;
   op.b[1] = (const _Float16*)Slo;
   op.lda = op.ldb = (size_t)n_mp;
-  const bool pc = ep.P != nullptr;
-  // one workgroup per compute unit: the tiles of the last, partial round
-  // (all of them when there are fewer tiles than units) are split along k
-  // into the number of ranges that finishes them soonest, ceil(tail * split /
-  // units) rounds of 1 / split of a tile each
-  Engine& e = engine();
-  const int rounds_full = nt / e.n_cu, tail = nt - rounds_full * e.n_cu, nk = n_mp / BK;
-  int split = 1, n_main = nt;
-  if (e.tail_split && !pc && tail > 0) {
-    double best = 1.0;
-    for (int sp = 2; sp <= std::min(16, nk); sp++) {
-      const double t = (double)((tail * sp + e.n_cu - 1) / e.n_cu) / sp;
-      if (t < best - 1e-9) {
-        best = t;
-        split = sp;
-      }
-    }
-    if (split > 1 && e.d_tail_part.ensure(sizeof(float) * (size_t)tail * split * kTile16 * kTile16))
-      n_main = nt - tail;
-    else
-      split = 1;
-  }
-  float* part = e.d_tail_part.as<float>();
-  const dim3 grid(n_main + (nt - n_main) * split), block(512);
-  if (ksplit) {
-    if (pc) hipLaunchKernelGGL((k_estep16<true, true>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, n_main, split,
-                               part, ep);
-    else hipLaunchKernelGGL((k_estep16<true, false>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, n_main, split,
-                            part, ep);
-  } else {
-    if (pc) hipLaunchKernelGGL((k_estep16<false, true>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, n_main, split,
-                               part, ep);
-    else hipLaunchKernelGGL((k_estep16<false, false>), grid, block, 0, st, op, n_mp, tiles_m, tiles_n, n_main, split,
-                            part, ep);
-  }
-  EMC_LAUNCHED();
-  if (n_main < nt) {
-    hipLaunchKernelGGL(k_estep16_tail, dim3(nt - n_main, kE16TM), dim3(512), 0, st, part, tiles_m, tiles_n, n_main,
-                       split, ep);
-    EMC_LAUNCHED();
-  }
+  if (engine().gemm_small)
+    estep16_shape<4, 4, 2, 2, 2>(ksplit, op, N_pad, R_pad, n_mp, ep, st);
+  else
+    estep16_shape<8, 4, 2, 4, 1>(ksplit, op, N_pad, R_pad, n_mp, ep, st);
 }
 
 void launch_mstep16(bool ksplit, const void* Phi, const void* Plo, const void* Kt, const void* Ktlo,

@@ -73,6 +73,10 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 nonzero P_ri (lists built from d_respons, P kept in FP32)
  *                 instead of the dense contraction; "dense", "sparse": always
  *                 that form.  The dropped terms are exact zeros.
+ *   "gemm_tile"  "256" (default): 256 x 256 tiles of the FP16 likelihood
+ *                 GEMM, one workgroup per compute unit; "128": 128 x 128
+ *                 tiles, two workgroups per compute unit (same results, bit
+ *                 for bit; slower on MI355X).
  *   "gemm_tail"  "split" (default): the FP16 likelihood GEMM's tiles that
  *                 do not fill a whole round of the compute units (all of
  *                 them when there are fewer tiles than units) are split along

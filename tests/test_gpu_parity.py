@@ -394,3 +394,27 @@ def test_expand_pack_large_slices_match_separate(likelihood, interp):
         prob.free()
     finally:
         E.set_option("likelihood", "gaussian")
+
+
+def test_gemm_tile_128_matches_256():
+    """The 128 x 128 tile form of the FP16 likelihood GEMM (two workgroups per
+    compute unit) sums every output over k in the same order as the 256 x 256
+    form: bit-identical likelihoods (both unsplit along k)."""
+    P = problems.make(side=32, n_patterns=300, n_rot=4, n_blobs=20, radius=6.0, photons=2000.0, seed=9)
+    options("fp16")
+    out = {}
+    try:
+        E.set_option("gemm_tail", "whole")
+        for tile in ("256", "128"):
+            E.set_option("gemm_tile", tile)
+            prob = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
+            prob.get_slices()
+            L = np.zeros((prob.R, prob.N), np.float32)
+            prob.likelihood(0.4, L)
+            prob.free()
+            E.check()
+            out[tile] = L
+    finally:
+        E.set_option("gemm_tile", "256")
+        E.set_option("gemm_tail", "split")
+    assert np.array_equal(out["256"], out["128"])
