@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--sigma", type=float, default=0.4, help="noise level of the gaussian likelihood")
     ap.add_argument("--gemm-tile", choices=["256", "128"], default=None,
                     help="tile of the FP16 likelihood GEMM (default: the library's)")
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="library option (emc_hip_set_option) set before the run; repeatable")
     ap.add_argument("--max-fail", type=float, default=0.005,
                     help="audited fraction below min mass that sends an iteration to the whole table")
     ap.add_argument("--plain-steps", type=int, default=-1, help="timed FP32 iterations (default = steps)")
@@ -305,6 +307,9 @@ def main():
 
     if args.gemm_tile:
         E.set_option("gemm_tile", args.gemm_tile)
+    for kv in args.option:
+        k, v = kv.split("=", 1)
+        E.set_option(k, v)
     t0 = time.perf_counter()
     W = configs.build(args.config, args.patterns or None)
     N, R, n_m = W.N, W.R, W.n_m

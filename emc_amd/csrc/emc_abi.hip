@@ -981,6 +981,10 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (is(value, "gaussian")) e.likelihood = LIKE_GAUSSIAN;
     else if (is(value, "poisson")) e.likelihood = LIKE_POISSON;
     else return 1;
+  } else if (is(key, "expand_pack")) {
+    if (is(value, "auto")) e.expand_tiled = false;
+    else if (is(value, "tiled")) e.expand_tiled = true;
+    else return 1;
   } else if (is(key, "gemm_tile")) {
     if (is(value, "256")) e.gemm_small = false;
     else if (is(value, "128")) e.gemm_small = true;

@@ -80,6 +80,7 @@ struct Engine {
   bool respons_delta = true;    // proposal iterations rewrite only the changed responsibilities
   bool tail_split = true;       // FP16 likelihood GEMM: last partial round of tiles split along k
   bool gemm_small = false;      // FP16 likelihood GEMM: 128 x 128 tiles, two workgroups per compute unit
+  bool expand_tiled = false;    // fused expand + pack in rotation blocks x pixel tiles even where the slice fits LDS
   bool timing = false;
   bool verbose = false;
 
@@ -129,6 +130,7 @@ struct Engine {
   DBuf d_LSt;           // half2 [n_mp][R_pad]: (hi, lo) transposed (sparse mode)
   DBuf d_kidx;          // int [N2d]: packed index of each masked-in pixel (inverse of d_pix), -1 elsewhere
   DBuf d_redo;          // int [R_pad]: rows the provisional-scale pack must redo
+  DBuf d_xpart_max, d_xpart_A;  // [R_pad][pixel tiles]: per-tile row maximum and A_r partial of that pack
   int sscale_R = -1;    // rotations whose row scales d_Sscale holds (-1: none)
   uint64_t s16_gen = 0;  // bumped whenever d_S16 / d_S16lo are written
   uint64_t lst_gen = 0;  // s16_gen of the slices d_LSt holds (built when the CSR E-step needs it)

@@ -406,83 +406,126 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
 // rotation's scale from the previous pack, prov_inv = 1 / scale, 0 when
 // there is none) and the row is flagged when its own scale differs;
 // k_pack_fix then repacks the flagged rows from the FP32 slice.  The pairs
-// are the separate pack's either way (sc is a power of two); A_r is summed
-// over pixels in row order.  Pixels in row order: a wave's 64 packed
-// entries are then consecutive (whole 128-byte lines of the pair).
+// are the separate pack's either way (sc is a power of two).
+//
+// A workgroup takes kXR consecutive rotations on one kXT x kXT pixel tile:
+// nearby orientations on a small patch gather from one small region of the
+// model, which stays in cache across the block's rotations (a whole slice
+// per workgroup, with hundreds of slices in flight per XCD, streamed the
+// model from memory for every slice).  Per (rotation, tile) it leaves the
+// maximum |operand| and the partial A_r; k_pack_tile_finish reduces them in
+// tile order (A_r is reproducible), sets the scale and the redo flag and
+// zeroes the padding entries.
+constexpr int kXT = 64, kXR = 8;
+
 template <bool TRI, bool BRICK>
-__global__ __launch_bounds__(256) void k_expand_pack_prov(const float* __restrict__ model, int S,
+__global__ __launch_bounds__(256) void k_expand_pack_tile(const float* __restrict__ model, int S,
                                                           const float* __restrict__ rot,
                                                           const float* __restrict__ x, const float* __restrict__ y,
                                                           const float* __restrict__ z, float* __restrict__ slices,
-                                                          const int* __restrict__ kidx, int R, int n_m, int n_mp,
+                                                          const int* __restrict__ kidx, int R, int R_pad, int n_mp,
                                                           const float* __restrict__ prov_inv,
                                                           _Float16* __restrict__ S16, _Float16* __restrict__ S16lo,
-                                                          float* __restrict__ scale, double* __restrict__ A,
-                                                          int* __restrict__ redo, int poisson) {
-  const int r = logical_of_block(gridDim.x);
-  const bool live = r < R;
+                                                          float* __restrict__ part_max, double* __restrict__ part_A,
+                                                          int poisson) {
+  const int ntx = (S + kXT - 1) / kXT, ntiles = ntx * ntx;
+  const int tile = blockIdx.y, tx0 = (tile % ntx) * kXT, ty0 = (tile / ntx) * kXT;
   const int N2d = S * S;
-  _Float16* hrow = S16 + (size_t)r * n_mp;
-  _Float16* lrow = S16lo + (size_t)r * n_mp;
-  for (int k = n_m + threadIdx.x; k < n_mp; k += 256) {
-    hrow[k] = (_Float16)0.0f;
-    lrow[k] = (_Float16)0.0f;
-  }
-  if (!live) {
-    for (int k = threadIdx.x; k < n_m; k += 256) {
-      hrow[k] = (_Float16)0.0f;
-      lrow[k] = (_Float16)0.0f;
-    }
-    return;
-  }
-  const float pv = prov_inv ? prov_inv[r] : 0.0f;
-  const float sc0 = pv > 0.0f ? 1.0f / pv : 1.0f;  // exact: pv is a power of two
-  float e[9];
-  rot_matrix(&rot[4 * (size_t)r], e);
   const float half = (float)S / 2.0f + 0.5f;
+  for (int rr = 0; rr < kXR; rr++) {
+    const int r = blockIdx.x * kXR + rr;
+    if (r >= R_pad) break;  // uniform over the workgroup
+    const bool live = r < R;
+    _Float16* hrow = S16 + (size_t)r * n_mp;
+    _Float16* lrow = S16lo + (size_t)r * n_mp;
+    float e[9];
+    float sc0 = 1.0f;
+    if (live) {
+      rot_matrix(&rot[4 * (size_t)r], e);
+      const float pv = prov_inv ? prov_inv[r] : 0.0f;
+      if (pv > 0.0f) sc0 = 1.0f / pv;  // exact: pv is a power of two
+    }
+    float mx = 0.0f;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int t = threadIdx.x; t < kXT * kXT; t += 256) {
+      const int px = tx0 + (t & (kXT - 1)), py = ty0 + t / kXT;
+      if (px >= S || py >= S) continue;
+      const int p = py * S + px;
+      const int k = kidx[p];
+      if (!live) {
+        if (k >= 0) {
+          hrow[k] = (_Float16)0.0f;
+          lrow[k] = (_Float16)0.0f;
+        }
+        continue;
+      }
+      float ux, uy, uz;
+      voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+      float v = 0.0f;
+      if (TRI) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          int ix, iy, iz;
+          float w;
+          if (tri_corner_xyz(c, ux, uy, uz, S, ix, iy, iz, w)) v += w * model_at<BRICK>(model, ix, iy, iz, S);
+        }
+      } else {
+        const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
+        if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S) v = model_at<BRICK>(model, vx, vy, vz, S);
+      }
+      slices[(size_t)r * N2d + p] = v;
+      if (k >= 0) {
+        const float o = slice_operand(v, poisson);
+        mx = fmaxf(mx, fabsf(o));
+        acc += poisson ? (double)v : (double)v * v;
+        const float sx = o * sc0;
+        const _Float16 hi = (_Float16)sx;
+        hrow[k] = hi;
+        lrow[k] = (_Float16)(sx - (float)hi);
+      }
+    }
+    mx = block_max_f32(mx);
+    acc = block_sum_f64(acc);
+    if (threadIdx.x == 0 && live) {
+      part_max[(size_t)r * ntiles + tile] = mx;
+      part_A[(size_t)r * ntiles + tile] = acc;
+    }
+  }
+}
+
+// One workgroup per rotation: the tiles' maxima and A partials in tile order,
+// the row scale, the redo flag against the provisional scale, and zero
+// padding entries n_m .. n_mp - 1.
+__global__ __launch_bounds__(64) void k_pack_tile_finish(const float* __restrict__ part_max,
+                                                         const double* __restrict__ part_A, int ntiles, int R,
+                                                         int n_m, int n_mp, float* __restrict__ scale,
+                                                         double* __restrict__ A, int* __restrict__ redo,
+                                                         int have_prov, _Float16* __restrict__ S16,
+                                                         _Float16* __restrict__ S16lo) {
+  const int r = blockIdx.x;
+  for (int k = n_m + threadIdx.x; k < n_mp; k += 64) {
+    S16[(size_t)r * n_mp + k] = (_Float16)0.0f;
+    S16lo[(size_t)r * n_mp + k] = (_Float16)0.0f;
+  }
+  if (r >= R || threadIdx.x != 0) return;
   float mx = 0.0f;
   double acc = 0.0;
-#pragma unroll 4
-  for (int p = threadIdx.x; p < N2d; p += 256) {
-    float ux, uy, uz;
-    voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
-    float v = 0.0f;
-    if (TRI) {
-#pragma unroll
-      for (int c = 0; c < 8; c++) {
-        int ix, iy, iz;
-        float w;
-        if (tri_corner_xyz(c, ux, uy, uz, S, ix, iy, iz, w)) v += w * model_at<BRICK>(model, ix, iy, iz, S);
-      }
-    } else {
-      const int vx = (int)roundf(ux), vy = (int)roundf(uy), vz = (int)roundf(uz);
-      if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S) v = model_at<BRICK>(model, vx, vy, vz, S);
-    }
-    slices[(size_t)r * N2d + p] = v;
-    const int k = kidx[p];
-    if (k >= 0) {
-      const float o = slice_operand(v, poisson);
-      mx = fmaxf(mx, fabsf(o));
-      acc += poisson ? (double)v : (double)v * v;
-      const float sx = o * sc0;
-      const _Float16 hi = (_Float16)sx;
-      hrow[k] = hi;
-      lrow[k] = (_Float16)(sx - (float)hi);
-    }
+  for (int t = 0; t < ntiles; t++) {
+    mx = fmaxf(mx, part_max[(size_t)r * ntiles + t]);
+    acc += part_A[(size_t)r * ntiles + t];
   }
-  mx = block_max_f32(mx);
   float sc = 1.0f;
   if (mx > 0.0f && mx < 3.0e38f) {
     int ex;
     frexpf(mx, &ex);  // mx < 2^ex
     sc = ldexpf(1.0f, max(-120, min(120, 14 - ex)));
   }
-  acc = block_sum_f64(acc);
-  if (threadIdx.x == 0) {
-    A[r] = acc;
-    scale[r] = 1.0f / sc;
-    redo[r] = sc != sc0;
-  }
+  const float pv = have_prov ? scale[r] : 0.0f;
+  const float sc0 = pv > 0.0f ? 1.0f / pv : 1.0f;
+  A[r] = acc;
+  scale[r] = 1.0f / sc;
+  redo[r] = sc != sc0;
 }
 
 // Rows whose provisional scale was not theirs: the pair again from the FP32
@@ -519,26 +562,35 @@ void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, 
   const size_t lds = (size_t)S * S * sizeof(float);
   const dim3 g(R_pad), b(kPackThreads);
   const int po = (int)poisson;
-  if (!expand_pack_lds(S, n_mp)) {
+  if (!expand_pack_lds(S, n_mp) || engine().expand_tiled) {
     Engine& e = engine();
-    if (!e.d_redo.ensure(sizeof(int) * (size_t)R_pad)) return;
+    const int ntx = (S + kXT - 1) / kXT, ntiles = ntx * ntx;
+    if (!e.d_redo.ensure(sizeof(int) * (size_t)R_pad) ||
+        !e.d_xpart_max.ensure(sizeof(float) * (size_t)R_pad * ntiles) ||
+        !e.d_xpart_A.ensure(sizeof(double) * (size_t)R_pad * ntiles))
+      return;
     // the previous pack's row scales, if they were for these rotations
     const float* prov = e.sscale_R == R ? scale : nullptr;
     const bool tri = interp == INTERP_TRILINEAR;
-    const dim3 gb(256);
+    const dim3 gt((R_pad + kXR - 1) / kXR, ntiles), gb(256);
     int* redo = e.d_redo.as<int>();
+    float* pmx = e.d_xpart_max.as<float>();
+    double* pA = e.d_xpart_A.as<double>();
     if (tri && bricked)
-      hipLaunchKernelGGL((k_expand_pack_prov<true, true>), g, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R, n_m,
-                         n_mp, prov, S16, S16lo, scale, A, redo, po);
+      hipLaunchKernelGGL((k_expand_pack_tile<true, true>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po);
     else if (tri)
-      hipLaunchKernelGGL((k_expand_pack_prov<true, false>), g, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R, n_m,
-                         n_mp, prov, S16, S16lo, scale, A, redo, po);
+      hipLaunchKernelGGL((k_expand_pack_tile<true, false>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po);
     else if (bricked)
-      hipLaunchKernelGGL((k_expand_pack_prov<false, true>), g, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         n_m, n_mp, prov, S16, S16lo, scale, A, redo, po);
+      hipLaunchKernelGGL((k_expand_pack_tile<false, true>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po);
     else
-      hipLaunchKernelGGL((k_expand_pack_prov<false, false>), g, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         n_m, n_mp, prov, S16, S16lo, scale, A, redo, po);
+      hipLaunchKernelGGL((k_expand_pack_tile<false, false>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po);
+    EMC_LAUNCHED();
+    hipLaunchKernelGGL(k_pack_tile_finish, dim3(R_pad), dim3(64), 0, st, pmx, pA, ntiles, R, n_m, n_mp, scale, A,
+                       redo, (int)(prov != nullptr), S16, S16lo);
     EMC_LAUNCHED();
     hipLaunchKernelGGL(k_pack_fix, dim3(R), gb, 0, st, slices, pix, redo, R, S * S, n_m, n_mp, scale, S16, S16lo, po);
     EMC_LAUNCHED();

@@ -73,6 +73,10 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 nonzero P_ri (lists built from d_respons, P kept in FP32)
  *                 instead of the dense contraction; "dense", "sparse": always
  *                 that form.  The dropped terms are exact zeros.
+ *   "expand_pack"  "auto" (default): the fused expand + FP16 pack keeps a
+ *                 slice of at most 64 KB in LDS, larger slices run in blocks
+ *                 of 8 rotations x 64 x 64 pixel tiles with provisional row
+ *                 scales; "tiled": the latter at every size.
  *   "gemm_tile"  "256" (default): 256 x 256 tiles of the FP16 likelihood
  *                 GEMM, one workgroup per compute unit; "128": 128 x 128
  *                 tiles, two workgroups per compute unit (same results, bit
