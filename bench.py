@@ -200,7 +200,8 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
             f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f} (mean mass {st['audit_mean_mass']:.5f}), "
             f"full search {st['full_search']:.0f}, re-centred {st['recentred']:.0f}, "
             f"shared cells {st['shared_cells']:.0f}, pairs {st['pairs_scored']:.3g}, "
-            f"re-centring {st['recentre_ms']:.1f} ms host, group-block pairs {st['group_pairs']:.3g}")
+            f"re-centring {st['recentre_ms']:.1f} ms host, group-block pairs {st['group_pairs']:.3g}, "
+            f"per-cell pairs {st['resid_pairs']:.3g}")
     history += hist
     total_s = time.perf_counter() - t_start
     model = prob.model()

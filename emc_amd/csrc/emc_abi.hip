@@ -1026,6 +1026,11 @@ int emc_hip_set_option(const char* key, const char* value) {
     const long v = strtol(value, &end, 10);
     if (!end || *end != 0 || v < 0 || v > (1L << 30)) return 1;
     proposal().group_min = (int)v;
+  } else if (is(key, "proposal_core_div")) {
+    char* end = nullptr;
+    const long v = strtol(value, &end, 10);
+    if (!end || *end != 0 || v < 1 || v > 64) return 1;
+    proposal().core_div = (int)v;
   } else if (is(key, "proposal_track")) {
     if (is(value, "on")) proposal().track = true;
     else if (is(value, "off")) proposal().track = false;

@@ -1,7 +1,7 @@
 // Pattern groups of the candidate E-step.  Patterns whose candidate lists
 // start with the same cell (with re-centring: the nearest free cell of the
 // same best cell) share most of their candidate cells.  For such a group,
-// the core cells (those among the candidates of at least half of its
+// the core cells (those among the candidates of at least 1 / core_div (a quarter) of its
 // patterns, at most 2 * topk of them) are scored as one dense block: every
 // pattern of the group against every rotation of the core, gathered count
 // rows and gathered slice rows through the FP16 main loop, each count row
@@ -40,10 +40,11 @@ __global__ __launch_bounds__(256) void k_group_hist(const int* __restrict__ cand
 }
 
 // One workgroup per group key: its core cells in increasing order (held by
-// at least half of the group's patterns; groups smaller than min_size get
+// at least 1 / core_div of the group's patterns; groups smaller than min_size get
 // none), their count and fine-rotation total.
 __global__ __launch_bounds__(256) void k_group_core(const int* __restrict__ hist, const int* __restrict__ count,
-                                                    int C, int min_size, const int* __restrict__ cell_off,
+                                                    int C, int min_size, int core_div,
+                                                    const int* __restrict__ cell_off,
                                                     int* __restrict__ core, int* __restrict__ ncore,
                                                     int* __restrict__ nrot) {
   const int g = blockIdx.x;
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(256) void k_group_core(const int* __restrict__ hist
     __shared__ int s_wc[4];
     for (int c0 = 0; c0 < C; c0 += 256) {
       const int c = c0 + threadIdx.x;
-      const bool in = c < C && 2 * hist[(size_t)g * C + c] >= n;
+      const bool in = c < C && core_div * hist[(size_t)g * C + c] >= n;
       const unsigned long long m = __ballot(in);
       if (lane == 0) s_wc[w] = __popcll(m);
       __syncthreads();
@@ -146,12 +147,13 @@ bool build_group_keys(const int* cand, int N, int topk, int C, int* key, int* co
 }
 
 void launch_group_core(const int* cand, const int* key, const int* count, int N, int topk, int C, int min_size,
-                       const int* cell_off, int* hist, int* core, int* ncore, int* nrot, hipStream_t st) {
+                       int core_div, const int* cell_off, int* hist, int* core, int* ncore, int* nrot, hipStream_t st) {
   launch_fill(reinterpret_cast<float*>(hist), (size_t)C * C, 0.0f, st);
   const long n = (long)N * topk;
   hipLaunchKernelGGL(k_group_hist, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cand, key, N, topk, C, hist);
   EMC_LAUNCHED();
-  hipLaunchKernelGGL(k_group_core, dim3(C), dim3(256), 0, st, hist, count, C, min_size, cell_off, core, ncore, nrot);
+  hipLaunchKernelGGL(k_group_core, dim3(C), dim3(256), 0, st, hist, count, C, min_size, core_div, cell_off, core, ncore,
+                     nrot);
   EMC_LAUNCHED();
 }
 

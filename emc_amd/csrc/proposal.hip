@@ -197,7 +197,7 @@ static bool build_groups(Proposal& p) {
   if (!build_group_keys(p.d_cand.as<int>(), N, K, C, p.d_gkey.as<int>(), p.d_gcount.as<int>(), p.d_gpat.as<int>(),
                         p.d_gkey_s, p.d_sort_tmp, st))
     return false;
-  launch_group_core(p.d_cand.as<int>(), p.d_gkey.as<int>(), p.d_gcount.as<int>(), N, K, C, p.group_min,
+  launch_group_core(p.d_cand.as<int>(), p.d_gkey.as<int>(), p.d_gcount.as<int>(), N, K, C, p.group_min, p.core_div,
                     p.d_cell_off.as<int>(), p.d_ghist.as<int>(), p.d_gcore.as<int>(), p.d_gncore.as<int>(),
                     p.d_gnrot.as<int>(), st);
   p.h_gcount.resize(C);

@@ -82,6 +82,7 @@ struct Proposal {
   // candidate cell, whose core cells (candidates of at least half of them)
   // are scored as dense blocks; the per-cell E-step takes the other entries
   int group_min = 64;                // smallest group scored as blocks (0: no groups)
+  int core_div = 4;                  // core cell: held by >= 1 / core_div of its group's patterns
   DBuf d_gkey, d_gcount, d_gpat, d_gkey_s, d_ghist, d_gcore, d_gncore, d_gnrot, d_gcofs, d_crot, d_ccell, d_cslot,
       d_rank, d_rkey, d_gwork, d_rcount, d_rncand;
   std::vector<int> h_gcount, h_gncore, h_gnrot, h_gcofs, h_rcount, h_rent_off;

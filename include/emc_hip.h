@@ -125,6 +125,10 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 one dense block (default "64"; "0": every candidate cell
  *                 by the per-cell kernel).  Takes effect when the candidate
  *                 lists are next built.
+ *   "proposal_core_div"  a candidate cell is in its group's dense block when
+ *                 at least 1 / this many of the group's patterns hold it
+ *                 (default "4"; 1 .. 64).  Takes effect when the candidate
+ *                 lists are next built.
  *   "proposal_gate"  "predicted" (default): a pattern also starts in
  *                 whole-table search when the network's top-k cells hold less
  *                 than min_mass of its predicted distribution; "audit": only
