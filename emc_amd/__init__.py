@@ -98,6 +98,8 @@ _SIGS = [
     ("emc_hip_shutdown", None, []),
     ("emc_hip_total_respons_n", C.c_float, [C.c_void_p, C.c_size_t]),
     ("emc_hip_allocate_bytes", None, [c_void_pp, C.c_size_t]),
+    ("emc_hip_host_alloc", C.c_int, [c_void_pp, C.c_size_t]),
+    ("emc_hip_host_free", None, [C.c_void_p]),
     ("emc_hip_memcpy_to_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("emc_hip_memcpy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("emc_hip_set_option", C.c_int, [C.c_char_p, C.c_char_p]),
@@ -296,7 +298,18 @@ class EmcProblem:
         self.N = int(images.shape[0])
         self.R = int(q.shape[0])
         self.N2d = side * side
-        self.model_host = np.ascontiguousarray(model, np.float32).copy()
+        # the model array cuda_update_slices copies back every iteration, in
+        # page-locked memory (copied at the link's rate)
+        self._model_pin = C.c_void_p()
+        nb = 4 * side ** 3
+        if L.emc_hip_host_alloc(C.byref(self._model_pin), C.c_size_t(nb)) == 0 and self._model_pin.value:
+            buf = (C.c_float * side ** 3).from_address(self._model_pin.value)
+            self.model_host = np.ctypeslib.as_array(buf)
+            self.model_host[:] = np.ascontiguousarray(model, np.float32).reshape(-1)
+        else:
+            L.emc_hip_clear_error()
+            self._model_pin = C.c_void_p()
+            self.model_host = np.ascontiguousarray(model, np.float32).reshape(-1).copy()
         self.weights = np.ascontiguousarray(w, np.float32)
         self.q = np.ascontiguousarray(q, np.float32)
         if x is None:
@@ -323,6 +336,11 @@ class EmcProblem:
         for d in (self.d_images, self.d_mask, self.d_rot, self.d_x, self.d_y, self.d_z, self.d_model,
                   self.d_weight, self.d_slices, self.d_respons, self.d_scaling):
             d.free()
+        if self._model_pin.value:
+            self.model_host = None
+            self.sm = None
+            lib().emc_hip_host_free(self._model_pin)
+            self._model_pin = C.c_void_p()
 
     # ---- the reference's per-iteration calls (emc.c:989-1084) ----
     def get_slices(self):

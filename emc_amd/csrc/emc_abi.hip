@@ -940,6 +940,25 @@ void emc_hip_allocate_bytes(void** d_ptr, size_t bytes) {
   *d_ptr = alloc_dev(bytes);
 }
 
+int emc_hip_host_alloc(void** h_ptr, size_t bytes) {
+  if (!ready() || !h_ptr) return 1;
+  *h_ptr = nullptr;
+  if (hipHostMalloc(h_ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *h_ptr = nullptr;
+    set_error("out of page-locked host memory allocating %zu bytes", bytes);
+    return 1;
+  }
+  return 0;
+}
+
+void emc_hip_host_free(void* h_ptr) {
+  if (!h_ptr) return;
+  Engine& e = engine();
+  if (e.stream) (void)hipStreamSynchronize(e.stream);
+  (void)hipHostFree(h_ptr);
+}
+
 int emc_hip_memcpy_to_host(void* dst, const void* d_src, size_t bytes) {
   if (!ready()) return 1;
   EMC_CHECK(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, engine().stream));

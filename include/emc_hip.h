@@ -44,6 +44,13 @@ real emc_hip_total_respons_n(const real *d_respons, size_t n);
 
 /* Device buffer of `bytes` bytes (the size_t form of cuda_allocate_real). */
 void emc_hip_allocate_bytes(void **d_ptr, size_t bytes);
+/* Page-locked host buffer (no counterpart in the reference): host arrays
+ * the library fills every iteration, such as the model array
+ * cuda_update_slices copies back (emc_cuda.cu:352), are copied at the
+ * link's rate instead of through the runtime's staging of pageable memory.
+ * Returns 0 on success; free with emc_hip_host_free. */
+int emc_hip_host_alloc(void **h_ptr, size_t bytes);
+void emc_hip_host_free(void *h_ptr);
 /* Copies between host and device on the library stream; they return after
  * the copy is complete.  Return 0 on success. */
 int emc_hip_memcpy_to_host(void *dst, const void *d_src, size_t bytes);
