@@ -5,7 +5,7 @@
 // patterns, at most 2 * topk of them) are scored as one dense block: every
 // pattern of the group against every rotation of the core, gathered count
 // rows and gathered slice rows through the FP16 main loop, each count row
-// read once per 128 core rotations instead of once per candidate cell.  The
+// read once per 256 core rotations instead of once per candidate cell.  The
 // block's values for (pattern, rotation) pairs that are candidates go to Lc;
 // the others are dropped.  Candidate cells outside their group's core are
 // scored by the per-cell kernel as before.
@@ -171,11 +171,14 @@ void launch_group_rank(const int* cand, const int* key, int N, int topk, int C, 
 }
 
 // ---------------------------------------------------------------------------
-// Dense block of a group: 128 patterns x 128 core rotations per workgroup.
+// Dense block of a group: 256 patterns x 256 core rotations per workgroup
+// (8 waves, one workgroup per compute unit as in the dense GEMM): every
+// gathered count row is read once per 256 core rotations and every slice row
+// once per 256 patterns of the group.
 // ---------------------------------------------------------------------------
 template <bool KSPLIT>
-__global__ __launch_bounds__(256) void k_group_estep(GroupEstepArgs a) {
-  constexpr int TM = 4, TN = 4, WGM = 2, WGN = 2, NA = KSPLIT ? 2 : 1, NB = 2, STAGES = 3;
+__global__ __launch_bounds__(512, 1) void k_group_estep(GroupEstepArgs a) {
+  constexpr int TM = 8, TN = 4, WGM = 2, WGN = 4, NA = KSPLIT ? 2 : 1, NB = 2, STAGES = KSPLIT ? 2 : 3;
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   static_assert(BM == kGroupTile && BN == kGroupTile, "tile must match the work items");
   __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
@@ -225,9 +228,9 @@ __global__ __launch_bounds__(256) void k_group_estep(GroupEstepArgs a) {
 void launch_group_estep(bool ksplit, const GroupEstepArgs& a, hipStream_t st) {
   if (a.n_work <= 0) return;
   if (ksplit)
-    hipLaunchKernelGGL(k_group_estep<true>, dim3(a.n_work), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_group_estep<true>, dim3(a.n_work), dim3(512), 0, st, a);
   else
-    hipLaunchKernelGGL(k_group_estep<false>, dim3(a.n_work), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_group_estep<false>, dim3(a.n_work), dim3(512), 0, st, a);
   EMC_LAUNCHED();
 }
 

@@ -12,7 +12,7 @@ constexpr int kCandRows = 128;  // entries per E-step workgroup
 constexpr int kSegChunks = 24;  // 32-entry chunks per M-step segment
 constexpr int kMaxTopk = 64;    // candidate cells per pattern at most
 constexpr int kMaxCore = 64;    // core cells of a pattern group at most
-constexpr int kGroupTile = 128; // patterns and core rotations per group-block workgroup
+constexpr int kGroupTile = 256; // patterns and core rotations per group-block workgroup
 
 struct CandEstepArgs {
   const _Float16 *K, *Klo;          // [N_pad][n_mp] packed counts (lo only when split)
