@@ -1004,6 +1004,11 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (is(value, "auto")) e.expand_tiled = false;
     else if (is(value, "tiled")) e.expand_tiled = true;
     else return 1;
+  } else if (is(key, "expand_rotations")) {
+    char* end = nullptr;
+    const long v = strtol(value, &end, 10);
+    if (!end || *end != 0 || v < 1 || v > 64) return 1;
+    e.expand_rotations = (int)v;
   } else if (is(key, "gemm_tile")) {
     if (is(value, "256")) e.gemm_small = false;
     else if (is(value, "128")) e.gemm_small = true;

@@ -81,6 +81,7 @@ struct Engine {
   bool tail_split = true;       // FP16 likelihood GEMM: last partial round of tiles split along k
   bool gemm_small = false;      // FP16 likelihood GEMM: 128 x 128 tiles, two workgroups per compute unit
   bool expand_tiled = false;    // fused expand + pack in rotation blocks x pixel tiles even where the slice fits LDS
+  int expand_rotations = 8;     // rotations per workgroup of that form
   bool timing = false;
   bool verbose = false;
 

@@ -408,7 +408,8 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
 // k_pack_fix then repacks the flagged rows from the FP32 slice.  The pairs
 // are the separate pack's either way (sc is a power of two).
 //
-// A workgroup takes kXR consecutive rotations on one kXT x kXT pixel tile:
+// A workgroup takes xr (option expand_rotations, 8) consecutive rotations on
+// one kXT x kXT pixel tile:
 // nearby orientations on a small patch gather from one small region of the
 // model, which stays in cache across the block's rotations (a whole slice
 // per workgroup, with hundreds of slices in flight per XCD, streamed the
@@ -416,7 +417,7 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
 // maximum |operand| and the partial A_r; k_pack_tile_finish reduces them in
 // tile order (A_r is reproducible), sets the scale and the redo flag and
 // zeroes the padding entries.
-constexpr int kXT = 64, kXR = 8;
+constexpr int kXT = 64;
 
 template <bool TRI, bool BRICK>
 __global__ __launch_bounds__(256) void k_expand_pack_tile(const float* __restrict__ model, int S,
@@ -427,13 +428,13 @@ __global__ __launch_bounds__(256) void k_expand_pack_tile(const float* __restric
                                                           const float* __restrict__ prov_inv,
                                                           _Float16* __restrict__ S16, _Float16* __restrict__ S16lo,
                                                           float* __restrict__ part_max, double* __restrict__ part_A,
-                                                          int poisson) {
+                                                          int poisson, int xr) {
   const int ntx = (S + kXT - 1) / kXT, ntiles = ntx * ntx;
   const int tile = blockIdx.y, tx0 = (tile % ntx) * kXT, ty0 = (tile / ntx) * kXT;
   const int N2d = S * S;
   const float half = (float)S / 2.0f + 0.5f;
-  for (int rr = 0; rr < kXR; rr++) {
-    const int r = blockIdx.x * kXR + rr;
+  for (int rr = 0; rr < xr; rr++) {
+    const int r = blockIdx.x * xr + rr;
     if (r >= R_pad) break;  // uniform over the workgroup
     const bool live = r < R;
     _Float16* hrow = S16 + (size_t)r * n_mp;
@@ -572,22 +573,23 @@ void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, 
     // the previous pack's row scales, if they were for these rotations
     const float* prov = e.sscale_R == R ? scale : nullptr;
     const bool tri = interp == INTERP_TRILINEAR;
-    const dim3 gt((R_pad + kXR - 1) / kXR, ntiles), gb(256);
+    const int xr = e.expand_rotations;
+    const dim3 gt((R_pad + xr - 1) / xr, ntiles), gb(256);
     int* redo = e.d_redo.as<int>();
     float* pmx = e.d_xpart_max.as<float>();
     double* pA = e.d_xpart_A.as<double>();
     if (tri && bricked)
       hipLaunchKernelGGL((k_expand_pack_tile<true, true>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po);
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr);
     else if (tri)
       hipLaunchKernelGGL((k_expand_pack_tile<true, false>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po);
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr);
     else if (bricked)
       hipLaunchKernelGGL((k_expand_pack_tile<false, true>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po);
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr);
     else
       hipLaunchKernelGGL((k_expand_pack_tile<false, false>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po);
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr);
     EMC_LAUNCHED();
     hipLaunchKernelGGL(k_pack_tile_finish, dim3(R_pad), dim3(64), 0, st, pmx, pA, ntiles, R, n_m, n_mp, scale, A,
                        redo, (int)(prov != nullptr), S16, S16lo);

@@ -84,6 +84,8 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 slice of at most 64 KB in LDS, larger slices run in blocks
  *                 of 8 rotations x 64 x 64 pixel tiles with provisional row
  *                 scales; "tiled": the latter at every size.
+ *   "expand_rotations"  rotations per workgroup of the tiled expand + pack
+ *                 (default "8"; 1 .. 64).
  *   "gemm_tile"  "256" (default): 256 x 256 tiles of the FP16 likelihood
  *                 GEMM, one workgroup per compute unit; "128": 128 x 128
  *                 tiles, two workgroups per compute unit (same results, bit
