@@ -250,7 +250,8 @@ __global__ __launch_bounds__(256) void k_mstep_zero(const int* __restrict__ rows
 
 // the same with 16-byte stores: four pixels per lane, all four written at
 // once when all are masked in (m4: bit q for pixel 4v + q), else one by
-// one.  kZeroRows rows per workgroup share the mask words, loaded once.
+// one.  A workgroup takes one chunk of 1,024 groups (blockIdx.y) of kZeroRows
+// rows, which share the mask words, loaded once.
 constexpr int kZeroRows = 4;
 
 __global__ __launch_bounds__(256) void k_mstep_zero4(const int* __restrict__ rows, int nrows,
@@ -258,7 +259,8 @@ __global__ __launch_bounds__(256) void k_mstep_zero4(const int* __restrict__ row
   typedef float f4 __attribute__((ext_vector_type(4)));
   const int nv = a.N2d / 4;
   const f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int v0 = 0; v0 < nv; v0 += 256 * 4) {
+  {
+    const int v0 = blockIdx.y * 256 * 4;  // this workgroup's quarter-row chunk
     int m[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -377,8 +379,8 @@ bool launch_mstep_sparse(const std::vector<int>& h_off, const int* off, const in
       e.mask4_gen = e.pix_gen;
     }
     if (vec)
-      hipLaunchKernelGGL(k_mstep_zero4, dim3((nz + kZeroRows - 1) / kZeroRows), dim3(256), 0, st,
-                         e.d_sp_zero.as<int>(), nz, e.d_mask4.as<int>(), a);
+      hipLaunchKernelGGL(k_mstep_zero4, dim3((nz + kZeroRows - 1) / kZeroRows, (a.N2d / 4 + 1023) / 1024), dim3(256),
+                         0, st, e.d_sp_zero.as<int>(), nz, e.d_mask4.as<int>(), a);
     else
       hipLaunchKernelGGL(k_mstep_zero, dim3(nz), dim3(256), 0, st, e.d_sp_zero.as<int>(), a);
     EMC_LAUNCHED();
