@@ -105,6 +105,113 @@ def test_half_table_distinct_and_weights(n):
     assert (dd.max(axis=1) > 1 - 1e-6).all()
 
 
+def _cells_of_600_cell():
+    """The 600 tetrahedral cells of the 600-cell, found from the n = 1 table
+    (whose vertices are pinned above) as the 4-cliques of its edge graph
+    (neighbours at cos 36 deg = phi / 2), and the distance h of every cell's
+    flat hyperplane from the origin."""
+    v = O.rotation_list(1)[0].astype(np.float64)
+    adj = np.abs(v @ v.T - PHI / 2) < 1e-5
+    cells = []
+    for a in range(120):
+        nb = [b for b in np.nonzero(adj[a])[0] if b > a]
+        for b, c, d in __import__("itertools").combinations(nb, 3):
+            if adj[b, c] and adj[b, d] and adj[c, d]:
+                cells.append((a, b, c, d))
+    cen = v[np.array(cells)].mean(axis=1)
+    h = np.linalg.norm(cen, axis=1)
+    return cen / h[:, None], h
+
+
+def _unnormalised_norms(q, normals, h):
+    """|q_un| of every table point before it was projected onto the sphere
+    (emc.c:355-360): the table interpolates linearly inside the flat cells of
+    the 600-cell, so q_un lies in the hyperplane {x : x . c = h} of the cell
+    containing it, the one whose unit normal c makes the largest q . c."""
+    return h / (q @ normals.T).max(axis=1)
+
+
+def _classes(n, count):
+    """0 vertex, 1 edge, 2 face, 3 cell interior: by position in the table."""
+    ev, fv = n - 1, (n - 1) * (n - 2) // 2
+    cls = np.full(count, 3)
+    cls[:120] = 0
+    cls[120:120 + 720 * ev] = 1
+    cls[120 + 720 * ev:120 + 720 * ev + 1200 * fv] = 2
+    return cls
+
+
+def test_600_cell_has_600_equidistant_cells():
+    normals, h = _cells_of_600_cell()
+    assert normals.shape == (600, 4)
+    # float32 vertices: equal to 1e-7
+    assert np.ptp(h) < 1e-7
+    # a regular tetrahedron with vertices on the unit sphere and edge 1/phi
+    assert abs(h[0] - math.sqrt(1 - 3 * (1 / PHI) ** 2 / 8)) < 1e-7
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 6])
+def test_table_weights_follow_inverse_cube_law(n):
+    """Weights are f_class / (N |q_un|^3) (emc.c:212-217,229-334): within
+    each class of points w * |q_un|^3 is one constant.  |q_un| comes from
+    the cell geometry, not from the table code; an exponent 2 or 4, or a
+    weight computed after normalising q, makes w * |q_un|^3 vary within the
+    edge class (n >= 4) or the face and cell classes."""
+    normals, h = _cells_of_600_cell()
+    q, w = O.rotation_list(n)
+    q = q.astype(np.float64)
+    r = _unnormalised_norms(q, normals, h[0])
+    assert r.max() <= 1 + 1e-6 and r.min() > 0.9
+    k = w.astype(np.float64) * r ** 3
+    cls = _classes(n, q.shape[0])
+    for c in range(4):
+        if (cls == c).any():
+            kc = k[cls == c]
+            assert np.ptp(kc) / kc.mean() < 2e-6, (c, np.ptp(kc) / kc.mean())
+    if n >= 4:
+        # the test discriminates: |q_un| varies along the edges
+        re = r[cls == 1]
+        assert np.ptp(re) > 1e-3
+
+
+def _haar_mean_exp(a):
+    """E[exp(a (u.q)^2)] for q uniform on S^3 (the Haar measure of SO(3)):
+    q0 has density (2/pi) sqrt(1 - t^2), which gives
+    exp(a/2) (I0(a/2) - I1(a/2))."""
+    from scipy import special
+    return math.exp(a / 2) * (special.i0(a / 2) - special.i1(a / 2))
+
+
+def test_haar_closed_form():
+    from scipy import integrate
+    a = 10.0
+    num = integrate.quad(lambda t: 2 / math.pi * math.exp(a * t * t) * math.sqrt(1 - t * t), -1, 1)[0]
+    assert abs(num - _haar_mean_exp(a)) < 1e-9 * num
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_table_weights_are_a_haar_quadrature(half):
+    """emc.c:212-345: the weighted n = 4 table integrates exp(10 (u.q)^2)
+    over SO(3) to within 5e-5 of the Haar mean for six directions u.  Equal
+    weights miss by up to 2.8e-3; swapping the vertex and edge factors, using
+    1 for both, or an exponent 2 or 4 misses by more than 6e-4."""
+    a = 10.0
+    exact = _haar_mean_exp(a)
+    q, w = O.rotation_list(4, half=half)
+    q = q.astype(np.float64)
+    w = w.astype(np.float64)
+    w = w / w.sum()  # the table's float running sum leaves sum(w) = 1 +- 1e-5
+    rng = np.random.default_rng(1)
+    u = rng.normal(size=(6, 4))
+    u[0] = (1, 0, 0, 0)
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    f = np.exp(a * (q @ u.T) ** 2)
+    err = np.abs(w @ f - exact) / exact
+    assert err.max() < 5e-5, err
+    eq = np.abs(f.mean(axis=0) - exact) / exact
+    assert eq.max() > 1e-3  # equal weights fail the same check
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 10])
 def test_product_table_matches_oracle_bitwise(n):
     for half in (False, True):
