@@ -66,12 +66,12 @@ def phase_times(L):
 
 
 def proposal_stats(L):
-    out = (C.c_double * 21)()
-    k = L.emc_hip_proposal_stats(out, 21)
+    out = (C.c_double * 22)()
+    k = L.emc_hip_proposal_stats(out, 22)
     keys = ["iteration", "full_search", "audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass",
             "pairs_scored", "gated_at_prediction", "train_loss", "train_ms", "predict_ms", "whole_table",
             "candidate_pairs", "recentred", "whole_table_iteration", "shared_cells", "recentre_ms", "group_pairs",
-            "resid_pairs", "shared_rotations", "respons_cleared"]
+            "resid_pairs", "shared_rotations", "respons_cleared", "core_cells_truncated"]
     return {keys[j]: out[j] for j in range(k)}
 
 

@@ -1,8 +1,10 @@
 // Pattern groups of the candidate E-step.  Patterns whose candidate lists
 // start with the same cell (with re-centring: the nearest free cell of the
 // same best cell) share most of their candidate cells.  For such a group,
-// the core cells (those among the candidates of at least 1 / core_div (a quarter) of its
-// patterns, at most 2 * topk of them) are scored as one dense block: every
+// the core cells (those among the candidates of at least 1 / core_div of its
+// patterns, so at most core_div * topk of them; the kMaxCore lowest-numbered
+// are kept and the number left out is counted in `trunc`, a proposal
+// statistic) are scored as one dense block: every
 // pattern of the group against every rotation of the core, gathered count
 // rows and gathered slice rows through the FP16 main loop, each count row
 // read once per 256 core rotations instead of once per candidate cell.  The
@@ -41,12 +43,13 @@ __global__ __launch_bounds__(256) void k_group_hist(const int* __restrict__ cand
 
 // One workgroup per group key: its core cells in increasing order (held by
 // at least 1 / core_div of the group's patterns; groups smaller than min_size get
-// none), their count and fine-rotation total.
+// none), their count and fine-rotation total; qualifying cells beyond the
+// first kMaxCore are left to the per-cell kernel and counted in *trunc.
 __global__ __launch_bounds__(256) void k_group_core(const int* __restrict__ hist, const int* __restrict__ count,
                                                     int C, int min_size, int core_div,
                                                     const int* __restrict__ cell_off,
                                                     int* __restrict__ core, int* __restrict__ ncore,
-                                                    int* __restrict__ nrot) {
+                                                    int* __restrict__ nrot, int* __restrict__ trunc) {
   const int g = blockIdx.x;
   const int n = count[g];
   __shared__ int s_n, s_rot;
@@ -79,6 +82,7 @@ __global__ __launch_bounds__(256) void k_group_core(const int* __restrict__ hist
   if (threadIdx.x == 0) {
     ncore[g] = min(s_n, kMaxCore);
     nrot[g] = s_rot;
+    if (s_n > kMaxCore) atomicAdd(trunc, s_n - kMaxCore);
   }
 }
 
@@ -147,13 +151,15 @@ bool build_group_keys(const int* cand, int N, int topk, int C, int* key, int* co
 }
 
 void launch_group_core(const int* cand, const int* key, const int* count, int N, int topk, int C, int min_size,
-                       int core_div, const int* cell_off, int* hist, int* core, int* ncore, int* nrot, hipStream_t st) {
+                       int core_div, const int* cell_off, int* hist, int* core, int* ncore, int* nrot, int* trunc,
+                       hipStream_t st) {
   launch_fill(reinterpret_cast<float*>(hist), (size_t)C * C, 0.0f, st);
+  launch_fill(reinterpret_cast<float*>(trunc), 1, 0.0f, st);
   const long n = (long)N * topk;
   hipLaunchKernelGGL(k_group_hist, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cand, key, N, topk, C, hist);
   EMC_LAUNCHED();
   hipLaunchKernelGGL(k_group_core, dim3(C), dim3(256), 0, st, hist, count, C, min_size, core_div, cell_off, core, ncore,
-                     nrot);
+                     nrot, trunc);
   EMC_LAUNCHED();
 }
 

@@ -33,7 +33,7 @@ void Proposal::release() {
                   &d_Psub16, &d_Psub16lo, &d_mass, &d_tot, &d_iota, &d_part_c, &d_part_m, &d_sh_cells, &d_sh_off, &d_sh_rot,
                   &d_Ssh, &d_Sshlo, &d_Ash, &d_sscale_sh, &d_Lsh, &d_top_s, &d_topm_s, &d_net_cand,
                   &d_nbr, &d_mark, &d_shared, &d_count, &d_ncand, &d_moved, &d_keys_s, &d_vals_s, &d_sort_tmp, &d_colmap,
-                  &d_gkey, &d_gcount, &d_gpat, &d_gkey_s, &d_ghist, &d_gcore, &d_gncore, &d_gnrot, &d_gcofs, &d_crot,
+                  &d_gkey, &d_gcount, &d_gpat, &d_gkey_s, &d_ghist, &d_gcore, &d_gncore, &d_gnrot, &d_gcofs, &d_gtrunc, &d_crot,
                   &d_ccell, &d_cslot, &d_rank, &d_rkey, &d_gwork, &d_rcount, &d_rncand, &d_rent_off, &d_rent_pat,
                   &d_rent_j, &d_rwork, &d_rkeys_s, &d_rvals_s};
   for (DBuf* b : bufs) b->release();
@@ -188,26 +188,29 @@ static bool build_groups(Proposal& p) {
   p.n_gwork = 0;
   p.n_rwork = 0;
   p.st_group_pairs = 0;
+  p.st_core_trunc = 0;
   p.st_resid_pairs = p.st_cand_pairs;
   if (p.group_min <= 0) return true;
   if (!p.d_gkey.ensure(sizeof(int) * N) || !p.d_gcount.ensure(sizeof(int) * C) || !p.d_gpat.ensure(sizeof(int) * N) ||
       !p.d_ghist.ensure(sizeof(int) * (size_t)C * C) || !p.d_gcore.ensure(sizeof(int) * (size_t)C * kMaxCore) ||
-      !p.d_gncore.ensure(sizeof(int) * C) || !p.d_gnrot.ensure(sizeof(int) * C))
+      !p.d_gncore.ensure(sizeof(int) * C) || !p.d_gnrot.ensure(sizeof(int) * C) || !p.d_gtrunc.ensure(sizeof(int)))
     return false;
   if (!build_group_keys(p.d_cand.as<int>(), N, K, C, p.d_gkey.as<int>(), p.d_gcount.as<int>(), p.d_gpat.as<int>(),
                         p.d_gkey_s, p.d_sort_tmp, st))
     return false;
   launch_group_core(p.d_cand.as<int>(), p.d_gkey.as<int>(), p.d_gcount.as<int>(), N, K, C, p.group_min, p.core_div,
                     p.d_cell_off.as<int>(), p.d_ghist.as<int>(), p.d_gcore.as<int>(), p.d_gncore.as<int>(),
-                    p.d_gnrot.as<int>(), st);
+                    p.d_gnrot.as<int>(), p.d_gtrunc.as<int>(), st);
   p.h_gcount.resize(C);
   p.h_gncore.resize(C);
   p.h_gnrot.resize(C);
   EMC_CHECK(hipMemcpyAsync(p.h_gcount.data(), p.d_gcount.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
   EMC_CHECK(hipMemcpyAsync(p.h_gncore.data(), p.d_gncore.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
   EMC_CHECK(hipMemcpyAsync(p.h_gnrot.data(), p.d_gnrot.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
+  EMC_CHECK(hipMemcpyAsync(&p.h_gtrunc, p.d_gtrunc.p, sizeof(int), hipMemcpyDeviceToHost, st));
   sync_stream();
   if (failed()) return false;
+  p.st_core_trunc = p.h_gtrunc;
   // block work items: (first pattern in the sorted list, count, first core
   // rotation, count), 128 x 128
   p.h_gcofs.assign(C + 1, 0);
@@ -1196,13 +1199,13 @@ int emc_hip_proposal_shared(int* cells, int max_cells) {
 
 int emc_hip_proposal_stats(double* out, int n) {
   const Proposal& p = proposal();
-  constexpr int kStats = 21;
+  constexpr int kStats = 22;
   const double v[kStats] = {(double)p.iteration, p.st_full,     p.st_audited,    p.st_fail,
                             p.st_min_mass,       p.st_mean_mass, p.st_pairs,      p.st_gated,
                             p.st_train_loss,     p.st_train_ms,  p.st_predict_ms, (double)p.n_sub,
                             p.st_cand_pairs,     p.st_moved,     p.st_dense,      p.st_shared,
                             p.st_recentre_ms,    p.st_group_pairs, p.st_resid_pairs, (double)p.R_sh,
-                            p.st_cleared};
+                            p.st_cleared,        p.st_core_trunc};
   int k = 0;
   for (; k < n && k < kStats; k++) out[k] = v[k];
   return k;

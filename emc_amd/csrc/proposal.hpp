@@ -81,15 +81,18 @@ struct Proposal {
   // pattern groups of the candidate E-step: patterns sharing their first
   // candidate cell, whose core cells (candidates of at least half of them)
   // are scored as dense blocks; the per-cell E-step takes the other entries
+  // (core cells beyond kMaxCore included: st_core_trunc counts them)
   int group_min = 64;                // smallest group scored as blocks (0: no groups)
   int core_div = 4;                  // core cell: held by >= 1 / core_div of its group's patterns
   DBuf d_gkey, d_gcount, d_gpat, d_gkey_s, d_ghist, d_gcore, d_gncore, d_gnrot, d_gcofs, d_crot, d_ccell, d_cslot,
-      d_rank, d_rkey, d_gwork, d_rcount, d_rncand;
+      d_gtrunc, d_rank, d_rkey, d_gwork, d_rcount, d_rncand;
   std::vector<int> h_gcount, h_gncore, h_gnrot, h_gcofs, h_rcount, h_rent_off;
   std::vector<int4> h_gwork;
   int n_gwork = 0;
   double st_group_pairs = 0;         // (pattern, rotation) pairs of the group blocks
   double st_resid_pairs = 0;         // candidate pairs left to the per-cell kernel
+  double st_core_trunc = 0;          // qualifying group-core cells beyond kMaxCore (scored per cell)
+  int h_gtrunc = 0;
   double st_cleared = 0;             // responsibility entries cleared by the last write (N * R unless delta)
   DBuf d_rent_off, d_rent_pat, d_rent_j, d_rwork, d_rkeys_s, d_rvals_s;
   int n_rwork = 0;

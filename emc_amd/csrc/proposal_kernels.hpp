@@ -168,9 +168,11 @@ bool sort_entries(const int* cand, int N, int topk, int C, DBuf& keys_sorted, DB
 // pattern groups of the candidate E-step (k_groups.hip)
 bool build_group_keys(const int* cand, int N, int topk, int C, int* key, int* count, int* gpat, DBuf& key_sorted,
                       DBuf& temp, hipStream_t st);
-// hist: C * C ints of scratch; core: [C][kMaxCore]
+// hist: C * C ints of scratch; core: [C][kMaxCore]; *trunc: qualifying core
+// cells left out by the kMaxCore cap, over all groups
 void launch_group_core(const int* cand, const int* key, const int* count, int N, int topk, int C, int min_size,
-                       int core_div, const int* cell_off, int* hist, int* core, int* ncore, int* nrot, hipStream_t st);
+                       int core_div, const int* cell_off, int* hist, int* core, int* ncore, int* nrot, int* trunc,
+                       hipStream_t st);
 void launch_group_rots(const int* core, const int* ncore, const int* cofs, const int* cell_off, const int* cell_rot,
                        int C, int* crot, unsigned char* ccell, unsigned char* cslot, hipStream_t st);
 void launch_group_rank(const int* cand, const int* key, int N, int topk, int C, const int* core, const int* ncore,

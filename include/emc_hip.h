@@ -235,8 +235,9 @@ int emc_hip_proposal_shared(int *cells, int max_cells);
  * rotation) pairs of the group blocks, [18] candidate pairs left to the
  * per-cell kernel, [19] shared rotations, [20] responsibility entries the
  * last iteration cleared before writing its own (N_images * N_slices, or the
- * previous iteration's nonzeros with "respons_fill" "delta").  Returns the
- * number of values
+ * previous iteration's nonzeros with "respons_fill" "delta"), [21] cells
+ * that qualified for a group's core beyond its cap of 64 (scored by the
+ * per-cell kernel instead).  Returns the number of values
  * written (<= n). */
 int emc_hip_proposal_stats(double *out, int n);
 

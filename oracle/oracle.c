@@ -830,3 +830,174 @@ void o64_normalize_model(double *model, const double *weight, int n) {
   s /= n;
   for (int k = 0; k < n; k++) model[k] /= s;
 }
+
+/* ------------------------------------------------------------------------ */
+/* FP64 iteration for problems too large for the dense loops above: the same  */
+/* equations as o64_expand .. o64_normalize_model (nearest voxel), evaluated  */
+/* over the masked pixels only, with every pattern's nonzero counts as a list */
+/* and the zero responsibilities skipped in the scaling and M-step sums.     */
+/* The Gaussian likelihood uses the expansion                                 */
+/*   sum m (S - K/phi)^2 = sum m S^2 - 2 sum m K S / phi + sum m K^2 / phi^2, */
+/* exact in real arithmetic; in FP64 its cancellation costs ~1e-12 relative.  */
+/* Pinned against the dense o64_* functions on small problems by the tests.   */
+/* allowed ([R][N], may be NULL): the softmax of pattern i runs over the r    */
+/* with allowed[r*N+i] != 0 only (o64_restricted_softmax's rule).             */
+/* ------------------------------------------------------------------------ */
+double o64_iteration_large(const double *model_in, const float *images, const int *mask,
+                           const double *scaling_in, int N, int R, int S, const float *q,
+                           const float *weights, const float *x, const float *y, const float *z,
+                           double sigma, int kind, const unsigned char *allowed, double *L,
+                           double *P, double *scaling_out, double *model_out) {
+  const int N2d = S * S;
+  const size_t n3 = (size_t)S * S * S;
+  int nm = 0;
+  for (int p = 0; p < N2d; p++) nm += mask[p] != 0;
+  int *pix = (int *)malloc(sizeof(int) * (nm > 0 ? nm : 1));
+  nm = 0;
+  for (int p = 0; p < N2d; p++)
+    if (mask[p] != 0) pix[nm++] = p;
+  /* nonzero counts of each pattern over the masked pixels (compact index) */
+  size_t *off = (size_t *)malloc(sizeof(size_t) * (N + 1));
+  off[0] = 0;
+  for (int i = 0; i < N; i++) {
+    size_t c = 0;
+    for (int k = 0; k < nm; k++) c += images[(size_t)i * N2d + pix[k]] != 0.0f;
+    off[i + 1] = off[i] + c;
+  }
+  int *nzk = (int *)malloc(sizeof(int) * (off[N] > 0 ? off[N] : 1));
+  double *nzv = (double *)malloc(sizeof(double) * (off[N] > 0 ? off[N] : 1));
+  double *B = (double *)calloc(N, sizeof(double)); /* Gaussian: sum K^2; Poisson: sum K */
+  for (int i = 0; i < N; i++) {
+    size_t c = off[i];
+    for (int k = 0; k < nm; k++) {
+      const double v = images[(size_t)i * N2d + pix[k]];
+      if (v != 0.0) {
+        nzk[c] = k;
+        nzv[c++] = v;
+        B[i] += kind == O_GAUSSIAN ? v * v : v;
+      }
+    }
+  }
+  double *A = (double *)calloc(R, sizeof(double)); /* Gaussian: sum S^2; Poisson: sum S */
+  double *Cx = (double *)malloc(sizeof(double) * (size_t)R * N); /* sum m K S (Gaussian) */
+  const double inv = 1.0 / (2.0 * (double)nm * sigma * sigma);
+  /* expand + likelihood, one rotation per task */
+#pragma omp parallel
+  {
+    double *sl = (double *)malloc(sizeof(double) * (nm > 0 ? nm : 1));
+    double *op = (double *)malloc(sizeof(double) * (nm > 0 ? nm : 1));
+#pragma omp for schedule(dynamic, 4)
+    for (int r = 0; r < R; r++) {
+      double a = 0.0;
+      for (int k = 0; k < nm; k++) {
+        const int p = pix[k];
+        int vx, vy, vz;
+        o_voxel_index(&q[4 * r], x[p], y[p], z[p], S, &vx, &vy, &vz);
+        double v = 0.0;
+        if (vx > 0 && vx < S && vy > 0 && vy < S && vz > 0 && vz < S)
+          v = model_in[(size_t)vz * S * S + (size_t)vy * S + vx];
+        sl[k] = v;
+        a += kind == O_GAUSSIAN ? v * v : v;
+        op[k] = kind == O_GAUSSIAN ? v : log(v > 1e-6 ? v : 1e-6);
+      }
+      A[r] = a;
+      for (int i = 0; i < N; i++) {
+        double c = 0.0;
+        for (size_t e = off[i]; e < off[i + 1]; e++) c += nzv[e] * op[nzk[e]];
+        const double phi = scaling_in[i];
+        Cx[(size_t)r * N + i] = c;
+        L[(size_t)r * N + i] = kind == O_GAUSSIAN ? -(a - 2.0 * c / phi + B[i] / (phi * phi)) * inv
+                                                  : c + B[i] * log(phi) - phi * a;
+      }
+    }
+    free(sl);
+    free(op);
+  }
+  /* responsibilities: per-pattern softmax (restricted to the allowed set) */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double m = -1e10;
+    for (int r = 0; r < R; r++)
+      if (!allowed || allowed[(size_t)r * N + i]) m = L[(size_t)r * N + i] > m ? L[(size_t)r * N + i] : m;
+    double s = 0.0;
+    for (int r = 0; r < R; r++) {
+      const double d = L[(size_t)r * N + i] - m;
+      const double e = (!allowed || allowed[(size_t)r * N + i]) && d > -1e10 ? exp(d) : 0.0;
+      P[(size_t)r * N + i] = e;
+      s += e;
+    }
+    for (int r = 0; r < R; r++) P[(size_t)r * N + i] /= s;
+  }
+  double tot = 0.0;
+  for (size_t k = 0; k < (size_t)R * N; k++) tot += P[k] > 0 ? P[k] * log(P[k]) : 0.0;
+  /* scaling: Gaussian sum K^2 / sum_r P sum K S; Poisson sum K / sum_r P sum S */
+  for (int i = 0; i < N; i++) {
+    double den = 0.0;
+    for (int r = 0; r < R; r++) {
+      const double pr = P[(size_t)r * N + i];
+      if (pr != 0.0) den += pr * (kind == O_GAUSSIAN ? Cx[(size_t)r * N + i] : A[r]);
+    }
+    scaling_out[i] = B[i] / den;
+  }
+  /* M-step and insertion (nearest voxel, mask == 1, T_r > 1e-10): rotations
+   * in chunks; each chunk's updated slices and voxel indices are computed in
+   * parallel, then added to the model sequentially in rotation order */
+  enum { CH = 128 };
+  double *sv = (double *)malloc(sizeof(double) * (size_t)CH * (nm > 0 ? nm : 1));
+  long *vi = (long *)malloc(sizeof(long) * (size_t)CH * (nm > 0 ? nm : 1));
+  double *wr = (double *)malloc(sizeof(double) * CH);
+  double *wsum = (double *)calloc(n3, sizeof(double));
+  memset(model_out, 0, sizeof(double) * n3);
+  for (int r0 = 0; r0 < R; r0 += CH) {
+    const int nr = R - r0 < CH ? R - r0 : CH;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 0; j < nr; j++) {
+      const int r = r0 + j;
+      double *u = sv + (size_t)j * nm;
+      long *vv = vi + (size_t)j * nm;
+      double t = 0.0, tphi = 0.0;
+      for (int k = 0; k < nm; k++) u[k] = 0.0;
+      for (int i = 0; i < N; i++) {
+        const double pr = P[(size_t)r * N + i];
+        if (pr == 0.0) continue;
+        t += pr;
+        tphi += pr * scaling_out[i];
+        const double f = kind == O_GAUSSIAN ? pr / scaling_out[i] : pr;
+        for (size_t e = off[i]; e < off[i + 1]; e++) u[nzk[e]] += nzv[e] * f;
+      }
+      wr[j] = t > 1e-10 ? (double)weights[r] * t : 0.0;
+      if (!(t > 1e-10)) continue;
+      for (int k = 0; k < nm; k++) {
+        const int p = pix[k];
+        u[k] = kind == O_GAUSSIAN ? u[k] / t : (tphi > 1e-30 ? u[k] / tphi : 0.0);
+        int vx, vy, vz;
+        o_voxel_index(&q[4 * r], x[p], y[p], z[p], S, &vx, &vy, &vz);
+        vv[k] = (mask[p] == 1 && vx >= 0 && vx < S && vy >= 0 && vy < S && vz >= 0 && vz < S)
+                    ? (long)vz * S * S + (long)vy * S + vx
+                    : -1;
+      }
+    }
+    for (int j = 0; j < nr; j++) {
+      if (wr[j] == 0.0) continue;
+      for (int k = 0; k < nm; k++) {
+        const long v = vi[(size_t)j * nm + k];
+        if (v < 0) continue;
+        model_out[v] += wr[j] * sv[(size_t)j * nm + k];
+        wsum[v] += wr[j];
+      }
+    }
+  }
+  o64_normalize_model(model_out, wsum, (int)n3);
+  free(wsum);
+  free(wr);
+  free(vi);
+  free(sv);
+  free(Cx);
+  free(A);
+  free(B);
+  free(nzv);
+  free(nzk);
+  free(off);
+  free(pix);
+  return tot;
+}

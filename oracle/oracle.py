@@ -52,6 +52,8 @@ _SIGS = [
     ("r32_iteration", C.c_double, [_f, _f, _f, _f, _i, _f, _f, C.c_int, C.c_int, C.c_int, _f, _f, _f, _f, _f,
                                    C.c_float, C.c_int]),
     ("o_max_threads", C.c_int, []),
+    ("o64_iteration_large", C.c_double, [_d, _f, _i, _d, C.c_int, C.c_int, C.c_int, _f, _f, _f, _f, _f,
+                                         C.c_double, C.c_int, C.c_void_p, _d, _d, _d, _d]),
 ]
 
 _LIB = None
@@ -312,6 +314,30 @@ def o64_iteration(model, images, mask, scaling, S, q, weights, x, y, z, sigma=0.
     mn = o64_normalize_model(m, wg)
     return {"slices": sl, "L": L, "P": P, "total": tot, "scaling": phi, "slices_new": sl2, "T": T,
             "model_raw": m, "weight": wg, "model": mn}
+
+
+def o64_iteration_large(model, images, mask, scaling, S, q, weights, x, y, z, sigma=0.4, kind=GAUSSIAN,
+                        allowed=None):
+    """One FP64 iteration (nearest voxel) for problems too large for
+    o64_iteration: the same equations over the masked pixels, with sparse
+    count lists and zero responsibilities skipped.  `allowed` ([R][N] bool)
+    restricts each pattern's softmax as o64_restricted_softmax does."""
+    images = c32(images)
+    N = images.shape[0]
+    R = q.shape[0]
+    L = np.zeros((R, N), np.float64)
+    P = np.zeros((R, N), np.float64)
+    phi = np.zeros(N, np.float64)
+    m = np.zeros(S ** 3, np.float64)
+    al = None
+    if allowed is not None:
+        allowed = np.ascontiguousarray(allowed, np.uint8)
+        assert allowed.shape == (R, N)
+        al = allowed.ctypes.data_as(C.c_void_p)
+    tot = lib().o64_iteration_large(d(c64(model)), f(images), i(ci(mask)), d(c64(scaling)), N, R, S, f(c32(q)),
+                                    f(c32(weights)), f(c32(x)), f(c32(y)), f(c32(z)), sigma, kind, al, d(L), d(P),
+                                    d(phi), d(m))
+    return {"L": L, "P": P, "total": tot, "scaling": phi, "model": m}
 
 
 # ---- learned proposal: candidate-restricted iteration (numpy, FP64) ----

@@ -81,8 +81,8 @@ def init_cells(P, topk, n_coarse=2, batch=64):
 
 
 def stats():
-    out = (C.c_double * 21)()
-    assert E.lib().emc_hip_proposal_stats(out, 21) == 21
+    out = (C.c_double * 22)()
+    assert E.lib().emc_hip_proposal_stats(out, 22) == 22
     return list(out)
 
 
@@ -480,3 +480,76 @@ def test_delta_fill_matches_full(prob):
         assert d["nnz"] == (d["P"] != 0).sum()
         for k in ("P", "scaling", "model"):
             assert np.array_equal(f[k], d[k]), (it, k)
+
+
+def test_headline_path_poisson_matches_oracle():
+    """The benchmark's learned path end to end at 32^3: Poisson likelihood,
+    FP16 hi/lo, one dense iteration whose responsibilities train the network,
+    prediction of the top-k cells, then proposal iterations with posterior
+    tracking, shared cells, group blocks, the M-step over nonzero lists, a
+    whole-table audit of every pattern in the first proposal iteration and
+    of 2% afterwards (fallback of the iteration at 0.5% failures, per-pattern
+    full search below 0.999).  Every proposal iteration, from the GPU's own
+    model and scaling, must equal the FP64 oracle restricted to the rotations
+    it scored (each pattern's candidate cells + the shared cells' rotations,
+    the whole table for the audited and full-search patterns, or everything
+    when the audit sent the iteration to the whole table)."""
+    P = problems.make(side=32, n_patterns=400, n_rot=4, n_blobs=20, radius=6.0, photons=2000.0, phi_sigma=0.3,
+                      beamstop=3.0, gap_cross=1, seed=21, model="random")
+    topk = 6
+    nc, off, rot = init_cells(P, topk, batch=64)
+    L = E.lib()
+    opts(insert="atomic", mstep="auto", group_min="16", shared="1000", track="on", min_mass="0.999",
+         max_fail="0.005", audit="1")
+    E.set_option("likelihood", "poisson")
+    try:
+        prob_ = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
+        prob_.iteration(0.4)
+        loss = C.c_double()
+        assert L.emc_hip_proposal_train(prob_.d_images.vp, prob_.d_mask.vp, P.side ** 2, prob_.d_respons.vp, P.N,
+                                        P.R, 4, 3e-3, C.byref(loss)) == 0
+        assert L.emc_hip_proposal_predict(prob_.d_images.vp, prob_.d_mask.vp, P.side ** 2, P.N) == 0
+        E.set_option("proposal", "on")
+        state = {"model": prob_.model(), "scaling": prob_.scaling()}
+        mode_prev = np.zeros(P.N, np.int32)
+        worst = [0.0, 0.0, 0.0]
+        for it in range(1, 5):
+            frac = 1.0 if it == 1 else 0.02
+            E.set_option("audit_fraction", str(frac))
+            res = prob_.iteration(0.4)
+            g = {"P": prob_.respons(), "scaling": prob_.scaling(), "model": prob_.model()}
+            st = stats()
+            cand = np.zeros((P.N, topk), np.int32)
+            mode = np.zeros(P.N, np.int32)
+            assert L.emc_hip_proposal_get(E.iptr(cand), E.iptr(mode), None, P.N) == 0
+            n_sh = L.emc_hip_proposal_shared(None, 0)
+            shared = np.zeros(max(n_sh, 1), np.int32)
+            assert L.emc_hip_proposal_shared(E.iptr(shared), n_sh) == n_sh
+            sub = np.union1d(audited(P.N, it, frac), np.nonzero(mode_prev)[0])
+            if st[14]:
+                allowed = np.ones((P.R, P.N), bool)
+            else:
+                allowed = O.allowed_from_candidates(cand, off, rot, P.R, sub)
+                for c in shared[:n_sh]:
+                    allowed[rot[off[c]:off[c + 1]], :] = True
+            ref = O.o64_proposal_iteration(state["model"].astype(np.float64), P.images, P.mask, state["scaling"],
+                                           P.side, P.q, P.w, P.x, P.y, P.z, allowed, kind=O.POISSON)
+            dP = np.abs(g["P"] - ref["P"]).max()
+            ds = rel(g["scaling"], ref["scaling"])
+            dm = rel(g["model"], ref["model"])
+            dl = abs(res["likelihood"] - ref["total"]) / max(abs(ref["total"]), 1e-30)
+            print(f"proposal iteration {it}: whole table {st[11]:.0f} (expected {len(sub)}), whole-table "
+                  f"iteration {st[14]:.0f}, shared cells {n_sh}, full search {st[1]:.0f}, pairs {st[6]:.0f} "
+                  f"(allowed {allowed.sum()}), audit fails {st[3]:.0f}; P {dP:.2e}, scaling {ds:.2e}, "
+                  f"model {dm:.2e}, sum PlogP {dl:.2e}")
+            if not st[14]:
+                assert st[11] == len(sub) and st[6] == allowed.sum()
+            assert dP < 1e-4 and ds < 1e-4 and dm < 1e-4
+            worst = [max(worst[0], dP), max(worst[1], ds), max(worst[2], dm)]
+            state = g
+            mode_prev = mode
+        print(f"worst over the proposal iterations: P {worst[0]:.2e}, scaling {worst[1]:.2e}, model {worst[2]:.2e}")
+    finally:
+        prob_.free()
+        E.set_option("likelihood", "gaussian")
+    E.check()
