@@ -51,7 +51,8 @@ sys.path.insert(0, ROOT)
 PEAK_FP16 = 2.5e15     # MI355X dense FP16 MFMA, FLOP/s
 PEAK_FP32 = 157.3e12   # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12      # bytes/s
-NPHASE = 14
+NPHASE = 15
+INSERT = ["atomic"]  # insertion mode of every run (--insert)
 
 
 def log(*a):
@@ -112,41 +113,44 @@ def run_dense(E, W, precision, warmup, steps, sigma, likelihood, pre=0):
     E.set_option("likelihood", likelihood)
     E.set_option("sparse", "on" if W.cfg.csr and precision == "fp16" else "off")
     E.set_option("precision", precision)
-    E.set_option("insert", "atomic")
+    E.set_option("insert", INSERT[0])
     E.set_option("proposal", "off")
     E.set_option("timing", "0")
     prob = E.EmcProblem(W.cfg.side, W.images, W.mask, W.q, W.w, W.model0)
     history = []
+    t0 = time.perf_counter()
     for it in range(pre + warmup):
         history.append(prob.iteration(sigma))
         log(f"  {precision} untimed iteration {it + 1} done")
     dt, ph, _, hist, its = timed(E, prob, sigma, steps)
+    total_s = time.perf_counter() - t0
     history += hist
     model = prob.model()
     prob.free()
     L.emc_hip_invalidate()
     log(f"  {precision}: {dt * 1e3:.1f} ms/iteration")
-    return {"dt": dt, "phases": ph, "model": model, "history": history, "iteration_s": its}
+    return {"dt": dt, "phases": ph, "model": model, "history": history, "iteration_s": its, "total_s": total_s}
 
 
 def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr, topk, n_coarse, max_fail,
-                seed=0):
+                seed=0, batch=256, track=True, shared=512, one_step=True):
     L = E.lib()
     E.set_option("likelihood", likelihood)
     E.set_option("sparse", "on" if W.cfg.csr else "off")
     E.set_option("precision", "fp16")
-    E.set_option("insert", "atomic")
+    E.set_option("insert", INSERT[0])
     E.set_option("proposal", "off")
     E.set_option("audit_fraction", "0.02")
     E.set_option("min_mass", "0.999")
     E.set_option("audit_max_fail", str(max_fail))
     E.set_option("proposal_gate", "audit")
-    E.set_option("proposal_track", "on")
+    E.set_option("proposal_track", "on" if track else "off")
+    E.set_option("proposal_shared", str(shared))
     E.set_option("timing", "0")
     N, R, side = W.N, W.R, W.cfg.side
     prob = E.EmcProblem(side, W.images, W.mask, W.q, W.w, W.model0)
     t_start = time.perf_counter()
-    if L.emc_hip_proposal_init(E.fptr(W.q), R, n_coarse, side, topk, 256, seed) != 0:
+    if L.emc_hip_proposal_init(E.fptr(W.q), R, n_coarse, side, topk, batch, seed) != 0:
         E.check()
     dense_s = train_s = 0.0
     losses, history = [], []
@@ -205,10 +209,21 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
     history += hist
     total_s = time.perf_counter() - t_start
     model = prob.model()
+    state_model, state_scaling = model.copy(), prob.scaling().copy()
+    res = {"dt": dt, "phases": ph, "model": model, "scaling": state_scaling, "per_step": per, "iteration_s": its,
+           "dense_s": dense_s, "train_s": train_s, "predict_s": predict_s, "losses": losses, "gated": gated,
+           "total_s": total_s, "predicted_mass_q1_q10_q50": pq, "history": history, "first_audit": first_audit}
+    if not one_step:
+        prob.free()
+        E.set_option("proposal", "off")
+        E.set_option("proposal_track", "on")
+        E.set_option("proposal_shared", "512")
+        L.emc_hip_invalidate()
+        log(f"  learned: {dt * 1e3:.1f} ms/iteration")
+        return res
     # one more iteration from this state, learned and plain FP32, for the
     # accuracy of a single iteration (free of the run-to-run divergence of
     # the iterated map)
-    state_model, state_scaling = model.copy(), prob.scaling().copy()
     prob.iteration(sigma)
     one_st = proposal_stats(L)
     one_learned = prob.model()
@@ -222,12 +237,59 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
     ref.free()
     L.emc_hip_invalidate()
     log(f"  learned: {dt * 1e3:.1f} ms/iteration")
-    return {"dt": dt, "phases": ph, "model": model, "per_step": per, "iteration_s": its, "dense_s": dense_s,
-            "train_s": train_s,
-            "predict_s": predict_s, "losses": losses, "gated": gated, "total_s": total_s,
-            "predicted_mass_q1_q10_q50": pq, "history": history, "first_audit": first_audit,
-            "one_step_cc": cc(one_learned, one_plain),
-            "one_step_whole_table": one_st["whole_table_iteration"]}
+    res.update({"one_step_cc": cc(one_learned, one_plain), "one_step_whole_table": one_st["whole_table_iteration"]})
+    return res
+
+
+def ablation_summary(r, plain_total_s=None):
+    st = r["per_step"]
+    out = {"ms_per_iteration": r["dt"] * 1e3,
+           "whole_table_iterations_timed": int(sum(s["whole_table_iteration"] for s in st)),
+           "audit_below_min_mass_total": float(sum(s["audit_below_min_mass"] for s in st)),
+           "audited_total": float(sum(s["audited"] for s in st)),
+           "full_search_patterns_per_step": float(np.mean([s["full_search"] for s in st])),
+           "pairs_scored_per_step": float(np.mean([s["pairs_scored"] for s in st])),
+           "training_s": r["train_s"], "whole_schedule_s": r["total_s"],
+           "network_topk_mass_q1_q10_q50": r["predicted_mass_q1_q10_q50"],
+           "train_loss_per_dense_iteration": r["losses"]}
+    if plain_total_s:
+        out["whole_schedule_speedup_vs_plain_fp32"] = plain_total_s / r["total_s"]
+    return out
+
+
+def fp64_check(E, O, W, model, scaling, n_check):
+    """One FP16 hi/lo iteration (Poisson, the whole rotation table, dense)
+    of the first `n_check` patterns from the learned run's final model and
+    scaling, against the FP64 oracle of the same equations on the same
+    inputs: the errors of responsibilities (absolute), scaling and model
+    (relative to their maxima)."""
+    L = E.lib()
+    Nc = min(n_check, W.N)
+    for k, v in (("precision", "fp16"), ("likelihood", "poisson"), ("proposal", "off"), ("insert", "atomic"),
+                 ("mstep", "auto"), ("sparse", "off")):
+        E.set_option(k, v)
+    imgs = np.ascontiguousarray(W.images[:Nc])
+    sc = np.ascontiguousarray(scaling[:Nc], np.float32)
+    prob = E.EmcProblem(W.cfg.side, imgs, W.mask, W.q, W.w, model, scaling=sc)
+    prob.iteration(0.4)  # sigma is not used by the Poisson likelihood
+    g = {"P": prob.respons(), "scaling": prob.scaling(), "model": prob.model()}
+    prob.free()
+    L.emc_hip_invalidate()
+    E.check()
+    log(f"fp64 check: FP64 oracle iteration on {Nc} patterns x {W.R} rotations ...")
+    t0 = time.perf_counter()
+    ref = O.o64_iteration_large(model, imgs, W.mask, sc, W.cfg.side, W.q, W.w, W.x, W.y, W.z, kind=O.POISSON)
+    t_oracle = time.perf_counter() - t0
+
+    def rel(a, b):
+        return float(np.abs(np.asarray(a, np.float64) - b).max() / np.abs(b).max())
+
+    return {"patterns": Nc, "rotations": W.R, "masked_pixels": W.n_m, "state": "final model and scaling of the run",
+            "form": "FP16 hi/lo, Poisson, whole table",
+            "responsibilities_max_abs_error": float(np.abs(g["P"] - ref["P"]).max()),
+            "scaling_max_rel_error": rel(g["scaling"], ref["scaling"]),
+            "model_max_rel_error": rel(g["model"], ref["model"]),
+            "model_cc": cc(g["model"], ref["model"]), "oracle_s": t_oracle}
 
 
 def cc(a, b):
@@ -289,12 +351,22 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--topk", type=int, default=32)
     ap.add_argument("--coarse-n", type=int, default=4)
+    ap.add_argument("--train-batch", type=int, default=256, help="patterns per network training step")
+    ap.add_argument("--insert", choices=["atomic", "deterministic"], default="atomic",
+                    help="insertion mode: float atomics, or the deterministic sorted-scatter gather")
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip the two comparison runs (untrained network; network without tracking and shared "
+                         "cells)")
     ap.add_argument("--cpu-patterns", type=int, default=8, help="pattern sample for the CPU baseline")
+    ap.add_argument("--cpu-patterns-1t", type=int, default=1, help="pattern sample of the single-thread CPU baseline")
+    ap.add_argument("--check-patterns", type=int, default=256,
+                    help="pattern sample of the one-iteration check against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device three times (HBM counter calibration) and exit")
     args = ap.parse_args()
+    INSERT[0] = args.insert
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -327,10 +399,24 @@ def main():
     pre = args.train_iters if learned else 0
     sigma = args.sigma if like == "gaussian" else None
     log(f"likelihood {like}; {pre} training + {args.warmup} untimed + {args.steps} timed iterations")
+    epochs = [int(v) for v in args.epochs.split(",")]
+    ablation = {}
     if learned:
-        fast = run_learned(E, W, args.warmup, args.steps, args.sigma, like, args.train_iters,
-                           [int(v) for v in args.epochs.split(",")], args.lr, args.topk, args.coarse_n,
-                           args.max_fail)
+        fast = run_learned(E, W, args.warmup, args.steps, args.sigma, like, args.train_iters, epochs, args.lr,
+                           args.topk, args.coarse_n, args.max_fail, batch=args.train_batch)
+        if not args.no_ablation:
+            # what the trained network adds: the same schedule with the
+            # network left untrained, and the trained network without
+            # posterior tracking and shared cells
+            log("ablation: untrained network")
+            ablation["untrained_network"] = run_learned(E, W, args.warmup, args.steps, args.sigma, like,
+                                                        args.train_iters, [0], args.lr, args.topk, args.coarse_n,
+                                                        args.max_fail, batch=args.train_batch, one_step=False)
+            log("ablation: network only (no tracking, no shared cells)")
+            ablation["network_only"] = run_learned(E, W, args.warmup, args.steps, args.sigma, like,
+                                                   args.train_iters, epochs, args.lr, args.topk, args.coarse_n,
+                                                   args.max_fail, batch=args.train_batch, track=False, shared=0,
+                                                   one_step=False)
     else:
         fast = run_dense(E, W, "fp16", args.warmup, args.steps, args.sigma, like)
     plain = None
@@ -392,7 +478,8 @@ def main():
         "data": "synthetic (Gaussian-blob object, Poisson patterns), random initial model, inputs resident in HBM",
         "config": {"workload": W.cfg.name, "side": W.cfg.side, "patterns": N, "rotations": R,
                    "masked_pixels": n_m, "sampling_n": W.cfg.n_rot, "likelihood": like, "sigma": sigma,
-                   "iterations": pre + args.warmup + args.steps, "parallelism": "single GPU"},
+                   "iterations": pre + args.warmup + args.steps, "insert": args.insert,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "kernel": kernel, "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP16,
                      "traffic": read_traffic(kname, W.cfg.name), "launch_ms": like_avg * 1e3,
@@ -433,6 +520,10 @@ def main():
                                            ("audited", "audit_below_min_mass", "audit_min_mass", "audit_mean_mass")}
             if fast["first_audit"] else None,
             "model_cc_one_iteration_vs_plain_fp32": fast["one_step_cc"],
+            "whole_schedule_speedup_vs_plain_fp32": (plain["total_s"] / fast["total_s"]) if plain else None,
+            "ablation": {"full": ablation_summary(fast, plain["total_s"] if plain else None),
+                         **{k: ablation_summary(v, plain["total_s"] if plain else None)
+                            for k, v in ablation.items()}} if ablation else None,
             "one_iteration_ran_on_whole_table": bool(fast["one_step_whole_table"]),
             "sum_plogp_per_iteration": [h["likelihood"] for h in fast["history"]],
         }
@@ -449,6 +540,10 @@ def main():
         Ns = min(args.cpu_patterns, N)
         log(f"cpu baseline: FP32 restatement on {Ns} patterns x {R} rotations ...")
         tr = O.time_reference_iteration(W.model0, W.images[:Ns], W.mask, W.cfg.side, W.x, W.y, W.z, W.q, W.w, N)
+        N1 = min(args.cpu_patterns_1t, N)
+        log(f"cpu baseline: the same on one thread, {N1} pattern(s) ...")
+        t1 = O.time_reference_iteration(W.model0, W.images[:N1], W.mask, W.cfg.side, W.x, W.y, W.z, W.q, W.w, N,
+                                        nthreads=1)
         out["cpu_baseline"] = {
             "value": pairs / tr["full_s"], "unit": "pairs/s", "cores": tr["threads"], "kind": "port",
             "sample": (f"one iteration of the oracle's FP32 restatement of the reference's CPU path "
@@ -457,7 +552,12 @@ def main():
                        f"per-pattern phases scaled by {N}/{Ns}, expand/insert/normalise counted once: "
                        f"{tr['full_s']:.1f} s per full iteration"),
             "phases_s": tr["phases"],
+            "single_thread": {"value": pairs / t1["full_s"], "unit": "pairs/s", "cores": 1,
+                              "sample": f"{N1} of {N} patterns x all {R} rotations ({t1['sample_s']:.1f} s), "
+                                        f"scaled the same way: {t1['full_s']:.1f} s per full iteration"},
         }
+        if learned and like == "poisson" and not W.cfg.csr:
+            out["fp64_check"] = fp64_check(E, O, W, fast["model"], fast["scaling"], args.check_patterns)
     print(json.dumps(out), flush=True)
 
 

@@ -40,6 +40,12 @@ class Sp3Matrix(C.Structure):
     _fields_ = [("x", C.c_uint), ("y", C.c_uint), ("z", C.c_uint), ("data", c_float_p)]
 
 
+class H5Dataset(C.Structure):
+    """emc_h5_dataset (emc_io.h)."""
+    _fields_ = [("name", C.c_char_p), ("type", C.c_int), ("rank", C.c_int), ("dims", C.c_longlong * 3),
+                ("data", C.c_void_p)]
+
+
 class Quaternion(C.Structure):
     _fields_ = [("q", C.c_float * 4)]
 
@@ -140,6 +146,16 @@ _SIGS = [
     ("emc_synth_random_model", None, [C.c_int, C.c_uint64, c_float_p]),
     ("emc_quaternion_from_uniforms", None, [C.c_double, C.c_double, C.c_double, c_float_p]),
     ("emc_uniform", C.c_double, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
+    # emc_io.h
+    ("emc_h5_write", C.c_int, [C.c_char_p, C.c_void_p, C.c_int]),
+    ("emc_h5_read", C.c_longlong, [C.c_char_p, C.c_char_p, C.c_int, C.c_void_p, C.c_longlong, c_int_p,
+                                   C.POINTER(C.c_longlong)]),
+    ("emc_io_error", C.c_char_p, []),
+    ("emc_write_spimage", C.c_int, [C.c_char_p, c_float_p, c_int_p, C.c_int, C.c_int, C.c_int]),
+    ("emc_read_spimage", C.c_int, [C.c_char_p, c_float_p, c_int_p, C.c_int, C.c_int, C.c_int]),
+    ("emc_spimage_shape", C.c_int, [C.c_char_p, c_int_p, c_int_p, c_int_p]),
+    ("emc_read_patterns", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, c_float_p]),
+    ("emc_read_mask", C.c_int, [C.c_char_p, C.c_int, C.c_int, c_int_p]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -12,6 +12,8 @@ namespace emc {
 
 static constexpr size_t kRedDoubles = 1 << 16;  // partial sums of the bias gradients
 
+static long floats(const DBuf& b) { return (long)(b.bytes / sizeof(float)); }
+
 bool Cnn::init(int side_, int ncell_, int batch_, uint64_t seed) {
   release();
   if (side_ < 16 || side_ % 16 || ncell_ <= 0 || batch_ <= 0) {
@@ -98,9 +100,9 @@ void Cnn::forward(const float* images, const int* mask, int N2d, const int* d_id
     const int H = hin[l], oh = H / 2, K = 9 * ic[l];
     im2col(in, B, H, H, ic[l], col[l].as<float>(), st);
     GemmDesc d;
-    d.A = col[l].as<float>(); d.sam = K; d.sak = 1;
-    d.B = W(l); d.sbn = K; d.sbk = 1;
-    d.C = act[l].as<float>(); d.scm = oc[l]; d.scn = 1;
+    d.A = col[l].as<float>(); d.sam = K; d.sak = 1; d.a_len = floats(col[l]);
+    d.B = W(l); d.sbn = K; d.sbk = 1; d.b_len = off_b[l] - off_w[l];
+    d.C = act[l].as<float>(); d.scm = oc[l]; d.scn = 1; d.c_len = floats(act[l]);
     d.bias_n = Bv(l);
     d.M = B * oh * oh; d.N = oc[l]; d.K = K;
     d.relu = 1;
@@ -108,16 +110,16 @@ void Cnn::forward(const float* images, const int* mask, int N2d, const int* d_id
     in = act[l].as<float>();
   }
   GemmDesc d1;  // dense 1: h[b][j] = relu(sum_k a3[b][k] W4[j][k] + b4[j])
-  d1.A = act[kConvs - 1].as<float>(); d1.sam = feat; d1.sak = 1;
-  d1.B = W(4); d1.sbn = feat; d1.sbk = 1;
-  d1.C = h.as<float>(); d1.scm = hidden; d1.scn = 1;
+  d1.A = act[kConvs - 1].as<float>(); d1.sam = feat; d1.sak = 1; d1.a_len = floats(act[kConvs - 1]);
+  d1.B = W(4); d1.sbn = feat; d1.sbk = 1; d1.b_len = off_b[4] - off_w[4];
+  d1.C = h.as<float>(); d1.scm = hidden; d1.scn = 1; d1.c_len = floats(h);
   d1.bias_n = Bv(4);
   d1.M = B; d1.N = hidden; d1.K = feat; d1.relu = 1;
   gemm_bf16(d1, sc, scratch_floats, st);
   GemmDesc d2;  // dense 2, stored [C][B]: logits[c][b] = sum_k W5[c][k] h[b][k] + b5[c]
-  d2.A = W(5); d2.sam = hidden; d2.sak = 1;
-  d2.B = h.as<float>(); d2.sbn = hidden; d2.sbk = 1;
-  d2.C = logits.as<float>(); d2.scm = B; d2.scn = 1;
+  d2.A = W(5); d2.sam = hidden; d2.sak = 1; d2.a_len = off_b[5] - off_w[5];
+  d2.B = h.as<float>(); d2.sbn = hidden; d2.sbk = 1; d2.b_len = floats(h);
+  d2.C = logits.as<float>(); d2.scm = B; d2.scn = 1; d2.c_len = floats(logits);
   d2.bias_m = Bv(5);
   d2.M = ncell; d2.N = B; d2.K = hidden;
   gemm_bf16(d2, sc, scratch_floats, st);
@@ -128,18 +130,18 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
   const float* dl = dlogits.as<float>();
   {  // dW5[c][k] = sum_b dlog[c][b] h[b][k]
     GemmDesc d;
-    d.A = dl; d.sam = B; d.sak = 1;
-    d.B = h.as<float>(); d.sbn = 1; d.sbk = hidden;
-    d.C = dW(5); d.scm = hidden; d.scn = 1;
+    d.A = dl; d.sam = B; d.sak = 1; d.a_len = floats(dlogits);
+    d.B = h.as<float>(); d.sbn = 1; d.sbk = hidden; d.b_len = floats(h);
+    d.C = dW(5); d.scm = hidden; d.scn = 1; d.c_len = off_b[5] - off_w[5];
     d.M = ncell; d.N = hidden; d.K = B;
     gemm_bf16(d, sc, scratch_floats, st);
     reduce_rows(dl, B, ncell, 1, B, dB(5), red.as<double>(), kRedDoubles, st);
   }
   {  // dh[b][k] = sum_c dlog[c][b] W5[c][k], then ReLU'
     GemmDesc d;
-    d.A = dl; d.sam = 1; d.sak = B;
-    d.B = W(5); d.sbn = 1; d.sbk = hidden;
-    d.C = dh.as<float>(); d.scm = hidden; d.scn = 1;
+    d.A = dl; d.sam = 1; d.sak = B; d.a_len = floats(dlogits);
+    d.B = W(5); d.sbn = 1; d.sbk = hidden; d.b_len = off_b[5] - off_w[5];
+    d.C = dh.as<float>(); d.scm = hidden; d.scn = 1; d.c_len = floats(dh);
     d.M = B; d.N = hidden; d.K = ncell;
     gemm_bf16(d, sc, scratch_floats, st);
     relu_bwd(dh.as<float>(), h.as<float>(), (long)B * hidden, st);
@@ -150,16 +152,16 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
   int cur = 0;
   {  // dW4[j][k] = sum_b dh[b][j] a3[b][k]; da3[b][k] = sum_j dh[b][j] W4[j][k]
     GemmDesc d;
-    d.A = dh.as<float>(); d.sam = 1; d.sak = hidden;
-    d.B = act[kConvs - 1].as<float>(); d.sbn = 1; d.sbk = feat;
-    d.C = dW(4); d.scm = feat; d.scn = 1;
+    d.A = dh.as<float>(); d.sam = 1; d.sak = hidden; d.a_len = floats(dh);
+    d.B = act[kConvs - 1].as<float>(); d.sbn = 1; d.sbk = feat; d.b_len = floats(act[kConvs - 1]);
+    d.C = dW(4); d.scm = feat; d.scn = 1; d.c_len = off_b[4] - off_w[4];
     d.M = hidden; d.N = feat; d.K = B;
     gemm_bf16(d, sc, scratch_floats, st);
     reduce_rows(dh.as<float>(), B, hidden, hidden, 1, dB(4), red.as<double>(), kRedDoubles, st);
     GemmDesc e;
-    e.A = dh.as<float>(); e.sam = hidden; e.sak = 1;
-    e.B = W(4); e.sbn = 1; e.sbk = feat;
-    e.C = dbufs[cur]; e.scm = feat; e.scn = 1;
+    e.A = dh.as<float>(); e.sam = hidden; e.sak = 1; e.a_len = floats(dh);
+    e.B = W(4); e.sbn = 1; e.sbk = feat; e.b_len = off_b[4] - off_w[4];
+    e.C = dbufs[cur]; e.scm = feat; e.scn = 1; e.c_len = half;
     e.M = B; e.N = feat; e.K = hidden;
     gemm_bf16(e, sc, scratch_floats, st);
   }
@@ -169,17 +171,17 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     float* dz = dbufs[cur];
     relu_bwd(dz, act[l].as<float>(), M * OC, st);
     GemmDesc d;  // dW[oc][kk] = sum_m dz[m][oc] col[m][kk]
-    d.A = dz; d.sam = 1; d.sak = OC;
-    d.B = col[l].as<float>(); d.sbn = 1; d.sbk = K;
-    d.C = dW(l); d.scm = K; d.scn = 1;
+    d.A = dz; d.sam = 1; d.sak = OC; d.a_len = half;
+    d.B = col[l].as<float>(); d.sbn = 1; d.sbk = K; d.b_len = floats(col[l]);
+    d.C = dW(l); d.scm = K; d.scn = 1; d.c_len = off_b[l] - off_w[l];
     d.M = OC; d.N = K; d.K = (int)M;
     gemm_bf16(d, sc, scratch_floats, st);
     reduce_rows(dz, M, OC, OC, 1, dB(l), red.as<double>(), kRedDoubles, st);
     if (l > 0) {
       GemmDesc e;  // dcol[m][kk] = sum_oc dz[m][oc] W[oc][kk]
-      e.A = dz; e.sam = OC; e.sak = 1;
-      e.B = W(l); e.sbn = 1; e.sbk = K;
-      e.C = dcol.as<float>(); e.scm = K; e.scn = 1;
+      e.A = dz; e.sam = OC; e.sak = 1; e.a_len = half;
+      e.B = W(l); e.sbn = 1; e.sbk = K; e.b_len = off_b[l] - off_w[l];
+      e.C = dcol.as<float>(); e.scm = K; e.scn = 1; e.c_len = floats(dcol);
       e.M = (int)M; e.N = K; e.K = OC;
       gemm_bf16(e, sc, scratch_floats, st);
       col2im(dcol.as<float>(), B, H, H, ic[l], dbufs[cur ^ 1], st);

@@ -13,7 +13,9 @@ namespace emc {
 
 // C(m, n) = sum_k A(m, k) B(n, k) [+ bias_n[n] + bias_m[m]] [ReLU] [+= C]
 // with A(m, k) = A[m*sam + k*sak], B(n, k) = B[n*sbn + k*sbk],
-// C(m, n) = C[m*scm + n*scn].
+// C(m, n) = C[m*scm + n*scn].  a_len, b_len, c_len: floats available at A,
+// B and C (and bias_n >= N, bias_m >= M floats); gemm_bf16 checks every
+// descriptor against them on the host and launches nothing out of bounds.
 struct GemmDesc {
   const float* A = nullptr;
   long sam = 0, sak = 0;
@@ -26,9 +28,12 @@ struct GemmDesc {
   int M = 0, N = 0, K = 0;
   int relu = 0;
   int accumulate = 0;
+  long a_len = 0, b_len = 0, c_len = 0;
 };
 
-void gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st);
+// False (error set, nothing launched) when the descriptor reaches outside
+// a_len / b_len / c_len or has negative strides.
+bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st);
 void cnn_input(const float* images, const int* mask, int N2d, const int* idx, int B, float* x, hipStream_t st);
 void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t st);
 void col2im(const float* col, int B, int H, int W, int C, float* in, hipStream_t st);
@@ -37,11 +42,13 @@ void relu_bwd(float* g, const float* a, long n, hipStream_t st);
 // partial sums (cols * splits doubles)
 void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, double* scratch,
                  size_t scratch_doubles, hipStream_t st);
+// targets pattern-major [Nt][C]; pattern b of the batch takes row tidx[b]
 void softmax_xent(const float* logits, int C, int B, const float* targets, long Nt, const int* tidx, float scale,
                   float* grad, double* loss, hipStream_t st);
 void adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps, int step,
           hipStream_t st);
 void topk(const float* logits, int C, int B, int k, const int* out_rows, int* out, float* mass, hipStream_t st);
+// T [N][C] (pattern-major) = per-cell sums of P [R][N]
 void coarse_targets(const float* P, int N, const int* cell_off, const int* cell_rot, int C, float* T, hipStream_t st);
 
 // The network.  Parameters in one FP32 buffer:

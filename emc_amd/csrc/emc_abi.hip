@@ -88,8 +88,11 @@ void Engine::release_all() {
   DBuf* bufs[] = {&d_pix, &d_B, &d_K32, &d_Kt32, &d_K16, &d_K16lo, &d_Kt16, &d_Kt16lo, &d_flags,
                   &d_S32, &d_S16, &d_S16lo, &d_Sscale, &d_A, &d_phi_used, &d_smax, &d_ssum, &d_sacc,
                   &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Tphi, &d_ones, &d_Pscale, &d_P16, &d_P16lo,
-                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_tiled, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt};
+                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_tiled, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt,
+                  &d_cnt_off, &d_cnt_px, &d_cnt_val, &d_ref_ent, &d_ref_count, &d_ref_pat, &d_ref_tot,
+                  &d_ref_slot, &d_ref_tmp};
   for (DBuf* b : bufs) b->release();
+  cnt_images = nullptr;
   pack_key = {};
   map_key = {};
   like = {};
@@ -501,6 +504,30 @@ void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images
   EMC_CHECK(hipMemcpyAsync(&total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, e.stream));
   sync();
   if (failed()) return;
+  // the significant responsibilities from FP64 likelihoods, when the
+  // likelihood this buffer holds is the library's own of these slices
+  if (e.refine && fuse && lk.slices && lk.images) {
+    PhaseTimer pt(PH_REFINE);
+    RefineInputs in;
+    in.respons = d_respons;
+    in.N = N_images;
+    in.R = N_slices;
+    in.N2d = lk.N2d;
+    in.slices = lk.slices;
+    in.images = lk.images;
+    in.rows[0].P = d_respons;
+    in.rows[0].rows = N_slices;
+    in.rows[0].ld = N_images;
+    in.rows[0].cols = N_images;
+    in.nrows = 1;
+    in.phiden = e.d_phiden.as<double>();
+    in.poisson = e.likelihood == LIKE_POISSON;
+    in.sigma2x = lk.sigma2x;
+    in.tau = e.refine_tau;
+    double dt = 0.0;
+    if (!refine_posterior(in, &dt)) return;
+    total += dt;
+  }
   if (lk.respons == d_respons && lk.N == N_images && lk.R == N_slices) {
     lk.normalized = fuse;
     lk.have_total = true;
@@ -1009,6 +1036,15 @@ int emc_hip_set_option(const char* key, const char* value) {
     const long v = strtol(value, &end, 10);
     if (!end || *end != 0 || v < 1 || v > 64) return 1;
     e.expand_rotations = (int)v;
+  } else if (is(key, "refine")) {
+    if (is(value, "on")) e.refine = true;
+    else if (is(value, "off")) e.refine = false;
+    else return 1;
+  } else if (is(key, "refine_tau")) {
+    char* end = nullptr;
+    const double v = strtod(value, &end);
+    if (!end || *end != 0 || !(v > 0.0) || v > 1.0) return 1;
+    e.refine_tau = (float)v;
   } else if (is(key, "gemm_tile")) {
     if (is(value, "256")) e.gemm_small = false;
     else if (is(value, "128")) e.gemm_small = true;
@@ -1079,7 +1115,7 @@ void emc_hip_invalidate(void) {
 
 static const char* kPhaseNames[PH_COUNT] = {"expand", "pack", "likelihood", "softmax", "scaling",
                                             "mstep_prep", "mstep", "insert", "normalize", "other", "audit",
-                                            "shared", "groups", "audit_gemm"};
+                                            "shared", "groups", "audit_gemm", "refine"};
 
 int emc_hip_phase_times(double* ms, long long* launches, int n) {
   Engine& e = engine();

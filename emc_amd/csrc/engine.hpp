@@ -38,7 +38,7 @@ struct DBuf {
 // ---- phases for timing -----------------------------------------------------
 enum Phase {
   PH_EXPAND = 0, PH_PACK, PH_LIKE, PH_SOFTMAX, PH_SCALING, PH_ROWSUM, PH_MSTEP,
-  PH_INSERT, PH_NORMALIZE, PH_OTHER, PH_AUDIT, PH_SHARED, PH_GROUPS, PH_AUDIT_GEMM, PH_COUNT
+  PH_INSERT, PH_NORMALIZE, PH_OTHER, PH_AUDIT, PH_SHARED, PH_GROUPS, PH_AUDIT_GEMM, PH_REFINE, PH_COUNT
 };
 
 enum Precision { PREC_FP32 = 0, PREC_FP16 = 1 };
@@ -82,6 +82,8 @@ struct Engine {
   bool gemm_small = false;      // FP16 likelihood GEMM: 128 x 128 tiles, two workgroups per compute unit
   bool expand_tiled = false;    // fused expand + pack in rotation blocks x pixel tiles even where the slice fits LDS
   int expand_rotations = 8;     // rotations per workgroup of that form
+  bool refine = true;           // responsibilities >= refine_tau recomputed from FP64 likelihoods (k_refine.hip)
+  float refine_tau = 1e-3f;
   bool timing = false;
   bool verbose = false;
 
@@ -121,6 +123,15 @@ struct Engine {
   // index and count of every nonzero
   DBuf d_csr_off, d_csr_col, d_csr_cnt;
   size_t csr_nnz = 0;
+  // every pattern's nonzero masked counts for the refinement: offsets
+  // [N+1], pixel index, count (float); built for cnt_images at cnt_pix_gen
+  DBuf d_cnt_off, d_cnt_px, d_cnt_val;
+  const float* cnt_images = nullptr;
+  int cnt_N = 0;
+  uint64_t cnt_pix_gen = 0, cnt_stamp = 0;
+  // refinement scratch: entries, their count, per-pattern sums, sum P log P change
+  DBuf d_ref_ent, d_ref_count, d_ref_pat, d_ref_tot, d_ref_slot, d_ref_tmp;
+  double ref_entries = 0;  // entries the last refinement recomputed
 
   // ---- packed slices (per likelihood call) ----
   int R_pad = 0;
@@ -345,6 +356,41 @@ void launch_insert_gather(const float* Sc, int n_mp, const int* ins_col, int n_i
                           const float* ent_wt, int S, int interp, float* model, float* weight,
                           hipStream_t st);
 void launch_set_wT(const float* wts, const float* Tf, int R, float* wT, hipStream_t st);
+// Refinement of the significant responsibilities (k_refine.hip).  Sources:
+// [rows][ld] blocks of responsibilities (row -> rotation row_rot[row] or
+// row, column -> pattern col_pat[col] or col, patterns with skip set left
+// out; own: the refined value is also written back there) and the
+// candidate slots Lc [N][topk][slots] of the patterns outside insub.
+struct RefineRows {
+  float* P = nullptr;
+  int rows = 0, ld = 0, cols = 0;
+  const int* row_rot = nullptr;
+  const int* col_pat = nullptr;
+  const unsigned char* skip = nullptr;
+  bool own = false;
+};
+struct RefineInputs {
+  float* respons = nullptr;  // [R][N] responsibility matrix (every refined entry written)
+  int N = 0, R = 0, N2d = 0;
+  const float* slices = nullptr;  // [R][N2d] FP32 slices the likelihood used
+  const float* images = nullptr;  // [N][N2d] counts
+  RefineRows rows[3];
+  int nrows = 0;
+  float* Lc = nullptr;
+  const int *cand = nullptr, *cell_off = nullptr, *cell_rot = nullptr;
+  const unsigned char* insub = nullptr;
+  int topk = 0, slots = 0;
+  double* phiden = nullptr;  // [N] scaling denominators to correct, or nullptr
+  bool poisson = false;
+  double sigma2x = 0.0;      // 2 n_m sigma^2 (Gaussian)
+  float tau = 1e-3f;
+};
+// Recompute the entries >= in.tau; *dtotal receives the change of sum P log P.
+bool refine_posterior(const RefineInputs& in, double* dtotal);
+
+// red: kNormMaxBlocks partial sums, then the factor (kNormMaxBlocks + 2
+// doubles); the partial-sum grid never exceeds kNormMaxBlocks blocks.
+constexpr int kNormMaxBlocks = 1024;
 void launch_normalize_model(float* model, const float* weight, int n, double* red,
                             hipStream_t st);
 void launch_model_max(const float* model, int n, float* out, hipStream_t st);

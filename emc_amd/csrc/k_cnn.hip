@@ -22,57 +22,123 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 // Generic strided GEMM: C(m, n) = sum_k A(m, k) B(n, k), operands FP32 in
 // memory (any strides), rounded to BF16 on the way to LDS.  64 x 64 tiles,
-// 4 waves of 32 x 32, BK = 32.  Split-K over gridDim.z writes partial tiles
-// to `partial` ([z][M][N]); gemm_finish adds them in a fixed order.
+// 4 waves of 32 x 32, BK = 32, two LDS stages: the next k-step's operand
+// values are loaded into registers while the current stage's products run.
+// Each thread moves 8 values of each operand per k-step: 8 consecutive k of
+// one row (load mode kKFast: 16-byte loads when the row's k run is
+// contiguous and aligned), 8 consecutive rows at one k (kRowFast: the same
+// along rows), or 8 strided scalars (kStrided).  Split-K over gridDim.z
+// writes partial tiles to `partial` ([z][M][N]); gemm_finish adds them in a
+// fixed order.
 // ---------------------------------------------------------------------------
 constexpr int GT = 64, GK = 32, GLD = GK + 8;
+enum { kStrided = 0, kKFast = 1, kRowFast = 2 };
 
+struct OpView {
+  const float* p;
+  long sr, sk;  // element (row, k) at p[row * sr + k * sk]
+  int rows;
+};
+
+// the 8 values thread t moves for rows r0.. of the tile and k0.. of the
+// k-step (bounds: rows < v.rows, k < k_end), zero outside
+template <int MODE>
+__device__ __forceinline__ void load8(const OpView& v, int row0, int k0, int k_end, int t, float (&x)[8]) {
+  if (MODE == kRowFast) {
+    const int k = k0 + (t >> 3), r = row0 + (t & 7) * 8;
+    if (k < k_end && r + 8 <= v.rows) {
+      const float4* q = reinterpret_cast<const float4*>(v.p + (long)r + (long)k * v.sk);
+      const float4 a = q[0], b = q[1];
+      x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) x[e] = (k < k_end && r + e < v.rows) ? v.p[(long)(r + e) + (long)k * v.sk] : 0.0f;
+    }
+  } else if (MODE == kKFast) {
+    const int r = row0 + (t >> 2), k = k0 + (t & 3) * 8;
+    if (r < v.rows && k + 8 <= k_end) {
+      const float4* q = reinterpret_cast<const float4*>(v.p + (long)r * v.sr + (long)k);
+      const float4 a = q[0], b = q[1];
+      x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) x[e] = (r < v.rows && k + e < k_end) ? v.p[(long)r * v.sr + (long)(k + e)] : 0.0f;
+    }
+  } else {
+    const bool kf = v.sk == 1;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      int r, k;
+      if (kf) { r = t >> 2; k = (t & 3) * 8 + e; }
+      else { k = t >> 3; r = (t & 7) * 8 + e; }
+      r += row0;
+      k += k0;
+      x[e] = (r < v.rows && k < k_end) ? v.p[(long)r * v.sr + (long)k * v.sk] : 0.0f;
+    }
+  }
+}
+
+// the same 8 values into the LDS tile [row][k] as BF16
+template <int MODE>
+__device__ __forceinline__ void store8(__bf16 (*s)[GLD], int t, bool kf_strided, const float (&x)[8]) {
+  if (MODE == kKFast || (MODE == kStrided && kf_strided)) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = (__bf16)x[e];
+    *reinterpret_cast<bf16x8*>(&s[t >> 2][(t & 3) * 8]) = v;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; e++) s[(t & 7) * 8 + e][t >> 3] = (__bf16)x[e];
+  }
+}
+
+template <int AM, int BM>
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmDesc d, float* __restrict__ partial, int ksplit_len) {
-  __shared__ __attribute__((aligned(16))) __bf16 sA[GT][GLD];
-  __shared__ __attribute__((aligned(16))) __bf16 sB[GT][GLD];
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][GT][GLD];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][GT][GLD];
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
   const int kz0 = blockIdx.z * ksplit_len, kz1 = min(d.K, kz0 + ksplit_len);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
+  const OpView va{d.A, d.sam, d.sak, d.M}, vb{d.B, d.sbn, d.sbk, d.N};
+  const bool a_kf = d.sak == 1, b_kf = d.sbk == 1;
   floatx4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) acc[a][b] = floatx4{0, 0, 0, 0};
-  // element (row, k) of each thread's 8 loads: k-contiguous operands are read
-  // along k, the others along the row
-  const bool a_kfast = d.sak == 1, b_kfast = d.sbk == 1;
+  float xa[8], xb[8];
+  int cur = 0;
+  if (kz0 < kz1) {
+    load8<AM>(va, m0, kz0, kz1, t, xa);
+    load8<BM>(vb, n0, kz0, kz1, t, xb);
+    store8<AM>(sA[0], t, a_kf, xa);
+    store8<BM>(sB[0], t, b_kf, xb);
+  }
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
   for (int k0 = kz0; k0 < kz1; k0 += GK) {
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-      int r, k;
-      if (a_kfast) { r = t >> 2; k = (t & 3) * 8 + e; }
-      else { k = t >> 3; r = (t & 7) * 8 + e; }
-      const int gm = m0 + r, gk = k0 + k;
-      float v = 0.0f;
-      if (gm < d.M && gk < kz1) v = d.A[(long)gm * d.sam + (long)gk * d.sak];
-      sA[r][k] = (__bf16)v;
-      int rn, kn;
-      if (b_kfast) { rn = t >> 2; kn = (t & 3) * 8 + e; }
-      else { kn = t >> 3; rn = (t & 7) * 8 + e; }
-      const int gn = n0 + rn, gk2 = k0 + kn;
-      float u = 0.0f;
-      if (gn < d.N && gk2 < kz1) u = d.B[(long)gn * d.sbn + (long)gk2 * d.sbk];
-      sB[rn][kn] = (__bf16)u;
+    const bool more = k0 + GK < kz1;
+    if (more) {
+      load8<AM>(va, m0, k0 + GK, kz1, t, xa);
+      load8<BM>(vb, n0, k0 + GK, kz1, t, xb);
     }
-    __syncthreads();
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
     bf16x8 fa[2], fb[2];
 #pragma unroll
-    for (int a = 0; a < 2; a++) fa[a] = *reinterpret_cast<const bf16x8*>(&sA[wm * 32 + a * 16 + fr][fk]);
+    for (int a = 0; a < 2; a++) fa[a] = *reinterpret_cast<const bf16x8*>(&sA[cur][wm * 32 + a * 16 + fr][fk]);
 #pragma unroll
-    for (int b = 0; b < 2; b++) fb[b] = *reinterpret_cast<const bf16x8*>(&sB[wn * 32 + b * 16 + fr][fk]);
+    for (int b = 0; b < 2; b++) fb[b] = *reinterpret_cast<const bf16x8*>(&sB[cur][wn * 32 + b * 16 + fr][fk]);
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
       for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    if (more) {
+      store8<AM>(sA[cur ^ 1], t, a_kf, xa);
+      store8<BM>(sB[cur ^ 1], t, b_kf, xb);
+    }
     __syncthreads();
+    cur ^= 1;
   }
-  const int g = lane >> 4, fr = lane & 15;
+  const int g = lane >> 4;
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -107,8 +173,32 @@ __global__ void k_gemm_finish(GemmDesc d, const float* __restrict__ partial, int
   *c = d.accumulate ? *c + v : v;
 }
 
-void gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st) {
-  if (d.M <= 0 || d.N <= 0) return;
+// Load mode of an operand view: 16-byte loads along k (k contiguous, rows
+// 16-byte aligned) or along rows (rows contiguous, every k 16-byte aligned),
+// else strided scalars.
+static int load_mode(const float* p, long sr, long sk) {
+  const bool al = ((uintptr_t)p & 15) == 0;
+  if (al && sk == 1 && sr % 4 == 0) return kKFast;
+  if (al && sr == 1 && sk % 4 == 0) return kRowFast;
+  return kStrided;
+}
+
+// Highest element index a strided 2-D view (rows x cols, strides sr, sc)
+// touches, or -1 for an empty view.
+static long last_index(long rows, long cols, long sr, long sc) {
+  return rows <= 0 || cols <= 0 ? -1 : (rows - 1) * sr + (cols - 1) * sc;
+}
+
+bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st) {
+  if (d.M <= 0 || d.N <= 0) return true;
+  if (d.K < 0 || d.sam < 0 || d.sak < 0 || d.sbn < 0 || d.sbk < 0 || d.scm < 0 || d.scn < 0 ||
+      last_index(d.M, d.K, d.sam, d.sak) >= d.a_len || last_index(d.N, d.K, d.sbn, d.sbk) >= d.b_len ||
+      last_index(d.M, d.N, d.scm, d.scn) >= d.c_len || !d.A || !d.B || !d.C) {
+    set_error("gemm_bf16: descriptor out of bounds (M %d N %d K %d; A %ld/%ld, B %ld/%ld, C %ld/%ld)", d.M, d.N, d.K,
+              last_index(d.M, d.K, d.sam, d.sak), d.a_len, last_index(d.N, d.K, d.sbn, d.sbk), d.b_len,
+              last_index(d.M, d.N, d.scm, d.scn), d.c_len);
+    return false;
+  }
   const int gm = (d.M + GT - 1) / GT, gn = (d.N + GT - 1) / GT;
   // split K so that a long reduction still spreads over the chip
   int nz = 1;
@@ -116,15 +206,28 @@ void gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStre
   while (tiles * nz < 512 && d.K / (nz * 2) >= 4 * GK && (size_t)(nz * 2) * d.M * d.N <= scratch_floats) nz *= 2;
   const int klen = ((d.K + nz - 1) / nz + GK - 1) / GK * GK;
   nz = (d.K + klen - 1) / klen;
-  if (nz <= 1) {
-    hipLaunchKernelGGL(k_gemm_bf16, dim3(gn, gm, 1), dim3(256), 0, st, d, nullptr, d.K);
-  } else {
-    hipLaunchKernelGGL(k_gemm_bf16, dim3(gn, gm, nz), dim3(256), 0, st, d, scratch, klen);
+  if (nz > 1 && (!scratch || (size_t)nz * d.M * d.N > scratch_floats)) {
+    set_error("gemm_bf16: split-k partials (%d x %d x %d) exceed the scratch (%zu floats)", nz, d.M, d.N,
+              scratch_floats);
+    return false;
+  }
+  const int am = load_mode(d.A, d.sam, d.sak), bm = load_mode(d.B, d.sbn, d.sbk);
+  const dim3 grid(gn, gm, nz > 1 ? nz : 1);
+  float* part = nz > 1 ? scratch : nullptr;
+  const int kl = nz > 1 ? klen : d.K;
+#define EMC_GEMM_CASE(A_, B_)                                                                  \
+  if (am == A_ && bm == B_) hipLaunchKernelGGL((k_gemm_bf16<A_, B_>), grid, dim3(256), 0, st, d, part, kl);
+  EMC_GEMM_CASE(kStrided, kStrided) EMC_GEMM_CASE(kStrided, kKFast) EMC_GEMM_CASE(kStrided, kRowFast)
+  EMC_GEMM_CASE(kKFast, kStrided) EMC_GEMM_CASE(kKFast, kKFast) EMC_GEMM_CASE(kKFast, kRowFast)
+  EMC_GEMM_CASE(kRowFast, kStrided) EMC_GEMM_CASE(kRowFast, kKFast) EMC_GEMM_CASE(kRowFast, kRowFast)
+#undef EMC_GEMM_CASE
+  if (nz > 1) {
     EMC_LAUNCHED();
     const long n = (long)d.M * d.N;
     hipLaunchKernelGGL(k_gemm_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, scratch, nz);
   }
   EMC_LAUNCHED();
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -208,8 +311,9 @@ __global__ void k_reduce_rows_fin(const double* __restrict__ part, int nsplit, i
 }
 
 // Softmax cross-entropy over logits stored [C][B], one wave per pattern
-// (lanes over classes).  targets: [C][Nt] at columns tidx[b];
-// grad = (p - t) * scale; loss accumulates -sum t log p.
+// (lanes over classes).  targets: pattern-major [Nt][C], row tidx[b] (one
+// contiguous row per pattern); grad = (p - t) * scale; loss accumulates
+// -sum t log p.
 __global__ __launch_bounds__(256) void k_softmax_xent(const float* __restrict__ logits, int C, int B,
                                                       const float* __restrict__ targets, long Nt,
                                                       const int* __restrict__ tidx, float scale,
@@ -226,11 +330,11 @@ __global__ __launch_bounds__(256) void k_softmax_xent(const float* __restrict__ 
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   const double ls = log(s);
-  const long col = tidx[b];
+  const float* trow = targets + (long)tidx[b] * C;
   double l = 0.0;
   for (int c = lane; c < C; c += 64) {
     const double lp = (double)logits[(long)c * B + b] - m - ls;
-    const float tc = targets[(long)c * Nt + col];
+    const float tc = trow[c];
     if (tc > 0.0f) l -= tc * lp;
     if (grad) grad[(long)c * B + b] = (float)((exp(lp) - tc) * scale);
   }
@@ -303,16 +407,31 @@ __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ logits, 
   }
 }
 
-// Coarse targets: T[c][i] = sum over fine rotations r of cell c of P[r][i]
-// (P in the ABI's [R][N] layout; cells as CSR lists of fine rotations).
-__global__ void k_coarse_targets(const float* __restrict__ P, int N, const int* __restrict__ cell_off,
-                                 const int* __restrict__ cell_rot, int C, float* __restrict__ T) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int c = blockIdx.y;
-  if (i >= N || c >= C) return;
-  double s = 0.0;
-  for (int j = cell_off[c]; j < cell_off[c + 1]; j++) s += P[(long)cell_rot[j] * N + i];
-  T[(long)c * N + i] = (float)s;
+// Coarse targets, pattern-major: T[i][c] = sum over the fine rotations r of
+// cell c of P[r][i] (P in the ABI's [R][N] layout; cells as CSR lists of
+// fine rotations).  A workgroup takes 64 patterns x 64 cells: lanes along
+// patterns read the rows of P coalesced, the 64 x 64 sums are transposed in
+// LDS and written along cells.
+__global__ __launch_bounds__(256) void k_coarse_targets(const float* __restrict__ P, int N,
+                                                        const int* __restrict__ cell_off,
+                                                        const int* __restrict__ cell_rot, int C,
+                                                        float* __restrict__ T) {
+  __shared__ float tile[64][65];
+  const int i0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = i0 + lane;
+  for (int cc = w; cc < 64; cc += 4) {
+    const int c = c0 + cc;
+    double s = 0.0;
+    if (c < C && i < N)
+      for (int j = cell_off[c]; j < cell_off[c + 1]; j++) s += P[(long)cell_rot[j] * N + i];
+    tile[lane][cc] = (float)s;
+  }
+  __syncthreads();
+  for (int ii = w; ii < 64; ii += 4) {
+    const int c = c0 + lane;
+    if (i0 + ii < N && c < C) T[(long)(i0 + ii) * C + c] = tile[ii][lane];
+  }
 }
 
 static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
@@ -360,7 +479,8 @@ void topk(const float* logits, int C, int B, int k, const int* out_rows, int* ou
   EMC_LAUNCHED();
 }
 void coarse_targets(const float* P, int N, const int* cell_off, const int* cell_rot, int C, float* T, hipStream_t st) {
-  hipLaunchKernelGGL(k_coarse_targets, dim3(nblk(N), C), dim3(256), 0, st, P, N, cell_off, cell_rot, C, T);
+  hipLaunchKernelGGL(k_coarse_targets, dim3((N + 63) / 64, (C + 63) / 64), dim3(256), 0, st, P, N, cell_off, cell_rot,
+                     C, T);
   EMC_LAUNCHED();
 }
 

@@ -368,10 +368,10 @@ __global__ __launch_bounds__(256) void k_norm_scale(float* __restrict__ model, i
   for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) model[k] *= f;
 }
 
-// red must hold >= 1024 + 2 doubles.
+// red must hold >= kNormMaxBlocks + 2 doubles.
 void launch_normalize_model(float* model, const float* weight, int n, double* red, hipStream_t st) {
-  const int nb = std::min(1024, (n + 255) / 256);
-  float* fac = reinterpret_cast<float*>(red + 1024);
+  const int nb = (int)std::min<long>(kNormMaxBlocks, ((long)n + 255) / 256);
+  float* fac = reinterpret_cast<float*>(red + kNormMaxBlocks);
   hipLaunchKernelGGL(k_norm_divide, dim3(nb), dim3(256), 0, st, model, weight, n, red);
   EMC_LAUNCHED();
   hipLaunchKernelGGL(k_norm_factor, dim3(1), dim3(256), 0, st, red, nb, n, fac);

@@ -35,7 +35,7 @@ void Proposal::release() {
                   &d_nbr, &d_mark, &d_shared, &d_count, &d_ncand, &d_moved, &d_keys_s, &d_vals_s, &d_sort_tmp, &d_colmap,
                   &d_gkey, &d_gcount, &d_gpat, &d_gkey_s, &d_ghist, &d_gcore, &d_gncore, &d_gnrot, &d_gcofs, &d_gtrunc, &d_crot,
                   &d_ccell, &d_cslot, &d_rank, &d_rkey, &d_gwork, &d_rcount, &d_rncand, &d_rent_off, &d_rent_pat,
-                  &d_rent_j, &d_rwork, &d_rkeys_s, &d_rvals_s};
+                  &d_rent_j, &d_rwork, &d_rkeys_s, &d_rvals_s, &d_targets, &d_perm};
   for (DBuf* b : bufs) b->release();
   cnn.release();
   configured = predicted = have_p = false;
@@ -680,11 +680,55 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   EMC_CHECK(hipMemcpyAsync(&tot, p.d_tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
   sync_stream();
   if (failed()) return false;
+  double dtot = 0.0;
+  if (e.refine && e.like.slices && e.like.images) {
+    // the significant responsibilities from FP64 likelihoods, in the matrix
+    // and in the candidate, shared and subset buffers the M-step reads
+    PhaseTimer pt(PH_REFINE);
+    RefineInputs in;
+    in.respons = respons;
+    in.N = N;
+    in.R = R;
+    in.N2d = e.like.N2d;
+    in.slices = e.like.slices;
+    in.images = e.like.images;
+    if (p.R_sh > 0) {
+      RefineRows& s = in.rows[in.nrows++];
+      s.P = p.d_Lsh.as<float>();
+      s.rows = p.R_sh;
+      s.ld = N;
+      s.cols = N;
+      s.row_rot = p.d_sh_rot.as<int>();
+      s.skip = p.d_insub.as<unsigned char>();
+      s.own = true;
+    }
+    if (p.n_sub > 0) {
+      RefineRows& s = in.rows[in.nrows++];
+      s.P = p.d_Lsub.as<float>();
+      s.rows = R;
+      s.ld = p.n_sub;
+      s.cols = p.n_sub;
+      s.col_pat = p.d_sub.as<int>();
+      s.own = true;
+    }
+    in.Lc = p.d_Lc.as<float>();
+    in.cand = p.d_cand.as<int>();
+    in.cell_off = p.d_cell_off.as<int>();
+    in.cell_rot = p.d_cell_rot.as<int>();
+    in.insub = p.d_insub.as<unsigned char>();
+    in.topk = p.topk;
+    in.slots = kSlot;
+    in.phiden = e.d_phiden.as<double>();
+    in.poisson = e.likelihood == LIKE_POISSON;
+    in.sigma2x = sigma2x;
+    in.tau = e.refine_tau;
+    if (!refine_posterior(in, &dtot)) return false;
+  }
   double pairs = (double)p.n_sub * R + (double)(N - p.n_sub) * p.R_sh;
   for (int i = 0; i < N; i++)
     if (!p.h_insub[i]) pairs += p.h_ncand[i];
   p.st_pairs = pairs;
-  *total = tot + p.sub_total;
+  *total = tot + p.sub_total + dtot;
   p.have_p = true;
   p.post = Proposal::POST_CAND;
   p.respons = respons;
@@ -929,7 +973,12 @@ int emc_hip_cnn_step(const real* d_images, const int* d_mask, int N2d, const int
   if (!t.ensure(sizeof(float) * (size_t)n.ncell * B) || !ti.ensure(sizeof(int) * B)) return 1;
   std::vector<int> iota(B);
   std::iota(iota.begin(), iota.end(), 0);
-  EMC_CHECK(hipMemcpyAsync(t.p, targets_host, sizeof(float) * (size_t)n.ncell * B, hipMemcpyHostToDevice, e.stream));
+  // the caller's [n_cells][B] targets, pattern-major for the loss kernel
+  std::vector<float> tt((size_t)n.ncell * B);
+  for (int c = 0; c < n.ncell; c++)
+    for (int b = 0; b < B; b++) tt[(size_t)b * n.ncell + c] = targets_host[(size_t)c * B + b];
+  EMC_CHECK(hipMemcpyAsync(t.p, tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice, e.stream));
+  EMC_CHECK(hipStreamSynchronize(e.stream));
   EMC_CHECK(hipMemcpyAsync(ti.p, iota.data(), sizeof(int) * B, hipMemcpyHostToDevice, e.stream));
   EMC_CHECK(hipMemcpyAsync(n.idx.p, idx_host, sizeof(int) * B, hipMemcpyHostToDevice, e.stream));
   n.forward(d_images, d_mask, N2d, n.idx.as<int>(), B, e.stream);
@@ -1070,7 +1119,8 @@ int emc_hip_proposal_train(const real* d_images, const int* d_mask, int N_2d, co
   hipStream_t st = e.stream;
   const auto t0 = std::chrono::steady_clock::now();
   const int N = N_images, B = p.batch;
-  DBuf T, perm;
+  DBuf& T = p.d_targets;
+  DBuf& perm = p.d_perm;
   if (!T.ensure(sizeof(float) * (size_t)p.C * N) || !perm.ensure(sizeof(int) * N)) return 1;
   coarse_targets(d_respons, N, p.d_cell_off.as<int>(), p.d_cell_rot.as<int>(), p.C, T.as<float>(), st);
   std::vector<int> order(N);
@@ -1094,8 +1144,6 @@ int emc_hip_proposal_train(const real* d_images, const int* d_mask, int N_2d, co
     if (failed()) break;
     last /= N;
   }
-  T.release();
-  perm.release();
   p.st_train_loss = last;
   p.st_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (loss) *loss = last;

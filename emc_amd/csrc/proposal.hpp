@@ -96,6 +96,8 @@ struct Proposal {
   double st_cleared = 0;             // responsibility entries cleared by the last write (N * R unless delta)
   DBuf d_rent_off, d_rent_pat, d_rent_j, d_rwork, d_rkeys_s, d_rvals_s;
   int n_rwork = 0;
+  DBuf d_targets;  // float [N][C] training targets (coarse-cell posterior mass)
+  DBuf d_perm;     // int [N] shuffled pattern order of a training epoch
   DBuf d_Lc;      // float [N][topk][32]: L, then P
   DBuf d_panels;  // half [chunks][2][32][32] M-step operand
   DBuf d_crowscale;

@@ -86,6 +86,13 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 scales; "tiled": the latter at every size.
  *   "expand_rotations"  rotations per workgroup of the tiled expand + pack
  *                 (default "8"; 1 .. 64).
+ *   "refine"     "on" (default): after the softmax, every responsibility of
+ *                 at least "refine_tau" (default "1e-3") is recomputed from
+ *                 its likelihood summed in FP64 over the pattern's nonzero
+ *                 counts (the matrix cores' FP32 sums over the pixels and
+ *                 the FP32 storage of L are off by up to ~5e-3 at 15,000
+ *                 pixels); each pattern's refined entries are renormalised
+ *                 among themselves, the others keep their values; "off".
  *   "gemm_tile"  "256" (default): 256 x 256 tiles of the FP16 likelihood
  *                 GEMM, one workgroup per compute unit; "128": 128 x 128
  *                 tiles, two workgroups per compute unit (same results, bit
@@ -160,7 +167,8 @@ void emc_hip_invalidate(void);
  *   every pattern on the shared cells' rotations), 12 groups (proposal mode:
  *   the dense blocks of pattern groups; phase 2 is then the per-cell kernel
  *   for the other candidate entries), 13 audit_gemm (proposal mode: the
- *   whole-table likelihood of the subset; its other work is phase 10).
+ *   whole-table likelihood of the subset; its other work is phase 10), 14
+ *   refine (the FP64 recomputation of the significant responsibilities).
  * Returns the number of phases written (<= n). */
 int emc_hip_phase_times(double *ms, long long *launches, int n);
 void emc_hip_reset_phase_times(void);

@@ -24,6 +24,14 @@ int o_max_threads(void) {
 #endif
 }
 
+void o_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* Rotation table: restatement of emc.c:40-363.                              */
 /* ------------------------------------------------------------------------ */

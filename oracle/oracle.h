@@ -118,6 +118,8 @@ double r32_iteration(float *model, float *weight, float *slices, const float *im
                      const float *x, const float *y, const float *z, const float *q,
                      const float *weights, float sigma, int nthreads);
 int o_max_threads(void);
+/* Threads of the OpenMP loops from now on (n > 0). */
+void o_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,7 @@ _SIGS = [
     ("r32_iteration", C.c_double, [_f, _f, _f, _f, _i, _f, _f, C.c_int, C.c_int, C.c_int, _f, _f, _f, _f, _f,
                                    C.c_float, C.c_int]),
     ("o_max_threads", C.c_int, []),
+    ("o_set_threads", None, [C.c_int]),
     ("o64_iteration_large", C.c_double, [_d, _f, _i, _d, C.c_int, C.c_int, C.c_int, _f, _f, _f, _f, _f,
                                          C.c_double, C.c_int, C.c_void_p, _d, _d, _d, _d]),
 ]
@@ -181,6 +182,12 @@ def r32_normalize_model(model, weight):
     return m
 
 
+def r32_model_max(model):
+    """max(0, max M) as cuda_model_max returns it (emc_cuda.cu:364-368)."""
+    m = c32(model).reshape(-1)
+    return float(lib().r32_model_max(f(m), m.size))
+
+
 def r32_iteration(model, images, mask, scaling, S, x, y, z, q, weights, sigma=0.4, nthreads=0):
     """One full iteration; returns (model, weight, scaling, P, total)."""
     m = c32(model).copy()
@@ -195,7 +202,7 @@ def r32_iteration(model, images, mask, scaling, S, x, y, z, q, weights, sigma=0.
     return m, wgt, sc, P, tot
 
 
-def time_reference_iteration(model, images, mask, S, x, y, z, q, weights, n_full, sigma=0.4):
+def time_reference_iteration(model, images, mask, S, x, y, z, q, weights, n_full, sigma=0.4, nthreads=0):
     """Wall time of one FP32-restatement iteration (emc.c:989-1084 order) on
     the pattern sample `images`, phase by phase, and its extrapolation to
     n_full patterns: the per-pattern phases (likelihood, softmax, sum P log P,
@@ -224,6 +231,10 @@ def time_reference_iteration(model, images, mask, S, x, y, z, q, weights, n_full
         fn()
         t[name] = time.perf_counter() - t0
 
+    saved = L.o_max_threads()
+    if nthreads > 0:
+        L.o_set_threads(nthreads)
+    threads = L.o_max_threads()
     run("expand", lambda: L.r32_expand(f(m), S, f(q), R, f(x), f(y), f(z), f(sl)))
     run("likelihood", lambda: L.r32_likelihood(f(sl), f(images), i(mask), sigma, f(sc), f(P), N2d, Ns, R))
     run("softmax", lambda: L.r32_softmax(f(P), R, Ns))
@@ -236,8 +247,8 @@ def time_reference_iteration(model, images, mask, S, x, y, z, q, weights, n_full
     run("normalize", lambda: L.r32_normalize_model(f(m), f(wg), m.size))
     dep = t["likelihood"] + t["softmax"] + t["total"] + t["scaling"] + t["mstep"]
     indep = t["expand"] + t["insert"] + t["normalize"]
-    return {"phases": t, "sample_s": dep + indep, "full_s": indep + dep * n_full / Ns,
-            "threads": L.o_max_threads()}
+    L.o_set_threads(saved)
+    return {"phases": t, "sample_s": dep + indep, "full_s": indep + dep * n_full / Ns, "threads": threads}
 
 
 # ---- FP64 oracle ----

@@ -13,7 +13,7 @@ INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 
 def declared_functions():
     names = set()
-    for h in ("emc.h", "emc_hip.h", "emc_host.h"):
+    for h in ("emc.h", "emc_hip.h", "emc_host.h", "emc_io.h"):
         src = open(os.path.join(INCLUDE, h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         src = re.sub(r"static inline[^{]*\{[^}]*\}", "", src)

@@ -2,9 +2,10 @@
 FP64 oracle of the same equations, dense (FP32 and FP16 log-slice
 operand) and candidate-restricted.
 
-Tolerances: responsibilities 3e-4 absolute (L ~ 10^3 here, accumulated in
-FP32 over the masked pixels: ~1e-4 relative error in L differences), scaling
-and model 1e-4 relative."""
+Tolerances: responsibilities 1e-4 absolute, scaling and model 1e-4
+relative.  (The matrix cores' FP32 sums of L ~ 10^3 here are off by ~1e-4
+in L; the responsibilities of at least 1e-3 are recomputed from FP64
+likelihoods after the softmax, option "refine".)"""
 import numpy as np
 import pytest
 
@@ -61,7 +62,7 @@ def test_poisson_iteration_vs_fp64(low, precision):
     dt = abs(g["total"] - ref["total"]) / abs(ref["total"])
     print(f"Poisson {precision} vs FP64: P {dP:.2e}, scaling {ds:.2e}, model {dm:.2e}, sum PlogP {dt:.2e}; "
           f"P max {ref['P'].max():.3f}")
-    assert dP < 3e-4 and ds < 1e-4 and dm < 1e-4
+    assert dP < 1e-4 and ds < 1e-4 and dm < 1e-4
 
 
 def test_poisson_candidates_all_cells_reproduce_dense(low):
@@ -121,4 +122,4 @@ def test_sparse_estep_matches_dense_and_fp64(low):
     dm = rel(sp["model"], ref["model"])
     ds = rel(sp["scaling"], ref["scaling"])
     print(f"sparse Poisson E-step: vs dense FP16 P {dPd:.2e}; vs FP64 P {dP:.2e}, scaling {ds:.2e}, model {dm:.2e}")
-    assert dPd < 1e-4 and dP < 3e-4 and ds < 1e-4 and dm < 1e-4
+    assert dPd < 1e-4 and dP < 1e-4 and ds < 1e-4 and dm < 1e-4

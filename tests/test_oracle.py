@@ -556,3 +556,14 @@ def test_host_insert_slice_identity():
                 v = (vz * S + vy) * S + vx
                 assert model[v] == 2.0 * sl[yy * S + xx] and weight[v] == 2.0
     assert weight.sum() == 2.0 * (S - 1) * (S - 1)
+
+
+def test_model_max_is_clamped_maximum():
+    """cuda_model_max (emc_cuda.cu:364-368) reduces with max starting from 0:
+    max(0, max M); an all-negative model gives 0."""
+    rng = np.random.default_rng(4)
+    m = rng.normal(size=5000).astype(np.float32)
+    assert O.r32_model_max(m) == float(m.max())
+    assert O.r32_model_max(-np.abs(m) - 1.0) == 0.0
+    m[1234] = 7.5
+    assert O.r32_model_max(m) == 7.5
