@@ -103,6 +103,7 @@ _SIGS = [
     ("emc_hip_synchronize", None, []),
     ("emc_hip_shutdown", None, []),
     ("emc_hip_total_respons_n", C.c_float, [C.c_void_p, C.c_size_t]),
+    ("emc_hip_normalize_model_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ("emc_hip_allocate_bytes", None, [c_void_pp, C.c_size_t]),
     ("emc_hip_host_alloc", C.c_int, [c_void_pp, C.c_size_t]),
     ("emc_hip_host_free", None, [C.c_void_p]),

@@ -81,7 +81,8 @@ bool Engine::init() {
     set_error("hipStreamCreate failed");
     return false;
   }
-  return true;
+  // the allocation-free normalisation's scratch address on this device
+  return noalloc_scratch(device) != nullptr;
 }
 
 void Engine::release_all() {

@@ -391,6 +391,10 @@ bool refine_posterior(const RefineInputs& in, double* dtotal);
 // red: kNormMaxBlocks partial sums, then the factor (kNormMaxBlocks + 2
 // doubles); the partial-sum grid never exceeds kNormMaxBlocks blocks.
 constexpr int kNormMaxBlocks = 1024;
+// Device address of the normalisation scratch of the allocation-free
+// entry point on `device` (noalloc.hip), resolved on first use; nullptr
+// and an error set on failure.
+double* noalloc_scratch(int device);
 void launch_normalize_model(float* model, const float* weight, int n, double* red,
                             hipStream_t st);
 void launch_model_max(const float* model, int n, float* out, hipStream_t st);

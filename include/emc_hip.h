@@ -42,6 +42,18 @@ void emc_hip_shutdown(void);
  * int the reference's driver passes when it matches the buffer's shape). */
 real emc_hip_total_respons_n(const real *d_respons, size_t n);
 
+/* cuda_normalize_model (emc.h; reference emc_cuda.cu:414-433) that
+ * allocates nothing, for callers that cannot call hipMalloc (code under
+ * stream capture, tight serving loops).  No counterpart in the reference.
+ * Same kernels and result, bit for bit; d_model and d_weight hold
+ * model_size voxels.  Its scratch (1,024 FP64 partial sums and the factor,
+ * 8 KB) is a static device array of the library, one per GPU, held until
+ * the process ends: any model_size from 0 to INT_MAX fits; a negative size
+ * or a null pointer is rejected (-1, error set).  Calls on one GPU share
+ * that scratch, so calls that overlap (other host threads, or streams other
+ * than the library's) race.  Returns 0 on success. */
+int emc_hip_normalize_model_noalloc(real *d_model, const real *d_weight, int model_size);
+
 /* Device buffer of `bytes` bytes (the size_t form of cuda_allocate_real). */
 void emc_hip_allocate_bytes(void **d_ptr, size_t bytes);
 /* Page-locked host buffer (no counterpart in the reference): host arrays
