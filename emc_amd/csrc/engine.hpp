@@ -83,7 +83,7 @@ struct Engine {
   bool expand_tiled = false;    // fused expand + pack in rotation blocks x pixel tiles even where the slice fits LDS
   int expand_rotations = 8;     // rotations per workgroup of that form
   bool refine = true;           // responsibilities >= refine_tau recomputed from FP64 likelihoods (k_refine.hip)
-  float refine_tau = 1e-3f;
+  float refine_tau = 1e-6f;
   bool timing = false;
   bool verbose = false;
 
@@ -383,7 +383,7 @@ struct RefineInputs {
   double* phiden = nullptr;  // [N] scaling denominators to correct, or nullptr
   bool poisson = false;
   double sigma2x = 0.0;      // 2 n_m sigma^2 (Gaussian)
-  float tau = 1e-3f;
+  float tau = 1e-6f;
 };
 // Recompute the entries >= in.tau; *dtotal receives the change of sum P log P.
 bool refine_posterior(const RefineInputs& in, double* dtotal);

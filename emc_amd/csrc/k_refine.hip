@@ -5,9 +5,12 @@
 // likelihood, S for the Gaussian one) and stores L in FP32.  At the
 // benchmark depth (15,576 pixels, |L| ~ 4e4) the accumulation and storage
 // rounding reach ~5e-3 in L, which moves a responsibility shared by two or
-// three rotations by up to ~1e-3.  Entries that far below the maximum do not
-// matter (a responsibility P changes by about P * dL), so after the softmax
-// every entry with P >= tau is recomputed: C_ir over the pattern's nonzero
+// three rotations by up to ~1e-3 and every responsibility by ~5e-3 of
+// itself.  That relative error still matters for small entries: a voxel
+// reached only by rotations of small T_r takes its value M / W from them
+// alone.  So after the softmax every entry with P >= tau (1e-6 by default;
+// a few per pattern under the Poisson likelihood) is recomputed: C_ir over
+// the pattern's nonzero
 // counts in FP64 (S read from the FP32 slices, f in FP64), L_ir in FP64,
 // then per pattern
 //   P'_ri = (1 - f_i) exp(L_ri - M_i) / sum_refined exp(L - M_i),

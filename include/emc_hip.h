@@ -99,12 +99,16 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *   "expand_rotations"  rotations per workgroup of the tiled expand + pack
  *                 (default "8"; 1 .. 64).
  *   "refine"     "on" (default): after the softmax, every responsibility of
- *                 at least "refine_tau" (default "1e-3") is recomputed from
+ *                 at least "refine_tau" (default "1e-6") is recomputed from
  *                 its likelihood summed in FP64 over the pattern's nonzero
  *                 counts (the matrix cores' FP32 sums over the pixels and
  *                 the FP32 storage of L are off by up to ~5e-3 at 15,000
- *                 pixels); each pattern's refined entries are renormalised
- *                 among themselves, the others keep their values; "off".
+ *                 pixels, a relative error of ~5e-3 in every P); each
+ *                 pattern's refined entries are renormalised among
+ *                 themselves, the others keep their values.  The threshold
+ *                 reaches down to 1e-6 because voxels whose whole weight
+ *                 comes from small responsibilities take their value from
+ *                 those alone (M / W); "off".
  *   "gemm_tile"  "256" (default): 256 x 256 tiles of the FP16 likelihood
  *                 GEMM, one workgroup per compute unit; "128": 128 x 128
  *                 tiles, two workgroups per compute unit (same results, bit

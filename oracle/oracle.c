@@ -855,7 +855,8 @@ double o64_iteration_large(const double *model_in, const float *images, const in
                            const double *scaling_in, int N, int R, int S, const float *q,
                            const float *weights, const float *x, const float *y, const float *z,
                            double sigma, int kind, const unsigned char *allowed, double *L,
-                           double *P, double *scaling_out, double *model_out) {
+                           double *P, double *scaling_out, double *model_out, double *raw_out,
+                           double *weight_out) {
   const int N2d = S * S;
   const size_t n3 = (size_t)S * S * S;
   int nm = 0;
@@ -995,6 +996,8 @@ double o64_iteration_large(const double *model_in, const float *images, const in
       }
     }
   }
+  if (raw_out) memcpy(raw_out, model_out, sizeof(double) * n3);
+  if (weight_out) memcpy(weight_out, wsum, sizeof(double) * n3);
   o64_normalize_model(model_out, wsum, (int)n3);
   free(wsum);
   free(wr);

@@ -103,12 +103,15 @@ void o64_normalize_model(double *model, const double *weight, int n);
 /* One FP64 iteration (nearest voxel) for large problems: masked pixels only,
  * nonzero counts as lists, zero responsibilities skipped.  allowed [R][N] or
  * NULL restricts each pattern's softmax.  Writes L and P [R][N], the new
- * scaling [N] and the normalised model [S^3]; returns sum P log P. */
+ * scaling [N] and the normalised model [S^3], and when given the
+ * accumulators before normalisation (raw_out, weight_out [S^3]); returns
+ * sum P log P. */
 double o64_iteration_large(const double *model_in, const float *images, const int *mask,
                            const double *scaling_in, int N, int R, int S, const float *q,
                            const float *weights, const float *x, const float *y, const float *z,
                            double sigma, int kind, const unsigned char *allowed, double *L,
-                           double *P, double *scaling_out, double *model_out);
+                           double *P, double *scaling_out, double *model_out, double *raw_out,
+                           double *weight_out);
 
 /* ---- whole iteration (driver order emc.c:989-1084), FP32 restatement ----
  * model/weight in-out; scaling in-out; respons scratch [R][N]; slices scratch

@@ -54,7 +54,7 @@ _SIGS = [
     ("o_max_threads", C.c_int, []),
     ("o_set_threads", None, [C.c_int]),
     ("o64_iteration_large", C.c_double, [_d, _f, _i, _d, C.c_int, C.c_int, C.c_int, _f, _f, _f, _f, _f,
-                                         C.c_double, C.c_int, C.c_void_p, _d, _d, _d, _d]),
+                                         C.c_double, C.c_int, C.c_void_p, _d, _d, _d, _d, _d, _d]),
 ]
 
 _LIB = None
@@ -340,6 +340,8 @@ def o64_iteration_large(model, images, mask, scaling, S, q, weights, x, y, z, si
     P = np.zeros((R, N), np.float64)
     phi = np.zeros(N, np.float64)
     m = np.zeros(S ** 3, np.float64)
+    raw = np.zeros(S ** 3, np.float64)
+    wgt = np.zeros(S ** 3, np.float64)
     al = None
     if allowed is not None:
         allowed = np.ascontiguousarray(allowed, np.uint8)
@@ -347,8 +349,8 @@ def o64_iteration_large(model, images, mask, scaling, S, q, weights, x, y, z, si
         al = allowed.ctypes.data_as(C.c_void_p)
     tot = lib().o64_iteration_large(d(c64(model)), f(images), i(ci(mask)), d(c64(scaling)), N, R, S, f(c32(q)),
                                     f(c32(weights)), f(c32(x)), f(c32(y)), f(c32(z)), sigma, kind, al, d(L), d(P),
-                                    d(phi), d(m))
-    return {"L": L, "P": P, "total": tot, "scaling": phi, "model": m}
+                                    d(phi), d(m), d(raw), d(wgt))
+    return {"L": L, "P": P, "total": tot, "scaling": phi, "model": m, "model_raw": raw, "weight": wgt}
 
 
 # ---- learned proposal: candidate-restricted iteration (numpy, FP64) ----

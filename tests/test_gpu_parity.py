@@ -93,10 +93,13 @@ def check_one_step(P, prob, out, sigma=0.4, l_rtol=2e-5, fp16=False):
     assert np.array_equal(out["L"], out["L_host"])
     scale = np.abs(L).max()
     assert np.abs(out["L"] - L).max() <= l_rtol * scale, np.abs(out["L"] - L).max() / scale
-    # responsibilities: GPU softmax of GPU L vs oracle softmax of oracle L
+    # responsibilities: GPU softmax of GPU L vs oracle softmax of oracle L;
+    # against the FP64 responsibilities of the same inputs (the significant
+    # entries are recomputed from FP64 likelihoods after the softmax)
     Pref = O.r32_softmax(L)
     assert np.abs(out["P"] - Pref).max() <= P_TOL
-    assert np.abs(out["P"] - O.r32_softmax(out["L"])).max() <= 2e-6
+    L64 = O.o64_likelihood(sl.astype(np.float64), P.images, P.mask, sigma, sc_in.astype(np.float64))
+    assert np.abs(out["P"] - O.o64_softmax(L64)).max() <= 2e-6
     t = O.r32_total_respons(out["P"])
     assert abs(out["total"] - t) <= 1e-5 * abs(t) + 1e-3
     phi = O.r32_update_scaling(P.images, sl, P.mask, out["P"])

@@ -26,6 +26,7 @@ def test_large_iteration_equals_dense(kind, start):
     assert np.abs(g["P"] - ref["P"]).max() < 1e-12
     assert rel(g["scaling"], ref["scaling"]) < 1e-12
     assert rel(g["model"], ref["model"]) < 1e-12
+    assert rel(g["model_raw"], ref["model_raw"]) < 1e-12 and rel(g["weight"], ref["weight"]) < 1e-12
     assert abs(g["total"] - ref["total"]) < 1e-9 * abs(ref["total"])
     allowed = np.random.default_rng(1).random((P.R, P.N)) < 0.5
     ra = O.o64_proposal_iteration(m64, P.images, P.mask, phi, P.side, P.q, P.w, P.x, P.y, P.z, allowed, kind=kind)
