@@ -86,12 +86,13 @@ def timed(E, prob, sigma, steps, learned=False):
     L.emc_hip_reset_phase_times()
     L.emc_hip_synchronize()
     t0 = time.perf_counter()
-    per, hist, its = [], [], []
+    per, hist, its, refined = [], [], [], []
     prev = {}
     for _ in range(steps):
         t1 = time.perf_counter()
         hist.append(prob.iteration(sigma))
         its.append(time.perf_counter() - t1)
+        refined.append(L.emc_hip_refine_count())
         if learned:
             st = proposal_stats(L)
             ph = phase_times(L)
@@ -104,7 +105,7 @@ def timed(E, prob, sigma, steps, learned=False):
     E.check()
     ph = phase_times(L)
     E.set_option("timing", "0")
-    return dt, ph, per, hist, its
+    return dt, ph, per, hist, its, float(np.mean(refined))
 
 
 def run_dense(E, W, precision, warmup, steps, sigma, likelihood, pre=0):
@@ -122,14 +123,15 @@ def run_dense(E, W, precision, warmup, steps, sigma, likelihood, pre=0):
     for it in range(pre + warmup):
         history.append(prob.iteration(sigma))
         log(f"  {precision} untimed iteration {it + 1} done")
-    dt, ph, _, hist, its = timed(E, prob, sigma, steps)
+    dt, ph, _, hist, its, refined = timed(E, prob, sigma, steps)
     total_s = time.perf_counter() - t0
     history += hist
     model = prob.model()
     prob.free()
     L.emc_hip_invalidate()
     log(f"  {precision}: {dt * 1e3:.1f} ms/iteration")
-    return {"dt": dt, "phases": ph, "model": model, "history": history, "iteration_s": its, "total_s": total_s}
+    return {"dt": dt, "phases": ph, "model": model, "history": history, "iteration_s": its, "total_s": total_s,
+            "refined": refined}
 
 
 def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr, topk, n_coarse, max_fail,
@@ -197,7 +199,7 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
             f"{st['audit_below_min_mass']:.0f}/{st['audited']:.0f}, min mass {st['audit_min_mass']:.6f}, "
             f"mean {st['audit_mean_mass']:.6f}, whole-table iteration {st['whole_table_iteration']:.0f}")
     E.set_option("audit_fraction", "0.02")
-    dt, ph, per, hist, its = timed(E, prob, sigma, steps, learned=True)
+    dt, ph, per, hist, its, refined = timed(E, prob, sigma, steps, learned=True)
     for k, st in enumerate(per):
         log(f"  proposal iteration {train_iters + warmup + k + 1}: {its[k] * 1e3:.1f} ms, "
             f"{'whole table' if st['whole_table_iteration'] else 'candidates'}, audit below min mass "
@@ -211,6 +213,7 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
     model = prob.model()
     state_model, state_scaling = model.copy(), prob.scaling().copy()
     res = {"dt": dt, "phases": ph, "model": model, "scaling": state_scaling, "per_step": per, "iteration_s": its,
+           "refined": refined,
            "dense_s": dense_s, "train_s": train_s, "predict_s": predict_s, "losses": losses, "gated": gated,
            "total_s": total_s, "predicted_mass_q1_q10_q50": pq, "history": history, "first_audit": first_audit}
     if not one_step:
@@ -352,8 +355,8 @@ def main():
     ap.add_argument("--topk", type=int, default=32)
     ap.add_argument("--coarse-n", type=int, default=4)
     ap.add_argument("--train-batch", type=int, default=256, help="patterns per network training step")
-    ap.add_argument("--insert", choices=["atomic", "deterministic"], default="atomic",
-                    help="insertion mode: float atomics, or the deterministic sorted-scatter gather")
+    ap.add_argument("--insert", choices=["atomic", "global_atomic", "deterministic"], default="atomic",
+                    help="insertion mode: brick-owned LDS atomics (atomic), plain global float atomics, or the deterministic sorted-scatter gather")
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip the two comparison runs (untrained network; network without tracking and shared "
                          "cells)")
@@ -485,6 +488,7 @@ def main():
                      "traffic": read_traffic(kname, W.cfg.name), "launch_ms": like_avg * 1e3,
                      "algorithmic_flop_per_launch": flop},
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in fast["phases"].items()},
+        "refined_responsibilities_per_step": fast["refined"],
     }
     if learned:
         steps_st = fast["per_step"]

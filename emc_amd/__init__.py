@@ -104,6 +104,7 @@ _SIGS = [
     ("emc_hip_shutdown", None, []),
     ("emc_hip_total_respons_n", C.c_float, [C.c_void_p, C.c_size_t]),
     ("emc_hip_normalize_model_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("emc_hip_respons_summary", C.c_int, [C.c_void_p, C.c_int, C.c_int, c_int_p, c_float_p]),
     ("emc_hip_allocate_bytes", None, [c_void_pp, C.c_size_t]),
     ("emc_hip_host_alloc", C.c_int, [c_void_pp, C.c_size_t]),
     ("emc_hip_host_free", None, [C.c_void_p]),
@@ -116,6 +117,7 @@ _SIGS = [
     ("emc_hip_phase_name", C.c_char_p, [C.c_int]),
     ("emc_hip_pack_info", C.c_int, [c_int_p, C.c_int]),
     ("emc_hip_mstep_nnz", C.c_longlong, []),
+    ("emc_hip_refine_count", C.c_longlong, []),
     ("emc_hip_cnn_init", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong]),
     ("emc_hip_cnn_param_count", C.c_long, []),
     ("emc_hip_cnn_get", C.c_int, [c_float_p, c_float_p]),

@@ -89,7 +89,7 @@ void Engine::release_all() {
   DBuf* bufs[] = {&d_pix, &d_B, &d_K32, &d_Kt32, &d_K16, &d_K16lo, &d_Kt16, &d_Kt16lo, &d_flags,
                   &d_S32, &d_S16, &d_S16lo, &d_Sscale, &d_A, &d_phi_used, &d_smax, &d_ssum, &d_sacc,
                   &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Tphi, &d_ones, &d_Pscale, &d_P16, &d_P16lo,
-                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_tiled, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt,
+                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_tiled, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt, &d_act, &d_zrange,
                   &d_cnt_off, &d_cnt_px, &d_cnt_val, &d_ref_ent, &d_ref_count, &d_ref_pat, &d_ref_tot,
                   &d_ref_slot, &d_ref_tmp};
   for (DBuf* b : bufs) b->release();
@@ -507,6 +507,7 @@ void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images
   if (failed()) return;
   // the significant responsibilities from FP64 likelihoods, when the
   // likelihood this buffer holds is the library's own of these slices
+  e.ref_entries = 0;
   if (e.refine && fuse && lk.slices && lk.images) {
     PhaseTimer pt(PH_REFINE);
     RefineInputs in;
@@ -790,6 +791,10 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
                          e.d_map_off.as<uint32_t>(), e.d_map_ent.as<uint32_t>(),
                          e.interp == INTERP_NEAREST ? nullptr : e.d_map_wt.as<float>(), S, e.interp, d_model, d_weight,
                          e.stream);
+  } else if (e.insert_mode == INSERT_ATOMIC && insert_brick_fits(S)) {
+    PhaseTimer pt(PH_INSERT);
+    launch_insert_brick(e.d_Sc.as<float>(), e.n_mp, e.d_kidx.as<int>(), d_mask, e.d_wT.as<float>(), d_rotations, R,
+                        S, x_coordinates, y_coordinates, z_coordinates, d_model, d_weight, e.interp, e.stream);
   } else {
     PhaseTimer pt(PH_INSERT);
     launch_insert_atomic(e.d_Sc.as<float>(), e.n_mp, e.d_pix.as<int>(), e.d_ins_tiled.as<int>(), e.d_wT.as<float>(),
@@ -1014,6 +1019,7 @@ int emc_hip_set_option(const char* key, const char* value) {
     else return 1;
   } else if (is(key, "insert")) {
     if (is(value, "atomic")) e.insert_mode = INSERT_ATOMIC;
+    else if (is(value, "global_atomic")) e.insert_mode = INSERT_GLOBAL_ATOMIC;
     else if (is(value, "deterministic")) e.insert_mode = INSERT_DETERMINISTIC;
     else return 1;
   } else if (is(key, "interp")) {
@@ -1183,5 +1189,7 @@ int emc_hip_pack_info(int* out, int n) {
 }
 
 long long emc_hip_mstep_nnz(void) { return engine().p_nnz; }
+
+long long emc_hip_refine_count(void) { return engine().ref_entries; }
 
 }  // extern "C"

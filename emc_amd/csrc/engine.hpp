@@ -42,7 +42,10 @@ enum Phase {
 };
 
 enum Precision { PREC_FP32 = 0, PREC_FP16 = 1 };
-enum InsertMode { INSERT_ATOMIC = 0, INSERT_DETERMINISTIC = 1 };
+// atomic: brick-owned LDS accumulation (per-pixel global atomics where the
+// model side is not a multiple of the brick); global_atomic: per-pixel
+// global atomics; deterministic: the sorted-scatter gather
+enum InsertMode { INSERT_ATOMIC = 0, INSERT_DETERMINISTIC = 1, INSERT_GLOBAL_ATOMIC = 2 };
 // auto: the nonzero-list M-step when at most 1 in kSparseMstepRatio
 // responsibilities is nonzero, else the dense matrix-core M-step
 enum MstepMode { MSTEP_AUTO = 0, MSTEP_DENSE = 1, MSTEP_SPARSE = 2 };
@@ -83,7 +86,7 @@ struct Engine {
   bool expand_tiled = false;    // fused expand + pack in rotation blocks x pixel tiles even where the slice fits LDS
   int expand_rotations = 8;     // rotations per workgroup of that form
   bool refine = true;           // responsibilities >= refine_tau recomputed from FP64 likelihoods (k_refine.hip)
-  float refine_tau = 1e-6f;
+  float refine_tau = 1e-9f;
   bool timing = false;
   bool verbose = false;
 
@@ -205,6 +208,8 @@ struct Engine {
   DBuf d_model_brick;  // float [S^3]: the model in 4 x 4 x 4 bricks for the expand's gathers
   DBuf d_wts;        // float [R]: rotation weights (uploaded per call)
   DBuf d_wT;         // float [R]: weights[r] * T_r
+  DBuf d_act;        // int [R + 1]: rotations with T_r > 1e-10, then their count (brick insertion)
+  DBuf d_zrange;     // float [2]: range of the detector's z over the inserted pixels
 
   // ---- insertion map for the deterministic mode ----
   struct {
@@ -356,6 +361,11 @@ void launch_insert_gather(const float* Sc, int n_mp, const int* ins_col, int n_i
                           const float* ent_wt, int S, int interp, float* model, float* weight,
                           hipStream_t st);
 void launch_set_wT(const float* wts, const float* Tf, int R, float* wT, hipStream_t st);
+// brick-owned insertion (k_model.hip); the model side must be a multiple of the brick edge
+bool insert_brick_fits(int S);
+void launch_insert_brick(const float* Sc, int n_mp, const int* kidx, const int* mask, const float* wT,
+                         const float* rot, int R, int S, const float* x, const float* y, const float* z,
+                         float* model, float* weight, int interp, hipStream_t st);
 // Refinement of the significant responsibilities (k_refine.hip).  Sources:
 // [rows][ld] blocks of responsibilities (row -> rotation row_rot[row] or
 // row, column -> pattern col_pat[col] or col, patterns with skip set left
@@ -383,7 +393,7 @@ struct RefineInputs {
   double* phiden = nullptr;  // [N] scaling denominators to correct, or nullptr
   bool poisson = false;
   double sigma2x = 0.0;      // 2 n_m sigma^2 (Gaussian)
-  float tau = 1e-6f;
+  float tau = 1e-9f;
 };
 // Recompute the entries >= in.tau; *dtotal receives the change of sum P log P.
 bool refine_posterior(const RefineInputs& in, double* dtotal);

@@ -155,6 +155,176 @@ void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* 
 }
 
 // ---------------------------------------------------------------------------
+// Brick-owned insertion (the default "atomic" mode).  One workgroup owns a
+// kIB^3 brick of the model: it accumulates, in LDS, every contribution of
+// every active rotation (T_r > 1e-10) whose voxel lies in the brick, then
+// adds the brick to the model and weight with plain read-modify-writes (no
+// other workgroup touches those voxels, so no global atomics at all).  The
+// pixels that can reach the brick under rotation r are found from the
+// brick's box mapped back to detector coordinates, p = R^T (u - h): the
+// plane test (the box's range of p_z against the detector's z range) drops
+// most rotations for bricks away from the centre, and the box's range of
+// (p_x, p_y) bounds the detector pixels to test; each of those is mapped
+// forward exactly as the reference maps it (rot_matrix, voxel_coord,
+// roundf) and added when its voxel falls in the brick.  LDS adds within a
+// workgroup are float atomics, so the order of the additions is open, as in
+// the reference's insertion (emc_atomic.cu:56-66).
+// ---------------------------------------------------------------------------
+constexpr int kIB = 8;  // brick edge (voxels)
+
+__global__ __launch_bounds__(256) void k_insert_brick(const float* __restrict__ Sc, int n_mp,
+                                                      const int* __restrict__ kidx, const int* __restrict__ mask,
+                                                      const int* __restrict__ act, const int* __restrict__ n_act_p,
+                                                      const float* __restrict__ wT, const float* __restrict__ rot,
+                                                      int S, const float* __restrict__ x, const float* __restrict__ y,
+                                                      const float* __restrict__ z, const float* __restrict__ zrange,
+                                                      float* __restrict__ model, float* __restrict__ weight,
+                                                      int interp) {
+#pragma clang fp contract(off)
+  constexpr int NV = kIB * kIB * kIB;
+  __shared__ float acc_m[NV], acc_w[NV];
+  __shared__ int list[256];
+  __shared__ int nlist;
+  const int t = threadIdx.x;
+  const int nb = S / kIB;
+  const int bx = blockIdx.x % nb, by = (blockIdx.x / nb) % nb, bz = blockIdx.x / (nb * nb);
+  const int x0 = bx * kIB, y0 = by * kIB, z0 = bz * kIB;
+  for (int l = t; l < NV; l += 256) {
+    acc_m[l] = 0.0f;
+    acc_w[l] = 0.0f;
+  }
+  const float half = (float)S / 2.0f + 0.5f;
+  const bool tri = interp != INTERP_NEAREST;
+  // continuous coordinates that land in the brick: round to [x0, x0 + kIB)
+  // (nearest) or have a trilinear corner there; kept 0.05 voxel wide
+  const float lo = tri ? -1.0f : -0.5f, hi = tri ? (float)kIB : (float)kIB - 0.5f;
+  const float hs = 0.5f * (hi - lo) + 0.05f;  // half-size of the box
+  const float cx = (float)x0 + 0.5f * (lo + hi) - half, cy = (float)y0 + 0.5f * (lo + hi) - half,
+              cz = (float)z0 + 0.5f * (lo + hi) - half;  // box centre relative to the origin
+  const float zlo = zrange[0] - 0.05f, zhi = zrange[1] + 0.05f;
+  const float dlo = 0.5f - 0.5f * (float)S - 0.05f, dhi = 0.5f * (float)S - 0.5f + 0.05f;  // detector extent
+  const int n_act = *n_act_p;
+  for (int base = 0; base < n_act; base += 256) {
+    if (t == 0) nlist = 0;
+    __syncthreads();
+    if (base + t < n_act) {
+      const int r = act[base + t];
+      float e[9];
+      rot_matrix(&rot[4 * (size_t)r], e);
+      // p = R^T (u - h): column j of R gives detector axis j
+      const float pz = e[2] * cx + e[5] * cy + e[8] * cz, rz = hs * (fabsf(e[2]) + fabsf(e[5]) + fabsf(e[8]));
+      const float px = e[0] * cx + e[3] * cy + e[6] * cz, rx = hs * (fabsf(e[0]) + fabsf(e[3]) + fabsf(e[6]));
+      const float py = e[1] * cx + e[4] * cy + e[7] * cz, ry = hs * (fabsf(e[1]) + fabsf(e[4]) + fabsf(e[7]));
+      if (pz - rz <= zhi && pz + rz >= zlo && px - rx <= dhi && px + rx >= dlo && py - ry <= dhi && py + ry >= dlo)
+        list[atomicAdd(&nlist, 1)] = r;
+    }
+    __syncthreads();
+    const int n = nlist;
+    for (int j = 0; j < n; j++) {
+      const int r = list[j];
+      float e[9];
+      rot_matrix(&rot[4 * (size_t)r], e);
+      const float px = e[0] * cx + e[3] * cy + e[6] * cz, rx = hs * (fabsf(e[0]) + fabsf(e[3]) + fabsf(e[6]));
+      const float py = e[1] * cx + e[4] * cy + e[7] * cz, ry = hs * (fabsf(e[1]) + fabsf(e[4]) + fabsf(e[7]));
+      // pixel index = detector coordinate + S/2 - 0.5
+      const float off = 0.5f * (float)S - 0.5f;
+      const int ix0 = max(0, (int)ceilf(px - rx + off)), ix1 = min(S - 1, (int)floorf(px + rx + off));
+      const int iy0 = max(0, (int)ceilf(py - ry + off)), iy1 = min(S - 1, (int)floorf(py + ry + off));
+      if (ix1 < ix0 || iy1 < iy0) continue;
+      const int w_ = ix1 - ix0 + 1, npx = w_ * (iy1 - iy0 + 1);
+      const float w = wT[r];
+      for (int k = t; k < npx; k += 256) {
+        const int p = (iy0 + k / w_) * S + ix0 + k % w_;
+        if (mask[p] != 1) continue;
+        float ux, uy, uz;
+        voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+        const float val = w * Sc[(size_t)r * n_mp + kidx[p]];
+        if (!tri) {
+          const int vx = (int)roundf(ux) - x0, vy = (int)roundf(uy) - y0, vz = (int)roundf(uz) - z0;
+          if ((unsigned)vx < (unsigned)kIB && (unsigned)vy < (unsigned)kIB && (unsigned)vz < (unsigned)kIB) {
+            const int l = (vz * kIB + vy) * kIB + vx;
+            atomicAdd(&acc_m[l], val);
+            atomicAdd(&acc_w[l], w);
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 8; c++) {
+            int gx, gy, gz;
+            float cw;
+            if (!tri_corner_xyz(c, ux, uy, uz, S, gx, gy, gz, cw)) continue;
+            const int vx = gx - x0, vy = gy - y0, vz = gz - z0;
+            if ((unsigned)vx < (unsigned)kIB && (unsigned)vy < (unsigned)kIB && (unsigned)vz < (unsigned)kIB) {
+              const int l = (vz * kIB + vy) * kIB + vx;
+              atomicAdd(&acc_m[l], cw * val);
+              atomicAdd(&acc_w[l], cw * w);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int l = t; l < NV; l += 256) {
+    if (acc_w[l] == 0.0f && acc_m[l] == 0.0f) continue;
+    const int vx = l % kIB, vy = (l / kIB) % kIB, vz = l / (kIB * kIB);
+    const size_t v = ((size_t)(z0 + vz) * S + (y0 + vy)) * S + (x0 + vx);
+    model[v] += acc_m[l];
+    weight[v] += acc_w[l];
+  }
+}
+
+// Active rotations (T_r > 1e-10) into act[0 .. *n), in no particular order.
+__global__ __launch_bounds__(256) void k_active_rotations(const float* __restrict__ Tf, int R, int* __restrict__ act,
+                                                          int* __restrict__ n) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  const bool on = r < R && Tf[r] > 1e-10f;
+  const unsigned long long m = __ballot(on);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(n, __popcll(m));
+  base = __shfl(base, 0, 64);
+  if (on) act[base + __popcll(m & ((1ull << lane) - 1ull))] = r;
+}
+
+// zrange = (min, max) of z over the pixels with mask == 1
+__global__ __launch_bounds__(256) void k_z_range(const float* __restrict__ z, const int* __restrict__ mask, int N2d,
+                                                 float* __restrict__ zrange) {
+  float lo = 3.0e38f, hi = -3.0e38f;
+  for (int p = threadIdx.x; p < N2d; p += 256)
+    if (mask[p] == 1) {
+      lo = fminf(lo, z[p]);
+      hi = fmaxf(hi, z[p]);
+    }
+  lo = -block_max_f32(-lo);
+  hi = block_max_f32(hi);
+  if (threadIdx.x == 0) {
+    zrange[0] = lo;
+    zrange[1] = hi;
+  }
+}
+
+void launch_insert_brick(const float* Sc, int n_mp, const int* kidx, const int* mask, const float* wT,
+                         const float* rot, int R, int S, const float* x, const float* y, const float* z,
+                         float* model, float* weight, int interp, hipStream_t st) {
+  Engine& eng = engine();
+  if (!eng.d_act.ensure(sizeof(int) * ((size_t)R + 4)) || !eng.d_zrange.ensure(16)) return;
+  int* n_act = eng.d_act.as<int>() + R;
+  launch_fill(reinterpret_cast<float*>(n_act), 1, 0.0f, st);
+  hipLaunchKernelGGL(k_active_rotations, dim3((R + 255) / 256), dim3(256), 0, st, eng.d_Tf.as<float>(), R,
+                     eng.d_act.as<int>(), n_act);
+  EMC_LAUNCHED();
+  hipLaunchKernelGGL(k_z_range, dim3(1), dim3(256), 0, st, z, mask, S * S, eng.d_zrange.as<float>());
+  EMC_LAUNCHED();
+  const int nb = S / kIB;
+  hipLaunchKernelGGL(k_insert_brick, dim3((unsigned)nb * nb * nb), dim3(256), 0, st, Sc, n_mp, kidx, mask,
+                     eng.d_act.as<int>(), n_act, wT, rot, S, x, y, z, eng.d_zrange.as<float>(), model, weight,
+                     interp);
+  EMC_LAUNCHED();
+}
+
+bool insert_brick_fits(int S) { return S % kIB == 0; }
+
+// ---------------------------------------------------------------------------
 // Deterministic insertion.  The voxel hit by (rotation r, pixel t) depends
 // only on geometry, so the inverse map voxel -> entries is built once: entry
 // e = r * n_ins + t (t enumerates inserted pixels x-major, the reference's

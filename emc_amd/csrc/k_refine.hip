@@ -8,7 +8,7 @@
 // three rotations by up to ~1e-3 and every responsibility by ~5e-3 of
 // itself.  That relative error still matters for small entries: a voxel
 // reached only by rotations of small T_r takes its value M / W from them
-// alone.  So after the softmax every entry with P >= tau (1e-6 by default;
+// alone.  So after the softmax every entry with P >= tau (1e-9 by default;
 // a few per pattern under the Poisson likelihood) is recomputed: C_ir over
 // the pattern's nonzero
 // counts in FP64 (S read from the FP32 slices, f in FP64), L_ir in FP64,

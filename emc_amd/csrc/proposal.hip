@@ -681,6 +681,7 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
   sync_stream();
   if (failed()) return false;
   double dtot = 0.0;
+  e.ref_entries = 0;
   if (e.refine && e.like.slices && e.like.images) {
     // the significant responsibilities from FP64 likelihoods, in the matrix
     // and in the candidate, shared and subset buffers the M-step reads

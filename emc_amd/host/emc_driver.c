@@ -9,14 +9,31 @@
  * likelihood.data (sum P log P per iteration), scale_error.data (RMS error of
  * the mean-normalised scaling factors against the true ones) and the model.
  *
+ * Files, as the reference's (libspimage's HDF5 layout, written and read by
+ * this library's own HDF5 code, emc_io.h):
+ *   - in: with --data, patterns from printf(PATTERN, i) (the reference's
+ *     Data/image_%04d.h5) and the mask from --mask (mask.h5), both sampled
+ *     with --read-stride about their centre and the mask cropped to a circle
+ *     (emc.c:777-864); --model-file starts from a model file (emc.c:915-928);
+ *   - out: model_init.h5 and model_init_weight.h5 (emc.c:929-950),
+ *     model_%04d.h5 every --save-every iterations (emc.c:1072-1082: the host
+ *     model the last cuda_update_slices copied back, i.e. the accumulators
+ *     before normalisation, with the host weight's mask, exactly as the
+ *     reference writes them), likelihood.data, scale_error.data;
+ *   - with --viewer-output, what the reference's viewer reads (viewer/):
+ *     state.h5 ("iteration"), best_scaling.h5 ("scaling" [iterations][N]),
+ *     average_resp_%04d.h5 ("data" [R]), best_quaternion_%04d.data (one
+ *     quaternion per pattern), and responsabilities_%04d.h5 ("data" [N][R])
+ *     and scaling_%04d.h5 ("data" [R][N]) while N * R stays below 2^26.
+ *
  * Differences from the reference, all additive:
  *   - the configuration comes from the command line instead of compile-time
  *     constants (emc.c:714-736,753,766-778);
- *   - patterns are synthetic (the reference reads Data/image_%04d.h5 through
- *     libspimage, which is not available); --config selects the workload;
+ *   - without --data, patterns are synthetic (--config selects the
+ *     workload), and scale_error.data compares with the true scales;
  *   - the host copy of the raw log-likelihoods (N*R floats) is made only
  *     with --dump-respons;
- *   - models are written as raw float32 volumes (model_%04d.raw).
+ *   - model_final.h5 holds the normalised model of the last iteration.
  */
 #include <errno.h>
 #include <math.h>
@@ -30,6 +47,7 @@
 #include "emc.h"
 #include "emc_hip.h"
 #include "emc_host.h"
+#include "emc_io.h"
 
 typedef struct {
   int side, n_blobs, n_patterns, n_rot, iterations, gap_h, gap_cross;
@@ -68,6 +86,12 @@ static void usage(void) {
           "           [--save-every K] [--quiet]\n"
           "           [--proposal T] [--topk K] [--coarse-n n] [--epochs E] [--lr LR]\n"
           "           [--likelihood gaussian|poisson] [--sparse]\n"
+          "           [--data PATTERN] [--mask FILE] [--read-stride K] [--side S]\n"
+          "           [--model random|truth|orientations] [--model-file FILE] [--viewer-output]\n"
+          "  --data PATTERN: read pattern i from printf(PATTERN, i), e.g.\n"
+          "                Data/image_%%04d.h5 (libspimage HDF5 layout), sampled on a\n"
+          "                side x side grid with stride K (default 4) about its centre\n"
+          "  --save-every K: model_%%04d.h5 every K iterations (default 1; 0: none)\n"
           "  --proposal T: train the rotation-proposal network on iterations 1..T,\n"
           "                then score only its top-K coarse cells (default: 5 for\n"
           "                --config 3 and 4, else 0 = off)\n"
@@ -77,8 +101,8 @@ static void usage(void) {
 }
 
 static void write_raw(const char *dir, const char *name, const float *v, size_t n) {
-  char path[1024];
-  snprintf(path, sizeof path, "%s/%s", dir, name);
+  char path[4096];
+  if (snprintf(path, sizeof path, "%s/%s", dir, name) >= (int)sizeof path) return;
   FILE *f = fopen(path, "wb");
   if (!f) {
     fprintf(stderr, "emc: cannot write %s: %s\n", path, strerror(errno));
@@ -88,9 +112,31 @@ static void write_raw(const char *dir, const char *name, const float *v, size_t 
   fclose(f);
 }
 
+/* A volume in libspimage's HDF5 layout with the mask weight > 0
+ * (emc.c:929-940, 1072-1080). */
+static void write_volume(const char *dir, const char *name, const float *v, const float *weight, int side) {
+  char path[4096];
+  if (snprintf(path, sizeof path, "%s/%s", dir, name) >= (int)sizeof path) return;
+  const size_t n = (size_t)side * side * side;
+  int *mask = malloc(sizeof(int) * n);
+  for (size_t k = 0; k < n; k++) mask[k] = weight[k] > 0.0f ? 1 : 0;
+  if (emc_write_spimage(path, v, mask, side, side, side)) fprintf(stderr, "emc: %s\n", emc_io_error());
+  free(mask);
+}
+
+static void write_h5(const char *dir, const char *name, const char *dset, int type, int rank, long long d0,
+                     long long d1, const void *data) {
+  char path[4096];
+  if (snprintf(path, sizeof path, "%s/%s", dir, name) >= (int)sizeof path) return;
+  emc_h5_dataset s = {dset, type, rank, {d0, d1, 0}, data};
+  if (emc_h5_write(path, &s, 1)) fprintf(stderr, "emc: %s\n", emc_io_error());
+}
+
 int main(int argc, char **argv) {
   int cfg = 0, N_images = -1, iterations = -1, n_rot = -1, full_table = 0, dump = 0, quiet = 0;
-  int save_every = 0, model_truth = 0, prop_iters = -1, topk = 32, coarse_n = 4, epochs = 2;
+  int save_every = 1, model_truth = 0, model_orient = 0, prop_iters = -1, topk = 32, coarse_n = 4, epochs = 2;
+  int read_stride = 4, side_arg = 0, viewer = 0; /* emc.c:777 */
+  const char *data_pattern = NULL, *mask_file = NULL, *model_file = NULL;
   float lr = 1e-3f;
   float sigma = 0.4f; /* emc.c:734-735 */
   uint64_t seed = 0;
@@ -118,7 +164,18 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "--out")) { if (!v) { usage(); return 2; } out = v; a++; }
     else if (!strcmp(k, "--likelihood")) { if (!v) { usage(); return 2; } like_kind = v; a++; }
     else if (!strcmp(k, "--sparse")) sparse = 1;
-    else if (!strcmp(k, "--model")) { if (!v) { usage(); return 2; } model_truth = !strcmp(v, "truth"); a++; }
+    else if (!strcmp(k, "--model")) {
+      if (!v) { usage(); return 2; }
+      model_truth = !strcmp(v, "truth");
+      model_orient = !strcmp(v, "orientations");
+      a++;
+    }
+    else if (!strcmp(k, "--data")) { if (!v) { usage(); return 2; } data_pattern = v; a++; }
+    else if (!strcmp(k, "--mask")) { if (!v) { usage(); return 2; } mask_file = v; a++; }
+    else if (!strcmp(k, "--model-file")) { if (!v) { usage(); return 2; } model_file = v; a++; }
+    else if (!strcmp(k, "--read-stride")) TAKE(read_stride, atoi);
+    else if (!strcmp(k, "--side")) TAKE(side_arg, atoi);
+    else if (!strcmp(k, "--viewer-output")) viewer = 1;
     else if (!strcmp(k, "--full-table")) full_table = 1;
     else if (!strcmp(k, "--dump-respons")) dump = 1;
     else if (!strcmp(k, "--quiet")) quiet = 1;
@@ -160,8 +217,12 @@ int main(int argc, char **argv) {
   }
   printf("%d rotations sampled (n = %d%s)\n", N_slices, n_rot, full_table ? "" : ", antipodal half");
 
+  if (data_pattern && model_truth) {
+    fprintf(stderr, "emc: --model truth needs synthetic data\n");
+    return 2;
+  }
   /* setup (emc.c:766-771), coordinates (emc.c:872-875) */
-  Setup setup = {P.side, 5.6e-9f, 16e-6f, 512, 0.15f};
+  Setup setup = {side_arg > 0 ? side_arg : P.side, 5.6e-9f, 16e-6f, 512, 0.15f};
   const int side = setup.side;
   const int N_2d = side * side;
   const size_t N_model = (size_t)side * side * side;
@@ -171,16 +232,31 @@ int main(int argc, char **argv) {
 
   /* mask and patterns */
   sp_imatrix mask = {side, side, malloc(sizeof(int) * N_2d)};
-  const int n_m = emc_make_mask(side, P.beamstop, P.gap_h, P.gap_cross, 0, mask.data);
-  emc_synth_params sp = {side, P.n_blobs, P.blob_radius, 1.0f, 2.0f, N_images, P.phi_sigma, P.photons, seed};
-  float *truth = malloc(sizeof(float) * N_model);
+  float *truth = NULL;
   float *image_data = malloc(sizeof(float) * (size_t)N_images * N_2d);
   float *intensities = malloc(sizeof(float) * N_images);
   double t0 = now_ms();
-  emc_synth_intensity(&sp, truth);
-  emc_synth_patterns(&sp, truth, xc.data, yc.data, zc.data, mask.data, image_data, NULL, intensities);
-  printf("%d patterns of %dx%d, %d masked pixels (synthetic data in %.1f s)\n", N_images, side, side, n_m,
-         (now_ms() - t0) / 1e3);
+  int n_m = 0;
+  if (data_pattern) {
+    /* emc.c:777-864 (rescale_intensity 0: every intensity 1) */
+    if (emc_read_patterns(data_pattern, N_images, side, read_stride, image_data) ||
+        emc_read_mask(mask_file, side, read_stride, mask.data)) {
+      fprintf(stderr, "emc: %s\n", emc_io_error());
+      return 1;
+    }
+    for (int i = 0; i < N_images; i++) intensities[i] = 1.0f;
+    for (int p = 0; p < N_2d; p++) n_m += mask.data[p] != 0;
+    printf("%d patterns of %dx%d read (%s, stride %d), %d masked pixels (%.1f s)\n", N_images, side, side,
+           data_pattern, read_stride, n_m, (now_ms() - t0) / 1e3);
+  } else {
+    n_m = emc_make_mask(side, P.beamstop, P.gap_h, P.gap_cross, 0, mask.data);
+    emc_synth_params sp = {side, P.n_blobs, P.blob_radius, 1.0f, 2.0f, N_images, P.phi_sigma, P.photons, seed};
+    truth = malloc(sizeof(float) * N_model);
+    emc_synth_intensity(&sp, truth);
+    emc_synth_patterns(&sp, truth, xc.data, yc.data, zc.data, mask.data, image_data, NULL, intensities);
+    printf("%d patterns of %dx%d, %d masked pixels (synthetic data in %.1f s)\n", N_images, side, side, n_m,
+           (now_ms() - t0) / 1e3);
+  }
   sp_matrix *image_mats = malloc(sizeof(sp_matrix) * (size_t)N_images);
   sp_matrix **images = malloc(sizeof(sp_matrix *) * (size_t)N_images);
   for (int i = 0; i < N_images; i++) {
@@ -190,12 +266,35 @@ int main(int argc, char **argv) {
     images[i] = &image_mats[i];
   }
 
-  /* model init (emc.c:888-928): uniform random, or the true intensity */
+  /* model init (emc.c:888-928): uniform random (model_input 0), every
+   * pattern inserted at a random orientation (1), a model file (2), or the
+   * true intensity of synthetic data */
   sp_3matrix model = {side, side, side, malloc(sizeof(real) * N_model)};
   sp_3matrix weight = {side, side, side, calloc(N_model, sizeof(real))};
-  if (model_truth) memcpy(model.data, truth, sizeof(float) * N_model);
-  else emc_synth_random_model(side, seed + 12345, model.data);
-  write_raw(out, "model_init.raw", model.data, N_model);
+  if (model_file) {
+    int mx, my, mz;
+    if (emc_spimage_shape(model_file, &mx, &my, &mz) || mx != side || my != side || mz != side ||
+        emc_read_spimage(model_file, model.data, NULL, side, side, side)) {
+      fprintf(stderr, "emc: model file %s: %s\n", model_file,
+              emc_io_error()[0] ? emc_io_error() : "wrong size (emc.c:918-923)");
+      return 1;
+    }
+  } else if (model_orient) {
+    memset(model.data, 0, sizeof(float) * N_model);
+    for (int i = 0; i < N_images; i++) {
+      Quaternion qr;
+      emc_quaternion_from_uniforms(emc_uniform(seed, 0x0121, i, 0), emc_uniform(seed, 0x0121, i, 1),
+                                   emc_uniform(seed, 0x0121, i, 2), qr.q);
+      insert_slice(&model, &weight, images[i], &mask, 1.0f, &qr, &xc, &yc, &zc);
+    }
+    for (size_t k = 0; k < N_model; k++) model.data[k] = weight.data[k] > 0.0f ? model.data[k] / weight.data[k] : 0.0f;
+  } else if (model_truth) {
+    memcpy(model.data, truth, sizeof(float) * N_model);
+  } else {
+    emc_synth_random_model(side, seed + 12345, model.data);
+  }
+  write_volume(out, "model_init.h5", model.data, weight.data, side);
+  write_volume(out, "model_init_weight.h5", weight.data, weight.data, side);
   real *scaling = malloc(sizeof(real) * N_images);
   for (int i = 0; i < N_images; i++) scaling[i] = 1.0f;
   real *respons = dump ? malloc(sizeof(real) * (size_t)N_slices * N_images) : NULL;
@@ -226,6 +325,12 @@ int main(int argc, char **argv) {
   if (prop_iters > 0 && emc_hip_proposal_init(qbuf, N_slices, coarse_n, P.side, topk, 256, seed) != 0)
     die_if_error("proposal setup");
   double t_total = 0;
+  int *best = viewer ? malloc(sizeof(int) * N_images) : NULL;
+  float *avg = viewer ? malloc(sizeof(float) * N_slices) : NULL;
+  float *scale_hist = viewer ? malloc(sizeof(float) * (size_t)N_images * (iterations > 0 ? iterations : 1)) : NULL;
+  const int viewer_full = viewer && (size_t)N_images * N_slices <= (1u << 26);
+  float *pbuf = viewer_full ? malloc(sizeof(float) * (size_t)N_images * N_slices) : NULL;
+  float *tbuf = viewer_full ? malloc(sizeof(float) * (size_t)N_images * N_slices) : NULL;
   for (int iteration = 0; iteration < iterations; iteration++) {
     const double t_it = now_ms();
     real mx = cuda_model_max(d_model, (int)N_model);
@@ -244,6 +349,28 @@ int main(int argc, char **argv) {
     fprintf(likelihood, "%g\n", total_respons);
     fflush(likelihood);
     if (!quiet) printf("likelihood = %g\n", total_respons);
+    if (viewer) {
+      /* viewer/rotations_module.py:43-47,116-117, viewer/slice_module.py:42 */
+      if (emc_hip_respons_summary(d_respons, N_slices, N_images, best, avg)) die_if_error("respons summary");
+      snprintf(path, sizeof path, "average_resp_%04d.h5", iteration);
+      write_h5(out, path, "data", EMC_H5_F32, 1, N_slices, 0, avg);
+      snprintf(path, sizeof path, "%s/best_quaternion_%04d.data", out, iteration);
+      FILE *fq = fopen(path, "w");
+      if (fq) {
+        for (int i = 0; i < N_images; i++) {
+          const float *qq = &qbuf[4 * (size_t)best[i]];
+          fprintf(fq, "%g %g %g %g\n", qq[0], qq[1], qq[2], qq[3]);
+        }
+        fclose(fq);
+      }
+      if (viewer_full) {
+        emc_hip_memcpy_to_host(pbuf, d_respons, sizeof(float) * (size_t)N_images * N_slices);
+        for (int r = 0; r < N_slices; r++)
+          for (int i = 0; i < N_images; i++) tbuf[(size_t)i * N_slices + r] = pbuf[(size_t)r * N_images + i];
+        snprintf(path, sizeof path, "responsabilities_%04d.h5", iteration);
+        write_h5(out, path, "data", EMC_H5_F32, 2, N_images, N_slices, tbuf);
+      }
+    }
     cuda_reset_model(&model, d_model);
     cuda_reset_model(&weight, d_weight);
     cuda_update_scaling(d_images, slices, d_mask, d_respons, d_scaling, N_images, N_slices, N_2d, scaling);
@@ -256,6 +383,16 @@ int main(int argc, char **argv) {
     if (!quiet) printf("scaling error = %g\n", err);
     fprintf(scale, "%g\n", err);
     fflush(scale);
+    if (viewer) {
+      /* viewer/scaling_module.py:19-20, viewer/slice_module.py:77,151 */
+      memcpy(scale_hist + (size_t)iteration * N_images, scaling, sizeof(float) * N_images);
+      write_h5(out, "best_scaling.h5", "scaling", EMC_H5_F32, 2, iteration + 1, N_images, scale_hist);
+      if (viewer_full) {
+        for (int r = 0; r < N_slices; r++) memcpy(tbuf + (size_t)r * N_images, scaling, sizeof(float) * N_images);
+        snprintf(path, sizeof path, "scaling_%04d.h5", iteration);
+        write_h5(out, path, "data", EMC_H5_F32, 2, N_slices, N_images, tbuf);
+      }
+    }
     real overall = cuda_update_slices(d_images, slices, d_mask, d_respons, d_scaling, N_images, N_slices, N_2d,
                                       &model, d_model, d_x, d_y, d_z, d_rotations, weights, d_weight, setup, images);
     t_e = now_ms();
@@ -288,21 +425,29 @@ int main(int argc, char **argv) {
     printf("iteration %d: %.1f ms, %.3g pattern x rotation pairs/s, likelihood %g, scaling error %g\n", iteration,
            dt, (double)N_images * N_slices / (dt / 1e3), total_respons, err);
     if (save_every > 0 && (iteration % save_every == 0 || iteration == iterations - 1)) {
-      /* the host model holds the accumulators before normalisation
-       * (emc_cuda.cu:352 copies it before cuda_normalize_model) */
-      snprintf(path, sizeof path, "model_%04d.raw", iteration);
-      write_raw(out, path, model.data, N_model);
+      /* emc.c:1072-1082: the host model holds the accumulators before
+       * normalisation (emc_cuda.cu:352 copies it before cuda_normalize_model),
+       * the mask is the host weight's, which the iterations do not update */
+      snprintf(path, sizeof path, "model_%04d.h5", iteration);
+      write_volume(out, path, model.data, weight.data, side);
+    }
+    if (viewer) {
+      /* viewer/viewer.py:14-21,96-97 */
+      const int it = iteration;
+      write_h5(out, "state.h5", "iteration", EMC_H5_I32, 0, 0, 0, &it);
     }
     if (respons) {
       snprintf(path, sizeof path, "respons_%04d.raw", iteration);
       write_raw(out, path, respons, (size_t)N_slices * N_images);
     }
   }
-  /* final normalised model */
+  /* final normalised model (and the device weight as its mask) */
   float *final_model = malloc(sizeof(float) * N_model);
+  float *final_weight = malloc(sizeof(float) * N_model);
   emc_hip_memcpy_to_host(final_model, d_model, sizeof(float) * N_model);
-  write_raw(out, "model_final.raw", final_model, N_model);
-  write_raw(out, "truth.raw", truth, N_model);
+  emc_hip_memcpy_to_host(final_weight, d_weight, sizeof(float) * N_model);
+  write_volume(out, "model_final.h5", final_model, final_weight, side);
+  if (truth) write_volume(out, "truth.h5", truth, truth, side);
   printf("%d iterations in %.1f s\n", iterations, t_total / 1e3);
   fclose(scale);
   fclose(likelihood);

@@ -54,6 +54,15 @@ real emc_hip_total_respons_n(const real *d_respons, size_t n);
  * than the library's) race.  Returns 0 on success. */
 int emc_hip_normalize_model_noalloc(real *d_model, const real *d_weight, int model_size);
 
+/* Per-pattern best rotation (argmax_r P, lowest r on ties) into best[N]
+ * and the responsibility of every rotation averaged over the patterns into
+ * average[R] (either may be NULL), from d_respons [R][N]: what the
+ * reference's viewer reads as best_quaternion_%04d.data and
+ * average_resp_%04d.h5 (viewer/slice_module.py:42,
+ * viewer/rotations_module.py:43-47).  No counterpart in the reference's
+ * emc.h.  Returns 0 on success. */
+int emc_hip_respons_summary(const real *d_respons, int N_slices, int N_images, int *best, float *average);
+
 /* Device buffer of `bytes` bytes (the size_t form of cuda_allocate_real). */
 void emc_hip_allocate_bytes(void **d_ptr, size_t bytes);
 /* Page-locked host buffer (no counterpart in the reference): host arrays
@@ -74,7 +83,15 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                "fp16": the pattern x slice and M-step contractions on FP16
  *                 matrix cores with hi/lo operand splitting and FP32
  *                 accumulation, FP64 normalisations.
- *   "insert"     "atomic" (default): float atomics, order not fixed;
+ *   "insert"     "atomic" (default): each workgroup owns an 8^3 brick of
+ *                 the model and adds into LDS every contribution that lands
+ *                 in it, found from the brick's box mapped back onto the
+ *                 detector for each rotation with T_r > 1e-10, then adds
+ *                 the brick to the model with plain stores (no global
+ *                 atomics; LDS float atomics, so the order of the additions
+ *                 is not fixed); per-pixel global atomics where the model
+ *                 side is not a multiple of 8;
+ *                "global_atomic": per-pixel global float atomics;
  *                "deterministic": per-voxel gather in a fixed order,
  *                 bit-reproducible run to run.
  *   "interp"     "nearest" (default, the reference's rule) or "trilinear".
@@ -99,16 +116,18 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *   "expand_rotations"  rotations per workgroup of the tiled expand + pack
  *                 (default "8"; 1 .. 64).
  *   "refine"     "on" (default): after the softmax, every responsibility of
- *                 at least "refine_tau" (default "1e-6") is recomputed from
+ *                 at least "refine_tau" (default "1e-9") is recomputed from
  *                 its likelihood summed in FP64 over the pattern's nonzero
  *                 counts (the matrix cores' FP32 sums over the pixels and
  *                 the FP32 storage of L are off by up to ~5e-3 at 15,000
  *                 pixels, a relative error of ~5e-3 in every P); each
  *                 pattern's refined entries are renormalised among
  *                 themselves, the others keep their values.  The threshold
- *                 reaches down to 1e-6 because voxels whose whole weight
+ *                 reaches down to 1e-9 because voxels whose whole weight
  *                 comes from small responsibilities take their value from
- *                 those alone (M / W); "off".
+ *                 those alone (M / W): at 1e-6 such voxels of a 256^3
+ *                 model still differ from the FP64 iteration by ~2e-4 of
+ *                 the model's maximum, at 1e-9 by ~3e-5; "off".
  *   "gemm_tile"  "256" (default): 256 x 256 tiles of the FP16 likelihood
  *                 GEMM, one workgroup per compute unit; "128": 128 x 128
  *                 tiles, two workgroups per compute unit (same results, bit
@@ -282,6 +301,10 @@ int emc_hip_pack_info(int *out, int n);
  * counterpart in the reference): >= 0 when it ran over the per-rotation lists
  * of nonzero P (option "mstep"), -1 when it ran as the dense contraction. */
 long long emc_hip_mstep_nnz(void);
+
+/* Responsibilities the last refinement recomputed in FP64 (option
+ * "refine"; no counterpart in the reference); 0 when it is off. */
+long long emc_hip_refine_count(void);
 
 #ifdef __cplusplus
 }
