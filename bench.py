@@ -172,6 +172,7 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
         losses.append(loss.value)
         log(f"  dense iteration {it + 1}: {(t1 - t0) * 1e3:.0f} ms, training {(t2 - t1) * 1e3:.0f} ms, "
             f"loss {loss.value:.4f}")
+    model_at_prediction = prob.model()  # the state the network was trained on
     t0 = time.perf_counter()
     if L.emc_hip_proposal_predict(prob.d_images.vp, prob.d_mask.vp, side * side, N) != 0:
         E.check()
@@ -213,7 +214,7 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
     model = prob.model()
     state_model, state_scaling = model.copy(), prob.scaling().copy()
     res = {"dt": dt, "phases": ph, "model": model, "scaling": state_scaling, "per_step": per, "iteration_s": its,
-           "refined": refined,
+           "refined": refined, "model_at_prediction": model_at_prediction,
            "dense_s": dense_s, "train_s": train_s, "predict_s": predict_s, "losses": losses, "gated": gated,
            "total_s": total_s, "predicted_mass_q1_q10_q50": pq, "history": history, "first_audit": first_audit}
     if not one_step:
@@ -242,6 +243,47 @@ def run_learned(E, W, warmup, steps, sigma, likelihood, train_iters, epochs, lr,
     log(f"  learned: {dt * 1e3:.1f} ms/iteration")
     res.update({"one_step_cc": cc(one_learned, one_plain), "one_step_whole_table": one_st["whole_table_iteration"]})
     return res
+
+
+def fresh_capture(E, W, model, fresh, topk, min_mass=0.999):
+    """Patterns the run never saw (same object and generator): the network
+    loaded in the library predicts their top-k coarse cells; their posterior
+    comes from FP16 whole-table E-steps on `model` (the model of the last
+    training iteration, the frame the network learned), the scaling updated
+    once between two E-steps; the captured mass is the posterior each
+    pattern's predicted cells hold.  What a trained network gains where no
+    previous posterior exists to track."""
+    L = E.lib()
+    M = fresh.shape[0]
+    for k, v in (("precision", "fp16"), ("proposal", "off")):
+        E.set_option(k, v)
+    prob = E.EmcProblem(W.cfg.side, fresh, W.mask, W.q, W.w, model)
+    if L.emc_hip_proposal_predict(prob.d_images.vp, prob.d_mask.vp, W.cfg.side ** 2, M) != 0:
+        E.check()
+    cand = np.zeros((M, topk), np.int32)
+    L.emc_hip_proposal_get(E.iptr(cand), None, None, M)
+    prob.get_slices()
+    prob.likelihood(0.4)
+    prob.normalize_respons()
+    prob.update_scaling()
+    prob.likelihood(0.4)
+    prob.normalize_respons()
+    P = prob.respons()
+    prob.free()
+    L.emc_hip_invalidate()
+    nc = L.emc_hip_proposal_cells(None, None)
+    off = np.zeros(nc + 1, np.int32)
+    rot = np.zeros(W.R, np.int32)
+    L.emc_hip_proposal_cells(E.iptr(off), E.iptr(rot))
+    cell = np.empty(W.R, np.int64)
+    for c in range(nc):
+        cell[rot[off[c]:off[c + 1]]] = c
+    cm = np.zeros((nc, M))
+    np.add.at(cm, cell, P.astype(np.float64))
+    mass = np.take_along_axis(cm, cand.T, axis=0).sum(axis=0)
+    return {"patterns": int(M), "captured_min_mass_fraction": float(np.mean(mass >= min_mass)),
+            "captured_mass_q1_q10_q50": np.percentile(mass, [1, 10, 50]).tolist(),
+            "candidate_pairs_fraction": float(topk * np.mean(np.diff(off))) / W.R}
 
 
 def ablation_summary(r, plain_total_s=None):
@@ -341,7 +383,7 @@ def main():
     ap.add_argument("--likelihood", choices=["gaussian", "poisson"], default=None,
                     help="default: the workload's (poisson = count x log-slice contraction)")
     ap.add_argument("--sigma", type=float, default=0.4, help="noise level of the gaussian likelihood")
-    ap.add_argument("--gemm-tile", choices=["256", "128"], default=None,
+    ap.add_argument("--gemm-tile", choices=["256", "128", "256x128"], default=None,
                     help="tile of the FP16 likelihood GEMM (default: the library's)")
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="library option (emc_hip_set_option) set before the run; repeatable")
@@ -349,14 +391,17 @@ def main():
                     help="audited fraction below min mass that sends an iteration to the whole table")
     ap.add_argument("--plain-steps", type=int, default=-1, help="timed FP32 iterations (default = steps)")
     ap.add_argument("--train-iters", type=int, default=5, help="dense iterations whose responsibilities train")
-    ap.add_argument("--epochs", default="0,0,0,2,8",
+    ap.add_argument("--epochs", default="1,1,1,1,2",
                     help="training passes after each dense iteration (comma list; the last repeats)")
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--topk", type=int, default=32)
     ap.add_argument("--coarse-n", type=int, default=4)
-    ap.add_argument("--train-batch", type=int, default=256, help="patterns per network training step")
-    ap.add_argument("--insert", choices=["atomic", "global_atomic", "deterministic"], default="atomic",
-                    help="insertion mode: brick-owned LDS atomics (atomic), plain global float atomics, or the deterministic sorted-scatter gather")
+    ap.add_argument("--train-batch", type=int, default=1024, help="patterns per network training step")
+    ap.add_argument("--insert", choices=["atomic", "global_atomic", "deterministic", "deterministic_brick",
+                                         "sorted_scatter"], default="atomic",
+                    help="insertion mode (the library's option insert)")
+    ap.add_argument("--fresh-patterns", type=int, default=1000,
+                    help="patterns the run never saw, scored with the trained and the untrained network (0: off)")
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip the two comparison runs (untrained network; network without tracking and shared "
                          "cells)")
@@ -407,6 +452,22 @@ def main():
     if learned:
         fast = run_learned(E, W, args.warmup, args.steps, args.sigma, like, args.train_iters, epochs, args.lr,
                            args.topk, args.coarse_n, args.max_fail, batch=args.train_batch)
+        fresh = None
+        if args.fresh_patterns > 0 and not W.cfg.csr:
+            # the same object, mask and generator; with another pattern count
+            # the generator draws other orientations and counts
+            Wf = configs.build(args.config, args.fresh_patterns)
+            assert np.array_equal(Wf.truth, W.truth)
+            fresh = np.ascontiguousarray(Wf.images)
+            del Wf
+            ablation_fresh = {"trained": fresh_capture(E, W, fast["model_at_prediction"], fresh, args.topk)}
+            log(f"fresh patterns, trained network: {ablation_fresh['trained']}")
+            # the same with a freshly initialised (untrained) network
+            if E.lib().emc_hip_proposal_init(E.fptr(W.q), R, args.coarse_n, W.cfg.side, args.topk,
+                                             args.train_batch, 1) != 0:
+                E.check()
+            ablation_fresh["untrained"] = fresh_capture(E, W, fast["model_at_prediction"], fresh, args.topk)
+            log(f"fresh patterns, untrained network: {ablation_fresh['untrained']}")
         if not args.no_ablation:
             # what the trained network adds: the same schedule with the
             # network left untrained, and the trained network without
@@ -525,6 +586,7 @@ def main():
             if fast["first_audit"] else None,
             "model_cc_one_iteration_vs_plain_fp32": fast["one_step_cc"],
             "whole_schedule_speedup_vs_plain_fp32": (plain["total_s"] / fast["total_s"]) if plain else None,
+            "network_on_fresh_patterns": ablation_fresh if fresh is not None else None,
             "ablation": {"full": ablation_summary(fast, plain["total_s"] if plain else None),
                          **{k: ablation_summary(v, plain["total_s"] if plain else None)
                             for k, v in ablation.items()}} if ablation else None,

@@ -69,7 +69,9 @@ bool Cnn::init(int side_, int ncell_, int batch_, uint64_t seed) {
   for (int l = 0; l < kConvs; l++) {
     const long oh = hin[l] / 2;
     const long m = B * oh * oh;
-    if (!col[l].ensure(4 * m * 9 * ic[l]) || !act[l].ensure(4 * m * oc[l])) return false;
+    // layers whose input has a multiple of 8 channels read it through the
+    // GEMM's implicit im2col; only the others keep an im2col buffer
+    if ((!implicit(l) && !col[l].ensure(4 * m * 9 * ic[l])) || !act[l].ensure(4 * m * oc[l])) return false;
     maxact = std::max(maxact, m * oc[l]);
     maxcol = std::max(maxcol, m * 9 * ic[l]);
   }
@@ -98,9 +100,13 @@ void Cnn::forward(const float* images, const int* mask, int N2d, const int* d_id
   const float* in = x.as<float>();
   for (int l = 0; l < kConvs; l++) {
     const int H = hin[l], oh = H / 2, K = 9 * ic[l];
-    im2col(in, B, H, H, ic[l], col[l].as<float>(), st);
     GemmDesc d;
-    d.A = col[l].as<float>(); d.sam = K; d.sak = 1; d.a_len = floats(col[l]);
+    if (implicit(l)) {
+      d.A = in; d.conv_a = 1; d.cH = H; d.cW = H; d.cC = ic[l]; d.a_len = floats(act[l - 1]);
+    } else {
+      im2col(in, B, H, H, ic[l], col[l].as<float>(), st);
+      d.A = col[l].as<float>(); d.sam = K; d.sak = 1; d.a_len = floats(col[l]);
+    }
     d.B = W(l); d.sbn = K; d.sbk = 1; d.b_len = off_b[l] - off_w[l];
     d.C = act[l].as<float>(); d.scm = oc[l]; d.scn = 1; d.c_len = floats(act[l]);
     d.bias_n = Bv(l);
@@ -172,7 +178,11 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     relu_bwd(dz, act[l].as<float>(), M * OC, st);
     GemmDesc d;  // dW[oc][kk] = sum_m dz[m][oc] col[m][kk]
     d.A = dz; d.sam = 1; d.sak = OC; d.a_len = half;
-    d.B = col[l].as<float>(); d.sbn = 1; d.sbk = K; d.b_len = floats(col[l]);
+    if (implicit(l)) {
+      d.B = act[l - 1].as<float>(); d.conv_b = 1; d.cH = H; d.cW = H; d.cC = ic[l]; d.b_len = floats(act[l - 1]);
+    } else {
+      d.B = col[l].as<float>(); d.sbn = 1; d.sbk = K; d.b_len = floats(col[l]);
+    }
     d.C = dW(l); d.scm = K; d.scn = 1; d.c_len = off_b[l] - off_w[l];
     d.M = OC; d.N = K; d.K = (int)M;
     gemm_bf16(d, sc, scratch_floats, st);

@@ -29,6 +29,12 @@ struct GemmDesc {
   int relu = 0;
   int accumulate = 0;
   long a_len = 0, b_len = 0, c_len = 0;
+  // Implicit im2col of a 3x3 / stride-2 / pad-1 convolution over an NHWC
+  // input [batch][cH][cW][cC] (cC a multiple of 8): conv_a reads A(m, k) =
+  // in[b][2oy+ky-1][2ox+kx-1][c] with m = (b*OH + oy)*OW + ox and
+  // k = (ky*3 + kx)*cC + c (sam, sak unused); conv_b reads B(n, k) the same
+  // way with n = (ky*3 + kx)*cC + c and k = m.  Taps outside the input are 0.
+  int conv_a = 0, conv_b = 0, cH = 0, cW = 0, cC = 0;
 };
 
 // False (error set, nothing launched) when the descriptor reaches outside
@@ -80,6 +86,9 @@ struct Cnn {
   float* Bv(int l) const { return params.as<float>() + off_b[l]; }
   float* dW(int l) const { return grads.as<float>() + off_w[l]; }
   float* dB(int l) const { return grads.as<float>() + off_b[l]; }
+  // conv l reads its input through the GEMM's implicit im2col (input
+  // channels a multiple of 8: every layer but the first)
+  bool implicit(int l) const { return l > 0 && ic[l] % 8 == 0; }
 };
 
 }  // namespace emc

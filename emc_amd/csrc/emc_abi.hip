@@ -89,10 +89,13 @@ void Engine::release_all() {
   DBuf* bufs[] = {&d_pix, &d_B, &d_K32, &d_Kt32, &d_K16, &d_K16lo, &d_Kt16, &d_Kt16lo, &d_flags,
                   &d_S32, &d_S16, &d_S16lo, &d_Sscale, &d_A, &d_phi_used, &d_smax, &d_ssum, &d_sacc,
                   &d_phiden, &d_red, &d_invphi, &d_T, &d_Tf, &d_Tphi, &d_ones, &d_Pscale, &d_P16, &d_P16lo,
-                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_tiled, &d_ins_pix, &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt, &d_act, &d_zrange,
-                  &d_cnt_off, &d_cnt_px, &d_cnt_val, &d_ref_ent, &d_ref_count, &d_ref_pat, &d_ref_tot,
-                  &d_ref_slot, &d_ref_tmp};
+                  &d_Sc, &d_wts, &d_wT, &d_csr_off, &d_csr_col, &d_csr_cnt, &d_LSt, &d_ins_tiled, &d_ins_pix,
+                  &d_ins_col, &d_map_off, &d_map_ent, &d_map_wt, &d_map_heavy, &d_act, &d_zrange, &d_cnt_off, &d_cnt_px,
+                  &d_cnt_val, &d_ref_ent, &d_ref_count, &d_ref_pat, &d_ref_tot, &d_ref_slot, &d_ref_tmp,
+                  &d_rot_order};
   for (DBuf* b : bufs) b->release();
+  order_key = {};
+  map_heavy = 0;
   cnt_images = nullptr;
   pack_key = {};
   map_key = {};
@@ -764,7 +767,26 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   }
   g_slices_epoch++;
   launch_set_wT(e.d_wts.as<float>(), e.d_Tf.as<float>(), R, e.d_wT.as<float>(), e.stream);
-  if (e.insert_mode == INSERT_DETERMINISTIC) {
+  // deterministic: the brick form when few rotations are active (its work
+  // follows them), the sorted map otherwise (its work is fixed); the two give
+  // the same bits
+  bool det_brick = e.insert_mode == INSERT_DET_BRICK && insert_brick_fits(S, true);
+  if (e.insert_mode == INSERT_DETERMINISTIC && insert_brick_fits(S, true)) {
+    std::vector<float> Th(R);
+    EMC_CHECK(hipMemcpyAsync(Th.data(), e.d_Tf.p, sizeof(float) * R, hipMemcpyDeviceToHost, e.stream));
+    sync();
+    if (failed()) return 0.0f;
+    size_t n_act = 0;
+    for (int r = 0; r < R; r++) n_act += Th[r] > 1e-10f;
+    det_brick = n_act * 16 <= (size_t)R;
+  }
+  if (det_brick || (e.insert_mode == INSERT_ATOMIC && insert_brick_fits(S, false))) {
+    PhaseTimer pt(PH_INSERT);
+    launch_insert_brick(e.d_Sc.as<float>(), e.n_mp, e.d_kidx.as<int>(), d_mask, e.d_wT.as<float>(), d_rotations, R,
+                        S, x_coordinates, y_coordinates, z_coordinates, d_model, d_weight, e.interp, det_brick,
+                        e.stream);
+  } else if (e.insert_mode == INSERT_DETERMINISTIC || e.insert_mode == INSERT_SORTED ||
+             e.insert_mode == INSERT_DET_BRICK) {
     auto& mk = e.map_key;
     const bool have = mk.valid && mk.rot == d_rotations && mk.x == x_coordinates && mk.y == y_coordinates &&
                       mk.z == z_coordinates && mk.mask == d_mask && mk.R == R && mk.S == S && mk.interp == e.interp &&
@@ -791,10 +813,6 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
                          e.d_map_off.as<uint32_t>(), e.d_map_ent.as<uint32_t>(),
                          e.interp == INTERP_NEAREST ? nullptr : e.d_map_wt.as<float>(), S, e.interp, d_model, d_weight,
                          e.stream);
-  } else if (e.insert_mode == INSERT_ATOMIC && insert_brick_fits(S)) {
-    PhaseTimer pt(PH_INSERT);
-    launch_insert_brick(e.d_Sc.as<float>(), e.n_mp, e.d_kidx.as<int>(), d_mask, e.d_wT.as<float>(), d_rotations, R,
-                        S, x_coordinates, y_coordinates, z_coordinates, d_model, d_weight, e.interp, e.stream);
   } else {
     PhaseTimer pt(PH_INSERT);
     launch_insert_atomic(e.d_Sc.as<float>(), e.n_mp, e.d_pix.as<int>(), e.d_ins_tiled.as<int>(), e.d_wT.as<float>(),
@@ -1021,6 +1039,8 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (is(value, "atomic")) e.insert_mode = INSERT_ATOMIC;
     else if (is(value, "global_atomic")) e.insert_mode = INSERT_GLOBAL_ATOMIC;
     else if (is(value, "deterministic")) e.insert_mode = INSERT_DETERMINISTIC;
+    else if (is(value, "sorted_scatter")) e.insert_mode = INSERT_SORTED;
+    else if (is(value, "deterministic_brick")) e.insert_mode = INSERT_DET_BRICK;
     else return 1;
   } else if (is(key, "interp")) {
     if (is(value, "nearest")) e.interp = INTERP_NEAREST;
@@ -1038,6 +1058,14 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (is(value, "auto")) e.expand_tiled = false;
     else if (is(value, "tiled")) e.expand_tiled = true;
     else return 1;
+  } else if (is(key, "expand_pixels")) {
+    if (is(value, "tiles")) e.expand_rows = false;
+    else if (is(value, "rows")) e.expand_rows = true;
+    else return 1;
+  } else if (is(key, "expand_order")) {
+    if (is(value, "normal")) e.expand_sorted = true;
+    else if (is(value, "table")) e.expand_sorted = false;
+    else return 1;
   } else if (is(key, "expand_rotations")) {
     char* end = nullptr;
     const long v = strtol(value, &end, 10);
@@ -1053,8 +1081,9 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (!end || *end != 0 || !(v > 0.0) || v > 1.0) return 1;
     e.refine_tau = (float)v;
   } else if (is(key, "gemm_tile")) {
-    if (is(value, "256")) e.gemm_small = false;
-    else if (is(value, "128")) e.gemm_small = true;
+    if (is(value, "256")) e.gemm_shape = 0;
+    else if (is(value, "128")) e.gemm_shape = 1;
+    else if (is(value, "256x128")) e.gemm_shape = 2;
     else return 1;
   } else if (is(key, "gemm_tail")) {
     if (is(value, "split")) e.tail_split = true;

@@ -45,7 +45,13 @@ enum Precision { PREC_FP32 = 0, PREC_FP16 = 1 };
 // atomic: brick-owned LDS accumulation (per-pixel global atomics where the
 // model side is not a multiple of the brick); global_atomic: per-pixel
 // global atomics; deterministic: the sorted-scatter gather
-enum InsertMode { INSERT_ATOMIC = 0, INSERT_DETERMINISTIC = 1, INSERT_GLOBAL_ATOMIC = 2 };
+enum InsertMode {
+  INSERT_ATOMIC = 0,
+  INSERT_DETERMINISTIC = 1,
+  INSERT_GLOBAL_ATOMIC = 2,
+  INSERT_SORTED = 3,
+  INSERT_DET_BRICK = 4
+};
 // auto: the nonzero-list M-step when at most 1 in kSparseMstepRatio
 // responsibilities is nonzero, else the dense matrix-core M-step
 enum MstepMode { MSTEP_AUTO = 0, MSTEP_DENSE = 1, MSTEP_SPARSE = 2 };
@@ -82,8 +88,10 @@ struct Engine {
   int mstep_mode = MSTEP_AUTO;  // FP16 form: M-step over the nonzero responsibilities
   bool respons_delta = true;    // proposal iterations rewrite only the changed responsibilities
   bool tail_split = true;       // FP16 likelihood GEMM: last partial round of tiles split along k
-  bool gemm_small = false;      // FP16 likelihood GEMM: 128 x 128 tiles, two workgroups per compute unit
+  int gemm_shape = 0;           // FP16 likelihood GEMM tiles: 0 256 x 256, 1 128 x 128 (two per unit), 2 256 x 128
   bool expand_tiled = false;    // fused expand + pack in rotation blocks x pixel tiles even where the slice fits LDS
+  bool expand_sorted = true;    // expand takes the rotations in the order of their slice normals (k_expand.hip)
+  bool expand_rows = true;      // fused expand + pack: pixels in rows (true) or 8 x 8 tiles per wave
   int expand_rotations = 8;     // rotations per workgroup of that form
   bool refine = true;           // responsibilities >= refine_tau recomputed from FP64 likelihoods (k_refine.hip)
   float refine_tau = 1e-9f;
@@ -146,6 +154,12 @@ struct Engine {
   DBuf d_kidx;          // int [N2d]: packed index of each masked-in pixel (inverse of d_pix), -1 elsewhere
   DBuf d_redo;          // int [R_pad]: rows the provisional-scale pack must redo
   DBuf d_xpart_max, d_xpart_A;  // [R_pad][pixel tiles]: per-tile row maximum and A_r partial of that pack
+  DBuf d_rot_order;     // int [R]: expand's processing order of the rotations, for the table in order_key
+  struct {
+    const float* rot = nullptr;
+    int R = 0;
+    uint64_t stamp = 0;
+  } order_key;
   int sscale_R = -1;    // rotations whose row scales d_Sscale holds (-1: none)
   uint64_t s16_gen = 0;  // bumped whenever d_S16 / d_S16lo are written
   uint64_t lst_gen = 0;  // s16_gen of the slices d_LSt holds (built when the CSR E-step needs it)
@@ -226,7 +240,9 @@ struct Engine {
   DBuf d_map_off;    // uint32 [S^3 + 1]
   DBuf d_map_ent;    // uint32 [entries]: entry id e = r * n_ins + t
   DBuf d_map_wt;     // float [entries]: trilinear weight (trilinear only)
+  DBuf d_map_heavy;  // uint32 [S^3 + 1]: voxels with >= 64 entries (the count last)
   size_t map_entries = 0;
+  uint32_t map_heavy = 0;
 
   uint64_t invalidate_stamp = 1;
 
@@ -361,11 +377,13 @@ void launch_insert_gather(const float* Sc, int n_mp, const int* ins_col, int n_i
                           const float* ent_wt, int S, int interp, float* model, float* weight,
                           hipStream_t st);
 void launch_set_wT(const float* wts, const float* Tf, int R, float* wT, hipStream_t st);
-// brick-owned insertion (k_model.hip); the model side must be a multiple of the brick edge
-bool insert_brick_fits(int S);
+// brick-owned insertion (k_model.hip), LDS atomics or the deterministic
+// sequential order; the model side must be a multiple of the brick edge
+// (8 atomic, 4 deterministic)
+bool insert_brick_fits(int S, bool deterministic);
 void launch_insert_brick(const float* Sc, int n_mp, const int* kidx, const int* mask, const float* wT,
                          const float* rot, int R, int S, const float* x, const float* y, const float* z,
-                         float* model, float* weight, int interp, hipStream_t st);
+                         float* model, float* weight, int interp, bool deterministic, hipStream_t st);
 // Refinement of the significant responsibilities (k_refine.hip).  Sources:
 // [rows][ld] blocks of responsibilities (row -> rotation row_rot[row] or
 // row, column -> pattern col_pat[col] or col, patterns with skip set left

@@ -32,19 +32,65 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // fixed order.
 // ---------------------------------------------------------------------------
 constexpr int GT = 64, GK = 32, GLD = GK + 8;
-enum { kStrided = 0, kKFast = 1, kRowFast = 2 };
+enum { kStrided = 0, kKFast = 1, kRowFast = 2, kConvMK = 3, kConvKM = 4 };
 
 struct OpView {
   const float* p;
   long sr, sk;  // element (row, k) at p[row * sr + k * sk]
   int rows;
+  int H, W, C;  // convolution input (kConvMK / kConvKM)
 };
+
+// Element (m, kk) of the implicit im2col matrix of a 3x3 / stride-2 / pad-1
+// convolution over NHWC [b][H][W][C]: its address, or nullptr for a tap in
+// the padding.
+__device__ __forceinline__ const float* conv_at(const OpView& v, int m, int kk) {
+  const int OH = v.H >> 1, OW = v.W >> 1;
+  const int ox = m % OW, t = m / OW, oy = t % OH, b = t / OH;
+  const int c = kk % v.C, t2 = kk / v.C, kx = t2 % 3, ky = t2 / 3;
+  const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
+  if (y < 0 || y >= v.H || x < 0 || x >= v.W) return nullptr;
+  return v.p + (((size_t)b * v.H + y) * v.W + x) * v.C + c;
+}
+
+// 8 consecutive columns c .. c + 7 of one (m, tap): contiguous, 16-byte aligned
+__device__ __forceinline__ void conv8(const float* q, float (&x)[8]) {
+  if (q) {
+    const float4 a = reinterpret_cast<const float4*>(q)[0], b = reinterpret_cast<const float4*>(q)[1];
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; e++) x[e] = 0.0f;
+  }
+}
 
 // the 8 values thread t moves for rows r0.. of the tile and k0.. of the
 // k-step (bounds: rows < v.rows, k < k_end), zero outside
 template <int MODE>
 __device__ __forceinline__ void load8(const OpView& v, int row0, int k0, int k_end, int t, float (&x)[8]) {
-  if (MODE == kRowFast) {
+  if (MODE == kConvMK) {  // rows m, k = taps x channels: as kKFast
+    const int r = row0 + (t >> 2), k = k0 + (t & 3) * 8;
+    if (r < v.rows && k + 8 <= k_end) {
+      conv8(conv_at(v, r, k), x);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float* q = (r < v.rows && k + e < k_end) ? conv_at(v, r, k + e) : nullptr;
+        x[e] = q ? *q : 0.0f;
+      }
+    }
+  } else if (MODE == kConvKM) {  // rows = taps x channels, k = m: as kRowFast
+    const int k = k0 + (t >> 3), r = row0 + (t & 7) * 8;
+    if (k < k_end && r + 8 <= v.rows) {
+      conv8(conv_at(v, k, r), x);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float* q = (k < k_end && r + e < v.rows) ? conv_at(v, k, r + e) : nullptr;
+        x[e] = q ? *q : 0.0f;
+      }
+    }
+  } else if (MODE == kRowFast) {
     const int k = k0 + (t >> 3), r = row0 + (t & 7) * 8;
     if (k < k_end && r + 8 <= v.rows) {
       const float4* q = reinterpret_cast<const float4*>(v.p + (long)r + (long)k * v.sk);
@@ -81,7 +127,7 @@ __device__ __forceinline__ void load8(const OpView& v, int row0, int k0, int k_e
 // the same 8 values into the LDS tile [row][k] as BF16
 template <int MODE>
 __device__ __forceinline__ void store8(__bf16 (*s)[GLD], int t, bool kf_strided, const float (&x)[8]) {
-  if (MODE == kKFast || (MODE == kStrided && kf_strided)) {
+  if (MODE == kKFast || MODE == kConvMK || (MODE == kStrided && kf_strided)) {
     bf16x8 v;
 #pragma unroll
     for (int e = 0; e < 8; e++) v[e] = (__bf16)x[e];
@@ -99,7 +145,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmDesc d, float* __restrict
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
   const int kz0 = blockIdx.z * ksplit_len, kz1 = min(d.K, kz0 + ksplit_len);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
-  const OpView va{d.A, d.sam, d.sak, d.M}, vb{d.B, d.sbn, d.sbk, d.N};
+  const OpView va{d.A, d.sam, d.sak, d.M, d.cH, d.cW, d.cC}, vb{d.B, d.sbn, d.sbk, d.N, d.cH, d.cW, d.cC};
   const bool a_kf = d.sak == 1, b_kf = d.sbk == 1;
   floatx4 acc[2][2];
 #pragma unroll
@@ -189,14 +235,26 @@ static long last_index(long rows, long cols, long sr, long sc) {
   return rows <= 0 || cols <= 0 ? -1 : (rows - 1) * sr + (cols - 1) * sc;
 }
 
+// Highest element a convolution view reads (the whole NHWC input of
+// `positions` output pixels' batch), or -2 when the view's shape is invalid.
+static long conv_last(const GemmDesc& d, long positions, long taps) {
+  if (d.cH < 2 || d.cW < 2 || d.cH % 2 || d.cW % 2 || d.cC < 8 || d.cC % 8 || taps != 9L * d.cC ||
+      ((uintptr_t)(d.conv_a ? d.A : d.B) & 15))
+    return -2;
+  const long per = (long)(d.cH / 2) * (d.cW / 2);
+  if (positions % per) return -2;
+  return positions / per * d.cH * d.cW * d.cC - 1;
+}
+
 bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st) {
   if (d.M <= 0 || d.N <= 0) return true;
-  if (d.K < 0 || d.sam < 0 || d.sak < 0 || d.sbn < 0 || d.sbk < 0 || d.scm < 0 || d.scn < 0 ||
-      last_index(d.M, d.K, d.sam, d.sak) >= d.a_len || last_index(d.N, d.K, d.sbn, d.sbk) >= d.b_len ||
+  const long la = d.conv_a ? conv_last(d, d.M, d.K) : last_index(d.M, d.K, d.sam, d.sak);
+  const long lb = d.conv_b ? conv_last(d, d.K, d.N) : last_index(d.N, d.K, d.sbn, d.sbk);
+  if (d.K < 0 || d.sam < 0 || d.sak < 0 || d.sbn < 0 || d.sbk < 0 || d.scm < 0 || d.scn < 0 || la < -1 ||
+      lb < -1 || (d.conv_a && d.conv_b) || la >= d.a_len || lb >= d.b_len ||
       last_index(d.M, d.N, d.scm, d.scn) >= d.c_len || !d.A || !d.B || !d.C) {
     set_error("gemm_bf16: descriptor out of bounds (M %d N %d K %d; A %ld/%ld, B %ld/%ld, C %ld/%ld)", d.M, d.N, d.K,
-              last_index(d.M, d.K, d.sam, d.sak), d.a_len, last_index(d.N, d.K, d.sbn, d.sbk), d.b_len,
-              last_index(d.M, d.N, d.scm, d.scn), d.c_len);
+              la, d.a_len, lb, d.b_len, last_index(d.M, d.N, d.scm, d.scn), d.c_len);
     return false;
   }
   const int gm = (d.M + GT - 1) / GT, gn = (d.N + GT - 1) / GT;
@@ -211,7 +269,8 @@ bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStre
               scratch_floats);
     return false;
   }
-  const int am = load_mode(d.A, d.sam, d.sak), bm = load_mode(d.B, d.sbn, d.sbk);
+  const int am = d.conv_a ? kConvMK : load_mode(d.A, d.sam, d.sak);
+  const int bm = d.conv_b ? kConvKM : load_mode(d.B, d.sbn, d.sbk);
   const dim3 grid(gn, gm, nz > 1 ? nz : 1);
   float* part = nz > 1 ? scratch : nullptr;
   const int kl = nz > 1 ? klen : d.K;
@@ -220,6 +279,8 @@ bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStre
   EMC_GEMM_CASE(kStrided, kStrided) EMC_GEMM_CASE(kStrided, kKFast) EMC_GEMM_CASE(kStrided, kRowFast)
   EMC_GEMM_CASE(kKFast, kStrided) EMC_GEMM_CASE(kKFast, kKFast) EMC_GEMM_CASE(kKFast, kRowFast)
   EMC_GEMM_CASE(kRowFast, kStrided) EMC_GEMM_CASE(kRowFast, kKFast) EMC_GEMM_CASE(kRowFast, kRowFast)
+  EMC_GEMM_CASE(kConvMK, kStrided) EMC_GEMM_CASE(kConvMK, kKFast) EMC_GEMM_CASE(kConvMK, kRowFast)
+  EMC_GEMM_CASE(kStrided, kConvKM) EMC_GEMM_CASE(kKFast, kConvKM) EMC_GEMM_CASE(kRowFast, kConvKM)
 #undef EMC_GEMM_CASE
   if (nz > 1) {
     EMC_LAUNCHED();

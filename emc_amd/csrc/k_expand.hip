@@ -7,6 +7,12 @@
 // oracle's float evaluation bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <utility>
+#include <vector>
+
 #include "engine.hpp"
 #include "gemm_common.hpp"
 #include "geom.hpp"
@@ -46,6 +52,83 @@ void launch_brick_model(const float* model, int S, float* out, hipStream_t st) {
   EMC_LAUNCHED();
 }
 
+// ---------------------------------------------------------------------------
+// Processing order of the rotations.  The voxels a slice gathers lie on the
+// rotated detector surface, whose normal is R e_z = (e[2], e[5], e[8]):
+// rotations with nearby normals (whatever their in-plane angle) gather from
+// nearly the same voxels.  The expand kernels therefore take the rotations
+// in the order of a Hilbert curve over the normal's direction (the
+// octahedral map of the hemisphere n_z >= 0; n and -n give the same plane
+// through the centre), so that the workgroups in flight on one XCD share
+// their model bricks in that XCD's L2.  Each rotation's output stays at its
+// own row: the order changes nothing but the schedule.
+// ---------------------------------------------------------------------------
+static uint32_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
+  uint32_t d = 0;
+  for (uint32_t s = n / 2; s > 0; s /= 2) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {  // rotate the quadrant so the curve stays continuous
+      if (rx == 1) {
+        x = n - 1 - x;
+        y = n - 1 - y;
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
+}
+
+static const int* rotation_order(const float* rot, int R, hipStream_t st) {
+  Engine& e = engine();
+  if (!e.expand_sorted || R <= 1) return nullptr;
+  auto& k = e.order_key;
+  if (k.rot == rot && k.R == R && k.stamp == e.invalidate_stamp) return e.d_rot_order.as<int>();
+  std::vector<float> q((size_t)R * 4);
+  EMC_CHECK(hipMemcpyAsync(q.data(), rot, sizeof(float) * 4 * (size_t)R, hipMemcpyDeviceToHost, st));
+  EMC_CHECK(hipStreamSynchronize(st));
+  if (failed()) return nullptr;
+  constexpr uint32_t kGrid = 1024;
+  std::vector<std::pair<uint32_t, int>> key((size_t)R);
+  for (int r = 0; r < R; r++) {
+    const double a = q[4 * (size_t)r], b = q[4 * (size_t)r + 1], c = q[4 * (size_t)r + 2], d = q[4 * (size_t)r + 3];
+    double nx = 2.0 * b * d + 2.0 * a * c, ny = 2.0 * c * d - 2.0 * a * b, nz = a * a - b * b - c * c + d * d;
+    if (nz < 0.0) {
+      nx = -nx;
+      ny = -ny;
+      nz = -nz;
+    }
+    const double l1 = std::fabs(nx) + std::fabs(ny) + nz;
+    const double u = l1 > 0.0 ? nx / l1 : 0.0, v = l1 > 0.0 ? ny / l1 : 0.0;  // |u| + |v| <= 1
+    // the diamond |u| + |v| <= 1 turned into the square [-1, 1]^2
+    const double su = u + v, sv = u - v;
+    const auto cell = [](double w) {
+      const long c = (long)std::floor((w + 1.0) * 0.5 * kGrid);
+      return (uint32_t)std::min<long>(kGrid - 1, std::max<long>(0, c));
+    };
+    key[(size_t)r] = {hilbert_index(kGrid, cell(su), cell(sv)), r};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> order((size_t)R);
+  for (int r = 0; r < R; r++) order[(size_t)r] = key[(size_t)r].second;
+  if (!e.d_rot_order.ensure(sizeof(int) * (size_t)R)) return nullptr;
+  EMC_CHECK(hipMemcpyAsync(e.d_rot_order.p, order.data(), sizeof(int) * (size_t)R, hipMemcpyHostToDevice, st));
+  EMC_CHECK(hipStreamSynchronize(st));
+  if (failed()) return nullptr;
+  k.rot = rot;
+  k.R = R;
+  k.stamp = e.invalidate_stamp;
+  return e.d_rot_order.as<int>();
+}
+
+// Rotation of logical index lg (order == nullptr: the table order; indices
+// at or beyond R, the padding rows, map to themselves).
+__device__ __forceinline__ int rotation_of(const int* __restrict__ order, int lg, int R) {
+  return order && lg < R ? order[lg] : lg;
+}
+
 // One workgroup per rotation; model reads are gathers served by L2 (the
 // pixel loops are unrolled so that several gathers per lane are in flight).  Each
 // XCD takes a contiguous range of rotations (logical_of_block): neighbouring
@@ -57,8 +140,9 @@ __global__ __launch_bounds__(256) void k_expand_nearest(const float* __restrict_
                                                         const float* __restrict__ x,
                                                         const float* __restrict__ y,
                                                         const float* __restrict__ z,
-                                                        float* __restrict__ slices) {
-  const int r = logical_of_block(gridDim.x);
+                                                        float* __restrict__ slices,
+                                                        const int* __restrict__ order) {
+  const int r = rotation_of(order, logical_of_block(gridDim.x), gridDim.x);
   float e[9];
   rot_matrix(&rot[4 * (size_t)r], e);
   const float half = (float)S / 2.0f + 0.5f;
@@ -85,8 +169,9 @@ __global__ __launch_bounds__(256) void k_expand_trilinear(const float* __restric
                                                           const float* __restrict__ x,
                                                           const float* __restrict__ y,
                                                           const float* __restrict__ z,
-                                                          float* __restrict__ slices) {
-  const int r = logical_of_block(gridDim.x);
+                                                          float* __restrict__ slices,
+                                                          const int* __restrict__ order) {
+  const int r = rotation_of(order, logical_of_block(gridDim.x), gridDim.x);
   float e[9];
   rot_matrix(&rot[4 * (size_t)r], e);
   const float half = (float)S / 2.0f + 0.5f;
@@ -111,16 +196,18 @@ __global__ __launch_bounds__(256) void k_expand_trilinear(const float* __restric
 void launch_expand(const float* model, int S, const float* rot, int R, const float* x,
                    const float* y, const float* z, float* slices, int interp, hipStream_t st, bool bricked) {
   if (R <= 0) return;
+  const int* ord = rotation_order(rot, R, st);
+  if (failed()) return;
   if (interp == INTERP_TRILINEAR) {
     if (bricked)
-      hipLaunchKernelGGL(k_expand_trilinear<true>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+      hipLaunchKernelGGL(k_expand_trilinear<true>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices, ord);
     else
-      hipLaunchKernelGGL(k_expand_trilinear<false>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+      hipLaunchKernelGGL(k_expand_trilinear<false>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices, ord);
   } else {
     if (bricked)
-      hipLaunchKernelGGL(k_expand_nearest<true>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+      hipLaunchKernelGGL(k_expand_nearest<true>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices, ord);
     else
-      hipLaunchKernelGGL(k_expand_nearest<false>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices);
+      hipLaunchKernelGGL(k_expand_nearest<false>, dim3(R), dim3(256), 0, st, model, S, rot, x, y, z, slices, ord);
   }
   EMC_LAUNCHED();
 }
@@ -328,9 +415,9 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
                                                          const int* __restrict__ pix, int R, int n_mp,
                                                          _Float16* __restrict__ S16, _Float16* __restrict__ S16lo,
                                                          float* __restrict__ scale, double* __restrict__ A,
-                                                         int poisson) {
+                                                         int poisson, const int* __restrict__ order, int rows) {
   extern __shared__ float sv[];
-  const int r = logical_of_block(gridDim.x);
+  const int r = rotation_of(order, logical_of_block(gridDim.x), R);
   const bool live = r < R;
   const int N2d = S * S;
   if (live) {
@@ -340,7 +427,7 @@ __global__ __launch_bounds__(kPackThreads) void k_expand_pack_f16(const float* _
     float* out = slices + (size_t)r * N2d;
   #pragma unroll 4
   for (int t = threadIdx.x; t < N2d; t += blockDim.x) {
-      const int p = expand_pixel(t, S);
+      const int p = rows ? t : expand_pixel(t, S);
       float ux, uy, uz;
       voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
       float v = 0.0f;
@@ -428,14 +515,15 @@ __global__ __launch_bounds__(256) void k_expand_pack_tile(const float* __restric
                                                           const float* __restrict__ prov_inv,
                                                           _Float16* __restrict__ S16, _Float16* __restrict__ S16lo,
                                                           float* __restrict__ part_max, double* __restrict__ part_A,
-                                                          int poisson, int xr) {
+                                                          int poisson, int xr, const int* __restrict__ order) {
   const int ntx = (S + kXT - 1) / kXT, ntiles = ntx * ntx;
   const int tile = blockIdx.y, tx0 = (tile % ntx) * kXT, ty0 = (tile / ntx) * kXT;
   const int N2d = S * S;
   const float half = (float)S / 2.0f + 0.5f;
   for (int rr = 0; rr < xr; rr++) {
-    const int r = blockIdx.x * xr + rr;
-    if (r >= R_pad) break;  // uniform over the workgroup
+    const int lg = blockIdx.x * xr + rr;
+    if (lg >= R_pad) break;  // uniform over the workgroup
+    const int r = rotation_of(order, lg, R);
     const bool live = r < R;
     _Float16* hrow = S16 + (size_t)r * n_mp;
     _Float16* lrow = S16lo + (size_t)r * n_mp;
@@ -563,6 +651,8 @@ void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, 
   const size_t lds = (size_t)S * S * sizeof(float);
   const dim3 g(R_pad), b(kPackThreads);
   const int po = (int)poisson;
+  const int* ord = rotation_order(rot, R, st);
+  if (failed()) return;
   if (!expand_pack_lds(S, n_mp) || engine().expand_tiled) {
     Engine& e = engine();
     const int ntx = (S + kXT - 1) / kXT, ntiles = ntx * ntx;
@@ -580,16 +670,16 @@ void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, 
     double* pA = e.d_xpart_A.as<double>();
     if (tri && bricked)
       hipLaunchKernelGGL((k_expand_pack_tile<true, true>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr);
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr, ord);
     else if (tri)
       hipLaunchKernelGGL((k_expand_pack_tile<true, false>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr);
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr, ord);
     else if (bricked)
       hipLaunchKernelGGL((k_expand_pack_tile<false, true>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr);
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr, ord);
     else
       hipLaunchKernelGGL((k_expand_pack_tile<false, false>), gt, gb, 0, st, model, S, rot, x, y, z, slices, kidx, R,
-                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr);
+                         R_pad, n_mp, prov, S16, S16lo, pmx, pA, po, xr, ord);
     EMC_LAUNCHED();
     hipLaunchKernelGGL(k_pack_tile_finish, dim3(R_pad), dim3(64), 0, st, pmx, pA, ntiles, R, n_m, n_mp, scale, A,
                        redo, (int)(prov != nullptr), S16, S16lo);
@@ -602,17 +692,17 @@ void launch_expand_pack_f16(const float* model, int S, const float* rot, int R, 
   if (interp == INTERP_TRILINEAR) {
     if (bricked)
       hipLaunchKernelGGL((k_expand_pack_f16<true, true>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
-                         S16, S16lo, scale, A, po);
+                         S16, S16lo, scale, A, po, ord, (int)engine().expand_rows);
     else
       hipLaunchKernelGGL((k_expand_pack_f16<true, false>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
-                         S16, S16lo, scale, A, po);
+                         S16, S16lo, scale, A, po, ord, (int)engine().expand_rows);
   } else {
     if (bricked)
       hipLaunchKernelGGL((k_expand_pack_f16<false, true>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
-                         S16, S16lo, scale, A, po);
+                         S16, S16lo, scale, A, po, ord, (int)engine().expand_rows);
     else
       hipLaunchKernelGGL((k_expand_pack_f16<false, false>), g, b, lds, st, model, S, rot, x, y, z, slices, pix, R, n_mp,
-                         S16, S16lo, scale, A, po);
+                         S16, S16lo, scale, A, po, ord, (int)engine().expand_rows);
   }
   EMC_LAUNCHED();
   engine().sscale_R = R;

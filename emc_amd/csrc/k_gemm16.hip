@@ -37,14 +37,15 @@ namespace emc {
 // than units) would take a whole tile time on a few units while the rest of
 // the device idles.
 // Tile shapes: 256 x 256 (8 waves of 128 x 64, one workgroup per compute
-// unit) or, with option gemm_tile "128", 128 x 128 (4 waves of 64 x 64, 72 KB
-// of LDS, two independent workgroups per compute unit, so one computes
-// while the other waits at its stage barrier).
-template <int TM, int TN, int WGM, int WGN, int MINB, bool KSPLIT, bool PC>
+// unit, three 48 KB stages); with option gemm_tile "128", 128 x 128 (4 waves
+// of 64 x 64, 72 KB of LDS, two independent workgroups per compute unit, so
+// one computes while the other waits at its stage barrier); "256x128":
+// 8 waves of 128 x 32, four 32 KB stages (three stages of loads in flight).
+template <int TM, int TN, int WGM, int WGN, int MINB, int ST, bool KSPLIT, bool PC>
 __global__ __launch_bounds__(64 * WGM * WGN, MINB) void k_estep16(Operands op, int n_mp, int tiles_m, int tiles_n,
                                                                   int n_main, int split, float* __restrict__ part,
                                                                   LikeEpilogue ep) {
-  constexpr int NA = KSPLIT ? 2 : 1, NB = 2, STAGES = KSPLIT ? 2 : 3, NT = 64 * WGM * WGN;
+  constexpr int NA = KSPLIT ? 2 : 1, NB = 2, STAGES = KSPLIT ? 2 : ST, NT = 64 * WGM * WGN;
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
   int tm, tn, logical, range = -1, k0 = 0, k1 = n_mp / BK;
@@ -180,7 +181,7 @@ void launch_convert_p(const MstepArgs& a, _Float16* hi, _Float16* lo, hipStream_
   EMC_LAUNCHED();
 }
 
-template <int TM, int TN, int WGM, int WGN, int MINB>
+template <int TM, int TN, int WGM, int WGN, int MINB, int ST>
 static void estep16_shape(bool ksplit, const Operands& op, int N_pad, int R_pad, int n_mp, const LikeEpilogue& ep,
                           hipStream_t st) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN, NT = 64 * WGM * WGN;
@@ -212,17 +213,17 @@ static void estep16_shape(bool ksplit, const Operands& op, int N_pad, int R_pad,
   const dim3 grid(n_main + (nt - n_main) * split), block(NT);
   if (ksplit) {
     if (pc)
-      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, true, true>), grid, block, 0, st, op, n_mp, tiles_m,
+      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, ST, true, true>), grid, block, 0, st, op, n_mp, tiles_m,
                          tiles_n, n_main, split, part, ep);
     else
-      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, true, false>), grid, block, 0, st, op, n_mp, tiles_m,
+      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, ST, true, false>), grid, block, 0, st, op, n_mp, tiles_m,
                          tiles_n, n_main, split, part, ep);
   } else {
     if (pc)
-      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, false, true>), grid, block, 0, st, op, n_mp, tiles_m,
+      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, ST, false, true>), grid, block, 0, st, op, n_mp, tiles_m,
                          tiles_n, n_main, split, part, ep);
     else
-      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, false, false>), grid, block, 0, st, op, n_mp, tiles_m,
+      hipLaunchKernelGGL((k_estep16<TM, TN, WGM, WGN, MINB, ST, false, false>), grid, block, 0, st, op, n_mp, tiles_m,
                          tiles_n, n_main, split, part, ep);
   }
   EMC_LAUNCHED();
@@ -241,10 +242,13 @@ void launch_estep16(bool ksplit, const void* K, const void* Klo, const void* S, 
   op.b[0] = (const _Float16*)S;
   op.b[1] = (const _Float16*)Slo;
   op.lda = op.ldb = (size_t)n_mp;
-  if (engine().gemm_small)
-    estep16_shape<4, 4, 2, 2, 2>(ksplit, op, N_pad, R_pad, n_mp, ep, st);
+  const int shape = engine().gemm_shape;
+  if (shape == 1)
+    estep16_shape<4, 4, 2, 2, 2, 3>(ksplit, op, N_pad, R_pad, n_mp, ep, st);
+  else if (shape == 2 && R_pad % 128 == 0)
+    estep16_shape<8, 2, 2, 4, 1, 4>(ksplit, op, N_pad, R_pad, n_mp, ep, st);
   else
-    estep16_shape<8, 4, 2, 4, 1>(ksplit, op, N_pad, R_pad, n_mp, ep, st);
+    estep16_shape<8, 4, 2, 4, 1, 3>(ksplit, op, N_pad, R_pad, n_mp, ep, st);
 }
 
 void launch_mstep16(bool ksplit, const void* Phi, const void* Plo, const void* Kt, const void* Ktlo,

@@ -155,123 +155,33 @@ void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* 
 }
 
 // ---------------------------------------------------------------------------
-// Brick-owned insertion (the default "atomic" mode).  One workgroup owns a
-// kIB^3 brick of the model: it accumulates, in LDS, every contribution of
-// every active rotation (T_r > 1e-10) whose voxel lies in the brick, then
-// adds the brick to the model and weight with plain read-modify-writes (no
-// other workgroup touches those voxels, so no global atomics at all).  The
-// pixels that can reach the brick under rotation r are found from the
-// brick's box mapped back to detector coordinates, p = R^T (u - h): the
-// plane test (the box's range of p_z against the detector's z range) drops
-// most rotations for bricks away from the centre, and the box's range of
-// (p_x, p_y) bounds the detector pixels to test; each of those is mapped
-// forward exactly as the reference maps it (rot_matrix, voxel_coord,
-// roundf) and added when its voxel falls in the brick.  LDS adds within a
-// workgroup are float atomics, so the order of the additions is open, as in
-// the reference's insertion (emc_atomic.cu:56-66).
+// Brick-owned insertion.  One workgroup owns an IB^3 brick of the model and a
+// share (group g of `groups`) of the active rotations (T_r > 1e-10): it
+// accumulates in LDS every contribution of its rotations whose voxel lies in
+// the brick, then adds the brick to the model and weight.  The pixels that
+// can reach the brick under rotation r are found from the brick's box mapped
+// back to detector coordinates, p = R^T (u - h): the plane test (the box's
+// range of p_z against the detector's z range) drops most rotations for
+// bricks away from the centre, and the box's range of (p_x, p_y) bounds the
+// detector pixels to test; each is mapped forward exactly as the reference
+// maps it (rot_matrix, voxel_coord, roundf) and added when its voxel falls
+// in the brick.
+//
+// "atomic" (8^3 bricks, 256 threads): LDS float atomics, so the order of the
+// additions is open, as in the reference's insertion (emc_atomic.cu:56-66);
+// with one group per brick the flush is a plain read-modify-write (no other
+// workgroup touches those voxels), with several (small models: too few
+// bricks to fill the chip) a global float atomic per voxel.
+//
+// "deterministic" (4^3 bricks, one wave): the brick starts from the model's
+// current values and takes its rotations in increasing order, each
+// rotation's pixels in the reference's x-major order; within a chunk of 64
+// pixels, lanes whose voxels collide add one at a time in pixel order (each
+// round the lowest pixel index holding a voxel, found with an LDS atomicMin,
+// adds).  Every voxel therefore receives the float additions of the
+// sequential insertion (emc.c:491-539 per rotation) in the same order, and
+// the result is the same bits on every run.
 // ---------------------------------------------------------------------------
-constexpr int kIB = 8;  // brick edge (voxels)
-
-__global__ __launch_bounds__(256) void k_insert_brick(const float* __restrict__ Sc, int n_mp,
-                                                      const int* __restrict__ kidx, const int* __restrict__ mask,
-                                                      const int* __restrict__ act, const int* __restrict__ n_act_p,
-                                                      const float* __restrict__ wT, const float* __restrict__ rot,
-                                                      int S, const float* __restrict__ x, const float* __restrict__ y,
-                                                      const float* __restrict__ z, const float* __restrict__ zrange,
-                                                      float* __restrict__ model, float* __restrict__ weight,
-                                                      int interp) {
-#pragma clang fp contract(off)
-  constexpr int NV = kIB * kIB * kIB;
-  __shared__ float acc_m[NV], acc_w[NV];
-  __shared__ int list[256];
-  __shared__ int nlist;
-  const int t = threadIdx.x;
-  const int nb = S / kIB;
-  const int bx = blockIdx.x % nb, by = (blockIdx.x / nb) % nb, bz = blockIdx.x / (nb * nb);
-  const int x0 = bx * kIB, y0 = by * kIB, z0 = bz * kIB;
-  for (int l = t; l < NV; l += 256) {
-    acc_m[l] = 0.0f;
-    acc_w[l] = 0.0f;
-  }
-  const float half = (float)S / 2.0f + 0.5f;
-  const bool tri = interp != INTERP_NEAREST;
-  // continuous coordinates that land in the brick: round to [x0, x0 + kIB)
-  // (nearest) or have a trilinear corner there; kept 0.05 voxel wide
-  const float lo = tri ? -1.0f : -0.5f, hi = tri ? (float)kIB : (float)kIB - 0.5f;
-  const float hs = 0.5f * (hi - lo) + 0.05f;  // half-size of the box
-  const float cx = (float)x0 + 0.5f * (lo + hi) - half, cy = (float)y0 + 0.5f * (lo + hi) - half,
-              cz = (float)z0 + 0.5f * (lo + hi) - half;  // box centre relative to the origin
-  const float zlo = zrange[0] - 0.05f, zhi = zrange[1] + 0.05f;
-  const float dlo = 0.5f - 0.5f * (float)S - 0.05f, dhi = 0.5f * (float)S - 0.5f + 0.05f;  // detector extent
-  const int n_act = *n_act_p;
-  for (int base = 0; base < n_act; base += 256) {
-    if (t == 0) nlist = 0;
-    __syncthreads();
-    if (base + t < n_act) {
-      const int r = act[base + t];
-      float e[9];
-      rot_matrix(&rot[4 * (size_t)r], e);
-      // p = R^T (u - h): column j of R gives detector axis j
-      const float pz = e[2] * cx + e[5] * cy + e[8] * cz, rz = hs * (fabsf(e[2]) + fabsf(e[5]) + fabsf(e[8]));
-      const float px = e[0] * cx + e[3] * cy + e[6] * cz, rx = hs * (fabsf(e[0]) + fabsf(e[3]) + fabsf(e[6]));
-      const float py = e[1] * cx + e[4] * cy + e[7] * cz, ry = hs * (fabsf(e[1]) + fabsf(e[4]) + fabsf(e[7]));
-      if (pz - rz <= zhi && pz + rz >= zlo && px - rx <= dhi && px + rx >= dlo && py - ry <= dhi && py + ry >= dlo)
-        list[atomicAdd(&nlist, 1)] = r;
-    }
-    __syncthreads();
-    const int n = nlist;
-    for (int j = 0; j < n; j++) {
-      const int r = list[j];
-      float e[9];
-      rot_matrix(&rot[4 * (size_t)r], e);
-      const float px = e[0] * cx + e[3] * cy + e[6] * cz, rx = hs * (fabsf(e[0]) + fabsf(e[3]) + fabsf(e[6]));
-      const float py = e[1] * cx + e[4] * cy + e[7] * cz, ry = hs * (fabsf(e[1]) + fabsf(e[4]) + fabsf(e[7]));
-      // pixel index = detector coordinate + S/2 - 0.5
-      const float off = 0.5f * (float)S - 0.5f;
-      const int ix0 = max(0, (int)ceilf(px - rx + off)), ix1 = min(S - 1, (int)floorf(px + rx + off));
-      const int iy0 = max(0, (int)ceilf(py - ry + off)), iy1 = min(S - 1, (int)floorf(py + ry + off));
-      if (ix1 < ix0 || iy1 < iy0) continue;
-      const int w_ = ix1 - ix0 + 1, npx = w_ * (iy1 - iy0 + 1);
-      const float w = wT[r];
-      for (int k = t; k < npx; k += 256) {
-        const int p = (iy0 + k / w_) * S + ix0 + k % w_;
-        if (mask[p] != 1) continue;
-        float ux, uy, uz;
-        voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
-        const float val = w * Sc[(size_t)r * n_mp + kidx[p]];
-        if (!tri) {
-          const int vx = (int)roundf(ux) - x0, vy = (int)roundf(uy) - y0, vz = (int)roundf(uz) - z0;
-          if ((unsigned)vx < (unsigned)kIB && (unsigned)vy < (unsigned)kIB && (unsigned)vz < (unsigned)kIB) {
-            const int l = (vz * kIB + vy) * kIB + vx;
-            atomicAdd(&acc_m[l], val);
-            atomicAdd(&acc_w[l], w);
-          }
-        } else {
-#pragma unroll
-          for (int c = 0; c < 8; c++) {
-            int gx, gy, gz;
-            float cw;
-            if (!tri_corner_xyz(c, ux, uy, uz, S, gx, gy, gz, cw)) continue;
-            const int vx = gx - x0, vy = gy - y0, vz = gz - z0;
-            if ((unsigned)vx < (unsigned)kIB && (unsigned)vy < (unsigned)kIB && (unsigned)vz < (unsigned)kIB) {
-              const int l = (vz * kIB + vy) * kIB + vx;
-              atomicAdd(&acc_m[l], cw * val);
-              atomicAdd(&acc_w[l], cw * w);
-            }
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int l = t; l < NV; l += 256) {
-    if (acc_w[l] == 0.0f && acc_m[l] == 0.0f) continue;
-    const int vx = l % kIB, vy = (l / kIB) % kIB, vz = l / (kIB * kIB);
-    const size_t v = ((size_t)(z0 + vz) * S + (y0 + vy)) * S + (x0 + vx);
-    model[v] += acc_m[l];
-    weight[v] += acc_w[l];
-  }
-}
 
 // Active rotations (T_r > 1e-10) into act[0 .. *n), in no particular order.
 __global__ __launch_bounds__(256) void k_active_rotations(const float* __restrict__ Tf, int R, int* __restrict__ act,
@@ -284,6 +194,31 @@ __global__ __launch_bounds__(256) void k_active_rotations(const float* __restric
   if (lane == 0 && m) base = atomicAdd(n, __popcll(m));
   base = __shfl(base, 0, 64);
   if (on) act[base + __popcll(m & ((1ull << lane) - 1ull))] = r;
+}
+
+// The same list in increasing order: one workgroup, chunks of 1,024
+// rotations, wave ballots and a prefix over the 16 waves.
+__global__ __launch_bounds__(1024) void k_active_ordered(const float* __restrict__ Tf, int R, int* __restrict__ act,
+                                                         int* __restrict__ n) {
+  __shared__ int wcnt[16];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  int base = 0;
+  for (int c0 = 0; c0 < R; c0 += 1024) {
+    const int r = c0 + t;
+    const bool on = r < R && Tf[r] > 1e-10f;
+    const unsigned long long m = __ballot(on);
+    if (lane == 0) wcnt[wave] = __popcll(m);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < 16; w++) {
+      before += w < wave ? wcnt[w] : 0;
+      total += wcnt[w];
+    }
+    if (on) act[base + before + __popcll(m & ((1ull << lane) - 1ull))] = r;
+    base += total;
+    __syncthreads();
+  }
+  if (t == 0) *n = base;
 }
 
 // zrange = (min, max) of z over the pixels with mask == 1
@@ -303,26 +238,227 @@ __global__ __launch_bounds__(256) void k_z_range(const float* __restrict__ z, co
   }
 }
 
+template <int IB, int NT, bool DET>
+__global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ Sc, int n_mp,
+                                                     const int* __restrict__ kidx, const int* __restrict__ mask,
+                                                     const int* __restrict__ act, const int* __restrict__ n_act_p,
+                                                     int groups, const float* __restrict__ wT,
+                                                     const float* __restrict__ rot, int S,
+                                                     const float* __restrict__ x, const float* __restrict__ y,
+                                                     const float* __restrict__ z, const float* __restrict__ zrange,
+                                                     float* __restrict__ model, float* __restrict__ weight,
+                                                     int interp) {
+#pragma clang fp contract(off)
+  static_assert(!DET || NT == 64, "the deterministic form runs one wave per brick");
+  constexpr int NV = IB * IB * IB;
+  __shared__ float acc_m[NV], acc_w[NV];
+  __shared__ int own[DET ? NV : 1];
+  __shared__ int list[NT];
+  __shared__ int nlist;
+  const int t = threadIdx.x;
+  const int nb = S / IB, nbricks = nb * nb * nb;
+  const int brick = blockIdx.x % nbricks, g = blockIdx.x / nbricks;
+  const int bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
+  const int x0 = bx * IB, y0 = by * IB, z0 = bz * IB;
+  for (int l = t; l < NV; l += NT) {
+    if (DET) {
+      const int vx = l % IB, vy = (l / IB) % IB, vz = l / (IB * IB);
+      const size_t v = ((size_t)(z0 + vz) * S + (y0 + vy)) * S + (x0 + vx);
+      acc_m[l] = model[v];
+      acc_w[l] = weight[v];
+      own[l] = 0x7fffffff;
+    } else {
+      acc_m[l] = 0.0f;
+      acc_w[l] = 0.0f;
+    }
+  }
+  const float half = (float)S / 2.0f + 0.5f;
+  const bool tri = interp != INTERP_NEAREST;
+  // continuous coordinates that land in the brick: round to [x0, x0 + IB)
+  // (nearest) or have a trilinear corner there; kept 0.05 voxel wide
+  const float lo = tri ? -1.0f : -0.5f, hi = tri ? (float)IB : (float)IB - 0.5f;
+  const float hs = 0.5f * (hi - lo) + 0.05f;  // half-size of the box
+  const float cx = (float)x0 + 0.5f * (lo + hi) - half, cy = (float)y0 + 0.5f * (lo + hi) - half,
+              cz = (float)z0 + 0.5f * (lo + hi) - half;  // box centre relative to the origin
+  const float zlo = zrange[0] - 0.05f, zhi = zrange[1] + 0.05f;
+  const float dlo = 0.5f - 0.5f * (float)S - 0.05f, dhi = 0.5f * (float)S - 0.5f + 0.05f;  // detector extent
+  const int n_act = *n_act_p;
+  const int a0 = (int)((long long)n_act * g / groups), a1 = (int)((long long)n_act * (g + 1) / groups);
+  for (int base = a0; base < a1; base += NT) {
+    if (!DET && t == 0) nlist = 0;
+    __syncthreads();
+    bool pass = false;
+    int r = 0;
+    if (base + t < a1) {
+      r = act[base + t];
+      float e[9];
+      rot_matrix(&rot[4 * (size_t)r], e);
+      // p = R^T (u - h): column j of R gives detector axis j
+      const float pz = e[2] * cx + e[5] * cy + e[8] * cz, rz = hs * (fabsf(e[2]) + fabsf(e[5]) + fabsf(e[8]));
+      const float px = e[0] * cx + e[3] * cy + e[6] * cz, rx = hs * (fabsf(e[0]) + fabsf(e[3]) + fabsf(e[6]));
+      const float py = e[1] * cx + e[4] * cy + e[7] * cz, ry = hs * (fabsf(e[1]) + fabsf(e[4]) + fabsf(e[7]));
+      pass = pz - rz <= zhi && pz + rz >= zlo && px - rx <= dhi && px + rx >= dlo && py - ry <= dhi && py + ry >= dlo;
+    }
+    int n;
+    if (DET) {  // one wave: the passing rotations in increasing order
+      const unsigned long long m = __ballot(pass);
+      if (pass) list[__popcll(m & ((1ull << t) - 1ull))] = r;
+      n = __popcll(m);
+    } else {
+      if (pass) list[atomicAdd(&nlist, 1)] = r;
+    }
+    __syncthreads();
+    if (!DET) n = nlist;
+    for (int j = 0; j < n; j++) {
+      const int r = list[j];
+      float e[9];
+      rot_matrix(&rot[4 * (size_t)r], e);
+      const float px = e[0] * cx + e[3] * cy + e[6] * cz, rx = hs * (fabsf(e[0]) + fabsf(e[3]) + fabsf(e[6]));
+      const float py = e[1] * cx + e[4] * cy + e[7] * cz, ry = hs * (fabsf(e[1]) + fabsf(e[4]) + fabsf(e[7]));
+      // pixel index = detector coordinate + S/2 - 0.5
+      const float off = 0.5f * (float)S - 0.5f;
+      const int ix0 = max(0, (int)ceilf(px - rx + off)), ix1 = min(S - 1, (int)floorf(px + rx + off));
+      const int iy0 = max(0, (int)ceilf(py - ry + off)), iy1 = min(S - 1, (int)floorf(py + ry + off));
+      if (ix1 < ix0 || iy1 < iy0) continue;  // uniform over the workgroup
+      const int w_ = ix1 - ix0 + 1, h_ = iy1 - iy0 + 1, npx = w_ * h_;
+      const float w = wT[r];
+      for (int k0 = 0; k0 < npx; k0 += NT) {  // uniform over the workgroup
+        const int k = k0 + t;
+        // this lane's contributions, one slot per trilinear corner (slot 0
+        // for the nearest voxel): voxel in the brick (-1: none), model and
+        // weight terms
+        int lv[8];
+        float vm[8], vw[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          lv[c] = -1;
+          vm[c] = 0.0f;
+          vw[c] = 0.0f;
+        }
+        if (k < npx) {
+          // deterministic: x-major (the reference's pixel order); atomic: rows
+          const int p = DET ? (iy0 + k % h_) * S + ix0 + k / h_ : (iy0 + k / w_) * S + ix0 + k % w_;
+          if (mask[p] == 1) {
+            float ux, uy, uz;
+            voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
+            const float val = w * Sc[(size_t)r * n_mp + kidx[p]];
+            if (!tri) {
+              const int vx = (int)roundf(ux) - x0, vy = (int)roundf(uy) - y0, vz = (int)roundf(uz) - z0;
+              if ((unsigned)vx < (unsigned)IB && (unsigned)vy < (unsigned)IB && (unsigned)vz < (unsigned)IB) {
+                lv[0] = (vz * IB + vy) * IB + vx;
+                vm[0] = val;
+                vw[0] = w;
+              }
+            } else {
+#pragma unroll
+              for (int c = 0; c < 8; c++) {
+                int gx, gy, gz;
+                float cw;
+                if (!tri_corner_xyz(c, ux, uy, uz, S, gx, gy, gz, cw)) continue;
+                const int vx = gx - x0, vy = gy - y0, vz = gz - z0;
+                if ((unsigned)vx < (unsigned)IB && (unsigned)vy < (unsigned)IB && (unsigned)vz < (unsigned)IB) {
+                  lv[c] = (vz * IB + vy) * IB + vx;
+                  vm[c] = cw * val;
+                  vw[c] = cw * w;
+                }
+              }
+            }
+          }
+        }
+        if (!DET) {
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            if (lv[q] >= 0) {
+              atomicAdd(&acc_m[lv[q]], vm[q]);
+              atomicAdd(&acc_w[lv[q]], vw[q]);
+            }
+        } else {
+          // rounds: the lowest pixel index holding a voxel adds to it (a
+          // pixel's corners are distinct voxels)
+          unsigned pend = 0;
+#pragma unroll
+          for (int q = 0; q < 8; q++) pend |= (lv[q] >= 0 ? 1u : 0u) << q;
+          for (;;) {
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+              if ((pend >> q) & 1u) atomicMin(&own[lv[q]], k);
+            __syncthreads();
+            unsigned mine = 0;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+              if (((pend >> q) & 1u) && own[lv[q]] == k) {
+                acc_m[lv[q]] += vm[q];
+                acc_w[lv[q]] += vw[q];
+                mine |= 1u << q;
+              }
+            pend &= ~mine;
+            const int more = __syncthreads_or(pend != 0u);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+              if ((mine >> q) & 1u) own[lv[q]] = 0x7fffffff;
+            __syncthreads();
+            if (!more) break;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int l = t; l < NV; l += NT) {
+    const int vx = l % IB, vy = (l / IB) % IB, vz = l / (IB * IB);
+    const size_t v = ((size_t)(z0 + vz) * S + (y0 + vy)) * S + (x0 + vx);
+    if (DET) {
+      model[v] = acc_m[l];
+      weight[v] = acc_w[l];
+    } else if (acc_w[l] != 0.0f || acc_m[l] != 0.0f) {
+      if (groups == 1) {  // this workgroup owns the voxel
+        model[v] += acc_m[l];
+        weight[v] += acc_w[l];
+      } else {
+        atomicAdd(&model[v], acc_m[l]);
+        atomicAdd(&weight[v], acc_w[l]);
+      }
+    }
+  }
+}
+
+constexpr int kIB = 8;   // atomic form: brick edge (voxels), 256 threads
+constexpr int kIBd = 4;  // deterministic form: brick edge, one wave
+
 void launch_insert_brick(const float* Sc, int n_mp, const int* kidx, const int* mask, const float* wT,
                          const float* rot, int R, int S, const float* x, const float* y, const float* z,
-                         float* model, float* weight, int interp, hipStream_t st) {
+                         float* model, float* weight, int interp, bool deterministic, hipStream_t st) {
   Engine& eng = engine();
   if (!eng.d_act.ensure(sizeof(int) * ((size_t)R + 4)) || !eng.d_zrange.ensure(16)) return;
-  int* n_act = eng.d_act.as<int>() + R;
-  launch_fill(reinterpret_cast<float*>(n_act), 1, 0.0f, st);
-  hipLaunchKernelGGL(k_active_rotations, dim3((R + 255) / 256), dim3(256), 0, st, eng.d_Tf.as<float>(), R,
-                     eng.d_act.as<int>(), n_act);
+  int* act = eng.d_act.as<int>();
+  int* n_act = act + R;
+  if (deterministic) {
+    hipLaunchKernelGGL(k_active_ordered, dim3(1), dim3(1024), 0, st, eng.d_Tf.as<float>(), R, act, n_act);
+  } else {
+    launch_fill(reinterpret_cast<float*>(n_act), 1, 0.0f, st);
+    hipLaunchKernelGGL(k_active_rotations, dim3((R + 255) / 256), dim3(256), 0, st, eng.d_Tf.as<float>(), R, act,
+                       n_act);
+  }
   EMC_LAUNCHED();
   hipLaunchKernelGGL(k_z_range, dim3(1), dim3(256), 0, st, z, mask, S * S, eng.d_zrange.as<float>());
   EMC_LAUNCHED();
-  const int nb = S / kIB;
-  hipLaunchKernelGGL(k_insert_brick, dim3((unsigned)nb * nb * nb), dim3(256), 0, st, Sc, n_mp, kidx, mask,
-                     eng.d_act.as<int>(), n_act, wT, rot, S, x, y, z, eng.d_zrange.as<float>(), model, weight,
-                     interp);
+  if (deterministic) {
+    const int nb = S / kIBd;
+    hipLaunchKernelGGL((k_insert_brick<kIBd, 64, true>), dim3((unsigned)nb * nb * nb), dim3(64), 0, st, Sc, n_mp,
+                       kidx, mask, act, n_act, 1, wT, rot, S, x, y, z, eng.d_zrange.as<float>(), model, weight,
+                       interp);
+  } else {
+    // rotation groups per brick: enough workgroups to fill the chip
+    const int nb = S / kIB, nbricks = nb * nb * nb;
+    const int groups = nbricks >= 4096 ? 1 : std::min(64, (4096 + nbricks - 1) / nbricks);
+    hipLaunchKernelGGL((k_insert_brick<kIB, 256, false>), dim3((unsigned)(nbricks * groups)), dim3(256), 0, st, Sc,
+                       n_mp, kidx, mask, act, n_act, groups, wT, rot, S, x, y, z, eng.d_zrange.as<float>(), model,
+                       weight, interp);
+  }
   EMC_LAUNCHED();
 }
 
-bool insert_brick_fits(int S) { return S % kIB == 0; }
+bool insert_brick_fits(int S, bool deterministic) { return deterministic ? S % kIBd == 0 : S % kIB == 0; }
 
 // ---------------------------------------------------------------------------
 // Deterministic insertion.  The voxel hit by (rotation r, pixel t) depends
@@ -407,6 +543,21 @@ __global__ void k_map_weights(const uint32_t* __restrict__ ent, size_t n, const 
   wt[j] = cw;
 }
 
+// Voxels whose entry list holds at least kHeavy entries, in no particular
+// order (each is summed by one wave; the order of the voxels is immaterial).
+constexpr uint32_t kHeavy = 64;
+__global__ __launch_bounds__(256) void k_map_heavy(const uint32_t* __restrict__ off, uint32_t nvox,
+                                                   uint32_t* __restrict__ heavy, uint32_t* __restrict__ n_heavy) {
+  const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+  const bool h = v < nvox && off[v + 1] - off[v] >= kHeavy;
+  const unsigned long long m = __ballot(h);
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0;
+  if (lane == 0 && m) base = atomicAdd(n_heavy, (uint32_t)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (h) heavy[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
 void launch_build_map(const int* ins_pix, int n_ins, const float* rot, int R, int S, const float* x,
                       const float* y, const float* z, int interp, Engine& eng, hipStream_t st) {
   const size_t per = interp == INTERP_NEAREST ? 1 : 8;
@@ -453,16 +604,31 @@ void launch_build_map(const int* ins_pix, int n_ins, const float* rot, int R, in
       EMC_LAUNCHED();
     }
   }
+  // voxels with at least kHeavy entries, gathered by a wave each
+  if (!eng.d_map_heavy.ensure(sizeof(uint32_t) * ((size_t)nvox + 1))) {
+    keys.release();
+    keys_sorted.release();
+    temp.release();
+    return;
+  }
+  uint32_t* heavy = eng.d_map_heavy.as<uint32_t>();
+  launch_fill(reinterpret_cast<float*>(heavy + nvox), 1, 0.0f, st);
+  hipLaunchKernelGGL(k_map_heavy, dim3((nvox + 255) / 256), dim3(256), 0, st, eng.d_map_off.as<uint32_t>(), nvox,
+                     heavy, heavy + nvox);
+  EMC_LAUNCHED();
+  uint32_t nh = 0;
+  EMC_CHECK(hipMemcpyAsync(&nh, heavy + nvox, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   EMC_CHECK(hipStreamSynchronize(st));
   temp.release();
   keys.release();
   keys_sorted.release();
+  eng.map_heavy = nh;
   eng.map_entries = n;
 }
 
-// One thread per voxel, contributions added sequentially in entry order,
-// starting from the accumulator's current value (the caller may not have
-// reset it).
+// One thread per voxel with fewer than kHeavy entries, contributions added
+// sequentially in entry order, starting from the accumulator's current value
+// (the caller may not have reset it).
 __global__ __launch_bounds__(256) void k_insert_gather(const float* __restrict__ Sc, int n_mp,
                                                        const int* __restrict__ ins_col, int n_ins,
                                                        const float* __restrict__ wT, const float* __restrict__ Tf,
@@ -473,7 +639,7 @@ __global__ __launch_bounds__(256) void k_insert_gather(const float* __restrict__
   const int v = blockIdx.x * 256 + threadIdx.x;
   if (v >= nvox) return;
   const uint32_t b = off[v], en = off[v + 1];
-  if (b == en) return;
+  if (b == en || en - b >= kHeavy) return;
   float m = model[v], wsum = weight[v];
   for (uint32_t j = b; j < en; j++) {
     const uint32_t code = ent[j];
@@ -495,13 +661,79 @@ __global__ __launch_bounds__(256) void k_insert_gather(const float* __restrict__
   weight[v] = wsum;
 }
 
+// One wave per voxel with at least kHeavy entries (near the centre every
+// rotation's slice passes through a voxel, so its list is as long as the
+// table): the wave loads 64 consecutive entries at once, then every lane
+// adds the 64 terms in entry order, read lane by lane -- the same float
+// additions in the same order as the single thread above, without a memory
+// round trip per entry.
+__global__ __launch_bounds__(256) void k_insert_gather_wave(const float* __restrict__ Sc, int n_mp,
+                                                            const int* __restrict__ ins_col, int n_ins,
+                                                            const float* __restrict__ wT,
+                                                            const float* __restrict__ Tf,
+                                                            const uint32_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ ent,
+                                                            const float* __restrict__ ent_wt,
+                                                            const uint32_t* __restrict__ heavy, uint32_t n_heavy,
+                                                            int tri, float* __restrict__ model,
+                                                            float* __restrict__ weight) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const uint32_t h = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (h >= n_heavy) return;  // uniform over the wave
+  const uint32_t v = heavy[h];
+  const uint32_t b = off[v], en = off[v + 1];
+  float m = model[v], wsum = weight[v];
+  for (uint32_t j0 = b; j0 < en; j0 += 64) {
+    const uint32_t j = j0 + lane;
+    int on = 0;
+    float dm = 0.0f, dw = 0.0f;
+    if (j < en) {
+      const uint32_t code = ent[j];
+      const uint32_t eidx = tri ? code >> 3 : code;
+      const int r = (int)(eidx / (uint32_t)n_ins), t = (int)(eidx % (uint32_t)n_ins);
+      if (Tf[r] > 1e-10f) {
+        const float w = wT[r];
+        const float val = w * Sc[(size_t)r * n_mp + ins_col[t]];
+        on = 1;
+        if (tri) {
+          const float cw = ent_wt[j];
+          dm = cw * val;
+          dw = cw * w;
+        } else {
+          dm = val;
+          dw = w;
+        }
+      }
+    }
+    const int cnt = (int)min(64u, en - j0);
+    for (int l = 0; l < cnt; l++) {  // uniform: every lane adds the same terms
+      if (__builtin_amdgcn_readlane(on, l)) {
+        m += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dm), l));
+        wsum += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dw), l));
+      }
+    }
+  }
+  if (lane == 0) {
+    model[v] = m;
+    weight[v] = wsum;
+  }
+}
+
 void launch_insert_gather(const float* Sc, int n_mp, const int* ins_col, int n_ins, const float* wT,
                           const uint32_t* off, const uint32_t* ent, const float* ent_wt, int S, int interp,
                           float* model, float* weight, hipStream_t st) {
   const int nvox = S * S * S;
+  Engine& eng = engine();
   hipLaunchKernelGGL(k_insert_gather, dim3((nvox + 255) / 256), dim3(256), 0, st, Sc, n_mp, ins_col, n_ins, wT,
-                     engine().d_Tf.as<float>(), off, ent, ent_wt, nvox, (int)(interp != INTERP_NEAREST), model, weight);
+                     eng.d_Tf.as<float>(), off, ent, ent_wt, nvox, (int)(interp != INTERP_NEAREST), model, weight);
   EMC_LAUNCHED();
+  if (eng.map_heavy > 0) {
+    hipLaunchKernelGGL(k_insert_gather_wave, dim3((eng.map_heavy + 3) / 4), dim3(256), 0, st, Sc, n_mp, ins_col,
+                       n_ins, wT, eng.d_Tf.as<float>(), off, ent, ent_wt, eng.d_map_heavy.as<uint32_t>(),
+                       (uint32_t)eng.map_heavy, (int)(interp != INTERP_NEAREST), model, weight);
+    EMC_LAUNCHED();
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -24,7 +24,10 @@
  *     state.h5 ("iteration"), best_scaling.h5 ("scaling" [iterations][N]),
  *     average_resp_%04d.h5 ("data" [R]), best_quaternion_%04d.data (one
  *     quaternion per pattern), and responsabilities_%04d.h5 ("data" [N][R])
- *     and scaling_%04d.h5 ("data" [R][N]) while N * R stays below 2^26.
+ *     and scaling_%04d.h5 ("data" [R][N]) while N * R stays below 2^26,
+ *     and rotations.h5 ("rotations" [R][5]: q0 q1 q2 q3 weight, the layout
+ *     viewer/calculate_rotation_table.py:15-21 reads); --rotations-file
+ *     reads a table in that layout instead of sampling one.
  *
  * Differences from the reference, all additive:
  *   - the configuration comes from the command line instead of compile-time
@@ -88,10 +91,15 @@ static void usage(void) {
           "           [--likelihood gaussian|poisson] [--sparse]\n"
           "           [--data PATTERN] [--mask FILE] [--read-stride K] [--side S]\n"
           "           [--model random|truth|orientations] [--model-file FILE] [--viewer-output]\n"
+          "           [--rotations-file FILE]\n"
           "  --data PATTERN: read pattern i from printf(PATTERN, i), e.g.\n"
           "                Data/image_%%04d.h5 (libspimage HDF5 layout), sampled on a\n"
           "                side x side grid with stride K (default 4) about its centre\n"
           "  --save-every K: model_%%04d.h5 every K iterations (default 1; 0: none)\n"
+          "  --rotations-file FILE: the rotation table from FILE's \"rotations\" dataset\n"
+          "                [R][5] (q0 q1 q2 q3 weight, the viewer's rotations file)\n"
+          "                instead of the sampled one; --viewer-output writes\n"
+          "                rotations.h5 in that layout\n"
           "  --proposal T: train the rotation-proposal network on iterations 1..T,\n"
           "                then score only its top-K coarse cells (default: 5 for\n"
           "                --config 3 and 4, else 0 = off)\n"
@@ -136,7 +144,7 @@ int main(int argc, char **argv) {
   int cfg = 0, N_images = -1, iterations = -1, n_rot = -1, full_table = 0, dump = 0, quiet = 0;
   int save_every = 1, model_truth = 0, model_orient = 0, prop_iters = -1, topk = 32, coarse_n = 4, epochs = 2;
   int read_stride = 4, side_arg = 0, viewer = 0; /* emc.c:777 */
-  const char *data_pattern = NULL, *mask_file = NULL, *model_file = NULL;
+  const char *data_pattern = NULL, *mask_file = NULL, *model_file = NULL, *rot_file = NULL;
   float lr = 1e-3f;
   float sigma = 0.4f; /* emc.c:734-735 */
   uint64_t seed = 0;
@@ -162,6 +170,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "--insert")) { if (!v) { usage(); return 2; } insert = v; a++; }
     else if (!strcmp(k, "--interp")) { if (!v) { usage(); return 2; } interp = v; a++; }
     else if (!strcmp(k, "--out")) { if (!v) { usage(); return 2; } out = v; a++; }
+    else if (!strcmp(k, "--rotations-file")) { if (!v) { usage(); return 2; } rot_file = v; a++; }
     else if (!strcmp(k, "--likelihood")) { if (!v) { usage(); return 2; } like_kind = v; a++; }
     else if (!strcmp(k, "--sparse")) sparse = 1;
     else if (!strcmp(k, "--model")) {
@@ -204,18 +213,61 @@ int main(int argc, char **argv) {
   if (!quiet) emc_hip_set_option("verbose", "0");
   mkdir(out, 0755);
 
-  /* rotations (emc.c:753-757) */
-  const int nfull = emc_n_to_samples(n_rot);
-  float *qbuf = malloc(sizeof(float) * 4 * (size_t)nfull);
-  real *weights = malloc(sizeof(real) * (size_t)nfull);
-  const int N_slices = emc_rotation_table(n_rot, full_table ? 0 : 1, qbuf, weights);
+  /* rotations (emc.c:753-757), or a rotations file: dataset "rotations"
+   * [R][5], q0 q1 q2 q3 weight per row (viewer/calculate_rotation_table.py:
+   * 15-21 reads that layout) */
+  float *qbuf = NULL;
+  real *weights = NULL;
+  int N_slices = 0;
+  if (rot_file) {
+    int rank = 0;
+    long long dims[3] = {0, 0, 0};
+    /* the shape first (a zero-element buffer: the call fails after
+     * reporting rank and dims), then the data */
+    (void)emc_h5_read(rot_file, "rotations", EMC_H5_F32, NULL, 0, &rank, dims);
+    if (rank != 2 || dims[1] != 5 || dims[0] <= 0 || dims[0] > (1LL << 26)) {
+      fprintf(stderr, "emc: %s: \"rotations\" must be [R][5] (%s)\n", rot_file, emc_io_error());
+      return 1;
+    }
+    N_slices = (int)dims[0];
+    float *rows = malloc(sizeof(float) * 5 * (size_t)N_slices);
+    if (emc_h5_read(rot_file, "rotations", EMC_H5_F32, rows, 5LL * N_slices, NULL, NULL) != 5LL * N_slices) {
+      fprintf(stderr, "emc: %s: %s\n", rot_file, emc_io_error());
+      return 1;
+    }
+    qbuf = malloc(sizeof(float) * 4 * (size_t)N_slices);
+    weights = malloc(sizeof(real) * (size_t)N_slices);
+    for (int r = 0; r < N_slices; r++) {
+      memcpy(&qbuf[4 * (size_t)r], &rows[5 * (size_t)r], sizeof(float) * 4);
+      weights[r] = rows[5 * (size_t)r + 4];
+    }
+    free(rows);
+  } else {
+    const int nfull = emc_n_to_samples(n_rot);
+    qbuf = malloc(sizeof(float) * 4 * (size_t)nfull);
+    weights = malloc(sizeof(real) * (size_t)nfull);
+    N_slices = emc_rotation_table(n_rot, full_table ? 0 : 1, qbuf, weights);
+  }
   if (N_slices <= 0) { fprintf(stderr, "emc: rotation table failed\n"); return 1; }
   Quaternion **rotations = malloc(sizeof(Quaternion *) * (size_t)N_slices);
   for (int r = 0; r < N_slices; r++) {
     rotations[r] = malloc(sizeof(Quaternion));
     memcpy(rotations[r]->q, &qbuf[4 * (size_t)r], sizeof(float) * 4);
   }
-  printf("%d rotations sampled (n = %d%s)\n", N_slices, n_rot, full_table ? "" : ", antipodal half");
+  if (rot_file)
+    printf("%d rotations from %s\n", N_slices, rot_file);
+  else
+    printf("%d rotations sampled (n = %d%s)\n", N_slices, n_rot, full_table ? "" : ", antipodal half");
+  if (viewer) {
+    /* the table as the viewer's rotations file: [R][5], q0..q3 and weight */
+    float *rows = malloc(sizeof(float) * 5 * (size_t)N_slices);
+    for (int r = 0; r < N_slices; r++) {
+      memcpy(&rows[5 * (size_t)r], &qbuf[4 * (size_t)r], sizeof(float) * 4);
+      rows[5 * (size_t)r + 4] = weights[r];
+    }
+    write_h5(out, "rotations.h5", "rotations", EMC_H5_F32, 2, N_slices, 5, rows);
+    free(rows);
+  }
 
   if (data_pattern && model_truth) {
     fprintf(stderr, "emc: --model truth needs synthetic data\n");

@@ -84,16 +84,31 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 matrix cores with hi/lo operand splitting and FP32
  *                 accumulation, FP64 normalisations.
  *   "insert"     "atomic" (default): each workgroup owns an 8^3 brick of
- *                 the model and adds into LDS every contribution that lands
- *                 in it, found from the brick's box mapped back onto the
- *                 detector for each rotation with T_r > 1e-10, then adds
- *                 the brick to the model with plain stores (no global
- *                 atomics; LDS float atomics, so the order of the additions
- *                 is not fixed); per-pixel global atomics where the model
- *                 side is not a multiple of 8;
+ *                 the model (and a share of the rotations with T_r > 1e-10
+ *                 when there are too few bricks to fill the GPU) and adds
+ *                 into LDS every contribution that lands in it, found from
+ *                 the brick's box mapped back onto the detector, then adds
+ *                 the brick to the model: plain stores when it holds all
+ *                 the rotations, one global atomic per voxel otherwise (LDS
+ *                 float atomics, so the order of the additions is not
+ *                 fixed); per-pixel global atomics where the model side is
+ *                 not a multiple of 8;
  *                "global_atomic": per-pixel global float atomics;
- *                "deterministic": per-voxel gather in a fixed order,
- *                 bit-reproducible run to run.
+ *                "deterministic": every voxel's contributions added in the
+ *                 order of the reference's sequential insertion (rotations
+ *                 ascending, pixels x-major), so bit-reproducible run to run
+ *                 and equal to that insertion: by "deterministic_brick"
+ *                 while at most 1/16 of the rotations have T_r > 1e-10
+ *                 (and the side is a multiple of 4), else by
+ *                 "sorted_scatter" -- the two give the same bits;
+ *                "deterministic_brick": 4^3 bricks, one wave each, taking
+ *                 the active rotations in order; colliding pixels add one
+ *                 at a time in pixel order (work follows the active
+ *                 rotations);
+ *                "sorted_scatter": the same sums from a voxel-sorted map of
+ *                 every (rotation, pixel) entry, built once per rotation
+ *                 table (memory: 4 bytes per entry, 36 trilinear; work
+ *                 fixed by the table).
  *   "interp"     "nearest" (default, the reference's rule) or "trilinear".
  *   "timing"     "1" records per-phase GPU time (adds a synchronisation per
  *                 entry point), "0" off.
@@ -113,6 +128,18 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 slice of at most 64 KB in LDS, larger slices run in blocks
  *                 of 8 rotations x 64 x 64 pixel tiles with provisional row
  *                 scales; "tiled": the latter at every size.
+ *   "expand_pixels"  "rows" (default): in the fused expand + pack each
+ *                 wave takes 64 consecutive pixels of a row (full 256-byte
+ *                 slice writes); "tiles": an 8 x 8 pixel tile (fewer model
+ *                 cache lines per wave; same results, measured 8% slower at
+ *                 128^3).
+ *   "expand_order"  "normal" (default): the expand kernels take the
+ *                 rotations in the order of a Hilbert curve over the
+ *                 direction of their slice normals, so that the slices in
+ *                 flight gather from the same model region; "table": in
+ *                 table order.  Results are the same either way (each
+ *                 rotation's row is its own); the order is computed once
+ *                 per rotation buffer (address and count).
  *   "expand_rotations"  rotations per workgroup of the tiled expand + pack
  *                 (default "8"; 1 .. 64).
  *   "refine"     "on" (default): after the softmax, every responsibility of
@@ -122,16 +149,19 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 the FP32 storage of L are off by up to ~5e-3 at 15,000
  *                 pixels, a relative error of ~5e-3 in every P); each
  *                 pattern's refined entries are renormalised among
- *                 themselves, the others keep their values.  The threshold
+ *                 themselves, the others keep their values.  A pattern with
+ *                 more than 1,024 entries at the threshold refines those
+ *                 >= 1e-6 instead, or those >= 1e-3.  The threshold
  *                 reaches down to 1e-9 because voxels whose whole weight
  *                 comes from small responsibilities take their value from
  *                 those alone (M / W): at 1e-6 such voxels of a 256^3
  *                 model still differ from the FP64 iteration by ~2e-4 of
  *                 the model's maximum, at 1e-9 by ~3e-5; "off".
  *   "gemm_tile"  "256" (default): 256 x 256 tiles of the FP16 likelihood
- *                 GEMM, one workgroup per compute unit; "128": 128 x 128
- *                 tiles, two workgroups per compute unit (same results, bit
- *                 for bit; slower on MI355X).
+ *                 GEMM, one workgroup per compute unit, three LDS stages;
+ *                 "128": 128 x 128 tiles, two workgroups per compute unit;
+ *                 "256x128": 256 x 128 tiles with four LDS stages (same
+ *                 results, bit for bit).
  *   "gemm_tail"  "split" (default): the FP16 likelihood GEMM's tiles that
  *                 do not fill a whole round of the compute units (all of
  *                 them when there are fewer tiles than units) are split along

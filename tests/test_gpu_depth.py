@@ -24,8 +24,9 @@ from a model with the structure EMC puts in it.
 
 At this depth the matrix cores' FP32 sums and the FP32 storage of L are off
 by up to ~5e-3 in L (asserted below for the raw log-likelihoods); the
-responsibilities meet 1e-4 because every entry of at least 1e-6 is
-recomputed from its FP64 likelihood after the softmax (option "refine").
+responsibilities meet 1e-4 because every entry of at least 1e-9 is
+recomputed from its FP64 likelihood after the softmax (option "refine",
+threshold 1e-9).
 """
 import numpy as np
 import pytest

@@ -113,3 +113,16 @@ def test_driver_reads_hdf5_data_and_writes_viewer_outputs(tmp_path):
     assert np.array_equal(final, py)
     assert np.allclose(bq, W.q[best], atol=1e-6)
     assert np.allclose(resp.argmax(axis=1), best)
+    # the rotation table written as the viewer's rotations file and read back
+    # through --rotations-file: the same run, the same bits
+    rot = read_h5(out / "rotations.h5", "rotations")
+    assert rot.shape == (R, 5)
+    assert np.array_equal(rot[:, :4], W.q) and np.array_equal(rot[:, 4], W.w.astype(np.float32))
+    out2 = tmp_path / "run2"
+    r = subprocess.run([EXE, "--data", str(data / "image_%04d.h5"), "--mask", str(tmp_path / "mask.h5"),
+                        "--read-stride", "1", "--side", str(side), "--patterns", str(W.N), "--iterations", "2",
+                        "--rotations-file", str(out / "rotations.h5"), "--insert", "deterministic", "--out",
+                        str(out2), "--quiet"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-500:], r.stderr[-500:])
+    assert r.returncode == 0
+    assert np.array_equal(read_volume(out2 / "model_final.h5", side), final)

@@ -1,14 +1,21 @@
 """Insertion modes through the C-ABI.
 
 "atomic" (the default) is brick-owned: each workgroup owns an 8^3 brick of
-the model and finds, for every active rotation, the detector pixels that can
-reach it from the brick's box mapped back onto the detector.  It must add
-exactly the contributions the per-pixel kernels add (the same voxels, the
-same values, in another order): against "global_atomic" (one float atomic
-per pixel) and "deterministic" (the sorted-scatter gather, the order of the
-reference's sequential insertion, emc.c:491-539), nearest and trilinear,
-dense and sparse posteriors; a model side that is not a multiple of 8 falls
-back to the per-pixel kernel."""
+the model (and a share of the active rotations when there are too few
+bricks to fill the GPU) and finds, for every active rotation, the detector
+pixels that can reach it from the brick's box mapped back onto the
+detector.  It must add exactly the contributions the per-pixel kernels add
+(the same voxels, the same values, in another order): against
+"global_atomic" (one float atomic per pixel) and "deterministic", nearest
+and trilinear, dense and sparse posteriors; a model side that is not a
+multiple of 8 falls back to the per-pixel kernel.
+
+"deterministic_brick" runs 4^3 bricks in the order of the reference's
+sequential insertion (emc.c:491-539: rotations ascending, pixels x-major),
+and "sorted_scatter" sums the same contributions in the same order from a
+voxel-sorted map (a wave per voxel where the list is long): the two give the
+same bits, on every run, and "deterministic" (either, by the number of
+active rotations) gives them too."""
 import numpy as np
 import pytest
 
@@ -94,3 +101,26 @@ def test_side_not_multiple_of_brick_falls_back():
     b = run(P, "atomic", "nearest")
     d = run(P, "deterministic", "nearest")
     assert np.array_equal(b["weight"] != 0, d["weight"] != 0) and rel(b["raw"], d["raw"]) < 1e-5
+
+
+@pytest.mark.parametrize("side,gap_h,gap_cross", [(32, 0, 1), (64, 2, 0), (36, 0, 0), (18, 0, 0)])
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+@pytest.mark.parametrize("likelihood", ["gaussian", "poisson"])
+def test_deterministic_brick_equals_sorted_scatter(side, gap_h, gap_cross, interp, likelihood):
+    """Same bits from the deterministic bricks and the sorted-scatter map
+    (side 18: not a multiple of 4, both take the map), and again on a second
+    run; the atomic bricks (with rotation groups at these sizes) within FP32
+    reordering of them."""
+    P = problems.make(side=side, n_patterns=120, n_rot=4, n_blobs=12, radius=side / 6.0, photons=2000.0,
+                      beamstop=2.0, gap_h=gap_h, gap_cross=gap_cross, seed=side, model="truth")
+    d = run(P, "deterministic_brick", interp, likelihood)
+    m = run(P, "sorted_scatter", interp, likelihood)
+    d2 = run(P, "deterministic_brick", interp, likelihood)
+    dd = run(P, "deterministic", interp, likelihood)
+    a = run(P, "atomic", interp, likelihood)
+    for k in ("raw", "weight", "model"):
+        assert np.array_equal(d[k].view(np.uint32), m[k].view(np.uint32)), k
+        assert np.array_equal(d[k].view(np.uint32), d2[k].view(np.uint32)), k
+        assert np.array_equal(d[k].view(np.uint32), dd[k].view(np.uint32)), k
+    assert np.array_equal(a["weight"] != 0, d["weight"] != 0)
+    assert rel(a["raw"], d["raw"]) < 1e-5 and rel(a["weight"], d["weight"]) < 1e-5
