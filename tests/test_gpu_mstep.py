@@ -103,3 +103,20 @@ def test_sparse_inexact_counts(sharp):
     dm = rel(s["model"], d["model"])
     print(f"inexact counts: sparse vs dense model {dm:.2e}")
     assert s["nnz"] >= 0 and dm < 1e-5
+
+
+def test_sparse_matches_dense_side_80():
+    """80 x 80 patterns (n_mp = 4,864 packed pixels): the list M-step's zero
+    fill for rotations without a nonzero responsibility runs over more than
+    one row of its 2D grid; sparse and dense M-steps agree as at 32^3."""
+    P = problems.make(side=80, n_patterns=64, n_rot=2, n_blobs=20, radius=14.0, photons=3000.0,
+                      phi_sigma=0.2, beamstop=4.0, seed=11, model="truth")
+    d = iterate(P, "dense", "poisson")
+    s = iterate(P, "sparse", "poisson")
+    nz = int((s["P"] != 0).sum())
+    rows_without = int(((s["P"] != 0).sum(axis=1) == 0).sum())
+    dm, dsl = rel(s["model"], d["model"]), rel(s["slices"], d["slices"])
+    print(f"nonzero P {nz} of {P.N * P.R}, rotations without one {rows_without}; model {dm:.2e} slices {dsl:.2e}")
+    assert rows_without > 0 and s["nnz"] == nz
+    assert np.array_equal(s["P"], d["P"]) and np.array_equal(s["scaling"], d["scaling"])
+    assert dm < 1e-5 and dsl < 1e-5

@@ -467,3 +467,40 @@ def test_gap_masks_three_iterations(mask_kind, precision, insert):
         model = prob.model()
         sc = prob.scaling()
     prob.free()
+
+
+def test_expand_schedules_give_the_same_bits(tiny):
+    """The expand's schedule options change which workgroup computes what,
+    never a value: the fused LDS form with pixels in rows or 8 x 8 tiles and
+    rotations in table or slice-normal order give the same slices and
+    likelihoods, as do the rotation-block tiled forms with 4, 8 or 32
+    rotations per block among themselves; between the two forms the slices
+    are the same and the likelihoods differ by the order A_r is summed in."""
+    options("fp16")
+    out = {}
+    variants = [("auto", "8", "rows", "normal"), ("auto", "8", "tiles", "normal"), ("auto", "8", "rows", "table"),
+                ("tiled", "4", "rows", "normal"), ("tiled", "8", "rows", "table"), ("tiled", "32", "rows", "normal")]
+    try:
+        for v in variants:
+            for k, val in zip(("expand_pack", "expand_rotations", "expand_pixels", "expand_order"), v):
+                E.set_option(k, val)
+            prob = E.EmcProblem(tiny.side, tiny.images, tiny.mask, tiny.q, tiny.w, tiny.model)
+            prob.iteration()  # builds the pixel list, so the next expand packs
+            prob.get_slices()
+            L = np.zeros((prob.R, prob.N), np.float32)
+            prob.likelihood(0.4, L)
+            out[v] = (prob.slices(), L)
+            prob.free()
+            E.check()
+            E.lib().emc_hip_invalidate()
+    finally:
+        for k, val in (("expand_pack", "auto"), ("expand_rotations", "8"), ("expand_pixels", "rows"),
+                       ("expand_order", "normal")):
+            E.set_option(k, val)
+    s0, L0 = out[variants[0]]
+    t0 = out[variants[3]][1]
+    for v in variants[1:]:
+        assert np.array_equal(out[v][0], s0), v
+        assert np.array_equal(out[v][1], L0 if v[0] == "auto" else t0), v
+    d = float(np.abs(t0.astype(np.float64) - L0).max() / np.abs(L0).max())
+    assert d < 1e-6, d

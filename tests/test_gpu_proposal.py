@@ -444,6 +444,35 @@ def test_group_blocks_match_per_cell(prob):
     assert np.abs(g1["P"] - ref["P"]).max() < 1e-4 and rel(g1["model"], ref["model"]) < 1e-4
 
 
+def test_group_block_core_beyond_one_rotation_tile(prob):
+    """A group whose core holds more than 256 rotations (the 28 largest
+    cells shared by all 300 patterns, then each pattern's own best cells):
+    the group block runs a second 256-rotation tile, and the iteration equals
+    the per-cell kernel's bit for bit."""
+    P = prob
+    topk = 32
+    nc, off, rot = init_cells(P, topk)
+    sizes = np.diff(off)
+    common = np.sort(np.argsort(-sizes, kind="stable")[:28]).astype(np.int32)
+    core_rot = int(sizes[common].sum())
+    assert core_rot > 256, core_rot
+    best, _ = oracle_candidates(P, off, rot, topk + 28)
+    cand = np.zeros((P.N, topk), np.int32)
+    for i in range(P.N):
+        own = [c for c in best[i] if c not in common][:topk - 28]
+        cand[i] = np.concatenate([common, own])
+    out = {}
+    for gmin in ("0", "16"):
+        opts(proposal="on", audit="0.3", group_min=gmin)
+        assert E.lib().emc_hip_proposal_set_candidates(E.iptr(cand), P.N, topk) == 0
+        out[gmin] = run(P)[0]
+    g0, g1 = out["0"], out["16"]
+    print(f"core rotations {core_rot}; group-block pairs {g1['stats'][17]:.0f}")
+    assert g0["stats"][17] == 0 and g1["stats"][17] >= P.N * core_rot
+    for k in ("P", "scaling", "model"):
+        assert np.array_equal(g0[k], g1[k]), k
+
+
 def test_delta_fill_matches_full(prob):
     """Four proposal iterations (whole-table audit first, then 30%, shared
     cells, re-centring, M-step lists from the posterior's sources): with
