@@ -475,8 +475,10 @@ def test_expand_schedules_give_the_same_bits(tiny):
     rotations in table or slice-normal order give the same slices and
     likelihoods, as do the rotation-block tiled forms with 4, 8 or 32
     rotations per block among themselves; between the two forms the slices
-    are the same and the likelihoods differ by the order A_r is summed in."""
-    options("fp16")
+    are the same and the likelihoods differ by the order A_r is summed in.
+    (Deterministic insertion, so that the model after the first iteration is
+    the same bits in every variant.)"""
+    options("fp16", "deterministic")
     out = {}
     variants = [("auto", "8", "rows", "normal"), ("auto", "8", "tiles", "normal"), ("auto", "8", "rows", "table"),
                 ("tiled", "4", "rows", "normal"), ("tiled", "8", "rows", "table"), ("tiled", "32", "rows", "normal")]
@@ -497,6 +499,7 @@ def test_expand_schedules_give_the_same_bits(tiny):
         for k, val in (("expand_pack", "auto"), ("expand_rotations", "8"), ("expand_pixels", "rows"),
                        ("expand_order", "normal")):
             E.set_option(k, val)
+        options("fp32", "atomic")
     s0, L0 = out[variants[0]]
     t0 = out[variants[3]][1]
     for v in variants[1:]:
