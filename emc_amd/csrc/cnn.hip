@@ -64,7 +64,7 @@ bool Cnn::init(int side_, int ncell_, int batch_, uint64_t seed) {
   EMC_CHECK(hipMemset(grads.p, 0, nparam * 4));
   // activations for one batch
   const long B = batch;
-  long maxact = 0, maxcol = 0;
+  long maxact = 0, maxw = 0;
   if (!x.ensure(4 * B * side * side)) return false;
   for (int l = 0; l < kConvs; l++) {
     const long oh = hin[l] / 2;
@@ -73,11 +73,11 @@ bool Cnn::init(int side_, int ncell_, int batch_, uint64_t seed) {
     // GEMM's implicit im2col; only the others keep an im2col buffer
     if ((!implicit(l) && !col[l].ensure(4 * m * 9 * ic[l])) || !act[l].ensure(4 * m * oc[l])) return false;
     maxact = std::max(maxact, m * oc[l]);
-    maxcol = std::max(maxcol, m * 9 * ic[l]);
+    maxw = std::max(maxw, 9L * oc[l] * ic[l]);
   }
   scratch_floats = std::max<size_t>((size_t)16 << 20, (size_t)4 * hidden * feat);
   if (!h.ensure(4 * B * hidden) || !logits.ensure(4 * (long)ncell * B) || !dact.ensure(4 * 2 * maxact) ||
-      !dcol.ensure(4 * maxcol) || !dh.ensure(4 * B * hidden) || !dlogits.ensure(4 * (long)ncell * B) ||
+      !wt.ensure(4 * maxw) || !dh.ensure(4 * B * hidden) || !dlogits.ensure(4 * (long)ncell * B) ||
       !scratch.ensure(4 * scratch_floats) || !idx.ensure(4 * B) || !loss.ensure(8) ||
       !red.ensure(8 * kRedDoubles))
     return false;
@@ -85,7 +85,7 @@ bool Cnn::init(int side_, int ncell_, int batch_, uint64_t seed) {
 }
 
 void Cnn::release() {
-  DBuf* bufs[] = {&params, &grads, &adam_m, &adam_v, &x, &h, &logits, &dact, &dcol, &dh, &dlogits, &scratch, &idx, &loss,
+  DBuf* bufs[] = {&params, &grads, &adam_m, &adam_v, &x, &h, &logits, &dact, &wt, &dh, &dlogits, &scratch, &idx, &loss,
                   &red};
   for (DBuf* b : bufs) b->release();
   for (int l = 0; l < kConvs; l++) {
@@ -143,20 +143,20 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     gemm_bf16(d, sc, scratch_floats, st);
     reduce_rows(dl, B, ncell, 1, B, dB(5), red.as<double>(), kRedDoubles, st);
   }
-  {  // dh[b][k] = sum_c dlog[c][b] W5[c][k], then ReLU'
+  {  // dh[b][k] = sum_c dlog[c][b] W5[c][k], gated by ReLU' (h > 0)
     GemmDesc d;
     d.A = dl; d.sam = 1; d.sak = B; d.a_len = floats(dlogits);
     d.B = W(5); d.sbn = 1; d.sbk = hidden; d.b_len = off_b[5] - off_w[5];
     d.C = dh.as<float>(); d.scm = hidden; d.scn = 1; d.c_len = floats(dh);
+    d.gate = h.as<float>();
     d.M = B; d.N = hidden; d.K = ncell;
     gemm_bf16(d, sc, scratch_floats, st);
-    relu_bwd(dh.as<float>(), h.as<float>(), (long)B * hidden, st);
   }
   float* da = dact.as<float>();
   const long half = dact.bytes / 8;  // two ping-pong halves
   float* dbufs[2] = {da, da + half};
   int cur = 0;
-  {  // dW4[j][k] = sum_b dh[b][j] a3[b][k]; da3[b][k] = sum_j dh[b][j] W4[j][k]
+  {  // dW4[j][k] = sum_b dh[b][j] a3[b][k]; da3[b][k] = sum_j dh[b][j] W4[j][k] gated by a3 > 0
     GemmDesc d;
     d.A = dh.as<float>(); d.sam = 1; d.sak = hidden; d.a_len = floats(dh);
     d.B = act[kConvs - 1].as<float>(); d.sbn = 1; d.sbk = feat; d.b_len = floats(act[kConvs - 1]);
@@ -168,14 +168,14 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     e.A = dh.as<float>(); e.sam = hidden; e.sak = 1; e.a_len = floats(dh);
     e.B = W(4); e.sbn = 1; e.sbk = feat; e.b_len = off_b[4] - off_w[4];
     e.C = dbufs[cur]; e.scm = feat; e.scn = 1; e.c_len = half;
+    e.gate = act[kConvs - 1].as<float>();
     e.M = B; e.N = feat; e.K = hidden;
     gemm_bf16(e, sc, scratch_floats, st);
   }
   for (int l = kConvs - 1; l >= 0; l--) {
     const int H = hin[l], oh = H / 2, K = 9 * ic[l], OC = oc[l];
     const long M = (long)B * oh * oh;
-    float* dz = dbufs[cur];
-    relu_bwd(dz, act[l].as<float>(), M * OC, st);
+    float* dz = dbufs[cur];  // already gated by act[l] > 0
     GemmDesc d;  // dW[oc][kk] = sum_m dz[m][oc] col[m][kk]
     d.A = dz; d.sam = 1; d.sak = OC; d.a_len = half;
     if (implicit(l)) {
@@ -188,13 +188,18 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     gemm_bf16(d, sc, scratch_floats, st);
     reduce_rows(dz, M, OC, OC, 1, dB(l), red.as<double>(), kRedDoubles, st);
     if (l > 0) {
-      GemmDesc e;  // dcol[m][kk] = sum_oc dz[m][oc] W[oc][kk]
-      e.A = dz; e.sam = OC; e.sak = 1; e.a_len = half;
-      e.B = W(l); e.sbn = 1; e.sbk = K; e.b_len = off_b[l] - off_w[l];
-      e.C = dcol.as<float>(); e.scm = K; e.scn = 1; e.c_len = floats(dcol);
-      e.M = (int)M; e.N = K; e.K = OC;
+      // input gradient as the transposed convolution:
+      // dx[b][y][x][c] = sum over taps and oc of dz at the output pixel the
+      // tap covers x W[oc][tap][c], gated by act[l - 1] > 0 (the previous
+      // layer's ReLU')
+      conv_weights_t(W(l), OC, ic[l], wt.as<float>(), st);
+      GemmDesc e;
+      e.A = dz; e.conv_t = 1; e.cH = H; e.cW = H; e.cC = OC; e.a_len = half;
+      e.B = wt.as<float>(); e.sbn = 1; e.sbk = ic[l]; e.b_len = floats(wt);
+      e.C = dbufs[cur ^ 1]; e.scm = ic[l]; e.scn = 1; e.c_len = half;
+      e.gate = act[l - 1].as<float>();
+      e.M = B * H * H; e.N = ic[l]; e.K = 9 * OC;
       gemm_bf16(e, sc, scratch_floats, st);
-      col2im(dcol.as<float>(), B, H, H, ic[l], dbufs[cur ^ 1], st);
       cur ^= 1;
     }
   }

@@ -34,7 +34,15 @@ struct GemmDesc {
   // in[b][2oy+ky-1][2ox+kx-1][c] with m = (b*OH + oy)*OW + ox and
   // k = (ky*3 + kx)*cC + c (sam, sak unused); conv_b reads B(n, k) the same
   // way with n = (ky*3 + kx)*cC + c and k = m.  Taps outside the input are 0.
-  int conv_a = 0, conv_b = 0, cH = 0, cW = 0, cC = 0;
+  // conv_t: the transposed convolution's operand (input gradients): A(m, k)
+  // with m = (b*cH + y)*cW + x a pixel of the convolution's input and
+  // k = (ky*3 + kx)*cC + oc reads the output gradient dz[b][oy][ox][oc]
+  // ([batch][cH/2][cW/2][cC]) at 2oy = y + 1 - ky, 2ox = x + 1 - kx, 0 where
+  // those are odd or outside.
+  int conv_a = 0, conv_b = 0, conv_t = 0, cH = 0, cW = 0, cC = 0;
+  // gate (optional, C's layout): outputs where gate <= 0 are 0 (the ReLU's
+  // backward of the activation gate holds)
+  const float* gate = nullptr;
 };
 
 // False (error set, nothing launched) when the descriptor reaches outside
@@ -42,8 +50,9 @@ struct GemmDesc {
 bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st);
 void cnn_input(const float* images, const int* mask, int N2d, const int* idx, int B, float* x, hipStream_t st);
 void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t st);
-void col2im(const float* col, int B, int H, int W, int C, float* in, hipStream_t st);
-void relu_bwd(float* g, const float* a, long n, hipStream_t st);
+// wt[(t*OC + o)*IC + c] = w[(o*9 + t)*IC + c]: conv weights [OC][3][3][IC]
+// as [3][3][OC][IC], the B operand of the transposed convolution
+void conv_weights_t(const float* w, int OC, int IC, float* wt, hipStream_t st);
 // out[j] = sum over rows i of G[i*si + j*sj]; scratch holds the per-split
 // partial sums (cols * splits doubles)
 void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, double* scratch,
@@ -73,7 +82,7 @@ struct Cnn {
   DBuf params, grads, adam_m, adam_v;
   // activations for one batch
   DBuf x, col[kConvs], act[kConvs], h, logits;
-  DBuf dact, dcol, dh, dlogits, scratch, idx, loss, red;
+  DBuf dact, dh, dlogits, scratch, idx, loss, red, wt;
   size_t scratch_floats = 0;
 
   bool init(int side, int ncell, int batch, uint64_t seed);

@@ -3,9 +3,12 @@
 // Network: x = log1p(counts) on the S x S detector (unmasked pixels 0);
 // four 3x3 / stride-2 / pad-1 convolutions with ReLU (channels 16, 32, 64,
 // 64), flatten, dense 256 + ReLU, dense C (coarse rotation cells), softmax.
-// Activations are NHWC FP32; convolutions run as im2col + GEMM; every GEMM
-// rounds its operands to BF16 and accumulates in FP32 on
-// v_mfma_f32_16x16x32_bf16 (BF16 compute, FP32 master weights).
+// Activations are NHWC FP32; convolutions run as GEMMs that gather their
+// im2col operand on the fly (the first layer, one channel, from an explicit
+// im2col buffer), input gradients as the transposed convolution's GEMM with
+// the ReLU' gate in its epilogue; every GEMM rounds its operands to BF16 and
+// accumulates in FP32 on v_mfma_f32_16x16x32_bf16 (BF16 compute, FP32
+// master weights).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,7 +35,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // fixed order.
 // ---------------------------------------------------------------------------
 constexpr int GT = 64, GK = 32, GLD = GK + 8;
-enum { kStrided = 0, kKFast = 1, kRowFast = 2, kConvMK = 3, kConvKM = 4 };
+enum { kStrided = 0, kKFast = 1, kRowFast = 2, kConvMK = 3, kConvKM = 4, kConvT = 5 };
 
 struct OpView {
   const float* p;
@@ -53,6 +56,21 @@ __device__ __forceinline__ const float* conv_at(const OpView& v, int m, int kk) 
   return v.p + (((size_t)b * v.H + y) * v.W + x) * v.C + c;
 }
 
+// Element (m, k) of the transposed convolution's operand: m a pixel
+// (b, y, x) of the convolution's H x W input, k = tap * C + oc; the output
+// gradient [b][H/2][W/2][C] at the output pixel whose tap (ky, kx) covers
+// (y, x), or nullptr where none does.
+__device__ __forceinline__ const float* convt_at(const OpView& v, int m, int k) {
+  const int OH = v.H >> 1, OW = v.W >> 1;
+  const int x = m % v.W, t = m / v.W, y = t % v.H, b = t / v.H;
+  const int oc = k % v.C, tap = k / v.C, kx = tap % 3, ky = tap / 3;
+  const int ty = y + 1 - ky, tx = x + 1 - kx;
+  if (ty < 0 || tx < 0 || (ty & 1) || (tx & 1)) return nullptr;
+  const int oy = ty >> 1, ox = tx >> 1;
+  if (oy >= OH || ox >= OW) return nullptr;
+  return v.p + (((size_t)b * OH + oy) * OW + ox) * v.C + oc;
+}
+
 // 8 consecutive columns c .. c + 7 of one (m, tap): contiguous, 16-byte aligned
 __device__ __forceinline__ void conv8(const float* q, float (&x)[8]) {
   if (q) {
@@ -68,7 +86,18 @@ __device__ __forceinline__ void conv8(const float* q, float (&x)[8]) {
 // k-step (bounds: rows < v.rows, k < k_end), zero outside
 template <int MODE>
 __device__ __forceinline__ void load8(const OpView& v, int row0, int k0, int k_end, int t, float (&x)[8]) {
-  if (MODE == kConvMK) {  // rows m, k = taps x channels: as kKFast
+  if (MODE == kConvT) {  // rows = input pixels, k = taps x output channels: as kKFast
+    const int r = row0 + (t >> 2), k = k0 + (t & 3) * 8;
+    if (r < v.rows && k + 8 <= k_end) {
+      conv8(convt_at(v, r, k), x);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float* q = (r < v.rows && k + e < k_end) ? convt_at(v, r, k + e) : nullptr;
+        x[e] = q ? *q : 0.0f;
+      }
+    }
+  } else if (MODE == kConvMK) {  // rows m, k = taps x channels: as kKFast
     const int r = row0 + (t >> 2), k = k0 + (t & 3) * 8;
     if (r < v.rows && k + 8 <= k_end) {
       conv8(conv_at(v, r, k), x);
@@ -127,7 +156,7 @@ __device__ __forceinline__ void load8(const OpView& v, int row0, int k0, int k_e
 // the same 8 values into the LDS tile [row][k] as BF16
 template <int MODE>
 __device__ __forceinline__ void store8(__bf16 (*s)[GLD], int t, bool kf_strided, const float (&x)[8]) {
-  if (MODE == kKFast || MODE == kConvMK || (MODE == kStrided && kf_strided)) {
+  if (MODE == kKFast || MODE == kConvMK || MODE == kConvT || (MODE == kStrided && kf_strided)) {
     bf16x8 v;
 #pragma unroll
     for (int e = 0; e < 8; e++) v[e] = (__bf16)x[e];
@@ -200,6 +229,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmDesc d, float* __restrict
           if (d.bias_n) v += d.bias_n[n];
           if (d.bias_m) v += d.bias_m[m];
           if (d.relu) v = fmaxf(v, 0.0f);
+          if (d.gate && !(d.gate[(long)m * d.scm + (long)n * d.scn] > 0.0f)) v = 0.0f;
           float* c = d.C + (long)m * d.scm + (long)n * d.scn;
           *c = d.accumulate ? *c + v : v;
         }
@@ -215,6 +245,7 @@ __global__ void k_gemm_finish(GemmDesc d, const float* __restrict__ partial, int
   if (d.bias_n) v += d.bias_n[n];
   if (d.bias_m) v += d.bias_m[m];
   if (d.relu) v = fmaxf(v, 0.0f);
+  if (d.gate && !(d.gate[(long)m * d.scm + (long)n * d.scn] > 0.0f)) v = 0.0f;
   float* c = d.C + (long)m * d.scm + (long)n * d.scn;
   *c = d.accumulate ? *c + v : v;
 }
@@ -246,12 +277,24 @@ static long conv_last(const GemmDesc& d, long positions, long taps) {
   return positions / per * d.cH * d.cW * d.cC - 1;
 }
 
+// Highest element the transposed-convolution view reads (the output
+// gradient of the batch whose input pixels are the M rows), or -2 when the
+// shape is invalid.
+static long convt_last(const GemmDesc& d) {
+  if (d.cH < 2 || d.cW < 2 || d.cH % 2 || d.cW % 2 || d.cC < 8 || d.cC % 8 || d.K != 9 * d.cC ||
+      ((uintptr_t)d.A & 15))
+    return -2;
+  const long per = (long)d.cH * d.cW;
+  if (d.M % per) return -2;
+  return d.M / per * (d.cH / 2) * (d.cW / 2) * d.cC - 1;
+}
+
 bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st) {
   if (d.M <= 0 || d.N <= 0) return true;
-  const long la = d.conv_a ? conv_last(d, d.M, d.K) : last_index(d.M, d.K, d.sam, d.sak);
+  const long la = d.conv_t ? convt_last(d) : d.conv_a ? conv_last(d, d.M, d.K) : last_index(d.M, d.K, d.sam, d.sak);
   const long lb = d.conv_b ? conv_last(d, d.K, d.N) : last_index(d.N, d.K, d.sbn, d.sbk);
   if (d.K < 0 || d.sam < 0 || d.sak < 0 || d.sbn < 0 || d.sbk < 0 || d.scm < 0 || d.scn < 0 || la < -1 ||
-      lb < -1 || (d.conv_a && d.conv_b) || la >= d.a_len || lb >= d.b_len ||
+      lb < -1 || (d.conv_a + d.conv_b + d.conv_t > 1) || la >= d.a_len || lb >= d.b_len ||
       last_index(d.M, d.N, d.scm, d.scn) >= d.c_len || !d.A || !d.B || !d.C) {
     set_error("gemm_bf16: descriptor out of bounds (M %d N %d K %d; A %ld/%ld, B %ld/%ld, C %ld/%ld)", d.M, d.N, d.K,
               la, d.a_len, lb, d.b_len, last_index(d.M, d.N, d.scm, d.scn), d.c_len);
@@ -269,7 +312,7 @@ bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStre
               scratch_floats);
     return false;
   }
-  const int am = d.conv_a ? kConvMK : load_mode(d.A, d.sam, d.sak);
+  const int am = d.conv_t ? kConvT : d.conv_a ? kConvMK : load_mode(d.A, d.sam, d.sak);
   const int bm = d.conv_b ? kConvKM : load_mode(d.B, d.sbn, d.sbk);
   const dim3 grid(gn, gm, nz > 1 ? nz : 1);
   float* part = nz > 1 ? scratch : nullptr;
@@ -281,6 +324,7 @@ bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStre
   EMC_GEMM_CASE(kRowFast, kStrided) EMC_GEMM_CASE(kRowFast, kKFast) EMC_GEMM_CASE(kRowFast, kRowFast)
   EMC_GEMM_CASE(kConvMK, kStrided) EMC_GEMM_CASE(kConvMK, kKFast) EMC_GEMM_CASE(kConvMK, kRowFast)
   EMC_GEMM_CASE(kStrided, kConvKM) EMC_GEMM_CASE(kKFast, kConvKM) EMC_GEMM_CASE(kRowFast, kConvKM)
+  EMC_GEMM_CASE(kConvT, kStrided) EMC_GEMM_CASE(kConvT, kKFast) EMC_GEMM_CASE(kConvT, kRowFast)
 #undef EMC_GEMM_CASE
   if (nz > 1) {
     EMC_LAUNCHED();
@@ -318,35 +362,12 @@ __global__ void k_im2col(const float* __restrict__ in, int B, int H, int W, int 
   col[t] = (y >= 0 && y < H && x >= 0 && x < W) ? in[(((long)b * H + y) * W + x) * C + c] : 0.0f;
 }
 
-// Adjoint of k_im2col as a gather (each input element sums the col entries
-// that copied it): deterministic, no atomics.
-__global__ void k_col2im(const float* __restrict__ col, int B, int H, int W, int C, float* __restrict__ in) {
-  const int OH = H / 2, OW = W / 2, K = 9 * C;
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (long)B * H * W * C) return;
-  const int c = (int)(t % C), x = (int)((t / C) % W), y = (int)((t / ((long)C * W)) % H);
-  const int b = (int)(t / ((long)C * W * H));
-  float s = 0.0f;
-  for (int ky = 0; ky < 3; ky++) {
-    const int ty = y + 1 - ky;
-    if (ty < 0 || (ty & 1)) continue;
-    const int oy = ty >> 1;
-    if (oy >= OH) continue;
-    for (int kx = 0; kx < 3; kx++) {
-      const int tx = x + 1 - kx;
-      if (tx < 0 || (tx & 1)) continue;
-      const int ox = tx >> 1;
-      if (ox >= OW) continue;
-      s += col[(((long)b * OH + oy) * OW + ox) * K + (ky * 3 + kx) * C + c];
-    }
-  }
-  in[t] = s;
-}
-
-// g *= (a > 0)
-__global__ void k_relu_bwd(float* __restrict__ g, const float* __restrict__ a, long n) {
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t < n && !(a[t] > 0.0f)) g[t] = 0.0f;
+// wt[(t*OC + o)*IC + c] = w[(o*9 + t)*IC + c]
+__global__ void k_conv_weights_t(const float* __restrict__ w, int OC, int IC, float* __restrict__ wt) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= 9 * OC * IC) return;
+  const int c = i % IC, o = (i / IC) % OC, t = i / (IC * OC);
+  wt[i] = w[((long)o * 9 + t) * IC + c];
 }
 
 // out[j] = sum_i G[i*si + j*sj], i < rows (fixed order per column).
@@ -505,12 +526,8 @@ void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t
   hipLaunchKernelGGL(k_im2col, dim3(nblk((long)B * (H / 2) * (W / 2) * 9 * C)), dim3(256), 0, st, in, B, H, W, C, col);
   EMC_LAUNCHED();
 }
-void col2im(const float* col, int B, int H, int W, int C, float* in, hipStream_t st) {
-  hipLaunchKernelGGL(k_col2im, dim3(nblk((long)B * H * W * C)), dim3(256), 0, st, col, B, H, W, C, in);
-  EMC_LAUNCHED();
-}
-void relu_bwd(float* g, const float* a, long n, hipStream_t st) {
-  hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n)), dim3(256), 0, st, g, a, n);
+void conv_weights_t(const float* w, int OC, int IC, float* wt, hipStream_t st) {
+  hipLaunchKernelGGL(k_conv_weights_t, dim3(nblk(9L * OC * IC)), dim3(256), 0, st, w, OC, IC, wt);
   EMC_LAUNCHED();
 }
 void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, double* scratch,
