@@ -95,6 +95,7 @@ void Engine::release_all() {
                   &d_rot_order};
   for (DBuf* b : bufs) b->release();
   order_key = {};
+  zrange_key = {};
   map_heavy = 0;
   cnt_images = nullptr;
   pack_key = {};

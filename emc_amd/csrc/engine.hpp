@@ -156,6 +156,12 @@ struct Engine {
   DBuf d_xpart_max, d_xpart_A;  // [R_pad][pixel tiles]: per-tile row maximum and A_r partial of that pack
   DBuf d_rot_order;     // int [R]: expand's processing order of the rotations, for the table in order_key
   struct {
+    const float* z = nullptr;
+    const int* mask = nullptr;
+    int S = 0;
+    uint64_t stamp = 0;
+  } zrange_key;         // d_zrange holds the z range of these coordinates and mask
+  struct {
     const float* rot = nullptr;
     int R = 0;
     uint64_t stamp = 0;

@@ -440,8 +440,15 @@ void launch_insert_brick(const float* Sc, int n_mp, const int* kidx, const int* 
                        n_act);
   }
   EMC_LAUNCHED();
-  hipLaunchKernelGGL(k_z_range, dim3(1), dim3(256), 0, st, z, mask, S * S, eng.d_zrange.as<float>());
-  EMC_LAUNCHED();
+  auto& zk = eng.zrange_key;  // the detector's z range, once per (coordinates, mask)
+  if (!(zk.z == z && zk.mask == mask && zk.S == S && zk.stamp == eng.invalidate_stamp)) {
+    hipLaunchKernelGGL(k_z_range, dim3(1), dim3(256), 0, st, z, mask, S * S, eng.d_zrange.as<float>());
+    EMC_LAUNCHED();
+    zk.z = z;
+    zk.mask = mask;
+    zk.S = S;
+    zk.stamp = eng.invalidate_stamp;
+  }
   if (deterministic) {
     const int nb = S / kIBd;
     hipLaunchKernelGGL((k_insert_brick<kIBd, 64, true>), dim3((unsigned)nb * nb * nb), dim3(64), 0, st, Sc, n_mp,

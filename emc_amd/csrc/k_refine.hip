@@ -116,13 +116,22 @@ __global__ __launch_bounds__(256) void k_refine_rows(float* __restrict__ P, int 
   }
   const float thr = ok ? lv.T[lv.lvl ? lv.lvl[i] : 0] : 3.0e38f;
   int c0 = 0, c1 = 0, c2 = 0;
-  for (int row = r0; row < r1; row++) {  // uniform over the workgroup
-    float* q = P + (size_t)row * ld + col;
-    const float p = ok ? *q : 0.0f;
-    c0 += p >= lv.T[0];
-    c1 += p >= lv.T[1];
-    c2 += p >= lv.T[2];
-    push(out, count, cap, ok && p >= thr, own ? q : nullptr, row_rot ? row_rot[row] : row, i, p);
+  constexpr int U = 8;  // rows loaded together, so that U loads are in flight
+  for (int row0 = r0; row0 < r1; row0 += U) {  // uniform over the workgroup
+    float pv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) pv[u] = ok && row0 + u < r1 ? P[(size_t)(row0 + u) * ld + col] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int row = row0 + u;
+      if (row >= r1) break;  // uniform
+      const float p = pv[u];
+      c0 += p >= lv.T[0];
+      c1 += p >= lv.T[1];
+      c2 += p >= lv.T[2];
+      push(out, count, cap, ok && p >= thr, own ? P + (size_t)row * ld + col : nullptr, row_rot ? row_rot[row] : row,
+           i, p);
+    }
   }
   if (ok && !lv.lvl && c0) {
     atomicAdd(&lv.cnt[i], c0);
@@ -146,22 +155,34 @@ __global__ __launch_bounds__(256) void k_refine_cand(float* __restrict__ Lc, int
   const float thr = lv.T[lv.lvl ? lv.lvl[i] : 0];
   float* row = Lc + (size_t)i * topk * slots;
   int c0 = 0, c1 = 0, c2 = 0;
-  for (int t0 = 0; t0 < topk * slots; t0 += 64) {
-    const int t = t0 + lane;
-    int r = 0;
-    float p = 0.0f;
-    if (t < topk * slots) {
-      const int j = t / slots, s = t - j * slots;
-      const int c = cand[(size_t)i * topk + j];
-      if (s < cell_off[c + 1] - cell_off[c]) {
-        p = row[t];
-        r = cell_rot[cell_off[c] + s];
+  constexpr int U = 4;  // slot groups loaded together
+  for (int t0 = 0; t0 < topk * slots; t0 += 64 * U) {
+    int rv[U];
+    float pv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = t0 + u * 64 + lane;
+      rv[u] = 0;
+      pv[u] = 0.0f;
+      if (t < topk * slots) {
+        const int j = t / slots, s = t - j * slots;
+        const int c = cand[(size_t)i * topk + j];
+        const int o = cell_off[c];
+        if (s < cell_off[c + 1] - o) {
+          pv[u] = row[t];
+          rv[u] = cell_rot[o + s];
+        }
       }
     }
-    c0 += p >= lv.T[0];
-    c1 += p >= lv.T[1];
-    c2 += p >= lv.T[2];
-    push(out, count, cap, p >= thr, row + (t < topk * slots ? t : 0), r, i, p);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = t0 + u * 64 + lane;
+      const float p = pv[u];
+      c0 += p >= lv.T[0];
+      c1 += p >= lv.T[1];
+      c2 += p >= lv.T[2];
+      push(out, count, cap, p >= thr, row + (t < topk * slots ? t : 0), rv[u], i, p);
+    }
   }
   c0 = wave_sum(c0);
   c1 = wave_sum(c1);

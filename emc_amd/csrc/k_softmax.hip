@@ -117,9 +117,9 @@ __global__ __launch_bounds__(256) void k_softmax_reduce(int N, int chunks, const
 }
 
 // Sum of n doubles in a fixed order, result in out[0].
-__global__ __launch_bounds__(256) void k_sum_f64(const double* __restrict__ in, int n, double* __restrict__ out) {
+__global__ __launch_bounds__(1024) void k_sum_f64(const double* __restrict__ in, int n, double* __restrict__ out) {
   double s = 0.0;
-  for (int k = threadIdx.x; k < n; k += 256) s += in[k];
+  for (int k = threadIdx.x; k < n; k += 1024) s += in[k];
   s = block_sum_f64(s);
   if (threadIdx.x == 0) out[0] = s;
 }
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_plogp_partial(const float* __restrict__
 }
 
 void launch_sum_f64(const double* v, int n, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, st, v, n, out);
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, v, n, out);
   EMC_LAUNCHED();
 }
 
