@@ -768,20 +768,22 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   }
   g_slices_epoch++;
   launch_set_wT(e.d_wts.as<float>(), e.d_Tf.as<float>(), R, e.d_wT.as<float>(), e.stream);
-  // deterministic: the brick form when few rotations are active (its work
-  // follows them), the sorted map otherwise (its work is fixed); the two give
-  // the same bits
-  bool det_brick = e.insert_mode == INSERT_DET_BRICK && insert_brick_fits(S, true);
-  if (e.insert_mode == INSERT_DETERMINISTIC && insert_brick_fits(S, true)) {
+  // deterministic: the brick form or the sorted map, whichever costs less
+  // (the two give the same bits).  The brick's longest chain, at the centre
+  // of the model, takes every active rotation in turn, about 1 us each; the
+  // map reads all R x n_ins (x 8 trilinear) entries, about 4 ps each.
+  bool det_brick = e.insert_mode == INSERT_DET_BRICK && insert_brick_fits(S, R, true);
+  if (e.insert_mode == INSERT_DETERMINISTIC && insert_brick_fits(S, R, true)) {
     std::vector<float> Th(R);
     EMC_CHECK(hipMemcpyAsync(Th.data(), e.d_Tf.p, sizeof(float) * R, hipMemcpyDeviceToHost, e.stream));
     sync();
     if (failed()) return 0.0f;
     size_t n_act = 0;
     for (int r = 0; r < R; r++) n_act += Th[r] > 1e-10f;
-    det_brick = n_act * 16 <= (size_t)R;
+    const double entries = (double)R * e.n_ins * (e.interp == INTERP_NEAREST ? 1.0 : 8.0);
+    det_brick = (double)n_act * 1.0e-6 <= entries * 4.0e-12;
   }
-  if (det_brick || (e.insert_mode == INSERT_ATOMIC && insert_brick_fits(S, false))) {
+  if (det_brick || (e.insert_mode == INSERT_ATOMIC && insert_brick_fits(S, R, false))) {
     PhaseTimer pt(PH_INSERT);
     launch_insert_brick(e.d_Sc.as<float>(), e.n_mp, e.d_kidx.as<int>(), d_mask, e.d_wT.as<float>(), d_rotations, R,
                         S, x_coordinates, y_coordinates, z_coordinates, d_model, d_weight, e.interp, det_brick,

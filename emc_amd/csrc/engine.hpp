@@ -386,7 +386,7 @@ void launch_set_wT(const float* wts, const float* Tf, int R, float* wT, hipStrea
 // brick-owned insertion (k_model.hip), LDS atomics or the deterministic
 // sequential order; the model side must be a multiple of the brick edge
 // (8 atomic, 4 deterministic)
-bool insert_brick_fits(int S, bool deterministic);
+bool insert_brick_fits(int S, int R, bool deterministic);
 void launch_insert_brick(const float* Sc, int n_mp, const int* kidx, const int* mask, const float* wT,
                          const float* rot, int R, int S, const float* x, const float* y, const float* z,
                          float* model, float* weight, int interp, bool deterministic, hipStream_t st);

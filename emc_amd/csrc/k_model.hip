@@ -173,14 +173,8 @@ void launch_insert_atomic(const float* Sc, int n_mp, const int* pix, const int* 
 // workgroup touches those voxels), with several (small models: too few
 // bricks to fill the chip) a global float atomic per voxel.
 //
-// "deterministic" (4^3 bricks, one wave): the brick starts from the model's
-// current values and takes its rotations in increasing order, each
-// rotation's pixels in the reference's x-major order; within a chunk of 64
-// pixels, lanes whose voxels collide add one at a time in pixel order (each
-// round the lowest pixel index holding a voxel, found with an LDS atomicMin,
-// adds).  Every voxel therefore receives the float additions of the
-// sequential insertion (emc.c:491-539 per rotation) in the same order, and
-// the result is the same bits on every run.
+// "deterministic" (4^3 bricks, one wave, k_insert_brick_det below): the
+// same culling, additions in the sequential insertion's order.
 // ---------------------------------------------------------------------------
 
 // Active rotations (T_r > 1e-10) into act[0 .. *n), in no particular order.
@@ -238,7 +232,7 @@ __global__ __launch_bounds__(256) void k_z_range(const float* __restrict__ z, co
   }
 }
 
-template <int IB, int NT, bool DET>
+template <int IB, int NT>
 __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ Sc, int n_mp,
                                                      const int* __restrict__ kidx, const int* __restrict__ mask,
                                                      const int* __restrict__ act, const int* __restrict__ n_act_p,
@@ -249,10 +243,8 @@ __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ S
                                                      float* __restrict__ model, float* __restrict__ weight,
                                                      int interp) {
 #pragma clang fp contract(off)
-  static_assert(!DET || NT == 64, "the deterministic form runs one wave per brick");
   constexpr int NV = IB * IB * IB;
   __shared__ float acc_m[NV], acc_w[NV];
-  __shared__ int own[DET ? NV : 1];
   __shared__ int list[NT];
   __shared__ int nlist;
   const int t = threadIdx.x;
@@ -261,16 +253,8 @@ __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ S
   const int bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
   const int x0 = bx * IB, y0 = by * IB, z0 = bz * IB;
   for (int l = t; l < NV; l += NT) {
-    if (DET) {
-      const int vx = l % IB, vy = (l / IB) % IB, vz = l / (IB * IB);
-      const size_t v = ((size_t)(z0 + vz) * S + (y0 + vy)) * S + (x0 + vx);
-      acc_m[l] = model[v];
-      acc_w[l] = weight[v];
-      own[l] = 0x7fffffff;
-    } else {
-      acc_m[l] = 0.0f;
-      acc_w[l] = 0.0f;
-    }
+    acc_m[l] = 0.0f;
+    acc_w[l] = 0.0f;
   }
   const float half = (float)S / 2.0f + 0.5f;
   const bool tri = interp != INTERP_NEAREST;
@@ -285,7 +269,7 @@ __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ S
   const int n_act = *n_act_p;
   const int a0 = (int)((long long)n_act * g / groups), a1 = (int)((long long)n_act * (g + 1) / groups);
   for (int base = a0; base < a1; base += NT) {
-    if (!DET && t == 0) nlist = 0;
+    if (t == 0) nlist = 0;
     __syncthreads();
     bool pass = false;
     int r = 0;
@@ -299,16 +283,9 @@ __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ S
       const float py = e[1] * cx + e[4] * cy + e[7] * cz, ry = hs * (fabsf(e[1]) + fabsf(e[4]) + fabsf(e[7]));
       pass = pz - rz <= zhi && pz + rz >= zlo && px - rx <= dhi && px + rx >= dlo && py - ry <= dhi && py + ry >= dlo;
     }
-    int n;
-    if (DET) {  // one wave: the passing rotations in increasing order
-      const unsigned long long m = __ballot(pass);
-      if (pass) list[__popcll(m & ((1ull << t) - 1ull))] = r;
-      n = __popcll(m);
-    } else {
-      if (pass) list[atomicAdd(&nlist, 1)] = r;
-    }
+    if (pass) list[atomicAdd(&nlist, 1)] = r;
     __syncthreads();
-    if (!DET) n = nlist;
+    const int n = nlist;
     for (int j = 0; j < n; j++) {
       const int r = list[j];
       float e[9];
@@ -336,8 +313,7 @@ __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ S
           vw[c] = 0.0f;
         }
         if (k < npx) {
-          // deterministic: x-major (the reference's pixel order); atomic: rows
-          const int p = DET ? (iy0 + k % h_) * S + ix0 + k / h_ : (iy0 + k / w_) * S + ix0 + k % w_;
+          const int p = (iy0 + k / w_) * S + ix0 + k % w_;
           if (mask[p] == 1) {
             float ux, uy, uz;
             voxel_coord(e, half, x[p], y[p], z[p], ux, uy, uz);
@@ -365,41 +341,12 @@ __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ S
             }
           }
         }
-        if (!DET) {
 #pragma unroll
-          for (int q = 0; q < 8; q++)
-            if (lv[q] >= 0) {
-              atomicAdd(&acc_m[lv[q]], vm[q]);
-              atomicAdd(&acc_w[lv[q]], vw[q]);
-            }
-        } else {
-          // rounds: the lowest pixel index holding a voxel adds to it (a
-          // pixel's corners are distinct voxels)
-          unsigned pend = 0;
-#pragma unroll
-          for (int q = 0; q < 8; q++) pend |= (lv[q] >= 0 ? 1u : 0u) << q;
-          for (;;) {
-#pragma unroll
-            for (int q = 0; q < 8; q++)
-              if ((pend >> q) & 1u) atomicMin(&own[lv[q]], k);
-            __syncthreads();
-            unsigned mine = 0;
-#pragma unroll
-            for (int q = 0; q < 8; q++)
-              if (((pend >> q) & 1u) && own[lv[q]] == k) {
-                acc_m[lv[q]] += vm[q];
-                acc_w[lv[q]] += vw[q];
-                mine |= 1u << q;
-              }
-            pend &= ~mine;
-            const int more = __syncthreads_or(pend != 0u);
-#pragma unroll
-            for (int q = 0; q < 8; q++)
-              if ((mine >> q) & 1u) own[lv[q]] = 0x7fffffff;
-            __syncthreads();
-            if (!more) break;
+        for (int q = 0; q < 8; q++)
+          if (lv[q] >= 0) {
+            atomicAdd(&acc_m[lv[q]], vm[q]);
+            atomicAdd(&acc_w[lv[q]], vw[q]);
           }
-        }
       }
     }
     __syncthreads();
@@ -407,10 +354,7 @@ __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ S
   for (int l = t; l < NV; l += NT) {
     const int vx = l % IB, vy = (l / IB) % IB, vz = l / (IB * IB);
     const size_t v = ((size_t)(z0 + vz) * S + (y0 + vy)) * S + (x0 + vx);
-    if (DET) {
-      model[v] = acc_m[l];
-      weight[v] = acc_w[l];
-    } else if (acc_w[l] != 0.0f || acc_m[l] != 0.0f) {
+    if (acc_w[l] != 0.0f || acc_m[l] != 0.0f) {
       if (groups == 1) {  // this workgroup owns the voxel
         model[v] += acc_m[l];
         weight[v] += acc_w[l];
@@ -419,6 +363,243 @@ __global__ __launch_bounds__(NT) void k_insert_brick(const float* __restrict__ S
         atomicAdd(&weight[v], acc_w[l]);
       }
     }
+  }
+}
+
+
+// Deterministic form: one wave per IB^3 brick, starting from the model's
+// values.  The rotations that can reach the brick are listed 64 at a time in
+// increasing order, with their matrices, pixel boxes and weights in LDS.
+// Work items are (listed rotation, chunk of 64 of its box's pixels, taken
+// x-major); NB items at a time have their pixels' contributions gathered,
+// every item's loads issued before any is waited for, so a brick that many
+// rotations reach pays one memory latency per NB items.  The items are then
+// added in order, and within one item lanes whose voxels collide add one at
+// a time in pixel order (nearest: the lowest pixel of each voxel, found with
+// one LDS atomicMax of stamped keys, adds, then the rest lane by lane;
+// trilinear: rounds in which the lowest pixel index holding a voxel, found
+// with an LDS atomicMin, adds).  Every voxel receives the float
+// additions of the sequential insertion (emc.c:491-539 per rotation) in the
+// same order.
+template <int IB, int NB, bool TRI>
+__global__ __launch_bounds__(64) void k_insert_brick_det(const float* __restrict__ Sc, int n_mp,
+                                                         const int* __restrict__ kidx, const int* __restrict__ mask,
+                                                         const int* __restrict__ act, const int* __restrict__ n_act_p,
+                                                         const float* __restrict__ wT,
+                                                         const float* __restrict__ rot, int S,
+                                                         const float* __restrict__ x, const float* __restrict__ y,
+                                                         const float* __restrict__ z,
+                                                         const float* __restrict__ zrange,
+                                                         float* __restrict__ model, float* __restrict__ weight) {
+#pragma clang fp contract(off)
+  constexpr int NV = IB * IB * IB, NC = TRI ? 8 : 1;
+  __shared__ float acc_m[NV], acc_w[NV];
+  __shared__ int own[NV];
+  __shared__ float le[9][64];  // listed rotations: matrix entries
+  __shared__ int4 lbox[64];    // first pixel column, first row, width, pixel count
+  __shared__ int lr[64];
+  __shared__ float lw[64];
+  const int t = threadIdx.x;
+  const int nb = S / IB;
+  const int brick = blockIdx.x;
+  const int x0 = (brick % nb) * IB, y0 = ((brick / nb) % nb) * IB, z0 = (brick / (nb * nb)) * IB;
+  for (int l = t; l < NV; l += 64) {
+    const int vx = l % IB, vy = (l / IB) % IB, vz = l / (IB * IB);
+    const size_t v = ((size_t)(z0 + vz) * S + (y0 + vy)) * S + (x0 + vx);
+    acc_m[l] = model[v];
+    acc_w[l] = weight[v];
+    own[l] = TRI ? 0x7fffffff : -1;
+  }
+  int seq = 0;  // items added so far (nearest: key stamps)
+  const float half = (float)S / 2.0f + 0.5f;
+  const float lo = TRI ? -1.0f : -0.5f, hi = TRI ? (float)IB : (float)IB - 0.5f;
+  const float hs = 0.5f * (hi - lo) + 0.05f;
+  const float cx = (float)x0 + 0.5f * (lo + hi) - half, cy = (float)y0 + 0.5f * (lo + hi) - half,
+              cz = (float)z0 + 0.5f * (lo + hi) - half;
+  const float zlo = zrange[0] - 0.05f, zhi = zrange[1] + 0.05f;
+  const float dlo = 0.5f - 0.5f * (float)S - 0.05f, dhi = 0.5f * (float)S - 0.5f + 0.05f;
+  const float off = 0.5f * (float)S - 0.5f;
+  const int n_act = *n_act_p;
+  for (int base = 0; base < n_act; base += 64) {
+    bool pass = false;
+    int r = 0, ix0 = 0, iy0 = 0, w_ = 0, npx = 0;
+    float e[9];
+    if (base + t < n_act) {
+      r = act[base + t];
+      rot_matrix(&rot[4 * (size_t)r], e);
+      const float pz = e[2] * cx + e[5] * cy + e[8] * cz, rz = hs * (fabsf(e[2]) + fabsf(e[5]) + fabsf(e[8]));
+      const float px = e[0] * cx + e[3] * cy + e[6] * cz, rx = hs * (fabsf(e[0]) + fabsf(e[3]) + fabsf(e[6]));
+      const float py = e[1] * cx + e[4] * cy + e[7] * cz, ry = hs * (fabsf(e[1]) + fabsf(e[4]) + fabsf(e[7]));
+      if (pz - rz <= zhi && pz + rz >= zlo && px - rx <= dhi && px + rx >= dlo && py - ry <= dhi &&
+          py + ry >= dlo) {
+        ix0 = max(0, (int)ceilf(px - rx + off));
+        iy0 = max(0, (int)ceilf(py - ry + off));
+        const int ix1 = min(S - 1, (int)floorf(px + rx + off)), iy1 = min(S - 1, (int)floorf(py + ry + off));
+        pass = ix1 >= ix0 && iy1 >= iy0;
+        w_ = ix1 - ix0 + 1;
+        npx = w_ * (iy1 - iy0 + 1);
+      }
+    }
+    const unsigned long long m = __ballot(pass);
+    const int n = __popcll(m);
+    __syncthreads();  // the previous list is no longer read
+    if (pass) {
+      const int i = __popcll(m & ((1ull << t) - 1ull));
+#pragma unroll
+      for (int c = 0; c < 9; c++) le[c][i] = e[c];
+      lbox[i] = make_int4(ix0, iy0, w_, npx);
+      lr[i] = r;
+      lw[i] = wT[r];
+    }
+    __syncthreads();
+    int j = 0, k0 = 0;  // next item: listed rotation j, its pixels k0 .. k0 + 63 (uniform)
+    while (j < n) {
+      int ij[NB], ik[NB];
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        ij[b] = j;
+        ik[b] = k0 + t;
+        if (j < n) {
+          k0 += 64;
+          if (k0 >= lbox[j].w) {
+            j++;
+            k0 = 0;
+          }
+        }
+      }
+      // gather: pixel, then its coordinates and column, then the slice value,
+      // each level issued for all NB items before it is waited for
+      int p[NB], msk[NB], col[NB];
+      float px_[NB], py_[NB], pz_[NB], sv[NB];
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        const int4 bx = lbox[ij[b] < n ? ij[b] : 0];
+        const bool in = ij[b] < n && ik[b] < bx.w;
+        const int h_ = bx.w / bx.z;  // rows of the box (npx = width x rows)
+        p[b] = in ? (bx.y + ik[b] % h_) * S + bx.x + ik[b] / h_ : -1;
+      }
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        const int q = p[b] < 0 ? 0 : p[b];
+        msk[b] = mask[q];
+        col[b] = kidx[q];
+        px_[b] = x[q];
+        py_[b] = y[q];
+        pz_[b] = z[q];
+      }
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        if (p[b] < 0 || msk[b] != 1) p[b] = -1;
+        const int jj = ij[b] < n ? ij[b] : 0;
+        sv[b] = Sc[(size_t)lr[jj] * n_mp + (p[b] < 0 ? 0 : col[b])];
+      }
+      // add: item by item in order
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        if (ij[b] >= n) break;  // uniform
+        const int jj = ij[b];
+        float em[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) em[c] = le[c][jj];
+        const float w = lw[jj];
+        int lv[NC];
+        float vm[NC], vw[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          lv[c] = -1;
+          vm[c] = 0.0f;
+          vw[c] = 0.0f;
+        }
+        if (p[b] >= 0) {
+          float ux, uy, uz;
+          voxel_coord(em, half, px_[b], py_[b], pz_[b], ux, uy, uz);
+          const float val = w * sv[b];
+          if (!TRI) {
+            const int vx = (int)roundf(ux) - x0, vy = (int)roundf(uy) - y0, vz = (int)roundf(uz) - z0;
+            if ((unsigned)vx < (unsigned)IB && (unsigned)vy < (unsigned)IB && (unsigned)vz < (unsigned)IB) {
+              lv[0] = (vz * IB + vy) * IB + vx;
+              vm[0] = val;
+              vw[0] = w;
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+              int gx, gy, gz;
+              float cw;
+              if (!tri_corner_xyz(c, ux, uy, uz, S, gx, gy, gz, cw)) continue;
+              const int vx = gx - x0, vy = gy - y0, vz = gz - z0;
+              if ((unsigned)vx < (unsigned)IB && (unsigned)vy < (unsigned)IB && (unsigned)vz < (unsigned)IB) {
+                lv[c] = (vz * IB + vy) * IB + vx;
+                vm[c] = cw * val;
+                vw[c] = cw * w;
+              }
+            }
+          }
+        }
+        const int k = ik[b];
+        if (!TRI) {
+          // the lowest pixel of each voxel adds first (stamped keys: the
+          // largest key of this item is its lowest pixel, and keys of
+          // earlier items are smaller), then the others one lane at a time
+          // in pixel order
+          seq++;
+          const int key = (seq << 6) | (63 - t);
+          const int v = lv[0];
+          if (v >= 0) atomicMax(&own[v], key);
+          __syncthreads();
+          const bool win = v >= 0 && own[v] == key;
+          if (win) {
+            acc_m[v] += vm[0];
+            acc_w[v] += vw[0];
+          }
+          unsigned long long lose = __ballot(v >= 0 && !win);
+          while (lose) {  // uniform
+            __syncthreads();
+            const int l = __ffsll((long long)lose) - 1;
+            lose &= lose - 1ull;
+            if (t == l) {
+              acc_m[v] += vm[0];
+              acc_w[v] += vw[0];
+            }
+          }
+          __syncthreads();
+          continue;
+        }
+        // rounds: the lowest pixel index holding a voxel adds to it (a
+        // pixel's corners are distinct voxels)
+        unsigned pend = 0;
+#pragma unroll
+        for (int q = 0; q < NC; q++) pend |= (lv[q] >= 0 ? 1u : 0u) << q;
+        for (;;) {
+#pragma unroll
+          for (int q = 0; q < NC; q++)
+            if ((pend >> q) & 1u) atomicMin(&own[lv[q]], k);
+          __syncthreads();
+          unsigned mine = 0;
+#pragma unroll
+          for (int q = 0; q < NC; q++)
+            if (((pend >> q) & 1u) && own[lv[q]] == k) {
+              acc_m[lv[q]] += vm[q];
+              acc_w[lv[q]] += vw[q];
+              mine |= 1u << q;
+            }
+          pend &= ~mine;
+          const int more = __syncthreads_or(pend != 0u);
+#pragma unroll
+          for (int q = 0; q < NC; q++)
+            if ((mine >> q) & 1u) own[lv[q]] = 0x7fffffff;
+          __syncthreads();
+          if (!more) break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = t; l < NV; l += 64) {
+    const int vx = l % IB, vy = (l / IB) % IB, vz = l / (IB * IB);
+    const size_t v = ((size_t)(z0 + vz) * S + (y0 + vy)) * S + (x0 + vx);
+    model[v] = acc_m[l];
+    weight[v] = acc_w[l];
   }
 }
 
@@ -451,21 +632,29 @@ void launch_insert_brick(const float* Sc, int n_mp, const int* kidx, const int* 
   }
   if (deterministic) {
     const int nb = S / kIBd;
-    hipLaunchKernelGGL((k_insert_brick<kIBd, 64, true>), dim3((unsigned)nb * nb * nb), dim3(64), 0, st, Sc, n_mp,
-                       kidx, mask, act, n_act, 1, wT, rot, S, x, y, z, eng.d_zrange.as<float>(), model, weight,
-                       interp);
+    if (interp == INTERP_NEAREST)
+      hipLaunchKernelGGL((k_insert_brick_det<kIBd, 8, false>), dim3((unsigned)nb * nb * nb), dim3(64), 0, st, Sc,
+                         n_mp, kidx, mask, act, n_act, wT, rot, S, x, y, z, eng.d_zrange.as<float>(), model, weight);
+    else
+      hipLaunchKernelGGL((k_insert_brick_det<kIBd, 4, true>), dim3((unsigned)nb * nb * nb), dim3(64), 0, st, Sc,
+                         n_mp, kidx, mask, act, n_act, wT, rot, S, x, y, z, eng.d_zrange.as<float>(), model, weight);
   } else {
     // rotation groups per brick: enough workgroups to fill the chip
     const int nb = S / kIB, nbricks = nb * nb * nb;
     const int groups = nbricks >= 4096 ? 1 : std::min(64, (4096 + nbricks - 1) / nbricks);
-    hipLaunchKernelGGL((k_insert_brick<kIB, 256, false>), dim3((unsigned)(nbricks * groups)), dim3(256), 0, st, Sc,
+    hipLaunchKernelGGL((k_insert_brick<kIB, 256>), dim3((unsigned)(nbricks * groups)), dim3(256), 0, st, Sc,
                        n_mp, kidx, mask, act, n_act, groups, wT, rot, S, x, y, z, eng.d_zrange.as<float>(), model,
                        weight, interp);
   }
   EMC_LAUNCHED();
 }
 
-bool insert_brick_fits(int S, bool deterministic) { return deterministic ? S % kIBd == 0 : S % kIB == 0; }
+// The deterministic nearest form stamps its keys with the item count of a
+// brick (at most one item per rotation, 64 pixels): R < 2^24 keeps them in
+// an int.
+bool insert_brick_fits(int S, int R, bool deterministic) {
+  return deterministic ? S % kIBd == 0 && R < (1 << 24) : S % kIB == 0;
+}
 
 // ---------------------------------------------------------------------------
 // Deterministic insertion.  The voxel hit by (rotation r, pixel t) depends
@@ -713,12 +902,15 @@ __global__ __launch_bounds__(256) void k_insert_gather_wave(const float* __restr
         }
       }
     }
-    const int cnt = (int)min(64u, en - j0);
-    for (int l = 0; l < cnt; l++) {  // uniform: every lane adds the same terms
-      if (__builtin_amdgcn_readlane(on, l)) {
-        m += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dm), l));
-        wsum += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dw), l));
-      }
+    // uniform: every lane adds the same terms, the active entries only, in
+    // increasing lane order (most rotations are inactive once the posterior
+    // is sharp, so the serial chain follows the active entries)
+    unsigned long long act = __ballot(on);
+    while (act) {
+      const int l = __ffsll((long long)act) - 1;
+      act &= act - 1ull;
+      m += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dm), l));
+      wsum += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dw), l));
     }
   }
   if (lane == 0) {

@@ -98,13 +98,15 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 order of the reference's sequential insertion (rotations
  *                 ascending, pixels x-major), so bit-reproducible run to run
  *                 and equal to that insertion: by "deterministic_brick"
- *                 while at most 1/16 of the rotations have T_r > 1e-10
- *                 (and the side is a multiple of 4), else by
- *                 "sorted_scatter" -- the two give the same bits;
+ *                 or "sorted_scatter", whichever the number of rotations
+ *                 with T_r > 1e-10 makes cheaper (the brick form needs a
+ *                 side that is a multiple of 4) -- the two give the same
+ *                 bits;
  *                "deterministic_brick": 4^3 bricks, one wave each, taking
  *                 the active rotations in order; colliding pixels add one
  *                 at a time in pixel order (work follows the active
- *                 rotations);
+ *                 rotations; side a multiple of 4 and fewer than 2^24
+ *                 rotations, else "sorted_scatter" is used);
  *                "sorted_scatter": the same sums from a voxel-sorted map of
  *                 every (rotation, pixel) entry, built once per rotation
  *                 table (memory: 4 bytes per entry, 36 trilinear; work
