@@ -383,7 +383,7 @@ def main():
     ap.add_argument("--likelihood", choices=["gaussian", "poisson"], default=None,
                     help="default: the workload's (poisson = count x log-slice contraction)")
     ap.add_argument("--sigma", type=float, default=0.4, help="noise level of the gaussian likelihood")
-    ap.add_argument("--gemm-tile", choices=["256", "128", "256x128"], default=None,
+    ap.add_argument("--gemm-tile", choices=["256", "256w8", "128", "256x128"], default=None,
                     help="tile of the FP16 likelihood GEMM (default: the library's)")
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="library option (emc_hip_set_option) set before the run; repeatable")

@@ -1087,6 +1087,7 @@ int emc_hip_set_option(const char* key, const char* value) {
     if (is(value, "256")) e.gemm_shape = 0;
     else if (is(value, "128")) e.gemm_shape = 1;
     else if (is(value, "256x128")) e.gemm_shape = 2;
+    else if (is(value, "256w8")) e.gemm_shape = 3;
     else return 1;
   } else if (is(key, "gemm_tail")) {
     if (is(value, "split")) e.tail_split = true;

@@ -88,7 +88,8 @@ struct Engine {
   int mstep_mode = MSTEP_AUTO;  // FP16 form: M-step over the nonzero responsibilities
   bool respons_delta = true;    // proposal iterations rewrite only the changed responsibilities
   bool tail_split = true;       // FP16 likelihood GEMM: last partial round of tiles split along k
-  int gemm_shape = 0;           // FP16 likelihood GEMM tiles: 0 256 x 256, 1 128 x 128 (two per unit), 2 256 x 128
+  int gemm_shape = 0;           // FP16 likelihood GEMM tiles: 0 256 x 256 (16 waves), 1 128 x 128 (two per unit),
+                                // 2 256 x 128, 3 256 x 256 (8 waves)
   bool expand_tiled = false;    // fused expand + pack in rotation blocks x pixel tiles even where the slice fits LDS
   bool expand_sorted = true;    // expand takes the rotations in the order of their slice normals (k_expand.hip)
   bool expand_rows = true;      // fused expand + pack: pixels in rows (true) or 8 x 8 tiles per wave

@@ -178,13 +178,14 @@ void launch_group_rank(const int* cand, const int* key, int N, int topk, int C, 
 
 // ---------------------------------------------------------------------------
 // Dense block of a group: 256 patterns x 256 core rotations per workgroup
-// (8 waves, one workgroup per compute unit as in the dense GEMM): every
+// (16 waves of 128 x 32, one workgroup per compute unit as in the dense
+// GEMM): every
 // gathered count row is read once per 256 core rotations and every slice row
 // once per 256 patterns of the group.
 // ---------------------------------------------------------------------------
 template <bool KSPLIT>
-__global__ __launch_bounds__(512, 1) void k_group_estep(GroupEstepArgs a) {
-  constexpr int TM = 8, TN = 4, WGM = 2, WGN = 4, NA = KSPLIT ? 2 : 1, NB = 2, STAGES = KSPLIT ? 2 : 3;
+__global__ __launch_bounds__(1024, 1) void k_group_estep(GroupEstepArgs a) {
+  constexpr int TM = 8, TN = 2, WGM = 2, WGN = 8, NA = KSPLIT ? 2 : 1, NB = 2, STAGES = KSPLIT ? 2 : 3;
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   static_assert(BM == kGroupTile && BN == kGroupTile, "tile must match the work items");
   __shared__ __attribute__((aligned(1024))) _Float16 lds[STAGES * (NA * BM + NB * BN) * 32];
@@ -234,9 +235,9 @@ __global__ __launch_bounds__(512, 1) void k_group_estep(GroupEstepArgs a) {
 void launch_group_estep(bool ksplit, const GroupEstepArgs& a, hipStream_t st) {
   if (a.n_work <= 0) return;
   if (ksplit)
-    hipLaunchKernelGGL(k_group_estep<true>, dim3(a.n_work), dim3(512), 0, st, a);
+    hipLaunchKernelGGL(k_group_estep<true>, dim3(a.n_work), dim3(1024), 0, st, a);
   else
-    hipLaunchKernelGGL(k_group_estep<false>, dim3(a.n_work), dim3(512), 0, st, a);
+    hipLaunchKernelGGL(k_group_estep<false>, dim3(a.n_work), dim3(1024), 0, st, a);
   EMC_LAUNCHED();
 }
 

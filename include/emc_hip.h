@@ -160,8 +160,9 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 model still differ from the FP64 iteration by ~2e-4 of
  *                 the model's maximum, at 1e-9 by ~3e-5; "off".
  *   "gemm_tile"  "256" (default): 256 x 256 tiles of the FP16 likelihood
- *                 GEMM, one workgroup per compute unit, three LDS stages;
- *                 "128": 128 x 128 tiles, two workgroups per compute unit;
+ *                 GEMM, one 16-wave workgroup per compute unit, three LDS
+ *                 stages; "256w8": the same tiles with 8 waves; "128":
+ *                 128 x 128 tiles, two workgroups per compute unit;
  *                 "256x128": 256 x 128 tiles with four LDS stages (same
  *                 results, bit for bit).
  *   "gemm_tail"  "split" (default): the FP16 likelihood GEMM's tiles that

@@ -402,15 +402,15 @@ def test_expand_pack_large_slices_match_separate(likelihood, interp):
 
 def test_gemm_tile_128_matches_256():
     """The 128 x 128 tile form of the FP16 likelihood GEMM (two workgroups per
-    compute unit) and the 256 x 128 form (four LDS stages) sum every output
-    over k in the same order as the 256 x 256 form: bit-identical
-    likelihoods (all unsplit along k)."""
+    compute unit), the 256 x 128 form (four LDS stages) and the 8-wave
+    256 x 256 form sum every output over k in the same order as the 16-wave
+    256 x 256 form: bit-identical likelihoods (all unsplit along k)."""
     P = problems.make(side=32, n_patterns=300, n_rot=4, n_blobs=20, radius=6.0, photons=2000.0, seed=9)
     options("fp16")
     out = {}
     try:
         E.set_option("gemm_tail", "whole")
-        for tile in ("256", "128", "256x128"):
+        for tile in ("256", "256w8", "128", "256x128"):
             E.set_option("gemm_tile", tile)
             prob = E.EmcProblem(P.side, P.images, P.mask, P.q, P.w, P.model)
             prob.get_slices()
@@ -424,6 +424,7 @@ def test_gemm_tile_128_matches_256():
         E.set_option("gemm_tail", "split")
     assert np.array_equal(out["256"], out["128"])
     assert np.array_equal(out["256"], out["256x128"])
+    assert np.array_equal(out["256"], out["256w8"])
 
 
 @pytest.mark.parametrize("mask_kind", ["gap_h", "gap_cross"])
