@@ -119,26 +119,50 @@ bool ready() {
   return e.stream != nullptr;
 }
 
+hipEvent_t Engine::next_event() {
+  if (ev_used == ev_pool.size()) {
+    hipEvent_t ev = nullptr;
+    EMC_CHECK(hipEventCreate(&ev));
+    if (failed() || !ev) return nullptr;
+    ev_pool.push_back(ev);
+  }
+  return ev_pool[ev_used++];
+}
+
+// Adds the recorded pairs to phase_ms (waits for the last one) and hands the
+// pool's events out again.
+void Engine::resolve_phase_times() {
+  if (!ev_pending.empty()) {
+    EMC_CHECK(hipEventSynchronize(ev_pending.back().b));
+    for (const PhaseEvents& pe : ev_pending) {
+      float ms = 0.0f;
+      EMC_CHECK(hipEventElapsedTime(&ms, pe.a, pe.b));
+      phase_ms[pe.phase] += ms;
+      phase_launches[pe.phase] += 1;
+    }
+    ev_pending.clear();
+  }
+  ev_used = 0;
+}
+
 PhaseTimer::PhaseTimer(int ph) : phase(ph), on(false) {
   Engine& e = engine();
   if (!e.timing || failed()) return;
-  on = true;
-  if (!e.ev0) {
-    EMC_CHECK(hipEventCreate(&e.ev0));
-    EMC_CHECK(hipEventCreate(&e.ev1));
-  }
-  EMC_CHECK(hipEventRecord(e.ev0, e.stream));
+  // at most 512 pairs outstanding: resolve (one wait) when the list is full
+  if (e.ev_pending.size() >= 512) e.resolve_phase_times();
+  start = e.next_event();
+  if (!start) return;
+  EMC_CHECK(hipEventRecord(start, e.stream));
+  on = !failed();
 }
 
 PhaseTimer::~PhaseTimer() {
   if (!on) return;
   Engine& e = engine();
-  EMC_CHECK(hipEventRecord(e.ev1, e.stream));
-  EMC_CHECK(hipEventSynchronize(e.ev1));
-  float ms = 0.0f;
-  EMC_CHECK(hipEventElapsedTime(&ms, e.ev0, e.ev1));
-  e.phase_ms[phase] += ms;
-  e.phase_launches[phase] += 1;
+  hipEvent_t end = e.next_event();
+  if (!end) return;
+  EMC_CHECK(hipEventRecord(end, e.stream));
+  if (!failed()) e.ev_pending.push_back({start, end, phase});
 }
 
 static void sync() { EMC_CHECK(hipStreamSynchronize(engine().stream)); }
@@ -982,8 +1006,7 @@ void emc_hip_shutdown(void) {
   if (!g_engine) return;
   if (g_engine->stream) (void)hipStreamSynchronize(g_engine->stream);
   g_engine->release_all();
-  if (g_engine->ev0) (void)hipEventDestroy(g_engine->ev0);
-  if (g_engine->ev1) (void)hipEventDestroy(g_engine->ev1);
+  for (hipEvent_t ev : g_engine->ev_pool) (void)hipEventDestroy(ev);
   if (g_engine->stream) (void)hipStreamDestroy(g_engine->stream);
   delete g_engine;
   g_engine = nullptr;
@@ -1159,6 +1182,7 @@ static const char* kPhaseNames[PH_COUNT] = {"expand", "pack", "likelihood", "sof
 
 int emc_hip_phase_times(double* ms, long long* launches, int n) {
   Engine& e = engine();
+  e.resolve_phase_times();
   int k = 0;
   for (; k < n && k < PH_COUNT; k++) {
     if (ms) ms[k] = e.phase_ms[k];
@@ -1169,6 +1193,7 @@ int emc_hip_phase_times(double* ms, long long* launches, int n) {
 
 void emc_hip_reset_phase_times(void) {
   Engine& e = engine();
+  e.resolve_phase_times();
   for (int k = 0; k < PH_COUNT; k++) {
     e.phase_ms[k] = 0;
     e.phase_launches[k] = 0;

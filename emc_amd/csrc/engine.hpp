@@ -253,10 +253,20 @@ struct Engine {
 
   uint64_t invalidate_stamp = 1;
 
-  // ---- timing ----
+  // ---- timing: an event pair per timed phase, recorded on the stream and
+  // resolved only when the times are read (no synchronisation inside an
+  // iteration) ----
   double phase_ms[PH_COUNT] = {0};
   long long phase_launches[PH_COUNT] = {0};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  struct PhaseEvents {
+    hipEvent_t a, b;
+    int phase;
+  };
+  std::vector<hipEvent_t> ev_pool;       // created events, reused after each resolve
+  size_t ev_used = 0;                    // pool events recorded since the last resolve
+  std::vector<PhaseEvents> ev_pending;   // recorded pairs not yet added to phase_ms
+  hipEvent_t next_event();
+  void resolve_phase_times();
 
   bool init();
   void release_all();
@@ -269,6 +279,7 @@ bool ready();              // engine initialised and no sticky error
 struct PhaseTimer {
   int phase;
   bool on;
+  hipEvent_t start = nullptr;
   explicit PhaseTimer(int ph);
   ~PhaseTimer();
 };

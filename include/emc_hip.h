@@ -112,8 +112,10 @@ int emc_hip_memcpy_to_device(void *d_dst, const void *src, size_t bytes);
  *                 table (memory: 4 bytes per entry, 36 trilinear; work
  *                 fixed by the table).
  *   "interp"     "nearest" (default, the reference's rule) or "trilinear".
- *   "timing"     "1" records per-phase GPU time (adds a synchronisation per
- *                 entry point), "0" off.
+ *   "timing"     "1" records per-phase GPU time (an event pair per phase on
+ *                 the library's stream, read back when the times are
+ *                 queried: no synchronisation inside an entry point), "0"
+ *                 off.
  *   "verbose"    "1" prints the reference's per-call diagnostics.
  *   "likelihood" "gaussian" (default, the reference's); "poisson":
  *                 L = sum_p m (K log(phi max(S, 1e-6)) - phi S), scaling
