@@ -529,10 +529,11 @@ void cuda_normalize_responsabilities(real* d_respons, int N_slices, int N_images
                           fuse ? e.d_phi_used.as<float>() : nullptr, lk.sigma2x, e.d_phiden.as<double>(),
                           e.d_red.as<double>(), e.stream, e.likelihood == LIKE_POISSON);
   }
-  double total = 0.0;
-  EMC_CHECK(hipMemcpyAsync(&total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  double* h_total = e.host_scalar<double>(3);
+  EMC_CHECK(hipMemcpyAsync(h_total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, e.stream));
   sync();
   if (failed()) return;
+  double total = *h_total;
   // the significant responsibilities from FP64 likelihoods, when the
   // likelihood this buffer holds is the library's own of these slices
   e.ref_entries = 0;
@@ -584,8 +585,10 @@ real emc_hip_total_respons_n(const real* d_respons, size_t n) {
     PhaseTimer pt(PH_OTHER);
     launch_total_respons(d_respons, n, e.d_red.as<double>(), e.stream);
   }
-  EMC_CHECK(hipMemcpyAsync(&total, e.d_red.as<double>() + 1024, sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  double* h_total = e.host_scalar<double>(4);
+  EMC_CHECK(hipMemcpyAsync(h_total, e.d_red.as<double>() + 1024, sizeof(double), hipMemcpyDeviceToHost, e.stream));
   sync();
+  total = *h_total;
   return (real)total;
 }
 
@@ -798,7 +801,8 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
   // map reads all R x n_ins (x 8 trilinear) entries, about 4 ps each.
   bool det_brick = e.insert_mode == INSERT_DET_BRICK && insert_brick_fits(S, R, true);
   if (e.insert_mode == INSERT_DETERMINISTIC && insert_brick_fits(S, R, true)) {
-    std::vector<float> Th(R);
+    HostVec<float>& Th = e.h_T;
+    Th.resize(R);
     EMC_CHECK(hipMemcpyAsync(Th.data(), e.d_Tf.p, sizeof(float) * R, hipMemcpyDeviceToHost, e.stream));
     sync();
     if (failed()) return 0.0f;
@@ -846,7 +850,8 @@ real cuda_update_slices(real* d_images, real* slices, int* d_mask, real* d_respo
                          d_rotations, R, S,
                          x_coordinates, y_coordinates, z_coordinates, d_model, d_weight, e.interp, e.stream);
   }
-  std::vector<float> T(R);
+  HostVec<float>& T = e.h_T;
+  T.resize(R);
   EMC_CHECK(hipMemcpyAsync(T.data(), e.d_Tf.p, sizeof(float) * R, hipMemcpyDeviceToHost, e.stream));
   // the model to the host (emc_cuda.cu:352)
   EMC_CHECK(hipMemcpyAsync(model->data, d_model, sizeof(float) * (size_t)S * S * S, hipMemcpyDeviceToHost, e.stream));
@@ -868,10 +873,10 @@ real cuda_model_max(real* model, int model_size) {
   float* out = e.d_red.as<float>();
   launch_fill(out, 1, 0.0f, e.stream);
   launch_model_max(model, model_size, out, e.stream);
-  float m = 0.0f;
-  EMC_CHECK(hipMemcpyAsync(&m, out, sizeof(float), hipMemcpyDeviceToHost, e.stream));
+  float* m = e.host_scalar<float>(5);
+  EMC_CHECK(hipMemcpyAsync(m, out, sizeof(float), hipMemcpyDeviceToHost, e.stream));
   sync();
-  return m;
+  return *m;
 }
 
 static void* alloc_dev(size_t bytes) {

@@ -5,6 +5,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
+#include <new>
 #include <vector>
 
 #include "emc.h"
@@ -23,6 +25,47 @@ bool failed();  // sticky error present
                                            hipGetErrorString(e_));                            \
   } while (0)
 #define EMC_LAUNCHED() EMC_CHECK(hipGetLastError())
+
+// ---- page-locked host memory for the library's own transfers -------------
+// A device->host copy into pageable memory is staged by the runtime copy by
+// copy, the host waiting for each; into page-locked memory the copies queue
+// on the stream and the host waits once.  HostVec is a std::vector in
+// page-locked memory (ordinary memory if page-locking fails).
+template <class T>
+struct PinnedAllocator {
+  using value_type = T;
+  static constexpr size_t kHead = 64;  // in front of each block: 1 page-locked, 0 malloc
+  PinnedAllocator() = default;
+  template <class U>
+  PinnedAllocator(const PinnedAllocator<U>&) {}
+  T* allocate(size_t n) {
+    const size_t bytes = kHead + n * sizeof(T);
+    char* p = nullptr;
+    if (hipHostMalloc((void**)&p, bytes, hipHostMallocDefault) == hipSuccess && p) {
+      *reinterpret_cast<int*>(p) = 1;
+    } else {
+      (void)hipGetLastError();
+      p = static_cast<char*>(std::malloc(bytes));
+      if (!p) throw std::bad_alloc();
+      *reinterpret_cast<int*>(p) = 0;
+    }
+    return reinterpret_cast<T*>(p + kHead);
+  }
+  void deallocate(T* q, size_t) {
+    if (!q) return;
+    char* p = reinterpret_cast<char*>(q) - kHead;
+    if (*reinterpret_cast<int*>(p) == 1)
+      (void)hipHostFree(p);
+    else
+      std::free(p);
+  }
+  template <class U>
+  bool operator==(const PinnedAllocator<U>&) const { return true; }
+  template <class U>
+  bool operator!=(const PinnedAllocator<U>&) const { return false; }
+};
+template <class T>
+using HostVec = std::vector<T, PinnedAllocator<T>>;
 
 // ---- device buffers owned by the engine ----------------------------------
 struct DBuf {
@@ -204,7 +247,8 @@ struct Engine {
   DBuf d_poff;       // int [R+1]: their offsets in the lists below
   DBuf d_pcol, d_pval;  // int / float [nnz]: pattern, P * operand scale (patterns ascending)
   long p_nnz = -1;      // nonzeros of the last M-step (-1: dense form)
-  std::vector<int> h_pnnz, h_poff;
+  HostVec<int> h_pnnz;       // read back from the device
+  std::vector<int> h_poff;   // uploaded (staged by the runtime: the vector may change right after)
   DBuf d_nz_keys, d_nz_keys_s, d_nz_vals, d_nz_vals_s, d_nz_tail, d_nz_tmp;  // nonzero lists from sources
   // What a responsibility buffer holds, as far as this library wrote it:
   // when valid, every nonzero of `ptr` ([R][N]) is at one of the nnz
@@ -267,6 +311,15 @@ struct Engine {
   std::vector<PhaseEvents> ev_pending;   // recorded pairs not yet added to phase_ms
   hipEvent_t next_event();
   void resolve_phase_times();
+
+  // page-locked slots for device->host reads of single values (slot k used
+  // by one call site; each is read after the stream is synchronised)
+  HostVec<double> h_scalars = HostVec<double>(16);
+  HostVec<float> h_T;  // per-rotation totals read back by the M-step
+  template <class T>
+  T* host_scalar(int k) {
+    return reinterpret_cast<T*>(h_scalars.data() + k);
+  }
 
   bool init();
   void release_all();

@@ -81,8 +81,9 @@ static void* pinned_take(size_t bytes) {
   return at;
 }
 
-template <class T>
-static bool upload(DBuf& d, const std::vector<T>& h, hipStream_t st) {
+template <class V>
+static bool upload(DBuf& d, const V& h, hipStream_t st) {
+  using T = typename V::value_type;
   if (!d.ensure(sizeof(T) * std::max<size_t>(h.size(), 1))) return false;
   if (h.empty()) return !failed();
   const size_t bytes = sizeof(T) * h.size();
@@ -97,6 +98,76 @@ static bool upload(DBuf& d, const std::vector<T>& h, hipStream_t st) {
   }
   return !failed();
 }
+
+// Several host arrays to their device buffers in one host->device copy:
+// each hipMemcpyAsync costs tens of microseconds of host time, during which
+// the GPU, waiting for these lists, idles.  The arrays go back to back into
+// the pinned arena, followed by a table of (destination, offset, bytes);
+// one copy moves the block to a staging buffer and k_scatter_segments puts
+// each array in place (stream order keeps the staging buffer's uses apart).
+struct UploadSeg {
+  void* dst;
+  unsigned long long src, bytes;
+};
+
+__global__ __launch_bounds__(256) void k_scatter_segments(const unsigned char* __restrict__ stage,
+                                                          unsigned long long table) {
+  const UploadSeg s = reinterpret_cast<const UploadSeg*>(stage + table)[blockIdx.x];
+  const unsigned* src = reinterpret_cast<const unsigned*>(stage + s.src);
+  unsigned* dst = reinterpret_cast<unsigned*>(s.dst);
+  const unsigned long long words = s.bytes / 4;
+  for (unsigned long long k = threadIdx.x; k < words; k += 256) dst[k] = src[k];
+  if (threadIdx.x < (s.bytes & 3))
+    reinterpret_cast<unsigned char*>(s.dst)[words * 4 + threadIdx.x] = stage[s.src + words * 4 + threadIdx.x];
+}
+
+class UploadBatch {
+ public:
+  template <class V>
+  bool add(DBuf& d, const V& h) {
+    using T = typename V::value_type;
+    if (!d.ensure(sizeof(T) * std::max<size_t>(h.size(), 1))) return false;
+    if (!h.empty()) items_.push_back({&d, h.data(), sizeof(T) * h.size()});
+    return true;
+  }
+  bool flush(hipStream_t st) {
+    if (items_.empty()) return !failed();
+    std::vector<UploadSeg> segs;
+    size_t total = 0;
+    for (const Item& it : items_) {
+      segs.push_back({it.d->p, total, it.bytes});
+      total += (it.bytes + 255) / 256 * 256;
+    }
+    const size_t table = total;
+    total += segs.size() * sizeof(UploadSeg);
+    DBuf& stage = proposal().d_stage;
+    char* pin = static_cast<char*>(pinned_take(total));
+    if (!pin || !stage.ensure(total)) {
+      // no pinned memory: one copy each from the callers' arrays, waited for
+      (void)hipGetLastError();
+      for (const Item& it : items_) EMC_CHECK(hipMemcpyAsync(it.d->p, it.src, it.bytes, hipMemcpyHostToDevice, st));
+      EMC_CHECK(hipStreamSynchronize(st));
+      items_.clear();
+      return !failed();
+    }
+    for (size_t k = 0; k < items_.size(); k++) std::memcpy(pin + segs[k].src, items_[k].src, items_[k].bytes);
+    std::memcpy(pin + table, segs.data(), segs.size() * sizeof(UploadSeg));
+    EMC_CHECK(hipMemcpyAsync(stage.p, pin, total, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_scatter_segments, dim3((unsigned)segs.size()), dim3(256), 0, st,
+                       stage.as<unsigned char>(), (unsigned long long)table);
+    EMC_LAUNCHED();
+    items_.clear();
+    return !failed();
+  }
+
+ private:
+  struct Item {
+    DBuf* d;
+    const void* src;
+    size_t bytes;
+  };
+  std::vector<Item> items_;
+};
 
 // Entries of each cell (patterns ascending), E-step work items, 32-entry
 // chunks of the M-step, candidate rotation count of each pattern, from the
@@ -164,11 +235,11 @@ static bool build_lists(Proposal& p) {
   p.n_seg = (int)seg_cell.size();
   p.n_mcell = (int)mcell.size();
   p.n_parts = parts;
-  if (!upload(p.d_ent_off, p.h_ent_off, st) || !upload(p.d_work, work, st) ||
-      !upload(p.d_chunk_off, p.h_chunk_off, st) || !upload(p.d_chunk_cell, chunk_cell, st) ||
-      !upload(p.d_seg_cell, seg_cell, st) || !upload(p.d_seg_chunk0, seg_chunk0, st) ||
-      !upload(p.d_seg_nch, seg_nch, st) || !upload(p.d_seg_part, seg_part, st) || !upload(p.d_mcell, mcell, st) ||
-      !upload(p.d_mcell_part, mcell_part, st))
+  UploadBatch ub;
+  if (!ub.add(p.d_ent_off, p.h_ent_off) || !ub.add(p.d_work, work) || !ub.add(p.d_chunk_off, p.h_chunk_off) ||
+      !ub.add(p.d_chunk_cell, chunk_cell) || !ub.add(p.d_seg_cell, seg_cell) || !ub.add(p.d_seg_chunk0, seg_chunk0) ||
+      !ub.add(p.d_seg_nch, seg_nch) || !ub.add(p.d_seg_part, seg_part) || !ub.add(p.d_mcell, mcell) ||
+      !ub.add(p.d_mcell_part, mcell_part) || !ub.flush(st))
     return false;
   const size_t nq = (size_t)std::max<long>(p.n_chunks, 1) * kSlot;
   if (!p.d_part_T.ensure(8 * nq) || !p.d_part_Tphi.ensure(8 * nq) || !p.d_part_max.ensure(4 * nq)) return false;
@@ -207,9 +278,11 @@ static bool build_groups(Proposal& p) {
   EMC_CHECK(hipMemcpyAsync(p.h_gcount.data(), p.d_gcount.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
   EMC_CHECK(hipMemcpyAsync(p.h_gncore.data(), p.d_gncore.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
   EMC_CHECK(hipMemcpyAsync(p.h_gnrot.data(), p.d_gnrot.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
-  EMC_CHECK(hipMemcpyAsync(&p.h_gtrunc, p.d_gtrunc.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  int* gtrunc = e.host_scalar<int>(0);
+  EMC_CHECK(hipMemcpyAsync(gtrunc, p.d_gtrunc.p, sizeof(int), hipMemcpyDeviceToHost, st));
   sync_stream();
   if (failed()) return false;
+  p.h_gtrunc = *gtrunc;
   p.st_core_trunc = p.h_gtrunc;
   // block work items: (first pattern in the sorted list, count, first core
   // rotation, count), 128 x 128
@@ -231,7 +304,9 @@ static bool build_groups(Proposal& p) {
   if (p.n_gwork == 0) return true;
   const int ncr = p.h_gcofs[C];
   const long n = (long)N * K;
-  if (!upload(p.d_gcofs, p.h_gcofs, st) || !upload(p.d_gwork, p.h_gwork, st) || !p.d_crot.ensure(sizeof(int) * ncr) ||
+  UploadBatch ub;
+  if (!ub.add(p.d_gcofs, p.h_gcofs) || !ub.add(p.d_gwork, p.h_gwork) || !ub.flush(st) ||
+      !p.d_crot.ensure(sizeof(int) * ncr) ||
       !p.d_ccell.ensure(ncr) || !p.d_cslot.ensure(ncr) || !p.d_rank.ensure((size_t)N * kMaxCore) ||
       !p.d_rkey.ensure(sizeof(int) * n) || !p.d_rcount.ensure(sizeof(int) * (C + 1)) ||
       !p.d_rncand.ensure(sizeof(int) * N) || !p.d_rent_pat.ensure(sizeof(int) * n) ||
@@ -249,7 +324,8 @@ static bool build_groups(Proposal& p) {
     return false;
   launch_counts_sorted(p.d_rkeys_s.as<int>(), n, C, p.d_rcount.as<int>(), st);
   p.h_rcount.resize(C + 1);
-  std::vector<int> rn(N);
+  HostVec<int>& rn = p.h_rn;
+  rn.resize(N);
   EMC_CHECK(hipMemcpyAsync(p.h_rcount.data(), p.d_rcount.p, sizeof(int) * (C + 1), hipMemcpyDeviceToHost, st));
   EMC_CHECK(hipMemcpyAsync(rn.data(), p.d_rncand.p, sizeof(int) * N, hipMemcpyDeviceToHost, st));
   sync_stream();
@@ -262,7 +338,7 @@ static bool build_groups(Proposal& p) {
   for (int c = 0; c < C; c++)
     for (int e0 = p.h_rent_off[c]; e0 < p.h_rent_off[c + 1]; e0 += kCandRows) rwork.push_back(make_int2(c, e0));
   p.n_rwork = (int)rwork.size();
-  if (!upload(p.d_rent_off, p.h_rent_off, st) || !upload(p.d_rwork, rwork, st)) return false;
+  if (!ub.add(p.d_rent_off, p.h_rent_off) || !ub.add(p.d_rwork, rwork) || !ub.flush(st)) return false;
   return !failed();
 }
 
@@ -332,6 +408,7 @@ static bool recentre(Proposal& p) {
   const size_t cb = ((size_t)p.C + 3) / 4 * 4;
   if (!p.d_mark.ensure(cb) || !p.d_shared.ensure(cb) || !p.d_moved.ensure(sizeof(int))) return false;
   p.h_shared.assign(cb, 0);
+  UploadBatch ub;
   if (p.max_shared > 0) {
     launch_mark_support(p.d_top.as<int>(), p.d_topm.as<float>(), N, K, Proposal::kShareMass, p.C,
                         p.d_mark.as<unsigned char>(), st);
@@ -352,12 +429,11 @@ static bool recentre(Proposal& p) {
       p.n_shc = (int)p.h_sh_cells.size();
       p.R_sh = (int)p.h_sh_rot.size();
       p.R_sh_pad = (int)round_up(std::max(p.R_sh, 1), kPadR);
-      if (!upload(p.d_sh_cells, p.h_sh_cells, st) || !upload(p.d_sh_off, p.h_sh_off, st) ||
-          !upload(p.d_sh_rot, p.h_sh_rot, st))
+      if (!ub.add(p.d_sh_cells, p.h_sh_cells) || !ub.add(p.d_sh_off, p.h_sh_off) || !ub.add(p.d_sh_rot, p.h_sh_rot))
         return false;
     }
   }
-  if (!upload(p.d_shared, p.h_shared, st)) return false;
+  if (!ub.add(p.d_shared, p.h_shared) || !ub.flush(st)) return false;
   // every pattern's candidates, re-centred (GPU), then the lists if any changed
   launch_fill(p.d_moved.as<float>(), 1, 0.0f, st);
   launch_recentre_cand(p.d_top.as<int>(), p.d_topm.as<float>(), N, K, p.d_nbr.as<int>(), Proposal::kNbr,
@@ -417,7 +493,8 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
   }
   p.n_sub = (int)p.h_sub.size();
   p.n_sub_pad = (int)round_up(std::max(p.n_sub, 1), kPadN);
-  if (!upload(p.d_sub, p.h_sub, st) || !upload(p.d_insub, p.h_insub, st)) return false;
+  UploadBatch ub;
+  if (!ub.add(p.d_sub, p.h_sub) || !ub.add(p.d_insub, p.h_insub) || !ub.flush(st)) return false;
   if (!p.d_zero.ensure(2 * (size_t)e.n_mp) || !e.d_phiden.ensure(sizeof(double) * N)) return false;
   launch_fill(p.d_zero.as<float>(), (size_t)e.n_mp / 2, 0.0f, st);
   const bool split = !e.k_exact;
@@ -471,11 +548,14 @@ bool proposal_estep(int N, int R, float sigma, const float* phi_used, bool* dens
     launch_sub_finish(p.d_Lsub.as<float>(), ns, p.d_sub.as<int>(), p.d_phiden_sub.as<double>(),
                       e.d_phiden.as<double>(), p.d_cand.as<int>(), p.topk, p.d_cell_off.as<int>(),
                       p.d_cell_rot.as<int>(), p.d_sh_rot.as<int>(), p.R_sh, p.d_mass.as<float>(), st);
-    std::vector<float> mass(ns);
-    EMC_CHECK(hipMemcpyAsync(&p.sub_total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, st));
+    HostVec<float>& mass = p.h_mass;
+    mass.resize(ns);
+    double* sub_total = e.host_scalar<double>(1);
+    EMC_CHECK(hipMemcpyAsync(sub_total, e.d_red.as<double>() + nb, sizeof(double), hipMemcpyDeviceToHost, st));
     EMC_CHECK(hipMemcpyAsync(mass.data(), p.d_mass.p, sizeof(float) * ns, hipMemcpyDeviceToHost, st));
     sync_stream();
     if (failed()) return false;
+    p.sub_total = *sub_total;
     // search mode of every pattern scored on the whole table: full search
     // while its candidates hold less than min_mass of its posterior
     double fails = 0, mn = 1.0, sum = 0.0;
@@ -676,10 +756,11 @@ bool proposal_softmax(float* respons, int N, int R, double sigma2x, double* tota
     PhaseTimer pt(PH_AUDIT);
     launch_sub_scatter(p.d_Lsub.as<float>(), R, p.n_sub, p.d_sub.as<int>(), N, respons, false, st);
   }
-  double tot = 0.0;
-  EMC_CHECK(hipMemcpyAsync(&tot, p.d_tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  double* h_tot = e.host_scalar<double>(2);
+  EMC_CHECK(hipMemcpyAsync(h_tot, p.d_tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
   sync_stream();
   if (failed()) return false;
+  const double tot = *h_tot;
   double dtot = 0.0;
   e.ref_entries = 0;
   if (e.refine && e.like.slices && e.like.images) {

@@ -31,7 +31,7 @@ struct Proposal {
   int iteration = 0;       // candidate-restricted E-steps run
 
   // coarse cells: fine rotations of each cell (CSR), nearest cells of each
-  std::vector<int> h_cell_off, h_cell_rot, h_nbr;
+  std::vector<int> h_cell_off, h_cell_rot, h_nbr;  // set-up only
   static constexpr int kNbr = 12;
   static constexpr int kTop = 4;     // posterior cells per pattern used to re-centre
   DBuf d_net_cand;                   // the network's candidates [N][topk]
@@ -39,9 +39,9 @@ struct Proposal {
   DBuf d_mark, d_shared;             // [C] support marks, shared cells (bytes)
   DBuf d_count, d_ncand, d_moved;    // entries per cell [C+1], rotations per pattern [N], changed lists
   DBuf d_keys_s, d_vals_s, d_sort_tmp;  // cell-sorted entries and the sort's scratch
-  std::vector<int> h_count, h_colmap;  // entries per cell; subset index of each pattern (-1: none)
+  HostVec<int> h_count, h_colmap;  // entries per cell; subset index of each pattern (-1: none)
   DBuf d_colmap;
-  std::vector<unsigned char> h_mark;
+  HostVec<unsigned char> h_mark;
   DBuf d_top, d_topm;                // [N][kTop] cells, masses of the last posterior
   // where the last posterior is: none, Lc / Lsub (candidate iteration), or
   // already reduced into d_top / d_topm (whole-table iteration of post_N patterns)
@@ -54,23 +54,24 @@ struct Proposal {
   // by the dense kernel, and the candidate cells exclude them.
   int max_shared = 512;
   static constexpr float kShareMass = 0.01f;
-  std::vector<int> h_sh_cells, h_sh_off, h_sh_rot;  // cells, CSR of their rotations
-  std::vector<unsigned char> h_shared;              // [C]
+  HostVec<int> h_sh_cells, h_sh_off, h_sh_rot;  // cells, CSR of their rotations
+  HostVec<unsigned char> h_shared;              // [C]
   int n_shc = 0, R_sh = 0, R_sh_pad = 0;
   DBuf d_sh_cells, d_sh_off, d_sh_rot, d_Ssh, d_Sshlo, d_Ash, d_sscale_sh, d_Lsh, d_top_s, d_topm_s;
   void clear_shared() { h_sh_cells.clear(); h_sh_off.assign(1, 0); h_sh_rot.clear(); n_shc = R_sh = R_sh_pad = 0; }
   DBuf d_iota;                       // 0..max(N, R_sh)-1
+  DBuf d_stage;                      // staging of the batched list uploads
   DBuf d_part_c, d_part_m;           // top-cell partials of the subset columns
   DBuf d_cell_off, d_cell_rot;
   // per pattern: candidate rotation count, network top-k mass, search mode
   // (the candidate cells [N][topk] live in d_cand)
-  std::vector<int> h_ncand;
-  std::vector<float> h_pmass;
-  std::vector<unsigned char> h_mode;  // 1 = whole table
+  HostVec<int> h_ncand;
+  HostVec<float> h_pmass;
+  HostVec<unsigned char> h_mode;  // 1 = whole table
   DBuf d_cand, d_pmass;
   // entries per cell (CSR, patterns ascending) and E-step work items
-  std::vector<int> h_ent_off;
-  std::vector<long> h_chunk_off;
+  HostVec<int> h_ent_off;
+  HostVec<long> h_chunk_off;
   DBuf d_ent_off, d_ent_pat, d_ent_j, d_work, d_chunk_off, d_chunk_cell;
   int n_work = 0;
   long n_entries = 0, n_chunks = 0;
@@ -86,8 +87,9 @@ struct Proposal {
   int core_div = 4;                  // core cell: held by >= 1 / core_div of its group's patterns
   DBuf d_gkey, d_gcount, d_gpat, d_gkey_s, d_ghist, d_gcore, d_gncore, d_gnrot, d_gcofs, d_crot, d_ccell, d_cslot,
       d_gtrunc, d_rank, d_rkey, d_gwork, d_rcount, d_rncand;
-  std::vector<int> h_gcount, h_gncore, h_gnrot, h_gcofs, h_rcount, h_rent_off;
-  std::vector<int4> h_gwork;
+  HostVec<int> h_gcount, h_gncore, h_gnrot, h_gcofs, h_rcount, h_rent_off, h_rn;
+  HostVec<float> h_mass;
+  HostVec<int4> h_gwork;
   int n_gwork = 0;
   double st_group_pairs = 0;         // (pattern, rotation) pairs of the group blocks
   double st_resid_pairs = 0;         // candidate pairs left to the per-cell kernel
@@ -104,8 +106,8 @@ struct Proposal {
   DBuf d_plogp;   // double [N]
   DBuf d_zero;    // zero halves (gather target of padding rows)
   // whole-table subset of this iteration (audited + full search)
-  std::vector<int> h_sub;
-  std::vector<unsigned char> h_insub, h_audit;
+  HostVec<int> h_sub;
+  HostVec<unsigned char> h_insub, h_audit;
   int n_sub = 0, n_sub_pad = 0;
   DBuf d_sub, d_insub, d_Ksub, d_Ksublo, d_Ktsub, d_Ktsublo, d_Bsub, d_phisub, d_invphisub, d_Lsub, d_phiden_sub,
       d_Psub16, d_Psub16lo, d_mass, d_tot;
