@@ -71,7 +71,8 @@ bool Cnn::init(int side_, int ncell_, int batch_, uint64_t seed) {
     const long m = B * oh * oh;
     // layers whose input has a multiple of 8 channels read it through the
     // GEMM's implicit im2col; only the others keep an im2col buffer
-    if ((!implicit(l) && !col[l].ensure(4 * m * 9 * ic[l])) || !act[l].ensure(4 * m * oc[l])) return false;
+    const bool need_col = !implicit(l) && !(l == 0 && direct0());
+    if ((need_col && !col[l].ensure(4 * m * 9 * ic[l])) || !act[l].ensure(4 * m * oc[l])) return false;
     maxact = std::max(maxact, m * oc[l]);
     maxw = std::max(maxw, 9L * oc[l] * ic[l]);
   }
@@ -100,6 +101,11 @@ void Cnn::forward(const float* images, const int* mask, int N2d, const int* d_id
   const float* in = x.as<float>();
   for (int l = 0; l < kConvs; l++) {
     const int H = hin[l], oh = H / 2, K = 9 * ic[l];
+    if (l == 0 && direct0()) {
+      conv0_forward(in, B, H, W(0), Bv(0), act[0].as<float>(), st);
+      in = act[0].as<float>();
+      continue;
+    }
     GemmDesc d;
     if (implicit(l)) {
       d.A = in; d.conv_a = 1; d.cH = H; d.cW = H; d.cC = ic[l]; d.a_len = floats(act[l - 1]);
@@ -176,6 +182,11 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
     const int H = hin[l], oh = H / 2, K = 9 * ic[l], OC = oc[l];
     const long M = (long)B * oh * oh;
     float* dz = dbufs[cur];  // already gated by act[l] > 0
+    if (l == 0 && direct0()) {
+      // partial sums in the GEMMs' split-k scratch (free here: stream order)
+      conv0_wgrad(dz, x.as<float>(), B, H, dW(0), dB(0), reinterpret_cast<double*>(sc), scratch_floats / 2, st);
+      continue;
+    }
     GemmDesc d;  // dW[oc][kk] = sum_m dz[m][oc] col[m][kk]
     d.A = dz; d.sam = 1; d.sak = OC; d.a_len = half;
     if (implicit(l)) {

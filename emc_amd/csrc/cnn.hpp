@@ -50,6 +50,12 @@ struct GemmDesc {
 bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st);
 void cnn_input(const float* images, const int* mask, int N2d, const int* idx, int B, float* x, hipStream_t st);
 void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t st);
+// the first convolution when it has 1 input and 16 output channels: direct
+// forward (bias, ReLU) into out [B][H/2][H/2][16], and its weight and bias
+// gradients from dz [B][H/2][H/2][16] (scratch: partial sums)
+void conv0_forward(const float* in, int B, int H, const float* w, const float* bias, float* out, hipStream_t st);
+bool conv0_wgrad(const float* dz, const float* in, int B, int H, float* dW, float* dB, double* scratch,
+                 size_t scratch_doubles, hipStream_t st);
 // wt[(t*OC + o)*IC + c] = w[(o*9 + t)*IC + c]: conv weights [OC][3][3][IC]
 // as [3][3][OC][IC], the B operand of the transposed convolution
 void conv_weights_t(const float* w, int OC, int IC, float* wt, hipStream_t st);
@@ -98,6 +104,7 @@ struct Cnn {
   // conv l reads its input through the GEMM's implicit im2col (input
   // channels a multiple of 8: every layer but the first)
   bool implicit(int l) const { return l > 0 && ic[l] % 8 == 0; }
+  bool direct0() const { return ic[0] == 1 && oc[0] == 16; }  // first convolution by conv0_forward / conv0_wgrad
 };
 
 }  // namespace emc

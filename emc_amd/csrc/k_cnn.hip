@@ -362,6 +362,100 @@ __global__ void k_im2col(const float* __restrict__ in, int B, int H, int W, int 
   col[t] = (y >= 0 && y < H && x >= 0 && x < W) ? in[(((long)b * H + y) * W + x) * C + c] : 0.0f;
 }
 
+// First convolution (one input channel, kC0 output channels, 3 x 3, stride
+// 2, padding 1) computed directly: its 9-long sums are too short for a
+// matrix-core tile (a 64 x 64 tile with k padded to 32 wasted 3/4 of the
+// columns and 2/3 of k).  Forward: a thread per output pixel gathers its 9
+// inputs and writes its kC0 channels (bias, ReLU) as one 64-byte row.  The
+// operands are rounded to BF16 as the GEMM rounds them (FP32 products and
+// sums), so every layer computes the same function.
+constexpr int kC0 = 16;
+
+__device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
+
+__global__ __launch_bounds__(256) void k_conv0_fwd(const float* __restrict__ in, int B, int H,
+                                                   const float* __restrict__ w, const float* __restrict__ bias,
+                                                   float* __restrict__ out) {
+  const int OH = H / 2;
+  const long m = (long)blockIdx.x * 256 + threadIdx.x;
+  if (m >= (long)B * OH * OH) return;
+  const int ox = (int)(m % OH), oy = (int)((m / OH) % OH);
+  const long b = m / ((long)OH * OH);
+  float v[9];
+#pragma unroll
+  for (int t = 0; t < 9; t++) {
+    const int y = 2 * oy + t / 3 - 1, x = 2 * ox + t % 3 - 1;
+    v[t] = (y >= 0 && y < H && x >= 0 && x < H) ? bf16_round(in[(b * H + y) * H + x]) : 0.0f;
+  }
+  float o[kC0];
+#pragma unroll
+  for (int c = 0; c < kC0; c++) {
+    float s = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 9; t++) s += v[t] * bf16_round(w[c * 9 + t]);
+    o[c] = fmaxf(s + bias[c], 0.0f);
+  }
+  float4* dst = reinterpret_cast<float4*>(out + m * kC0);
+#pragma unroll
+  for (int q = 0; q < kC0 / 4; q++) dst[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+}
+
+// Its weight and bias gradients: dW[c][t] = sum_m dz[m][c] v_t(m) and
+// dB[c] = sum_m dz[m][c] over the output pixels m.  A workgroup takes `per`
+// consecutive pixels; thread (pixel lane t / 16, channel t % 16) takes
+// every 16th of them (a wave reads 4 pixels x 16 channels, 256 contiguous
+// bytes of dz), the 16 lanes of a channel are added in order into the
+// workgroup's partial sums, and k_conv0_wgrad_fin adds the workgroups' in
+// order.
+__global__ __launch_bounds__(256) void k_conv0_wgrad(const float* __restrict__ dz, const float* __restrict__ in,
+                                                     int B, int H, long per, double* __restrict__ part) {
+  static_assert(kC0 == 16, "one channel per lane of 16");
+  __shared__ float red[10][256];
+  const int OH = H / 2;
+  const long M = (long)B * OH * OH;
+  const int c = threadIdx.x & 15, pl = threadIdx.x >> 4;
+  const long m0 = (long)blockIdx.x * per, m1 = min(M, m0 + per);
+  float acc[10];
+#pragma unroll
+  for (int k = 0; k < 10; k++) acc[k] = 0.0f;
+  for (long m = m0 + pl; m < m1; m += 16) {
+    const float gf = dz[m * kC0 + c], g = bf16_round(gf);
+    const int ox = (int)(m % OH), oy = (int)((m / OH) % OH);
+    const long b = m / ((long)OH * OH);
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+      const int y = 2 * oy + t / 3 - 1, x = 2 * ox + t % 3 - 1;
+      const float v = (y >= 0 && y < H && x >= 0 && x < H) ? bf16_round(in[(b * H + y) * H + x]) : 0.0f;
+      acc[t] += g * v;
+    }
+    acc[9] += gf;  // the bias gradient sums dz itself (FP32, as the column sums do)
+  }
+#pragma unroll
+  for (int k = 0; k < 10; k++) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 10 * kC0) {
+    const int k = threadIdx.x / kC0, cc = threadIdx.x % kC0;
+    double s = 0.0;
+    for (int p = 0; p < 16; p++) s += (double)red[k][p * 16 + cc];
+    part[(long)blockIdx.x * 10 * kC0 + cc * 10 + k] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_conv0_wgrad_fin(const double* __restrict__ part, int nblk,
+                                                         float* __restrict__ dW, float* __restrict__ dB) {
+  const int j = blockIdx.x;  // channel j / 10; taps 0..8, then the bias
+  double s = 0.0;
+  for (int sp = threadIdx.x; sp < nblk; sp += 256) s += part[(long)sp * 10 * kC0 + j];
+  s = block_sum_f64(s);
+  if (threadIdx.x == 0) {
+    const int c = j / 10, k = j % 10;
+    if (k < 9)
+      dW[c * 9 + k] = (float)s;
+    else
+      dB[c] = (float)s;
+  }
+}
+
 // wt[(t*OC + o)*IC + c] = w[(o*9 + t)*IC + c]
 __global__ void k_conv_weights_t(const float* __restrict__ w, int OC, int IC, float* __restrict__ wt) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -382,6 +476,39 @@ __global__ void k_reduce_rows_part(const float* __restrict__ G, long rows, int c
   for (long i = r0 + threadIdx.x; i < r1; i += 256) s += G[i * si + (long)j * sj];
   s = block_sum_f64(s);
   if (threadIdx.x == 0) part[(long)sp * cols + j] = s;
+}
+
+// The same for a row-major [rows][cols] matrix whose row length divides 256
+// (the bias gradients of the convolutions and of the first dense layer):
+// a workgroup takes whole rows, its threads read consecutive elements (one
+// contiguous 1 KB per step), and thread t always meets column t % cols; the
+// threads of a column are then added in thread order.
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ G, long rows, int cols, long per,
+                                                     double* __restrict__ part) {
+  __shared__ double red[256];
+  const int sp = blockIdx.x;
+  const long e0 = (long)sp * per * cols, e1 = min(rows, (long)(sp + 1) * per) * cols;
+  double s = 0.0;
+  for (long e = e0 + threadIdx.x; e < e1; e += 256) s += G[e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if ((int)threadIdx.x < cols) {
+    double t = 0.0;
+    for (int k = threadIdx.x; k < 256; k += cols) t += red[k];
+    part[(long)sp * cols + threadIdx.x] = t;
+  }
+}
+
+// The partial sums of one column added by one workgroup: thread t takes
+// splits t, t + 256, ... in order, then the threads are added in a fixed
+// tree.
+__global__ __launch_bounds__(256) void k_colsum_fin(const double* __restrict__ part, int nsplit, int cols,
+                                                    float* __restrict__ out) {
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (int sp = threadIdx.x; sp < nsplit; sp += 256) s += part[(long)sp * cols + j];
+  s = block_sum_f64(s);
+  if (threadIdx.x == 0) out[j] = (float)s;
 }
 
 __global__ void k_reduce_rows_fin(const double* __restrict__ part, int nsplit, int cols, float* __restrict__ out) {
@@ -526,12 +653,48 @@ void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t
   hipLaunchKernelGGL(k_im2col, dim3(nblk((long)B * (H / 2) * (W / 2) * 9 * C)), dim3(256), 0, st, in, B, H, W, C, col);
   EMC_LAUNCHED();
 }
+void conv0_forward(const float* in, int B, int H, const float* w, const float* bias, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_conv0_fwd, dim3(nblk((long)B * (H / 2) * (H / 2))), dim3(256), 0, st, in, B, H, w, bias, out);
+  EMC_LAUNCHED();
+}
+bool conv0_wgrad(const float* dz, const float* in, int B, int H, float* dW, float* dB, double* scratch,
+                 size_t scratch_doubles, hipStream_t st) {
+  const long M = (long)B * (H / 2) * (H / 2);
+  const long maxblk = (long)(scratch_doubles / (10 * kC0));
+  if (M <= 0 || maxblk < 1) {
+    set_error("conv0_wgrad: no pixels or no scratch");
+    return false;
+  }
+  long per = 1024;  // 64 pixels per thread, thousands of workgroups
+  if ((M + per - 1) / per > maxblk) per = (M + maxblk - 1) / maxblk;
+  const long nb = (M + per - 1) / per;
+  hipLaunchKernelGGL(k_conv0_wgrad, dim3((unsigned)nb), dim3(256), 0, st, dz, in, B, H, per, scratch);
+  EMC_LAUNCHED();
+  hipLaunchKernelGGL(k_conv0_wgrad_fin, dim3(10 * kC0), dim3(256), 0, st, scratch, (int)nb, dW, dB);
+  EMC_LAUNCHED();
+  return !failed();
+}
 void conv_weights_t(const float* w, int OC, int IC, float* wt, hipStream_t st) {
   hipLaunchKernelGGL(k_conv_weights_t, dim3(nblk(9L * OC * IC)), dim3(256), 0, st, w, OC, IC, wt);
   EMC_LAUNCHED();
 }
 void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, double* scratch,
                  size_t scratch_doubles, hipStream_t st) {
+  if (sj == 1 && si == cols && cols > 0 && 256 % cols == 0 && rows > 0) {
+    // row-major with rows of a length dividing 256: contiguous reads
+    long per = std::max<long>(1, 65536 / cols);  // ~64K elements per workgroup
+    long nsplit = (rows + per - 1) / per;
+    if (nsplit * cols > (long)scratch_doubles) {
+      nsplit = std::max<long>(1, (long)scratch_doubles / cols);
+      per = (rows + nsplit - 1) / nsplit;
+      nsplit = (rows + per - 1) / per;
+    }
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)nsplit), dim3(256), 0, st, G, rows, cols, per, scratch);
+    EMC_LAUNCHED();
+    hipLaunchKernelGGL(k_colsum_fin, dim3(cols), dim3(256), 0, st, scratch, (int)nsplit, cols, out);
+    EMC_LAUNCHED();
+    return;
+  }
   long nsplit = std::max<long>(1, std::min<long>((rows + 8191) / 8192, (long)(scratch_doubles / std::max(cols, 1))));
   nsplit = std::min<long>(nsplit, 65535);
   const long per = (rows + nsplit - 1) / nsplit;
