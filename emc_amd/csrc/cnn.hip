@@ -203,14 +203,19 @@ void Cnn::backward_and_step(int B, float lr, hipStream_t st) {
       // dx[b][y][x][c] = sum over taps and oc of dz at the output pixel the
       // tap covers x W[oc][tap][c], gated by act[l - 1] > 0 (the previous
       // layer's ReLU')
-      conv_weights_t(W(l), OC, ic[l], wt.as<float>(), st);
-      GemmDesc e;
-      e.A = dz; e.conv_t = 1; e.cH = H; e.cW = H; e.cC = OC; e.a_len = half;
-      e.B = wt.as<float>(); e.sbn = 1; e.sbk = ic[l]; e.b_len = floats(wt);
-      e.C = dbufs[cur ^ 1]; e.scm = ic[l]; e.scn = 1; e.c_len = half;
-      e.gate = act[l - 1].as<float>();
-      e.M = B * H * H; e.N = ic[l]; e.K = 9 * OC;
-      gemm_bf16(e, sc, scratch_floats, st);
+      // by the parity classes of the input pixels: each class has 1, 2 or 4
+      // valid taps of the 9 (the rest land between output pixels)
+      conv_weights_tc(W(l), OC, ic[l], wt.as<float>(), st);
+      for (int q = 0; q < 4; q++) {
+        GemmDesc e;
+        e.A = dz; e.conv_t = 1; e.tcls = q; e.cH = H; e.cW = H; e.cC = OC; e.a_len = half;
+        e.B = wt.as<float>() + (long)kTapClassOff[q] * OC * ic[l]; e.sbn = 1; e.sbk = ic[l];
+        e.b_len = (long)kTapClassN[q] * OC * ic[l];
+        e.C = dbufs[cur ^ 1]; e.scm = ic[l]; e.scn = 1; e.c_len = half;
+        e.gate = act[l - 1].as<float>();
+        e.M = B * oh * oh; e.N = ic[l]; e.K = kTapClassN[q] * OC;
+        gemm_bf16(e, sc, scratch_floats, st);
+      }
       cur ^= 1;
     }
   }

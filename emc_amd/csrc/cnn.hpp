@@ -40,6 +40,12 @@ struct GemmDesc {
   // ([batch][cH/2][cW/2][cC]) at 2oy = y + 1 - ky, 2ox = x + 1 - kx, 0 where
   // those are odd or outside.
   int conv_a = 0, conv_b = 0, conv_t = 0, cH = 0, cW = 0, cC = 0;
+  // tcls >= 0 (with conv_t): one parity class (py, px) = (tcls >> 1,
+  // tcls & 1) of the input pixels, whose valid taps are ky = 1 (py = 0) or
+  // 0, 2 (py = 1) and likewise kx: rows m = (b*cH/2 + yy)*cW/2 + xx are the
+  // pixels (2yy + py, 2xx + px), k = j*cC + oc runs over the class's taps j
+  // (ky-major), and C's row of m is that pixel's (b*cH + y)*cW + x.
+  int tcls = -1;
   // gate (optional, C's layout): outputs where gate <= 0 are 0 (the ReLU's
   // backward of the activation gate holds)
   const float* gate = nullptr;
@@ -59,6 +65,11 @@ bool conv0_wgrad(const float* dz, const float* in, int B, int H, float* dW, floa
 // wt[(t*OC + o)*IC + c] = w[(o*9 + t)*IC + c]: conv weights [OC][3][3][IC]
 // as [3][3][OC][IC], the B operand of the transposed convolution
 void conv_weights_t(const float* w, int OC, int IC, float* wt, hipStream_t st);
+// the same grouped by the parity classes of the input pixels: class q's
+// taps (kTapClassN[q] of them, ky-major) from kTapClassOff[q] * OC * IC on
+constexpr int kTapClassN[4] = {1, 2, 2, 4};
+constexpr int kTapClassOff[4] = {0, 1, 3, 5};
+void conv_weights_tc(const float* w, int OC, int IC, float* wtc, hipStream_t st);
 // out[j] = sum over rows i of G[i*si + j*sj]; scratch holds the per-split
 // partial sums (cols * splits doubles)
 void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, double* scratch,

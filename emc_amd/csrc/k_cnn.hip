@@ -11,6 +11,8 @@
 // master weights).
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <algorithm>
 
 #include "engine.hpp"
@@ -41,8 +43,18 @@ struct OpView {
   const float* p;
   long sr, sk;  // element (row, k) at p[row * sr + k * sk]
   int rows;
-  int H, W, C;  // convolution input (kConvMK / kConvKM)
+  int H, W, C;   // convolution input (kConvMK / kConvKM)
+  int tcls = -1;  // kConvT: parity class of the rows (-1: all pixels, all 9 taps)
 };
+
+// Tap (ky, kx) of local tap j of parity class (py, px): ky = 1 when py = 0,
+// else 0 or 2 (likewise kx), ky-major.
+__device__ __forceinline__ void class_tap(int py, int px, int j, int& ky, int& kx) {
+  const int nkx = px ? 2 : 1;
+  const int jy = j / nkx, jx = j - jy * nkx;
+  ky = py ? 2 * jy : 1;
+  kx = px ? 2 * jx : 1;
+}
 
 // Element (m, kk) of the implicit im2col matrix of a 3x3 / stride-2 / pad-1
 // convolution over NHWC [b][H][W][C]: its address, or nullptr for a tap in
@@ -62,6 +74,16 @@ __device__ __forceinline__ const float* conv_at(const OpView& v, int m, int kk) 
 // (y, x), or nullptr where none does.
 __device__ __forceinline__ const float* convt_at(const OpView& v, int m, int k) {
   const int OH = v.H >> 1, OW = v.W >> 1;
+  if (v.tcls >= 0) {  // pixel (2yy + py, 2xx + px), the class's taps only
+    const int py = v.tcls >> 1, px = v.tcls & 1;
+    const int xx = m % OW, t = m / OW, yy = t % OH, b = t / OH;
+    const int oc = k % v.C;
+    int ky, kx;
+    class_tap(py, px, k / v.C, ky, kx);
+    const int oy = yy + ((py + 1 - ky) >> 1), ox = xx + ((px + 1 - kx) >> 1);
+    if (oy >= OH || ox >= OW) return nullptr;
+    return v.p + (((size_t)b * OH + oy) * OW + ox) * v.C + oc;
+  }
   const int x = m % v.W, t = m / v.W, y = t % v.H, b = t / v.H;
   const int oc = k % v.C, tap = k / v.C, kx = tap % 3, ky = tap / 3;
   const int ty = y + 1 - ky, tx = x + 1 - kx;
@@ -167,6 +189,14 @@ __device__ __forceinline__ void store8(__bf16 (*s)[GLD], int t, bool kf_strided,
   }
 }
 
+// C's row of output row m (the pixel's row for a parity class)
+__device__ __forceinline__ long c_row(const GemmDesc& d, int m) {
+  if (!d.conv_t || d.tcls < 0) return m;
+  const int OH = d.cH >> 1, OW = d.cW >> 1;
+  const int xx = m % OW, t = m / OW, yy = t % OH, b = t / OH;
+  return ((long)b * d.cH + 2 * yy + (d.tcls >> 1)) * d.cW + 2 * xx + (d.tcls & 1);
+}
+
 template <int AM, int BM>
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmDesc d, float* __restrict__ partial, int ksplit_len) {
   __shared__ __attribute__((aligned(16))) __bf16 sA[2][GT][GLD];
@@ -174,7 +204,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmDesc d, float* __restrict
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
   const int kz0 = blockIdx.z * ksplit_len, kz1 = min(d.K, kz0 + ksplit_len);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
-  const OpView va{d.A, d.sam, d.sak, d.M, d.cH, d.cW, d.cC}, vb{d.B, d.sbn, d.sbk, d.N, d.cH, d.cW, d.cC};
+  OpView va{d.A, d.sam, d.sak, d.M, d.cH, d.cW, d.cC}, vb{d.B, d.sbn, d.sbk, d.N, d.cH, d.cW, d.cC};
+  va.tcls = d.conv_t ? d.tcls : -1;
   const bool a_kf = d.sak == 1, b_kf = d.sbk == 1;
   floatx4 acc[2][2];
 #pragma unroll
@@ -229,8 +260,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmDesc d, float* __restrict
           if (d.bias_n) v += d.bias_n[n];
           if (d.bias_m) v += d.bias_m[m];
           if (d.relu) v = fmaxf(v, 0.0f);
-          if (d.gate && !(d.gate[(long)m * d.scm + (long)n * d.scn] > 0.0f)) v = 0.0f;
-          float* c = d.C + (long)m * d.scm + (long)n * d.scn;
+          const long cm = c_row(d, m);
+          if (d.gate && !(d.gate[cm * d.scm + (long)n * d.scn] > 0.0f)) v = 0.0f;
+          float* c = d.C + cm * d.scm + (long)n * d.scn;
           *c = d.accumulate ? *c + v : v;
         }
       }
@@ -245,8 +277,9 @@ __global__ void k_gemm_finish(GemmDesc d, const float* __restrict__ partial, int
   if (d.bias_n) v += d.bias_n[n];
   if (d.bias_m) v += d.bias_m[m];
   if (d.relu) v = fmaxf(v, 0.0f);
-  if (d.gate && !(d.gate[(long)m * d.scm + (long)n * d.scn] > 0.0f)) v = 0.0f;
-  float* c = d.C + (long)m * d.scm + (long)n * d.scn;
+  const long cm = c_row(d, m);
+  if (d.gate && !(d.gate[cm * d.scm + (long)n * d.scn] > 0.0f)) v = 0.0f;
+  float* c = d.C + cm * d.scm + (long)n * d.scn;
   *c = d.accumulate ? *c + v : v;
 }
 
@@ -281,12 +314,24 @@ static long conv_last(const GemmDesc& d, long positions, long taps) {
 // gradient of the batch whose input pixels are the M rows), or -2 when the
 // shape is invalid.
 static long convt_last(const GemmDesc& d) {
-  if (d.cH < 2 || d.cW < 2 || d.cH % 2 || d.cW % 2 || d.cC < 8 || d.cC % 8 || d.K != 9 * d.cC ||
+  const int taps = d.tcls < 0 ? 9 : d.tcls < 4 ? kTapClassN[d.tcls] : 0;
+  if (d.cH < 2 || d.cW < 2 || d.cH % 2 || d.cW % 2 || d.cC < 8 || d.cC % 8 || taps == 0 || d.K != taps * d.cC ||
       ((uintptr_t)d.A & 15))
     return -2;
-  const long per = (long)d.cH * d.cW;
+  const long per = d.tcls < 0 ? (long)d.cH * d.cW : (long)(d.cH / 2) * (d.cW / 2);
   if (d.M % per) return -2;
   return d.M / per * (d.cH / 2) * (d.cW / 2) * d.cC - 1;
+}
+
+// Highest element of C written: the last row's, or for a parity class the
+// batch's last pixel row.
+static long c_last(const GemmDesc& d) {
+  if (d.conv_t && d.tcls >= 0) {
+    const long per = (long)(d.cH / 2) * (d.cW / 2);
+    if (per <= 0 || d.M % per) return LONG_MAX;
+    return last_index(d.M / per * d.cH * d.cW, d.N, d.scm, d.scn);
+  }
+  return last_index(d.M, d.N, d.scm, d.scn);
 }
 
 bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStream_t st) {
@@ -295,9 +340,9 @@ bool gemm_bf16(const GemmDesc& d, float* scratch, size_t scratch_floats, hipStre
   const long lb = d.conv_b ? conv_last(d, d.K, d.N) : last_index(d.N, d.K, d.sbn, d.sbk);
   if (d.K < 0 || d.sam < 0 || d.sak < 0 || d.sbn < 0 || d.sbk < 0 || d.scm < 0 || d.scn < 0 || la < -1 ||
       lb < -1 || (d.conv_a + d.conv_b + d.conv_t > 1) || la >= d.a_len || lb >= d.b_len ||
-      last_index(d.M, d.N, d.scm, d.scn) >= d.c_len || !d.A || !d.B || !d.C) {
+      c_last(d) >= d.c_len || !d.A || !d.B || !d.C) {
     set_error("gemm_bf16: descriptor out of bounds (M %d N %d K %d; A %ld/%ld, B %ld/%ld, C %ld/%ld)", d.M, d.N, d.K,
-              la, d.a_len, lb, d.b_len, last_index(d.M, d.N, d.scm, d.scn), d.c_len);
+              la, d.a_len, lb, d.b_len, c_last(d), d.c_len);
     return false;
   }
   const int gm = (d.M + GT - 1) / GT, gn = (d.N + GT - 1) / GT;
@@ -673,6 +718,22 @@ bool conv0_wgrad(const float* dz, const float* in, int B, int H, float* dW, floa
   hipLaunchKernelGGL(k_conv0_wgrad_fin, dim3(10 * kC0), dim3(256), 0, st, scratch, (int)nb, dW, dB);
   EMC_LAUNCHED();
   return !failed();
+}
+// wtc by parity class: slot s (0..8) is class q's local tap j (slots 0 | 1 2
+// | 3 4 | 5 6 7 8 for q = 0 .. 3), element (s*OC + o)*IC + c
+__global__ void k_conv_weights_tc(const float* __restrict__ w, int OC, int IC, float* __restrict__ wtc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= 9 * OC * IC) return;
+  const int c = i % IC, o = (i / IC) % OC, s = i / (IC * OC);
+  const int q = s == 0 ? 0 : s < 3 ? 1 : s < 5 ? 2 : 3;
+  const int j = s - (q == 0 ? 0 : q == 1 ? 1 : q == 2 ? 3 : 5);
+  int ky, kx;
+  class_tap(q >> 1, q & 1, j, ky, kx);
+  wtc[i] = w[((long)o * 9 + ky * 3 + kx) * IC + c];
+}
+void conv_weights_tc(const float* w, int OC, int IC, float* wtc, hipStream_t st) {
+  hipLaunchKernelGGL(k_conv_weights_tc, dim3(nblk(9L * OC * IC)), dim3(256), 0, st, w, OC, IC, wtc);
+  EMC_LAUNCHED();
 }
 void conv_weights_t(const float* w, int OC, int IC, float* wt, hipStream_t st) {
   hipLaunchKernelGGL(k_conv_weights_t, dim3(nblk(9L * OC * IC)), dim3(256), 0, st, w, OC, IC, wt);
