@@ -422,15 +422,19 @@ __global__ __launch_bounds__(256) void k_conv0_fwd(const float* __restrict__ in,
                                                    const float* __restrict__ w, const float* __restrict__ bias,
                                                    float* __restrict__ out) {
   const int OH = H / 2;
-  const long m = (long)blockIdx.x * 256 + threadIdx.x;
-  if (m >= (long)B * OH * OH) return;
-  const int ox = (int)(m % OH), oy = (int)((m / OH) % OH);
-  const long b = m / ((long)OH * OH);
+  const int m = blockIdx.x * 256 + threadIdx.x;  // B * OH * OH < 2^31 (host check)
+  if (m >= B * OH * OH) return;
+  const int ox = m % OH, oy = (m / OH) % OH;
+  const long b = m / (OH * OH);
+  // the 9 taps loaded without branches (a tap in the padding reads the
+  // pixel's own value and is then zeroed), so all are in flight together
   float v[9];
 #pragma unroll
   for (int t = 0; t < 9; t++) {
     const int y = 2 * oy + t / 3 - 1, x = 2 * ox + t % 3 - 1;
-    v[t] = (y >= 0 && y < H && x >= 0 && x < H) ? bf16_round(in[(b * H + y) * H + x]) : 0.0f;
+    const bool ok = y >= 0 && y < H && x >= 0 && x < H;
+    const float u = in[(b * H + (ok ? y : 2 * oy)) * H + (ok ? x : 2 * ox)];
+    v[t] = ok ? bf16_round(u) : 0.0f;
   }
   float o[kC0];
 #pragma unroll
@@ -457,23 +461,53 @@ __global__ __launch_bounds__(256) void k_conv0_wgrad(const float* __restrict__ d
   static_assert(kC0 == 16, "one channel per lane of 16");
   __shared__ float red[10][256];
   const int OH = H / 2;
-  const long M = (long)B * OH * OH;
+  const int M = B * OH * OH;  // < 2^31 (host check)
   const int c = threadIdx.x & 15, pl = threadIdx.x >> 4;
-  const long m0 = (long)blockIdx.x * per, m1 = min(M, m0 + per);
+  const int m0 = blockIdx.x * (int)per, m1 = min(M, m0 + (int)per);
   float acc[10];
 #pragma unroll
   for (int k = 0; k < 10; k++) acc[k] = 0.0f;
-  for (long m = m0 + pl; m < m1; m += 16) {
-    const float gf = dz[m * kC0 + c], g = bf16_round(gf);
-    const int ox = (int)(m % OH), oy = (int)((m / OH) % OH);
-    const long b = m / ((long)OH * OH);
-#pragma unroll
-    for (int t = 0; t < 9; t++) {
-      const int y = 2 * oy + t / 3 - 1, x = 2 * ox + t % 3 - 1;
-      const float v = (y >= 0 && y < H && x >= 0 && x < H) ? bf16_round(in[(b * H + y) * H + x]) : 0.0f;
-      acc[t] += g * v;
+  // pixel (b, oy, ox) of m, advanced by 16 pixels per step without
+  // divisions; 4 pixels (m, m + 16, m + 32, m + 48) per iteration, all their
+  // loads issued before any is used
+  int ox = (m0 + pl) % OH, oy = ((m0 + pl) / OH) % OH, b = (m0 + pl) / (OH * OH);
+  auto advance = [&](int& x_, int& y_, int& b_) {
+    x_ += 16;
+    while (x_ >= OH) {
+      x_ -= OH;
+      if (++y_ == OH) {
+        y_ = 0;
+        b_++;
+      }
     }
-    acc[9] += gf;  // the bias gradient sums dz itself (FP32, as the column sums do)
+  };
+  for (int m = m0 + pl; m < m1; m += 64) {
+    float gf[4], u[4][9];
+    bool ok[4][9];
+    int px = ox, py = oy, pb = b;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const bool live = m + 16 * q < m1;
+      gf[q] = live ? dz[(long)(m + 16 * q) * kC0 + c] : 0.0f;
+#pragma unroll
+      for (int t = 0; t < 9; t++) {  // branch-free loads (see k_conv0_fwd)
+        const int y = 2 * py + t / 3 - 1, x = 2 * px + t % 3 - 1;
+        ok[q][t] = live && y >= 0 && y < H && x >= 0 && x < H;
+        const long pix = live ? ((long)pb * H + (ok[q][t] ? y : 2 * py)) * H + (ok[q][t] ? x : 2 * px) : 0;
+        u[q][t] = in[pix];
+      }
+      advance(px, py, pb);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float g = bf16_round(gf[q]);
+#pragma unroll
+      for (int t = 0; t < 9; t++) acc[t] += g * (ok[q][t] ? bf16_round(u[q][t]) : 0.0f);
+      acc[9] += gf[q];  // the bias gradient sums dz itself (FP32, as the column sums do)
+    }
+    ox = px;
+    oy = py;
+    b = pb;
   }
 #pragma unroll
   for (int k = 0; k < 10; k++) red[k][threadIdx.x] = acc[k];
@@ -523,23 +557,34 @@ __global__ void k_reduce_rows_part(const float* __restrict__ G, long rows, int c
   if (threadIdx.x == 0) part[(long)sp * cols + j] = s;
 }
 
-// The same for a row-major [rows][cols] matrix whose row length divides 256
-// (the bias gradients of the convolutions and of the first dense layer):
-// a workgroup takes whole rows, its threads read consecutive elements (one
-// contiguous 1 KB per step), and thread t always meets column t % cols; the
-// threads of a column are then added in thread order.
+// The same for a row-major [rows][cols] matrix, cols a multiple of 4
+// dividing 1,024 (the bias gradients of the convolutions and of the first
+// dense layer): a workgroup takes whole rows, its threads read consecutive
+// 16-byte groups (4 KB contiguous per step), and element i of thread t's
+// group always falls in column (4t + i) % cols; the 1,024 slots are then
+// added per column in slot order.
 __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ G, long rows, int cols, long per,
                                                      double* __restrict__ part) {
-  __shared__ double red[256];
+  __shared__ double red[1024];
   const int sp = blockIdx.x;
-  const long e0 = (long)sp * per * cols, e1 = min(rows, (long)(sp + 1) * per) * cols;
-  double s = 0.0;
-  for (long e = e0 + threadIdx.x; e < e1; e += 256) s += G[e];
-  red[threadIdx.x] = s;
+  const long e0 = (long)sp * per * cols / 4, e1 = min(rows, (long)(sp + 1) * per) * cols / 4;
+  const float4* G4 = reinterpret_cast<const float4*>(G);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (long e = e0 + threadIdx.x; e < e1; e += 256) {
+    const float4 v = G4[e];
+    s0 += v.x;
+    s1 += v.y;
+    s2 += v.z;
+    s3 += v.w;
+  }
+  red[4 * threadIdx.x] = s0;
+  red[4 * threadIdx.x + 1] = s1;
+  red[4 * threadIdx.x + 2] = s2;
+  red[4 * threadIdx.x + 3] = s3;
   __syncthreads();
   if ((int)threadIdx.x < cols) {
     double t = 0.0;
-    for (int k = threadIdx.x; k < 256; k += cols) t += red[k];
+    for (int k = threadIdx.x; k < 1024; k += cols) t += red[k];
     part[(long)sp * cols + threadIdx.x] = t;
   }
 }
@@ -699,6 +744,10 @@ void im2col(const float* in, int B, int H, int W, int C, float* col, hipStream_t
   EMC_LAUNCHED();
 }
 void conv0_forward(const float* in, int B, int H, const float* w, const float* bias, float* out, hipStream_t st) {
+  if ((long)B * (H / 2) * (H / 2) >= (1L << 31)) {
+    set_error("conv0_forward: %d x %d x %d output pixels exceed 32-bit indices", B, H / 2, H / 2);
+    return;
+  }
   hipLaunchKernelGGL(k_conv0_fwd, dim3(nblk((long)B * (H / 2) * (H / 2))), dim3(256), 0, st, in, B, H, w, bias, out);
   EMC_LAUNCHED();
 }
@@ -706,8 +755,8 @@ bool conv0_wgrad(const float* dz, const float* in, int B, int H, float* dW, floa
                  size_t scratch_doubles, hipStream_t st) {
   const long M = (long)B * (H / 2) * (H / 2);
   const long maxblk = (long)(scratch_doubles / (10 * kC0));
-  if (M <= 0 || maxblk < 1) {
-    set_error("conv0_wgrad: no pixels or no scratch");
+  if (M <= 0 || M >= (1L << 31) || maxblk < 1) {
+    set_error("conv0_wgrad: %ld output pixels (32-bit indices) or no scratch", M);
     return false;
   }
   long per = 1024;  // 64 pixels per thread, thousands of workgroups
@@ -741,8 +790,9 @@ void conv_weights_t(const float* w, int OC, int IC, float* wt, hipStream_t st) {
 }
 void reduce_rows(const float* G, long rows, int cols, long si, long sj, float* out, double* scratch,
                  size_t scratch_doubles, hipStream_t st) {
-  if (sj == 1 && si == cols && cols > 0 && 256 % cols == 0 && rows > 0) {
-    // row-major with rows of a length dividing 256: contiguous reads
+  if (sj == 1 && si == cols && cols >= 4 && cols % 4 == 0 && 1024 % cols == 0 && rows > 0 &&
+      ((uintptr_t)G & 15) == 0) {
+    // row-major with rows of a length dividing 1,024: contiguous reads
     long per = std::max<long>(1, 65536 / cols);  // ~64K elements per workgroup
     long nsplit = (rows + per - 1) / per;
     if (nsplit * cols > (long)scratch_doubles) {
