@@ -722,8 +722,23 @@ __global__ __launch_bounds__(256) void k_coarse_targets(const float* __restrict_
   for (int cc = w; cc < 64; cc += 4) {
     const int c = c0 + cc;
     double s = 0.0;
-    if (c < C && i < N)
-      for (int j = cell_off[c]; j < cell_off[c + 1]; j++) s += P[(long)cell_rot[j] * N + i];
+    if (c < C && i < N) {
+      // 8 rotations' indices, then their responsibilities, loaded together;
+      // added in rotation order
+      const int j1 = cell_off[c + 1];
+      int j = cell_off[c];
+      for (; j + 8 <= j1; j += 8) {
+        int r[8];
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) r[u] = cell_rot[j + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = P[(long)r[u] * N + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+      }
+      for (; j < j1; j++) s += P[(long)cell_rot[j] * N + i];
+    }
     tile[lane][cc] = (float)s;
   }
   __syncthreads();

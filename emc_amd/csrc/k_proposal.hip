@@ -751,8 +751,23 @@ __global__ __launch_bounds__(64) void k_sub_topcells_part(const float* __restric
   int lc[4] = {-1, -1, -1, -1};
   const int c1 = min(C, (g + 1) * kTopGroup);
   for (int c = g * kTopGroup; c < c1; c++) {
+    // the cell's rotations 8 at a time: their indices, then their
+    // responsibilities, loaded together (a load per rotation waited for in
+    // turn made this a chain of dependent loads), added in rotation order
+    const int j1 = cell_off[c + 1];
     float m = 0.0f;
-    for (int j = cell_off[c]; j < cell_off[c + 1]; j++) m += Psub[(size_t)cell_rot[j] * n_sub + k];
+    int j = cell_off[c];
+    for (; j + 8 <= j1; j += 8) {
+      int r[8];
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) r[u] = cell_rot[j + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = Psub[(size_t)r[u] * n_sub + k];
+#pragma unroll
+      for (int u = 0; u < 8; u++) m += v[u];
+    }
+    for (; j < j1; j++) m += Psub[(size_t)cell_rot[j] * n_sub + k];
     top4_insert(m, c, lm, lc);
   }
   const size_t o = ((size_t)k * ng + g) * 4;
