@@ -96,7 +96,136 @@ void launch_cand_estep(bool ksplit, const CandEstepArgs& a, hipStream_t st) {
 // [R][N] matrix (zeroed beforehand).  Patterns scored on the whole table
 // this iteration (insub) get zero candidate responsibilities.
 // ---------------------------------------------------------------------------
+// One wave per pattern; with kSlot == 32, a lane's k-th entry t = lane + 64k
+// is slot lane % 32 of candidate cell 2k + lane / 32.  The lane's entries are
+// loaded once into registers (NPL of them, NPL >= ceil(topk / 2)), each load
+// kind issued for all entries before the next kind so that no entry waits on
+// a dependent load pair of its own; sums run over the entries in t order as
+// a loop over t would.
+static_assert(kSlot == 32, "k_cand_softmax maps two candidate cells to a wave's 64 lanes");
+template <int NPL>
 __global__ __launch_bounds__(256) void k_cand_softmax(CandSoftmaxArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= a.N) return;
+  float* row = a.Lc + (size_t)i * a.topk * kSlot;
+  const int n = a.topk * kSlot;
+  const int* cand = a.cand + (size_t)i * a.topk;
+  const size_t si = (size_t)a.slot * a.N + i;
+  if (a.insub[i]) {
+    if (a.mode == 1) {
+      if (lane == 0) {
+        a.smax[si] = -1.0e10f;
+        a.ssum[si] = 0.0;
+      }
+      return;
+    }
+    for (int t = lane; t < n; t += 64) row[t] = 0.0f;
+    if (lane == 0) {
+      if (a.mode == 2)
+        for (int q = 0; q < 4; q++) a.sacc[(size_t)q * a.nmerge * a.N + si] = 0.0;
+      else
+        a.plogp[i] = 0.0;
+    }
+    return;
+  }
+  const int s = lane & 31;
+  int o[NPL], lim[NPL];
+  float v[NPL];
+#pragma unroll
+  for (int k = 0; k < NPL; k++) o[k] = lane + 64 * k < n ? cand[2 * k + (lane >> 5)] : -1;
+#pragma unroll
+  for (int k = 0; k < NPL; k++) {
+    lim[k] = 0;
+    if (o[k] >= 0) {
+      const int c = o[k];
+      o[k] = a.cell_off[c];
+      lim[k] = a.cell_off[c + 1];
+    } else {
+      o[k] = 0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NPL; k++) v[k] = lane + 64 * k < n ? row[lane + 64 * k] : 0.0f;
+  unsigned long long on = 0;
+#pragma unroll
+  for (int k = 0; k < NPL; k++)
+    if (lane + 64 * k < n && s < lim[k] - o[k]) on |= 1ull << k;
+  float m;
+  double inv;
+  if (a.mode == 2) {
+    merge_stats(a.smax, a.ssum, a.nmerge, a.N, i, m, inv);
+  } else {
+    m = -1.0e10f;
+#pragma unroll
+    for (int k = 0; k < NPL; k++)
+      if (on >> k & 1) m = fmaxf(m, v[k]);
+    m = wave_max(m);
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < NPL; k++)
+      if (on >> k & 1) {
+        const float d = v[k] - m;
+        if (d > -1.0e10f) sum += (double)expf(d);
+      }
+    sum = wave_sum(sum);
+    if (a.mode == 1) {
+      if (lane == 0) {
+        a.smax[si] = m;
+        a.ssum[si] = sum;
+      }
+      return;
+    }
+    inv = 1.0 / sum;
+  }
+  // Rotation of each entry (entries off the cell read rotation 0 and are
+  // not used), then its A.
+  int r[NPL];
+#pragma unroll
+  for (int k = 0; k < NPL; k++) r[k] = a.cell_rot[(on >> k & 1) ? o[k] + s : 0];
+  double Ar[NPL];
+#pragma unroll
+  for (int k = 0; k < NPL; k++) Ar[k] = a.A[r[k]];
+  double plogp = 0.0, ps = 0.0, pa = 0.0, pl = 0.0;
+#pragma unroll
+  for (int k = 0; k < NPL; k++) {
+    const int t = lane + 64 * k;
+    if (t >= n) continue;
+    float p = 0.0f;
+    if (on >> k & 1) {
+      const float d = v[k] - m;
+      const float ex = d > -1.0e10f ? expf(d) : 0.0f;
+      p = (float)((double)ex * inv);
+      if (p > 0.0f) plogp += (double)(p * logf(p));
+      ps += p;
+      pa += (double)p * Ar[k];
+      pl += (double)p * (double)v[k];
+      if (a.respons) a.respons[(size_t)r[k] * a.N + i] = p;
+    }
+    row[t] = p;
+  }
+  plogp = wave_sum(plogp);
+  ps = wave_sum(ps);
+  pa = wave_sum(pa);
+  pl = wave_sum(pl);
+  if (lane == 0) {
+    if (a.mode == 2) {
+      const size_t cn = (size_t)a.nmerge * a.N;
+      a.sacc[si] = plogp;
+      a.sacc[cn + si] = ps;
+      a.sacc[2 * cn + si] = pa;
+      a.sacc[3 * cn + si] = pl;
+    } else {
+      const double f = (double)a.phi[i];
+      a.phiden[i] = a.poisson ? pa : 0.5 * f * (pa + a.B[i] / (f * f) * ps + a.sigma2x * pl);
+      a.plogp[i] = plogp;
+    }
+  }
+}
+
+// The same for topk above 32 (where the register form would run at one wave
+// per SIMD): each entry's cell bounds are read per pass.
+__global__ __launch_bounds__(256) void k_cand_softmax_wide(CandSoftmaxArgs a) {
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= a.N) return;
@@ -192,7 +321,13 @@ __global__ __launch_bounds__(256) void k_cand_softmax(CandSoftmaxArgs a) {
 }
 
 void launch_cand_softmax(const CandSoftmaxArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_cand_softmax, dim3((a.N + 3) / 4), dim3(256), 0, st, a);
+  const int npl = (a.topk + 1) / 2;
+  if (npl <= 8)
+    hipLaunchKernelGGL(k_cand_softmax<8>, dim3((a.N + 3) / 4), dim3(256), 0, st, a);
+  else if (npl <= 16)
+    hipLaunchKernelGGL(k_cand_softmax<16>, dim3((a.N + 3) / 4), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_cand_softmax_wide, dim3((a.N + 3) / 4), dim3(256), 0, st, a);
   EMC_LAUNCHED();
 }
 
