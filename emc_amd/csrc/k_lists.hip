@@ -40,49 +40,89 @@ void launch_mark_support(const int* top, const float* topm, int N, int ktop, flo
 // the nearest cells of each of those, the network's cells in rank order,
 // the nearest cells of the cells taken so far, the lowest-numbered free
 // cells.  moved counts the patterns whose list changed.
+//
+// Here one wave per pattern: lane q holds list entry q (topk <= 64), and
+// the offered cells come 64 at a time, one a lane, in offer order.  A lane's
+// cell passes the range and shared tests in parallel; the passing cells are
+// then taken one by one in lane order, each after one ballot over the list
+// for repeats, until topk are held.  (At least topk cells are free: the
+// shared set leaves them.)
+__device__ __forceinline__ void recentre_take(int c, bool offer, int C, const unsigned char* __restrict__ shared,
+                                              int topk, int lane, int& lst, int& n) {
+  const bool ok = offer && c >= 0 && c < C && !shared[c];
+  unsigned long long m = __ballot(ok);
+  while (m && n < topk) {
+    const int l = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    const int cl = __builtin_amdgcn_readlane(c, l);
+    if (__ballot(lane < n && lst == cl) == 0) {
+      if (lane == n) lst = cl;
+      n++;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_recentre_cand(const int* __restrict__ top, const float* __restrict__ topm,
                                                        int N, int ktop, const int* __restrict__ nbr, int nnbr,
                                                        const int* __restrict__ net,
                                                        const unsigned char* __restrict__ shared, int C, int topk,
                                                        int* __restrict__ cand, int* __restrict__ moved) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= N) return;
-  int next[kMaxTopk];
-  int n = 0;
-  auto add = [&](int c) {
-    if (c < 0 || c >= C || n >= topk || shared[c]) return;
-    for (int q = 0; q < n; q++)
-      if (next[q] == c) return;
-    next[n++] = c;
-  };
+  int lst = -1, n = 0;
   const int* ti = top + (size_t)i * ktop;
   const float* mi = topm + (size_t)i * ktop;
+  const float m0 = mi[0];
   int used = 0;
-  for (int t = 0; t < ktop; t++)
-    if (ti[t] >= 0 && ti[t] < C && (t == 0 || mi[t] > 1e-6f * mi[0])) {
-      add(ti[t]);
-      used = t + 1;
-    }
-  for (int t = 0; t < used; t++)
-    if (ti[t] >= 0 && ti[t] < C)
-      for (int g = 0; g < nnbr; g++) add(nbr[(size_t)ti[t] * nnbr + g]);
-  for (int j = 0; j < topk; j++) add(net[(size_t)i * topk + j]);
-  for (int q = 0; q < n && n < topk; q++)
-    for (int g = 0; g < nnbr; g++) add(nbr[(size_t)next[q] * nnbr + g]);
-  for (int c = 0; c < C && n < topk; c++) add(c);
+  for (int t0 = 0; t0 < ktop; t0 += 64) {
+    const int t = t0 + lane;
+    const int c = t < ktop ? ti[t] : -1;
+    const bool pick = t < ktop && c >= 0 && c < C && (t == 0 || mi[t] > 1e-6f * m0);
+    const unsigned long long b = __ballot(pick);
+    if (b) used = t0 + 64 - __clzll((long long)b);
+    recentre_take(c, pick, C, shared, topk, lane, lst, n);
+  }
+  for (int f0 = 0; f0 < used * nnbr; f0 += 64) {
+    const int f = f0 + lane;
+    const int t = f / nnbr;
+    const int c = f < used * nnbr ? ti[t] : -1;
+    const bool has = c >= 0 && c < C;
+    const int d = has ? nbr[(size_t)c * nnbr + (f - t * nnbr)] : -1;
+    recentre_take(d, has, C, shared, topk, lane, lst, n);
+  }
+  for (int j0 = 0; j0 < topk && n < topk; j0 += 64) {
+    const int j = j0 + lane;
+    const int c = j < topk ? net[(size_t)i * topk + j] : -1;
+    recentre_take(c, j < topk, C, shared, topk, lane, lst, n);
+  }
+  // nearest cells of the held cells, the held list growing meanwhile: a
+  // batch offers the cells of entries below the count at its start, and
+  // entries taken in it are offered in a later batch
+  for (int f0 = 0; n < topk;) {
+    const int fend = n * nnbr;
+    if (f0 >= fend) break;
+    const int f = f0 + lane;
+    const int q = f / nnbr;
+    const bool has = f < fend;
+    const int cq = __shfl(lst, has ? q : 0);
+    const int d = has ? nbr[(size_t)cq * nnbr + (f - q * nnbr)] : -1;
+    recentre_take(d, has, C, shared, topk, lane, lst, n);
+    f0 = min(f0 + 64, fend);
+  }
+  for (int c0 = 0; c0 < C && n < topk; c0 += 64) recentre_take(c0 + lane, c0 + lane < C, C, shared, topk, lane, lst, n);
   int* ci = cand + (size_t)i * topk;
-  bool same = true;
-  for (int j = 0; j < topk; j++) same = same && ci[j] == next[j];
-  if (!same) {
-    for (int j = 0; j < topk; j++) ci[j] = next[j];
-    atomicAdd(moved, 1);
+  const bool differ = lane < topk && ci[lane] != lst;
+  if (__ballot(differ)) {
+    if (lane < topk) ci[lane] = lst;
+    if (lane == 0) atomicAdd(moved, 1);
   }
 }
 
 void launch_recentre_cand(const int* top, const float* topm, int N, int ktop, const int* nbr, int nnbr,
                           const int* net, const unsigned char* shared, int C, int topk, int* cand, int* moved,
                           hipStream_t st) {
-  hipLaunchKernelGGL(k_recentre_cand, dim3((N + 255) / 256), dim3(256), 0, st, top, topm, N, ktop, nbr, nnbr, net,
+  hipLaunchKernelGGL(k_recentre_cand, dim3((N + 3) / 4), dim3(256), 0, st, top, topm, N, ktop, nbr, nnbr, net,
                      shared, C, topk, cand, moved);
   EMC_LAUNCHED();
 }

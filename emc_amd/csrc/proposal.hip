@@ -371,7 +371,8 @@ static bool ensure_iota(Proposal& p, int n) {
 // in mass order, then the network's cells in rank order, up to topk distinct
 // cells.  Before that, the shared cells of the next iteration: every cell
 // that is some pattern's best cell or holds >= kShareMass of its posterior,
-// if there are at most max_shared of them; candidate cells exclude them.
+// if there are at most max_shared of them and they leave topk cells free;
+// candidate cells exclude them.
 static bool recentre(Proposal& p) {
   Engine& e = engine();
   hipStream_t st = e.stream;
@@ -418,7 +419,7 @@ static bool recentre(Proposal& p) {
     if (failed()) return false;
     int n = 0;
     for (int c = 0; c < p.C; c++) n += p.h_mark[c] != 0;
-    if (n <= p.max_shared && n < p.C) {
+    if (n <= p.max_shared && n <= p.C - p.topk) {
       for (int c = 0; c < p.C; c++)
         if (p.h_mark[c]) {
           p.h_shared[c] = 1;
