@@ -215,50 +215,85 @@ bool sort_entries(const int* cand, int N, int topk, int C, DBuf& keys_sorted, DB
 // gives every rotation's list in pattern order, independent of that order.
 // ---------------------------------------------------------------------------
 // mode 0: count into rownnz; mode 1: append to keys / vals at *tail.  Called
-// by every lane of the wave (take = lane has an entry): one atomic per wave
-// for the append position, and for the count when r is the same for the
-// whole wave (r_uniform).  The append order is irrelevant: the keys are
-// unique and sorted afterwards.
-__device__ __forceinline__ void nz_take(int mode, bool take, float p, int r, int i, int N, bool r_uniform,
+// by every thread of the workgroup (256) for its kNzItems entries, entry q
+// of thread t being the workgroup's entry q * 256 + t (take[q] = it exists):
+// one atomic per workgroup for the append position, and for the count when
+// r is the same for all its entries (r_uniform), else one per nonzero.  The
+// append order is irrelevant: the keys are unique and sorted afterwards.
+constexpr int kNzItems = 4;
+
+__device__ __forceinline__ void nz_take(int mode, const bool (&take)[kNzItems], const float (&p)[kNzItems],
+                                        const int (&r)[kNzItems], const int (&i)[kNzItems], int N, bool r_uniform,
                                         int* rownnz, unsigned* tail, unsigned* keys, float* vals) {
-  const bool nz = take && p != 0.0f;
-  const unsigned long long m = __ballot(nz);
-  if (m == 0ull) return;
-  const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1, cnt = __popcll(m);
-  if (mode == 0) {
-    if (!r_uniform) {
-      if (nz) atomicAdd(&rownnz[r], 1);
-    } else if (lane == leader) {
-      atomicAdd(&rownnz[r], cnt);
-    }
+  __shared__ unsigned s_cnt[kNzItems][4];
+  __shared__ unsigned s_base;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long m[kNzItems];
+#pragma unroll
+  for (int q = 0; q < kNzItems; q++) m[q] = __ballot(take[q] && p[q] != 0.0f);
+  if (mode == 0 && !r_uniform) {
+#pragma unroll
+    for (int q = 0; q < kNzItems; q++)
+      if (m[q] >> lane & 1) atomicAdd(&rownnz[r[q]], 1);
     return;
   }
-  unsigned base = 0;
-  if (lane == leader) base = atomicAdd(tail, (unsigned)cnt);
-  base = __shfl(base, leader);
-  if (nz) {
-    const unsigned at = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-    keys[at] = (unsigned)r * (unsigned)N + (unsigned)i;
-    vals[at] = p;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < kNzItems; q++) s_cnt[q][w] = (unsigned)__popcll(m[q]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tot = 0;
+#pragma unroll
+    for (int q = 0; q < kNzItems; q++) tot += s_cnt[q][0] + s_cnt[q][1] + s_cnt[q][2] + s_cnt[q][3];
+    if (mode == 0) {
+      if (tot) atomicAdd(&rownnz[r[0]], (int)tot);
+    } else {
+      s_base = tot ? atomicAdd(tail, tot) : 0u;
+    }
+  }
+  if (mode == 0) return;
+  __syncthreads();
+  unsigned at = s_base;
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int q = 0; q < kNzItems; q++) {
+    unsigned mine = at;
+    for (int v = 0; v < 4; v++) {
+      if (v == w) mine = at;
+      at += s_cnt[q][v];
+    }
+    if (m[q] >> lane & 1) {
+      const unsigned k = mine + (unsigned)__popcll(m[q] & below);
+      keys[k] = (unsigned)r[q] * (unsigned)N + (unsigned)i[q];
+      vals[k] = p[q];
+    }
   }
 }
 
 __global__ __launch_bounds__(256) void k_nz_cand(NzSources a, int mode, int* rownnz, unsigned* tail, unsigned* keys,
                                                  float* vals) {
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;  // (i, j, slot)
-  bool take = t < (long)a.N * a.topk * kSlot;
-  int r = 0, i = 0;
-  float p = 0.0f;
-  if (take) {
-    const int s = (int)(t % kSlot);
-    const long ij = t / kSlot;
-    i = (int)(ij / a.topk);
-    const int c = a.cand[ij];
-    const int o = a.cell_off[c];
-    take = !a.insub[i] && s < a.cell_off[c + 1] - o;
-    if (take) {
-      p = a.Pc[t];
-      r = a.cell_rot[o + s];
+  const long nc = (long)a.N * a.topk * kSlot;
+  bool take[kNzItems];
+  int r[kNzItems], i[kNzItems];
+  float p[kNzItems];
+#pragma unroll
+  for (int q = 0; q < kNzItems; q++) {
+    const long t = (long)blockIdx.x * (256 * kNzItems) + q * 256 + threadIdx.x;  // (i, j, slot)
+    take[q] = t < nc;
+    r[q] = 0;
+    i[q] = 0;
+    p[q] = 0.0f;
+    if (take[q]) {
+      const int s = (int)(t % kSlot);
+      const int ij = (int)(t / kSlot);  // < N * topk, the size of cand
+      i[q] = ij / a.topk;
+      const int c = a.cand[ij];
+      const int o = a.cell_off[c];
+      take[q] = !a.insub[i[q]] && s < a.cell_off[c + 1] - o;
+      if (take[q]) {
+        p[q] = a.Pc[t];
+        r[q] = a.cell_rot[o + s];
+      }
     }
   }
   nz_take(mode, take, p, r, i, a.N, false, rownnz, tail, keys, vals);
@@ -267,35 +302,52 @@ __global__ __launch_bounds__(256) void k_nz_cand(NzSources a, int mode, int* row
 __global__ __launch_bounds__(256) void k_nz_shared(NzSources a, int mode, int* rownnz, unsigned* tail,
                                                    unsigned* keys, float* vals) {
   const int rs = blockIdx.y;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const bool take = i < a.N && !a.insub[i];
-  const float p = take ? a.Psh[(size_t)rs * a.N + i] : 0.0f;
-  nz_take(mode, take, p, a.sh_rot[rs], i, a.N, true, rownnz, tail, keys, vals);
+  const int rot = a.sh_rot[rs];
+  bool take[kNzItems];
+  int r[kNzItems], i[kNzItems];
+  float p[kNzItems];
+#pragma unroll
+  for (int q = 0; q < kNzItems; q++) {
+    i[q] = blockIdx.x * (256 * kNzItems) + q * 256 + threadIdx.x;
+    take[q] = i[q] < a.N && !a.insub[i[q]];
+    p[q] = take[q] ? a.Psh[(size_t)rs * a.N + i[q]] : 0.0f;
+    r[q] = rot;
+  }
+  nz_take(mode, take, p, r, i, a.N, true, rownnz, tail, keys, vals);
 }
 
 __global__ __launch_bounds__(256) void k_nz_sub(NzSources a, int mode, int* rownnz, unsigned* tail, unsigned* keys,
                                                 float* vals) {
-  const int r = blockIdx.x;
-  const int k = blockIdx.y * 256 + threadIdx.x;
-  const bool take = k < a.n_sub;
-  nz_take(mode, take, take ? a.Psub[(size_t)r * a.n_sub + k] : 0.0f, r, take ? a.sub[k] : 0, a.N, true, rownnz,
-          tail, keys, vals);
+  const int rot = blockIdx.x;
+  bool take[kNzItems];
+  int r[kNzItems], i[kNzItems];
+  float p[kNzItems];
+#pragma unroll
+  for (int q = 0; q < kNzItems; q++) {
+    const int k = blockIdx.y * (256 * kNzItems) + q * 256 + threadIdx.x;
+    take[q] = k < a.n_sub;
+    p[q] = take[q] ? a.Psub[(size_t)rot * a.n_sub + k] : 0.0f;
+    i[q] = take[q] ? a.sub[k] : 0;
+    r[q] = rot;
+  }
+  nz_take(mode, take, p, r, i, a.N, true, rownnz, tail, keys, vals);
 }
 
 static void nz_pass(const NzSources& a, int mode, int* rownnz, unsigned* tail, unsigned* keys, float* vals,
                     hipStream_t st) {
+  constexpr int span = 256 * kNzItems;
   const long nc = (long)a.N * a.topk * kSlot;
-  hipLaunchKernelGGL(k_nz_cand, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, a, mode, rownnz, tail, keys,
-                     vals);
+  hipLaunchKernelGGL(k_nz_cand, dim3((unsigned)((nc + span - 1) / span)), dim3(256), 0, st, a, mode, rownnz, tail,
+                     keys, vals);
   EMC_LAUNCHED();
   if (a.R_sh > 0) {
-    hipLaunchKernelGGL(k_nz_shared, dim3((a.N + 255) / 256, a.R_sh), dim3(256), 0, st, a, mode, rownnz, tail, keys,
-                       vals);
+    hipLaunchKernelGGL(k_nz_shared, dim3((a.N + span - 1) / span, a.R_sh), dim3(256), 0, st, a, mode, rownnz, tail,
+                       keys, vals);
     EMC_LAUNCHED();
   }
   if (a.n_sub > 0) {
-    hipLaunchKernelGGL(k_nz_sub, dim3(a.R, (a.n_sub + 255) / 256), dim3(256), 0, st, a, mode, rownnz, tail, keys,
-                       vals);
+    hipLaunchKernelGGL(k_nz_sub, dim3(a.R, (a.n_sub + span - 1) / span), dim3(256), 0, st, a, mode, rownnz, tail,
+                       keys, vals);
     EMC_LAUNCHED();
   }
 }
