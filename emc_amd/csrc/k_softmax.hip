@@ -25,6 +25,8 @@ int softmax_chunks(int N, int R) {
   return c;
 }
 
+constexpr int kSmBatch = 8;
+
 __global__ __launch_bounds__(256) void k_softmax_partial(const float* __restrict__ L, int R, int N, int rows,
                                                          float* __restrict__ smax, double* __restrict__ ssum) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -33,14 +35,23 @@ __global__ __launch_bounds__(256) void k_softmax_partial(const float* __restrict
   const int r0 = c * rows, r1 = min(R, r0 + rows);
   float m = -1.0e10f;
   double s = 0.0;
-  for (int r = r0; r < r1; r++) {
-    const float v = L[(size_t)r * N + i];
-    if (m < v) {
-      s *= exp((double)m - (double)v);
-      m = v;
+  // the rows' values are loaded kSmBatch at a time before they are used (the
+  // arithmetic and its order are those of one row at a time)
+  for (int rb = r0; rb < r1; rb += kSmBatch) {
+    float vb[kSmBatch];
+#pragma unroll
+    for (int u = 0; u < kSmBatch; u++) vb[u] = rb + u < r1 ? L[(size_t)(rb + u) * N + i] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kSmBatch; u++) {
+      if (rb + u >= r1) break;
+      const float v = vb[u];
+      if (m < v) {
+        s *= exp((double)m - (double)v);
+        m = v;
+      }
+      const float d = v - m;
+      if (d > -1.0e10f) s += (double)expf(d);
     }
-    const float d = v - m;
-    if (d > -1.0e10f) s += (double)expf(d);
   }
   smax[(size_t)c * N + i] = m;
   ssum[(size_t)c * N + i] = s;
@@ -63,18 +74,25 @@ __global__ __launch_bounds__(256) void k_softmax_finish(float* __restrict__ L, i
   merge_stats(smax, ssum, nmerge, N, i, m, inv);
   const int r0 = c * rows, r1 = min(R, r0 + rows);
   double plogp = 0.0, psum = 0.0, pa = 0.0, pl = 0.0;
-  for (int r = r0; r < r1; r++) {
-    const size_t k = (size_t)r * N + i;
-    const float v = L[k];
-    const float d = v - m;
-    const float e = d > -1.0e10f ? expf(d) : 0.0f;
-    const float p = (float)((double)e * inv);
-    L[k] = p;
-    if (p > 0.0f) plogp += (double)(p * logf(p));
-    psum += p;
-    if (fuse) {
-      pa += (double)p * A[r];
-      pl += (double)p * (double)v;
+  for (int rb = r0; rb < r1; rb += kSmBatch) {
+    float vb[kSmBatch];
+#pragma unroll
+    for (int u = 0; u < kSmBatch; u++) vb[u] = rb + u < r1 ? L[(size_t)(rb + u) * N + i] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kSmBatch; u++) {
+      if (rb + u >= r1) break;
+      const int r = rb + u;
+      const float v = vb[u];
+      const float d = v - m;
+      const float e = d > -1.0e10f ? expf(d) : 0.0f;
+      const float p = (float)((double)e * inv);
+      L[(size_t)r * N + i] = p;
+      if (p > 0.0f) plogp += (double)(p * logf(p));
+      psum += p;
+      if (fuse) {
+        pa += (double)p * A[r];
+        pl += (double)p * (double)v;
+      }
     }
   }
   const size_t cn = (size_t)nmerge * N;
