@@ -440,23 +440,44 @@ void launch_sub_finish(const float* Psub, int n_sub, const int* sub, const doubl
 }
 
 // Shared rotations: respons[sh_rot[r]][i] = Psh[r][i] for the patterns not
-// scored on the whole table (one block per shared rotation, lanes along
-// patterns: coalesced both sides).
+// scored on the whole table.  Grid (patterns / 1,024, shared rotations);
+// a thread takes four consecutive patterns, with 16-byte loads and stores
+// when the rows allow them (vec: N % 4 == 0 and both buffers 16-byte
+// aligned).
 __global__ __launch_bounds__(256) void k_shared_scatter(const float* __restrict__ Psh, int N,
                                                         const int* __restrict__ sh_rot,
                                                         const unsigned char* __restrict__ insub,
-                                                        float* __restrict__ respons) {
-  const int r = blockIdx.x;
+                                                        float* __restrict__ respons, int vec) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const int r = blockIdx.y;
   float* out = respons + (size_t)sh_rot[r] * N;
   const float* in = Psh + (size_t)r * N;
-  for (int i = threadIdx.x; i < N; i += 256)
+  const int i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (i0 >= N) return;
+  if (vec) {
+    const f4 v = *reinterpret_cast<const f4*>(in + i0);
+    const uchar4 sk = *reinterpret_cast<const uchar4*>(insub + i0);
+    if (!(sk.x | sk.y | sk.z | sk.w)) {
+      *reinterpret_cast<f4*>(out + i0) = v;
+    } else {
+      if (!sk.x) out[i0] = v.x;
+      if (!sk.y) out[i0 + 1] = v.y;
+      if (!sk.z) out[i0 + 2] = v.z;
+      if (!sk.w) out[i0 + 3] = v.w;
+    }
+    return;
+  }
+  for (int i = i0; i < min(N, i0 + 4); i++)
     if (!insub[i]) out[i] = in[i];
 }
 
 void launch_shared_scatter(const float* Psh, int R_sh, int N, const int* sh_rot, const unsigned char* insub,
                            float* respons, hipStream_t st) {
   if (R_sh <= 0) return;
-  hipLaunchKernelGGL(k_shared_scatter, dim3(R_sh), dim3(256), 0, st, Psh, N, sh_rot, insub, respons);
+  const int vec = N % 4 == 0 && (uintptr_t)Psh % 16 == 0 && (uintptr_t)respons % 16 == 0 &&
+                  (uintptr_t)insub % 4 == 0;
+  hipLaunchKernelGGL(k_shared_scatter, dim3((N + 1023) / 1024, R_sh), dim3(256), 0, st, Psh, N, sh_rot, insub,
+                     respons, vec);
   EMC_LAUNCHED();
 }
 
