@@ -178,9 +178,9 @@ __device__ __forceinline__ void sparse_accumulate(const int* __restrict__ col, c
                                                   int j1, const _Float16* __restrict__ K,
                                                   const _Float16* __restrict__ Klo, int n_mp, int k0,
                                                   float (&acc)[kSpPix]) {
-  // four nonzeros per step: their count rows are loaded together (the loop
+  // eight nonzeros per step: their count rows are loaded together (the loop
   // is latency-bound otherwise); accumulation stays in list order
-  constexpr int U = 4;
+  constexpr int U = 8;
   for (; j + U <= j1; j += U) {
     half8 h[U], l[U];
     float w[U];
