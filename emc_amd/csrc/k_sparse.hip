@@ -250,17 +250,21 @@ __global__ __launch_bounds__(256) void k_mstep_zero(const int* __restrict__ rows
 
 // the same with 16-byte stores: four pixels per lane, all four written at
 // once when all are masked in (m4: bit q for pixel 4v + q), else one by
-// one.  A workgroup takes one chunk of 1,024 groups (blockIdx.y) of kZeroRows
-// rows, which share the mask words, loaded once.
+// one.  A workgroup takes one chunk of 1,024 groups of kZeroRows rows, which
+// share the mask words, loaded once.  With qfast the chunk is blockIdx.x and
+// the row group blockIdx.y, so that consecutive workgroups fill consecutive
+// chunks of the same rows; else the other way round (row groups beyond the
+// grid's y limit).
 constexpr int kZeroRows = 4;
 
 __global__ __launch_bounds__(256) void k_mstep_zero4(const int* __restrict__ rows, int nrows,
-                                                     const int* __restrict__ m4, MstepArgs a) {
+                                                     const int* __restrict__ m4, MstepArgs a, int qfast) {
+  const int chunk = qfast ? blockIdx.x : blockIdx.y, grp = qfast ? blockIdx.y : blockIdx.x;
   typedef float f4 __attribute__((ext_vector_type(4)));
   const int nv = a.N2d / 4;
   const f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
   {
-    const int v0 = blockIdx.y * 256 * 4;  // this workgroup's quarter-row chunk
+    const int v0 = chunk * 256 * 4;  // this workgroup's chunk of the row
     int m[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(256) void k_mstep_zero4(const int* __restrict__ row
       m[u] = v < nv ? m4[v] : 0;
     }
     for (int h = 0; h < kZeroRows; h++) {
-      const int j = blockIdx.x * kZeroRows + h;
+      const int j = grp * kZeroRows + h;
       if (j >= nrows) break;
       float* out = a.slices + (size_t)rows[j] * a.N2d;
 #pragma unroll
@@ -378,9 +382,13 @@ bool launch_mstep_sparse(const std::vector<int>& h_off, const int* off, const in
       EMC_LAUNCHED();
       e.mask4_gen = e.pix_gen;
     }
-    if (vec)
-      hipLaunchKernelGGL(k_mstep_zero4, dim3((nz + kZeroRows - 1) / kZeroRows, (a.N2d / 4 + 1023) / 1024), dim3(256),
-                         0, st, e.d_sp_zero.as<int>(), nz, e.d_mask4.as<int>(), a);
+    const int ngrp = (nz + kZeroRows - 1) / kZeroRows, nchunk = (a.N2d / 4 + 1023) / 1024;
+    if (vec && ngrp <= 65535)
+      hipLaunchKernelGGL(k_mstep_zero4, dim3(nchunk, ngrp), dim3(256), 0, st, e.d_sp_zero.as<int>(), nz,
+                         e.d_mask4.as<int>(), a, 1);
+    else if (vec)
+      hipLaunchKernelGGL(k_mstep_zero4, dim3(ngrp, nchunk), dim3(256), 0, st, e.d_sp_zero.as<int>(), nz,
+                         e.d_mask4.as<int>(), a, 0);
     else
       hipLaunchKernelGGL(k_mstep_zero, dim3(nz), dim3(256), 0, st, e.d_sp_zero.as<int>(), a);
     EMC_LAUNCHED();
